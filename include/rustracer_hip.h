@@ -1,0 +1,259 @@
+/*
+ * rustracer_hip.h — C ABI of the gfx950 (MI355X) backend for rustracer's
+ * rendering path.
+ *
+ * The reference boundary is the Rust crate API of `rustracer-renderer`
+ * (rustracer-renderer/src/lib.rs) plus the scene model of `rustracer-core`.
+ * wgpu types cannot cross a C ABI, so the adapter becomes a HIP device ordinal,
+ * `record_frame(encoder, view)` becomes rt_render_frame(), and read-backs
+ * (examples/common/mod.rs) become copy-out calls. Every entry point returns an
+ * int status (RT_OK = 0); on failure rt_last_error() returns a message for the
+ * calling thread. Host pointers are borrowed for the duration of the call and
+ * copied. A renderer handle owns its device memory and one HIP stream and is
+ * not thread-safe (callers serialise, as `&mut self` does in the reference).
+ * No HIP or torch types appear in any signature.
+ */
+#ifndef RUSTRACER_HIP_H
+#define RUSTRACER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_OK 0
+#define RT_ERR_INVALID -1     /* bad argument / null pointer / size mismatch */
+#define RT_ERR_UNSUPPORTED -2 /* feature outside this backend (e.g. textures) */
+#define RT_ERR_HIP -3         /* HIP runtime error (message in rt_last_error) */
+#define RT_ERR_NO_DEVICE -4   /* no usable gfx950 device */
+#define RT_ERR_CAPACITY -5    /* a size exceeds a documented limit */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *rt_last_error(void);
+/* Library version string. */
+const char *rt_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Render configuration. Replaces RenderConfig, rustracer-renderer/src/lib.rs:21-65.
+ * Fields up to `ambient` mirror the reference one to one (bool -> uint32 0/1).
+ * The extension fields default to the reference's fixed behaviour.           */
+typedef struct rt_config {
+    uint32_t width;
+    uint32_t height;
+    uint32_t photon_count;
+    uint32_t spp;
+    float exposure;
+    float gamma;
+    float light_emission[3];
+    float point_light_emission[3];
+    uint32_t accumulate_frames; /* 0 = overwrite, N = running mean then EMA over N */
+    uint32_t enable_photon_map;
+    uint32_t enable_denoise;
+    uint32_t photon_debug;
+    uint32_t debug_view;
+    float photon_scale;
+    float photon_radius; /* overwritten by grid sizing at creation, lib.rs:431 */
+    float ambient;
+    /* --- extensions (no counterpart in RenderConfig) --- */
+    uint32_t max_depth;     /* bounce limit of camera and photon paths; reference: 6 */
+    float grid_cell_size;   /* 0: reference sizing (max_extent/24, <= 26 cells/axis);
+                               > 0: this cell size and radius, cells sized to the scene */
+    uint32_t denoise_passes;/* a-trous passes at steps 1,2,4..; reference: 2 */
+    uint32_t flags;         /* RT_FLAG_* */
+} rt_config;
+
+/* Time each pass with HIP events on the renderer's stream (rt_get_pass_stats). */
+#define RT_FLAG_PROFILE 1u
+/* Trace shadow rays with the reference's closest-hit query instead of the
+ * equivalent early-exit query (same result when no material is alpha-cutout). */
+#define RT_FLAG_CLOSEST_SHADOW 2u
+/* Use the reference-order BVH2 traversal (left child first) for every query. */
+#define RT_FLAG_REFERENCE_TRAVERSAL 4u
+
+/* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
+int rt_config_default(rt_config *cfg);
+
+/* ------------------------------------------------------------------------ */
+/* Scene model (rustracer-core). Replaces Scene, rustracer-core/src/scene.rs:158-291. */
+typedef struct rt_scene rt_scene;
+
+/* Light description (rustracer-core Light enum, scene.rs:101-126). */
+typedef struct rt_light_desc {
+    uint32_t kind;          /* RT_LIGHT_* */
+    float vec_a[3];         /* point: position; directional: direction; environment: zenith colour */
+    float vec_b[3];         /* environment: horizon colour */
+    float color[3];         /* point/directional/area colour */
+    float intensity;
+    uint32_t triangle_index;/* area: index into the flattened triangle list */
+} rt_light_desc;
+
+/* Host camera (rustracer-core Camera, camera.rs:10-32). */
+typedef struct rt_camera {
+    float position[3];
+    float look_at[3];
+    float up[3];
+    float fov_degrees;
+    float near_plane;
+    float aperture;
+    float focus_dist;
+} rt_camera;
+
+int rt_scene_new(rt_scene **out);
+void rt_scene_free(rt_scene *scene);
+/* Append a material record; returns its index in *index (Material::to_gpu, material.rs:143-161). */
+int rt_scene_add_material(rt_scene *scene, const rt_material *material, uint32_t *index);
+/* Append an indexed mesh (Mesh::new, scene.rs:24-38). normals/uvs may be NULL
+ * (flatten_triangles substitutes +Y / zero, scene.rs:233-250). */
+int rt_scene_add_mesh(rt_scene *scene, const float *positions, uint32_t n_positions,
+                      const float *normals, uint32_t n_normals, const float *uvs,
+                      uint32_t n_uvs, const uint32_t *indices, uint32_t n_indices,
+                      uint32_t material_id);
+int rt_scene_add_light(rt_scene *scene, const rt_light_desc *light);
+int rt_scene_clear_lights(rt_scene *scene);
+/* Built-in procedural scenes, seeded:
+ *   "cornell_box"   Scene::cornell_box (scene.rs:296-381), 60 triangles
+ *   "dark_room"     Scene::dark_room (scene.rs:386-460)
+ *   "smoke"         the 3-triangle scene of gpu_layout.rs:66-99
+ *   "cornell36"     36-triangle unit Cornell room, radiance-15 ceiling quad
+ *   "photon_plain"  displaced icosphere (~46k) + glass sphere (8k) + floor + area and point light
+ *   "multiview"     ~109k-triangle tori / bevelled boxes, Lambert/GGX/glass mix
+ *   "fullframe"     ~1M triangles: value-noise terrain (600k) + 200 meshes of 2k
+ *   "training"      one seeded procedural training scene (instanced primitives)   */
+int rt_scene_builtin(const char *name, uint64_t seed, rt_scene **out);
+/* Camera suited to a built-in scene (index selects one of several for "multiview"). */
+int rt_scene_builtin_camera(const char *name, uint64_t seed, uint32_t index, rt_camera *out);
+
+int rt_scene_counts(const rt_scene *scene, uint32_t *n_triangles, uint32_t *n_materials,
+                    uint32_t *n_lights, uint32_t *n_meshes);
+/* Scene::bounds (scene.rs:198-210); returns RT_ERR_INVALID when there is no geometry. */
+int rt_scene_bounds(const rt_scene *scene, float bmin[3], float bmax[3]);
+/* Scene::flatten_triangles (scene.rs:213-273). */
+int rt_scene_flatten_triangles(const rt_scene *scene, rt_triangle *out, uint32_t capacity);
+int rt_scene_flatten_materials(const rt_scene *scene, rt_material *out, uint32_t capacity);
+/* light_to_gpu over the scene's lights (lib.rs:1231-1282). */
+int rt_scene_lights_gpu(const rt_scene *scene, rt_light *out, uint32_t capacity);
+int rt_light_to_gpu(const rt_light_desc *light, rt_light *out);
+/* Scene::build_bvh (scene.rs:276-286) -> BVHBuilder::build (bvh.rs:135-151):
+ * binned SAH (12 bins, <= 4 prims per leaf), root at node 0. Call with NULL
+ * outputs to query the sizes. */
+int rt_scene_build_bvh(const rt_scene *scene, rt_bvh_node *nodes, uint32_t node_capacity,
+                       uint32_t *n_nodes, uint32_t *prims, uint32_t prim_capacity,
+                       uint32_t *n_prims);
+/* BVHBuilder over explicit centroids/boxes (bvh.rs:127-151), for the builder tests. */
+int rt_bvh_build_boxes(const float *centroids, const float *box_min, const float *box_max,
+                       uint32_t n, rt_bvh_node *nodes, uint32_t node_capacity,
+                       uint32_t *n_nodes, uint32_t *prims);
+
+/* Camera (camera.rs). */
+int rt_camera_default(rt_camera *cam);                                              /* :42-52  */
+int rt_camera_to_uniform(const rt_camera *cam, uint32_t w, uint32_t h,
+                         rt_camera_uniform *out);                                   /* :56-79  */
+int rt_camera_frame_scene(const rt_scene *scene, rt_camera *out);                   /* :83-99  */
+int rt_camera_orbit(rt_camera *cam, float delta_yaw, float delta_pitch);            /* :102-115 */
+int rt_camera_zoom(rt_camera *cam, float delta);                                    /* :118-123 */
+int rt_camera_pan(rt_camera *cam, float dx, float dy);                              /* :126-133 */
+
+/* Photon-grid sizing of Renderer::new (lib.rs:416-457), or the free-cell-size
+ * extension when cfg->grid_cell_size > 0. Writes the grid and the radius the
+ * renderer will use. */
+int rt_grid_params_for(const rt_config *cfg, const float bmin[3], const float bmax[3],
+                       int has_bounds, rt_grid_params *out, float *photon_radius);
+
+/* ------------------------------------------------------------------------ */
+/* Renderer (rustracer-renderer Renderer, lib.rs:189-1041). */
+typedef struct rt_renderer rt_renderer;
+
+/* Renderer::new (lib.rs:227-663): flatten, build the BVH, upload, size the grid. */
+int rt_renderer_create(const rt_config *cfg, const rt_scene *scene, int device,
+                       rt_renderer **out);
+/* Renderer::new over already-flattened GPU records (the upload contract of
+ * lib.rs:234-271). lights may be empty (a zeroed record is bound, buffer.rs:19-34).
+ * bounds: the scene bounds used for grid sizing (NULL = no geometry). */
+int rt_renderer_create_raw(const rt_config *cfg, const rt_triangle *tris, uint32_t n_tris,
+                           const rt_bvh_node *nodes, uint32_t n_nodes, const uint32_t *prims,
+                           uint32_t n_prims, const rt_material *mats, uint32_t n_mats,
+                           const rt_light *lights, uint32_t n_lights, const float *bounds_min,
+                           const float *bounds_max, int device, rt_renderer **out);
+void rt_renderer_destroy(rt_renderer *r);
+
+/* record_frame + queue.submit (lib.rs:666-851): gather, a-trous, composite,
+ * tonemap. Asynchronous; rt_synchronize waits (device.poll(Wait)). */
+int rt_render_frame(rt_renderer *r);
+int rt_render_frames(rt_renderer *r, uint32_t n_frames);
+int rt_synchronize(rt_renderer *r);
+int rt_update_camera(rt_renderer *r, const rt_camera *cam);                 /* lib.rs:853-857 */
+int rt_update_camera_uniform(rt_renderer *r, const rt_camera_uniform *u);
+int rt_update_lights(rt_renderer *r, const rt_light *lights, uint32_t n);   /* lib.rs:888-891 */
+int rt_reset_accumulation(rt_renderer *r);                                  /* lib.rs:893-902 */
+int rt_reset_accumulation_soft(rt_renderer *r);                             /* lib.rs:907-909 */
+int rt_rebuild_photons(rt_renderer *r);                                     /* lib.rs:913-1037 */
+uint32_t rt_sample_count(const rt_renderer *r);                             /* lib.rs:1039-1041 */
+/* Set the counter so that the next frame uses seed frame index `value + 1`. */
+int rt_set_sample_count(rt_renderer *r, uint32_t value);
+/* The mutable `pub config` (main.rs:505,545): width/height/photon_count/max_depth
+ * are fixed at creation and must not change. */
+int rt_get_config(const rt_renderer *r, rt_config *out);
+int rt_set_config(rt_renderer *r, const rt_config *cfg);
+int rt_get_grid_params(const rt_renderer *r, rt_grid_params *out);
+
+/* Read-backs (examples/common/mod.rs:6-30) and parity hooks. Sizes in elements. */
+int rt_read_accumulation(rt_renderer *r, float *rgba, size_t n_floats);
+int rt_read_frame(rt_renderer *r, float *rgba, size_t n_floats);
+int rt_read_moments(rt_renderer *r, float *rgba, size_t n_floats);
+int rt_read_gbuffer(rt_renderer *r, float *rgba, size_t n_floats);
+int rt_read_output_rgba8(rt_renderer *r, uint8_t *rgba, size_t n_bytes);
+int rt_write_accumulation(rt_renderer *r, const float *rgba, size_t n_floats);
+int rt_read_photons(rt_renderer *r, rt_photon *out, size_t capacity, uint32_t *n_slots);
+int rt_read_sorted_photons(rt_renderer *r, rt_photon *out, size_t capacity, uint32_t *n_stored);
+int rt_read_grid_meta(rt_renderer *r, uint32_t *meta, size_t n_uint32, uint32_t *n_cells);
+
+/* Closest-hit queries over the uploaded BVH for a fixed ray set (the trace_bvh
+ * of gather.wgsl:66-88 as an entry point). Host arrays. */
+int rt_trace_closest(rt_renderer *r, const rt_ray *rays, uint32_t n, rt_hit *out);
+/* Same, with ray and hit arrays already in device memory, on the renderer's stream. */
+int rt_trace_closest_device(rt_renderer *r, const rt_ray *d_rays, uint32_t n, rt_hit *d_out);
+/* Build the photon grid from a given photon array (count, scan, stable
+ * scatter: photon_count.wgsl, lib.rs:987-1007, photon_scatter.wgsl) into the
+ * renderer's grid, replacing traced photons. n_slots <= photon_count*max_depth. */
+int rt_build_photon_grid_from(rt_renderer *r, const rt_photon *photons, uint32_t n_slots);
+
+/* Per-pass timing with HIP events (cfg.flags & RT_FLAG_PROFILE). */
+typedef struct rt_pass_stats {
+    double ms[8];        /* gather, denoise, composite_tonemap, photon_trace, grid_build, trace_rays, accum64, spare */
+    uint64_t launches[8];
+} rt_pass_stats;
+#define RT_PASS_GATHER 0
+#define RT_PASS_DENOISE 1
+#define RT_PASS_COMPOSITE_TONEMAP 2
+#define RT_PASS_PHOTON_TRACE 3
+#define RT_PASS_GRID_BUILD 4
+#define RT_PASS_TRACE_RAYS 5
+#define RT_PASS_ACCUM64 6
+int rt_get_pass_stats(rt_renderer *r, rt_pass_stats *out);
+int rt_reset_pass_stats(rt_renderer *r);
+
+/* Traversal counters of one extra, untimed gather launch of the current frame
+ * state (rays, BVH nodes loaded, triangle records loaded, photon records read),
+ * for the algorithmic-bytes figure of the roofline. */
+typedef struct rt_trace_counters {
+    uint64_t ray_queries;
+    uint64_t nodes_visited;
+    uint64_t tris_tested;
+    uint64_t photon_reads;
+} rt_trace_counters;
+int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out);
+
+/* Convergence support: FP64 running sum of the displayed frames and its mean. */
+int rt_accum64_reset(rt_renderer *r);
+int rt_accum64_add_accumulation(rt_renderer *r);
+int rt_accum64_read_mean(rt_renderer *r, double *rgba, size_t n_doubles, uint64_t *n_frames);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RUSTRACER_HIP_H */

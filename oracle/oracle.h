@@ -1,0 +1,95 @@
+/*
+ * oracle.h — CPU restatement of rustracer's rendering kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (rustracer_amd/, include/)
+ * includes, links or loads this code; only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg use it, and only as the checker or the CPU
+ * baseline.
+ *
+ * Each function restates one WGSL entry point of the reference line by line
+ * (file:line cited at each definition in oracle.cpp), in IEEE float32 with
+ * FMA contraction off (built with -ffp-contract=off), WGSL vector semantics
+ * (dot = (x*x' + y*y') + z*z', select(f, t, c), mix(a, b, t) = a*(1-t) + b*t)
+ * and the implementation-defined builtins fixed in include/rt_wgsl_math.h.
+ *
+ * Not restated (parity unpinned, outside the configs this backend serves):
+ * texture sampling (every material must have tex_* == TEX_NONE; a sampled
+ * placeholder texel is (0,0,0,0) as in the reference's empty role arrays).
+ */
+#ifndef RT_ORACLE_H
+#define RT_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/rt_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_scene {
+    const rt_triangle *tris;
+    uint32_t n_tris;
+    const rt_bvh_node *nodes;
+    uint32_t n_nodes;
+    const uint32_t *prims;
+    uint32_t n_prims;
+    const rt_material *mats;
+    uint32_t n_mats;
+    const rt_light *lights; /* at least one record (the reference binds a zeroed one when empty) */
+    uint32_t n_lights;
+} or_scene;
+
+/* Photon-map inputs of the gather pass; grid_meta holds [start, count] per cell. */
+typedef struct or_photon_map {
+    const rt_grid_params *gp;
+    const uint32_t *grid_meta;
+    uint32_t meta_cells;
+    const rt_photon *sorted;
+    uint32_t n_sorted;
+} or_photon_map;
+
+/* Statistics of one gather call (all counts summed over the processed pixels). */
+typedef struct or_stats {
+    uint64_t ray_queries;   /* closest-hit traversals incl. shadow rays */
+    uint64_t nodes_visited; /* BVH nodes loaded */
+    uint64_t tris_tested;   /* triangle records loaded for an intersection test */
+    uint64_t photon_reads;  /* photon records read by the radius gather */
+    uint64_t light_index_overflow; /* rng_f32 * n rounded to n (reference reads lights[n]) */
+} or_stats;
+
+int or_trace_closest(const or_scene *s, const rt_ray *rays, uint32_t n, rt_hit *out, or_stats *st);
+int or_trace_brute(const or_scene *s, const rt_ray *rays, uint32_t n, rt_hit *out);
+
+int or_gather(const or_scene *s, const rt_camera_uniform *cam, const rt_image_params *ip,
+              const or_photon_map *pm, uint32_t max_depth, uint32_t row_begin, uint32_t row_end,
+              float *frame, float *moments, float *gbuffer, int n_threads, or_stats *st);
+
+int or_photon_trace(const or_scene *s, const rt_grid_params *gp, uint32_t n_photons,
+                    uint32_t max_depth, rt_photon *photons, int n_threads);
+
+int or_photon_grid(const rt_photon *photons, uint32_t n_slots, const rt_grid_params *gp,
+                   uint32_t meta_cells, uint32_t *grid_meta, rt_photon *sorted,
+                   uint32_t *n_sorted);
+
+int or_denoise_pass(const float *in, float *out, const float *gbuffer, const float *moments,
+                    const rt_denoise_params *p);
+int or_composite(const float *frame, float *accum, uint32_t w, uint32_t h, float alpha);
+int or_tonemap(const float *accum, const float *gbuffer, const rt_tonemap_uniform *p,
+               uint32_t w, uint32_t h, uint8_t *rgba);
+int or_tonemap_cpu(const float *accum, uint64_t n_px, uint8_t *rgba);
+
+/* PCG primitives, exposed for the known-answer tests. */
+uint32_t or_rng_init(uint32_t seed);
+uint32_t or_rng_next(uint32_t *state);
+float or_rng_f32(uint32_t *state);
+float or_intersect_triangle(const float *ro, const float *rd, const float *v0, const float *v1,
+                            const float *v2);
+/* The implementation-defined builtins, exposed for the accuracy tests. */
+void or_math_eval(int fn, const float *x, const float *y, float *out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT_ORACLE_H */

@@ -1,0 +1,366 @@
+// device_common.h — device-side vector math, PCG, Möller–Trumbore and BVH
+// traversal for the gfx950 kernels.
+//
+// Arithmetic follows WGSL's definitions in float32 with no FMA contraction
+// (every kernel is compiled with -ffp-contract=off): dot = (x*x' + y*y') + z*z',
+// normalize(v) = v * (1/sqrt(dot(v,v))), vector/scalar divides componentwise,
+// and the implementation-defined builtins come from rt_wgsl_math.h. That keeps
+// every per-pixel result identical to a CPU evaluation of the same WGSL.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rt_types.h"
+#include "../../include/rt_wgsl_math.h"
+
+namespace rtd {
+
+constexpr float kPi = 3.141592653589793f;  // common.wgsl:8
+constexpr float kInf = 1e38f;              // common.wgsl:9
+constexpr float kEps = 1e-6f;              // common.wgsl:10
+
+struct F3 {
+    float x, y, z;
+};
+struct F2 {
+    float x, y;
+};
+
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 fs(float s) { return F3{s, s, s}; }
+__device__ __forceinline__ F3 ld3(const float *p) { return F3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ F3 xyz(float4 v) { return F3{v.x, v.y, v.z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, F3 b) { return F3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ F3 operator/(F3 a, F3 b) { return F3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return F3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ F3 operator*(float s, F3 a) { return F3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ F3 operator/(F3 a, float s) { return F3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ F3 operator-(F3 a) { return F3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+    return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(F3 v) { return wgsl_sqrt(dot(v, v)); }
+__device__ __forceinline__ F3 normalize(F3 v) { return v * (1.0f / wgsl_sqrt(dot(v, v))); }
+__device__ __forceinline__ F3 vmin(F3 a, F3 b) { return F3{wmin(a.x, b.x), wmin(a.y, b.y), wmin(a.z, b.z)}; }
+__device__ __forceinline__ F3 mix(F3 a, F3 b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ F3 mix(F3 a, F3 b, F3 t) { return a * (fs(1.0f) - t) + b * t; }
+__device__ __forceinline__ float vmax3(F3 v) { return wmax(wmax(v.x, v.y), v.z); }
+
+// common.wgsl:12-25
+__device__ __forceinline__ F3 reflect(F3 v, F3 n) { return v - (2.0f * dot(v, n)) * n; }
+__device__ __forceinline__ F3 refract(F3 v, F3 n, float eta) {
+    const float cos_theta = wmin(dot(-v, n), 1.0f);
+    const F3 perp = eta * (v + cos_theta * n);
+    const F3 par = (-wgsl_sqrt(__builtin_fabsf(1.0f - dot(perp, perp)))) * n;
+    return perp + par;
+}
+__device__ __forceinline__ F3 fresnel_schlick(float cos_theta, F3 f0) {
+    return f0 + (fs(1.0f) - f0) * wgsl_pow(wclamp(1.0f - cos_theta, 0.0f, 1.0f), 5.0f);
+}
+
+// PCG of common.wgsl:29-56 (the increment is always 1)
+struct Rng {
+    uint32_t state;
+};
+__device__ __forceinline__ Rng rng_init(uint32_t seed) {
+    uint32_t s = seed * 747796405u + 2891336453u;
+    s = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return Rng{s};
+}
+__device__ __forceinline__ uint32_t rng_next(Rng &r) {
+    const uint32_t old = r.state;
+    r.state = old * 747796405u + 1u;
+    const uint32_t word = ((old >> ((old >> 28u) + 4u)) ^ old) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __forceinline__ float rng_f32(Rng &r) { return (float)rng_next(r) * 2.3283064365386963e-10f; }
+
+// common.wgsl:60-74
+__device__ __forceinline__ F3 sample_cosine_hemisphere(Rng &r) {
+    const float u1 = rng_f32(r);
+    const float u2 = rng_f32(r);
+    const float phi = (2.0f * kPi) * u1;
+    const float ct = wgsl_sqrt(u2);
+    const float st = wgsl_sqrt(1.0f - u2);
+    return F3{wgsl_cos(phi) * st, ct, wgsl_sin(phi) * st};
+}
+__device__ __forceinline__ F3 to_world(F3 l, F3 n) {
+    const F3 up = (__builtin_fabsf(n.y) < 0.999f) ? f3(0.0f, 1.0f, 0.0f) : f3(1.0f, 0.0f, 0.0f);
+    const F3 t = normalize(cross(up, n));
+    const F3 b = cross(n, t);
+    return t * l.x + n * l.y + b * l.z;
+}
+
+__device__ __forceinline__ F3 get_triangle_normal(const rt_triangle &tri) {
+    const F3 v0 = ld3(tri.v0);
+    return normalize(cross(ld3(tri.v1) - v0, ld3(tri.v2) - v0));
+}
+
+struct Bary {
+    F2 uv;
+    float u, v;
+};
+// common.wgsl:178-192
+__device__ __forceinline__ Bary barycentric(const rt_triangle &tri, F3 ro, F3 rd, float t) {
+    const F3 v0 = ld3(tri.v0);
+    const F3 e1 = ld3(tri.v1) - v0;
+    const F3 e2 = ld3(tri.v2) - v0;
+    const F3 p = (ro + rd * t) - v0;
+    const float d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2);
+    const float d20 = dot(p, e1), d21 = dot(p, e2);
+    const float denom = d00 * d11 - d01 * d01;
+    float u = 0.0f, v = 0.0f;
+    if (denom != 0.0f) {
+        u = (d11 * d20 - d01 * d21) / denom;
+        v = (d00 * d21 - d01 * d20) / denom;
+    }
+    const float w = 1.0f - u - v;
+    Bary b;
+    b.uv = F2{(tri.uv0[0] * w + tri.uv1[0] * u) + tri.uv2[0] * v, (tri.uv0[1] * w + tri.uv1[1] * u) + tri.uv2[1] * v};
+    b.u = u;
+    b.v = v;
+    return b;
+}
+// common.wgsl:196-201
+__device__ __forceinline__ F3 smooth_normal(const rt_triangle &tri, const Bary &b, F3 geometric) {
+    const float w = 1.0f - b.u - b.v;
+    const F3 n = ld3(tri.n0) * w + ld3(tri.n1) * b.u + ld3(tri.n2) * b.v;
+    if (dot(n, n) < 1e-8f) return geometric;
+    return normalize(n);
+}
+
+// common.wgsl:211-230 (used by the tangent/bitangent debug views)
+struct TanFrame {
+    F3 t, b;
+};
+__device__ __forceinline__ TanFrame tangent_frame(const rt_triangle &tri, F3 n) {
+    const F3 v0 = ld3(tri.v0);
+    const F3 e1 = ld3(tri.v1) - v0;
+    const F3 e2 = ld3(tri.v2) - v0;
+    const F2 d1 = F2{tri.uv1[0] - tri.uv0[0], tri.uv1[1] - tri.uv0[1]};
+    const F2 d2 = F2{tri.uv2[0] - tri.uv0[0], tri.uv2[1] - tri.uv0[1]};
+    const float det = d1.x * d2.y - d2.x * d1.y;
+    const float f = (__builtin_fabsf(det) < 1e-12f) ? 0.0f : 1.0f / det;
+    const F3 t = (e1 * d2.y - e2 * d1.y) * f;
+    const F3 b = (e2 * d1.x - e1 * d2.x) * f;
+    F3 tt = t - n * dot(n, t);
+    if (dot(tt, tt) < 1e-10f) {
+        tt = cross(n, f3(1.0f, 0.0f, 0.0f));
+        if (dot(tt, tt) < 1e-6f) tt = cross(n, f3(0.0f, 1.0f, 0.0f));
+    }
+    tt = normalize(tt);
+    const F3 bb = cross(n, tt) * ((dot(cross(n, tt), b) > 0.0f) ? 1.0f : -1.0f);
+    return TanFrame{tt, bb};
+}
+
+// ─────────────────────────── traversal ───────────────────────────
+
+// Slot-ordered traversal triangles: three float4 per BVH slot:
+// (v0, prim index bits), (v1 - v0, 0), (v2 - v0, 0). The edges are the same
+// float subtractions Möller–Trumbore performs, so results are unchanged.
+struct TraceScene {
+    const rt_bvh_node *nodes;  // reference layout, root at 0
+    const uint32_t *prims;     // slot -> primitive index
+    const float4 *slot_tris;   // 3 float4 per slot
+    const float4 *wnodes;      // child-box layout: 4 float4 per internal node
+    uint32_t n_nodes;
+    uint32_t pad;
+    float4 root_min;           // root box min (xyz) and root ref bits (w) for the child-box layout
+    float4 root_max;           // root box max (xyz)
+};
+
+struct Hit {
+    float t;
+    uint32_t prim;  // primitive index (closest hit)
+};
+
+struct TraceCount {
+    uint32_t nodes, tris;
+};
+
+// Möller–Trumbore on a slot record, common.wgsl:78-100 (edges precomputed).
+__device__ __forceinline__ float mt_slot(F3 ro, F3 rd, F3 v0, F3 edge1, F3 edge2) {
+    const F3 h = cross(rd, edge2);
+    const float a = dot(edge1, h);
+    if (__builtin_fabsf(a) < 1e-7f) return kInf;
+    const float f = 1.0f / a;
+    const F3 s = ro - v0;
+    const float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return kInf;
+    const F3 q = cross(s, edge1);
+    const float v = f * dot(rd, q);
+    if (v < 0.0f || u + v > 1.0f) return kInf;
+    const float t = f * dot(edge2, q);
+    if (t > kEps) return t;
+    return kInf;
+}
+
+// Per-lane traversal stack: the first kLdsStack entries live in LDS
+// (lane-interleaved rows, one bank per lane), deeper entries in private memory.
+constexpr int kLdsStack = 32;
+constexpr int kStackWords = 96;
+
+struct LaneStack {
+    uint32_t *lds;  // &shared[row 0][lane]; row stride = blockDim
+    uint32_t stride;
+    uint32_t spill[kStackWords - kLdsStack];
+    __device__ __forceinline__ void put(uint32_t sp, uint32_t v) {
+        if (sp < (uint32_t)kLdsStack) lds[sp * stride] = v;
+        else spill[sp - kLdsStack] = v;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t sp) const {
+        return sp < (uint32_t)kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+    }
+};
+
+// Slab test of gather.wgsl:72-75; min/max on the GPU use v_min/v_max whose
+// only difference from wmin/wmax is the sign of a zero, which no comparison sees.
+__device__ __forceinline__ void slab(F3 bmin, F3 bmax, F3 ro, F3 inv, float &te, float &ta) {
+    const float t0x = (bmin.x - ro.x) * inv.x, t1x = (bmax.x - ro.x) * inv.x;
+    const float t0y = (bmin.y - ro.y) * inv.y, t1y = (bmax.y - ro.y) * inv.y;
+    const float t0z = (bmin.z - ro.z) * inv.z, t1z = (bmax.z - ro.z) * inv.z;
+    te = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    ta = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+}
+
+// Reference-order traversal, gather.wgsl:66-88: left child first, strict
+// `te < hit.t` and `t < hit.t`, right child dropped when 63 entries are used.
+// ANY_HIT: stop at the first triangle with t < t_max (shadow query).
+template <bool ANY_HIT, bool COUNT>
+__device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
+    Hit hit{ANY_HIT ? t_max : kInf, 0u};
+    const F3 inv = fs(1.0f) / rd;
+    uint32_t idx = 0u, sp = 0u;
+    if (S.n_nodes == 0u) return hit;
+    for (;;) {
+        const rt_bvh_node &n = S.nodes[idx];
+        const float4 a = *reinterpret_cast<const float4 *>(&n.bbox_min[0]);
+        const float4 b = *reinterpret_cast<const float4 *>(&n.bbox_max[0]);
+        const uint32_t prim_count = n.prim_count;
+        if (COUNT) cnt.nodes++;
+        float te, ta;
+        slab(xyz(a), xyz(b), ro, inv, te, ta);
+        bool pop = true;
+        if (te <= ta && ta > 0.0f && te < hit.t) {
+            const uint32_t left_first = __float_as_uint(a.w);
+            if (prim_count > 0u) {
+                for (uint32_t i = 0u; i < prim_count; i++) {
+                    const uint32_t slot = left_first + i;
+                    const float4 q0 = S.slot_tris[3 * slot], q1 = S.slot_tris[3 * slot + 1], q2 = S.slot_tris[3 * slot + 2];
+                    if (COUNT) cnt.tris++;
+                    const float t = mt_slot(ro, rd, xyz(q0), xyz(q1), xyz(q2));
+                    if (t < hit.t) {
+                        hit.t = t;
+                        hit.prim = __float_as_uint(q0.w);
+                        if (ANY_HIT) return hit;
+                    }
+                }
+            } else {
+                if (sp < 63u) {
+                    stk.put(sp, __float_as_uint(b.w));
+                    sp += 1u;
+                }
+                idx = left_first;
+                pop = false;
+            }
+        }
+        if (pop) {
+            if (sp == 0u) break;
+            sp -= 1u;
+            idx = stk.get(sp);
+        }
+    }
+    return hit;
+}
+
+// Child-box traversal over the wide layout: each internal record holds both
+// children's boxes, so one 64-byte load tests two boxes, and children are
+// visited nearest first. To return exactly the reference's answer for any
+// visiting order, closest-hit queries prune a box only when te > best t (not
+// >=), and a hit replaces the best one when (t, slot) is lexicographically
+// smaller: the reference's left-first depth-first order meets slots in
+// ascending order, so its strict `t < hit.t` keeps the lowest slot among
+// equal t. Any-hit (shadow) queries keep the reference's strict pruning.
+// Record i (internal nodes only, root = 0):
+//   {lmin.xyz, lref}, {lmax.xyz, rref}, {rmin.xyz, 0}, {rmax.xyz, 0}
+// ref: internal child = node index; leaf = 0x80000000 | (count-1) << 27 | first slot.
+// Stack entries are two words: (ref, entry distance of the deferred child).
+__device__ __forceinline__ bool ref_is_leaf(uint32_t r) { return (r & 0x80000000u) != 0u; }
+
+template <bool ANY_HIT, bool COUNT>
+__device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
+    Hit hit{ANY_HIT ? t_max : kInf, 0u};
+    uint32_t best_slot = 0xFFFFFFFFu;
+    const F3 inv = fs(1.0f) / rd;
+    if (S.n_nodes == 0u) return hit;
+    float te, ta;
+    slab(xyz(S.root_min), xyz(S.root_max), ro, inv, te, ta);
+    if (COUNT) cnt.nodes++;
+    if (!(te <= ta && ta > 0.0f)) return hit;
+    uint32_t sp = 0u;
+    uint32_t cur = __float_as_uint(S.root_min.w);  // root ref (internal 0, or a leaf ref)
+    for (;;) {
+        if (ref_is_leaf(cur)) {
+            const uint32_t first = cur & 0x07FFFFFFu;
+            const uint32_t count = ((cur >> 27) & 15u) + 1u;
+            for (uint32_t i = 0u; i < count; i++) {
+                const uint32_t slot = first + i;
+                const float4 q0 = S.slot_tris[3 * slot], q1 = S.slot_tris[3 * slot + 1], q2 = S.slot_tris[3 * slot + 2];
+                if (COUNT) cnt.tris++;
+                const float t = mt_slot(ro, rd, xyz(q0), xyz(q1), xyz(q2));
+                if (ANY_HIT) {
+                    if (t < hit.t) {
+                        hit.t = t;
+                        hit.prim = __float_as_uint(q0.w);
+                        return hit;
+                    }
+                } else if (t < hit.t || (t == hit.t && t < kInf && slot < best_slot)) {
+                    hit.t = t;
+                    hit.prim = __float_as_uint(q0.w);
+                    best_slot = slot;
+                }
+            }
+        } else {
+            const float4 *w = S.wnodes + 4u * cur;
+            const float4 a = w[0], b = w[1], c = w[2], d = w[3];
+            if (COUNT) cnt.nodes += 2;
+            float tel, tal, ter, tar;
+            slab(xyz(a), xyz(b), ro, inv, tel, tal);
+            slab(xyz(c), xyz(d), ro, inv, ter, tar);
+            const bool hl = tel <= tal && tal > 0.0f && (ANY_HIT ? tel < hit.t : tel <= hit.t);
+            const bool hr = ter <= tar && tar > 0.0f && (ANY_HIT ? ter < hit.t : ter <= hit.t);
+            const uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(b.w);
+            if (hl && hr) {
+                const bool lnear = tel <= ter;
+                stk.put(sp, lnear ? rref : lref);
+                stk.put(sp + 1u, __float_as_uint(lnear ? ter : tel));
+                sp += 2u;
+                cur = lnear ? lref : rref;
+                continue;
+            }
+            if (hl || hr) {
+                cur = hl ? lref : rref;
+                continue;
+            }
+        }
+        // pop, skipping deferred children whose entry is already beyond the best hit
+        bool found = false;
+        while (sp > 0u) {
+            sp -= 2u;
+            const float et = __uint_as_float(stk.get(sp + 1u));
+            if (ANY_HIT ? et < hit.t : et <= hit.t) {
+                cur = stk.get(sp);
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return hit;
+}
+
+}  // namespace rtd

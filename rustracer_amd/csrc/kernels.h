@@ -1,0 +1,92 @@
+// kernels.h — launch wrappers of the gfx950 kernels (host-callable).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "../../include/rt_types.h"
+
+namespace rtk {
+
+// Traversal inputs shared by every tracing kernel (device pointers).
+struct TraceInputs {
+    const rt_bvh_node *nodes;
+    const uint32_t *prims;
+    const void *slot_tris;  // float4[3 * n_slots]
+    const void *wnodes;     // float4[4 * n_internal]
+    uint32_t n_nodes;
+    float root_min[4];      // w: root ref bits
+    float root_max[4];
+};
+
+struct GatherLaunch {
+    TraceInputs trace;
+    const rt_triangle *tris;
+    const rt_material *mats;
+    const rt_light *lights;
+    uint32_t n_lights;
+    uint32_t n_tris;
+    rt_camera_uniform cam;
+    rt_image_params ip;
+    rt_grid_params gp;
+    const uint32_t *grid_meta;  // [start, count] per cell
+    const rt_photon *sorted_photons;
+    void *frame;    // float4 per pixel
+    void *moments;  // float4 per pixel
+    void *gbuffer;  // float4 per pixel
+    uint32_t max_depth;
+    uint32_t wide;           // child-box traversal (else reference order)
+    uint32_t anyhit_shadow;  // early-exit shadow rays
+    unsigned long long *counters;  // non-null: count traffic (4 counters)
+};
+
+hipError_t launch_gather(const GatherLaunch &g, hipStream_t s);
+hipError_t launch_trace_rays(const TraceInputs &t, uint32_t wide, const rt_ray *rays, uint32_t n, rt_hit *out,
+                             hipStream_t s);
+
+struct PhotonLaunch {
+    TraceInputs trace;
+    const rt_triangle *tris;
+    const rt_material *mats;
+    const rt_light *lights;
+    uint32_t n_lights;
+    uint32_t n_tris;
+    rt_grid_params gp;
+    uint32_t n_photons;  // invocations (photon indices 0..n-1)
+    uint32_t max_depth;  // slots per photon
+    uint32_t wide;
+    rt_photon *photons;
+};
+hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s);
+
+// Grid build: keys + counts, scan, stable radix sort of (key, slot), gather.
+struct GridBuildLaunch {
+    const rt_photon *photons;
+    uint32_t n_slots;      // slots processed
+    rt_grid_params gp;
+    uint32_t meta_cells;   // >= num_cells
+    uint32_t *grid_meta;   // 2 per cell
+    rt_photon *sorted;
+    // temporaries (device), sized by grid_build_temp_bytes
+    void *temp;
+    size_t temp_bytes;
+    uint32_t *n_valid_out; // device u32
+};
+size_t grid_build_temp_bytes(uint32_t n_slots, uint32_t meta_cells);
+hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s);
+
+struct DenoiseLaunch {
+    const void *in;
+    void *out;
+    const void *gbuffer;
+    const void *moments;
+    rt_denoise_params p;
+};
+hipError_t launch_denoise(const DenoiseLaunch &d, hipStream_t s);
+
+hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rgba8, uint32_t w, uint32_t h,
+                                    float alpha, const rt_tonemap_uniform &tm, const void *gbuffer, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+hipError_t launch_accum64_add(const void *accum_f4, double *sum, size_t n_px, hipStream_t s);
+
+}  // namespace rtk

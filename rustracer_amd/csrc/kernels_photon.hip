@@ -1,0 +1,459 @@
+// kernels_photon.hip — photon tracing (rustracer-renderer/shaders/photon_trace.wgsl)
+// and the photon-grid build (photon_count.wgsl, the host scan of
+// rustracer-renderer/src/lib.rs:987-1007, photon_scatter.wgsl) for gfx950.
+//
+// The reference counts with global atomics, scans on the CPU after a read-back
+// and scatters with an atomic cursor, so photons inside a cell land in arrival
+// order. Here the whole build stays on the GPU and is stable (photons in a
+// cell keep ascending slot order):
+//   1. keys:    cell key per slot (invalid -> num_cells) + per-cell counts
+//   2. scan:    exclusive scan of the counts -> grid_meta [start, count]
+//   3. sort:    LSD radix sort of (key, slot), 8-bit digits; each pass builds
+//               per-workgroup digit histograms in LDS, scans them, and ranks
+//               equal digits inside a wavefront with ballots
+//   4. gather:  sorted_photons[i] = photons[slot of i]
+#include "device_common.h"
+#include "kernels.h"
+
+namespace rtd {
+
+// ─────────────────────────── photon tracing ───────────────────────────
+
+struct PhotonArgs {
+    TraceScene ts;
+    const rt_triangle *tris;
+    const rt_material *mats;
+    const rt_light *lights;
+    uint32_t n_lights;
+    uint32_t n_tris;
+    rt_grid_params gp;
+    uint32_t n_photons;
+    uint32_t max_depth;
+    rt_photon *photons;
+};
+
+// photon_trace.wgsl:91-97
+__device__ __forceinline__ F3 bounce_dir(F3 rd, F3 n, float rough, Rng &rng) {
+    const F3 mirror = reflect(rd, n);
+    const float blur = wgsl_smoothstep(0.0f, 1.0f, rough);
+    if (blur <= 0.001f) return mirror;
+    const F3 diff = sample_cosine_hemisphere(rng);
+    return normalize(mix(mirror, to_world(diff, n), blur));
+}
+
+template <bool WIDE>
+__global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    const uint32_t photon_index = blockIdx.x * 64u + threadIdx.x;
+    if (photon_index >= A.n_photons) return;
+    LaneStack stk;
+    stk.lds = &stack_rows[threadIdx.x];
+    stk.stride = 64u;
+    TraceCount cnt{0u, 0u};
+    const uint32_t num_lights = A.n_lights;
+    Rng rng = rng_init(photon_index * 1103515245u + 12345u);
+    uint32_t li = wgsl_u32(rng_f32(rng) * (float)num_lights);
+    li = li < num_lights ? li : num_lights - 1u;
+    const rt_light &light = A.lights[li];
+    const F3 ldata = ld3(light.data), lcolor = ld3(light.color);
+    F3 origin = ldata, dir;
+    F3 power;
+    // photon_trace.wgsl:121-175
+    if (light.kind == 0u) {
+        float u1, u2;
+        for (;;) {
+            u1 = rng_f32(rng) * 2.0f - 1.0f;
+            u2 = rng_f32(rng) * 2.0f - 1.0f;
+            if (u1 * u1 + u2 * u2 < 1.0f) break;
+        }
+        const float z = 1.0f - 2.0f * (u1 * u1 + u2 * u2);
+        const float r = wgsl_sqrt(1.0f - z * z);
+        dir = normalize(f3(r * 2.0f * u1, r * 2.0f * u2, z));
+        power = lcolor * light.intensity;
+    } else if (light.kind == 1u) {
+        const float disk_u = (rng_f32(rng) - 0.5f) * 10.0f;
+        const float disk_v = (rng_f32(rng) - 0.5f) * 10.0f;
+        const F3 up = (__builtin_fabsf(ldata.y) < 0.999f) ? f3(0.0f, 1.0f, 0.0f) : f3(1.0f, 0.0f, 0.0f);
+        const F3 tangent = normalize(cross(up, ldata));
+        const F3 bitangent = cross(ldata, tangent);
+        origin = disk_u * tangent + disk_v * bitangent - ldata * 100.0f;
+        dir = ldata;
+        power = lcolor * light.intensity;
+    } else if (light.kind == 2u) {
+        uint32_t ti = wgsl_u32(light.data[0]);
+        ti = ti < A.n_tris ? ti : A.n_tris - 1u;
+        const rt_triangle &tri = A.tris[ti];
+        const float u1 = rng_f32(rng);
+        const float u2 = rng_f32(rng);
+        const float su = wgsl_sqrt(u1);
+        const float u = 1.0f - su;
+        const float v = u2 * su;
+        const float w = 1.0f - u - v;
+        origin = ld3(tri.v0) * u + ld3(tri.v1) * v + ld3(tri.v2) * w;
+        const F3 n = get_triangle_normal(tri);
+        const F3 local_dir = sample_cosine_hemisphere(rng);
+        dir = to_world(local_dir, n);
+        power = lcolor * light.intensity;
+    } else if (light.kind == 3u) {
+        const F3 local_dir = sample_cosine_hemisphere(rng);
+        const F3 d = to_world(local_dir, f3(0.0f, 1.0f, 0.0f));
+        const rt_grid_params &gp = A.gp;
+        const F3 span = f3((float)gp.grid_x, (float)gp.grid_y, (float)gp.grid_z) * gp.cell_size;
+        const F3 center = f3(gp.room_min_x, gp.room_min_y, gp.room_min_z) + span * 0.5f;
+        const float R = wmax(wmax(span.x, span.y), span.z) * 1.2f + 0.05f;
+        origin = center + d * R;
+        dir = -d;
+        const float t = wclamp(d.y * 0.5f + 0.5f, 0.0f, 1.0f);
+        power = mix(lcolor, ldata, wgsl_pow(t, 0.5f)) * light.intensity;
+    } else {
+        return;
+    }
+
+    // photon_trace.wgsl:178-259
+    for (uint32_t bounce = 0u; bounce < A.max_depth; bounce++) {
+        const Hit hit = WIDE ? trace_wide<false, false>(A.ts, origin, dir, kInf, stk, cnt)
+                             : trace_reference<false, false>(A.ts, origin, dir, kInf, stk, cnt);
+        if (hit.t >= kInf) return;
+        const rt_triangle &tri = A.tris[hit.prim];
+        const rt_material &mat = A.mats[tri.material_id];
+        const F3 hit_pos = origin + dir * hit.t;
+        const Bary bb = barycentric(tri, origin, dir, hit.t);
+        const F3 normal = get_triangle_normal(tri);
+        const bool front_face = dot(dir, normal) < 0.0f;
+        const F3 sn_geom = smooth_normal(tri, bb, normal);
+        const F3 face_normal = front_face ? sn_geom : -sn_geom;
+        const F3 shn = face_normal;
+        if (mat.tex_emissive != RT_TEX_NONE || length(ld3(mat.emissive)) > 1e-4f || mat.kind == 3u) return;
+        if (mat.kind == 2u) {
+            const F3 f0 = fs(wgsl_pow((mat.ior - 1.0f) / (mat.ior + 1.0f), 2.0f));
+            const float cos_theta = __builtin_fabsf(dot(dir, face_normal));
+            const float fr = fresnel_schlick(cos_theta, f0).x;
+            if (rng_f32(rng) < fr) {
+                dir = reflect(dir, face_normal);
+                origin = hit_pos + face_normal * 0.001f;
+            } else {
+                const float eta = front_face ? 1.0f / mat.ior : mat.ior;
+                dir = refract(dir, face_normal, eta);
+                origin = hit_pos - face_normal * 0.001f;
+            }
+            continue;
+        }
+        float4 *rec = reinterpret_cast<float4 *>(A.photons + (size_t)photon_index * A.max_depth + bounce);
+        rec[0] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, 0.0f);
+        rec[1] = make_float4(power.x, power.y, power.z, 0.0f);
+        rec[2] = make_float4(-dir.x, -dir.y, -dir.z, 0.0f);
+        rec[3] = make_float4(shn.x, shn.y, shn.z, 0.0f);
+
+        const F3 albedo = f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
+        const F3 f0 = mix(fs(0.04f), albedo, fs(mat.metallic));
+        const float ct = __builtin_fabsf(dot(dir, shn));
+        const F3 fr = fresnel_schlick(ct, f0);
+        const F3 diffuse_weight = (1.0f - mat.metallic) * albedo;
+        const float spec_prob = wclamp(vmax3(fr), 0.0f, 0.98f);
+        const bool do_spec = rng_f32(rng) < spec_prob;
+        if (do_spec) {
+            dir = bounce_dir(dir, shn, mat.roughness, rng);
+            power = power * f0;
+        } else {
+            const F3 local_dir = sample_cosine_hemisphere(rng);
+            dir = to_world(local_dir, shn);
+            power = power * diffuse_weight;
+        }
+        origin = hit_pos + shn * 0.001f;
+        const float p = vmax3(power);
+        if (p <= 0.0f || rng_f32(rng) > p) return;
+        power = power / p;
+    }
+}
+
+// ─────────────────────────── grid build ───────────────────────────
+
+constexpr uint32_t kSortThreads = 256;  // 4 wavefronts
+constexpr uint32_t kSortItems = 8;      // keys per thread per tile
+constexpr uint32_t kSortTile = kSortThreads * kSortItems;
+constexpr uint32_t kRadixBits = 8;
+constexpr uint32_t kRadix = 1u << kRadixBits;
+
+// photon_count.wgsl:18-33: u32 cell of the photon, or `invalid` when the slot is
+// empty (power.r <= 0) or the cell index is outside the grid.
+__global__ __launch_bounds__(256) void grid_keys_kernel(const rt_photon *photons, uint32_t n_slots, rt_grid_params gp,
+                                                         uint32_t *keys, uint32_t *vals, uint32_t *counts) {
+    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
+    if (slot >= n_slots) return;
+    const float4 pos = *reinterpret_cast<const float4 *>(&photons[slot].position[0]);
+    const float pr = photons[slot].power[0];
+    uint32_t key = gp.num_cells;
+    if (pr > 0.0f) {
+        const uint32_t gx = wgsl_u32((pos.x - gp.room_min_x) / gp.cell_size);
+        const uint32_t gy = wgsl_u32((pos.y - gp.room_min_y) / gp.cell_size);
+        const uint32_t gz = wgsl_u32((pos.z - gp.room_min_z) / gp.cell_size);
+        const uint32_t cell = gx + gy * gp.grid_x + gz * gp.grid_x * gp.grid_y;
+        if (cell < gp.num_cells) {
+            key = cell;
+            atomicAdd(&counts[cell], 1u);
+        }
+    }
+    keys[slot] = key;
+    vals[slot] = slot;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads = 4 waves).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *lds_wave, uint32_t &total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64u; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63u) lds_wave[wave] = x;
+    __syncthreads();
+    uint32_t wave_off = 0, sum = 0;
+    for (uint32_t w = 0; w < kSortThreads / 64u; w++) {
+        const uint32_t t = lds_wave[w];
+        if (w < wave) wave_off += t;
+        sum += t;
+    }
+    __syncthreads();
+    total = sum;
+    return wave_off + x - v;
+}
+
+// Scan phase 1: per-tile sums of `in` (n values).
+__global__ __launch_bounds__(256) void scan_reduce_kernel(const uint32_t *in, uint32_t n, uint32_t *tile_sums) {
+    __shared__ uint32_t lds_wave[4];
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        const size_t i = base + (size_t)k * kSortThreads + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t total;
+    block_exclusive_scan(s, lds_wave, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Scan phase 2: exclusive scan of the tile sums in one workgroup (any count).
+__global__ __launch_bounds__(256) void scan_tiles_kernel(uint32_t *tile_sums, uint32_t n_tiles, uint32_t *grand_total) {
+    __shared__ uint32_t lds_wave[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n_tiles; base += kSortThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < n_tiles ? tile_sums[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, lds_wave, total);
+        if (i < n_tiles) tile_sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && grand_total) *grand_total = carry;
+}
+
+// Scan phase 3: exclusive scan inside each tile plus the tile offset.
+// Writes out[i] (or meta[2i] = start, meta[2i+1] = count when meta != null).
+__global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t *in, uint32_t n, const uint32_t *tile_offsets,
+                                                          uint32_t *out, uint32_t *meta) {
+    __shared__ uint32_t lds_wave[4];
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    // each thread owns kSortItems consecutive values
+    uint32_t v[kSortItems];
+    uint32_t s = 0;
+    const size_t my = base + (size_t)threadIdx.x * kSortItems;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        v[k] = (my + k < n) ? in[my + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = tile_offsets[blockIdx.x] + block_exclusive_scan(s, lds_wave, total);
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        if (my + k < n) {
+            if (meta) {
+                meta[2 * (my + k)] = run;
+                meta[2 * (my + k) + 1] = v[k];
+            } else {
+                out[my + k] = run;
+            }
+        }
+        run += v[k];
+    }
+}
+
+// Radix pass, step 1: digit histogram of each tile, stored digit-major
+// (hist[d * n_tiles + tile]) so one exclusive scan gives every (digit, tile) base.
+__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *keys, uint32_t n, uint32_t shift,
+                                                         uint32_t n_tiles, uint32_t *hist) {
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        const size_t i = base + (size_t)k * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1u)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Radix pass, step 2: stable scatter. The tile is consumed in kSortItems rounds
+// of 256 consecutive keys; inside a round each wavefront ranks its lanes among
+// equal digits with 8 ballots (bit-by-bit match), wavefronts are ordered
+// through LDS, and a running per-digit offset carries the order across rounds.
+__global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *keys_in, const uint32_t *vals_in,
+                                                            uint32_t n, uint32_t shift, uint32_t n_tiles,
+                                                            const uint32_t *hist_scanned, uint32_t *keys_out,
+                                                            uint32_t *vals_out) {
+    __shared__ uint32_t digit_base[kRadix];        // global base + items already placed
+    __shared__ uint32_t wave_counts[4][kRadix];    // per-round, per-wave digit counts
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    digit_base[threadIdx.x] = hist_scanned[(size_t)threadIdx.x * n_tiles + blockIdx.x];
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    const uint64_t lt_mask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        for (uint32_t w = 0; w < 4u; w++) wave_counts[w][threadIdx.x] = 0u;
+        __syncthreads();
+        const size_t i = base + (size_t)r * kSortThreads + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t digit = (key >> shift) & (kRadix - 1u);
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (uint32_t bit = 0; bit < kRadixBits; bit++) {
+            const uint64_t b = __ballot((digit >> bit) & 1u);
+            match &= ((digit >> bit) & 1u) ? b : ~b;
+        }
+        const uint32_t rank_in_wave = (uint32_t)__popcll(match & lt_mask);
+        if (valid && rank_in_wave == 0u) wave_counts[wave][digit] = (uint32_t)__popcll(match);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = digit_base[digit] + rank_in_wave;
+            for (uint32_t w = 0; w < wave; w++) off += wave_counts[w][digit];
+            keys_out[off] = key;
+            vals_out[off] = vals_in[i];
+        }
+        __syncthreads();
+        // advance the running base of every digit by this round's count
+        digit_base[threadIdx.x] += wave_counts[0][threadIdx.x] + wave_counts[1][threadIdx.x] +
+                                   wave_counts[2][threadIdx.x] + wave_counts[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// photon_scatter.wgsl:35-36 with the sorted order: copy the 64-byte records.
+__global__ __launch_bounds__(256) void grid_gather_kernel(const rt_photon *photons, const uint32_t *vals,
+                                                          const uint32_t *n_valid, rt_photon *sorted) {
+    // 4 lanes per photon, one float4 each
+    const size_t t = (size_t)blockIdx.x * 256u + threadIdx.x;
+    const size_t i = t >> 2;
+    const uint32_t part = (uint32_t)(t & 3u);
+    if (i >= *n_valid) return;
+    const float4 *src = reinterpret_cast<const float4 *>(photons + vals[i]);
+    float4 *dst = reinterpret_cast<float4 *>(sorted + i);
+    dst[part] = src[part];
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t value, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+TraceScene make_trace_scene(const rtk::TraceInputs &t);
+
+}  // namespace rtd
+
+namespace rtk {
+
+hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s) {
+    if (p.n_photons == 0) return hipSuccess;
+    rtd::PhotonArgs A;
+    A.ts = rtd::make_trace_scene(p.trace);
+    A.tris = p.tris;
+    A.mats = p.mats;
+    A.lights = p.lights;
+    A.n_lights = p.n_lights;
+    A.n_tris = p.n_tris;
+    A.gp = p.gp;
+    A.n_photons = p.n_photons;
+    A.max_depth = p.max_depth;
+    A.photons = p.photons;
+    const dim3 grid((p.n_photons + 63u) / 64u), block(64);
+    if (p.wide) hipLaunchKernelGGL((rtd::photon_trace_kernel<true>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((rtd::photon_trace_kernel<false>), grid, block, 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(rtd::fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+static uint32_t tiles_of(size_t n) { return (uint32_t)((n + rtd::kSortTile - 1) / rtd::kSortTile); }
+
+size_t grid_build_temp_bytes(uint32_t n_slots, uint32_t meta_cells) {
+    const size_t n = n_slots;
+    const uint32_t t = std::max(1u, tiles_of(n));
+    const uint32_t tc = std::max(1u, tiles_of(meta_cells));
+    const size_t hist = (size_t)rtd::kRadix * t;
+    const uint32_t th = std::max(1u, tiles_of(hist));
+    return 4 * align256(n * 4) +          // keys/vals x 2
+           align256((size_t)meta_cells * 4) +  // counts
+           align256(hist * 4) +           // histograms
+           align256((size_t)std::max(tc, th) * 4) + 256;  // tile sums + total
+}
+
+// Exclusive scan of `in[0..n)` into out (or into meta pairs).
+static hipError_t scan_u32(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *meta, uint32_t *tile_sums,
+                           uint32_t *total, hipStream_t s) {
+    const uint32_t tiles = std::max(1u, tiles_of(n));
+    hipLaunchKernelGGL(rtd::scan_reduce_kernel, dim3(tiles), dim3(256), 0, s, in, n, tile_sums);
+    hipLaunchKernelGGL(rtd::scan_tiles_kernel, dim3(1), dim3(256), 0, s, tile_sums, tiles, total);
+    hipLaunchKernelGGL(rtd::scan_apply_kernel, dim3(tiles), dim3(256), 0, s, in, n, tile_sums, out, meta);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
+    const uint32_t n = g.n_slots;
+    const uint32_t tiles = std::max(1u, tiles_of(n));
+    const size_t hist_n = (size_t)rtd::kRadix * tiles;
+    char *p = reinterpret_cast<char *>(g.temp);
+    uint32_t *keys_a = reinterpret_cast<uint32_t *>(p); p += align256((size_t)n * 4);
+    uint32_t *vals_a = reinterpret_cast<uint32_t *>(p); p += align256((size_t)n * 4);
+    uint32_t *keys_b = reinterpret_cast<uint32_t *>(p); p += align256((size_t)n * 4);
+    uint32_t *vals_b = reinterpret_cast<uint32_t *>(p); p += align256((size_t)n * 4);
+    uint32_t *counts = reinterpret_cast<uint32_t *>(p); p += align256((size_t)g.meta_cells * 4);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(p); p += align256(hist_n * 4);
+    uint32_t *tile_sums = reinterpret_cast<uint32_t *>(p);
+    hipError_t e;
+    if ((e = launch_fill_u32(counts, 0u, g.meta_cells, s)) != hipSuccess) return e;
+    if (n > 0) {
+        hipLaunchKernelGGL(rtd::grid_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g.photons, n, g.gp, keys_a,
+                           vals_a, counts);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    // grid_meta = exclusive scan of the counts; n_valid = their total
+    if ((e = scan_u32(counts, g.meta_cells, nullptr, g.grid_meta, tile_sums, g.n_valid_out, s)) != hipSuccess) return e;
+    if (n == 0) return hipSuccess;
+    // key bits needed for values 0..num_cells (num_cells = invalid)
+    uint32_t bits = 0;
+    while (bits < 32 && (1ull << bits) <= (uint64_t)g.gp.num_cells) bits++;
+    const uint32_t passes = std::max(1u, (bits + rtd::kRadixBits - 1) / rtd::kRadixBits);
+    uint32_t *kin = keys_a, *vin = vals_a, *kout = keys_b, *vout = vals_b;
+    for (uint32_t pass = 0; pass < passes; pass++) {
+        const uint32_t shift = pass * rtd::kRadixBits;
+        hipLaunchKernelGGL(rtd::radix_hist_kernel, dim3(tiles), dim3(256), 0, s, kin, n, shift, tiles, hist);
+        if ((e = scan_u32(hist, (uint32_t)hist_n, hist, nullptr, tile_sums, nullptr, s)) != hipSuccess) return e;
+        hipLaunchKernelGGL(rtd::radix_scatter_kernel, dim3(tiles), dim3(256), 0, s, kin, vin, n, shift, tiles, hist,
+                           kout, vout);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    const size_t threads = (size_t)n * 4;
+    hipLaunchKernelGGL(rtd::grid_gather_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g.photons,
+                       vin, g.n_valid_out, g.sorted);
+    return hipGetLastError();
+}
+
+}  // namespace rtk
