@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the rendering path on one MI355X (gfx950).
+
+Workload (default): the full 1920x1080 frame on the ~1M-triangle synthetic
+scene (600k-triangle value-noise terrain + 200 instanced 2k-triangle meshes,
+seed 1), 1 sample per pixel per frame, path depth 8, photon map of 1M photons
+traced once before timing (the reference's trace-once rebuild), a-trous on
+(2 passes), running-mean accumulation. One step = one Renderer::record_frame:
+gather + 2 a-trous passes + composite/tonemap. Inputs (scene, BVH, photons) are
+resident in HBM before the timed region.
+
+Prints one JSON line. `value` is Msamples/s (camera paths per second). The
+roofline block is for the dominant kernel (gather): algorithmic bytes per
+launch from the traversal counters of one extra, untimed launch, divided by the
+kernel's mean duration measured with HIP events on the renderer's stream.
+The cpu_baseline is the CPU oracle (a C++ restatement of the same WGSL, OpenMP)
+on a band of rows of the same frame, also checked bit-for-bit against the GPU rows.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s @1080p; ms to relMSE≤0.01: batched+learned vs plain à-trous"
+HBM_PEAK_GBS = 8000.0
+
+# Bytes per unit of work of the traversal layouts (roofline section).
+NODE_PAIR_BYTES = 64   # one child-box record tests two child boxes
+TRI_BYTES = 48         # slot triangle: v0, v1-v0, v2-v0 (3 x float4)
+RAY_BYTES = 32         # 24 B ray + 8 B hit
+PHOTON_BYTES = 64
+PIXEL_OUT_BYTES = 48   # frame + moments + gbuffer float4 writes
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--photons", type=int, default=1_000_000)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--reference-traversal", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import rustracer_amd as rt
+    from rustracer_amd import _ffi
+
+    t0 = time.perf_counter()
+    scene = rt.Scene.builtin("fullframe", 1)
+    cam = rt.Scene.builtin_camera("fullframe", 1)
+    t_scene = time.perf_counter() - t0
+    counts = scene.counts()
+    log(f"scene fullframe: {counts['triangles']} triangles ({t_scene:.1f}s)")
+    flags = _ffi.FLAG_PROFILE | (_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0)
+    cfg = rt.RenderConfig(width=args.width, height=args.height, spp=1, max_depth=args.depth,
+                          enable_photon_map=True, photon_count=args.photons, enable_denoise=True,
+                          accumulate_frames=1 << 20, flags=flags)
+    t0 = time.perf_counter()
+    r = rt.Renderer(cfg, scene, 0)
+    t_create = time.perf_counter() - t0
+    log(f"renderer created (flatten + BVH build + upload) in {t_create:.1f}s")
+    r.update_camera(cam)
+    t0 = time.perf_counter()
+    r.rebuild_photons()
+    r.synchronize()
+    t_photons = time.perf_counter() - t0
+    photon_stats = r.pass_stats()
+    r.reset_pass_stats()
+    log(f"photon map rebuilt in {t_photons*1e3:.1f} ms")
+
+    for _ in range(args.warmup):
+        r.record_frame()
+    r.synchronize()
+    r.reset_pass_stats()
+    log(f"warmup {args.warmup} frames done; timing {args.steps} frames")
+
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r.record_frame()
+        if (i + 1) % 10 == 0:
+            r.synchronize()
+            log(f"  {i + 1}/{args.steps} frames")
+    r.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = r.pass_stats()
+    w, h = args.width, args.height
+    samples = w * h * args.steps
+    msps = samples / elapsed / 1e6
+    ms_per_step = elapsed / args.steps * 1e3
+
+    # traversal traffic of the next frame (one untimed counting launch)
+    cnt = r.count_gather_traffic()
+    gather_ms, gather_n = stats["gather"]
+    gather_avg_ms = gather_ms / max(gather_n, 1)
+    alg_bytes = (cnt["nodes_visited"] / 2 * NODE_PAIR_BYTES + cnt["tris_tested"] * TRI_BYTES +
+                 cnt["ray_queries"] * RAY_BYTES + cnt["photon_reads"] * PHOTON_BYTES + w * h * PIXEL_OUT_BYTES)
+    achieved = alg_bytes / (gather_avg_ms * 1e-3) / 1e9
+    mrays = cnt["ray_queries"] * args.steps / elapsed / 1e6
+
+    result = {
+        "metric": METRIC,
+        "value": round(msps, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "fullframe-1080p: 1M-triangle terrain+instances, 1 spp/frame, depth "
+                               f"{args.depth}, {args.photons} photons, a-trous x2",
+                   "width": w, "height": h, "spp": 1, "depth": args.depth, "photons": args.photons,
+                   "triangles": counts["triangles"],
+                   "traversal": "reference-order" if args.reference_traversal else "child-box nearest-first"},
+        "mrays_per_s": round(mrays, 2),
+        "pass_ms_per_frame": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},
+        "photon_rebuild_ms": round(t_photons * 1e3, 2),
+        "photon_pass_ms": {k: round(v[0], 3) for k, v in photon_stats.items() if v[1]},
+        "setup_s": {"scene": round(t_scene, 2), "create_bvh_upload": round(t_create, 2)},
+        "gather_traffic_per_frame": cnt,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "gather", "kernel_avg_ms": round(gather_avg_ms, 4),
+                     "algorithmic_bytes_per_launch": int(alg_bytes)},
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(rt, r, scene, cam, cfg, args)
+    r.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def cpu_baseline(rt, r, scene, cam, cfg, args):
+    """The oracle on host cores over a band of rows of the frame the GPU just
+    rendered, until about --cpu-seconds of CPU time; the band is compared
+    bit-for-bit with the GPU's rows."""
+    import numpy as np
+
+    from tests.helpers import image_params
+    from tests.oracle_ffi import OracleScene
+
+    threads = min(16, os.cpu_count() or 1)
+    gp, meta, srt = r.grid_params(), r.read_grid_meta(), r.read_sorted_photons()
+    gpu_frame = r.read_frame()
+    c = r.config
+    frame_idx = r.sample_count()
+    o = OracleScene.from_scene(scene)
+    ip = image_params(c, frame_idx, alpha=1.0)
+    u = rt.camera_to_uniform(cam, c.width, c.height)
+    rows, elapsed, y = 0, 0.0, c.height // 2
+    band = 8
+    exact = True
+    while elapsed < args.cpu_seconds and y < c.height:
+        y1 = min(y + band, c.height)
+        t0 = time.perf_counter()
+        f, _, _, _ = o.gather(u, ip, max_depth=c.max_depth, rows=(y, y1), photon_map=(gp, meta, srt), threads=threads)
+        elapsed += time.perf_counter() - t0
+        exact = exact and bool(np.array_equal(f[y:y1], gpu_frame[y:y1]))
+        rows += y1 - y
+        y = y1
+    msps = rows * c.width / elapsed / 1e6
+    log(f"cpu baseline: {rows} rows on {threads} threads in {elapsed:.1f}s, bit-exact vs GPU: {exact}")
+    return {"value": round(msps, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} rows x {c.width} px x 1 spp of the same 1080p frame (rows from {c.height // 2})",
+            "sample_bit_exact_vs_gpu": exact}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

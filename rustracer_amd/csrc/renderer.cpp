@@ -1,0 +1,876 @@
+// renderer.cpp — the Renderer of rustracer-renderer/src/lib.rs on HIP: device
+// buffers, scene upload, per-frame pass sequence, photon-map rebuild,
+// read-backs, parity hooks and per-pass timing. Exposed through include/rustracer_hip.h.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "capi_util.h"
+#include "kernels.h"
+#include "scene.h"
+
+namespace {
+
+using rt::ApiError;
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw ApiError(RT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T *dev_alloc(size_t count, const char *what) {
+    void *p = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hip_check(hipMalloc(&p, bytes), what);
+    return static_cast<T *>(p);
+}
+
+struct PendingTiming {
+    int pass;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct rt_renderer {
+    rt_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    uint32_t n_tris = 0, n_nodes = 0, n_prims = 0, n_mats = 0, n_lights = 0, n_internal = 0;
+    rt_triangle *d_tris = nullptr;
+    rt_bvh_node *d_nodes = nullptr;
+    uint32_t *d_prims = nullptr;
+    rt_material *d_mats = nullptr;
+    rt_light *d_lights = nullptr;
+    float *d_slot_tris = nullptr;  // float4[3 * n_prims]
+    float *d_wnodes = nullptr;     // float4[4 * n_internal]
+    float root_min[4] = {0, 0, 0, 0}, root_max[4] = {0, 0, 0, 0};
+    bool wide_ok = false;
+    uint32_t bvh_depth = 0;
+
+    float *d_accum = nullptr, *d_frame = nullptr, *d_ping = nullptr, *d_moments = nullptr, *d_gbuf = nullptr;
+    uint32_t *d_rgba8 = nullptr;
+
+    rt_grid_params gp{};
+    uint32_t meta_cells = 0;
+    uint32_t photon_slots = 0;
+    rt_photon *d_photons = nullptr, *d_sorted = nullptr;
+    uint32_t *d_meta = nullptr;
+    void *d_temp = nullptr;
+    size_t temp_bytes = 0;
+    uint32_t *d_nvalid = nullptr;
+    uint32_t n_sorted = 0;
+    bool photons_built = false;
+
+    rt_camera_uniform cam{};
+    uint32_t sample_counter = 0;
+    uint32_t frames_since_reset = 0;
+
+    unsigned long long *d_counters = nullptr;
+    double *d_acc64 = nullptr;
+    uint64_t acc64_frames = 0;
+
+    std::vector<PendingTiming> pending;
+    rt_pass_stats stats{};
+
+    size_t n_px() const { return (size_t)cfg.width * cfg.height; }
+
+    ~rt_renderer() {
+        if (stream) hipStreamSynchronize(stream);
+        for (auto &p : pending) {
+            hipEventDestroy(p.start);
+            hipEventDestroy(p.stop);
+        }
+        void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
+                        d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
+                        d_temp,  d_nvalid, d_counters, d_acc64};
+        for (void *b : bufs)
+            if (b) hipFree(b);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    void begin(int pass, hipEvent_t &start) {
+        start = nullptr;
+        if (!(cfg.flags & RT_FLAG_PROFILE)) return;
+        hip_check(hipEventCreate(&start), "hipEventCreate");
+        hip_check(hipEventRecord(start, stream), "hipEventRecord");
+        (void)pass;
+    }
+    void end(int pass, hipEvent_t start) {
+        if (!start) return;
+        hipEvent_t stop;
+        hip_check(hipEventCreate(&stop), "hipEventCreate");
+        hip_check(hipEventRecord(stop, stream), "hipEventRecord");
+        pending.push_back({pass, start, stop});
+    }
+    void collect() {
+        for (auto &p : pending) {
+            hip_check(hipEventSynchronize(p.stop), "hipEventSynchronize");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, p.start, p.stop), "hipEventElapsedTime");
+            stats.ms[p.pass] += ms;
+            stats.launches[p.pass] += 1;
+            hipEventDestroy(p.start);
+            hipEventDestroy(p.stop);
+        }
+        pending.clear();
+    }
+
+    rtk::TraceInputs trace_inputs() const {
+        rtk::TraceInputs t;
+        t.nodes = d_nodes;
+        t.prims = d_prims;
+        t.slot_tris = d_slot_tris;
+        t.wnodes = d_wnodes;
+        t.n_nodes = n_nodes;
+        std::memcpy(t.root_min, root_min, sizeof(root_min));
+        std::memcpy(t.root_max, root_max, sizeof(root_max));
+        return t;
+    }
+    bool use_wide() const { return wide_ok && !(cfg.flags & RT_FLAG_REFERENCE_TRAVERSAL); }
+
+    rt_image_params image_params(float alpha) const {
+        rt_image_params ip;
+        std::memset(&ip, 0, sizeof(ip));
+        ip.width = cfg.width;
+        ip.height = cfg.height;
+        ip.spp = cfg.spp;
+        ip.frame = sample_counter;
+        std::memcpy(ip.light_emission, cfg.light_emission, sizeof(ip.light_emission));
+        ip.ambient = cfg.ambient;
+        ip.accumulate_alpha = alpha;
+        ip.use_photons = cfg.enable_photon_map ? 1u : 0u;
+        ip.photon_count = cfg.photon_count;
+        ip.photon_radius = cfg.photon_radius;
+        ip.photon_scale = cfg.photon_scale;
+        ip.photon_debug = cfg.photon_debug ? 1u : 0u;
+        std::memcpy(ip.point_light_emission, cfg.point_light_emission, sizeof(ip.point_light_emission));
+        ip._pad[0] = cfg.debug_view;
+        return ip;
+    }
+
+    rtk::GatherLaunch gather_launch(float alpha, float *frame, float *moments, float *gbuf) const {
+        rtk::GatherLaunch g;
+        g.trace = trace_inputs();
+        g.tris = d_tris;
+        g.mats = d_mats;
+        g.lights = d_lights;
+        g.n_lights = n_lights;
+        g.n_tris = std::max(n_tris, 1u);
+        g.cam = cam;
+        g.ip = image_params(alpha);
+        g.gp = gp;
+        g.grid_meta = d_meta;
+        g.sorted_photons = d_sorted;
+        g.frame = frame;
+        g.moments = moments;
+        g.gbuffer = gbuf;
+        g.max_depth = cfg.max_depth;
+        g.wide = use_wide() ? 1u : 0u;
+        g.anyhit_shadow = (cfg.flags & RT_FLAG_CLOSEST_SHADOW) ? 0u : 1u;
+        g.counters = nullptr;
+        return g;
+    }
+
+    float current_alpha() const {
+        if (cfg.accumulate_frames == 0) return 1.0f;
+        return 1.0f / (float)std::min(frames_since_reset, cfg.accumulate_frames);
+    }
+
+    // Renderer::record_frame, lib.rs:666-851 (the present pass is not part of this backend)
+    void render_frame() {
+        sample_counter = sample_counter + 1u;
+        frames_since_reset = frames_since_reset == 0xFFFFFFFFu ? frames_since_reset : frames_since_reset + 1u;
+        const float alpha = current_alpha();
+        if (cfg.enable_photon_map && !photons_built)
+            throw ApiError(RT_ERR_INVALID, "photon map enabled but rt_rebuild_photons has not run");
+        hipEvent_t ev;
+        begin(RT_PASS_GATHER, ev);
+        hip_check(rtk::launch_gather(gather_launch(alpha, d_frame, d_moments, d_gbuf), stream), "gather");
+        end(RT_PASS_GATHER, ev);
+        float *result = d_frame;
+        if (cfg.enable_denoise) {
+            float *src = d_frame, *dst = d_ping;
+            for (uint32_t it = 0; it < cfg.denoise_passes; it++) {
+                rtk::DenoiseLaunch d;
+                d.in = src;
+                d.out = dst;
+                d.gbuffer = d_gbuf;
+                d.moments = d_moments;
+                d.p.width = cfg.width;
+                d.p.height = cfg.height;
+                d.p.step_size = 1u << it;
+                d.p.lambda = 3.0f;
+                d.p.phi_normal = 12.0f;
+                d.p.phi_depth = 0.05f;
+                d.p.alpha = alpha;
+                d.p._pad = 0;
+                begin(RT_PASS_DENOISE, ev);
+                hip_check(rtk::launch_denoise(d, stream), "denoise");
+                end(RT_PASS_DENOISE, ev);
+                std::swap(src, dst);
+            }
+            result = src;
+        }
+        rt_tonemap_uniform tm;
+        std::memset(&tm, 0, sizeof(tm));
+        tm.exposure = cfg.exposure;
+        tm.gamma = cfg.gamma;
+        tm.width = cfg.width;
+        tm.glow = 1.0f;
+        begin(RT_PASS_COMPOSITE_TONEMAP, ev);
+        hip_check(rtk::launch_composite_tonemap(result, d_accum, d_rgba8, cfg.width, cfg.height, alpha, tm, d_gbuf,
+                                                stream),
+                  "composite_tonemap");
+        end(RT_PASS_COMPOSITE_TONEMAP, ev);
+    }
+
+    uint32_t traced_photons() const {
+        const uint32_t n = std::max(cfg.photon_count, 1u);
+        return n >= 256u ? (n / 256u) * 256u : n;
+    }
+    uint32_t processed_slots() const {
+        const uint64_t s = (uint64_t)std::max(cfg.photon_count, 1u) * cfg.max_depth;
+        const uint64_t d = std::max<uint64_t>(s / 256u, 1u) * 256u;
+        return (uint32_t)std::min<uint64_t>(d, photon_slots);
+    }
+
+    void grid_build(uint32_t n_slots) {
+        rtk::GridBuildLaunch g;
+        g.photons = d_photons;
+        g.n_slots = n_slots;
+        g.gp = gp;
+        g.meta_cells = meta_cells;
+        g.grid_meta = d_meta;
+        g.sorted = d_sorted;
+        g.temp = d_temp;
+        g.temp_bytes = temp_bytes;
+        g.n_valid_out = d_nvalid;
+        hipEvent_t ev;
+        begin(RT_PASS_GRID_BUILD, ev);
+        hip_check(rtk::launch_grid_build(g, stream), "grid build");
+        end(RT_PASS_GRID_BUILD, ev);
+        hip_check(hipMemcpyAsync(&n_sorted, d_nvalid, 4, hipMemcpyDeviceToHost, stream), "copy n_valid");
+        hip_check(hipStreamSynchronize(stream), "sync");
+        photons_built = true;
+    }
+
+    // Renderer::rebuild_photons, lib.rs:913-1037, with the scan and the
+    // scatter on the GPU and the buffers cleared first.
+    void rebuild_photons() {
+        if (!cfg.enable_photon_map) throw ApiError(RT_ERR_INVALID, "photon map disabled in the config");
+        hip_check(hipMemsetAsync(d_photons, 0, (size_t)photon_slots * sizeof(rt_photon), stream), "clear photons");
+        rtk::PhotonLaunch p;
+        p.trace = trace_inputs();
+        p.tris = d_tris;
+        p.mats = d_mats;
+        p.lights = d_lights;
+        p.n_lights = n_lights;
+        p.n_tris = std::max(n_tris, 1u);
+        p.gp = gp;
+        p.n_photons = traced_photons();
+        p.max_depth = cfg.max_depth;
+        p.wide = use_wide() ? 1u : 0u;
+        p.photons = d_photons;
+        hipEvent_t ev;
+        begin(RT_PASS_PHOTON_TRACE, ev);
+        hip_check(rtk::launch_photon_trace(p, stream), "photon trace");
+        end(RT_PASS_PHOTON_TRACE, ev);
+        grid_build(processed_slots());
+    }
+};
+
+namespace rt {
+
+// Renderer::new grid sizing, lib.rs:416-457; free cell size when requested.
+int grid_params_for(const rt_config &cfg, const float *bmin_in, const float *bmax_in, bool has_bounds,
+                    rt_grid_params &gp, float &radius, uint32_t &meta_cells) {
+    Vec3 bmin{-1.05f, -1.05f, -1.05f}, bmax{1.05f, 1.05f, 1.05f};
+    if (has_bounds) {
+        bmin = Vec3{bmin_in[0], bmin_in[1], bmin_in[2]};
+        bmax = Vec3{bmax_in[0], bmax_in[1], bmax_in[2]};
+    }
+    const Vec3 ext = bmax - bmin;
+    std::memset(&gp, 0, sizeof(gp));
+    auto sat_u32 = [](float v) -> uint64_t {
+        if (!(v > 0.0f)) return 0;
+        if (v >= 4294967296.0f) return 0xFFFFFFFFull;
+        return (uint64_t)v;
+    };
+    float cell;
+    uint64_t gx, gy, gz;
+    if (cfg.grid_cell_size > 0.0f) {
+        cell = cfg.grid_cell_size;
+        auto dim = [&](float e) { return std::max<uint64_t>(1, sat_u32(std::floor(e / cell + 0.5f)) + 1); };
+        gx = dim(ext.x);
+        gy = dim(ext.y);
+        gz = dim(ext.z);
+        radius = cell;
+    } else {
+        const float max_ext = std::fmax(max_element(ext), 1e-3f);
+        cell = max_ext / ((float)RT_GRID_MAX_DIM - 2.0f);
+        auto dim = [&](float e) {
+            return std::min<uint64_t>(std::max<uint64_t>(sat_u32(std::ceil(e / cell)), 1), RT_GRID_MAX_DIM);
+        };
+        gx = dim(ext.x);
+        gy = dim(ext.y);
+        gz = dim(ext.z);
+        radius = std::fmin(std::fmax(cell * 4.0f, 0.01f), 2.0f);
+    }
+    const uint64_t cells = gx * gy * gz;
+    if (cells > (1ull << 28)) return RT_ERR_CAPACITY;
+    const Vec3 room_min = bmin - Vec3{cell * 0.5f, cell * 0.5f, cell * 0.5f};
+    gp.room_min_x = room_min.x;
+    gp.room_min_y = room_min.y;
+    gp.room_min_z = room_min.z;
+    gp.cell_size = cell;
+    gp.grid_x = (uint32_t)gx;
+    gp.grid_y = (uint32_t)gy;
+    gp.grid_z = (uint32_t)gz;
+    gp.num_cells = (uint32_t)cells;
+    const uint64_t slots = cfg.enable_photon_map ? std::max<uint64_t>((uint64_t)cfg.photon_count * cfg.max_depth, 1) : 1;
+    if (slots > 0xFFFFFFFFull / 2) return RT_ERR_CAPACITY;
+    gp.photon_slots = (uint32_t)slots;
+    meta_cells = cfg.grid_cell_size > 0.0f ? (uint32_t)cells : std::max<uint32_t>((uint32_t)cells, RT_GRID_MAX_CELLS);
+    return RT_OK;
+}
+
+}  // namespace rt
+
+using rt::api_call;
+
+static void validate_config(const rt_config &c) {
+    if (c.width == 0 || c.height == 0 || (uint64_t)c.width * c.height > (1ull << 28))
+        throw ApiError(RT_ERR_INVALID, "width/height must be non-zero and width*height <= 2^28");
+    if (c.max_depth == 0 || c.max_depth > 64) throw ApiError(RT_ERR_INVALID, "max_depth must be in 1..64");
+    if (c.denoise_passes > 8) throw ApiError(RT_ERR_INVALID, "denoise_passes must be <= 8");
+    if (!(c.gamma != 0.0f)) throw ApiError(RT_ERR_INVALID, "gamma must be non-zero");
+}
+
+static void select_device(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw ApiError(RT_ERR_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) throw ApiError(RT_ERR_NO_DEVICE, "device ordinal out of range");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw ApiError(RT_ERR_NO_DEVICE, std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+    hip_check(hipSetDevice(device), "hipSetDevice");
+}
+
+// Derived traversal layouts: slot-ordered triangles and the child-box records.
+static void build_trace_layouts(rt_renderer &r, const rt_triangle *tris, const rt_bvh_node *nodes,
+                                const uint32_t *prims) {
+    std::vector<float> slot((size_t)std::max(r.n_prims, 1u) * 12, 0.0f);
+    for (uint32_t k = 0; k < r.n_prims; k++) {
+        if (prims[k] >= r.n_tris) throw ApiError(RT_ERR_INVALID, "BVH prim index out of range");
+        const rt_triangle &t = tris[prims[k]];
+        float *o = &slot[(size_t)k * 12];
+        uint32_t pid = prims[k];
+        o[0] = t.v0[0]; o[1] = t.v0[1]; o[2] = t.v0[2]; std::memcpy(&o[3], &pid, 4);
+        o[4] = t.v1[0] - t.v0[0]; o[5] = t.v1[1] - t.v0[1]; o[6] = t.v1[2] - t.v0[2]; o[7] = 0.0f;
+        o[8] = t.v2[0] - t.v0[0]; o[9] = t.v2[1] - t.v0[1]; o[10] = t.v2[2] - t.v0[2]; o[11] = 0.0f;
+    }
+    r.d_slot_tris = dev_alloc<float>(slot.size(), "slot triangles");
+    hip_check(hipMemcpy(r.d_slot_tris, slot.data(), slot.size() * 4, hipMemcpyHostToDevice), "upload slot triangles");
+
+    // validate the BVH and measure its depth
+    r.wide_ok = r.n_nodes > 0;
+    std::vector<uint32_t> compact(r.n_nodes, 0xFFFFFFFFu);
+    std::vector<uint32_t> order;
+    if (r.n_nodes > 0) {
+        std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 1u}};
+        std::vector<uint8_t> seen(r.n_nodes, 0);
+        while (!st.empty()) {
+            auto [i, d] = st.back();
+            st.pop_back();
+            if (i >= r.n_nodes || seen[i]) throw ApiError(RT_ERR_INVALID, "BVH child index invalid or shared");
+            seen[i] = 1;
+            r.bvh_depth = std::max(r.bvh_depth, d);
+            const rt_bvh_node &n = nodes[i];
+            if (n.prim_count > 0) {
+                if ((uint64_t)n.left_first + n.prim_count > r.n_prims)
+                    throw ApiError(RT_ERR_INVALID, "BVH leaf range out of range");
+                if (n.prim_count > 16 || n.left_first >= (1u << 27)) r.wide_ok = false;
+            } else {
+                compact[i] = (uint32_t)order.size();
+                order.push_back(i);
+                st.push_back({n.right_child, d + 1});
+                st.push_back({n.left_first, d + 1});
+            }
+        }
+    }
+    if (r.bvh_depth > 63) r.wide_ok = false;  // the reference would drop subtrees; keep its traversal
+    r.n_internal = (uint32_t)order.size();
+    auto ref_of = [&](uint32_t c) -> uint32_t {
+        const rt_bvh_node &n = nodes[c];
+        if (n.prim_count > 0) return 0x80000000u | ((n.prim_count - 1u) << 27) | n.left_first;
+        return compact[c];
+    };
+    std::vector<float> w((size_t)std::max(r.n_internal, 1u) * 16, 0.0f);
+    if (r.wide_ok) {
+        for (uint32_t k = 0; k < r.n_internal; k++) {
+            const rt_bvh_node &n = nodes[order[k]];
+            const rt_bvh_node &L = nodes[n.left_first], &R = nodes[n.right_child];
+            float *o = &w[(size_t)k * 16];
+            const uint32_t lref = ref_of(n.left_first), rref = ref_of(n.right_child);
+            o[0] = L.bbox_min[0]; o[1] = L.bbox_min[1]; o[2] = L.bbox_min[2]; std::memcpy(&o[3], &lref, 4);
+            o[4] = L.bbox_max[0]; o[5] = L.bbox_max[1]; o[6] = L.bbox_max[2]; std::memcpy(&o[7], &rref, 4);
+            o[8] = R.bbox_min[0]; o[9] = R.bbox_min[1]; o[10] = R.bbox_min[2];
+            o[12] = R.bbox_max[0]; o[13] = R.bbox_max[1]; o[14] = R.bbox_max[2];
+        }
+        const rt_bvh_node &root = nodes[0];
+        const uint32_t rootref = ref_of(0);
+        r.root_min[0] = root.bbox_min[0]; r.root_min[1] = root.bbox_min[1]; r.root_min[2] = root.bbox_min[2];
+        std::memcpy(&r.root_min[3], &rootref, 4);
+        r.root_max[0] = root.bbox_max[0]; r.root_max[1] = root.bbox_max[1]; r.root_max[2] = root.bbox_max[2];
+    }
+    r.d_wnodes = dev_alloc<float>(w.size(), "wide nodes");
+    hip_check(hipMemcpy(r.d_wnodes, w.data(), w.size() * 4, hipMemcpyHostToDevice), "upload wide nodes");
+}
+
+static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *tris, uint32_t n_tris,
+                                    const rt_bvh_node *nodes, uint32_t n_nodes, const uint32_t *prims, uint32_t n_prims,
+                                    const rt_material *mats, uint32_t n_mats, const rt_light *lights, uint32_t n_lights,
+                                    const float *bmin, const float *bmax, int device) {
+    validate_config(cfg_in);
+    for (uint32_t i = 0; i < n_mats; i++) {
+        const rt_material &m = mats[i];
+        if (m.tex_albedo != RT_TEX_NONE || m.tex_mr != RT_TEX_NONE || m.tex_emissive != RT_TEX_NONE ||
+            m.tex_normal != RT_TEX_NONE)
+            throw ApiError(RT_ERR_UNSUPPORTED, "textured materials are not supported by this backend");
+    }
+    for (uint32_t i = 0; i < n_tris; i++)
+        if (tris[i].material_id >= std::max(n_mats, 1u))
+            throw ApiError(RT_ERR_INVALID, "triangle material id out of range");
+    if ((n_nodes == 0) != (n_prims == 0) && n_tris > 0) throw ApiError(RT_ERR_INVALID, "inconsistent BVH arrays");
+    select_device(device);
+    auto r = std::make_unique<rt_renderer>();
+    r->cfg = cfg_in;
+    r->device = device;
+    hip_check(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking), "hipStreamCreate");
+    r->n_tris = n_tris;
+    r->n_nodes = n_nodes;
+    r->n_prims = n_prims;
+    r->n_mats = n_mats;
+    r->n_lights = std::max(n_lights, 1u);
+
+    // scene upload (lib.rs:234-271); empty arrays bind one zeroed record (buffer.rs:19-34)
+    r->d_tris = dev_alloc<rt_triangle>(n_tris, "triangles");
+    r->d_nodes = dev_alloc<rt_bvh_node>(n_nodes, "bvh nodes");
+    r->d_prims = dev_alloc<uint32_t>(n_prims, "bvh prims");
+    r->d_mats = dev_alloc<rt_material>(n_mats, "materials");
+    r->d_lights = dev_alloc<rt_light>(r->n_lights, "lights");
+    hip_check(hipMemset(r->d_tris, 0, sizeof(rt_triangle) * std::max(n_tris, 1u)), "memset");
+    hip_check(hipMemset(r->d_mats, 0, sizeof(rt_material) * std::max(n_mats, 1u)), "memset");
+    hip_check(hipMemset(r->d_lights, 0, sizeof(rt_light) * r->n_lights), "memset");
+    if (n_tris) hip_check(hipMemcpy(r->d_tris, tris, sizeof(rt_triangle) * n_tris, hipMemcpyHostToDevice), "upload");
+    if (n_nodes) hip_check(hipMemcpy(r->d_nodes, nodes, sizeof(rt_bvh_node) * n_nodes, hipMemcpyHostToDevice), "upload");
+    if (n_prims) hip_check(hipMemcpy(r->d_prims, prims, 4ull * n_prims, hipMemcpyHostToDevice), "upload");
+    if (n_mats) hip_check(hipMemcpy(r->d_mats, mats, sizeof(rt_material) * n_mats, hipMemcpyHostToDevice), "upload");
+    if (n_lights) hip_check(hipMemcpy(r->d_lights, lights, sizeof(rt_light) * n_lights, hipMemcpyHostToDevice), "upload");
+    build_trace_layouts(*r, tris, nodes, prims);
+
+    // per-pixel buffers (lib.rs:287-316)
+    const size_t npx = r->n_px();
+    r->d_accum = dev_alloc<float>(npx * 4, "accumulation");
+    r->d_frame = dev_alloc<float>(npx * 4, "frame");
+    r->d_ping = dev_alloc<float>(npx * 4, "denoise pingpong");
+    r->d_moments = dev_alloc<float>(npx * 4, "frame moments");
+    r->d_gbuf = dev_alloc<float>(npx * 4, "gbuffer");
+    r->d_rgba8 = dev_alloc<uint32_t>(npx, "output");
+    for (float *b : {r->d_accum, r->d_frame, r->d_ping, r->d_moments, r->d_gbuf})
+        hip_check(hipMemset(b, 0, npx * 16), "memset");
+    hip_check(hipMemset(r->d_rgba8, 0, npx * 4), "memset");
+
+    // photon grid (lib.rs:416-479)
+    float radius = 0.0f;
+    const int gs = rt::grid_params_for(r->cfg, bmin, bmax, bmin && bmax, r->gp, radius, r->meta_cells);
+    if (gs != RT_OK) throw ApiError(gs, "photon grid or photon buffer exceeds the supported size");
+    r->cfg.photon_radius = radius;
+    r->photon_slots = r->gp.photon_slots;
+    r->d_photons = dev_alloc<rt_photon>(r->photon_slots, "photons");
+    r->d_sorted = dev_alloc<rt_photon>(r->photon_slots, "sorted photons");
+    r->d_meta = dev_alloc<uint32_t>((size_t)r->meta_cells * 2, "grid meta");
+    hip_check(hipMemset(r->d_photons, 0, sizeof(rt_photon) * r->photon_slots), "memset");
+    hip_check(hipMemset(r->d_sorted, 0, sizeof(rt_photon) * r->photon_slots), "memset");
+    hip_check(hipMemset(r->d_meta, 0, 8ull * r->meta_cells), "memset");
+    r->temp_bytes = rtk::grid_build_temp_bytes(r->photon_slots, r->meta_cells);
+    r->d_temp = dev_alloc<char>(r->temp_bytes, "grid temp");
+    r->d_nvalid = dev_alloc<uint32_t>(1, "n_valid");
+    r->d_counters = dev_alloc<unsigned long long>(4, "counters");
+
+    // default camera (lib.rs:266-270)
+    r->cam = rt::camera_to_uniform(rt::camera_default(), r->cfg.width, r->cfg.height);
+    hip_check(hipDeviceSynchronize(), "upload sync");
+    return r.release();
+}
+
+extern "C" {
+
+int rt_config_default(rt_config *c) {
+    return api_call([&] {
+        if (!c) throw ApiError(RT_ERR_INVALID, "null config");
+        std::memset(c, 0, sizeof(*c));
+        c->width = 800;
+        c->height = 600;
+        c->photon_count = 1000000;
+        c->spp = 1;
+        c->exposure = 1.0f;
+        c->gamma = 2.2f;
+        for (int k = 0; k < 3; k++) c->light_emission[k] = c->point_light_emission[k] = 1.0f;
+        c->accumulate_frames = 0;
+        c->enable_photon_map = 0;
+        c->enable_denoise = 1;
+        c->photon_debug = 0;
+        c->debug_view = 0;
+        c->photon_scale = 0.02f;
+        c->photon_radius = 0.5f;
+        c->ambient = 0.03f;
+        c->max_depth = RT_MAX_BOUNCES;
+        c->grid_cell_size = 0.0f;
+        c->denoise_passes = 2;
+        c->flags = 0;
+    });
+}
+
+int rt_grid_params_for(const rt_config *cfg, const float bmin[3], const float bmax[3], int has_bounds,
+                       rt_grid_params *out, float *photon_radius) {
+    return api_call([&] {
+        if (!cfg || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        float radius = 0.0f;
+        uint32_t meta = 0;
+        const int s = rt::grid_params_for(*cfg, bmin, bmax, has_bounds && bmin && bmax, *out, radius, meta);
+        if (s != RT_OK) throw ApiError(s, "grid too large");
+        if (photon_radius) *photon_radius = radius;
+    });
+}
+
+int rt_renderer_create_raw(const rt_config *cfg, const rt_triangle *tris, uint32_t n_tris, const rt_bvh_node *nodes,
+                           uint32_t n_nodes, const uint32_t *prims, uint32_t n_prims, const rt_material *mats,
+                           uint32_t n_mats, const rt_light *lights, uint32_t n_lights, const float *bounds_min,
+                           const float *bounds_max, int device, rt_renderer **out) {
+    return api_call([&] {
+        if (!cfg || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        if ((n_tris && !tris) || (n_nodes && !nodes) || (n_prims && !prims) || (n_mats && !mats) ||
+            (n_lights && !lights))
+            throw ApiError(RT_ERR_INVALID, "null array with non-zero count");
+        *out = create_renderer(*cfg, tris, n_tris, nodes, n_nodes, prims, n_prims, mats, n_mats, lights, n_lights,
+                               bounds_min, bounds_max, device);
+    });
+}
+
+int rt_renderer_create(const rt_config *cfg, const rt_scene *scene, int device, rt_renderer **out) {
+    return api_call([&] {
+        if (!cfg || !scene || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        const rt::Scene &s = rt::scene_of(scene);
+        const std::vector<rt_triangle> tris = s.flatten_triangles();
+        const rt::BVH bvh = s.build_bvh();
+        const std::vector<rt_light> lights = s.lights_gpu();
+        rt::Vec3 bmin, bmax;
+        const bool hb = s.bounds(bmin, bmax);
+        const float lo[3] = {bmin.x, bmin.y, bmin.z}, hi[3] = {bmax.x, bmax.y, bmax.z};
+        *out = create_renderer(*cfg, tris.data(), (uint32_t)tris.size(), bvh.nodes.data(), (uint32_t)bvh.nodes.size(),
+                               bvh.prim_indices.data(), (uint32_t)bvh.prim_indices.size(), s.materials.data(),
+                               (uint32_t)s.materials.size(), lights.data(), (uint32_t)lights.size(),
+                               hb ? lo : nullptr, hb ? hi : nullptr, device);
+    });
+}
+
+void rt_renderer_destroy(rt_renderer *r) { delete r; }
+
+int rt_render_frame(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        r->render_frame();
+    });
+}
+
+int rt_render_frames(rt_renderer *r, uint32_t n) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        for (uint32_t i = 0; i < n; i++) r->render_frame();
+    });
+}
+
+int rt_synchronize(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        hip_check(hipStreamSynchronize(r->stream), "hipStreamSynchronize");
+        r->collect();
+    });
+}
+
+int rt_update_camera(rt_renderer *r, const rt_camera *cam) {
+    return api_call([&] {
+        if (!r || !cam) throw ApiError(RT_ERR_INVALID, "null argument");
+        r->cam = rt::camera_to_uniform(*cam, r->cfg.width, r->cfg.height);
+    });
+}
+
+int rt_update_camera_uniform(rt_renderer *r, const rt_camera_uniform *u) {
+    return api_call([&] {
+        if (!r || !u) throw ApiError(RT_ERR_INVALID, "null argument");
+        r->cam = *u;
+    });
+}
+
+int rt_update_lights(rt_renderer *r, const rt_light *lights, uint32_t n) {
+    return api_call([&] {
+        if (!r || (n && !lights)) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n > r->n_lights) throw ApiError(RT_ERR_CAPACITY, "more lights than uploaded at creation");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        if (n) hip_check(hipMemcpy(r->d_lights, lights, sizeof(rt_light) * n, hipMemcpyHostToDevice), "upload lights");
+        // TypedBuffer::write overwrites the leading records only (buffer.rs:64-66)
+    });
+}
+
+int rt_reset_accumulation(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        hip_check(hipMemsetAsync(r->d_accum, 0, r->n_px() * 16, r->stream), "clear accumulation");
+        r->frames_since_reset = 0;
+    });
+}
+
+int rt_reset_accumulation_soft(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        r->frames_since_reset = 0;
+    });
+}
+
+int rt_rebuild_photons(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        r->rebuild_photons();
+    });
+}
+
+uint32_t rt_sample_count(const rt_renderer *r) { return r ? r->sample_counter : 0u; }
+
+int rt_set_sample_count(rt_renderer *r, uint32_t value) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        r->sample_counter = value;
+    });
+}
+
+int rt_get_config(const rt_renderer *r, rt_config *out) {
+    return api_call([&] {
+        if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        *out = r->cfg;
+    });
+}
+
+int rt_set_config(rt_renderer *r, const rt_config *cfg) {
+    return api_call([&] {
+        if (!r || !cfg) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (cfg->width != r->cfg.width || cfg->height != r->cfg.height || cfg->photon_count != r->cfg.photon_count ||
+            cfg->max_depth != r->cfg.max_depth || cfg->grid_cell_size != r->cfg.grid_cell_size ||
+            cfg->enable_photon_map != r->cfg.enable_photon_map)
+            throw ApiError(RT_ERR_INVALID, "size, photon and grid fields are fixed at creation");
+        validate_config(*cfg);
+        r->cfg = *cfg;
+    });
+}
+
+int rt_get_grid_params(const rt_renderer *r, rt_grid_params *out) {
+    return api_call([&] {
+        if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        *out = r->gp;
+    });
+}
+
+static void read_dev(rt_renderer *r, void *dst, const void *src, size_t bytes) {
+    hip_check(hipStreamSynchronize(r->stream), "sync");
+    hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "read back");
+}
+
+static int read_px4(rt_renderer *r, float *dst, size_t n, const float *src) {
+    return api_call([&] {
+        if (!r || !dst) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n < r->n_px() * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than width*height*4 floats");
+        read_dev(r, dst, src, r->n_px() * 16);
+    });
+}
+
+int rt_read_accumulation(rt_renderer *r, float *rgba, size_t n) { return read_px4(r, rgba, n, r ? r->d_accum : nullptr); }
+int rt_read_frame(rt_renderer *r, float *rgba, size_t n) { return read_px4(r, rgba, n, r ? r->d_frame : nullptr); }
+int rt_read_moments(rt_renderer *r, float *rgba, size_t n) { return read_px4(r, rgba, n, r ? r->d_moments : nullptr); }
+int rt_read_gbuffer(rt_renderer *r, float *rgba, size_t n) { return read_px4(r, rgba, n, r ? r->d_gbuf : nullptr); }
+
+int rt_read_output_rgba8(rt_renderer *r, uint8_t *rgba, size_t n) {
+    return api_call([&] {
+        if (!r || !rgba) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n < r->n_px() * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than width*height*4 bytes");
+        read_dev(r, rgba, r->d_rgba8, r->n_px() * 4);
+    });
+}
+
+int rt_write_accumulation(rt_renderer *r, const float *rgba, size_t n) {
+    return api_call([&] {
+        if (!r || !rgba) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n < r->n_px() * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than width*height*4 floats");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        hip_check(hipMemcpy(r->d_accum, rgba, r->n_px() * 16, hipMemcpyHostToDevice), "write accumulation");
+    });
+}
+
+int rt_read_photons(rt_renderer *r, rt_photon *out, size_t cap, uint32_t *n_slots) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (n_slots) *n_slots = r->photon_slots;
+        if (!out) return;
+        if (cap < r->photon_slots) throw ApiError(RT_ERR_INVALID, "capacity below photon slot count");
+        read_dev(r, out, r->d_photons, sizeof(rt_photon) * r->photon_slots);
+    });
+}
+
+int rt_read_sorted_photons(rt_renderer *r, rt_photon *out, size_t cap, uint32_t *n_stored) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (n_stored) *n_stored = r->n_sorted;
+        if (!out) return;
+        if (cap < r->n_sorted) throw ApiError(RT_ERR_INVALID, "capacity below stored photon count");
+        if (r->n_sorted) read_dev(r, out, r->d_sorted, sizeof(rt_photon) * r->n_sorted);
+    });
+}
+
+int rt_read_grid_meta(rt_renderer *r, uint32_t *meta, size_t n, uint32_t *n_cells) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (n_cells) *n_cells = r->meta_cells;
+        if (!meta) return;
+        if (n < 2ull * r->meta_cells) throw ApiError(RT_ERR_INVALID, "capacity below 2*cells");
+        read_dev(r, meta, r->d_meta, 8ull * r->meta_cells);
+    });
+}
+
+int rt_trace_closest_device(rt_renderer *r, const rt_ray *d_rays, uint32_t n, rt_hit *d_out) {
+    return api_call([&] {
+        if (!r || (n && (!d_rays || !d_out))) throw ApiError(RT_ERR_INVALID, "null argument");
+        hipEvent_t ev;
+        r->begin(RT_PASS_TRACE_RAYS, ev);
+        hip_check(rtk::launch_trace_rays(r->trace_inputs(), r->use_wide() ? 1u : 0u, d_rays, n, d_out, r->stream),
+                  "trace rays");
+        r->end(RT_PASS_TRACE_RAYS, ev);
+    });
+}
+
+int rt_trace_closest(rt_renderer *r, const rt_ray *rays, uint32_t n, rt_hit *out) {
+    return api_call([&] {
+        if (!r || (n && (!rays || !out))) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n == 0) return;
+        rt_ray *d_r = dev_alloc<rt_ray>(n, "rays");
+        rt_hit *d_h = nullptr;
+        try {
+            d_h = dev_alloc<rt_hit>(n, "hits");
+            hip_check(hipMemcpy(d_r, rays, sizeof(rt_ray) * n, hipMemcpyHostToDevice), "upload rays");
+            hip_check(rtk::launch_trace_rays(r->trace_inputs(), r->use_wide() ? 1u : 0u, d_r, n, d_h, r->stream),
+                      "trace rays");
+            hip_check(hipStreamSynchronize(r->stream), "sync");
+            hip_check(hipMemcpy(out, d_h, sizeof(rt_hit) * n, hipMemcpyDeviceToHost), "read hits");
+        } catch (...) {
+            hipFree(d_r);
+            if (d_h) hipFree(d_h);
+            throw;
+        }
+        hipFree(d_r);
+        hipFree(d_h);
+    });
+}
+
+int rt_build_photon_grid_from(rt_renderer *r, const rt_photon *photons, uint32_t n_slots) {
+    return api_call([&] {
+        if (!r || (n_slots && !photons)) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (!r->cfg.enable_photon_map) throw ApiError(RT_ERR_INVALID, "photon map disabled in the config");
+        if (n_slots > r->photon_slots) throw ApiError(RT_ERR_CAPACITY, "more slots than the photon buffer holds");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        hip_check(hipMemset(r->d_photons, 0, sizeof(rt_photon) * r->photon_slots), "clear photons");
+        if (n_slots)
+            hip_check(hipMemcpy(r->d_photons, photons, sizeof(rt_photon) * n_slots, hipMemcpyHostToDevice), "upload");
+        r->grid_build(n_slots);
+    });
+}
+
+int rt_get_pass_stats(rt_renderer *r, rt_pass_stats *out) {
+    return api_call([&] {
+        if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        r->collect();
+        *out = r->stats;
+    });
+}
+
+int rt_reset_pass_stats(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        r->collect();
+        std::memset(&r->stats, 0, sizeof(r->stats));
+    });
+}
+
+int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
+    return api_call([&] {
+        if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (r->cfg.enable_photon_map && !r->photons_built)
+            throw ApiError(RT_ERR_INVALID, "photon map enabled but not built");
+        hip_check(hipMemsetAsync(r->d_counters, 0, 32, r->stream), "clear counters");
+        const uint32_t saved = r->sample_counter;
+        r->sample_counter = saved + 1u;  // the frame the next rt_render_frame would trace
+        rtk::GatherLaunch g = r->gather_launch(1.0f, r->d_ping, r->d_ping, r->d_ping);
+        r->sample_counter = saved;
+        g.counters = r->d_counters;
+        hip_check(rtk::launch_gather(g, r->stream), "gather (counting)");
+        unsigned long long c[4];
+        read_dev(r, c, r->d_counters, sizeof(c));
+        out->ray_queries = c[0];
+        out->nodes_visited = c[1];
+        out->tris_tested = c[2];
+        out->photon_reads = c[3];
+    });
+}
+
+int rt_accum64_reset(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (!r->d_acc64) r->d_acc64 = dev_alloc<double>(r->n_px() * 4, "fp64 sum");
+        hip_check(hipMemsetAsync(r->d_acc64, 0, r->n_px() * 32, r->stream), "clear fp64 sum");
+        r->acc64_frames = 0;
+    });
+}
+
+int rt_accum64_add_accumulation(rt_renderer *r) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (!r->d_acc64) throw ApiError(RT_ERR_INVALID, "call rt_accum64_reset first");
+        hipEvent_t ev;
+        r->begin(RT_PASS_ACCUM64, ev);
+        hip_check(rtk::launch_accum64_add(r->d_accum, r->d_acc64, r->n_px(), r->stream), "accum64");
+        r->end(RT_PASS_ACCUM64, ev);
+        r->acc64_frames++;
+    });
+}
+
+int rt_accum64_read_mean(rt_renderer *r, double *rgba, size_t n, uint64_t *n_frames) {
+    return api_call([&] {
+        if (!r || !rgba) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (!r->d_acc64) throw ApiError(RT_ERR_INVALID, "call rt_accum64_reset first");
+        if (n < r->n_px() * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than width*height*4 doubles");
+        read_dev(r, rgba, r->d_acc64, r->n_px() * 32);
+        const double k = r->acc64_frames ? 1.0 / (double)r->acc64_frames : 0.0;
+        for (size_t i = 0; i < r->n_px() * 4; i++) rgba[i] *= k;
+        if (n_frames) *n_frames = r->acc64_frames;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+"""Shared helpers: reference-side parameter records and the oracle frame pipeline."""
+from __future__ import annotations
+
+import numpy as np
+
+import rustracer_amd as rt
+from rustracer_amd import _ffi
+
+from . import oracle_ffi as orc
+
+
+def image_params(cfg, frame: int, photon_radius: float | None = None, alpha: float = 1.0) -> _ffi.ImageParams:
+    """ImageParams as Renderer::record_frame writes them (lib.rs:685-703)."""
+    ip = _ffi.ImageParams()
+    ip.width, ip.height, ip.spp, ip.frame = cfg.width, cfg.height, cfg.spp, frame
+    ip.light_emission[:] = tuple(cfg.light_emission)
+    ip.ambient = cfg.ambient
+    ip.accumulate_alpha = alpha
+    ip.use_photons = 1 if cfg.enable_photon_map else 0
+    ip.photon_count = cfg.photon_count
+    ip.photon_radius = cfg.photon_radius if photon_radius is None else photon_radius
+    ip.photon_scale = cfg.photon_scale
+    ip.photon_debug = 1 if cfg.photon_debug else 0
+    ip.point_light_emission[:] = tuple(cfg.point_light_emission)
+    ip._pad[0] = cfg.debug_view
+    return ip
+
+
+def alpha_for(accumulate_frames: int, frames_since_reset: int) -> float:
+    """lib.rs:679-684 in float32."""
+    if accumulate_frames == 0:
+        return 1.0
+    return float(np.float32(1.0) / np.float32(min(frames_since_reset, accumulate_frames)))
+
+
+def oracle_frames(oscene: orc.OracleScene, cfg, cam_uniform, n_frames: int, photon_map=None, grid_radius=None,
+                  threads: int = 8):
+    """Renderer::record_frame x n_frames on the oracle: gather, a-trous passes,
+    composite, tonemap. Returns (accumulation, rgba8, last frame, moments, gbuffer)."""
+    h, w = cfg.height, cfg.width
+    acc = np.zeros((h, w, 4), np.float32)
+    rgba = None
+    frame = mom = gb = None
+    for k in range(1, n_frames + 1):
+        alpha = alpha_for(cfg.accumulate_frames, k)
+        ip = image_params(cfg, k, grid_radius, alpha)
+        frame, mom, gb, _ = oscene.gather(cam_uniform, ip, max_depth=cfg.max_depth, photon_map=photon_map,
+                                          threads=threads)
+        cur = frame
+        if cfg.enable_denoise:
+            for it in range(cfg.denoise_passes):
+                p = _ffi.DenoiseParams(w, h, 1 << it, 3.0, 12.0, 0.05, alpha, 0)
+                cur = orc.denoise_pass(cur, gb, mom, p)
+        acc = orc.composite(cur, acc, alpha)
+        tm = _ffi.ToneMapUniform(cfg.exposure, cfg.gamma, w, 1.0, (0.0, 0.0, 0.0, 0.0))
+        rgba = orc.tonemap(acc, gb, tm)
+    return acc, rgba, frame, mom, gb
+
+
+def camera_rays(cam, w: int, h: int, jitter_seed: int = 0) -> np.ndarray:
+    """Primary rays of a pinhole camera through jittered pixel positions."""
+    u = rt.camera_to_uniform(cam, w, h)
+    rng = np.random.default_rng(jitter_seed)
+    ys, xs = np.mgrid[0:h, 0:w]
+    jx = rng.uniform(-0.5, 0.5, xs.shape).astype(np.float32)
+    jy = rng.uniform(-0.5, 0.5, ys.shape).astype(np.float32)
+    ux = (xs.astype(np.float32) + jx) * np.float32(u.inv_half_res[0]) - np.float32(1)
+    uy = np.float32(1) - (ys.astype(np.float32) + jy) * np.float32(u.inv_half_res[1])
+    fwd, right, up = (np.array(v, np.float32) for v in (u.forward, u.right, u.up))
+    d = fwd[None, None] + right[None, None] * ux[..., None] + up[None, None] * uy[..., None]
+    d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    rays = np.zeros(w * h, _ffi.RAY)
+    rays["origin"] = np.array(u.position, np.float32)
+    rays["dir"] = d.reshape(-1, 3).astype(np.float32)
+    return rays
+
+
+def random_rays(bmin, bmax, n: int, seed: int) -> np.ndarray:
+    """Rays from random points of the (slightly grown) bounds in random directions,
+    a share of them axis-aligned (zero direction components)."""
+    rng = np.random.default_rng(seed)
+    bmin = np.asarray(bmin, np.float64)
+    bmax = np.asarray(bmax, np.float64)
+    ext = bmax - bmin
+    o = bmin - 0.1 * ext + rng.random((n, 3)) * 1.2 * ext
+    d = rng.normal(size=(n, 3))
+    k = n // 8
+    d[:k] = 0.0
+    d[np.arange(k), rng.integers(0, 3, k)] = rng.choice([-1.0, 1.0], k)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, _ffi.RAY)
+    rays["origin"] = o.astype(np.float32)
+    rays["dir"] = d.astype(np.float32)
+    return rays
+
+
+def first_diff(a: np.ndarray, b: np.ndarray) -> str:
+    bad = np.argwhere(~((a == b) | (np.isnan(a) & np.isnan(b))))
+    if len(bad) == 0:
+        return "identical"
+    i = tuple(bad[0])
+    return f"{len(bad)} differing values, first at {i}: {a[i]!r} vs {b[i]!r}"

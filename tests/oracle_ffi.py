@@ -1,0 +1,187 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) for the tests.
+
+The oracle is test infrastructure: a CPU restatement of the reference's WGSL
+kernels. It is built by ``make -C oracle`` (also run by __graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from rustracer_amd._ffi import (BVH_NODE, HIT, LIGHT, MATERIAL, PHOTON, RAY, TRIANGLE, CameraUniform,
+                                DenoiseParams, GridParams, ImageParams, ToneMapUniform)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+class OrScene(C.Structure):
+    _fields_ = [("tris", C.c_void_p), ("n_tris", C.c_uint32), ("nodes", C.c_void_p), ("n_nodes", C.c_uint32),
+                ("prims", C.c_void_p), ("n_prims", C.c_uint32), ("mats", C.c_void_p), ("n_mats", C.c_uint32),
+                ("lights", C.c_void_p), ("n_lights", C.c_uint32)]
+
+
+class OrPhotonMap(C.Structure):
+    _fields_ = [("gp", C.POINTER(GridParams)), ("grid_meta", C.c_void_p), ("meta_cells", C.c_uint32),
+                ("sorted", C.c_void_p), ("n_sorted", C.c_uint32)]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("ray_queries", C.c_uint64), ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
+                ("photon_reads", C.c_uint64), ("light_index_overflow", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise ImportError(f"{ORACLE_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(ORACLE_PATH)
+        vp, u32 = C.c_void_p, C.c_uint32
+        L.or_trace_closest.argtypes = [C.POINTER(OrScene), vp, u32, vp, C.POINTER(OrStats)]
+        L.or_trace_brute.argtypes = [C.POINTER(OrScene), vp, u32, vp]
+        L.or_gather.argtypes = [C.POINTER(OrScene), C.POINTER(CameraUniform), C.POINTER(ImageParams),
+                                C.POINTER(OrPhotonMap), u32, u32, u32, vp, vp, vp, C.c_int, C.POINTER(OrStats)]
+        L.or_photon_trace.argtypes = [C.POINTER(OrScene), C.POINTER(GridParams), u32, u32, vp, C.c_int]
+        L.or_photon_grid.argtypes = [vp, u32, C.POINTER(GridParams), u32, vp, vp, C.POINTER(u32)]
+        L.or_denoise_pass.argtypes = [vp, vp, vp, vp, C.POINTER(DenoiseParams)]
+        L.or_composite.argtypes = [vp, vp, u32, u32, C.c_float]
+        L.or_tonemap.argtypes = [vp, vp, C.POINTER(ToneMapUniform), u32, u32, vp]
+        L.or_tonemap_cpu.argtypes = [vp, C.c_uint64, vp]
+        L.or_rng_init.argtypes = [u32]
+        L.or_rng_init.restype = u32
+        L.or_rng_next.argtypes = [C.POINTER(u32)]
+        L.or_rng_next.restype = u32
+        L.or_rng_f32.argtypes = [C.POINTER(u32)]
+        L.or_rng_f32.restype = C.c_float
+        L.or_intersect_triangle.argtypes = [vp, vp, vp, vp, vp]
+        L.or_intersect_triangle.restype = C.c_float
+        L.or_math_eval.argtypes = [C.c_int, vp, vp, vp, u32]
+        for fn in (L.or_trace_closest, L.or_trace_brute, L.or_gather, L.or_photon_trace, L.or_photon_grid,
+                   L.or_denoise_pass, L.or_composite, L.or_tonemap, L.or_tonemap_cpu):
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleScene:
+    """Keeps the arrays alive and exposes the or_scene view of them."""
+
+    def __init__(self, tris, nodes, prims, mats, lights):
+        self.tris = np.ascontiguousarray(tris, TRIANGLE)
+        self.nodes = np.ascontiguousarray(nodes, BVH_NODE)
+        self.prims = np.ascontiguousarray(prims, np.uint32)
+        self.mats = np.ascontiguousarray(mats, MATERIAL)
+        lights = np.ascontiguousarray(lights, LIGHT)
+        if len(lights) == 0:  # the reference binds one zeroed record (buffer.rs:19-34)
+            lights = np.zeros(1, LIGHT)
+        self.lights = lights
+        self.s = OrScene(_p(self.tris), len(self.tris), _p(self.nodes), len(self.nodes), _p(self.prims),
+                         len(self.prims), _p(self.mats), len(self.mats), _p(self.lights), len(self.lights))
+
+    @classmethod
+    def from_scene(cls, scene):
+        nodes, prims = scene.build_bvh()
+        return cls(scene.flatten_triangles(), nodes, prims, scene.materials(), scene.lights_gpu())
+
+    def trace(self, rays, stats=False):
+        r = np.ascontiguousarray(rays, RAY)
+        out = np.zeros(len(r), HIT)
+        st = OrStats()
+        assert lib().or_trace_closest(C.byref(self.s), _p(r), len(r), _p(out), C.byref(st)) == 0
+        return (out, st) if stats else out
+
+    def brute(self, rays):
+        r = np.ascontiguousarray(rays, RAY)
+        out = np.zeros(len(r), HIT)
+        assert lib().or_trace_brute(C.byref(self.s), _p(r), len(r), _p(out)) == 0
+        return out
+
+    def gather(self, cam, ip, max_depth=6, rows=None, photon_map=None, threads=8):
+        h, w = ip.height, ip.width
+        frame = np.zeros((h, w, 4), np.float32)
+        mom = np.zeros((h, w, 4), np.float32)
+        gb = np.zeros((h, w, 4), np.float32)
+        r0, r1 = (0, h) if rows is None else rows
+        st = OrStats()
+        pm = None
+        if photon_map is not None:
+            gp, meta, sorted_ph = photon_map
+            self._pm_keep = (gp, np.ascontiguousarray(meta, np.uint32), np.ascontiguousarray(sorted_ph, PHOTON))
+            pm = OrPhotonMap(C.pointer(gp), _p(self._pm_keep[1]), len(self._pm_keep[1]), _p(self._pm_keep[2]),
+                             len(self._pm_keep[2]))
+        rc = lib().or_gather(C.byref(self.s), C.byref(cam), C.byref(ip), C.byref(pm) if pm else None, max_depth,
+                             r0, r1, _p(frame), _p(mom), _p(gb), threads, C.byref(st))
+        assert rc == 0, rc
+        return frame, mom, gb, st
+
+    def photon_trace(self, gp, n_photons, max_depth=6, n_slots=None, threads=8):
+        n_slots = n_slots or gp.photon_slots
+        photons = np.zeros(n_slots, PHOTON)
+        assert lib().or_photon_trace(C.byref(self.s), C.byref(gp), n_photons, max_depth, _p(photons), threads) == 0
+        return photons
+
+
+def photon_grid(photons, n_slots, gp, meta_cells):
+    photons = np.ascontiguousarray(photons, PHOTON)
+    meta = np.zeros((meta_cells, 2), np.uint32)
+    sorted_ph = np.zeros(max(len(photons), 1), PHOTON)
+    n = C.c_uint32()
+    assert lib().or_photon_grid(_p(photons), n_slots, C.byref(gp), meta_cells, _p(meta), _p(sorted_ph),
+                                C.byref(n)) == 0
+    return meta, sorted_ph[:n.value]
+
+
+def denoise_pass(inp, gbuf, mom, params):
+    out = np.zeros_like(inp)
+    assert lib().or_denoise_pass(_p(np.ascontiguousarray(inp)), _p(out), _p(np.ascontiguousarray(gbuf)),
+                                 _p(np.ascontiguousarray(mom)), C.byref(params)) == 0
+    return out
+
+
+def composite(frame, accum, alpha):
+    acc = np.ascontiguousarray(accum, np.float32).copy()
+    h, w = acc.shape[:2]
+    assert lib().or_composite(_p(np.ascontiguousarray(frame, np.float32)), _p(acc), w, h, alpha) == 0
+    return acc
+
+
+def tonemap(accum, gbuf, tm):
+    h, w = accum.shape[:2]
+    out = np.zeros((h, w, 4), np.uint8)
+    assert lib().or_tonemap(_p(np.ascontiguousarray(accum, np.float32)), _p(np.ascontiguousarray(gbuf, np.float32)),
+                            C.byref(tm), w, h, _p(out)) == 0
+    return out
+
+
+def tonemap_cpu(accum):
+    a = np.ascontiguousarray(accum, np.float32)
+    out = np.zeros(a.shape[:-1] + (4,), np.uint8)
+    assert lib().or_tonemap_cpu(_p(a), a.size // 4, _p(out)) == 0
+    return out
+
+
+MATH_FN = {"sin": 0, "cos": 1, "exp2": 2, "log2": 3, "exp": 4, "pow": 5, "sqrt": 6, "inverse_sqrt": 7,
+           "u32": 8, "i32": 9, "smoothstep": 10}
+
+
+def math_eval(name, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, np.float32)
+    out = np.zeros_like(x)
+    lib().or_math_eval(MATH_FN[name], _p(x), _p(y), _p(out), x.size)
+    return out
+
+
+def rng_stream(seed, n):
+    state = C.c_uint32(lib().or_rng_init(seed))
+    return np.array([lib().or_rng_next(C.byref(state)) for _ in range(n)], np.uint32)

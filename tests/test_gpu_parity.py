@@ -1,0 +1,193 @@
+"""GPU parity: the gfx950 kernels through the C ABI against the CPU oracle.
+
+Integer/index results (hit primitive, photon-grid offsets, sorted photon order,
+rgba8) must match exactly; float outputs are compared bit for bit as well
+(tolerance 0): the kernels and the oracle evaluate the same WGSL arithmetic
+with the same builtin definitions and no FMA contraction.
+"""
+import numpy as np
+import pytest
+
+import rustracer_amd as rt
+from rustracer_amd import _ffi
+
+from . import oracle_ffi as orc
+from .helpers import camera_rays, first_diff, image_params, oracle_frames, random_rays
+
+pytestmark = pytest.mark.gpu
+
+FLAG_VARIANTS = [0, _ffi.FLAG_REFERENCE_TRAVERSAL, _ffi.FLAG_CLOSEST_SHADOW]
+
+
+def _renderer(scene, **cfg):
+    c = rt.RenderConfig(**cfg)
+    return rt.Renderer(c, scene, 0)
+
+
+def _check_equal(name, a, b):
+    eq = (a == b) | (np.isnan(a) & np.isnan(b))
+    assert eq.all(), f"{name}: {first_diff(a, b)}"
+
+
+@pytest.mark.parametrize("scene_name", ["cornell_box", "photon_plain"])
+@pytest.mark.parametrize("flags", [0, _ffi.FLAG_REFERENCE_TRAVERSAL])
+def test_trace_closest_bit_exact(scene_name, flags):
+    scene = rt.Scene.builtin(scene_name, 3)
+    o = orc.OracleScene.from_scene(scene)
+    lo, hi = scene.bounds()
+    rays = np.concatenate([camera_rays(rt.Scene.builtin_camera(scene_name, 3), 96, 64, 1),
+                           random_rays(lo, hi, 20000, 7)])
+    print(f"trace {scene_name} flags={flags} rays={len(rays)}", flush=True)
+    with _renderer(scene, width=8, height=8, flags=flags) as r:
+        got = r.trace_closest(rays)
+    ref = o.trace(rays)
+    brute = o.brute(rays)
+    _check_equal("t", got["t"], ref["t"])
+    assert np.array_equal(got["prim_id"], ref["prim_id"]), first_diff(got["prim_id"], ref["prim_id"])
+    # the BVH answer is the brute-force closest hit (ties: lowest slot)
+    _check_equal("t vs brute force", ref["t"], brute["t"])
+    assert (ref["t"] < 1e38).sum() > len(rays) // 10
+
+
+@pytest.mark.parametrize("scene_name,spp,depth", [("cornell_box", 4, 6), ("cornell36", 16, 5),
+                                                  ("smoke", 2, 6), ("photon_plain", 3, 6)])
+@pytest.mark.parametrize("flags", FLAG_VARIANTS)
+def test_gather_frame_bit_exact(scene_name, spp, depth, flags):
+    scene = rt.Scene.builtin(scene_name, 5)
+    cam = rt.Scene.builtin_camera(scene_name, 5)
+    w, h = 72, 40
+    print(f"gather {scene_name} spp={spp} depth={depth} flags={flags}", flush=True)
+    with _renderer(scene, width=w, height=h, spp=spp, max_depth=depth, enable_denoise=False, flags=flags) as r:
+        r.update_camera(cam)
+        r.render_frames(2)
+        got = (r.read_frame(), r.read_moments(), r.read_gbuffer())
+        cfg = r.config
+    o = orc.OracleScene.from_scene(scene)
+    ref = o.gather(rt.camera_to_uniform(cam, w, h), image_params(cfg, 2), max_depth=depth)
+    for name, a, b in zip(("frame", "moments", "gbuffer"), got, ref[:3]):
+        _check_equal(name, a, b)
+
+
+@pytest.mark.parametrize("view", [1, 2, 4, 13, 17, 19, 20, 21])
+def test_debug_views_bit_exact(view):
+    scene = rt.Scene.builtin("cornell_box")
+    cam = rt.Scene.builtin_camera("cornell_box")
+    with _renderer(scene, width=40, height=30, spp=2, debug_view=view, enable_denoise=False) as r:
+        r.update_camera(cam)
+        r.record_frame()
+        got = r.read_frame()
+        cfg = r.config
+    ref = orc.OracleScene.from_scene(scene).gather(rt.camera_to_uniform(cam, 40, 30), image_params(cfg, 1))
+    _check_equal("frame", got, ref[0])
+
+
+def _photon_setup(n_photons, cell=0.0, seed=11):
+    scene = rt.Scene.builtin("photon_plain", seed)
+    cfg = dict(width=48, height=32, spp=2, enable_denoise=False, enable_photon_map=True, photon_count=n_photons,
+               grid_cell_size=cell)
+    return scene, cfg
+
+
+@pytest.mark.parametrize("cell", [0.0, 0.05])
+def test_photon_trace_and_grid_bit_exact(cell):
+    n = 4096 + 77
+    scene, cfg = _photon_setup(n, cell)
+    print(f"photon trace n={n} cell={cell}", flush=True)
+    with _renderer(scene, **cfg) as r:
+        r.rebuild_photons()
+        photons = r.read_photons()
+        meta = r.read_grid_meta()
+        srt = r.read_sorted_photons()
+        gp = r.grid_params()
+    o = orc.OracleScene.from_scene(scene)
+    traced = (n // 256) * 256
+    ref_ph = o.photon_trace(gp, traced, 6, n_slots=gp.photon_slots)
+    _check_equal("photons", photons.view(np.float32), ref_ph.view(np.float32))
+    n_proc = min(max(n * 6 // 256, 1) * 256, gp.photon_slots)
+    ref_meta, ref_sorted = orc.photon_grid(ref_ph, n_proc, gp, len(meta))
+    assert np.array_equal(meta, ref_meta), first_diff(meta, ref_meta)
+    assert len(srt) == len(ref_sorted) > 0
+    _check_equal("sorted photons", srt.view(np.float32), ref_sorted.view(np.float32))
+
+
+def test_grid_build_large_matches_stable_sort():
+    """1.5M synthetic photon slots (a quarter empty) on a 0.05 grid: offsets and
+    order equal numpy's stable argsort of the cell keys."""
+    scene, cfg = _photon_setup(250_000, 0.05)
+    rng = np.random.default_rng(5)
+    with _renderer(scene, **cfg) as r:
+        gp = r.grid_params()
+        n = gp.photon_slots
+        ph = np.zeros(n, _ffi.PHOTON)
+        lo = np.array([gp.room_min_x, gp.room_min_y, gp.room_min_z], np.float32)
+        span = np.array([gp.grid_x, gp.grid_y, gp.grid_z], np.float32) * np.float32(gp.cell_size)
+        ph["position"] = (lo + rng.random((n, 3), np.float32) * span * np.float32(1.02)).astype(np.float32)
+        ph["power"] = rng.random((n, 3), np.float32) + np.float32(0.01)
+        ph["power"][rng.random(n) < 0.25, 0] = 0.0
+        ph["incident"] = rng.normal(size=(n, 3)).astype(np.float32)
+        r.build_photon_grid_from(ph)
+        meta = r.read_grid_meta()
+        srt = r.read_sorted_photons()
+    # keys exactly as photon_count.wgsl:18-23 computes them, in float32
+    rel = (ph["position"] - lo) / np.float32(gp.cell_size)
+    g = np.where(rel > 0, np.minimum(rel, np.float32(4294967040.0)), 0).astype(np.uint64)
+    key = g[:, 0] + g[:, 1] * gp.grid_x + g[:, 2] * gp.grid_x * gp.grid_y
+    key = key.astype(np.uint32)  # u32 wrap
+    valid = (ph["power"][:, 0] > 0) & (key < gp.num_cells)
+    order = np.argsort(np.where(valid, key, gp.num_cells), kind="stable")[: valid.sum()]
+    counts = np.bincount(key[valid], minlength=len(meta))
+    assert np.array_equal(meta[:, 1], counts)
+    assert np.array_equal(meta[:, 0], np.concatenate([[0], np.cumsum(counts)[:-1]]))
+    assert np.array_equal(srt.view(np.float32).reshape(-1, 16), ph[order].view(np.float32).reshape(-1, 16))
+
+
+def test_gather_with_photon_map_bit_exact():
+    scene, cfg = _photon_setup(20000, 0.05)
+    cam = rt.Scene.builtin_camera("photon_plain", 11)
+    with _renderer(scene, **cfg) as r:
+        r.update_camera(cam)
+        r.rebuild_photons()
+        r.record_frame()
+        got = (r.read_frame(), r.read_moments(), r.read_gbuffer())
+        gp, meta, srt, c = r.grid_params(), r.read_grid_meta(), r.read_sorted_photons(), r.config
+    o = orc.OracleScene.from_scene(scene)
+    ref = o.gather(rt.camera_to_uniform(cam, c.width, c.height), image_params(c, 1), photon_map=(gp, meta, srt))
+    for name, a, b in zip(("frame", "moments", "gbuffer"), got, ref[:3]):
+        _check_equal(name, a, b)
+
+
+@pytest.mark.parametrize("denoise,accum,passes", [(True, 0, 2), (True, 4, 2), (False, 3, 2), (True, 5, 3)])
+def test_full_pipeline_bit_exact(denoise, accum, passes):
+    """gather -> a-trous -> composite -> tonemap over 4 frames (record_frame x 4)."""
+    scene = rt.Scene.builtin("cornell_box")
+    cam = rt.Scene.builtin_camera("cornell_box")
+    w, h = 56, 44
+    with _renderer(scene, width=w, height=h, spp=2, enable_denoise=denoise, accumulate_frames=accum,
+                   denoise_passes=passes) as r:
+        r.update_camera(cam)
+        r.render_frames(4)
+        acc, rgba = r.read_accumulation(), r.read_output_rgba8()
+        cfg = r.config
+    o = orc.OracleScene.from_scene(scene)
+    racc, rrgba, *_ = oracle_frames(o, cfg, rt.camera_to_uniform(cam, w, h), 4)
+    _check_equal("accumulation", acc, racc)
+    assert np.array_equal(rgba, rrgba), first_diff(rgba, rrgba)
+
+
+def test_fullframe_1080p_properties():
+    """The 1M-triangle 1080p frame at full size: every pixel finite, moments
+    consistent (E[l^2] >= E[l]^2 up to the clamps), G-buffer normals unit length."""
+    scene = rt.Scene.builtin("fullframe", 1)
+    cam = rt.Scene.builtin_camera("fullframe", 1)
+    with _renderer(scene, width=1920, height=1080, spp=1, max_depth=8, enable_denoise=True) as r:
+        r.update_camera(cam)
+        r.render_frames(2)
+        f, m, g = r.read_frame(), r.read_moments(), r.read_gbuffer()
+        out = r.read_output_rgba8()
+    assert np.isfinite(f).all() and np.isfinite(m).all()
+    assert (f[..., :3] >= 0).all() and (f[..., :3] <= 10).all()
+    hit = g[..., 3] < 1e29
+    assert hit.mean() > 0.5
+    n = np.linalg.norm(g[..., :3][hit], axis=-1)
+    assert np.abs(n - 1).max() < 1e-5
+    assert (out[..., 3] == 255).all()
