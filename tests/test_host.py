@@ -1,0 +1,100 @@
+"""Host-side scene model, camera, lights and configuration (rustracer-core,
+rustracer-renderer/src/lib.rs) and the render_headless CLI. CPU only."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from rustracer_amd import _ffi
+
+F = np.float32
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_config_defaults_match_render_config_default(rt):
+    c = rt.RenderConfig()  # lib.rs:43-64
+    assert (c.width, c.height, c.photon_count, c.spp) == (800, 600, 1_000_000, 1)
+    assert (c.exposure, c.gamma) == (F(1.0), F(2.2))
+    assert list(c.light_emission) == [1, 1, 1] and list(c.point_light_emission) == [1, 1, 1]
+    assert (c.accumulate_frames, c.enable_photon_map, c.enable_denoise, c.photon_debug, c.debug_view) == (0, 0, 1, 0, 0)
+    assert (c.photon_scale, c.photon_radius, c.ambient) == (F(0.02), F(0.5), F(0.03))
+    assert (c.max_depth, c.denoise_passes, c.grid_cell_size) == (6, 2, 0.0)
+
+
+def test_camera_to_uniform_closed_form(rt):
+    cam = rt.camera([0, 0, 5], [0, 0, 0], fov_degrees=60)  # Camera::default, camera.rs:42-52
+    u = rt.camera_to_uniform(cam, 800, 600)
+    t = np.tan(np.radians(30))
+    assert np.allclose(u.forward, [0, 0, -1]) and np.allclose(u.right, [t * 800 / 600, 0, 0], rtol=1e-6)
+    assert np.allclose(u.up, [0, t, 0], rtol=1e-6)
+    assert list(u.inv_half_res) == [F(2 / 800), F(2 / 600)] and u.fov_scale == pytest.approx(t, rel=1e-6)
+
+
+def test_camera_frame_zoom_orbit_pan(rt):
+    s = rt.Scene.builtin("cornell_box")
+    c = s.frame_camera()  # camera.rs:83-99
+    lo, hi = s.bounds()
+    center = (lo + hi) / 2
+    assert np.allclose(c.look_at, center) and c.fov_degrees == 70
+    dist = np.linalg.norm(np.array(c.position) - center)
+    assert dist == pytest.approx(np.linalg.norm(hi - lo) * 0.5 / np.tan(np.radians(35)) * 1.3, rel=1e-5)
+    import ctypes as C
+    lib = rt.lib()
+    lib.rt_camera_zoom(C.byref(c), 1.0)
+    assert np.linalg.norm(np.array(c.position) - center) == pytest.approx(dist - 1.0, rel=1e-5)
+    lib.rt_camera_orbit(C.byref(c), 0.7, 0.2)  # orbit keeps the distance to look_at
+    assert np.linalg.norm(np.array(c.position) - center) == pytest.approx(dist - 1.0, rel=1e-5)
+    before = np.array(c.position) - np.array(c.look_at)
+    lib.rt_camera_pan(C.byref(c), 0.3, -0.2)
+    assert np.allclose(np.array(c.position) - np.array(c.look_at), before, atol=1e-6)
+
+
+def test_light_to_gpu_mapping(rt):
+    s = rt.Scene()
+    s.add_light(_ffi.LIGHT_POINT, vec_a=(1, 2, 3), color=(0.5, 0.6, 0.7), intensity=4)
+    s.add_light(_ffi.LIGHT_AREA, color=(9, 8, 7), intensity=2, triangle_index=12)
+    s.add_light(_ffi.LIGHT_ENVIRONMENT, vec_a=(0.1, 0.2, 0.3), vec_b=(0.9, 0.8, 0.7), intensity=0.5)
+    g = s.lights_gpu()  # lib.rs:1231-1282
+    assert list(g["kind"]) == [0, 2, 3]
+    assert list(g[0]["data"]) == [1, 2, 3, 0] and list(g[0]["color"]) == [F(0.5), F(0.6), F(0.7)]
+    assert g[1]["data"][0] == 12.0 and g[1]["intensity"] == 2
+    assert np.allclose(g[2]["data"][:3], [0.9, 0.8, 0.7]) and np.allclose(g[2]["color"], [0.1, 0.2, 0.3])
+
+
+def test_cornell_box_scene_matches_reference(rt):
+    s = rt.Scene.builtin("cornell_box")  # scene.rs:296-381
+    c = s.counts()
+    assert c == dict(triangles=60, materials=8, lights=2, meshes=30)
+    t = s.flatten_triangles()
+    m = s.materials()
+    assert list(m["kind"]) == [0, 0, 0, 0, 3, 1, 2, 0]
+    assert np.allclose(m[4]["emissive"], [25, 20, 14]) and m[5]["roughness"] == F(0.1)
+    assert (t["material_id"][10:12] == 4).all()  # the light quad's two triangles
+    g = s.lights_gpu()
+    assert list(g["data"][:, 0]) == [10.0, 11.0]
+    assert np.allclose(t[0]["n0"], [0, 1, 0])  # floor normal from create_quad
+
+
+def test_flatten_defaults_for_missing_attributes(rt):
+    s = rt.Scene()
+    m = s.add_material(rt.make_material(0))
+    s.add_mesh([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [0, 1, 2], m)
+    t = s.flatten_triangles()[0]
+    assert list(t["n0"]) == [0, 1, 0] and list(t["uv2"]) == [0, 0]  # scene.rs:233-250
+    with pytest.raises(rt.RtError):
+        s.add_mesh([[0, 0, 0]], [0, 1, 2], m)
+
+
+@pytest.mark.parametrize("name,tris", [("cornell36", 36), ("photon_plain", 54084), ("multiview", 108006),
+                                       ("fullframe", 1000610)])
+def test_builtin_scene_sizes(rt, name, tris):
+    assert rt.Scene.builtin(name, 1).counts()["triangles"] == tris
+
+
+def test_render_headless_cli_usage_and_no_device():
+    exe = os.path.join(ROOT, "rustracer_amd", "render_headless")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([exe, "no_such_scene"], capture_output=True, text=True)
+    assert r.returncode == 1 and "unknown built-in scene" in r.stderr
