@@ -71,8 +71,12 @@ typedef struct rt_config {
 /* Trace shadow rays with the reference's closest-hit query instead of the
  * equivalent early-exit query (same result when no material is alpha-cutout). */
 #define RT_FLAG_CLOSEST_SHADOW 2u
-/* Use the reference-order BVH2 traversal (left child first) for every query. */
+/* Traverse the reference's 48-byte BVH2 nodes instead of the child-box records
+ * (both give the reference's hits bit for bit). */
 #define RT_FLAG_REFERENCE_TRAVERSAL 4u
+/* Child-box traversal nearest child first: fewer nodes visited; equals the
+ * reference except where a hit's rounded t falls before its own box's entry. */
+#define RT_FLAG_NEAREST_FIRST 8u
 
 /* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
 int rt_config_default(rt_config *cfg);
@@ -246,6 +250,11 @@ typedef struct rt_trace_counters {
     uint64_t photon_reads;
 } rt_trace_counters;
 int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out);
+
+/* Measured HBM bandwidth: a plain device-to-device copy kernel of `bytes`,
+ * repeated `iters` times; *gbps counts bytes read + written. No counterpart in
+ * the reference (measurement support). */
+int rt_hbm_copy_probe(int device, size_t bytes, uint32_t iters, double *gbps, double *ms_per_copy);
 
 /* Convergence support: FP64 running sum of the displayed frames and its mean. */
 int rt_accum64_reset(rt_renderer *r);

@@ -168,6 +168,13 @@ def bvh_from_boxes(centroids, box_min, box_max):
     return nodes[:nn.value].copy(), prims[:n].copy()
 
 
+def hbm_copy_probe(device: int = 0, nbytes: int = 1 << 30, iters: int = 20):
+    """Measured HBM bandwidth of a plain copy kernel: (GB/s read+written, ms per copy)."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib().rt_hbm_copy_probe(device, nbytes, iters, C.byref(g), C.byref(ms)), "rt_hbm_copy_probe")
+    return g.value, ms.value
+
+
 def camera_to_uniform(cam: Camera, width: int, height: int) -> CameraUniform:
     u = CameraUniform()
     check(lib().rt_camera_to_uniform(C.byref(cam), width, height, C.byref(u)), "rt_camera_to_uniform")

@@ -42,7 +42,7 @@ LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_AREA, LIGHT_ENVIRONMENT = 0, 1, 2, 3
 GRID_MAX_CELLS = 26 * 26 * 26
 
 RT_OK, RT_ERR_INVALID, RT_ERR_UNSUPPORTED, RT_ERR_HIP, RT_ERR_NO_DEVICE, RT_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
-FLAG_PROFILE, FLAG_CLOSEST_SHADOW, FLAG_REFERENCE_TRAVERSAL = 1, 2, 4
+FLAG_PROFILE, FLAG_CLOSEST_SHADOW, FLAG_REFERENCE_TRAVERSAL, FLAG_NEAREST_FIRST = 1, 2, 4, 8
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
               "accum64", "spare")
 
@@ -183,6 +183,7 @@ SIGNATURES = {
     "rt_get_pass_stats": (_i, [_vp, _P(PassStats)]),
     "rt_reset_pass_stats": (_i, [_vp]),
     "rt_count_gather_traffic": (_i, [_vp, _P(TraceCounters)]),
+    "rt_hbm_copy_probe": (_i, [_i, C.c_size_t, _u32, _P(C.c_double), _P(C.c_double)]),
     "rt_accum64_reset": (_i, [_vp]),
     "rt_accum64_add_accumulation": (_i, [_vp]),
     "rt_accum64_read_mean": (_i, [_vp, _vp, C.c_size_t, _P(C.c_uint64)]),

@@ -229,10 +229,12 @@ __device__ __forceinline__ void slab(F3 bmin, F3 bmax, F3 ro, F3 inv, float &te,
 
 // Reference-order traversal, gather.wgsl:66-88: left child first, strict
 // `te < hit.t` and `t < hit.t`, right child dropped when 63 entries are used.
-// ANY_HIT: stop at the first triangle with t < t_max (shadow query).
-template <bool ANY_HIT, bool COUNT>
+// EARLY_EXIT (shadow queries): the same traversal, stopped as soon as the best
+// hit is nearer than t_max. Up to that point it is the reference's closest-hit
+// traversal step for step, so "closest hit < t_max" is answered exactly.
+template <bool EARLY_EXIT, bool COUNT>
 __device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
-    Hit hit{ANY_HIT ? t_max : kInf, 0u};
+    Hit hit{kInf, 0u};
     const F3 inv = fs(1.0f) / rd;
     uint32_t idx = 0u, sp = 0u;
     if (S.n_nodes == 0u) return hit;
@@ -256,9 +258,9 @@ __device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, L
                     if (t < hit.t) {
                         hit.t = t;
                         hit.prim = __float_as_uint(q0.w);
-                        if (ANY_HIT) return hit;
                     }
                 }
+                if (EARLY_EXIT && hit.t < t_max) return hit;
             } else {
                 if (sp < 63u) {
                     stk.put(sp, __float_as_uint(b.w));
@@ -278,22 +280,31 @@ __device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, L
 }
 
 // Child-box traversal over the wide layout: each internal record holds both
-// children's boxes, so one 64-byte load tests two boxes, and children are
-// visited nearest first. To return exactly the reference's answer for any
-// visiting order, closest-hit queries prune a box only when te > best t (not
-// >=), and a hit replaces the best one when (t, slot) is lexicographically
-// smaller: the reference's left-first depth-first order meets slots in
-// ascending order, so its strict `t < hit.t` keeps the lowest slot among
-// equal t. Any-hit (shadow) queries keep the reference's strict pruning.
+// children's boxes, so one 64-byte load tests two boxes.
+//
+// Default (NEAREST = false): the reference's order. The left child is entered
+// first; the right child's entry distance is kept on the stack and re-checked
+// against the best hit when popped, which is exactly when the reference would
+// test its box (gather.wgsl:75). Testing a child's box at its parent instead of
+// at the child changes no comparison (same operands), and a box that fails
+// `te < hit.t` now fails it later too. Together with the strict `t < hit.t` this
+// returns the reference's hit bit for bit, including its order-dependent
+// choice when a triangle's rounded t lands just before its own box's entry.
+// EARLY_EXIT: shadow query, stopped once the best hit is nearer than t_max.
+//
+// NEAREST = true: the nearer child first and lexicographic (t, slot) ties. It
+// equals the reference except where a hit rounds to before its box's entry
+// (measured, not guaranteed; opt-in via RT_FLAG_NEAREST_FIRST).
+//
 // Record i (internal nodes only, root = 0):
 //   {lmin.xyz, lref}, {lmax.xyz, rref}, {rmin.xyz, 0}, {rmax.xyz, 0}
 // ref: internal child = node index; leaf = 0x80000000 | (count-1) << 27 | first slot.
 // Stack entries are two words: (ref, entry distance of the deferred child).
 __device__ __forceinline__ bool ref_is_leaf(uint32_t r) { return (r & 0x80000000u) != 0u; }
 
-template <bool ANY_HIT, bool COUNT>
+template <bool NEAREST, bool EARLY_EXIT, bool COUNT>
 __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
-    Hit hit{ANY_HIT ? t_max : kInf, 0u};
+    Hit hit{kInf, 0u};
     uint32_t best_slot = 0xFFFFFFFFu;
     const F3 inv = fs(1.0f) / rd;
     if (S.n_nodes == 0u) return hit;
@@ -312,11 +323,10 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
                 const float4 q0 = S.slot_tris[3 * slot], q1 = S.slot_tris[3 * slot + 1], q2 = S.slot_tris[3 * slot + 2];
                 if (COUNT) cnt.tris++;
                 const float t = mt_slot(ro, rd, xyz(q0), xyz(q1), xyz(q2));
-                if (ANY_HIT) {
+                if (!NEAREST) {
                     if (t < hit.t) {
                         hit.t = t;
                         hit.prim = __float_as_uint(q0.w);
-                        return hit;
                     }
                 } else if (t < hit.t || (t == hit.t && t < kInf && slot < best_slot)) {
                     hit.t = t;
@@ -324,6 +334,7 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
                     best_slot = slot;
                 }
             }
+            if (EARLY_EXIT && hit.t < t_max) return hit;
         } else {
             const float4 *w = S.wnodes + 4u * cur;
             const float4 a = w[0], b = w[1], c = w[2], d = w[3];
@@ -331,15 +342,15 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
             float tel, tal, ter, tar;
             slab(xyz(a), xyz(b), ro, inv, tel, tal);
             slab(xyz(c), xyz(d), ro, inv, ter, tar);
-            const bool hl = tel <= tal && tal > 0.0f && (ANY_HIT ? tel < hit.t : tel <= hit.t);
-            const bool hr = ter <= tar && tar > 0.0f && (ANY_HIT ? ter < hit.t : ter <= hit.t);
+            const bool hl = tel <= tal && tal > 0.0f && (NEAREST ? tel <= hit.t : tel < hit.t);
+            const bool hr = ter <= tar && tar > 0.0f && (NEAREST ? ter <= hit.t : ter < hit.t);
             const uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(b.w);
             if (hl && hr) {
-                const bool lnear = tel <= ter;
-                stk.put(sp, lnear ? rref : lref);
-                stk.put(sp + 1u, __float_as_uint(lnear ? ter : tel));
+                const bool lfirst = NEAREST ? (tel <= ter) : true;
+                stk.put(sp, lfirst ? rref : lref);
+                stk.put(sp + 1u, __float_as_uint(lfirst ? ter : tel));
                 sp += 2u;
-                cur = lnear ? lref : rref;
+                cur = lfirst ? lref : rref;
                 continue;
             }
             if (hl || hr) {
@@ -347,12 +358,12 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
                 continue;
             }
         }
-        // pop, skipping deferred children whose entry is already beyond the best hit
+        // pop, skipping deferred children whose entry is no longer before the best hit
         bool found = false;
         while (sp > 0u) {
             sp -= 2u;
             const float et = __uint_as_float(stk.get(sp + 1u));
-            if (ANY_HIT ? et < hit.t : et <= hit.t) {
+            if (NEAREST ? et <= hit.t : et < hit.t) {
                 cur = stk.get(sp);
                 found = true;
                 break;
@@ -361,6 +372,15 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
         if (!found) break;
     }
     return hit;
+}
+
+// Traversal modes: 0 = the reference's BVH2 nodes, 1 = child-box records in the
+// reference's order (default), 2 = child-box records nearest child first.
+template <int MODE, bool EARLY_EXIT, bool COUNT>
+__device__ __forceinline__ Hit trace_mode(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk,
+                                          TraceCount &cnt) {
+    if (MODE == 0) return trace_reference<EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
+    return trace_wide<MODE == 2, EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
 }
 
 }  // namespace rtd

@@ -35,13 +35,13 @@ struct GatherLaunch {
     void *moments;  // float4 per pixel
     void *gbuffer;  // float4 per pixel
     uint32_t max_depth;
-    uint32_t wide;           // child-box traversal (else reference order)
-    uint32_t anyhit_shadow;  // early-exit shadow rays
+    uint32_t mode;           // 0 reference nodes, 1 child-box reference order, 2 child-box nearest first
+    uint32_t early_shadow;   // shadow rays stop at the first hit nearer than the light
     unsigned long long *counters;  // non-null: count traffic (4 counters)
 };
 
 hipError_t launch_gather(const GatherLaunch &g, hipStream_t s);
-hipError_t launch_trace_rays(const TraceInputs &t, uint32_t wide, const rt_ray *rays, uint32_t n, rt_hit *out,
+hipError_t launch_trace_rays(const TraceInputs &t, uint32_t mode, const rt_ray *rays, uint32_t n, rt_hit *out,
                              hipStream_t s);
 
 struct PhotonLaunch {
@@ -54,7 +54,7 @@ struct PhotonLaunch {
     rt_grid_params gp;
     uint32_t n_photons;  // invocations (photon indices 0..n-1)
     uint32_t max_depth;  // slots per photon
-    uint32_t wide;
+    uint32_t mode;
     rt_photon *photons;
 };
 hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s);
@@ -88,5 +88,6 @@ hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rg
                                     float alpha, const rt_tonemap_uniform &tm, const void *gbuffer, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_accum64_add(const void *accum_f4, double *sum, size_t n_px, hipStream_t s);
+hipError_t launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 
 }  // namespace rtk

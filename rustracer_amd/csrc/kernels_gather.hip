@@ -37,8 +37,8 @@ struct LightSample {
     F3 e, wi;
 };
 
-// The tracing context of one lane: query + shadow query with the chosen traversal.
-template <bool WIDE, bool ANYHIT, bool COUNT>
+// The tracing context of one lane: closest-hit query and shadow query.
+template <int MODE, bool EARLY, bool COUNT>
 struct Tracer {
     const TraceScene &ts;
     LaneStack &stk;
@@ -46,21 +46,15 @@ struct Tracer {
     uint32_t queries;
     __device__ Hit closest(F3 ro, F3 rd) {
         if (COUNT) queries++;
-        return WIDE ? trace_wide<false, COUNT>(ts, ro, rd, kInf, stk, cnt)
-                    : trace_reference<false, COUNT>(ts, ro, rd, kInf, stk, cnt);
+        return trace_mode<MODE, false, COUNT>(ts, ro, rd, kInf, stk, cnt);
     }
-    // gather.wgsl:91-103 without alpha-cutout materials: occluded iff the
-    // closest hit is nearer than max_dist - 0.001, i.e. iff any hit is.
+    // gather.wgsl:91-103 without alpha-cutout materials (no textures here):
+    // occluded iff the closest hit is nearer than max_dist - 0.001. EARLY
+    // stops the same traversal as soon as a hit that near is found.
     __device__ bool shadowed(F3 origin, F3 dir, float max_dist) {
         if (COUNT) queries++;
         const float lim = max_dist - 0.001f;
-        if (ANYHIT) {
-            const Hit h = WIDE ? trace_wide<true, COUNT>(ts, origin, dir, lim, stk, cnt)
-                               : trace_reference<true, COUNT>(ts, origin, dir, lim, stk, cnt);
-            return h.t < lim;
-        }
-        const Hit h = WIDE ? trace_wide<false, COUNT>(ts, origin, dir, kInf, stk, cnt)
-                           : trace_reference<false, COUNT>(ts, origin, dir, kInf, stk, cnt);
+        const Hit h = trace_mode<MODE, EARLY, COUNT>(ts, origin, dir, lim, stk, cnt);
         return !(h.t >= lim);
     }
 };
@@ -188,8 +182,8 @@ struct Acc {
 __device__ __forceinline__ int32_t iclamp(int32_t v, int32_t lo, int32_t hi) { return min(max(v, lo), hi); }
 
 // gather.wgsl:258-594 for the pixel (px, py).
-template <bool WIDE, bool ANYHIT, bool COUNT>
-__device__ void gather_pixel(const GatherArgs &A, Tracer<WIDE, ANYHIT, COUNT> &tr, uint32_t px, uint32_t py,
+template <int MODE, bool EARLY, bool COUNT>
+__device__ void gather_pixel(const GatherArgs &A, Tracer<MODE, EARLY, COUNT> &tr, uint32_t px, uint32_t py,
                              uint32_t &photon_reads) {
     const rt_image_params &ip = A.ip;
     const uint32_t base_seed = py * ip.width + px + ip.frame * 2654435761u;
@@ -444,7 +438,7 @@ __device__ void gather_pixel(const GatherArgs &A, Tracer<WIDE, ANYHIT, COUNT> &t
     A.gbuffer[idx] = make_float4(first_normal.x, first_normal.y, first_normal.z, first_depth);
 }
 
-template <bool WIDE, bool ANYHIT, bool COUNT>
+template <int MODE, bool EARLY, bool COUNT>
 __global__ __launch_bounds__(64) void gather_kernel(GatherArgs A) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a
@@ -458,9 +452,9 @@ __global__ __launch_bounds__(64) void gather_kernel(GatherArgs A) {
     LaneStack stk;
     stk.lds = &stack_rows[threadIdx.x];
     stk.stride = 64u;
-    Tracer<WIDE, ANYHIT, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
-    if (px < A.ip.width && py < A.ip.height) gather_pixel<WIDE, ANYHIT, COUNT>(A, tr, px, py, photon_reads);
+    if (px < A.ip.width && py < A.ip.height) gather_pixel<MODE, EARLY, COUNT>(A, tr, px, py, photon_reads);
     if (COUNT) {
         atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
@@ -469,7 +463,7 @@ __global__ __launch_bounds__(64) void gather_kernel(GatherArgs A) {
     }
 }
 
-template <bool WIDE>
+template <int MODE>
 __global__ __launch_bounds__(64) void trace_rays_kernel(TraceScene ts, const rt_ray *rays, uint32_t n, rt_hit *out) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
@@ -480,8 +474,7 @@ __global__ __launch_bounds__(64) void trace_rays_kernel(TraceScene ts, const rt_
     TraceCount cnt{0u, 0u};
     const rt_ray r = rays[i];
     const F3 ro = ld3(r.origin), rd = ld3(r.dir);
-    const Hit h = WIDE ? trace_wide<false, false>(ts, ro, rd, kInf, stk, cnt)
-                       : trace_reference<false, false>(ts, ro, rd, kInf, stk, cnt);
+    const Hit h = trace_mode<MODE, false, false>(ts, ro, rd, kInf, stk, cnt);
     out[i].t = h.t;
     out[i].prim_id = h.prim;
 }
@@ -525,34 +518,31 @@ hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
     A.n_tiles = A.tiles_x * tiles_y;
     A.counters = g.counters;
     const dim3 grid(A.n_tiles), block(64);
-    const bool cnt = g.counters != nullptr;
-    if (g.wide) {
-        if (g.anyhit_shadow) {
-            if (cnt) hipLaunchKernelGGL((rtd::gather_kernel<true, true, true>), grid, block, 0, s, A);
-            else hipLaunchKernelGGL((rtd::gather_kernel<true, true, false>), grid, block, 0, s, A);
-        } else {
-            if (cnt) hipLaunchKernelGGL((rtd::gather_kernel<true, false, true>), grid, block, 0, s, A);
-            else hipLaunchKernelGGL((rtd::gather_kernel<true, false, false>), grid, block, 0, s, A);
-        }
-    } else {
-        if (g.anyhit_shadow) {
-            if (cnt) hipLaunchKernelGGL((rtd::gather_kernel<false, true, true>), grid, block, 0, s, A);
-            else hipLaunchKernelGGL((rtd::gather_kernel<false, true, false>), grid, block, 0, s, A);
-        } else {
-            if (cnt) hipLaunchKernelGGL((rtd::gather_kernel<false, false, true>), grid, block, 0, s, A);
-            else hipLaunchKernelGGL((rtd::gather_kernel<false, false, false>), grid, block, 0, s, A);
-        }
+#define RT_GATHER(M, E, C) hipLaunchKernelGGL((rtd::gather_kernel<M, E, C>), grid, block, 0, s, A)
+#define RT_GATHER_M(M)                        \
+    if (g.early_shadow) {                     \
+        if (g.counters) RT_GATHER(M, true, true); \
+        else RT_GATHER(M, true, false);       \
+    } else {                                  \
+        if (g.counters) RT_GATHER(M, false, true); \
+        else RT_GATHER(M, false, false);      \
     }
+    if (g.mode == 0) { RT_GATHER_M(0) }
+    else if (g.mode == 1) { RT_GATHER_M(1) }
+    else { RT_GATHER_M(2) }
+#undef RT_GATHER_M
+#undef RT_GATHER
     return hipGetLastError();
 }
 
-hipError_t launch_trace_rays(const TraceInputs &t, uint32_t wide, const rt_ray *rays, uint32_t n, rt_hit *out,
+hipError_t launch_trace_rays(const TraceInputs &t, uint32_t mode, const rt_ray *rays, uint32_t n, rt_hit *out,
                              hipStream_t s) {
     if (n == 0) return hipSuccess;
     const rtd::TraceScene ts = rtd::make_trace_scene(t);
     const dim3 grid((n + 63u) / 64u), block(64);
-    if (wide) hipLaunchKernelGGL((rtd::trace_rays_kernel<true>), grid, block, 0, s, ts, rays, n, out);
-    else hipLaunchKernelGGL((rtd::trace_rays_kernel<false>), grid, block, 0, s, ts, rays, n, out);
+    if (mode == 0) hipLaunchKernelGGL((rtd::trace_rays_kernel<0>), grid, block, 0, s, ts, rays, n, out);
+    else if (mode == 1) hipLaunchKernelGGL((rtd::trace_rays_kernel<1>), grid, block, 0, s, ts, rays, n, out);
+    else hipLaunchKernelGGL((rtd::trace_rays_kernel<2>), grid, block, 0, s, ts, rays, n, out);
     return hipGetLastError();
 }
 

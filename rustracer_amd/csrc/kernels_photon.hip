@@ -41,7 +41,7 @@ __device__ __forceinline__ F3 bounce_dir(F3 rd, F3 n, float rough, Rng &rng) {
     return normalize(mix(mirror, to_world(diff, n), blur));
 }
 
-template <bool WIDE>
+template <int MODE>
 __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     const uint32_t photon_index = blockIdx.x * 64u + threadIdx.x;
@@ -111,8 +111,7 @@ __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
 
     // photon_trace.wgsl:178-259
     for (uint32_t bounce = 0u; bounce < A.max_depth; bounce++) {
-        const Hit hit = WIDE ? trace_wide<false, false>(A.ts, origin, dir, kInf, stk, cnt)
-                             : trace_reference<false, false>(A.ts, origin, dir, kInf, stk, cnt);
+        const Hit hit = trace_mode<MODE, false, false>(A.ts, origin, dir, kInf, stk, cnt);
         if (hit.t >= kInf) return;
         const rt_triangle &tri = A.tris[hit.prim];
         const rt_material &mat = A.mats[tri.material_id];
@@ -377,8 +376,9 @@ hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s) {
     A.max_depth = p.max_depth;
     A.photons = p.photons;
     const dim3 grid((p.n_photons + 63u) / 64u), block(64);
-    if (p.wide) hipLaunchKernelGGL((rtd::photon_trace_kernel<true>), grid, block, 0, s, A);
-    else hipLaunchKernelGGL((rtd::photon_trace_kernel<false>), grid, block, 0, s, A);
+    if (p.mode == 0) hipLaunchKernelGGL((rtd::photon_trace_kernel<0>), grid, block, 0, s, A);
+    else if (p.mode == 1) hipLaunchKernelGGL((rtd::photon_trace_kernel<1>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((rtd::photon_trace_kernel<2>), grid, block, 0, s, A);
     return hipGetLastError();
 }
 

@@ -204,9 +204,26 @@ __global__ __launch_bounds__(256) void accum64_add_kernel(const float4 *__restri
     sum[4 * i + 3] += (double)a.w;
 }
 
+// Plain streaming copy (16 B per lane per iteration): the measured-HBM
+// denominator of the rooflines and the calibration kernel of the traffic counters.
+__global__ __launch_bounds__(256) void hbm_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                       size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256u;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace rtd
 
 namespace rtk {
+
+hipError_t launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256u * 8u * 8u);
+    hipLaunchKernelGGL(rtd::hbm_copy_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4 *>(src),
+                       reinterpret_cast<float4 *>(dst), n);
+    return hipGetLastError();
+}
 
 hipError_t launch_denoise(const DenoiseLaunch &d, hipStream_t s) {
     const dim3 grid((d.p.width + rtd::kDnTile - 1) / rtd::kDnTile, (d.p.height + rtd::kDnTile - 1) / rtd::kDnTile);

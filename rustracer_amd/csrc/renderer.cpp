@@ -134,7 +134,10 @@ struct rt_renderer {
         std::memcpy(t.root_max, root_max, sizeof(root_max));
         return t;
     }
-    bool use_wide() const { return wide_ok && !(cfg.flags & RT_FLAG_REFERENCE_TRAVERSAL); }
+    uint32_t traversal_mode() const {
+        if (!wide_ok || (cfg.flags & RT_FLAG_REFERENCE_TRAVERSAL)) return 0u;
+        return (cfg.flags & RT_FLAG_NEAREST_FIRST) ? 2u : 1u;
+    }
 
     rt_image_params image_params(float alpha) const {
         rt_image_params ip;
@@ -173,8 +176,8 @@ struct rt_renderer {
         g.moments = moments;
         g.gbuffer = gbuf;
         g.max_depth = cfg.max_depth;
-        g.wide = use_wide() ? 1u : 0u;
-        g.anyhit_shadow = (cfg.flags & RT_FLAG_CLOSEST_SHADOW) ? 0u : 1u;
+        g.mode = traversal_mode();
+        g.early_shadow = (cfg.flags & RT_FLAG_CLOSEST_SHADOW) ? 0u : 1u;
         g.counters = nullptr;
         return g;
     }
@@ -277,7 +280,7 @@ struct rt_renderer {
         p.gp = gp;
         p.n_photons = traced_photons();
         p.max_depth = cfg.max_depth;
-        p.wide = use_wide() ? 1u : 0u;
+        p.mode = traversal_mode();
         p.photons = d_photons;
         hipEvent_t ev;
         begin(RT_PASS_PHOTON_TRACE, ev);
@@ -759,7 +762,7 @@ int rt_trace_closest_device(rt_renderer *r, const rt_ray *d_rays, uint32_t n, rt
         if (!r || (n && (!d_rays || !d_out))) throw ApiError(RT_ERR_INVALID, "null argument");
         hipEvent_t ev;
         r->begin(RT_PASS_TRACE_RAYS, ev);
-        hip_check(rtk::launch_trace_rays(r->trace_inputs(), r->use_wide() ? 1u : 0u, d_rays, n, d_out, r->stream),
+        hip_check(rtk::launch_trace_rays(r->trace_inputs(), r->traversal_mode(), d_rays, n, d_out, r->stream),
                   "trace rays");
         r->end(RT_PASS_TRACE_RAYS, ev);
     });
@@ -774,7 +777,7 @@ int rt_trace_closest(rt_renderer *r, const rt_ray *rays, uint32_t n, rt_hit *out
         try {
             d_h = dev_alloc<rt_hit>(n, "hits");
             hip_check(hipMemcpy(d_r, rays, sizeof(rt_ray) * n, hipMemcpyHostToDevice), "upload rays");
-            hip_check(rtk::launch_trace_rays(r->trace_inputs(), r->use_wide() ? 1u : 0u, d_r, n, d_h, r->stream),
+            hip_check(rtk::launch_trace_rays(r->trace_inputs(), r->traversal_mode(), d_r, n, d_h, r->stream),
                       "trace rays");
             hip_check(hipStreamSynchronize(r->stream), "sync");
             hip_check(hipMemcpy(out, d_h, sizeof(rt_hit) * n, hipMemcpyDeviceToHost), "read hits");
@@ -837,6 +840,46 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         out->nodes_visited = c[1];
         out->tris_tested = c[2];
         out->photon_reads = c[3];
+    });
+}
+
+int rt_hbm_copy_probe(int device, size_t bytes, uint32_t iters, double *gbps, double *ms_per_copy) {
+    return api_call([&] {
+        if (!gbps || bytes < 16 || iters == 0) throw ApiError(RT_ERR_INVALID, "bad argument");
+        select_device(device);
+        void *src = nullptr, *dst = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (src) hipFree(src);
+            if (dst) hipFree(dst);
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+            if (s) hipStreamDestroy(s);
+        };
+        try {
+            bytes &= ~(size_t)15;
+            hip_check(hipMalloc(&src, bytes), "probe alloc");
+            hip_check(hipMalloc(&dst, bytes), "probe alloc");
+            hip_check(hipMemset(src, 1, bytes), "probe fill");
+            hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "probe stream");
+            hip_check(hipEventCreate(&e0), "probe event");
+            hip_check(hipEventCreate(&e1), "probe event");
+            hip_check(rtk::launch_hbm_copy(src, dst, bytes, s), "probe warmup");
+            hip_check(hipEventRecord(e0, s), "probe event");
+            for (uint32_t i = 0; i < iters; i++) hip_check(rtk::launch_hbm_copy(src, dst, bytes, s), "probe copy");
+            hip_check(hipEventRecord(e1, s), "probe event");
+            hip_check(hipEventSynchronize(e1), "probe sync");
+            float ms = 0.f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "probe elapsed");
+            const double per = ms / iters;
+            *gbps = 2.0 * (double)bytes / (per * 1e-3) / 1e9;
+            if (ms_per_copy) *ms_per_copy = per;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
     });
 }
 

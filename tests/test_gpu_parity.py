@@ -16,7 +16,8 @@ from .helpers import camera_rays, first_diff, image_params, oracle_frames, rando
 
 pytestmark = pytest.mark.gpu
 
-FLAG_VARIANTS = [0, _ffi.FLAG_REFERENCE_TRAVERSAL, _ffi.FLAG_CLOSEST_SHADOW]
+FLAG_VARIANTS = [0, _ffi.FLAG_REFERENCE_TRAVERSAL, _ffi.FLAG_CLOSEST_SHADOW,
+                 _ffi.FLAG_REFERENCE_TRAVERSAL | _ffi.FLAG_CLOSEST_SHADOW]
 
 
 def _renderer(scene, **cfg):
@@ -44,9 +45,30 @@ def test_trace_closest_bit_exact(scene_name, flags):
     brute = o.brute(rays)
     _check_equal("t", got["t"], ref["t"])
     assert np.array_equal(got["prim_id"], ref["prim_id"]), first_diff(got["prim_id"], ref["prim_id"])
-    # the BVH answer is the brute-force closest hit (ties: lowest slot)
-    _check_equal("t vs brute force", ref["t"], brute["t"])
+    # the reference's BVH answer is the brute-force closest hit except where a
+    # hit's rounded t lies before its own box's entry (the strict pruning then
+    # depends on the visiting order); it is never nearer than brute force
+    assert (ref["t"] >= brute["t"]).all()
+    assert (ref["t"] == brute["t"]).mean() > 0.999
     assert (ref["t"] < 1e38).sum() > len(rays) // 10
+
+
+@pytest.mark.parametrize("scene_name", ["cornell_box", "photon_plain"])
+def test_trace_nearest_first_mismatch_count(scene_name):
+    """RT_FLAG_NEAREST_FIRST is not exact by construction: count its differences
+    from the reference order (expected: only rounding-boundary ties)."""
+    scene = rt.Scene.builtin(scene_name, 3)
+    lo, hi = scene.bounds()
+    rays = np.concatenate([camera_rays(rt.Scene.builtin_camera(scene_name, 3), 96, 64, 1),
+                           random_rays(lo, hi, 20000, 7)])
+    with _renderer(scene, width=8, height=8, flags=_ffi.FLAG_NEAREST_FIRST) as r:
+        got = r.trace_closest(rays)
+    ref = orc.OracleScene.from_scene(scene).trace(rays)
+    bad = (got["t"] != ref["t"]) | (got["prim_id"] != ref["prim_id"])
+    print(f"nearest-first {scene_name}: {bad.sum()} of {len(rays)} rays differ", flush=True)
+    assert bad.mean() < 1e-3
+    dt = np.abs(got["t"][bad].astype(np.float64) - ref["t"][bad]) / ref["t"][bad]
+    assert (dt < 1e-5).all()
 
 
 @pytest.mark.parametrize("scene_name,spp,depth", [("cornell_box", 4, 6), ("cornell36", 16, 5),
@@ -185,7 +207,9 @@ def test_fullframe_1080p_properties():
         f, m, g = r.read_frame(), r.read_moments(), r.read_gbuffer()
         out = r.read_output_rgba8()
     assert np.isfinite(f).all() and np.isfinite(m).all()
-    assert (f[..., :3] >= 0).all() and (f[..., :3] <= 10).all()
+    # the gather clamps to 10; the a-trous weighted mean may round one ulp above
+    assert (f[..., :3] >= 0).all() and (f[..., :3] <= np.float32(10.00001)).all()
+    assert (m[..., 0] <= 10).all() and (m[..., 1] <= 100).all()
     hit = g[..., 3] < 1e29
     assert hit.mean() > 0.5
     n = np.linalg.norm(g[..., :3][hit], axis=-1)
