@@ -14,7 +14,8 @@ roofline block is for the dominant kernel (gather): algorithmic bytes per
 launch from the traversal counters of one extra, untimed launch, divided by the
 kernel's mean duration measured with HIP events on the renderer's stream.
 The cpu_baseline is the CPU oracle (a C++ restatement of the same WGSL, OpenMP)
-on a band of rows of the same frame, also checked bit-for-bit against the GPU rows.
+rendering whole frames of the same workload (gather only); its first frame is
+checked bit for bit against the GPU's moments and G-buffer of that frame.
 """
 from __future__ import annotations
 
@@ -34,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 NODE_PAIR_BYTES = 64   # one child-box record tests two child boxes
 TRI_BYTES = 48         # slot triangle: v0, v1-v0, v2-v0 (3 x float4)
 RAY_BYTES = 32         # 24 B ray + 8 B hit
-PHOTON_BYTES = 64
+PHOTON_BYTES = 16       # position read per radius-gather candidate (power/incident only on accept)
 PIXEL_OUT_BYTES = 48   # frame + moments + gbuffer float4 writes
 
 
@@ -54,6 +55,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reference-traversal", action="store_true")
+    ap.add_argument("--nearest-first", action="store_true",
+                    help="child-box traversal nearest child first (not bit-exact by construction)")
+    ap.add_argument("--no-photons", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -71,9 +75,11 @@ def main() -> int:
     t_scene = time.perf_counter() - t0
     counts = scene.counts()
     log(f"scene fullframe: {counts['triangles']} triangles ({t_scene:.1f}s)")
-    flags = _ffi.FLAG_PROFILE | (_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0)
+    flags = (_ffi.FLAG_PROFILE | (_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0) |
+             (_ffi.FLAG_NEAREST_FIRST if args.nearest_first else 0))
+    photons_on = not args.no_photons
     cfg = rt.RenderConfig(width=args.width, height=args.height, spp=1, max_depth=args.depth,
-                          enable_photon_map=True, photon_count=args.photons, enable_denoise=True,
+                          enable_photon_map=photons_on, photon_count=args.photons, enable_denoise=True,
                           accumulate_frames=1 << 20, flags=flags)
     t0 = time.perf_counter()
     r = rt.Renderer(cfg, scene, 0)
@@ -81,7 +87,8 @@ def main() -> int:
     log(f"renderer created (flatten + BVH build + upload) in {t_create:.1f}s")
     r.update_camera(cam)
     t0 = time.perf_counter()
-    r.rebuild_photons()
+    if photons_on:
+        r.rebuild_photons()
     r.synchronize()
     t_photons = time.perf_counter() - t0
     photon_stats = r.pass_stats()
@@ -134,7 +141,10 @@ def main() -> int:
                                f"{args.depth}, {args.photons} photons, a-trous x2",
                    "width": w, "height": h, "spp": 1, "depth": args.depth, "photons": args.photons,
                    "triangles": counts["triangles"],
-                   "traversal": "reference-order" if args.reference_traversal else "child-box nearest-first"},
+                   "traversal": "reference BVH2 nodes" if args.reference_traversal
+                   else ("child-box records, nearest first" if args.nearest_first
+                         else "child-box records, reference order"),
+                   "photon_map": photons_on},
         "mrays_per_s": round(mrays, 2),
         "pass_ms_per_frame": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},
         "photon_rebuild_ms": round(t_photons * 1e3, 2),
@@ -160,9 +170,8 @@ def main() -> int:
 
 
 def cpu_baseline(rt, r, scene, cam, cfg, args):
-    """The oracle on host cores over a band of rows of the frame the GPU just
-    rendered, until about --cpu-seconds of CPU time; the band is compared
-    bit-for-bit with the GPU's rows."""
+    """The oracle on host cores over whole frames, from the frame the GPU just
+    rendered on, until about --cpu-seconds of CPU time."""
     import numpy as np
 
     from tests.helpers import image_params

@@ -31,6 +31,7 @@ struct GatherLaunch {
     rt_grid_params gp;
     const uint32_t *grid_meta;  // [start, count] per cell
     const rt_photon *sorted_photons;
+    const void *sorted_pos;     // float4 position per sorted photon
     void *frame;    // float4 per pixel
     void *moments;  // float4 per pixel
     void *gbuffer;  // float4 per pixel
@@ -67,6 +68,7 @@ struct GridBuildLaunch {
     uint32_t meta_cells;   // >= num_cells
     uint32_t *grid_meta;   // 2 per cell
     rt_photon *sorted;
+    void *sorted_pos;      // float4 per sorted photon
     // temporaries (device), sized by grid_build_temp_bytes
     void *temp;
     size_t temp_bytes;

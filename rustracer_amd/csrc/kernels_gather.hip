@@ -24,6 +24,7 @@ struct GatherArgs {
     rt_grid_params gp;
     const uint2 *grid_meta;
     const rt_photon *sorted;
+    const float4 *sorted_pos;
     float4 *frame;
     float4 *moments;
     float4 *gbuffer;
@@ -330,12 +331,12 @@ __device__ void gather_pixel(const GatherArgs &A, Tracer<MODE, EARLY, COUNT> &tr
                                     break;
                                 }
                                 budget -= 1u;
-                                const rt_photon &ph = A.sorted[meta.x + i];
+                                const float4 pp4 = A.sorted_pos[meta.x + i];
                                 if (COUNT) photon_reads++;
-                                const float4 pp4 = *reinterpret_cast<const float4 *>(&ph.position[0]);
                                 const F3 d = xyz(pp4) - hp;
                                 const float d2 = dot(d, d);
                                 if (d2 < r2) {
+                                    const rt_photon &ph = A.sorted[meta.x + i];
                                     const float4 pw = *reinterpret_cast<const float4 *>(&ph.power[0]);
                                     const float4 inc = *reinterpret_cast<const float4 *>(&ph.incident[0]);
                                     ph_sum = ph_sum + xyz(pw) * wmax(dot(shn, -xyz(inc)), 0.0f);
@@ -509,6 +510,7 @@ hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
     A.gp = g.gp;
     A.grid_meta = reinterpret_cast<const uint2 *>(g.grid_meta);
     A.sorted = g.sorted_photons;
+    A.sorted_pos = reinterpret_cast<const float4 *>(g.sorted_pos);
     A.frame = reinterpret_cast<float4 *>(g.frame);
     A.moments = reinterpret_cast<float4 *>(g.moments);
     A.gbuffer = reinterpret_cast<float4 *>(g.gbuffer);

@@ -338,9 +338,12 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *keys
     }
 }
 
-// photon_scatter.wgsl:35-36 with the sorted order: copy the 64-byte records.
+// photon_scatter.wgsl:35-36 with the sorted order: copy the 64-byte records,
+// and the first 16 bytes (position, zero pad) once more into a dense array that
+// the radius gather scans (16 B per candidate instead of 64).
 __global__ __launch_bounds__(256) void grid_gather_kernel(const rt_photon *photons, const uint32_t *vals,
-                                                          const uint32_t *n_valid, rt_photon *sorted) {
+                                                          const uint32_t *n_valid, rt_photon *sorted,
+                                                          float4 *sorted_pos) {
     // 4 lanes per photon, one float4 each
     const size_t t = (size_t)blockIdx.x * 256u + threadIdx.x;
     const size_t i = t >> 2;
@@ -348,7 +351,9 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const rt_photon *photo
     if (i >= *n_valid) return;
     const float4 *src = reinterpret_cast<const float4 *>(photons + vals[i]);
     float4 *dst = reinterpret_cast<float4 *>(sorted + i);
-    dst[part] = src[part];
+    const float4 v = src[part];
+    dst[part] = v;
+    if (part == 0u) sorted_pos[i] = v;
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t value, size_t n) {
@@ -452,7 +457,7 @@ hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
     }
     const size_t threads = (size_t)n * 4;
     hipLaunchKernelGGL(rtd::grid_gather_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g.photons,
-                       vin, g.n_valid_out, g.sorted);
+                       vin, g.n_valid_out, g.sorted, reinterpret_cast<float4 *>(g.sorted_pos));
     return hipGetLastError();
 }
 
