@@ -251,6 +251,40 @@ typedef struct rt_trace_counters {
 } rt_trace_counters;
 int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out);
 
+/* ------------------------------------------------------------------------ */
+/* Many views in one persistent launch. No counterpart in the reference, which
+ * renders one camera per dispatch: each view is the gather of gather.wgsl with
+ * its own camera, resolution, spp and seed frame (every other parameter comes
+ * from the renderer's config), and its pixels are bit-identical to a
+ * single-view render with those values. Persistent workgroups pull tiles from a
+ * global queue (most expensive first). */
+typedef struct rt_view {
+    rt_camera_uniform camera; /* Camera::to_uniform at this view's resolution */
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;
+    uint32_t frame; /* ImageParams.frame of the seed, gather.wgsl:262 */
+} rt_view;
+
+typedef struct rt_batch_tuning {
+    uint32_t tile_w, tile_h;  /* pixels per tile; tile_w * tile_h a multiple of 64 */
+    uint32_t waves_per_cu;    /* persistent workgroups (one wavefront each) per CU, 1..20 */
+    uint32_t tiles_per_fetch; /* tiles claimed per queue access */
+} rt_batch_tuning;
+
+int rt_batch_set_views(rt_renderer *r, const rt_view *views, uint32_t n);
+/* One launch over every view; NULL uses the current tuning (default 8x8, 8 waves/CU, 1 tile). */
+int rt_render_batch(rt_renderer *r, const rt_batch_tuning *tuning);
+/* Sweep tile shape, waves per CU and queue granularity on the current views,
+ * keep the fastest (mean of `repeats` timed launches after one warm-up). */
+int rt_batch_autotune(rt_renderer *r, uint32_t repeats, rt_batch_tuning *best, double *best_ms);
+int rt_batch_get_tuning(const rt_renderer *r, rt_batch_tuning *out);
+int rt_batch_totals(const rt_renderer *r, uint64_t *pixels, uint64_t *samples);
+/* frame / moments / G-buffer of one view (each width*height*4 floats; NULL skips). */
+int rt_read_batch_view(rt_renderer *r, uint32_t view, float *frame, float *moments, float *gbuffer,
+                       size_t n_floats);
+#define RT_PASS_BATCH 7
+
 /* Measured HBM bandwidth: a plain device-to-device copy kernel of `bytes`,
  * repeated `iters` times; *gbps counts bytes read + written. No counterpart in
  * the reference (measurement support). */

@@ -361,6 +361,41 @@ class Renderer:
         return dict(ray_queries=t.ray_queries, nodes_visited=t.nodes_visited, tris_tested=t.tris_tested,
                     photon_reads=t.photon_reads)
 
+    # -- many views in one persistent launch --
+    def set_views(self, views) -> None:
+        """views: sequence of (Camera, width, height, spp, frame)."""
+        arr = (_ffi.View * len(views))()
+        self._views = []
+        for k, (cam, w, h, spp, frame) in enumerate(views):
+            arr[k].camera = camera_to_uniform(cam, w, h)
+            arr[k].width, arr[k].height, arr[k].spp, arr[k].frame = w, h, spp, frame
+            self._views.append((w, h))
+        check(lib().rt_batch_set_views(self._h, arr, len(views)), "rt_batch_set_views")
+
+    def render_batch(self, tile_w=None, tile_h=8, waves_per_cu=8, tiles_per_fetch=1) -> None:
+        t = None
+        if tile_w is not None:
+            t = C.byref(_ffi.BatchTuning(tile_w, tile_h, waves_per_cu, tiles_per_fetch))
+        check(lib().rt_render_batch(self._h, t), "rt_render_batch")
+
+    def batch_autotune(self, repeats: int = 1):
+        best = _ffi.BatchTuning()
+        ms = C.c_double()
+        check(lib().rt_batch_autotune(self._h, repeats, C.byref(best), C.byref(ms)), "rt_batch_autotune")
+        return best.as_dict(), ms.value
+
+    def batch_totals(self):
+        px, smp = C.c_uint64(), C.c_uint64()
+        check(lib().rt_batch_totals(self._h, C.byref(px), C.byref(smp)), "rt_batch_totals")
+        return px.value, smp.value
+
+    def read_batch_view(self, k: int):
+        w, h = self._views[k]
+        out = [np.zeros((h, w, 4), np.float32) for _ in range(3)]
+        check(lib().rt_read_batch_view(self._h, k, ptr(out[0]), ptr(out[1]), ptr(out[2]), out[0].size),
+              "rt_read_batch_view")
+        return tuple(out)
+
     def accum64_reset(self) -> None:
         check(lib().rt_accum64_reset(self._h), "rt_accum64_reset")
 

@@ -44,7 +44,7 @@ GRID_MAX_CELLS = 26 * 26 * 26
 RT_OK, RT_ERR_INVALID, RT_ERR_UNSUPPORTED, RT_ERR_HIP, RT_ERR_NO_DEVICE, RT_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
 FLAG_PROFILE, FLAG_CLOSEST_SHADOW, FLAG_REFERENCE_TRAVERSAL, FLAG_NEAREST_FIRST = 1, 2, 4, 8
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
-              "accum64", "spare")
+              "accum64", "batch")
 
 
 class Config(C.Structure):
@@ -106,6 +106,19 @@ class Camera(C.Structure):
 
 class PassStats(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8)]
+
+
+class View(C.Structure):
+    _fields_ = [("camera", CameraUniform), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("frame", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("waves_per_cu", C.c_uint32),
+                ("tiles_per_fetch", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class TraceCounters(C.Structure):
@@ -184,6 +197,12 @@ SIGNATURES = {
     "rt_reset_pass_stats": (_i, [_vp]),
     "rt_count_gather_traffic": (_i, [_vp, _P(TraceCounters)]),
     "rt_hbm_copy_probe": (_i, [_i, C.c_size_t, _u32, _P(C.c_double), _P(C.c_double)]),
+    "rt_batch_set_views": (_i, [_vp, _vp, _u32]),
+    "rt_render_batch": (_i, [_vp, _vp]),
+    "rt_batch_autotune": (_i, [_vp, _u32, _vp, _P(C.c_double)]),
+    "rt_batch_get_tuning": (_i, [_vp, _vp]),
+    "rt_batch_totals": (_i, [_vp, _P(C.c_uint64), _P(C.c_uint64)]),
+    "rt_read_batch_view": (_i, [_vp, _u32, _vp, _vp, _vp, C.c_size_t]),
     "rt_accum64_reset": (_i, [_vp]),
     "rt_accum64_add_accumulation": (_i, [_vp]),
     "rt_accum64_read_mean": (_i, [_vp, _vp, C.c_size_t, _P(C.c_uint64)]),

@@ -42,6 +42,26 @@ struct GatherLaunch {
 };
 
 hipError_t launch_gather(const GatherLaunch &g, hipStream_t s);
+
+// One view of a batch as the batch kernel reads it (128 bytes).
+struct BatchViewRecord {
+    rt_camera_uniform cam;
+    uint32_t width, height, spp, frame;
+    uint64_t pixel_offset;
+    uint32_t tiles_x;
+    uint32_t pad;
+};
+struct BatchLaunch {
+    const BatchViewRecord *views;
+    const uint32_t *tiles;     // (view, tile) pairs
+    uint32_t n_tiles;
+    uint32_t tile_w, tile_h;
+    uint32_t tiles_per_fetch;
+    uint32_t n_workgroups;     // persistent workgroups (one wavefront each)
+    uint32_t *counter;
+};
+// g.frame / g.moments / g.gbuffer point at the batch outputs (sum of view pixels).
+hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipStream_t s);
 hipError_t launch_trace_rays(const TraceInputs &t, uint32_t mode, const rt_ray *rays, uint32_t n, rt_hit *out,
                              hipStream_t s);
 

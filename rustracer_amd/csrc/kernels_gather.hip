@@ -182,12 +182,20 @@ struct Acc {
 
 __device__ __forceinline__ int32_t iclamp(int32_t v, int32_t lo, int32_t hi) { return min(max(v, lo), hi); }
 
-// gather.wgsl:258-594 for the pixel (px, py).
+// The per-view part of ImageParams/CameraUniform: one camera at one resolution.
+struct ViewParams {
+    rt_camera_uniform cam;
+    uint32_t width, height, spp, frame;
+};
+
+// gather.wgsl:258-594 for the pixel (px, py) of view `V`; the result goes to
+// element `out_idx` of the frame / moments / G-buffer arrays.
 template <int MODE, bool EARLY, bool COUNT>
-__device__ void gather_pixel(const GatherArgs &A, Tracer<MODE, EARLY, COUNT> &tr, uint32_t px, uint32_t py,
-                             uint32_t &photon_reads) {
+__device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MODE, EARLY, COUNT> &tr, uint32_t px,
+                             uint32_t py, size_t out_idx, uint32_t &photon_reads) {
     const rt_image_params &ip = A.ip;
-    const uint32_t base_seed = py * ip.width + px + ip.frame * 2654435761u;
+    const rt_camera_uniform &cam = V.cam;
+    const uint32_t base_seed = py * V.width + px + V.frame * 2654435761u;
     Rng rng = rng_init(base_seed * 1664525u + 1013904223u);
     Acc acc;
     acc.col = fs(0.0f);
@@ -197,7 +205,7 @@ __device__ void gather_pixel(const GatherArgs &A, Tracer<MODE, EARLY, COUNT> &tr
     for (int i = 0; i < 8; i++) acc.sl[i] = 0.0f;
     F3 first_normal = f3(0.0f, 1.0f, 0.0f);
     float first_depth = 1e30f;
-    const uint32_t spp = max(1u, ip.spp);
+    const uint32_t spp = max(1u, V.spp);
     const uint32_t debug_view = ip._pad[0];
 
     float env_intensity = 0.0f;
@@ -211,14 +219,14 @@ __device__ void gather_pixel(const GatherArgs &A, Tracer<MODE, EARLY, COUNT> &tr
             break;
         }
     }
-    const F3 cam_pos = ld3(A.cam.position), cam_fwd = ld3(A.cam.forward);
-    const F3 cam_right = ld3(A.cam.right), cam_up = ld3(A.cam.up);
+    const F3 cam_pos = ld3(cam.position), cam_fwd = ld3(cam.forward);
+    const F3 cam_right = ld3(cam.right), cam_up = ld3(cam.up);
 
     for (uint32_t s = 0u; s < spp; s++) {
         const float jx = rng_f32(rng) - 0.5f;
         const float jy = rng_f32(rng) - 0.5f;
-        const float ux = ((float)px + jx) * A.cam.inv_half_res[0] - 1.0f;
-        const float uy = 1.0f - ((float)py + jy) * A.cam.inv_half_res[1];
+        const float ux = ((float)px + jx) * cam.inv_half_res[0] - 1.0f;
+        const float uy = 1.0f - ((float)py + jy) * cam.inv_half_res[1];
         F3 ro = cam_pos;
         F3 rd = normalize(cam_fwd + cam_right * ux + cam_up * uy);
         F3 thr = fs(1.0f);
@@ -433,10 +441,10 @@ __device__ void gather_pixel(const GatherArgs &A, Tracer<MODE, EARLY, COUNT> &tr
         acc.lum2_sum = cl2;
     }
     const F3 col = vmin(acc.col / (float)spp, fs(10.0f));
-    const size_t idx = (size_t)py * ip.width + px;
-    A.frame[idx] = make_float4(col.x, col.y, col.z, 1.0f);
-    A.moments[idx] = make_float4(wmin(acc.lum_sum / (float)spp, 10.0f), wmin(acc.lum2_sum / (float)spp, 100.0f), 0.0f, 0.0f);
-    A.gbuffer[idx] = make_float4(first_normal.x, first_normal.y, first_normal.z, first_depth);
+    A.frame[out_idx] = make_float4(col.x, col.y, col.z, 1.0f);
+    A.moments[out_idx] =
+        make_float4(wmin(acc.lum_sum / (float)spp, 10.0f), wmin(acc.lum2_sum / (float)spp, 100.0f), 0.0f, 0.0f);
+    A.gbuffer[out_idx] = make_float4(first_normal.x, first_normal.y, first_normal.z, first_depth);
 }
 
 template <int MODE, bool EARLY, bool COUNT>
@@ -455,12 +463,70 @@ __global__ __launch_bounds__(64) void gather_kernel(GatherArgs A) {
     stk.stride = 64u;
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
-    if (px < A.ip.width && py < A.ip.height) gather_pixel<MODE, EARLY, COUNT>(A, tr, px, py, photon_reads);
+    const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
+    if (px < A.ip.width && py < A.ip.height)
+        gather_pixel<MODE, EARLY, COUNT>(A, V, tr, px, py, (size_t)py * A.ip.width + px, photon_reads);
     if (COUNT) {
         atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
         atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
         atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+    }
+}
+
+// ─────────────────────────── many views, one persistent launch ───────────────────────────
+
+struct BatchView {
+    rt_camera_uniform cam;
+    uint32_t width, height, spp, frame;
+    uint64_t pixel_offset;  // first element of this view in the batch outputs
+    uint32_t tiles_x;
+    uint32_t pad;
+};
+
+struct BatchArgs {
+    GatherArgs g;             // scene, lights, global image parameters, batch output arrays
+    const BatchView *views;
+    const uint2 *tiles;       // (view, tile index inside the view), most expensive first
+    uint32_t n_tiles;
+    uint32_t tile_w, tile_h;  // tile shape; tile_w * tile_h is a multiple of 64
+    uint32_t tiles_per_fetch; // queue granularity
+    uint32_t *counter;        // next unclaimed tile (zeroed before the launch)
+};
+
+// Persistent workgroups (one wavefront each) pull tiles from a global queue
+// until it is empty; every pixel runs the same per-pixel code as gather_kernel,
+// with its own view's camera, resolution, spp and seed frame.
+template <int MODE, bool EARLY>
+__global__ __launch_bounds__(64) void gather_batch_kernel(BatchArgs B) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    LaneStack stk;
+    stk.lds = &stack_rows[threadIdx.x];
+    stk.stride = 64u;
+    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{0u, 0u}, 0u};
+    uint32_t photon_reads = 0u;
+    const uint32_t rounds = (B.tile_w * B.tile_h) / 64u;
+    for (;;) {
+        uint32_t first = 0u;
+        if (threadIdx.x == 0) first = atomicAdd(B.counter, B.tiles_per_fetch);
+        first = __builtin_amdgcn_readfirstlane(__shfl(first, 0, 64));
+        if (first >= B.n_tiles) break;
+        const uint32_t last = min(first + B.tiles_per_fetch, B.n_tiles);
+        for (uint32_t t = first; t < last; t++) {
+            const uint2 te = B.tiles[t];
+            const uint32_t vi = __builtin_amdgcn_readfirstlane(te.x);
+            const uint32_t ti = __builtin_amdgcn_readfirstlane(te.y);
+            const BatchView &bv = B.views[vi];
+            const ViewParams V{bv.cam, bv.width, bv.height, bv.spp, bv.frame};
+            const uint32_t tx = ti % bv.tiles_x, ty = ti / bv.tiles_x;
+            for (uint32_t r = 0; r < rounds; r++) {
+                const uint32_t k = r * 64u + threadIdx.x;
+                const uint32_t px = tx * B.tile_w + k % B.tile_w, py = ty * B.tile_h + k / B.tile_w;
+                if (px < V.width && py < V.height)
+                    gather_pixel<MODE, EARLY, false>(B.g, V, tr, px, py, bv.pixel_offset + (size_t)py * V.width + px,
+                                                     photon_reads);
+            }
+        }
     }
 }
 
@@ -497,7 +563,7 @@ TraceScene make_trace_scene(const rtk::TraceInputs &t) {
 
 namespace rtk {
 
-hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
+static rtd::GatherArgs gather_args(const GatherLaunch &g) {
     rtd::GatherArgs A;
     A.ts = rtd::make_trace_scene(g.trace);
     A.tris = g.tris;
@@ -519,6 +585,11 @@ hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
     const uint32_t tiles_y = (g.ip.height + 7u) / 8u;
     A.n_tiles = A.tiles_x * tiles_y;
     A.counters = g.counters;
+    return A;
+}
+
+hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
+    const rtd::GatherArgs A = gather_args(g);
     const dim3 grid(A.n_tiles), block(64);
 #define RT_GATHER(M, E, C) hipLaunchKernelGGL((rtd::gather_kernel<M, E, C>), grid, block, 0, s, A)
 #define RT_GATHER_M(M)                        \
@@ -534,6 +605,29 @@ hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
     else { RT_GATHER_M(2) }
 #undef RT_GATHER_M
 #undef RT_GATHER
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipStream_t s) {
+    static_assert(sizeof(rtd::BatchView) == sizeof(BatchViewRecord), "batch view record layout");
+    if (b.n_tiles == 0) return hipSuccess;
+    rtd::BatchArgs B;
+    B.g = gather_args(g);
+    B.views = reinterpret_cast<const rtd::BatchView *>(b.views);
+    B.tiles = reinterpret_cast<const uint2 *>(b.tiles);
+    B.n_tiles = b.n_tiles;
+    B.tile_w = b.tile_w;
+    B.tile_h = b.tile_h;
+    B.tiles_per_fetch = b.tiles_per_fetch;
+    B.counter = b.counter;
+    hipError_t e = launch_fill_u32(b.counter, 0u, 1, s);
+    if (e != hipSuccess) return e;
+    const dim3 grid(b.n_workgroups), block(64);
+#define RT_BATCH(M, E) hipLaunchKernelGGL((rtd::gather_batch_kernel<M, E>), grid, block, 0, s, B)
+    if (g.mode == 0) { if (g.early_shadow) RT_BATCH(0, true); else RT_BATCH(0, false); }
+    else if (g.mode == 1) { if (g.early_shadow) RT_BATCH(1, true); else RT_BATCH(1, false); }
+    else { if (g.early_shadow) RT_BATCH(2, true); else RT_BATCH(2, false); }
+#undef RT_BATCH
     return hipGetLastError();
 }
 

@@ -78,6 +78,21 @@ struct rt_renderer {
     double *d_acc64 = nullptr;
     uint64_t acc64_frames = 0;
 
+    // many-view batch
+    std::vector<rt_view> views;
+    std::vector<uint64_t> view_offsets;
+    uint64_t batch_px = 0, batch_samples = 0;
+    size_t batch_cap_px = 0;
+    float *d_bframe = nullptr, *d_bmom = nullptr, *d_bgb = nullptr;
+    rtk::BatchViewRecord *d_views = nullptr;
+    size_t views_cap = 0;
+    uint32_t *d_tiles = nullptr;
+    size_t tiles_cap = 0;
+    uint32_t n_tiles = 0, tiles_w = 0, tiles_h = 0;
+    uint32_t *d_counter = nullptr;
+    rt_batch_tuning tuning{8u, 8u, 8u, 1u};
+    int n_cu = 0;
+
     std::vector<PendingTiming> pending;
     rt_pass_stats stats{};
 
@@ -91,7 +106,8 @@ struct rt_renderer {
         }
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
-                        d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos};
+                        d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
+                        d_views, d_tiles, d_counter};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -845,6 +861,203 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         out->nodes_visited = c[1];
         out->tris_tested = c[2];
         out->photon_reads = c[3];
+    });
+}
+
+// ───────────────────────────── many-view batch ─────────────────────────────
+
+static void batch_build_tiles(rt_renderer &r, uint32_t tw, uint32_t th) {
+    std::vector<rtk::BatchViewRecord> recs(r.views.size());
+    struct T {
+        uint32_t view, tile, cost;
+    };
+    std::vector<T> tiles;
+    for (size_t v = 0; v < r.views.size(); v++) {
+        const rt_view &vw = r.views[v];
+        rtk::BatchViewRecord &rec = recs[v];
+        std::memset(&rec, 0, sizeof(rec));
+        rec.cam = vw.camera;
+        rec.width = vw.width;
+        rec.height = vw.height;
+        rec.spp = vw.spp;
+        rec.frame = vw.frame;
+        rec.pixel_offset = r.view_offsets[v];
+        rec.tiles_x = (vw.width + tw - 1) / tw;
+        const uint32_t ty = (vw.height + th - 1) / th;
+        for (uint32_t k = 0; k < rec.tiles_x * ty; k++) tiles.push_back({(uint32_t)v, k, std::max(vw.spp, 1u)});
+    }
+    // longest-processing-time first: the queue hands out the costliest tiles first
+    std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
+    std::vector<uint32_t> flat(tiles.size() * 2);
+    for (size_t i = 0; i < tiles.size(); i++) {
+        flat[2 * i] = tiles[i].view;
+        flat[2 * i + 1] = tiles[i].tile;
+    }
+    hip_check(hipStreamSynchronize(r.stream), "sync");
+    if (recs.size() > r.views_cap) {
+        if (r.d_views) hip_check(hipFree(r.d_views), "free");
+        r.d_views = dev_alloc<rtk::BatchViewRecord>(recs.size(), "batch views");
+        r.views_cap = recs.size();
+    }
+    if (flat.size() > r.tiles_cap) {
+        if (r.d_tiles) hip_check(hipFree(r.d_tiles), "free");
+        r.d_tiles = dev_alloc<uint32_t>(flat.size(), "batch tiles");
+        r.tiles_cap = flat.size();
+    }
+    hip_check(hipMemcpy(r.d_views, recs.data(), recs.size() * sizeof(rtk::BatchViewRecord), hipMemcpyHostToDevice),
+              "upload views");
+    hip_check(hipMemcpy(r.d_tiles, flat.data(), flat.size() * 4, hipMemcpyHostToDevice), "upload tiles");
+    r.n_tiles = (uint32_t)tiles.size();
+    r.tiles_w = tw;
+    r.tiles_h = th;
+}
+
+static void batch_validate(const rt_batch_tuning &t) {
+    if (t.tile_w == 0 || t.tile_h == 0 || (t.tile_w * t.tile_h) % 64u != 0 || t.tile_w * t.tile_h > 4096u)
+        throw ApiError(RT_ERR_INVALID, "tile_w * tile_h must be a multiple of 64 (at most 4096)");
+    if (t.waves_per_cu == 0 || t.waves_per_cu > 20)
+        throw ApiError(RT_ERR_INVALID, "waves_per_cu must be in 1..20 (8 KB of LDS stack per wavefront)");
+    if (t.tiles_per_fetch == 0 || t.tiles_per_fetch > 64) throw ApiError(RT_ERR_INVALID, "tiles_per_fetch in 1..64");
+}
+
+static void batch_launch(rt_renderer &r, const rt_batch_tuning &t) {
+    if (r.views.empty()) throw ApiError(RT_ERR_INVALID, "no views set (rt_batch_set_views)");
+    if (r.cfg.enable_photon_map && !r.photons_built)
+        throw ApiError(RT_ERR_INVALID, "photon map enabled but rt_rebuild_photons has not run");
+    batch_validate(t);
+    if (t.tile_w != r.tiles_w || t.tile_h != r.tiles_h) batch_build_tiles(r, t.tile_w, t.tile_h);
+    rtk::GatherLaunch g = r.gather_launch(1.0f, r.d_bframe, r.d_bmom, r.d_bgb);
+    rtk::BatchLaunch b;
+    b.views = r.d_views;
+    b.tiles = r.d_tiles;
+    b.n_tiles = r.n_tiles;
+    b.tile_w = t.tile_w;
+    b.tile_h = t.tile_h;
+    b.tiles_per_fetch = t.tiles_per_fetch;
+    b.n_workgroups = (uint32_t)r.n_cu * t.waves_per_cu;
+    b.counter = r.d_counter;
+    hipEvent_t ev;
+    r.begin(RT_PASS_BATCH, ev);
+    hip_check(rtk::launch_gather_batch(g, b, r.stream), "gather batch");
+    r.end(RT_PASS_BATCH, ev);
+}
+
+extern "C" int rt_batch_set_views(rt_renderer *r, const rt_view *views, uint32_t n) {
+    return api_call([&] {
+        if (!r || (n && !views)) throw ApiError(RT_ERR_INVALID, "null argument");
+        uint64_t px = 0, samples = 0;
+        std::vector<uint64_t> offs(n);
+        for (uint32_t v = 0; v < n; v++) {
+            if (views[v].width == 0 || views[v].height == 0 || views[v].width > 16384 || views[v].height > 16384)
+                throw ApiError(RT_ERR_INVALID, "view resolution must be in 1..16384");
+            offs[v] = px;
+            px += (uint64_t)views[v].width * views[v].height;
+            samples += (uint64_t)views[v].width * views[v].height * std::max(views[v].spp, 1u);
+        }
+        if (px > (1ull << 30)) throw ApiError(RT_ERR_CAPACITY, "batch exceeds 2^30 pixels");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        if (px > r->batch_cap_px) {
+            for (float **b : {&r->d_bframe, &r->d_bmom, &r->d_bgb}) {
+                if (*b) hip_check(hipFree(*b), "free");
+                *b = dev_alloc<float>(px * 4, "batch outputs");
+            }
+            r->batch_cap_px = px;
+        }
+        if (!r->d_counter) r->d_counter = dev_alloc<uint32_t>(1, "batch counter");
+        if (r->n_cu == 0) {
+            hip_check(hipDeviceGetAttribute(&r->n_cu, hipDeviceAttributeMultiprocessorCount, r->device), "CU count");
+        }
+        r->views.assign(views, views + n);
+        r->view_offsets = offs;
+        r->batch_px = px;
+        r->batch_samples = samples;
+        r->tiles_w = r->tiles_h = 0;  // rebuild the tile table at the next launch
+    });
+}
+
+extern "C" int rt_render_batch(rt_renderer *r, const rt_batch_tuning *tuning) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (tuning) {
+            batch_validate(*tuning);
+            r->tuning = *tuning;
+        }
+        batch_launch(*r, r->tuning);
+    });
+}
+
+extern "C" int rt_batch_autotune(rt_renderer *r, uint32_t repeats, rt_batch_tuning *best, double *best_ms) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        repeats = std::max(repeats, 1u);
+        const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {16, 16}, {32, 8}};
+        const uint32_t waves[] = {4, 8, 12, 16};
+        const uint32_t fetch[] = {1, 4};
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        rt_batch_tuning winner = r->tuning;
+        double winner_ms = 1e30;
+        try {
+            for (auto &sh : shapes) {
+                for (uint32_t w : waves) {
+                    for (uint32_t f : fetch) {
+                        const rt_batch_tuning t{sh[0], sh[1], w, f};
+                        batch_launch(*r, t);  // warm-up (also builds the tile table)
+                        hip_check(hipEventRecord(e0, r->stream), "event");
+                        for (uint32_t k = 0; k < repeats; k++) batch_launch(*r, t);
+                        hip_check(hipEventRecord(e1, r->stream), "event");
+                        hip_check(hipEventSynchronize(e1), "sync");
+                        float ms = 0.f;
+                        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+                        const double per = ms / repeats;
+                        if (per < winner_ms) {
+                            winner_ms = per;
+                            winner = t;
+                        }
+                    }
+                }
+            }
+        } catch (...) {
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            throw;
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        r->collect();
+        r->tuning = winner;
+        if (best) *best = winner;
+        if (best_ms) *best_ms = winner_ms;
+    });
+}
+
+extern "C" int rt_batch_get_tuning(const rt_renderer *r, rt_batch_tuning *out) {
+    return api_call([&] {
+        if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        *out = r->tuning;
+    });
+}
+
+extern "C" int rt_batch_totals(const rt_renderer *r, uint64_t *pixels, uint64_t *samples) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (pixels) *pixels = r->batch_px;
+        if (samples) *samples = r->batch_samples;
+    });
+}
+
+extern "C" int rt_read_batch_view(rt_renderer *r, uint32_t view, float *frame, float *moments, float *gbuffer,
+                                  size_t n_floats) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (view >= r->views.size()) throw ApiError(RT_ERR_INVALID, "view index out of range");
+        const size_t npx = (size_t)r->views[view].width * r->views[view].height;
+        if (n_floats < npx * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than width*height*4 floats");
+        const size_t off = r->view_offsets[view] * 4;
+        if (frame) read_dev(r, frame, r->d_bframe + off, npx * 16);
+        if (moments) read_dev(r, moments, r->d_bmom + off, npx * 16);
+        if (gbuffer) read_dev(r, gbuffer, r->d_bgb + off, npx * 16);
     });
 }
 
