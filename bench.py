@@ -32,11 +32,15 @@ METRIC = "Msamples/s @1080p; ms to relMSE≤0.01: batched+learned vs plain à-tr
 HBM_PEAK_GBS = 8000.0
 
 # Bytes per unit of work of the traversal layouts (roofline section).
-NODE_PAIR_BYTES = 64   # one child-box record tests two child boxes
+NODE_BOX_BYTES = 32    # one child box: min/max (binary record 64 B / 2, 4-wide record 128 B / 4)
 TRI_BYTES = 48         # slot triangle: v0, v1-v0, v2-v0 (3 x float4)
 RAY_BYTES = 32         # 24 B ray + 8 B hit
 PHOTON_BYTES = 16       # position read per radius-gather candidate (power/incident only on accept)
 PIXEL_OUT_BYTES = 48   # frame + moments + gbuffer float4 writes
+
+
+TRAVERSALS = {0: "reference BVH2 nodes", 1: "binary child-box records, reference order",
+              2: "binary child-box records, nearest first", 3: "4-wide records, reference order"}
 
 
 def log(msg: str) -> None:
@@ -57,6 +61,8 @@ def main() -> int:
     ap.add_argument("--reference-traversal", action="store_true")
     ap.add_argument("--nearest-first", action="store_true",
                     help="child-box traversal nearest child first (not bit-exact by construction)")
+    ap.add_argument("--binary-records", action="store_true",
+                    help="binary child-box records instead of the 4-wide records (both exact)")
     ap.add_argument("--no-photons", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -76,7 +82,8 @@ def main() -> int:
     counts = scene.counts()
     log(f"scene fullframe: {counts['triangles']} triangles ({t_scene:.1f}s)")
     flags = (_ffi.FLAG_PROFILE | (_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0) |
-             (_ffi.FLAG_NEAREST_FIRST if args.nearest_first else 0))
+             (_ffi.FLAG_NEAREST_FIRST if args.nearest_first else 0) |
+             (_ffi.FLAG_BINARY_RECORDS if args.binary_records else 0))
     photons_on = not args.no_photons
     cfg = rt.RenderConfig(width=args.width, height=args.height, spp=1, max_depth=args.depth,
                           enable_photon_map=photons_on, photon_count=args.photons, enable_denoise=True,
@@ -84,7 +91,8 @@ def main() -> int:
     t0 = time.perf_counter()
     r = rt.Renderer(cfg, scene, 0)
     t_create = time.perf_counter() - t0
-    log(f"renderer created (flatten + BVH build + upload) in {t_create:.1f}s")
+    tinfo = r.traversal_info()
+    log(f"renderer created (flatten + BVH build + upload) in {t_create:.1f}s; traversal {tinfo}")
     r.update_camera(cam)
     t0 = time.perf_counter()
     if photons_on:
@@ -119,7 +127,7 @@ def main() -> int:
     cnt = r.count_gather_traffic()
     gather_ms, gather_n = stats["gather"]
     gather_avg_ms = gather_ms / max(gather_n, 1)
-    alg_bytes = (cnt["nodes_visited"] / 2 * NODE_PAIR_BYTES + cnt["tris_tested"] * TRI_BYTES +
+    alg_bytes = (cnt["nodes_visited"] * NODE_BOX_BYTES + cnt["tris_tested"] * TRI_BYTES +
                  cnt["ray_queries"] * RAY_BYTES + cnt["photon_reads"] * PHOTON_BYTES + w * h * PIXEL_OUT_BYTES)
     achieved = alg_bytes / (gather_avg_ms * 1e-3) / 1e9
     mrays = cnt["ray_queries"] * args.steps / elapsed / 1e6
@@ -141,9 +149,7 @@ def main() -> int:
                                f"{args.depth}, {args.photons} photons, a-trous x2",
                    "width": w, "height": h, "spp": 1, "depth": args.depth, "photons": args.photons,
                    "triangles": counts["triangles"],
-                   "traversal": "reference BVH2 nodes" if args.reference_traversal
-                   else ("child-box records, nearest first" if args.nearest_first
-                         else "child-box records, reference order"),
+                   "traversal": TRAVERSALS[tinfo["mode"]], "bvh_depth": tinfo["bvh_depth"],
                    "photon_map": photons_on},
         "mrays_per_s": round(mrays, 2),
         "pass_ms_per_frame": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},

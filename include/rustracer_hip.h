@@ -77,6 +77,9 @@ typedef struct rt_config {
 /* Child-box traversal nearest child first: fewer nodes visited; equals the
  * reference except where a hit's rounded t falls before its own box's entry. */
 #define RT_FLAG_NEAREST_FIRST 8u
+/* Use the binary child-box records instead of the default 4-wide records
+ * (both exact; for measurement). */
+#define RT_FLAG_BINARY_RECORDS 16u
 
 /* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
 int rt_config_default(rt_config *cfg);
@@ -220,6 +223,11 @@ int rt_read_grid_meta(rt_renderer *r, uint32_t *meta, size_t n_uint32, uint32_t 
 int rt_trace_closest(rt_renderer *r, const rt_ray *rays, uint32_t n, rt_hit *out);
 /* Same, with ray and hit arrays already in device memory, on the renderer's stream. */
 int rt_trace_closest_device(rt_renderer *r, const rt_ray *d_rays, uint32_t n, rt_hit *d_out);
+/* Which traversal the renderer runs (0 reference BVH2 nodes, 1 binary
+ * child-box records, 2 binary nearest-first, 3 4-wide records), the BVH depth
+ * and the number of 4-wide records (0 when that layout is unavailable). No
+ * reference counterpart: it reports this backend's derived layouts. */
+int rt_traversal_info(const rt_renderer *r, uint32_t *mode, uint32_t *bvh_depth, uint32_t *n_records4);
 /* Build the photon grid from a given photon array (count, scan, stable
  * scatter: photon_count.wgsl, lib.rs:987-1007, photon_scatter.wgsl) into the
  * renderer's grid, replacing traced photons. n_slots <= photon_count*max_depth. */

@@ -342,6 +342,13 @@ class Renderer:
     def trace_closest_device(self, d_rays: int, n: int, d_out: int) -> None:
         check(lib().rt_trace_closest_device(self._h, d_rays, n, d_out), "rt_trace_closest_device")
 
+    def traversal_info(self) -> dict:
+        """Active traversal (0 BVH2 nodes, 1 binary records, 2 nearest-first, 3 4-wide)."""
+        mode, depth, n4 = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().rt_traversal_info(self._h, C.byref(mode), C.byref(depth), C.byref(n4)),
+              "rt_traversal_info")
+        return {"mode": mode.value, "bvh_depth": depth.value, "records4": n4.value}
+
     def build_photon_grid_from(self, photons: np.ndarray) -> None:
         p = np.ascontiguousarray(photons, PHOTON)
         check(lib().rt_build_photon_grid_from(self._h, ptr(p), len(p)), "rt_build_photon_grid_from")

@@ -43,6 +43,7 @@ GRID_MAX_CELLS = 26 * 26 * 26
 
 RT_OK, RT_ERR_INVALID, RT_ERR_UNSUPPORTED, RT_ERR_HIP, RT_ERR_NO_DEVICE, RT_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
 FLAG_PROFILE, FLAG_CLOSEST_SHADOW, FLAG_REFERENCE_TRAVERSAL, FLAG_NEAREST_FIRST = 1, 2, 4, 8
+FLAG_BINARY_RECORDS = 16
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
               "accum64", "batch")
 
@@ -192,6 +193,7 @@ SIGNATURES = {
     "rt_read_grid_meta": (_i, [_vp, _vp, C.c_size_t, _P(_u32)]),
     "rt_trace_closest": (_i, [_vp, _vp, _u32, _vp]),
     "rt_trace_closest_device": (_i, [_vp, _vp, _u32, _vp]),
+    "rt_traversal_info": (_i, [_vp, _vp, _vp, _vp]),
     "rt_build_photon_grid_from": (_i, [_vp, _vp, _u32]),
     "rt_get_pass_stats": (_i, [_vp, _P(PassStats)]),
     "rt_reset_pass_stats": (_i, [_vp]),

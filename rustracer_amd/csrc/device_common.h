@@ -167,6 +167,7 @@ struct TraceScene {
     const uint32_t *prims;     // slot -> primitive index
     const float4 *slot_tris;   // 3 float4 per slot
     const float4 *wnodes;      // child-box layout: 4 float4 per internal node
+    const float4 *w4nodes;     // 4-wide layout: 8 float4 per record
     uint32_t n_nodes;
     uint32_t pad;
     float4 root_min;           // root box min (xyz) and root ref bits (w) for the child-box layout
@@ -374,12 +375,120 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
     return hit;
 }
 
+// 4-wide traversal. Each record stands for a BVH2 node and holds up to four
+// of its descendants (a cut of its subtree, left-to-right order kept), so the
+// leaves are met in the reference's order. The BVH2 nodes between a record and
+// its children are not tested. That changes no decision: every child box lies
+// inside its parent's box (checked when the records are built), so per axis the
+// slab interval of a child lies inside that of a skipped node, and a skipped
+// node that would fail (te > ta, ta <= 0, or te >= best t at its visit) has a
+// descendant child that fails the same way no later than the reference would
+// test it. One exception comes from 1/rd = +-inf (a zero direction component):
+// a box with a face in the ray's plane then gets a NaN slab end and fails,
+// while a box flat on that axis and lying in the plane gets two NaN ends and
+// ignores the axis. Each child carries one "kill" bit per axis, set at build
+// time when it is flat on that axis at a face of a skipped non-flat node; such
+// a child fails when 1/rd is infinite on that axis and ro lies in its plane,
+// exactly when the skipped node would have failed.
+// Deferred children keep their entry distance and are re-checked at pop time,
+// as in trace_wide.
+//
+// Record (128 bytes): min.x[4], min.y[4], min.z[4], max.x[4], max.y[4],
+// max.z[4] (SoA over the children), refs[4] (absent child = 0xFFFFFFFF),
+// kill bits (3 per child: x, y, z), 12 bytes of padding.
+template <bool EARLY_EXIT, bool COUNT>
+__device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
+    Hit hit{kInf, 0u};
+    const F3 inv = fs(1.0f) / rd;
+    if (S.n_nodes == 0u) return hit;
+    const uint32_t infmask = (__builtin_isinf(inv.x) ? 1u : 0u) | (__builtin_isinf(inv.y) ? 2u : 0u) |
+                             (__builtin_isinf(inv.z) ? 4u : 0u);
+    const uint32_t inf4 = infmask * 0x249u;  // replicated for the four children
+    float te, ta;
+    slab(xyz(S.root_min), xyz(S.root_max), ro, inv, te, ta);
+    if (COUNT) cnt.nodes++;
+    if (!(te <= ta && ta > 0.0f)) return hit;
+    uint32_t sp = 0u;
+    uint32_t cur = __float_as_uint(S.root_min.w);
+    for (;;) {
+        if (ref_is_leaf(cur)) {
+            const uint32_t first = cur & 0x07FFFFFFu;
+            const uint32_t count = ((cur >> 27) & 15u) + 1u;
+            for (uint32_t i = 0u; i < count; i++) {
+                const uint32_t slot = first + i;
+                const float4 q0 = S.slot_tris[3 * slot], q1 = S.slot_tris[3 * slot + 1], q2 = S.slot_tris[3 * slot + 2];
+                if (COUNT) cnt.tris++;
+                const float t = mt_slot(ro, rd, xyz(q0), xyz(q1), xyz(q2));
+                if (t < hit.t) {
+                    hit.t = t;
+                    hit.prim = __float_as_uint(q0.w);
+                }
+            }
+            if (EARLY_EXIT && hit.t < t_max) return hit;
+        } else {
+            const float4 *w = S.w4nodes + 8u * cur;
+            const float4 mnx = w[0], mny = w[1], mnz = w[2], mxx = w[3], mxy = w[4], mxz = w[5];
+            const uint4 refs = *reinterpret_cast<const uint4 *>(&w[6]);
+            const uint32_t kill = inf4 != 0u ? __float_as_uint(w[7].x) & inf4 : 0u;
+            const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmny[4] = {mny.x, mny.y, mny.z, mny.w};
+            const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
+            const float bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+            const uint32_t r[4] = {refs.x, refs.y, refs.z, refs.w};
+            float tes[4];
+            uint32_t pass = 0u;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float e, a;
+                slab(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), ro, inv, e, a);
+                tes[k] = e;
+                bool ok = r[k] != 0xFFFFFFFFu && e <= a && a > 0.0f && e < hit.t;
+                const uint32_t kb = (kill >> (3 * k)) & 7u;
+                if (kb != 0u) {
+                    if ((kb & 1u) && ro.x == bmnx[k]) ok = false;
+                    if ((kb & 2u) && ro.y == bmny[k]) ok = false;
+                    if ((kb & 4u) && ro.z == bmnz[k]) ok = false;
+                }
+                if (ok) pass |= 1u << k;
+            }
+            if (COUNT) cnt.nodes += 4;
+            if (pass != 0u) {
+                // push passing children after the first, rightmost first
+#pragma unroll
+                for (int k = 3; k >= 1; k--) {
+                    if (((pass >> k) & 1u) && (pass & ((1u << k) - 1u))) {
+                        stk.put(sp, r[k]);
+                        stk.put(sp + 1u, __float_as_uint(tes[k]));
+                        sp += 2u;
+                    }
+                }
+                const int f = __builtin_ctz(pass);
+                cur = f == 0 ? r[0] : f == 1 ? r[1] : f == 2 ? r[2] : r[3];
+                continue;
+            }
+        }
+        bool found = false;
+        while (sp > 0u) {
+            sp -= 2u;
+            const float et = __uint_as_float(stk.get(sp + 1u));
+            if (et < hit.t) {
+                cur = stk.get(sp);
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return hit;
+}
+
 // Traversal modes: 0 = the reference's BVH2 nodes, 1 = child-box records in the
-// reference's order (default), 2 = child-box records nearest child first.
+// reference's order, 2 = child-box records nearest child first, 3 = 4-wide
+// records in the reference's order (default).
 template <int MODE, bool EARLY_EXIT, bool COUNT>
 __device__ __forceinline__ Hit trace_mode(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk,
                                           TraceCount &cnt) {
     if (MODE == 0) return trace_reference<EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
+    if (MODE == 3) return trace_wide4<EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
     return trace_wide<MODE == 2, EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
 }
 

@@ -8,12 +8,16 @@
 
 namespace rtk {
 
+// Per-lane traversal stack capacity in 32-bit words (LDS rows + private spill).
+constexpr uint32_t kTraversalStackWords = 96;
+
 // Traversal inputs shared by every tracing kernel (device pointers).
 struct TraceInputs {
     const rt_bvh_node *nodes;
     const uint32_t *prims;
     const void *slot_tris;  // float4[3 * n_slots]
     const void *wnodes;     // float4[4 * n_internal]
+    const void *w4nodes;    // float4[8 * n_records] (4-wide)
     uint32_t n_nodes;
     float root_min[4];      // w: root ref bits
     float root_max[4];
@@ -36,7 +40,7 @@ struct GatherLaunch {
     void *moments;  // float4 per pixel
     void *gbuffer;  // float4 per pixel
     uint32_t max_depth;
-    uint32_t mode;           // 0 reference nodes, 1 child-box reference order, 2 child-box nearest first
+    uint32_t mode;           // 0 reference nodes, 1 child-box reference order, 2 child-box nearest first, 3 4-wide
     uint32_t early_shadow;   // shadow rays stop at the first hit nearer than the light
     unsigned long long *counters;  // non-null: count traffic (4 counters)
 };

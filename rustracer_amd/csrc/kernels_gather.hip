@@ -552,6 +552,7 @@ TraceScene make_trace_scene(const rtk::TraceInputs &t) {
     s.prims = t.prims;
     s.slot_tris = reinterpret_cast<const float4 *>(t.slot_tris);
     s.wnodes = reinterpret_cast<const float4 *>(t.wnodes);
+    s.w4nodes = reinterpret_cast<const float4 *>(t.w4nodes);
     s.n_nodes = t.n_nodes;
     s.pad = 0;
     s.root_min = make_float4(t.root_min[0], t.root_min[1], t.root_min[2], t.root_min[3]);
@@ -602,7 +603,8 @@ hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
     }
     if (g.mode == 0) { RT_GATHER_M(0) }
     else if (g.mode == 1) { RT_GATHER_M(1) }
-    else { RT_GATHER_M(2) }
+    else if (g.mode == 2) { RT_GATHER_M(2) }
+    else { RT_GATHER_M(3) }
 #undef RT_GATHER_M
 #undef RT_GATHER
     return hipGetLastError();
@@ -626,7 +628,8 @@ hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipS
 #define RT_BATCH(M, E) hipLaunchKernelGGL((rtd::gather_batch_kernel<M, E>), grid, block, 0, s, B)
     if (g.mode == 0) { if (g.early_shadow) RT_BATCH(0, true); else RT_BATCH(0, false); }
     else if (g.mode == 1) { if (g.early_shadow) RT_BATCH(1, true); else RT_BATCH(1, false); }
-    else { if (g.early_shadow) RT_BATCH(2, true); else RT_BATCH(2, false); }
+    else if (g.mode == 2) { if (g.early_shadow) RT_BATCH(2, true); else RT_BATCH(2, false); }
+    else { if (g.early_shadow) RT_BATCH(3, true); else RT_BATCH(3, false); }
 #undef RT_BATCH
     return hipGetLastError();
 }
@@ -638,7 +641,8 @@ hipError_t launch_trace_rays(const TraceInputs &t, uint32_t mode, const rt_ray *
     const dim3 grid((n + 63u) / 64u), block(64);
     if (mode == 0) hipLaunchKernelGGL((rtd::trace_rays_kernel<0>), grid, block, 0, s, ts, rays, n, out);
     else if (mode == 1) hipLaunchKernelGGL((rtd::trace_rays_kernel<1>), grid, block, 0, s, ts, rays, n, out);
-    else hipLaunchKernelGGL((rtd::trace_rays_kernel<2>), grid, block, 0, s, ts, rays, n, out);
+    else if (mode == 2) hipLaunchKernelGGL((rtd::trace_rays_kernel<2>), grid, block, 0, s, ts, rays, n, out);
+    else hipLaunchKernelGGL((rtd::trace_rays_kernel<3>), grid, block, 0, s, ts, rays, n, out);
     return hipGetLastError();
 }
 

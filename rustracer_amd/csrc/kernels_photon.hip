@@ -383,7 +383,8 @@ hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s) {
     const dim3 grid((p.n_photons + 63u) / 64u), block(64);
     if (p.mode == 0) hipLaunchKernelGGL((rtd::photon_trace_kernel<0>), grid, block, 0, s, A);
     else if (p.mode == 1) hipLaunchKernelGGL((rtd::photon_trace_kernel<1>), grid, block, 0, s, A);
-    else hipLaunchKernelGGL((rtd::photon_trace_kernel<2>), grid, block, 0, s, A);
+    else if (p.mode == 2) hipLaunchKernelGGL((rtd::photon_trace_kernel<2>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((rtd::photon_trace_kernel<3>), grid, block, 0, s, A);
     return hipGetLastError();
 }
 

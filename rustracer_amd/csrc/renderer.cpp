@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -31,6 +32,8 @@ T *dev_alloc(size_t count, const char *what) {
     return static_cast<T *>(p);
 }
 
+constexpr uint32_t kTraversalStackWords = rtk::kTraversalStackWords;
+
 struct PendingTiming {
     int pass;
     hipEvent_t start, stop;
@@ -51,6 +54,9 @@ struct rt_renderer {
     rt_light *d_lights = nullptr;
     float *d_slot_tris = nullptr;  // float4[3 * n_prims]
     float *d_wnodes = nullptr;     // float4[4 * n_internal]
+    float *d_w4nodes = nullptr;    // float4[8 * n_rec4]
+    uint32_t n_rec4 = 0;
+    bool wide4_ok = false;
     float root_min[4] = {0, 0, 0, 0}, root_max[4] = {0, 0, 0, 0};
     bool wide_ok = false;
     uint32_t bvh_depth = 0;
@@ -107,7 +113,7 @@ struct rt_renderer {
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
-                        d_views, d_tiles, d_counter};
+                        d_views, d_tiles, d_counter, d_w4nodes};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -146,14 +152,18 @@ struct rt_renderer {
         t.prims = d_prims;
         t.slot_tris = d_slot_tris;
         t.wnodes = d_wnodes;
+        t.w4nodes = d_w4nodes;
         t.n_nodes = n_nodes;
         std::memcpy(t.root_min, root_min, sizeof(root_min));
         std::memcpy(t.root_max, root_max, sizeof(root_max));
         return t;
     }
+    // 0 reference BVH2 nodes, 1 binary child-box records, 2 binary nearest first, 3 4-wide records
     uint32_t traversal_mode() const {
         if (!wide_ok || (cfg.flags & RT_FLAG_REFERENCE_TRAVERSAL)) return 0u;
-        return (cfg.flags & RT_FLAG_NEAREST_FIRST) ? 2u : 1u;
+        if (cfg.flags & RT_FLAG_NEAREST_FIRST) return 2u;
+        if ((cfg.flags & RT_FLAG_BINARY_RECORDS) || !wide4_ok) return 1u;
+        return 3u;
     }
 
     rt_image_params image_params(float alpha) const {
@@ -405,6 +415,7 @@ static void build_trace_layouts(rt_renderer &r, const rt_triangle *tris, const r
 
     // validate the BVH and measure its depth
     r.wide_ok = r.n_nodes > 0;
+    bool nested = true;
     std::vector<uint32_t> compact(r.n_nodes, 0xFFFFFFFFu);
     std::vector<uint32_t> order;
     if (r.n_nodes > 0) {
@@ -422,6 +433,13 @@ static void build_trace_layouts(rt_renderer &r, const rt_triangle *tris, const r
                     throw ApiError(RT_ERR_INVALID, "BVH leaf range out of range");
                 if (n.prim_count > 16 || n.left_first >= (1u << 27)) r.wide_ok = false;
             } else {
+                // the 4-wide records skip intermediate boxes: they need children inside parents
+                for (uint32_t c : {n.left_first, n.right_child}) {
+                    if (c >= r.n_nodes) continue;  // reported when visited
+                    for (int a = 0; a < 3; a++)
+                        if (!(nodes[c].bbox_min[a] >= n.bbox_min[a] && nodes[c].bbox_max[a] <= n.bbox_max[a]))
+                            nested = false;
+                }
                 compact[i] = (uint32_t)order.size();
                 order.push_back(i);
                 st.push_back({n.right_child, d + 1});
@@ -456,6 +474,107 @@ static void build_trace_layouts(rt_renderer &r, const rt_triangle *tris, const r
     }
     r.d_wnodes = dev_alloc<float>(w.size(), "wide nodes");
     hip_check(hipMemcpy(r.d_wnodes, w.data(), w.size() * 4, hipMemcpyHostToDevice), "upload wide nodes");
+    // Stack words the traversals can need: one deferred (ref, te) pair per level
+    // for the binary records, (children - 1) pairs per level for the 4-wide ones.
+    if (2u * (r.bvh_depth > 0 ? r.bvh_depth - 1 : 0) > kTraversalStackWords) r.wide_ok = false;
+
+    // 4-wide records: each internal node's children are expanded (largest box
+    // first) into up to 4 descendants, keeping their left-to-right order, so a
+    // left-to-right depth-first walk meets the leaves in the reference's order.
+    // Record: min x/y/z, max x/y/z of the 4 children (SoA, 6 float4), refs (uint4),
+    // kill bits, 12 bytes of padding; absent children have ref 0xFFFFFFFF.
+    r.wide4_ok = r.wide_ok && nested;
+    std::vector<float> w4;
+    if (r.wide4_ok) {
+        auto area = [&](uint32_t i) {
+            const rt_bvh_node &n = nodes[i];
+            const float ex = n.bbox_max[0] - n.bbox_min[0], ey = n.bbox_max[1] - n.bbox_min[1],
+                        ez = n.bbox_max[2] - n.bbox_min[2];
+            return ex * ey + ey * ez + ez * ex;
+        };
+        // records in creation order (parents before children); kids: BVH2 node indices
+        std::vector<uint32_t> src;                   // BVH2 node of each record
+        std::vector<std::vector<uint32_t>> kids_of;  // BVH2 children of each record
+        std::vector<std::array<uint32_t, 4>> refs_of;
+        std::vector<uint32_t> kill_of;               // 3 kill bits per child
+        if (nodes[0].prim_count == 0) src.push_back(0u);
+        for (size_t rec = 0; rec < src.size(); rec++) {
+            std::vector<uint32_t> kids{nodes[src[rec]].left_first, nodes[src[rec]].right_child};
+            std::vector<std::vector<uint32_t>> skipped(2);  // untested BVH2 nodes above each child
+            while (kids.size() < 4) {
+                int best = -1;
+                float best_a = -1.0f;
+                for (size_t k = 0; k < kids.size(); k++)
+                    if (nodes[kids[k]].prim_count == 0 && area(kids[k]) > best_a) {
+                        best_a = area(kids[k]);
+                        best = (int)k;
+                    }
+                if (best < 0) break;
+                const uint32_t c = kids[best];
+                std::vector<uint32_t> sk = skipped[best];
+                sk.push_back(c);
+                kids[best] = nodes[c].left_first;
+                kids.insert(kids.begin() + best + 1, nodes[c].right_child);
+                skipped[best] = sk;
+                skipped.insert(skipped.begin() + best + 1, sk);
+            }
+            // kill bit (child k, axis a): the child is flat on a at a face of a
+            // skipped node that is not flat on a
+            uint32_t kill = 0;
+            for (size_t k = 0; k < kids.size(); k++) {
+                const rt_bvh_node &c = nodes[kids[k]];
+                for (int a = 0; a < 3; a++) {
+                    if (c.bbox_min[a] != c.bbox_max[a]) continue;
+                    for (uint32_t m : skipped[k]) {
+                        const rt_bvh_node &M = nodes[m];
+                        if (M.bbox_min[a] != M.bbox_max[a] &&
+                            (c.bbox_min[a] == M.bbox_min[a] || c.bbox_min[a] == M.bbox_max[a]))
+                            kill |= 1u << (3 * k + a);
+                    }
+                }
+            }
+            kill_of.push_back(kill);
+            std::array<uint32_t, 4> refs{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+            for (size_t k = 0; k < kids.size(); k++) {
+                const rt_bvh_node &c = nodes[kids[k]];
+                if (c.prim_count > 0) {
+                    refs[k] = 0x80000000u | ((c.prim_count - 1u) << 27) | c.left_first;
+                } else {
+                    refs[k] = (uint32_t)src.size();
+                    src.push_back(kids[k]);
+                }
+            }
+            kids_of.push_back(kids);
+            refs_of.push_back(refs);
+        }
+        const uint32_t n_rec = (uint32_t)src.size();
+        // worst-case deferred pairs from each record down (children come after parents)
+        std::vector<uint32_t> need(n_rec, 0);
+        for (size_t rec = n_rec; rec-- > 0;) {
+            uint32_t deeper = 0;
+            for (uint32_t ref : refs_of[rec])
+                if (ref != 0xFFFFFFFFu && !(ref & 0x80000000u)) deeper = std::max(deeper, need[ref]);
+            need[rec] = (uint32_t)kids_of[rec].size() - 1u + deeper;
+        }
+        if (n_rec > 0 && 2u * need[0] > kTraversalStackWords) r.wide4_ok = false;
+        w4.assign((size_t)std::max(n_rec, 1u) * 32, 0.0f);
+        for (uint32_t rec = 0; rec < n_rec && r.wide4_ok; rec++) {
+            float *o = &w4[(size_t)rec * 32];
+            for (size_t k = 0; k < kids_of[rec].size(); k++) {
+                const rt_bvh_node &c = nodes[kids_of[rec][k]];
+                for (int a = 0; a < 3; a++) {
+                    o[a * 4 + k] = c.bbox_min[a];
+                    o[12 + a * 4 + k] = c.bbox_max[a];
+                }
+            }
+            std::memcpy(&o[24], refs_of[rec].data(), 16);
+            std::memcpy(&o[28], &kill_of[rec], 4);
+        }
+        r.n_rec4 = n_rec;
+    }
+    if (w4.empty()) w4.assign(32, 0.0f);
+    r.d_w4nodes = dev_alloc<float>(w4.size(), "4-wide nodes");
+    hip_check(hipMemcpy(r.d_w4nodes, w4.data(), w4.size() * 4, hipMemcpyHostToDevice), "upload 4-wide nodes");
 }
 
 static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *tris, uint32_t n_tris,
@@ -775,6 +894,15 @@ int rt_read_grid_meta(rt_renderer *r, uint32_t *meta, size_t n, uint32_t *n_cell
         if (!meta) return;
         if (n < 2ull * r->meta_cells) throw ApiError(RT_ERR_INVALID, "capacity below 2*cells");
         read_dev(r, meta, r->d_meta, 8ull * r->meta_cells);
+    });
+}
+
+int rt_traversal_info(const rt_renderer *r, uint32_t *mode, uint32_t *bvh_depth, uint32_t *n_records4) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        if (mode) *mode = r->traversal_mode();
+        if (bvh_depth) *bvh_depth = r->bvh_depth;
+        if (n_records4) *n_records4 = r->wide4_ok ? r->n_rec4 : 0u;
     });
 }
 

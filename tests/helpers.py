@@ -94,6 +94,34 @@ def random_rays(bmin, bmax, n: int, seed: int) -> np.ndarray:
     return rays
 
 
+def face_rays(nodes: np.ndarray, n: int, seed: int) -> np.ndarray:
+    """Rays that lie in the plane of a BVH box face: the origin has one
+    coordinate exactly on a node's min or max, and the direction is exactly zero
+    on that axis (sometimes on a second axis too), so 1/rd is infinite there and
+    the slab test meets 0 * inf. Flat boxes (axis-aligned quads) make these the
+    rays whose box decisions depend on the NaN rule of min/max."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, len(nodes), n)
+    lo = nodes["bbox_min"][k].astype(np.float32)
+    hi = nodes["bbox_max"][k].astype(np.float32)
+    a = rng.integers(0, 3, n)
+    t = rng.random((n, 3)).astype(np.float32)
+    o = lo + (hi - lo) * t
+    on_max = rng.random(n) < 0.5
+    o[np.arange(n), a] = np.where(on_max, hi[np.arange(n), a], lo[np.arange(n), a])
+    d = rng.normal(size=(n, 3))
+    d[np.arange(n), a] = 0.0
+    b = (a + rng.integers(1, 3, n)) % 3
+    two = rng.random(n) < 0.3
+    d[np.arange(n)[two], b[two]] = 0.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, _ffi.RAY)
+    rays["origin"] = o
+    rays["dir"] = d.astype(np.float32)
+    assert (rays["dir"][np.arange(n), a] == 0).all()
+    return rays
+
+
 def first_diff(a: np.ndarray, b: np.ndarray) -> str:
     bad = np.argwhere(~((a == b) | (np.isnan(a) & np.isnan(b))))
     if len(bad) == 0:

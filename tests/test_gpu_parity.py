@@ -12,12 +12,14 @@ import rustracer_amd as rt
 from rustracer_amd import _ffi
 
 from . import oracle_ffi as orc
-from .helpers import camera_rays, first_diff, image_params, oracle_frames, random_rays
+from .helpers import camera_rays, face_rays, first_diff, image_params, oracle_frames, random_rays
 
 pytestmark = pytest.mark.gpu
 
 FLAG_VARIANTS = [0, _ffi.FLAG_REFERENCE_TRAVERSAL, _ffi.FLAG_CLOSEST_SHADOW,
-                 _ffi.FLAG_REFERENCE_TRAVERSAL | _ffi.FLAG_CLOSEST_SHADOW]
+                 _ffi.FLAG_REFERENCE_TRAVERSAL | _ffi.FLAG_CLOSEST_SHADOW, _ffi.FLAG_BINARY_RECORDS]
+# traversal each flag set selects: 3 = 4-wide records, 1 = binary records, 0 = BVH2 nodes
+EXPECTED_MODE = {0: 3, _ffi.FLAG_BINARY_RECORDS: 1, _ffi.FLAG_REFERENCE_TRAVERSAL: 0}
 
 
 def _renderer(scene, **cfg):
@@ -30,8 +32,8 @@ def _check_equal(name, a, b):
     assert eq.all(), f"{name}: {first_diff(a, b)}"
 
 
-@pytest.mark.parametrize("scene_name", ["cornell_box", "photon_plain"])
-@pytest.mark.parametrize("flags", [0, _ffi.FLAG_REFERENCE_TRAVERSAL])
+@pytest.mark.parametrize("scene_name", ["cornell_box", "photon_plain", "multiview"])
+@pytest.mark.parametrize("flags", [0, _ffi.FLAG_BINARY_RECORDS, _ffi.FLAG_REFERENCE_TRAVERSAL])
 def test_trace_closest_bit_exact(scene_name, flags):
     scene = rt.Scene.builtin(scene_name, 3)
     o = orc.OracleScene.from_scene(scene)
@@ -40,6 +42,7 @@ def test_trace_closest_bit_exact(scene_name, flags):
                            random_rays(lo, hi, 20000, 7)])
     print(f"trace {scene_name} flags={flags} rays={len(rays)}", flush=True)
     with _renderer(scene, width=8, height=8, flags=flags) as r:
+        assert r.traversal_info()["mode"] == EXPECTED_MODE[flags]
         got = r.trace_closest(rays)
     ref = o.trace(rays)
     brute = o.brute(rays)
@@ -51,6 +54,26 @@ def test_trace_closest_bit_exact(scene_name, flags):
     assert (ref["t"] >= brute["t"]).all()
     assert (ref["t"] == brute["t"]).mean() > 0.999
     assert (ref["t"] < 1e38).sum() > len(rays) // 10
+
+
+@pytest.mark.parametrize("scene_name", ["cornell_box", "cornell36", "photon_plain"])
+@pytest.mark.parametrize("flags", [0, _ffi.FLAG_BINARY_RECORDS, _ffi.FLAG_REFERENCE_TRAVERSAL])
+def test_trace_face_plane_rays_bit_exact(scene_name, flags):
+    """Rays lying in box-face planes with zero direction components: the slab
+    test sees 0 * inf = NaN, flat boxes (axis-aligned walls) ignore the axis and
+    non-flat boxes with a face in the plane fail. The 4-wide walk skips
+    intermediate boxes and must still take every decision the reference takes."""
+    scene = rt.Scene.builtin(scene_name, 3)
+    nodes, _ = scene.build_bvh()
+    rays = face_rays(nodes, 30000, 21)
+    print(f"face rays {scene_name} flags={flags} rays={len(rays)}", flush=True)
+    with _renderer(scene, width=8, height=8, flags=flags) as r:
+        assert r.traversal_info()["mode"] == EXPECTED_MODE[flags]
+        got = r.trace_closest(rays)
+    ref = orc.OracleScene.from_scene(scene).trace(rays)
+    _check_equal("t", got["t"], ref["t"])
+    assert np.array_equal(got["prim_id"], ref["prim_id"]), first_diff(got["prim_id"], ref["prim_id"])
+    assert (ref["t"] < 1e38).sum() > len(rays) // 20
 
 
 @pytest.mark.parametrize("scene_name", ["cornell_box", "photon_plain"])
@@ -215,3 +238,52 @@ def test_fullframe_1080p_properties():
     n = np.linalg.norm(g[..., :3][hit], axis=-1)
     assert np.abs(n - 1).max() < 1e-5
     assert (out[..., 3] == 255).all()
+
+
+def _view_params(cfg, w, h, spp, frame):
+    ip = image_params(cfg, frame)
+    ip.width, ip.height, ip.spp = w, h, spp
+    return ip
+
+
+@pytest.mark.parametrize("tuning", [None, (16, 16, 4, 4), (16, 4, 12, 2)])
+def test_batch_views_bit_exact(tuning):
+    """Many views of uneven resolution and spp in one persistent launch: every
+    view equals the oracle's gather for that camera, resolution, spp and seed."""
+    scene = rt.Scene.builtin("multiview", 2)
+    views = [(rt.Scene.builtin_camera("multiview", 2, k), w, h, spp, 3 + k)
+             for k, (w, h, spp) in enumerate([(40, 24, 1), (33, 17, 2), (64, 8, 4), (9, 70, 8), (24, 24, 16)])]
+    with _renderer(scene, width=8, height=8, max_depth=8, enable_denoise=False) as r:
+        r.set_views(views)
+        if tuning is None:
+            r.render_batch()
+        else:
+            r.render_batch(*tuning)
+        got = [r.read_batch_view(k) for k in range(len(views))]
+        cfg = r.config
+    o = orc.OracleScene.from_scene(scene)
+    for k, (cam, w, h, spp, frame) in enumerate(views):
+        print(f"view {k}: {w}x{h} spp={spp} frame={frame}", flush=True)
+        ref = o.gather(rt.camera_to_uniform(cam, w, h), _view_params(cfg, w, h, spp, frame), max_depth=8)
+        for name, a, b in zip(("frame", "moments", "gbuffer"), got[k], ref[:3]):
+            _check_equal(f"view {k} {name}", a, b)
+
+
+def test_batch_autotune_and_totals():
+    from rustracer_amd.workloads import multiview_views
+
+    scene = rt.Scene.builtin("multiview", 1)
+    views = multiview_views(12, seed=1, scale=0.25)
+    with _renderer(scene, width=8, height=8, max_depth=8, enable_denoise=False) as r:
+        r.set_views(views)
+        px, smp = r.batch_totals()
+        assert px == sum(w * h for _, w, h, _, _ in views)
+        assert smp == sum(w * h * spp for _, w, h, spp, _ in views)
+        r.render_batch()
+        first = [r.read_batch_view(k)[0] for k in range(len(views))]
+        best, ms = r.batch_autotune(repeats=1)
+        print(f"autotune best {best} {ms:.3f} ms", flush=True)
+        assert best["tile_w"] * best["tile_h"] % 64 == 0 and ms > 0
+        r.render_batch()  # with the tuned parameters: same pixels
+        for k in range(len(views)):
+            assert np.array_equal(r.read_batch_view(k)[0], first[k])
