@@ -37,6 +37,11 @@ TRI_BYTES = 48         # slot triangle: v0, v1-v0, v2-v0 (3 x float4)
 RAY_BYTES = 32         # 24 B ray + 8 B hit
 PHOTON_BYTES = 16       # position read per radius-gather candidate (power/incident only on accept)
 PIXEL_OUT_BYTES = 48   # frame + moments + gbuffer float4 writes
+DENOISE_PX_BYTES = 64  # per pass: color in + gbuffer + moments + out, 16 B each
+COMPOSITE_TONEMAP_PX_BYTES = 52  # frame 16 + accumulation 16 read + 16 written + rgba8 4
+GRID_SLOT_BYTES = 32   # count (position + power.r, 16) + scatter key read (16) per slot
+GRID_STORED_BYTES = 128  # stored photon: 64 B record read + 64 B written
+GRID_CELL_BYTES = 8    # [start, count] per cell
 
 
 TRAVERSALS = {0: "reference BVH2 nodes", 1: "binary child-box records, reference order",
@@ -163,6 +168,28 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": int(alg_bytes)},
         "cpu_baseline": None,
     }
+
+    # streaming passes: algorithmic bytes per launch / mean launch time
+    gp = r.grid_params()
+    pass_bytes = {"denoise": w * h * DENOISE_PX_BYTES,
+                  "composite_tonemap": w * h * COMPOSITE_TONEMAP_PX_BYTES}
+    result["pass_rooflines"] = {}
+    for name, nbytes in pass_bytes.items():
+        ms_, n_ = stats.get(name, (0.0, 0))
+        if n_:
+            gbs = nbytes / (ms_ / n_ * 1e-3) / 1e9
+            result["pass_rooflines"][name] = {"bound": "hbm", "achieved": round(gbs, 1), "unit": "GB/s",
+                                              "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                              "bytes_per_launch": int(nbytes)}
+    if photons_on and photon_stats.get("grid_build", (0, 0))[1]:
+        srt = r.read_sorted_photons()
+        n_slots = min(max(args.photons * 6 // 256, 1) * 256, gp.photon_slots)
+        nbytes = n_slots * GRID_SLOT_BYTES + len(srt) * GRID_STORED_BYTES + gp.num_cells * GRID_CELL_BYTES
+        ms_, n_ = photon_stats["grid_build"]
+        gbs = nbytes / (ms_ / n_ * 1e-3) / 1e9
+        result["pass_rooflines"]["grid_build"] = {"bound": "hbm", "achieved": round(gbs, 1), "unit": "GB/s",
+                                                  "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": int(nbytes),
+                                                  "slots": n_slots, "stored": len(srt)}
 
     copy_gbs, copy_ms = rt.hbm_copy_probe(0, 1 << 30, 20)
     result["roofline"]["measured_copy_gbs"] = round(copy_gbs, 1)

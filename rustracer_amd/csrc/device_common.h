@@ -201,20 +201,29 @@ __device__ __forceinline__ float mt_slot(F3 ro, F3 rd, F3 v0, F3 edge1, F3 edge2
 }
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS
-// (lane-interleaved rows, one bank per lane), deeper entries in private memory.
+// (lane-interleaved rows, one bank per lane), deeper entries in a private
+// array owned by the kernel (scratch memory). The stack holder itself is a
+// pair of registers; the spill array is only touched past kLdsStack entries.
 constexpr int kLdsStack = 32;
 constexpr int kStackWords = 96;
+constexpr int kSpillWords = kStackWords - kLdsStack;
+constexpr int kSpillAlloc = kSpillWords > 0 ? kSpillWords : 1;
+
+// Address-space-qualified pointers, so stack accesses compile to ds_* (LDS)
+// and scratch_* (private) instructions instead of flat ones.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(5))) uint32_t priv_u32;
 
 struct LaneStack {
-    uint32_t *lds;  // &shared[row 0][lane]; row stride = blockDim
+    lds_u32 *lds;     // &shared[row 0][lane]; row stride = blockDim
     uint32_t stride;
-    uint32_t spill[kStackWords - kLdsStack];
+    priv_u32 *spill;  // kSpillAlloc private words
     __device__ __forceinline__ void put(uint32_t sp, uint32_t v) {
-        if (sp < (uint32_t)kLdsStack) lds[sp * stride] = v;
+        if (kSpillWords == 0 || sp < (uint32_t)kLdsStack) lds[sp * stride] = v;
         else spill[sp - kLdsStack] = v;
     }
     __device__ __forceinline__ uint32_t get(uint32_t sp) const {
-        return sp < (uint32_t)kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+        return (kSpillWords == 0 || sp < (uint32_t)kLdsStack) ? lds[sp * stride] : spill[sp - kLdsStack];
     }
 };
 

@@ -163,20 +163,28 @@ __device__ F3 sample_spec_dir(F3 rd, F3 n, float a, Rng &rng) {
 
 __device__ __forceinline__ float luma_of(F3 c) { return dot(c, f3(0.2126f, 0.7152f, 0.0722f)); }
 
+// Per-pixel accumulators of gather.wgsl:36-42. The per-sample luma sums
+// (sample_lum[s] += l) build up in `cur` while sample s runs (the same additions
+// in the same order from 0) and are shifted into sl[] when the sample ends, so
+// after n samples they sit in order at sl[8-n .. 7]. Only static indices are
+// used: the array stays in registers.
 struct Acc {
     F3 col;
     float lum_sum, lum2_sum;
+    float cur;
     float sl[8];
-    // gather.wgsl:36-42
-    __device__ __forceinline__ void add(uint32_t s, F3 c) {
+    __device__ __forceinline__ void add(F3 c) {
         col = col + c;
         const float l = luma_of(c);
         lum_sum += l;
         lum2_sum += l * l;
-        // sl[s] += l with static indices, so the array stays in registers
+        cur += l;
+    }
+    __device__ __forceinline__ void end_sample() {
 #pragma unroll
-        for (uint32_t k = 0; k < 8u; k++)
-            if (s == k) sl[k] += l;
+        for (int k = 0; k < 7; k++) sl[k] = sl[k + 1];
+        sl[7] = cur;
+        cur = 0.0f;
     }
 };
 
@@ -201,6 +209,7 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
     acc.col = fs(0.0f);
     acc.lum_sum = 0.0f;
     acc.lum2_sum = 0.0f;
+    acc.cur = 0.0f;
 #pragma unroll
     for (int i = 0; i < 8; i++) acc.sl[i] = 0.0f;
     F3 first_normal = f3(0.0f, 1.0f, 0.0f);
@@ -234,7 +243,7 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
         for (uint32_t b = 0u; b < A.max_depth; b++) {
             const Hit hit = tr.closest(ro, rd);
             if (hit.t >= kInf) {
-                if (b > 0u) acc.add(s, thr * sky_color(rd, env_color, env_horizon, env_intensity));
+                if (b > 0u) acc.add(thr * sky_color(rd, env_color, env_horizon, env_intensity));
                 break;
             }
             const rt_triangle &trg = A.tris[hit.prim];
@@ -255,7 +264,7 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
             const float rough = mat.roughness;
             const float metal = mat.metallic;
             const F3 emissive = ld3(mat.emissive);
-            acc.add(s, thr * emissive);
+            acc.add(thr * emissive);
 
             if (debug_view != 0u) {
                 bool stop = true;
@@ -306,11 +315,11 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
             const LightSample ls = direct_light(A, tr, hp, shn, rng);
             if (!do_spec) {
                 if (env_intensity > 0.0f)
-                    acc.add(s, thr * dscale * sky_color(shn, env_color, env_horizon, env_intensity));
-                acc.add(s, thr * dscale * ls.e / kPi);
-                acc.add(s, thr * albedo * ip.ambient);
+                    acc.add(thr * dscale * sky_color(shn, env_color, env_horizon, env_intensity));
+                acc.add(thr * dscale * ls.e / kPi);
+                acc.add(thr * albedo * ip.ambient);
             } else {
-                acc.add(s, thr * spec_ggx(wo, shn, ls.wi, a, f0) * ls.e / spec_prob);
+                acc.add(thr * spec_ggx(wo, shn, ls.wi, a, f0) * ls.e / spec_prob);
             }
 
             if (ip.use_photons == 1u && !do_spec) {
@@ -359,7 +368,7 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
                     }
                 }
                 if (ph_hits > 0u)
-                    acc.add(s, thr * dscale * ph_sum * ip.photon_scale /
+                    acc.add(thr * dscale * ph_sum * ip.photon_scale /
                                    (kPi * (float)ph_hits * ip.photon_radius * ip.photon_radius));
                 if (ip.photon_debug == 1u) {
                     const float density = wmin((float)ph_hits / 64.0f, 1.0f);
@@ -384,43 +393,50 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
             if (p <= 0.0f || rng_f32(rng) > p) break;
             thr = thr / p;
         }
+        acc.end_sample();
     }
 
-    // gather.wgsl:548-579 firefly sigma clamp (spp <= 8): insertion sort in registers
+    // gather.wgsl:548-579 firefly sigma clamp (spp <= 8). The samples' lumas are
+    // at sl[w0 .. 7] (w0 = 8 - spp) in sample order; the insertion sort, median,
+    // spread and clamped sums run over that window with static indices.
     if (spp <= 8u) {
         float *sl = acc.sl;
+        const uint32_t w0 = 8u - spp;
 #pragma unroll
         for (uint32_t i = 1u; i < 8u; i++) {
-            if (i < spp) {
+            if (i > w0) {
                 const float key = sl[i];
-                // shift larger predecessors up (static indices keep sl in registers)
+                bool moving = true;
 #pragma unroll
-                for (int j = (int)i; j > 0; j--) {
-                    const bool move = sl[j - 1] > key;
-                    // once an element is not larger, every earlier one is not larger either
-                    sl[j] = move ? sl[j - 1] : sl[j];
-                    if (!move) {
-                        sl[j] = key;
-                        break;
+                for (uint32_t j = i; j > 0u; j--) {
+                    if (moving) {
+                        if (j - 1u >= w0 && sl[j - 1u] > key) {
+                            sl[j] = sl[j - 1u];
+                            if (j - 1u == w0) {
+                                sl[j - 1u] = key;
+                                moving = false;
+                            }
+                        } else {
+                            sl[j] = key;
+                            moving = false;
+                        }
                     }
-                    if (j == 1) sl[0] = key;
                 }
             }
         }
-        float median = 0.0f;
-        const uint32_t lo = (spp - 1u) / 2u, hi = spp / 2u;
+        const uint32_t lo = w0 + (spp - 1u) / 2u, hi = w0 + spp / 2u;
         float slo = 0.0f, shi = 0.0f;
 #pragma unroll
         for (uint32_t k = 0; k < 8u; k++) {
             if (k == lo) slo = sl[k];
             if (k == hi) shi = sl[k];
         }
-        median = (slo + shi) * 0.5f;
+        const float median = (slo + shi) * 0.5f;
         const uint32_t half = max(spp / 2u, 1u);
         float spread = 0.0f;
 #pragma unroll
         for (uint32_t k = 0; k < 8u; k++) {
-            if (k < half) {
+            if (k >= w0 && k < w0 + half) {
                 const float d = sl[k] - median;
                 spread += d * d;
             }
@@ -430,7 +446,7 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
         float cl = 0.0f, cl2 = 0.0f;
 #pragma unroll
         for (uint32_t k = 0; k < 8u; k++) {
-            if (k < spp) {
+            if (k >= w0) {
                 const float lc = wmin(sl[k], thresh);
                 cl += lc;
                 cl2 += lc * lc;
@@ -458,8 +474,10 @@ __global__ __launch_bounds__(64) void gather_kernel(GatherArgs A) {
     if (tile >= A.n_tiles) return;
     const uint32_t px = (tile % A.tiles_x) * 8u + (threadIdx.x & 7u);
     const uint32_t py = (tile / A.tiles_x) * 8u + (threadIdx.x >> 3);
+    uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.lds = &stack_rows[threadIdx.x];
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
     stk.stride = 64u;
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
@@ -500,8 +518,10 @@ struct BatchArgs {
 template <int MODE, bool EARLY>
 __global__ __launch_bounds__(64) void gather_batch_kernel(BatchArgs B) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
+    uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.lds = &stack_rows[threadIdx.x];
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
     stk.stride = 64u;
     Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
@@ -535,8 +555,10 @@ __global__ __launch_bounds__(64) void trace_rays_kernel(TraceScene ts, const rt_
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     const uint32_t i = blockIdx.x * 64u + threadIdx.x;
     if (i >= n) return;
+    uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.lds = &stack_rows[threadIdx.x];
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
     stk.stride = 64u;
     TraceCount cnt{0u, 0u};
     const rt_ray r = rays[i];

@@ -46,8 +46,10 @@ __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     const uint32_t photon_index = blockIdx.x * 64u + threadIdx.x;
     if (photon_index >= A.n_photons) return;
+    uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.lds = &stack_rows[threadIdx.x];
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
     stk.stride = 64u;
     TraceCount cnt{0u, 0u};
     const uint32_t num_lights = A.n_lights;
