@@ -48,6 +48,9 @@ TRAVERSALS = {0: "reference BVH2 nodes", 1: "binary child-box records, reference
               2: "binary child-box records, nearest first", 3: "4-wide records, reference order"}
 
 
+PROFILE_FRAMES = 5     # frames timed one at a time for the per-pass kernel times
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -68,6 +71,8 @@ def main() -> int:
                     help="child-box traversal nearest child first (not bit-exact by construction)")
     ap.add_argument("--binary-records", action="store_true",
                     help="binary child-box records instead of the 4-wide records (both exact)")
+    ap.add_argument("--path-queues", action="store_true",
+                    help="camera pass as path-queue passes instead of one kernel per frame (same results)")
     ap.add_argument("--no-photons", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -86,9 +91,10 @@ def main() -> int:
     t_scene = time.perf_counter() - t0
     counts = scene.counts()
     log(f"scene fullframe: {counts['triangles']} triangles ({t_scene:.1f}s)")
-    flags = (_ffi.FLAG_PROFILE | (_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0) |
+    flags = ((_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0) |
              (_ffi.FLAG_NEAREST_FIRST if args.nearest_first else 0) |
-             (_ffi.FLAG_BINARY_RECORDS if args.binary_records else 0))
+             (_ffi.FLAG_BINARY_RECORDS if args.binary_records else 0) |
+             (_ffi.FLAG_PATH_QUEUES if args.path_queues else 0))
     photons_on = not args.no_photons
     cfg = rt.RenderConfig(width=args.width, height=args.height, spp=1, max_depth=args.depth,
                           enable_photon_map=photons_on, photon_count=args.photons, enable_denoise=True,
@@ -99,6 +105,14 @@ def main() -> int:
     tinfo = r.traversal_info()
     log(f"renderer created (flatten + BVH build + upload) in {t_create:.1f}s; traversal {tinfo}")
     r.update_camera(cam)
+    base_cfg = r.config
+
+    def set_profile(on: bool) -> None:
+        c = r.config
+        c.flags = base_cfg.flags | (_ffi.FLAG_PROFILE if on else 0)
+        r.config = c
+
+    set_profile(True)
     t0 = time.perf_counter()
     if photons_on:
         r.rebuild_photons()
@@ -106,23 +120,30 @@ def main() -> int:
     t_photons = time.perf_counter() - t0
     photon_stats = r.pass_stats()
     r.reset_pass_stats()
+    set_profile(False)
     log(f"photon map rebuilt in {t_photons*1e3:.1f} ms")
 
     for _ in range(args.warmup):
         r.record_frame()
     r.synchronize()
-    r.reset_pass_stats()
     log(f"warmup {args.warmup} frames done; timing {args.steps} frames")
 
+    # timed region: frames back to back (consecutive frames overlap on two streams)
     t0 = time.perf_counter()
     for i in range(args.steps):
         r.record_frame()
-        if (i + 1) % 10 == 0:
-            r.synchronize()
-            log(f"  {i + 1}/{args.steps} frames")
     r.synchronize()
     elapsed = time.perf_counter() - t0
+    log(f"{args.steps} frames in {elapsed * 1e3:.1f} ms")
+
+    # per-pass kernel times, each frame on its own (no overlap), for the rooflines
+    set_profile(True)
+    r.reset_pass_stats()
+    for _ in range(PROFILE_FRAMES):
+        r.record_frame()
+        r.synchronize()
     stats = r.pass_stats()
+    set_profile(False)
     w, h = args.width, args.height
     samples = w * h * args.steps
     msps = samples / elapsed / 1e6
@@ -155,9 +176,11 @@ def main() -> int:
                    "width": w, "height": h, "spp": 1, "depth": args.depth, "photons": args.photons,
                    "triangles": counts["triangles"],
                    "traversal": TRAVERSALS[tinfo["mode"]], "bvh_depth": tinfo["bvh_depth"],
+                   "camera_pass": "path queues (primary + one pass per depth)" if args.path_queues else "one kernel",
                    "photon_map": photons_on},
         "mrays_per_s": round(mrays, 2),
         "pass_ms_per_frame": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},
+        "pass_ms_note": f"kernel times of {PROFILE_FRAMES} frames run one at a time (HIP events on each pass's stream)",
         "photon_rebuild_ms": round(t_photons * 1e3, 2),
         "photon_pass_ms": {k: round(v[0], 3) for k, v in photon_stats.items() if v[1]},
         "setup_s": {"scene": round(t_scene, 2), "create_bvh_upload": round(t_create, 2)},

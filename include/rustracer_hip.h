@@ -80,6 +80,10 @@ typedef struct rt_config {
 /* Use the binary child-box records instead of the default 4-wide records
  * (both exact; for measurement). */
 #define RT_FLAG_BINARY_RECORDS 16u
+/* Run the camera pass as path-queue passes (a primary pass, then one pass per
+ * depth over the compacted live paths) instead of the default one kernel per
+ * frame (same results). */
+#define RT_FLAG_PATH_QUEUES 32u
 
 /* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
 int rt_config_default(rt_config *cfg);

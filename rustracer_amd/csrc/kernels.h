@@ -43,7 +43,14 @@ struct GatherLaunch {
     uint32_t mode;           // 0 reference nodes, 1 child-box reference order, 2 child-box nearest first, 3 4-wide
     uint32_t early_shadow;   // shadow rays stop at the first hit nearer than the light
     unsigned long long *counters;  // non-null: count traffic (4 counters)
+    // path queues: when px_state is non-null the frame runs as a primary pass
+    // plus one pass per depth (see gather_wavefront_bytes for the sizes)
+    void *px_state;          // 5 * n_px float4
+    void *queue[2];          // 3 * n_px float4 each
+    uint32_t *path_count;    // max_depth + 1 counters
 };
+// Device bytes of the path-queue state for n_px pixels.
+size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth);
 
 hipError_t launch_gather(const GatherLaunch &g, hipStream_t s);
 

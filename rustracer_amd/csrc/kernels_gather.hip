@@ -7,6 +7,8 @@
 // levels a band's rays share). Per-pixel arithmetic follows gather.wgsl line
 // by line (cited below), so each pixel's RNG stream and result match a CPU
 // evaluation of the shader exactly.
+#include <algorithm>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -32,6 +34,12 @@ struct GatherArgs {
     uint32_t tiles_x;
     uint32_t n_tiles;
     unsigned long long *counters;
+    // path-queue (wavefront) state; unused by the one-pass kernels
+    float4 *px_state;       // kPixelPlanes planes of n_px float4
+    float4 *queue[2];       // kQueuePlanes planes of n_px float4 each (paths of even / odd depth)
+    uint32_t *path_count;   // paths queued per depth [max_depth + 1]
+    uint32_t n_px;
+    uint32_t sample;        // sample index of this pass
 };
 
 struct LightSample {
@@ -196,206 +204,249 @@ struct ViewParams {
     uint32_t width, height, spp, frame;
 };
 
-// gather.wgsl:258-594 for the pixel (px, py) of view `V`; the result goes to
-// element `out_idx` of the frame / moments / G-buffer arrays.
+// The environment light (gather.wgsl:274-288): the first kind-3 light, else
+// the shader's default sky.
+struct Env {
+    float intensity;
+    F3 color, horizon;
+};
+__device__ __forceinline__ Env find_env(const GatherArgs &A) {
+    Env env{0.0f, f3(0.55f, 0.6f, 0.75f), f3(0.95f, 0.93f, 0.9f)};
+    for (uint32_t i = 0; i < A.n_lights; i++) {
+        if (A.lights[i].kind == 3u) {
+            env.intensity = A.lights[i].intensity;
+            env.color = ld3(A.lights[i].color);
+            env.horizon = ld3(A.lights[i].data);
+            break;
+        }
+    }
+    return env;
+}
+
+// Normal and depth of the nearest first hit over the pixel's samples (G-buffer).
+struct FirstHit {
+    F3 normal;
+    float depth;
+};
+
+// One iteration of the bounce loop of gather.wgsl:298-537 at depth b: the
+// closest hit of (ro, rd) and its shading. Returns true when the path goes on
+// (ro, rd and thr then hold the next ray), false when it ends.
 template <int MODE, bool EARLY, bool COUNT>
-__device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MODE, EARLY, COUNT> &tr, uint32_t px,
-                             uint32_t py, size_t out_idx, uint32_t &photon_reads) {
+__device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, COUNT> &tr, uint32_t b, F3 &ro,
+                       F3 &rd, F3 &thr, Rng &rng, Acc &acc, FirstHit &fh, uint32_t &photon_reads) {
     const rt_image_params &ip = A.ip;
-    const rt_camera_uniform &cam = V.cam;
+    const uint32_t debug_view = ip._pad[0];
+    const Hit hit = tr.closest(ro, rd);
+    if (hit.t >= kInf) {
+        if (b > 0u) acc.add(thr * sky_color(rd, env.color, env.horizon, env.intensity));
+        return false;
+    }
+    const rt_triangle &trg = A.tris[hit.prim];
+    const rt_material &mat = A.mats[trg.material_id];
+    const F3 hp = ro + rd * hit.t;
+    const Bary bb = barycentric(trg, ro, rd, hit.t);
+    const F3 gn = get_triangle_normal(trg);
+    const bool ff = dot(rd, gn) < 0.0f;
+    const F3 sn_geom = smooth_normal(trg, bb, gn);
+    const F3 sn = ff ? sn_geom : -sn_geom;
+    const F3 shn = sn;
+
+    if (b == 0u && hit.t < fh.depth) {
+        fh.normal = sn;
+        fh.depth = hit.t;
+    }
+    const F3 albedo = f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
+    const float rough = mat.roughness;
+    const float metal = mat.metallic;
+    const F3 emissive = ld3(mat.emissive);
+    acc.add(thr * emissive);
+
+    if (debug_view != 0u) {
+        bool stop = true;
+        switch (debug_view) {
+            case 1u: acc.col = f3(bb.uv.x, bb.uv.y, 0.0f); break;
+            case 2u: case 3u: acc.col = sn * 0.5f + fs(0.5f); break;
+            case 4u: acc.col = gn * 0.5f + fs(0.5f); break;
+            case 5u: case 12u: acc.col = emissive; break;
+            case 8u: case 9u: case 11u: case 18u: acc.col = fs(0.0f); break;
+            case 13u: acc.col = f3((float)mat.tex_emissive / 65535.0f, 0.0f, 0.0f); break;
+            case 17u: acc.col = f3(bb.uv.x - floorf(bb.uv.x), bb.uv.y - floorf(bb.uv.y), 0.0f); break;
+            case 19u: acc.col = tangent_frame(trg, sn).t * 0.5f + fs(0.5f); break;
+            case 20u: acc.col = tangent_frame(trg, sn).b * 0.5f + fs(0.5f); break;
+            case 21u: acc.col = hp; break;
+            default: stop = false; break;
+        }
+        if (stop) return false;
+    }
+
+    if (mat.kind == 2u) {
+        const F3 f0 = fs(wgsl_pow((mat.ior - 1.0f) / (mat.ior + 1.0f), 2.0f));
+        const float ct = __builtin_fabsf(dot(rd, shn));
+        const float fr = fresnel_schlick(ct, f0).x;
+        if (rng_f32(rng) < fr) {
+            rd = reflect(rd, shn);
+            ro = hp + shn * 0.001f;
+        } else {
+            const float eta = ff ? 1.0f / mat.ior : mat.ior;
+            rd = refract(rd, shn, eta);
+            ro = hp - shn * 0.001f;
+            thr = thr * albedo;
+        }
+        return true;
+    }
+
+    const F3 f0 = mix(fs(0.04f), albedo, fs(metal));
+    const F3 wo = -rd;
+    const float a = wmax(rough * rough, 1e-3f);
+    const F3 fr = fresnel_schlick(__builtin_fabsf(dot(rd, shn)), f0);
+    const F3 diffuse_weight = (1.0f - metal) * albedo;
+    const float sp = vmax3(fr);
+    const float dp = vmax3(diffuse_weight);
+    const float sel = (sp + dp <= 0.0f) ? 0.5f : sp / (sp + dp);
+    const float spec_prob = wclamp(sel, 0.001f, 0.999f);
+    const bool do_spec = rng_f32(rng) < spec_prob;
+    const F3 dscale = diffuse_weight / (1.0f - spec_prob);
+
+    const LightSample ls = direct_light(A, tr, hp, shn, rng);
+    if (!do_spec) {
+        if (env.intensity > 0.0f)
+            acc.add(thr * dscale * sky_color(shn, env.color, env.horizon, env.intensity));
+        acc.add(thr * dscale * ls.e / kPi);
+        acc.add(thr * albedo * ip.ambient);
+    } else {
+        acc.add(thr * spec_ggx(wo, shn, ls.wi, a, f0) * ls.e / spec_prob);
+    }
+
+    if (ip.use_photons == 1u && !do_spec) {
+        // gather.wgsl:473-514
+        const rt_grid_params &gp = A.gp;
+        const float r2 = ip.photon_radius * ip.photon_radius;
+        F3 ph_sum = fs(0.0f);
+        uint32_t ph_hits = 0u, budget = 384u;
+        bool done = false;
+        const int32_t bx = wgsl_i32((hp.x - gp.room_min_x) / gp.cell_size);
+        const int32_t by = wgsl_i32((hp.y - gp.room_min_y) / gp.cell_size);
+        const int32_t bz = wgsl_i32((hp.z - gp.room_min_z) / gp.cell_size);
+        const int32_t gxmax = (int32_t)gp.grid_x - 1, gymax = (int32_t)gp.grid_y - 1,
+                      gzmax = (int32_t)gp.grid_z - 1;
+        // Same visiting order, budget and stop rules as the shader's loops; the
+        // loads are grouped to shorten the chain of dependent memory reads: the
+        // three cell records of a row together, then up to four candidates'
+        // positions, then the accepted candidates' power and incident
+        // direction. Sums are added in candidate order.
+        for (int32_t dz = -1; dz <= 1 && !done; dz++) {
+            for (int32_t dy = -1; dy <= 1 && !done; dy++) {
+                const int32_t cy = iclamp((int32_t)((uint32_t)by + (uint32_t)dy), 0, gymax);
+                const int32_t cz = iclamp((int32_t)((uint32_t)bz + (uint32_t)dz), 0, gzmax);
+                const uint32_t row = (uint32_t)cy * gp.grid_x + (uint32_t)cz * gp.grid_x * gp.grid_y;
+                uint2 metas[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int32_t cx = iclamp((int32_t)((uint32_t)bx + (uint32_t)(k - 1)), 0, gxmax);
+                    metas[k] = A.grid_meta[(uint32_t)cx + row];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const uint2 meta = metas[k];
+                    for (uint32_t i = 0u; i < meta.y && !done; i += 4u) {
+                        const uint32_t nb = min(meta.y - i, 4u);
+                        float4 pp[4];
+#pragma unroll
+                        for (uint32_t j = 0; j < 4u; j++)
+                            pp[j] = j < nb ? A.sorted_pos[meta.x + i + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        uint32_t accept = 0u;
+#pragma unroll
+                        for (uint32_t j = 0; j < 4u; j++) {
+                            if (j < nb && !done) {
+                                if (budget == 0u) {
+                                    done = true;
+                                } else {
+                                    budget -= 1u;
+                                    if (COUNT) photon_reads++;
+                                    const F3 d = xyz(pp[j]) - hp;
+                                    const float d2 = dot(d, d);
+                                    if (d2 < r2) {
+                                        accept |= 1u << j;
+                                        ph_hits += 1u;
+                                        if (ph_hits >= 64u) done = true;
+                                    }
+                                }
+                            }
+                        }
+                        float4 pw[4], inc[4];
+#pragma unroll
+                        for (uint32_t j = 0; j < 4u; j++) {
+                            if ((accept >> j) & 1u) {
+                                const rt_photon &ph = A.sorted[meta.x + i + j];
+                                pw[j] = *reinterpret_cast<const float4 *>(&ph.power[0]);
+                                inc[j] = *reinterpret_cast<const float4 *>(&ph.incident[0]);
+                            }
+                        }
+#pragma unroll
+                        for (uint32_t j = 0; j < 4u; j++)
+                            if ((accept >> j) & 1u) ph_sum = ph_sum + xyz(pw[j]) * wmax(dot(shn, -xyz(inc[j])), 0.0f);
+                    }
+                }
+            }
+        }
+        if (ph_hits > 0u)
+            acc.add(thr * dscale * ph_sum * ip.photon_scale /
+                           (kPi * (float)ph_hits * ip.photon_radius * ip.photon_radius));
+        if (ip.photon_debug == 1u) {
+            const float density = wmin((float)ph_hits / 64.0f, 1.0f);
+            acc.col = f3(1.0f - density, density, 0.0f) * 2.0f;
+        }
+    }
+
+    if (do_spec) {
+        rd = sample_spec_dir(rd, shn, a, rng);
+        ro = hp + shn * 0.001f;
+        const F3 h = normalize(wo + rd);
+        const float ndl = wmax(dot(shn, rd), 0.0f);
+        thr = thr * fresnel_schlick(wmax(dot(wo, h), 0.0f), f0) * smith_g1(ndl, a) / spec_prob;
+    } else {
+        const F3 local = sample_cosine_hemisphere(rng);
+        rd = to_world(local, shn);
+        ro = hp + shn * 0.001f;
+        thr = thr * dscale;
+        if (ip.use_photons == 1u) return false;
+    }
+    const float p = vmax3(thr);
+    if (p <= 0.0f || rng_f32(rng) > p) return false;
+    thr = thr / p;
+    return true;
+}
+
+// gather.wgsl:262-263: the pixel's seed; gather.wgsl:291-296: the jittered camera ray of one sample.
+__device__ __forceinline__ Rng pixel_rng(const ViewParams &V, uint32_t px, uint32_t py) {
     const uint32_t base_seed = py * V.width + px + V.frame * 2654435761u;
-    Rng rng = rng_init(base_seed * 1664525u + 1013904223u);
-    Acc acc;
+    return rng_init(base_seed * 1664525u + 1013904223u);
+}
+__device__ __forceinline__ void camera_ray(const ViewParams &V, uint32_t px, uint32_t py, Rng &rng, F3 &ro, F3 &rd) {
+    const rt_camera_uniform &cam = V.cam;
+    const float jx = rng_f32(rng) - 0.5f;
+    const float jy = rng_f32(rng) - 0.5f;
+    const float ux = ((float)px + jx) * cam.inv_half_res[0] - 1.0f;
+    const float uy = 1.0f - ((float)py + jy) * cam.inv_half_res[1];
+    ro = ld3(cam.position);
+    rd = normalize(ld3(cam.forward) + ld3(cam.right) * ux + ld3(cam.up) * uy);
+}
+
+__device__ __forceinline__ void acc_clear(Acc &acc, FirstHit &fh) {
     acc.col = fs(0.0f);
     acc.lum_sum = 0.0f;
     acc.lum2_sum = 0.0f;
     acc.cur = 0.0f;
 #pragma unroll
     for (int i = 0; i < 8; i++) acc.sl[i] = 0.0f;
-    F3 first_normal = f3(0.0f, 1.0f, 0.0f);
-    float first_depth = 1e30f;
-    const uint32_t spp = max(1u, V.spp);
-    const uint32_t debug_view = ip._pad[0];
+    fh.normal = f3(0.0f, 1.0f, 0.0f);
+    fh.depth = 1e30f;
+}
 
-    float env_intensity = 0.0f;
-    F3 env_color = f3(0.55f, 0.6f, 0.75f);
-    F3 env_horizon = f3(0.95f, 0.93f, 0.9f);
-    for (uint32_t i = 0; i < A.n_lights; i++) {
-        if (A.lights[i].kind == 3u) {
-            env_intensity = A.lights[i].intensity;
-            env_color = ld3(A.lights[i].color);
-            env_horizon = ld3(A.lights[i].data);
-            break;
-        }
-    }
-    const F3 cam_pos = ld3(cam.position), cam_fwd = ld3(cam.forward);
-    const F3 cam_right = ld3(cam.right), cam_up = ld3(cam.up);
-
-    for (uint32_t s = 0u; s < spp; s++) {
-        const float jx = rng_f32(rng) - 0.5f;
-        const float jy = rng_f32(rng) - 0.5f;
-        const float ux = ((float)px + jx) * cam.inv_half_res[0] - 1.0f;
-        const float uy = 1.0f - ((float)py + jy) * cam.inv_half_res[1];
-        F3 ro = cam_pos;
-        F3 rd = normalize(cam_fwd + cam_right * ux + cam_up * uy);
-        F3 thr = fs(1.0f);
-
-        for (uint32_t b = 0u; b < A.max_depth; b++) {
-            const Hit hit = tr.closest(ro, rd);
-            if (hit.t >= kInf) {
-                if (b > 0u) acc.add(thr * sky_color(rd, env_color, env_horizon, env_intensity));
-                break;
-            }
-            const rt_triangle &trg = A.tris[hit.prim];
-            const rt_material &mat = A.mats[trg.material_id];
-            const F3 hp = ro + rd * hit.t;
-            const Bary bb = barycentric(trg, ro, rd, hit.t);
-            const F3 gn = get_triangle_normal(trg);
-            const bool ff = dot(rd, gn) < 0.0f;
-            const F3 sn_geom = smooth_normal(trg, bb, gn);
-            const F3 sn = ff ? sn_geom : -sn_geom;
-            const F3 shn = sn;
-
-            if (b == 0u && hit.t < first_depth) {
-                first_normal = sn;
-                first_depth = hit.t;
-            }
-            const F3 albedo = f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
-            const float rough = mat.roughness;
-            const float metal = mat.metallic;
-            const F3 emissive = ld3(mat.emissive);
-            acc.add(thr * emissive);
-
-            if (debug_view != 0u) {
-                bool stop = true;
-                switch (debug_view) {
-                    case 1u: acc.col = f3(bb.uv.x, bb.uv.y, 0.0f); break;
-                    case 2u: case 3u: acc.col = sn * 0.5f + fs(0.5f); break;
-                    case 4u: acc.col = gn * 0.5f + fs(0.5f); break;
-                    case 5u: case 12u: acc.col = emissive; break;
-                    case 8u: case 9u: case 11u: case 18u: acc.col = fs(0.0f); break;
-                    case 13u: acc.col = f3((float)mat.tex_emissive / 65535.0f, 0.0f, 0.0f); break;
-                    case 17u: acc.col = f3(bb.uv.x - floorf(bb.uv.x), bb.uv.y - floorf(bb.uv.y), 0.0f); break;
-                    case 19u: acc.col = tangent_frame(trg, sn).t * 0.5f + fs(0.5f); break;
-                    case 20u: acc.col = tangent_frame(trg, sn).b * 0.5f + fs(0.5f); break;
-                    case 21u: acc.col = hp; break;
-                    default: stop = false; break;
-                }
-                if (stop) break;
-            }
-
-            if (mat.kind == 2u) {
-                const F3 f0 = fs(wgsl_pow((mat.ior - 1.0f) / (mat.ior + 1.0f), 2.0f));
-                const float ct = __builtin_fabsf(dot(rd, shn));
-                const float fr = fresnel_schlick(ct, f0).x;
-                if (rng_f32(rng) < fr) {
-                    rd = reflect(rd, shn);
-                    ro = hp + shn * 0.001f;
-                } else {
-                    const float eta = ff ? 1.0f / mat.ior : mat.ior;
-                    rd = refract(rd, shn, eta);
-                    ro = hp - shn * 0.001f;
-                    thr = thr * albedo;
-                }
-                continue;
-            }
-
-            const F3 f0 = mix(fs(0.04f), albedo, fs(metal));
-            const F3 wo = -rd;
-            const float a = wmax(rough * rough, 1e-3f);
-            const F3 fr = fresnel_schlick(__builtin_fabsf(dot(rd, shn)), f0);
-            const F3 diffuse_weight = (1.0f - metal) * albedo;
-            const float sp = vmax3(fr);
-            const float dp = vmax3(diffuse_weight);
-            const float sel = (sp + dp <= 0.0f) ? 0.5f : sp / (sp + dp);
-            const float spec_prob = wclamp(sel, 0.001f, 0.999f);
-            const bool do_spec = rng_f32(rng) < spec_prob;
-            const F3 dscale = diffuse_weight / (1.0f - spec_prob);
-
-            const LightSample ls = direct_light(A, tr, hp, shn, rng);
-            if (!do_spec) {
-                if (env_intensity > 0.0f)
-                    acc.add(thr * dscale * sky_color(shn, env_color, env_horizon, env_intensity));
-                acc.add(thr * dscale * ls.e / kPi);
-                acc.add(thr * albedo * ip.ambient);
-            } else {
-                acc.add(thr * spec_ggx(wo, shn, ls.wi, a, f0) * ls.e / spec_prob);
-            }
-
-            if (ip.use_photons == 1u && !do_spec) {
-                // gather.wgsl:473-514
-                const rt_grid_params &gp = A.gp;
-                const float r2 = ip.photon_radius * ip.photon_radius;
-                F3 ph_sum = fs(0.0f);
-                uint32_t ph_hits = 0u, budget = 384u;
-                bool done = false;
-                const int32_t bx = wgsl_i32((hp.x - gp.room_min_x) / gp.cell_size);
-                const int32_t by = wgsl_i32((hp.y - gp.room_min_y) / gp.cell_size);
-                const int32_t bz = wgsl_i32((hp.z - gp.room_min_z) / gp.cell_size);
-                const int32_t gxmax = (int32_t)gp.grid_x - 1, gymax = (int32_t)gp.grid_y - 1,
-                              gzmax = (int32_t)gp.grid_z - 1;
-                for (int32_t dz = -1; dz <= 1 && !done; dz++) {
-                    for (int32_t dy = -1; dy <= 1 && !done; dy++) {
-                        for (int32_t dx = -1; dx <= 1 && !done; dx++) {
-                            const int32_t cx = iclamp((int32_t)((uint32_t)bx + (uint32_t)dx), 0, gxmax);
-                            const int32_t cy = iclamp((int32_t)((uint32_t)by + (uint32_t)dy), 0, gymax);
-                            const int32_t cz = iclamp((int32_t)((uint32_t)bz + (uint32_t)dz), 0, gzmax);
-                            const uint32_t cell = (uint32_t)cx + (uint32_t)cy * gp.grid_x + (uint32_t)cz * gp.grid_x * gp.grid_y;
-                            const uint2 meta = A.grid_meta[cell];
-                            for (uint32_t i = 0u; i < meta.y && !done; i++) {
-                                if (budget == 0u) {
-                                    done = true;
-                                    break;
-                                }
-                                budget -= 1u;
-                                const float4 pp4 = A.sorted_pos[meta.x + i];
-                                if (COUNT) photon_reads++;
-                                const F3 d = xyz(pp4) - hp;
-                                const float d2 = dot(d, d);
-                                if (d2 < r2) {
-                                    const rt_photon &ph = A.sorted[meta.x + i];
-                                    const float4 pw = *reinterpret_cast<const float4 *>(&ph.power[0]);
-                                    const float4 inc = *reinterpret_cast<const float4 *>(&ph.incident[0]);
-                                    ph_sum = ph_sum + xyz(pw) * wmax(dot(shn, -xyz(inc)), 0.0f);
-                                    ph_hits += 1u;
-                                    if (ph_hits >= 64u) {
-                                        done = true;
-                                        break;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
-                if (ph_hits > 0u)
-                    acc.add(thr * dscale * ph_sum * ip.photon_scale /
-                                   (kPi * (float)ph_hits * ip.photon_radius * ip.photon_radius));
-                if (ip.photon_debug == 1u) {
-                    const float density = wmin((float)ph_hits / 64.0f, 1.0f);
-                    acc.col = f3(1.0f - density, density, 0.0f) * 2.0f;
-                }
-            }
-
-            if (do_spec) {
-                rd = sample_spec_dir(rd, shn, a, rng);
-                ro = hp + shn * 0.001f;
-                const F3 h = normalize(wo + rd);
-                const float ndl = wmax(dot(shn, rd), 0.0f);
-                thr = thr * fresnel_schlick(wmax(dot(wo, h), 0.0f), f0) * smith_g1(ndl, a) / spec_prob;
-            } else {
-                const F3 local = sample_cosine_hemisphere(rng);
-                rd = to_world(local, shn);
-                ro = hp + shn * 0.001f;
-                thr = thr * dscale;
-                if (ip.use_photons == 1u) break;
-            }
-            const float p = vmax3(thr);
-            if (p <= 0.0f || rng_f32(rng) > p) break;
-            thr = thr / p;
-        }
-        acc.end_sample();
-    }
-
+// gather.wgsl:540-595: firefly clamp over the samples, then the frame, moments
+// and G-buffer writes.
+__device__ void finalize(const GatherArgs &A, Acc &acc, uint32_t spp, const FirstHit &fh, size_t out_idx) {
     // gather.wgsl:548-579 firefly sigma clamp (spp <= 8). The samples' lumas are
     // at sl[w0 .. 7] (w0 = 8 - spp) in sample order; the insertion sort, median,
     // spread and clamped sums run over that window with static indices.
@@ -460,11 +511,36 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
     A.frame[out_idx] = make_float4(col.x, col.y, col.z, 1.0f);
     A.moments[out_idx] =
         make_float4(wmin(acc.lum_sum / (float)spp, 10.0f), wmin(acc.lum2_sum / (float)spp, 100.0f), 0.0f, 0.0f);
-    A.gbuffer[out_idx] = make_float4(first_normal.x, first_normal.y, first_normal.z, first_depth);
+    A.gbuffer[out_idx] = make_float4(fh.normal.x, fh.normal.y, fh.normal.z, fh.depth);
 }
 
+// gather.wgsl:258-594 for the pixel (px, py) of view `V` in one pass (all
+// samples and bounces); the result goes to element `out_idx` of the frame /
+// moments / G-buffer arrays.
 template <int MODE, bool EARLY, bool COUNT>
-__global__ __launch_bounds__(64) void gather_kernel(GatherArgs A) {
+__device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MODE, EARLY, COUNT> &tr, uint32_t px,
+                             uint32_t py, size_t out_idx, uint32_t &photon_reads) {
+    Rng rng = pixel_rng(V, px, py);
+    Acc acc;
+    FirstHit fh;
+    acc_clear(acc, fh);
+    const uint32_t spp = max(1u, V.spp);
+    const Env env = find_env(A);
+    for (uint32_t s = 0u; s < spp; s++) {
+        F3 ro, rd;
+        camera_ray(V, px, py, rng, ro, rd);
+        F3 thr = fs(1.0f);
+        for (uint32_t b = 0u; b < A.max_depth; b++)
+            if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads)) break;
+        acc.end_sample();
+    }
+    finalize(A, acc, spp, fh, out_idx);
+}
+
+// At least 4 wavefronts per SIMD (a 128-VGPR budget): measured faster than the
+// compiler's own choice of 130 VGPRs at 3 wavefronts.
+template <int MODE, bool EARLY, bool COUNT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gather_kernel(GatherArgs A) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     // XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a
     // contiguous band of tiles.
@@ -484,6 +560,193 @@ __global__ __launch_bounds__(64) void gather_kernel(GatherArgs A) {
     const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
     if (px < A.ip.width && py < A.ip.height)
         gather_pixel<MODE, EARLY, COUNT>(A, V, tr, px, py, (size_t)py * A.ip.width + px, photon_reads);
+    if (COUNT) {
+        atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
+        atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
+        atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
+        atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+    }
+}
+
+// ─────────────────────────── path queues (wavefront) ───────────────────────────
+//
+// The same per-pixel computation split at bounce boundaries. A primary pass
+// runs sample s of every pixel through its first bounce; paths that go on are
+// appended to a queue (wave-aggregated atomics), and one pass per depth takes
+// the queue of depth b and writes that of depth b + 1. Every pass runs full
+// wavefronts of live paths, so the few long paths (glass, metal chains) no
+// longer hold a wavefront of finished pixels. Each pixel's RNG stream, its
+// accumulators and every arithmetic step are those of gather_pixel, in the
+// same order; only the place where they are kept between bounces changes
+// (registers there, HBM here). Samples run one after another, as the RNG
+// stream requires.
+//
+// Pixel state (SoA planes of n_px float4):
+//   0: col.xyz, lum_sum   1: lum2_sum, cur, rng state, 0
+//   2: sl[0..3]           3: sl[4..7]           4: first normal.xyz, first depth
+// Queue entry (SoA planes): 0: ro.xyz, pixel   1: rd.xyz, rng state   2: thr.xyz, 0
+constexpr uint32_t kPixelPlanes = 5;
+constexpr uint32_t kQueuePlanes = 3;
+
+__device__ __forceinline__ void store_running(const GatherArgs &A, uint32_t i, const Acc &acc, uint32_t rng) {
+    A.px_state[i] = make_float4(acc.col.x, acc.col.y, acc.col.z, acc.lum_sum);
+    A.px_state[A.n_px + i] = make_float4(acc.lum2_sum, acc.cur, __uint_as_float(rng), 0.0f);
+}
+__device__ __forceinline__ void load_running(const GatherArgs &A, uint32_t i, Acc &acc, uint32_t &rng) {
+    const float4 p0 = A.px_state[i], p1 = A.px_state[A.n_px + i];
+    acc.col = xyz(p0);
+    acc.lum_sum = p0.w;
+    acc.lum2_sum = p1.x;
+    acc.cur = p1.y;
+    rng = __float_as_uint(p1.z);
+}
+__device__ __forceinline__ void store_samples(const GatherArgs &A, uint32_t i, const Acc &acc, const FirstHit &fh) {
+    A.px_state[2u * A.n_px + i] = make_float4(acc.sl[0], acc.sl[1], acc.sl[2], acc.sl[3]);
+    A.px_state[3u * A.n_px + i] = make_float4(acc.sl[4], acc.sl[5], acc.sl[6], acc.sl[7]);
+    A.px_state[4u * A.n_px + i] = make_float4(fh.normal.x, fh.normal.y, fh.normal.z, fh.depth);
+}
+__device__ __forceinline__ void load_samples(const GatherArgs &A, uint32_t i, Acc &acc, FirstHit &fh) {
+    const float4 a = A.px_state[2u * A.n_px + i], b = A.px_state[3u * A.n_px + i];
+    const float4 c = A.px_state[4u * A.n_px + i];
+    acc.sl[0] = a.x; acc.sl[1] = a.y; acc.sl[2] = a.z; acc.sl[3] = a.w;
+    acc.sl[4] = b.x; acc.sl[5] = b.y; acc.sl[6] = b.z; acc.sl[7] = b.w;
+    fh.normal = xyz(c);
+    fh.depth = c.w;
+}
+
+// Index of this lane's entry when the lanes with `push` append one entry each:
+// one atomic per wavefront, ranks from the ballot. All lanes must call it.
+__device__ __forceinline__ uint32_t wave_append(bool push, uint32_t *counter) {
+    const unsigned long long m = __ballot(push);
+    if (m == 0ull) return 0u;
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0u;
+    if (lane == (uint32_t)leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void queue_put(float4 *q, uint32_t n_px, uint32_t slot, F3 ro, F3 rd, F3 thr,
+                                          uint32_t pixel, uint32_t rng) {
+    q[slot] = make_float4(ro.x, ro.y, ro.z, __uint_as_float(pixel));
+    q[n_px + slot] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(rng));
+    q[2u * n_px + slot] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+}
+
+// Where a sample's path ended: close the sample; after the last sample write
+// the pixel's outputs, else park its state for the next primary pass.
+__device__ __forceinline__ void end_path(const GatherArgs &A, uint32_t i, Acc &acc, FirstHit &fh, uint32_t rng,
+                                         uint32_t spp) {
+    acc.end_sample();
+    if (A.sample + 1u >= spp) {
+        finalize(A, acc, spp, fh, i);
+    } else {
+        store_running(A, i, acc, rng);
+        store_samples(A, i, acc, fh);
+    }
+}
+
+template <int MODE, bool EARLY, bool COUNT>
+__global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb / 8u;
+    const uint32_t tile = (b < 8u * q) ? (b % 8u) * q + b / 8u : b;
+    if (tile >= A.n_tiles) return;
+    const uint32_t px = (tile % A.tiles_x) * 8u + (threadIdx.x & 7u);
+    const uint32_t py = (tile / A.tiles_x) * 8u + (threadIdx.x >> 3);
+    uint32_t spill_words[kSpillAlloc];
+    LaneStack stk;
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
+    stk.stride = 64u;
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
+    uint32_t photon_reads = 0u;
+    const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
+    const uint32_t spp = max(1u, V.spp);
+    const uint32_t idx = py * A.ip.width + px;
+    bool push = false;
+    F3 ro, rd, thr;
+    Rng rng{0u};
+    if (px < A.ip.width && py < A.ip.height) {
+        Acc acc;
+        FirstHit fh;
+        if (A.sample == 0u) {
+            rng = pixel_rng(V, px, py);
+            acc_clear(acc, fh);
+        } else {
+            load_running(A, idx, acc, rng.state);
+            load_samples(A, idx, acc, fh);
+        }
+        camera_ray(V, px, py, rng, ro, rd);
+        thr = fs(1.0f);
+        const Env env = find_env(A);
+        const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, 0u, ro, rd, thr, rng, acc, fh, photon_reads);
+        if (cont && A.max_depth > 1u) {
+            push = true;
+            store_running(A, idx, acc, 0u);
+            store_samples(A, idx, acc, fh);
+        } else {
+            end_path(A, idx, acc, fh, rng.state, spp);
+        }
+    }
+    const uint32_t slot = wave_append(push, &A.path_count[1]);
+    if (push) queue_put(A.queue[1], A.n_px, slot, ro, rd, thr, idx, rng.state);
+    if (COUNT) {
+        atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
+        atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
+        atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
+        atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+    }
+}
+
+// Depth b >= 1: the paths queued by the previous pass, grid-stride over the
+// count the previous pass left in path_count[b].
+template <int MODE, bool EARLY, bool COUNT>
+__global__ __launch_bounds__(64) void gather_bounce_kernel(GatherArgs A, uint32_t b) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    uint32_t spill_words[kSpillAlloc];
+    LaneStack stk;
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
+    stk.stride = 64u;
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
+    uint32_t photon_reads = 0u;
+    const uint32_t count = __builtin_amdgcn_readfirstlane(A.path_count[b]);
+    const float4 *qin = A.queue[b & 1u];
+    float4 *qout = A.queue[(b + 1u) & 1u];
+    const uint32_t spp = max(1u, A.ip.spp);
+    const Env env = find_env(A);
+    for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {
+        const uint32_t i = base + threadIdx.x;
+        bool push = false;
+        F3 ro, rd, thr;
+        Rng rng{0u};
+        uint32_t idx = 0u;
+        if (i < count) {
+            const float4 q0 = qin[i], q1 = qin[A.n_px + i], q2 = qin[2u * A.n_px + i];
+            ro = xyz(q0);
+            idx = __float_as_uint(q0.w);
+            rd = xyz(q1);
+            rng.state = __float_as_uint(q1.w);
+            thr = xyz(q2);
+            Acc acc;
+            FirstHit fh{f3(0.0f, 1.0f, 0.0f), 1e30f};  // only depth 0 touches it
+            uint32_t unused;
+            load_running(A, idx, acc, unused);
+            const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads);
+            if (cont && b + 1u < A.max_depth) {
+                push = true;
+                store_running(A, idx, acc, 0u);
+            } else {
+                load_samples(A, idx, acc, fh);
+                end_path(A, idx, acc, fh, rng.state, spp);
+            }
+        }
+        const uint32_t slot = wave_append(push, &A.path_count[b + 1u]);
+        if (push) queue_put(qout, A.n_px, slot, ro, rd, thr, idx, rng.state);
+    }
     if (COUNT) {
         atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
@@ -608,13 +871,40 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
     const uint32_t tiles_y = (g.ip.height + 7u) / 8u;
     A.n_tiles = A.tiles_x * tiles_y;
     A.counters = g.counters;
+    A.px_state = reinterpret_cast<float4 *>(g.px_state);
+    A.queue[0] = reinterpret_cast<float4 *>(g.queue[0]);
+    A.queue[1] = reinterpret_cast<float4 *>(g.queue[1]);
+    A.path_count = g.path_count;
+    A.n_px = g.ip.width * g.ip.height;
+    A.sample = 0u;
     return A;
+}
+
+size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth) {
+    return (size_t)n_px * 16u * (rtd::kPixelPlanes + 2u * rtd::kQueuePlanes) + 4u * ((size_t)max_depth + 1u);
+}
+
+template <int M, bool E, bool C>
+static void launch_wavefront_passes(rtd::GatherArgs A, hipStream_t s) {
+    const uint32_t spp = A.ip.spp > 0u ? A.ip.spp : 1u;
+    const uint32_t bounce_wgs = std::min<uint32_t>((A.n_px + 63u) / 64u, 4096u);
+    for (uint32_t k = 0; k < spp; k++) {
+        A.sample = k;
+        launch_fill_u32(A.path_count, 0u, (size_t)A.max_depth + 1u, s);
+        hipLaunchKernelGGL((rtd::gather_primary_kernel<M, E, C>), dim3(A.n_tiles), dim3(64), 0, s, A);
+        for (uint32_t b = 1; b < A.max_depth; b++)
+            hipLaunchKernelGGL((rtd::gather_bounce_kernel<M, E, C>), dim3(bounce_wgs), dim3(64), 0, s, A, b);
+    }
 }
 
 hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
     const rtd::GatherArgs A = gather_args(g);
     const dim3 grid(A.n_tiles), block(64);
-#define RT_GATHER(M, E, C) hipLaunchKernelGGL((rtd::gather_kernel<M, E, C>), grid, block, 0, s, A)
+#define RT_GATHER(M, E, C)                                                                    \
+    do {                                                                                      \
+        if (g.px_state) launch_wavefront_passes<M, E, C>(A, s);                              \
+        else hipLaunchKernelGGL((rtd::gather_kernel<M, E, C>), grid, block, 0, s, A);        \
+    } while (0)
 #define RT_GATHER_M(M)                        \
     if (g.early_shadow) {                     \
         if (g.counters) RT_GATHER(M, true, true); \

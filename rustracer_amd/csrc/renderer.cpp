@@ -63,6 +63,19 @@ struct rt_renderer {
 
     float *d_accum = nullptr, *d_frame = nullptr, *d_ping = nullptr, *d_moments = nullptr, *d_gbuf = nullptr;
     uint32_t *d_rgba8 = nullptr;
+    // Frame pipelining: consecutive frames alternate between two output sets
+    // (frame, a-trous ping-pong, moments, G-buffer) and two gather streams, so
+    // the camera pass of frame k+1 can start while the last long paths of
+    // frame k are still running. The main stream waits for each frame's
+    // gather before its a-trous and composite passes (which stay in frame
+    // order), and a gather waits until the composite two frames back has
+    // released its output set. d_frame etc. name the latest frame's set.
+    float *set_frame[2] = {nullptr, nullptr}, *set_ping[2] = {nullptr, nullptr};
+    float *set_mom[2] = {nullptr, nullptr}, *set_gb[2] = {nullptr, nullptr};
+    hipStream_t gstream[2] = {nullptr, nullptr};
+    hipEvent_t ev_gdone[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    bool free_recorded[2] = {false, false};
+    uint32_t parity = 0;
 
     rt_grid_params gp{};
     uint32_t meta_cells = 0;
@@ -82,6 +95,8 @@ struct rt_renderer {
 
     unsigned long long *d_counters = nullptr;
     double *d_acc64 = nullptr;
+    void *d_wave = nullptr;  // path-queue state of the camera pass
+    size_t wave_cap = 0;
     uint64_t acc64_frames = 0;
 
     // many-view batch
@@ -106,6 +121,19 @@ struct rt_renderer {
 
     ~rt_renderer() {
         if (stream) hipStreamSynchronize(stream);
+        for (int k = 0; k < 2; k++) {
+            if (gstream[k]) hipStreamSynchronize(gstream[k]);
+            if (ev_gdone[k]) hipEventDestroy(ev_gdone[k]);
+            if (ev_free[k]) hipEventDestroy(ev_free[k]);
+            if (gstream[k]) hipStreamDestroy(gstream[k]);
+            // set 0 is freed through d_frame etc. only when it is the current set
+        }
+        if (set_frame[1]) {
+            for (float *b : {set_frame[0], set_ping[0], set_mom[0], set_gb[0], set_frame[1], set_ping[1], set_mom[1],
+                             set_gb[1]})
+                if (b) hipFree(b);
+            d_frame = d_ping = d_moments = d_gbuf = nullptr;
+        }
         for (auto &p : pending) {
             hipEventDestroy(p.start);
             hipEventDestroy(p.stop);
@@ -113,24 +141,24 @@ struct rt_renderer {
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
-                        d_views, d_tiles, d_counter, d_w4nodes};
+                        d_views, d_tiles, d_counter, d_w4nodes, d_wave};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
     }
 
-    void begin(int pass, hipEvent_t &start) {
+    void begin(int pass, hipEvent_t &start, hipStream_t s = nullptr) {
         start = nullptr;
         if (!(cfg.flags & RT_FLAG_PROFILE)) return;
         hip_check(hipEventCreate(&start), "hipEventCreate");
-        hip_check(hipEventRecord(start, stream), "hipEventRecord");
+        hip_check(hipEventRecord(start, s ? s : stream), "hipEventRecord");
         (void)pass;
     }
-    void end(int pass, hipEvent_t start) {
+    void end(int pass, hipEvent_t start, hipStream_t s = nullptr) {
         if (!start) return;
         hipEvent_t stop;
         hip_check(hipEventCreate(&stop), "hipEventCreate");
-        hip_check(hipEventRecord(stop, stream), "hipEventRecord");
+        hip_check(hipEventRecord(stop, s ? s : stream), "hipEventRecord");
         pending.push_back({pass, start, stop});
     }
     void collect() {
@@ -207,7 +235,32 @@ struct rt_renderer {
         g.mode = traversal_mode();
         g.early_shadow = (cfg.flags & RT_FLAG_CLOSEST_SHADOW) ? 0u : 1u;
         g.counters = nullptr;
+        g.px_state = nullptr;
+        g.queue[0] = g.queue[1] = nullptr;
+        g.path_count = nullptr;
+        if (use_path_queues() && d_wave) {
+            char *w = static_cast<char *>(d_wave);
+            const size_t plane = n_px() * 16u;
+            g.px_state = w;
+            g.queue[0] = w + 5u * plane;
+            g.queue[1] = w + 8u * plane;
+            g.path_count = reinterpret_cast<uint32_t *>(w + 11u * plane);
+        }
         return g;
+    }
+
+    // RT_FLAG_PATH_QUEUES runs the camera pass as path-queue passes; the queue
+    // state is sized for the current image.
+    bool use_path_queues() const { return (cfg.flags & RT_FLAG_PATH_QUEUES) != 0; }
+    void ensure_path_queues() {
+        if (!use_path_queues()) return;
+        const size_t need = rtk::gather_wavefront_bytes((uint32_t)n_px(), cfg.max_depth);
+        if (need <= wave_cap && d_wave) return;
+        if (d_wave) hipFree(d_wave);
+        d_wave = nullptr;
+        wave_cap = 0;
+        hip_check(hipMalloc(&d_wave, need), "hipMalloc path queues");
+        wave_cap = need;
     }
 
     float current_alpha() const {
@@ -222,10 +275,24 @@ struct rt_renderer {
         const float alpha = current_alpha();
         if (cfg.enable_photon_map && !photons_built)
             throw ApiError(RT_ERR_INVALID, "photon map enabled but rt_rebuild_photons has not run");
+        ensure_path_queues();
+        const uint32_t si = parity;
+        parity ^= 1u;
+        d_frame = set_frame[si];
+        d_ping = set_ping[si];
+        d_moments = set_mom[si];
+        d_gbuf = set_gb[si];
+        // the path-queue state is shared by all frames: those gathers stay on the main stream
+        hipStream_t gs = use_path_queues() ? stream : gstream[si];
+        if (gs != stream && free_recorded[si]) hip_check(hipStreamWaitEvent(gs, ev_free[si], 0), "wait set");
         hipEvent_t ev;
-        begin(RT_PASS_GATHER, ev);
-        hip_check(rtk::launch_gather(gather_launch(alpha, d_frame, d_moments, d_gbuf), stream), "gather");
-        end(RT_PASS_GATHER, ev);
+        begin(RT_PASS_GATHER, ev, gs);
+        hip_check(rtk::launch_gather(gather_launch(alpha, d_frame, d_moments, d_gbuf), gs), "gather");
+        end(RT_PASS_GATHER, ev, gs);
+        if (gs != stream) {
+            hip_check(hipEventRecord(ev_gdone[si], gs), "record gather");
+            hip_check(hipStreamWaitEvent(stream, ev_gdone[si], 0), "wait gather");
+        }
         float *result = d_frame;
         if (cfg.enable_denoise) {
             float *src = d_frame, *dst = d_ping;
@@ -261,6 +328,8 @@ struct rt_renderer {
                                                 stream),
                   "composite_tonemap");
         end(RT_PASS_COMPOSITE_TONEMAP, ev);
+        hip_check(hipEventRecord(ev_free[si], stream), "record set release");
+        free_recorded[si] = true;
     }
 
     uint32_t traced_photons() const {
@@ -627,8 +696,22 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->d_moments = dev_alloc<float>(npx * 4, "frame moments");
     r->d_gbuf = dev_alloc<float>(npx * 4, "gbuffer");
     r->d_rgba8 = dev_alloc<uint32_t>(npx, "output");
-    for (float *b : {r->d_accum, r->d_frame, r->d_ping, r->d_moments, r->d_gbuf})
+    r->set_frame[0] = r->d_frame;
+    r->set_ping[0] = r->d_ping;
+    r->set_mom[0] = r->d_moments;
+    r->set_gb[0] = r->d_gbuf;
+    r->set_frame[1] = dev_alloc<float>(npx * 4, "frame (second set)");
+    r->set_ping[1] = dev_alloc<float>(npx * 4, "denoise pingpong (second set)");
+    r->set_mom[1] = dev_alloc<float>(npx * 4, "frame moments (second set)");
+    r->set_gb[1] = dev_alloc<float>(npx * 4, "gbuffer (second set)");
+    for (float *b : {r->d_accum, r->d_frame, r->d_ping, r->d_moments, r->d_gbuf, r->set_frame[1], r->set_ping[1],
+                     r->set_mom[1], r->set_gb[1]})
         hip_check(hipMemset(b, 0, npx * 16), "memset");
+    for (int k = 0; k < 2; k++) {
+        hip_check(hipStreamCreateWithFlags(&r->gstream[k], hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&r->ev_gdone[k], hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&r->ev_free[k], hipEventDisableTiming), "hipEventCreate");
+    }
     hip_check(hipMemset(r->d_rgba8, 0, npx * 4), "memset");
 
     // photon grid (lib.rs:416-479)
@@ -926,8 +1009,11 @@ int rt_trace_closest(rt_renderer *r, const rt_ray *rays, uint32_t n, rt_hit *out
         try {
             d_h = dev_alloc<rt_hit>(n, "hits");
             hip_check(hipMemcpy(d_r, rays, sizeof(rt_ray) * n, hipMemcpyHostToDevice), "upload rays");
+            hipEvent_t ev;
+            r->begin(RT_PASS_TRACE_RAYS, ev);
             hip_check(rtk::launch_trace_rays(r->trace_inputs(), r->traversal_mode(), d_r, n, d_h, r->stream),
                       "trace rays");
+            r->end(RT_PASS_TRACE_RAYS, ev);
             hip_check(hipStreamSynchronize(r->stream), "sync");
             hip_check(hipMemcpy(out, d_h, sizeof(rt_hit) * n, hipMemcpyDeviceToHost), "read hits");
         } catch (...) {
@@ -977,6 +1063,7 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         if (r->cfg.enable_photon_map && !r->photons_built)
             throw ApiError(RT_ERR_INVALID, "photon map enabled but not built");
         hip_check(hipMemsetAsync(r->d_counters, 0, 32, r->stream), "clear counters");
+        r->ensure_path_queues();
         const uint32_t saved = r->sample_counter;
         r->sample_counter = saved + 1u;  // the frame the next rt_render_frame would trace
         rtk::GatherLaunch g = r->gather_launch(1.0f, r->d_ping, r->d_ping, r->d_ping);
@@ -1055,6 +1142,7 @@ static void batch_launch(rt_renderer &r, const rt_batch_tuning &t) {
     batch_validate(t);
     if (t.tile_w != r.tiles_w || t.tile_h != r.tiles_h) batch_build_tiles(r, t.tile_w, t.tile_h);
     rtk::GatherLaunch g = r.gather_launch(1.0f, r.d_bframe, r.d_bmom, r.d_bgb);
+    g.px_state = nullptr;  // the batch kernel is one persistent pass
     rtk::BatchLaunch b;
     b.views = r.d_views;
     b.tiles = r.d_tiles;

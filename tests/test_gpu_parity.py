@@ -17,7 +17,8 @@ from .helpers import camera_rays, face_rays, first_diff, image_params, oracle_fr
 pytestmark = pytest.mark.gpu
 
 FLAG_VARIANTS = [0, _ffi.FLAG_REFERENCE_TRAVERSAL, _ffi.FLAG_CLOSEST_SHADOW,
-                 _ffi.FLAG_REFERENCE_TRAVERSAL | _ffi.FLAG_CLOSEST_SHADOW, _ffi.FLAG_BINARY_RECORDS]
+                 _ffi.FLAG_REFERENCE_TRAVERSAL | _ffi.FLAG_CLOSEST_SHADOW, _ffi.FLAG_BINARY_RECORDS,
+                 _ffi.FLAG_PATH_QUEUES]
 # traversal each flag set selects: 3 = 4-wide records, 1 = binary records, 0 = BVH2 nodes
 EXPECTED_MODE = {0: 3, _ffi.FLAG_BINARY_RECORDS: 1, _ffi.FLAG_REFERENCE_TRAVERSAL: 0}
 
@@ -287,3 +288,52 @@ def test_batch_autotune_and_totals():
         r.render_batch()  # with the tuned parameters: same pixels
         for k in range(len(views)):
             assert np.array_equal(r.read_batch_view(k)[0], first[k])
+
+
+@pytest.mark.parametrize("scene_name,spp,photons", [("cornell_box", 3, False), ("photon_plain", 2, True)])
+def test_path_queue_and_megakernel_same_work(scene_name, spp, photons):
+    """The path-queue passes and the one-pass kernel run the same per-pixel
+    work: identical outputs and identical traversal/photon counters."""
+    scene = rt.Scene.builtin(scene_name, 4)
+    cam = rt.Scene.builtin_camera(scene_name, 4)
+    out = {}
+    for flags in (0, _ffi.FLAG_PATH_QUEUES):
+        cfg = dict(width=96, height=64, spp=spp, max_depth=6, enable_denoise=True, accumulate_frames=4,
+                   enable_photon_map=photons, photon_count=20000, flags=flags)
+        with _renderer(scene, **cfg) as r:
+            r.update_camera(cam)
+            if photons:
+                r.rebuild_photons()
+            r.render_frames(2)
+            cnt = r.count_gather_traffic()
+            out[flags] = (r.read_frame(), r.read_moments(), r.read_gbuffer(), r.read_accumulation(),
+                          r.read_output_rgba8(), cnt)
+    a, b = out[0], out[_ffi.FLAG_PATH_QUEUES]
+    for name, x, y in zip(("frame", "moments", "gbuffer", "accumulation", "rgba8"), a[:5], b[:5]):
+        _check_equal(name, x, y)
+    assert a[5] == b[5], (a[5], b[5])
+    assert a[5]["ray_queries"] > 96 * 64 * spp
+
+
+def test_back_to_back_frames_match_one_at_a_time():
+    """Frames issued back to back (consecutive gathers overlap on two streams,
+    each frame with its own output set) give the same accumulation, output and
+    last-frame buffers as frames run one at a time with a sync in between."""
+    scene = rt.Scene.builtin("multiview", 2)
+    cam = rt.Scene.builtin_camera("multiview", 2, 5)
+    res = []
+    for sync_each in (False, True):
+        cfg = dict(width=640, height=360, spp=1, max_depth=8, enable_denoise=True, accumulate_frames=16,
+                   enable_photon_map=True, photon_count=100000)
+        with _renderer(scene, **cfg) as r:
+            r.update_camera(cam)
+            r.rebuild_photons()
+            for _ in range(5):
+                r.record_frame()
+                if sync_each:
+                    r.synchronize()
+            res.append((r.read_accumulation(), r.read_output_rgba8(), r.read_frame(), r.read_moments(),
+                        r.read_gbuffer(), r.sample_count()))
+    for name, a, b in zip(("accumulation", "rgba8", "frame", "moments", "gbuffer"), res[0][:5], res[1][:5]):
+        _check_equal(name, a, b)
+    assert res[0][5] == res[1][5] == 5
