@@ -214,6 +214,17 @@ def main() -> int:
                                                   "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": int(nbytes),
                                                   "slots": n_slots, "stored": len(srt)}
 
+    # HBM traffic of the gather dispatch, measured with rocprofv3 PMC counters on this
+    # workload (FETCH_SIZE / WRITE_SIZE, scaled against the copy probe) and kept
+    # with the profiles; reported only when the configuration is the one measured
+    traffic_file = os.path.join(ROOT, "profiles", "gather_fullframe_1080p_traffic.txt")
+    default_run = (w, h, args.depth, args.photons, photons_on) == (1920, 1080, 8, 1_000_000, True)
+    if default_run and os.path.exists(traffic_file):
+        t = json.load(open(traffic_file))
+        if t.get("traversal_mode") == tinfo["mode"]:
+            result["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
+            result["roofline"]["traffic_source"] = t["source"]
+
     copy_gbs, copy_ms = rt.hbm_copy_probe(0, 1 << 30, 20)
     result["roofline"]["measured_copy_gbs"] = round(copy_gbs, 1)
     result["roofline"]["frac_of_measured_copy"] = round(achieved / copy_gbs, 4)
