@@ -306,6 +306,11 @@ int rt_hbm_copy_probe(int device, size_t bytes, uint32_t iters, double *gbps, do
 int rt_accum64_reset(rt_renderer *r);
 int rt_accum64_add_accumulation(rt_renderer *r);
 int rt_accum64_read_mean(rt_renderer *r, double *rgba, size_t n_doubles, uint64_t *n_frames);
+/* relMSE = mean over pixels and rgb of (x - y)^2 / (y^2 + eps) of a float4
+ * image x (device pointer, or the accumulation buffer when null) against the
+ * FP64 mean y = sum / frames. No counterpart in the reference (the convergence
+ * measurement). */
+int rt_accum64_relmse(rt_renderer *r, const void *d_image, float eps, double *relmse);
 
 #ifdef __cplusplus
 }

@@ -414,3 +414,9 @@ class Renderer:
         n = C.c_uint64()
         check(lib().rt_accum64_read_mean(self._h, ptr(out), out.size, C.byref(n)), "rt_accum64_read_mean")
         return out, n.value
+
+    def accum64_relmse(self, eps: float = 1e-2, d_image: int | None = None) -> float:
+        """relMSE of the accumulation (or a device float4 image) against the FP64 mean."""
+        v = C.c_double()
+        check(lib().rt_accum64_relmse(self._h, d_image, eps, C.byref(v)), "rt_accum64_relmse")
+        return v.value

@@ -209,6 +209,7 @@ SIGNATURES = {
     "rt_accum64_reset": (_i, [_vp]),
     "rt_accum64_add_accumulation": (_i, [_vp]),
     "rt_accum64_read_mean": (_i, [_vp, _vp, C.c_size_t, _P(C.c_uint64)]),
+    "rt_accum64_relmse": (_i, [_vp, _vp, C.c_float, _P(C.c_double)]),
 }
 
 

@@ -1,0 +1,119 @@
+"""Convergence study: wall time until the displayed image reaches relMSE <= 0.01.
+
+relMSE = mean over pixels and rgb of (x - y)^2 / (y^2 + eps), eps = 1e-2, where y is
+an FP64 reference (the mean of `ref_frames` independent 1-spp frames, accumulated
+in double on the GPU) and x is the displayed linear image (the accumulation
+buffer) of the method under test. A method is run once frame by frame with the
+relMSE taken after every frame (untimed) to find the first frame count k that
+reaches the threshold, then the same k frames are re-run back to back from the
+same seed frame and timed. Run as `python -m rustracer_amd.convergence`.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import rustracer_amd as rt
+from rustracer_amd import _ffi
+
+EPS = 1e-2
+THRESHOLD = 0.01
+
+
+def _log(msg: str) -> None:
+    print(f"[convergence] {msg}", file=sys.stderr, flush=True)
+
+
+def _set(r, **fields) -> None:
+    c = r.config
+    for k, v in fields.items():
+        setattr(c, k, v)
+    r.config = c
+
+
+def build_reference(r, ref_frames: int) -> float:
+    """FP64 mean of ref_frames raw 1-spp frames (no a-trous, alpha = 1 so the
+    accumulation holds each frame). Returns the seconds it took."""
+    _set(r, enable_denoise=False, accumulate_frames=1)
+    r.reset_accumulation()
+    r.accum64_reset()
+    t0 = time.perf_counter()
+    for k in range(ref_frames):
+        r.record_frame()
+        r.accum64_add()
+        if (k + 1) % 2048 == 0:
+            r.synchronize()
+            _log(f"reference {k + 1}/{ref_frames} frames")
+    r.synchronize()
+    return time.perf_counter() - t0
+
+
+def run_method(r, name: str, denoise: bool, start_frame: int, max_frames: int):
+    """First frame count reaching the threshold, its timed back-to-back wall
+    time, and the relMSE curve at powers of two."""
+    _set(r, enable_denoise=denoise, accumulate_frames=1 << 20)
+    r.reset_accumulation()
+    r.set_sample_count(start_frame)
+    curve, hit = {}, None
+    for k in range(1, max_frames + 1):
+        r.record_frame()
+        e = r.accum64_relmse(EPS)
+        if k & (k - 1) == 0:
+            curve[k] = e
+        if e <= THRESHOLD:
+            hit = (k, e)
+            break
+    result = {"method": name, "curve": curve}
+    if hit is None:
+        result.update(frames=None, relmse_at_end=e, frames_run=max_frames)
+        _log(f"{name}: relMSE {e:.4f} after {max_frames} frames (threshold not reached)")
+        return result
+    k, e = hit
+    r.reset_accumulation()
+    r.set_sample_count(start_frame)
+    r.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        r.record_frame()
+    r.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    check = r.accum64_relmse(EPS)
+    result.update(frames=k, relmse=e, relmse_rerun=check, ms=round(ms, 3))
+    _log(f"{name}: relMSE {e:.4f} after {k} frames, {ms:.1f} ms back to back")
+    return result
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scene", default="fullframe")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--photons", type=int, default=1_000_000)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--ref-frames", type=int, default=16384)
+    ap.add_argument("--max-frames", type=int, default=4096)
+    args = ap.parse_args(argv)
+    scene = rt.Scene.builtin(args.scene, 1)
+    cam = rt.Scene.builtin_camera(args.scene, 1)
+    cfg = rt.RenderConfig(width=args.width, height=args.height, spp=1, max_depth=args.depth,
+                          enable_photon_map=args.photons > 0, photon_count=max(args.photons, 1))
+    with rt.Renderer(cfg, scene, 0) as r:
+        r.update_camera(cam)
+        if args.photons > 0:
+            r.rebuild_photons()
+        ref_s = build_reference(r, args.ref_frames)
+        _log(f"reference: {args.ref_frames} frames in {ref_s:.1f} s")
+        start = r.sample_count() + 1000
+        out = {"scene": args.scene, "width": args.width, "height": args.height, "photons": args.photons,
+               "depth": args.depth, "eps": EPS, "threshold": THRESHOLD, "reference_spp": args.ref_frames,
+               "reference_seconds": round(ref_s, 2), "methods": []}
+        out["methods"].append(run_method(r, "a-trous x2 + running mean", True, start, args.max_frames))
+        out["methods"].append(run_method(r, "running mean (no filter)", False, start, args.max_frames))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -122,5 +122,10 @@ hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rg
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_accum64_add(const void *accum_f4, double *sum, size_t n_px, hipStream_t s);
 hipError_t launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+// relMSE of a float4 image against sum / n_frames (double4 per pixel); the
+// result lands in scratch[relmse_scratch_doubles() - 1].
+size_t relmse_scratch_doubles();
+hipError_t launch_relmse(const void *img, const double *sum, uint64_t n_frames, float eps, size_t n_px,
+                         double *scratch, hipStream_t s);
 
 }  // namespace rtk

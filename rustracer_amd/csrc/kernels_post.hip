@@ -204,6 +204,48 @@ __global__ __launch_bounds__(256) void accum64_add_kernel(const float4 *__restri
     sum[4 * i + 3] += (double)a.w;
 }
 
+// relMSE of an image against the FP64 mean sum/n: sum over pixels and rgb of
+// (x - y)^2 / (y^2 + eps), in double. One partial per block (fixed grid-stride
+// order and a fixed tree inside the block), then one block adds the partials
+// in index order: the result does not depend on scheduling.
+constexpr int kRelBlocks = 1024;
+__global__ __launch_bounds__(256) void relmse_partial_kernel(const float4 *__restrict__ img,
+                                                             const double *__restrict__ sum, double inv_n, double eps,
+                                                             size_t n_px, double *__restrict__ partial) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n_px; i += (size_t)gridDim.x * 256u) {
+        const float4 x = img[i];
+        const double xs[3] = {(double)x.x, (double)x.y, (double)x.z};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double y = sum[4 * i + c] * inv_n;
+            const double d = xs[c] - y;
+            acc += d * d / (y * y + eps);
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void relmse_final_kernel(const double *__restrict__ partial, int n, double scale,
+                                                           double *__restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
 // Plain streaming copy (16 B per lane per iteration): the measured-HBM
 // denominator of the rooflines and the calibration kernel of the traffic counters.
 __global__ __launch_bounds__(256) void hbm_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
@@ -215,6 +257,18 @@ __global__ __launch_bounds__(256) void hbm_copy_kernel(const float4 *__restrict_
 }  // namespace rtd
 
 namespace rtk {
+
+size_t relmse_scratch_doubles() { return rtd::kRelBlocks + 1; }
+
+hipError_t launch_relmse(const void *img, const double *sum, uint64_t n_frames, float eps, size_t n_px,
+                         double *scratch, hipStream_t s) {
+    const double inv_n = n_frames ? 1.0 / (double)n_frames : 0.0;
+    hipLaunchKernelGGL(rtd::relmse_partial_kernel, dim3(rtd::kRelBlocks), dim3(256), 0, s,
+                       reinterpret_cast<const float4 *>(img), sum, inv_n, (double)eps, n_px, scratch);
+    hipLaunchKernelGGL(rtd::relmse_final_kernel, dim3(1), dim3(256), 0, s, scratch, rtd::kRelBlocks,
+                       1.0 / (3.0 * (double)n_px), scratch + rtd::kRelBlocks);
+    return hipGetLastError();
+}
 
 hipError_t launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 16;

@@ -95,6 +95,7 @@ struct rt_renderer {
 
     unsigned long long *d_counters = nullptr;
     double *d_acc64 = nullptr;
+    double *d_relmse = nullptr;  // reduction scratch of rt_accum64_relmse
     void *d_wave = nullptr;  // path-queue state of the camera pass
     size_t wave_cap = 0;
     uint64_t acc64_frames = 0;
@@ -141,7 +142,7 @@ struct rt_renderer {
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
-                        d_views, d_tiles, d_counter, d_w4nodes, d_wave};
+                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -1335,6 +1336,20 @@ int rt_accum64_add_accumulation(rt_renderer *r) {
         hip_check(rtk::launch_accum64_add(r->d_accum, r->d_acc64, r->n_px(), r->stream), "accum64");
         r->end(RT_PASS_ACCUM64, ev);
         r->acc64_frames++;
+    });
+}
+
+int rt_accum64_relmse(rt_renderer *r, const void *d_image, float eps, double *relmse) {
+    return api_call([&] {
+        if (!r || !relmse) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (!r->d_acc64 || r->acc64_frames == 0) throw ApiError(RT_ERR_INVALID, "the FP64 sum holds no frames");
+        if (!(eps > 0.0f)) throw ApiError(RT_ERR_INVALID, "eps must be positive");
+        const size_t nd = rtk::relmse_scratch_doubles();
+        if (!r->d_relmse) r->d_relmse = dev_alloc<double>(nd, "relmse scratch");
+        const void *img = d_image ? d_image : r->d_accum;
+        hip_check(rtk::launch_relmse(img, r->d_acc64, r->acc64_frames, eps, r->n_px(), r->d_relmse, r->stream),
+                  "relmse");
+        read_dev(r, relmse, r->d_relmse + (nd - 1), sizeof(double));
     });
 }
 

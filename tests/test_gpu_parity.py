@@ -337,3 +337,32 @@ def test_back_to_back_frames_match_one_at_a_time():
     for name, a, b in zip(("accumulation", "rgba8", "frame", "moments", "gbuffer"), res[0][:5], res[1][:5]):
         _check_equal(name, a, b)
     assert res[0][5] == res[1][5] == 5
+
+
+def test_accum64_relmse_matches_numpy():
+    """The GPU relMSE reduction equals a float64 numpy evaluation of the same
+    formula on the read-back buffers."""
+    scene = rt.Scene.builtin("cornell_box", 2)
+    cam = rt.Scene.builtin_camera("cornell_box", 2)
+    cfg = dict(width=123, height=77, spp=1, enable_denoise=False, accumulate_frames=1)
+    with _renderer(scene, **cfg) as r:
+        r.update_camera(cam)
+        r.accum64_reset()
+        for _ in range(6):
+            r.record_frame()
+            r.accum64_add()
+        ref, n = r.accum64_mean()
+        c = r.config
+        c.accumulate_frames = 1 << 20
+        c.enable_denoise = True
+        r.config = c
+        r.reset_accumulation()
+        for _ in range(3):
+            r.record_frame()
+        got = r.accum64_relmse(1e-2)
+        x = r.read_accumulation().astype(np.float64)
+    assert n == 6
+    eps = float(np.float32(1e-2))  # the entry point takes eps as a float
+    want = np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + eps))
+    assert abs(got - want) <= 1e-12 * max(abs(want), 1e-30), (got, want)
+    assert got > 0
