@@ -239,8 +239,8 @@ int rt_build_photon_grid_from(rt_renderer *r, const rt_photon *photons, uint32_t
 
 /* Per-pass timing with HIP events (cfg.flags & RT_FLAG_PROFILE). */
 typedef struct rt_pass_stats {
-    double ms[8];        /* gather, denoise, composite_tonemap, photon_trace, grid_build, trace_rays, accum64, spare */
-    uint64_t launches[8];
+    double ms[12];       /* indexed by RT_PASS_*; unused entries stay 0 */
+    uint64_t launches[12];
 } rt_pass_stats;
 #define RT_PASS_GATHER 0
 #define RT_PASS_DENOISE 1
@@ -296,6 +296,7 @@ int rt_batch_totals(const rt_renderer *r, uint64_t *pixels, uint64_t *samples);
 int rt_read_batch_view(rt_renderer *r, uint32_t view, float *frame, float *moments, float *gbuffer,
                        size_t n_floats);
 #define RT_PASS_BATCH 7
+#define RT_PASS_NETWORK 8   /* learned denoiser: albedo pass, network, tonemap of its output */
 
 /* Measured HBM bandwidth: a plain device-to-device copy kernel of `bytes`,
  * repeated `iters` times; *gbps counts bytes read + written. No counterpart in
@@ -311,6 +312,42 @@ int rt_accum64_read_mean(rt_renderer *r, double *rgba, size_t n_doubles, uint64_
  * FP64 mean y = sum / frames. No counterpart in the reference (the convergence
  * measurement). */
 int rt_accum64_relmse(rt_renderer *r, const void *d_image, float eps, double *relmse);
+/* The same for the displayed image: the learned denoiser's output when one is
+ * attached, else the accumulation. */
+int rt_accum64_relmse_display(rt_renderer *r, float eps, double *relmse);
+
+/* ───────────── learned denoiser (no counterpart in the reference) ─────────────
+ * A small convolutional U-Net on the displayed image: inputs log1p(accumulated
+ * radiance), first-hit albedo (pixel-centre ray), G-buffer normal and depth, and
+ * the accumulated frame count; output log-radiance residual. BF16 MFMA
+ * convolutions with FP32 accumulation; trained on the GPU with Adam on pairs
+ * the renderer itself produces (few-frame accumulation -> many-frame target). */
+typedef struct rt_unet rt_unet;
+int rt_unet_create(int device, uint64_t seed, rt_unet **out);
+int rt_unet_destroy(rt_unet *u);
+int rt_unet_param_count(const rt_unet *u, size_t *n);
+int rt_unet_get_params(rt_unet *u, float *out, size_t n);
+int rt_unet_set_params(rt_unet *u, const float *in, size_t n);
+/* Training set of `capacity` images of width x height (even, >= 32). */
+int rt_unet_dataset_reset(rt_unet *u, uint32_t width, uint32_t height, uint32_t capacity);
+/* Slot `slot` takes the renderer's current state: which = 0 the network input
+ * (accumulation, G-buffer, albedo, frame count), which = 1 the target (the
+ * accumulation). The renderer's size must equal the dataset's. */
+int rt_unet_dataset_set(rt_unet *u, uint32_t slot, int which, rt_renderer *r);
+/* `steps` Adam steps on random crops (crop x crop, a multiple of 32) of the
+ * slots that hold both; per-step mean squared log error into `losses` (nullable). */
+int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, float lr, uint64_t seed,
+                  float *losses);
+/* One forward pass (and, with a target, the backward pass) on host tensors:
+ * features n*h*w*16 bf16 bits, noisy and target n*h*w*3 log values; pred out
+ * n*h*w*3; grads out (param_count floats, same order as the parameters). */
+int rt_unet_eval(rt_unet *u, uint32_t n, uint32_t h, uint32_t w, const uint16_t *features, const float *noisy,
+                 const float *target, float *pred, float *grads, double *loss);
+/* Attach (or detach with NULL) a denoiser: each rt_render_frame then runs it on
+ * the accumulation and tonemaps its output into the rgba8 image. */
+int rt_renderer_set_denoiser(rt_renderer *r, rt_unet *u);
+int rt_read_display(rt_renderer *r, float *rgba, size_t n_floats);
+int rt_read_albedo(rt_renderer *r, float *rgba, size_t n_floats);
 
 #ifdef __cplusplus
 }

@@ -415,6 +415,24 @@ class Renderer:
         check(lib().rt_accum64_read_mean(self._h, ptr(out), out.size, C.byref(n)), "rt_accum64_read_mean")
         return out, n.value
 
+    def accum64_relmse_display(self, eps: float = 1e-2) -> float:
+        """relMSE of the displayed image (the learned denoiser's output when attached)."""
+        v = C.c_double()
+        check(lib().rt_accum64_relmse_display(self._h, eps, C.byref(v)), "rt_accum64_relmse_display")
+        return v.value
+
+    def set_denoiser(self, unet) -> None:
+        """Attach a learned denoiser (rustracer_amd.denoiser.UNet) or detach with None."""
+        check(lib().rt_renderer_set_denoiser(self._h, None if unet is None else unet.handle),
+              "rt_renderer_set_denoiser")
+        self._unet = unet  # keep it alive while attached
+
+    def read_display(self):
+        return self._px4(lib().rt_read_display, "rt_read_display")
+
+    def read_albedo(self):
+        return self._px4(lib().rt_read_albedo, "rt_read_albedo")
+
     def accum64_relmse(self, eps: float = 1e-2, d_image: int | None = None) -> float:
         """relMSE of the accumulation (or a device float4 image) against the FP64 mean."""
         v = C.c_double()

@@ -46,7 +46,7 @@ FLAG_PROFILE, FLAG_CLOSEST_SHADOW, FLAG_REFERENCE_TRAVERSAL, FLAG_NEAREST_FIRST 
 FLAG_BINARY_RECORDS = 16
 FLAG_PATH_QUEUES = 32
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
-              "accum64", "batch")
+              "accum64", "batch", "network")
 
 
 class Config(C.Structure):
@@ -107,7 +107,7 @@ class Camera(C.Structure):
 
 
 class PassStats(C.Structure):
-    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_uint64 * 8)]
+    _fields_ = [("ms", C.c_double * 12), ("launches", C.c_uint64 * 12)]
 
 
 class View(C.Structure):
@@ -210,6 +210,19 @@ SIGNATURES = {
     "rt_accum64_add_accumulation": (_i, [_vp]),
     "rt_accum64_read_mean": (_i, [_vp, _vp, C.c_size_t, _P(C.c_uint64)]),
     "rt_accum64_relmse": (_i, [_vp, _vp, C.c_float, _P(C.c_double)]),
+    "rt_accum64_relmse_display": (_i, [_vp, C.c_float, _P(C.c_double)]),
+    "rt_unet_create": (_i, [C.c_int, C.c_uint64, _P(_vp)]),
+    "rt_unet_destroy": (_i, [_vp]),
+    "rt_unet_param_count": (_i, [_vp, _P(C.c_size_t)]),
+    "rt_unet_get_params": (_i, [_vp, _vp, C.c_size_t]),
+    "rt_unet_set_params": (_i, [_vp, _vp, C.c_size_t]),
+    "rt_unet_dataset_reset": (_i, [_vp, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "rt_unet_dataset_set": (_i, [_vp, C.c_uint32, C.c_int, _vp]),
+    "rt_unet_train": (_i, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _vp]),
+    "rt_unet_eval": (_i, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp, _vp, _P(C.c_double)]),
+    "rt_renderer_set_denoiser": (_i, [_vp, _vp]),
+    "rt_read_display": (_i, [_vp, _vp, C.c_size_t]),
+    "rt_read_albedo": (_i, [_vp, _vp, C.c_size_t]),
 }
 
 

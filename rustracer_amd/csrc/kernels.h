@@ -119,6 +119,11 @@ hipError_t launch_denoise(const DenoiseLaunch &d, hipStream_t s);
 
 hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rgba8, uint32_t w, uint32_t h,
                                     float alpha, const rt_tonemap_uniform &tm, const void *gbuffer, hipStream_t s);
+hipError_t launch_tonemap(const void *img, uint32_t *rgba8, uint32_t w, uint32_t h, const rt_tonemap_uniform &tm,
+                          const void *gbuffer, hipStream_t s);
+// First-hit albedo of the pixel-centre camera ray (rgb; w = 1 on a hit, 0 on a miss).
+hipError_t launch_albedo(const TraceInputs &t, uint32_t mode, const rt_triangle *tris, const rt_material *mats,
+                         const rt_camera_uniform &cam, uint32_t w, uint32_t h, void *out, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_accum64_add(const void *accum_f4, double *sum, size_t n_px, hipStream_t s);
 hipError_t launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s);

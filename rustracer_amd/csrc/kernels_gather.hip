@@ -831,6 +831,34 @@ __global__ __launch_bounds__(64) void trace_rays_kernel(TraceScene ts, const rt_
     out[i].prim_id = h.prim;
 }
 
+// First-hit albedo along the pixel-centre ray (zero jitter): a guide image for
+// the learned denoiser, not part of the reference's gather.
+template <int MODE>
+__global__ __launch_bounds__(64) void albedo_kernel(TraceScene ts, const rt_triangle *tris, const rt_material *mats,
+                                                    rt_camera_uniform cam, uint32_t w, uint32_t h, float4 *out) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= w * h) return;
+    uint32_t spill_words[kSpillAlloc];
+    LaneStack stk;
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
+    stk.stride = 64u;
+    TraceCount cnt{0u, 0u};
+    const uint32_t px = i % w, py = i / w;
+    const float ux = (float)px * cam.inv_half_res[0] - 1.0f;
+    const float uy = 1.0f - (float)py * cam.inv_half_res[1];
+    const F3 ro = ld3(cam.position);
+    const F3 rd = normalize(ld3(cam.forward) + ld3(cam.right) * ux + ld3(cam.up) * uy);
+    const Hit hit = trace_mode<MODE, false, false>(ts, ro, rd, kInf, stk, cnt);
+    float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (hit.t < kInf) {
+        const rt_material &m = mats[tris[hit.prim].material_id];
+        o = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], 1.0f);
+    }
+    out[i] = o;
+}
+
 TraceScene make_trace_scene(const rtk::TraceInputs &t) {
     TraceScene s;
     s.nodes = t.nodes;
@@ -943,6 +971,19 @@ hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipS
     else if (g.mode == 2) { if (g.early_shadow) RT_BATCH(2, true); else RT_BATCH(2, false); }
     else { if (g.early_shadow) RT_BATCH(3, true); else RT_BATCH(3, false); }
 #undef RT_BATCH
+    return hipGetLastError();
+}
+
+hipError_t launch_albedo(const TraceInputs &t, uint32_t mode, const rt_triangle *tris, const rt_material *mats,
+                         const rt_camera_uniform &cam, uint32_t w, uint32_t h, void *out, hipStream_t s) {
+    const uint32_t n = w * h;
+    if (n == 0) return hipSuccess;
+    const rtd::TraceScene ts = rtd::make_trace_scene(t);
+    float4 *o = reinterpret_cast<float4 *>(out);
+    const dim3 grid((n + 63u) / 64u), block(64);
+    if (mode == 0) hipLaunchKernelGGL((rtd::albedo_kernel<0>), grid, block, 0, s, ts, tris, mats, cam, w, h, o);
+    else if (mode == 3) hipLaunchKernelGGL((rtd::albedo_kernel<3>), grid, block, 0, s, ts, tris, mats, cam, w, h, o);
+    else hipLaunchKernelGGL((rtd::albedo_kernel<1>), grid, block, 0, s, ts, tris, mats, cam, w, h, o);
     return hipGetLastError();
 }
 

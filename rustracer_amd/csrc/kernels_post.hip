@@ -151,20 +151,10 @@ __global__ __launch_bounds__(256) void denoise_global_kernel(const float4 *__res
     out[idx] = make_float4(o.x, o.y, o.z, 1.0f);
 }
 
-// composite.wgsl:16-26 then tonemap.wgsl:19-50 and the rgba8unorm store
-// (clamp to [0,1], scale by 255, round half up). One pass over the pixels.
-__global__ __launch_bounds__(256) void composite_tonemap_kernel(const float4 *__restrict__ frame,
-                                                                float4 *__restrict__ accum, uint32_t *__restrict__ rgba8,
-                                                                uint32_t n_px, uint32_t width, float alpha,
-                                                                rt_tonemap_uniform tm, const float4 *__restrict__ gbuf) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n_px) return;
-    const float4 fresh = frame[i];
-    const float4 prev = accum[i];
-    const float om = 1.0f - alpha;
-    const float4 a = make_float4(prev.x * om + fresh.x * alpha, prev.y * om + fresh.y * alpha,
-                                 prev.z * om + fresh.z * alpha, prev.w * om + fresh.w * alpha);
-    accum[i] = a;
+// tonemap.wgsl:19-50 for one pixel and the rgba8unorm store (clamp to [0,1],
+// scale by 255, round half up).
+__device__ __forceinline__ uint32_t tonemap_pixel(float4 a, uint32_t i, uint32_t width, const rt_tonemap_uniform &tm,
+                                                  const float4 *__restrict__ gbuf) {
     const F3 exposed = xyz(a) * tm.exposure;
     const F3 mapped = exposed / (exposed + fs(1.0f));
     const float inv_gamma = 1.0f / tm.gamma;
@@ -190,7 +180,32 @@ __global__ __launch_bounds__(256) void composite_tonemap_kernel(const float4 *__
         const float v = wclamp(c4[k], 0.0f, 1.0f);
         packed |= ((uint32_t)floorf(v * 255.0f + 0.5f) & 0xFFu) << (8 * k);
     }
-    rgba8[i] = packed;
+    return packed;
+}
+
+// composite.wgsl:16-26 then the tonemap of the new accumulation. One pass over the pixels.
+__global__ __launch_bounds__(256) void composite_tonemap_kernel(const float4 *__restrict__ frame,
+                                                                float4 *__restrict__ accum, uint32_t *__restrict__ rgba8,
+                                                                uint32_t n_px, uint32_t width, float alpha,
+                                                                rt_tonemap_uniform tm, const float4 *__restrict__ gbuf) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n_px) return;
+    const float4 fresh = frame[i];
+    const float4 prev = accum[i];
+    const float om = 1.0f - alpha;
+    const float4 a = make_float4(prev.x * om + fresh.x * alpha, prev.y * om + fresh.y * alpha,
+                                 prev.z * om + fresh.z * alpha, prev.w * om + fresh.w * alpha);
+    accum[i] = a;
+    rgba8[i] = tonemap_pixel(a, i, width, tm, gbuf);
+}
+
+// The tonemap alone, of another displayed image (the learned denoiser's output).
+__global__ __launch_bounds__(256) void tonemap_kernel(const float4 *__restrict__ img, uint32_t *__restrict__ rgba8,
+                                                      uint32_t n_px, uint32_t width, rt_tonemap_uniform tm,
+                                                      const float4 *__restrict__ gbuf) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n_px) return;
+    rgba8[i] = tonemap_pixel(img[i], i, width, tm, gbuf);
 }
 
 __global__ __launch_bounds__(256) void accum64_add_kernel(const float4 *__restrict__ acc, double *__restrict__ sum,
@@ -257,6 +272,15 @@ __global__ __launch_bounds__(256) void hbm_copy_kernel(const float4 *__restrict_
 }  // namespace rtd
 
 namespace rtk {
+
+hipError_t launch_tonemap(const void *img, uint32_t *rgba8, uint32_t w, uint32_t h, const rt_tonemap_uniform &tm,
+                          const void *gbuffer, hipStream_t s) {
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(rtd::tonemap_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s,
+                       reinterpret_cast<const float4 *>(img), rgba8, n, w, tm,
+                       reinterpret_cast<const float4 *>(gbuffer));
+    return hipGetLastError();
+}
 
 size_t relmse_scratch_doubles() { return rtd::kRelBlocks + 1; }
 

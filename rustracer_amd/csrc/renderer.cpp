@@ -15,6 +15,7 @@
 #include "capi_util.h"
 #include "kernels.h"
 #include "scene.h"
+#include "unet.h"
 
 namespace {
 
@@ -98,6 +99,11 @@ struct rt_renderer {
     double *d_relmse = nullptr;  // reduction scratch of rt_accum64_relmse
     void *d_wave = nullptr;  // path-queue state of the camera pass
     size_t wave_cap = 0;
+    // learned denoiser: attached network, its output image, the albedo guide
+    rt_unet *unet = nullptr;
+    float *d_display = nullptr;
+    float *d_albedo = nullptr;
+    bool albedo_ok = false;
     uint64_t acc64_frames = 0;
 
     // many-view batch
@@ -142,7 +148,7 @@ struct rt_renderer {
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
-                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse};
+                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -329,8 +335,38 @@ struct rt_renderer {
                                                 stream),
                   "composite_tonemap");
         end(RT_PASS_COMPOSITE_TONEMAP, ev);
+        if (unet) {
+            begin(RT_PASS_NETWORK, ev);
+            ensure_albedo();
+            rtu::denoise(unet, frame_views(), d_display, stream);
+            hip_check(rtk::launch_tonemap(d_display, d_rgba8, cfg.width, cfg.height, tm, d_gbuf, stream), "tonemap");
+            end(RT_PASS_NETWORK, ev);
+        }
         hip_check(hipEventRecord(ev_free[si], stream), "record set release");
         free_recorded[si] = true;
+    }
+
+    // First-hit albedo of the pixel-centre rays, recomputed after a camera change.
+    void ensure_albedo() {
+        if (!d_albedo) {
+            hip_check(hipMalloc(&d_albedo, n_px() * 16), "hipMalloc albedo");
+            albedo_ok = false;
+        }
+        if (albedo_ok) return;
+        hip_check(rtk::launch_albedo(trace_inputs(), traversal_mode(), d_tris, d_mats, cam, cfg.width, cfg.height,
+                                     d_albedo, stream),
+                  "albedo");
+        albedo_ok = true;
+    }
+    rtu::FrameViews frame_views() const {
+        rtu::FrameViews v;
+        v.accum = d_accum;
+        v.gbuf = d_gbuf;
+        v.albedo = d_albedo;
+        v.width = cfg.width;
+        v.height = cfg.height;
+        v.frames = std::min(frames_since_reset, cfg.accumulate_frames ? cfg.accumulate_frames : 1u);
+        return v;
     }
 
     uint32_t traced_photons() const {
@@ -839,6 +875,7 @@ int rt_update_camera(rt_renderer *r, const rt_camera *cam) {
     return api_call([&] {
         if (!r || !cam) throw ApiError(RT_ERR_INVALID, "null argument");
         r->cam = rt::camera_to_uniform(*cam, r->cfg.width, r->cfg.height);
+        r->albedo_ok = false;
     });
 }
 
@@ -846,6 +883,7 @@ int rt_update_camera_uniform(rt_renderer *r, const rt_camera_uniform *u) {
     return api_call([&] {
         if (!r || !u) throw ApiError(RT_ERR_INVALID, "null argument");
         r->cam = *u;
+        r->albedo_ok = false;
     });
 }
 
@@ -1353,6 +1391,38 @@ int rt_accum64_relmse(rt_renderer *r, const void *d_image, float eps, double *re
     });
 }
 
+int rt_accum64_relmse_display(rt_renderer *r, float eps, double *relmse) {
+    if (!r) return rt_accum64_relmse(r, nullptr, eps, relmse);
+    return rt_accum64_relmse(r, r->unet ? r->d_display : nullptr, eps, relmse);
+}
+
+int rt_renderer_set_denoiser(rt_renderer *r, rt_unet *u) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        if (u && (r->cfg.width % 2 || r->cfg.height % 2))
+            throw ApiError(RT_ERR_INVALID, "the denoiser needs even width and height");
+        if (u && !r->d_display) {
+            r->d_display = dev_alloc<float>(r->n_px() * 4, "display");
+            hip_check(hipMemset(r->d_display, 0, r->n_px() * 16), "memset");
+        }
+        r->unet = u;
+    });
+}
+
+int rt_read_display(rt_renderer *r, float *rgba, size_t n) {
+    return read_px4(r, rgba, n, r ? (r->unet ? r->d_display : r->d_accum) : nullptr);
+}
+
+int rt_read_albedo(rt_renderer *r, float *rgba, size_t n) {
+    return api_call([&] {
+        if (!r || !rgba) throw ApiError(RT_ERR_INVALID, "null argument");
+        r->ensure_albedo();
+        if (n < r->n_px() * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than width*height*4");
+        read_dev(r, rgba, r->d_albedo, r->n_px() * 16);
+    });
+}
+
 int rt_accum64_read_mean(rt_renderer *r, double *rgba, size_t n, uint64_t *n_frames) {
     return api_call([&] {
         if (!r || !rgba) throw ApiError(RT_ERR_INVALID, "null argument");
@@ -1366,3 +1436,14 @@ int rt_accum64_read_mean(rt_renderer *r, double *rgba, size_t n, uint64_t *n_fra
 }
 
 }  // extern "C"
+
+namespace rtu {
+
+FrameViews frame_views(rt_renderer *r) {
+    if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+    r->ensure_albedo();
+    hip_check(hipStreamSynchronize(r->stream), "sync");
+    return r->frame_views();
+}
+
+}  // namespace rtu

@@ -1,0 +1,877 @@
+// unet.hip — the learned denoiser: a small convolutional U-Net on the displayed
+// (accumulated) image, with hand-written BF16 MFMA convolution kernels for the
+// forward pass and both backward passes, FP32 accumulation, and Adam, for gfx950.
+//
+// This part has no counterpart in the reference renderer; its inputs are the
+// renderer's own buffers (accumulation, G-buffer) plus a first-hit albedo pass.
+//
+// Network (NHWC bf16 activations, channel counts multiples of 16/32):
+//   F  (16: log1p rgb, albedo, normal, depth, sample count, 5 zero)
+//   A1 = relu(conv3x3(F, 16->32))           A2 = relu(conv3x3(A1, 32->32)) -> CAT[0:32]
+//   P  = avgpool2(A2)                       B1 = relu(conv3x3(P, 32->64))  B2 = relu(conv3x3(B1, 64->64))
+//   CAT[32:96] = upsample2(B2)              D1 = relu(conv3x3(CAT, 96->32)) D2 = relu(conv3x3(D1, 32->32))
+//   pred = log1p(noisy) + conv1x1(D2, 32->3) + b   (FP32), image = expm1(pred)
+// Loss: mean over pixels and rgb of (pred - log1p(target))^2.
+//
+// Convolution as implicit GEMM on v_mfma_f32_32x32x16_bf16 (one wavefront per
+// workgroup): M = 32 consecutive pixels of one row, N = 32 output channels,
+// K = 9 taps x input channels in steps of 16. Lane layouts (checked on the
+// device): A lane l holds row l%32, k = (l/32)*8 .. +7; B lane l holds column
+// l%32, the same k; accumulator element e of lane l is row (e/4)*8 + (l/32)*4 +
+// e%4, column l%32.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/rustracer_hip.h"
+#include "capi_util.h"
+#include "unet.h"
+
+namespace rtu {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef uint16_t bf16;  // storage type
+
+__device__ __forceinline__ bf16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16 b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ bf16x8 load8(const bf16 *p) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
+
+// One 3x3 convolution (stride 1, zero padding), optionally with bias and ReLU,
+// or a mask (x > 0) of another tensor at the output position (the ReLU
+// derivative in the backward-data pass). X: [N][H][W][xs] with cin channels at
+// offset 0; Wt: [9][cout][cin]; Y: [N][H][W][ys], cout channels at offset yoff.
+// Grid: (ceil(W/32), H, N * cout/32).
+__global__ __launch_bounds__(64) void conv3x3_kernel(const bf16 *__restrict__ X, int xs, int cin,
+                                                     const bf16 *__restrict__ Wt, const float *__restrict__ bias,
+                                                     bf16 *__restrict__ Y, int ys, int yoff, int cout, int relu,
+                                                     const bf16 *__restrict__ mask, int ms, int H, int W) {
+    const int l = threadIdx.x;
+    const int x0 = blockIdx.x * 32;
+    const int y = blockIdx.y;
+    const int nco = cout / 32;
+    const int n = blockIdx.z / nco;
+    const int co0 = (blockIdx.z % nco) * 32;
+    const int r = l % 32, kg = (l / 32) * 8;
+    floatx16 acc = {};
+    for (int tap = 0; tap < 9; tap++) {
+        const int yy = y + tap / 3 - 1;
+        const int xx = x0 + r + tap % 3 - 1;
+        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const bf16 *xp = X + ((size_t)(n * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * xs + kg;
+        const bf16 *wp = Wt + ((size_t)tap * cout + co0 + r) * cin + kg;
+        for (int c0 = 0; c0 < cin; c0 += 16) {
+            const bf16x8 a = ok ? load8(xp + c0) : zero8();
+            const bf16x8 b = load8(wp + c0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+    }
+    const int co = co0 + r;
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int x = x0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
+        if (x >= W) continue;
+        const size_t p = (size_t)(n * H + y) * W + x;
+        float v = acc[e] + bv;
+        if (relu) v = fmaxf(v, 0.0f);
+        if (mask && !(bf2f(mask[p * ms + co]) > 0.0f)) v = 0.0f;
+        Y[p * ys + yoff + co] = f2bf(v);
+    }
+}
+
+// Weight gradient of a 3x3 convolution: gW[tap][co][ci] += sum over pixels of
+// G[p][co] * X[p + tap][ci]. Each workgroup takes one (tap, 32 co, 32 ci) block
+// and a strided set of 32-pixel row segments; G and X segments are staged
+// transposed in LDS ([channel][pixel]) so MFMA fragments are 16-byte reads.
+// Grid: (segment groups, 9, cout/32 * ceil(cin/32)).
+__global__ __launch_bounds__(64) void conv3x3_wgrad_kernel(const bf16 *__restrict__ G, int gs, int cout,
+                                                           const bf16 *__restrict__ X, int xs, int cin,
+                                                           float *__restrict__ gW, int N, int H, int W) {
+    __shared__ __attribute__((aligned(16))) bf16 gt[32][32];  // [co][pixel]
+    __shared__ __attribute__((aligned(16))) bf16 xt[32][32];  // [ci][pixel]
+    const int l = threadIdx.x;
+    const int tap = blockIdx.y;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int nci = (cin + 31) / 32;
+    const int co0 = (blockIdx.z / nci) * 32, ci0 = (blockIdx.z % nci) * 32;
+    const int segs_x = (W + 31) / 32;
+    const int total = N * H * segs_x;
+    const int lp = l / 2, lh = (l % 2) * 16;  // staging: pixel, 16-channel half
+    floatx16 acc = {};
+    for (int s = blockIdx.x; s < total; s += gridDim.x) {
+        const int sx = s % segs_x, ny = s / segs_x;
+        const int y = ny % H, n = ny / H;
+        const int x = sx * 32 + lp;
+        {
+            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+            if (x < W) {
+                const bf16 *gp = G + ((size_t)(n * H + y) * W + x) * gs + co0 + lh;
+                v0 = *reinterpret_cast<const uint4 *>(gp);
+                v1 = *reinterpret_cast<const uint4 *>(gp + 8);
+            }
+            const bf16 *e0 = reinterpret_cast<const bf16 *>(&v0), *e1 = reinterpret_cast<const bf16 *>(&v1);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                gt[lh + j][lp] = e0[j];
+                gt[lh + 8 + j][lp] = e1[j];
+            }
+        }
+        {
+            const int yy = y + dy, xx = x + dx;
+            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
+            if (x < W && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const bf16 *xp = X + ((size_t)(n * H + yy) * W + xx) * xs;
+                if (ci0 + lh < cin) v0 = *reinterpret_cast<const uint4 *>(xp + ci0 + lh);
+                if (ci0 + lh + 8 < cin) v1 = *reinterpret_cast<const uint4 *>(xp + ci0 + lh + 8);
+            }
+            const bf16 *e0 = reinterpret_cast<const bf16 *>(&v0), *e1 = reinterpret_cast<const bf16 *>(&v1);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                xt[lh + j][lp] = e0[j];
+                xt[lh + 8 + j][lp] = e1[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const bf16x8 a = load8(&gt[l % 32][ks * 16 + (l / 32) * 8]);
+            const bf16x8 b = load8(&xt[l % 32][ks * 16 + (l / 32) * 8]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int ci = ci0 + l % 32;
+    if (ci >= cin) return;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int co = co0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
+        atomicAdd(&gW[((size_t)tap * cout + co) * cin + ci], acc[e]);
+    }
+}
+
+// Bias gradient: gb[c] += sum over pixels of G[p][c]. Grid-stride over pixels.
+__global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ G, int gs, int c, size_t n_px,
+                                                        float *__restrict__ gb) {
+    const int t = threadIdx.x % 32, q = threadIdx.x / 32;  // 8 pixel lanes x 32 channels
+    for (int c0 = 0; c0 < c; c0 += 32) {
+        float s = 0.0f;
+        for (size_t p = (size_t)blockIdx.x * 8 + q; p < n_px; p += (size_t)gridDim.x * 8)
+            if (c0 + t < c) s += bf2f(G[p * gs + c0 + t]);
+        if (c0 + t < c) atomicAdd(&gb[c0 + t], s);
+    }
+}
+
+// 2x2 average pool of channels [0, c) of X (stride xs) into P (stride c), half resolution.
+__global__ __launch_bounds__(256) void pool_kernel(const bf16 *__restrict__ X, int xs, int c, bf16 *__restrict__ P,
+                                                   int N, int H, int W) {
+    const int h2 = H / 2, w2 = W / 2;
+    const size_t total = (size_t)N * h2 * w2 * c;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const size_t q = i / c;
+        const int x = (int)(q % w2), y = (int)((q / w2) % h2), n = (int)(q / ((size_t)w2 * h2));
+        const size_t p00 = ((size_t)(n * H + 2 * y) * W + 2 * x) * xs + ch;
+        const float s = bf2f(X[p00]) + bf2f(X[p00 + xs]) + bf2f(X[p00 + (size_t)W * xs]) + bf2f(X[p00 + (size_t)W * xs + xs]);
+        P[i] = f2bf(0.25f * s);
+    }
+}
+
+// Nearest 2x upsample of B (stride c, half resolution) into channels [off, off+c) of Y (stride ys).
+__global__ __launch_bounds__(256) void upsample_kernel(const bf16 *__restrict__ B, int c, bf16 *__restrict__ Y, int ys,
+                                                       int off, int N, int H, int W) {
+    const size_t total = (size_t)N * H * W * c;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const size_t p = i / c;
+        const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+        Y[p * ys + off + ch] = B[((size_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * c + ch];
+    }
+}
+
+// Backward of the pool into A2's gradient: gA[p][ch] = (gCat[p][ch] + gP[p/2][ch] / 4) * (A[p][ch] > 0).
+// gCat already carries the mask; the pool branch gets it here.
+__global__ __launch_bounds__(256) void pool_back_kernel(const bf16 *__restrict__ gcat, int cs, const bf16 *__restrict__ gp,
+                                                        const bf16 *__restrict__ A, int as, int c, bf16 *__restrict__ ga,
+                                                        int N, int H, int W) {
+    const size_t total = (size_t)N * H * W * c;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const size_t p = i / c;
+        const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
+        const float fromp = 0.25f * bf2f(gp[((size_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * c + ch]);
+        const float m = bf2f(A[p * as + ch]) > 0.0f ? 1.0f : 0.0f;
+        ga[i] = f2bf(bf2f(gcat[p * cs + ch]) + fromp * m);
+    }
+}
+
+// Backward of the nearest upsample: gB[q][ch] = sum of the 4 children's gY[p][off + ch].
+__global__ __launch_bounds__(256) void upsample_back_kernel(const bf16 *__restrict__ gy, int ys, int off, int c,
+                                                            bf16 *__restrict__ gb, int N, int H, int W) {
+    const int h2 = H / 2, w2 = W / 2;
+    const size_t total = (size_t)N * h2 * w2 * c;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const size_t q = i / c;
+        const int x = (int)(q % w2), y = (int)((q / w2) % h2), n = (int)(q / ((size_t)w2 * h2));
+        const size_t p00 = ((size_t)(n * H + 2 * y) * W + 2 * x) * ys + off + ch;
+        const float s = bf2f(gy[p00]) + bf2f(gy[p00 + ys]) + bf2f(gy[p00 + (size_t)W * ys]) + bf2f(gy[p00 + (size_t)W * ys + ys]);
+        gb[i] = f2bf(s);
+    }
+}
+
+// Output layer (1x1, 32 -> 3, FP32) plus the residual: pred = noisy + b + W·D.
+// With a target: loss partial sums, and the backward pass: gD = (W^T dpred) * (D > 0)
+// as bf16, gW += dpred D^T, gb += dpred (block reductions, then atomics).
+constexpr int kOutC = 32;
+__global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__ D, const float *__restrict__ noisy,
+                                                        const float *__restrict__ w, const float *__restrict__ b,
+                                                        float *__restrict__ pred, const float *__restrict__ target,
+                                                        size_t n_px, float grad_scale, bf16 *__restrict__ gd,
+                                                        float *__restrict__ gw, float *__restrict__ gbias,
+                                                        double *__restrict__ loss) {
+    __shared__ float red[256];
+    __shared__ float gacc[3 * kOutC + 3];
+    for (int i = threadIdx.x; i < 3 * kOutC + 3; i += 256) gacc[i] = 0.0f;
+    __syncthreads();
+    float lsum = 0.0f;
+    float lw[3 * kOutC + 3];
+    if (target)
+        for (int i = 0; i < 3 * kOutC + 3; i++) lw[i] = 0.0f;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < n_px; p += (size_t)gridDim.x * 256) {
+        float d[kOutC];
+#pragma unroll
+        for (int i = 0; i < kOutC; i += 8) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(D + p * kOutC + i);
+            const bf16 *e = reinterpret_cast<const bf16 *>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; j++) d[i + j] = bf2f(e[j]);
+        }
+        float o[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float s = b[c];
+#pragma unroll
+            for (int i = 0; i < kOutC; i++) s += w[c * kOutC + i] * d[i];
+            o[c] = noisy[p * 3 + c] + s;
+            pred[p * 3 + c] = o[c];
+        }
+        if (target) {
+            float g[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float diff = o[c] - target[p * 3 + c];
+                lsum += diff * diff;
+                g[c] = grad_scale * diff;
+                lw[3 * kOutC + c] += g[c];
+            }
+#pragma unroll
+            for (int i = 0; i < kOutC; i += 8) {
+                uint32_t packed[4];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int k = i + j;
+                    float s = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        s += w[c * kOutC + k] * g[c];
+                        lw[c * kOutC + k] += g[c] * d[k];
+                    }
+                    const uint32_t h = f2bf(d[k] > 0.0f ? s : 0.0f);
+                    if (j % 2 == 0) packed[j / 2] = h;
+                    else packed[j / 2] |= h << 16;
+                }
+                *reinterpret_cast<uint4 *>(gd + p * kOutC + i) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+            }
+        }
+    }
+    if (!target) return;
+    for (int i = 0; i < 3 * kOutC + 3; i++) atomicAdd(&gacc[i], lw[i]);
+    red[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
+    for (int i = threadIdx.x; i < 3 * kOutC; i += 256) atomicAdd(&gw[i], gacc[i]);
+    if (threadIdx.x < 3) atomicAdd(&gbias[threadIdx.x], gacc[3 * kOutC + threadIdx.x]);
+}
+
+// Features from the renderer's buffers: accumulation (float4), G-buffer
+// (normal, depth), albedo (float4), sample count. F: 16 bf16 per pixel; noisy:
+// log1p of the accumulation (FP32 rgb).
+__global__ __launch_bounds__(256) void features_kernel(const float4 *__restrict__ accum, const float4 *__restrict__ gbuf,
+                                                       const float4 *__restrict__ albedo, float frames_feature,
+                                                       size_t n_px, bf16 *__restrict__ F, float *__restrict__ noisy) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n_px) return;
+    const float4 a = accum[p], g = gbuf[p], al = albedo[p];
+    const float lr = log1pf(fmaxf(a.x, 0.0f)), lg = log1pf(fmaxf(a.y, 0.0f)), lb = log1pf(fmaxf(a.z, 0.0f));
+    noisy[p * 3 + 0] = lr;
+    noisy[p * 3 + 1] = lg;
+    noisy[p * 3 + 2] = lb;
+    const float depth = g.w < 1e29f ? log1pf(fminf(fmaxf(g.w, 0.0f), 1e4f)) * 0.25f : -1.0f;
+    const float f[16] = {lr, lg, lb, al.x, al.y, al.z, g.x, g.y, g.z, depth, frames_feature, 0, 0, 0, 0, 0};
+    uint32_t o[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = (uint32_t)f2bf(f[2 * i]) | ((uint32_t)f2bf(f[2 * i + 1]) << 16);
+    uint4 *dst = reinterpret_cast<uint4 *>(F + p * 16);
+    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// log1p of a float4 image's rgb (training targets).
+__global__ __launch_bounds__(256) void log_target_kernel(const float4 *__restrict__ img, size_t n_px,
+                                                         float *__restrict__ out) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n_px) return;
+    const float4 a = img[p];
+    out[p * 3 + 0] = log1pf(fmaxf(a.x, 0.0f));
+    out[p * 3 + 1] = log1pf(fmaxf(a.y, 0.0f));
+    out[p * 3 + 2] = log1pf(fmaxf(a.z, 0.0f));
+}
+
+__global__ __launch_bounds__(256) void image_kernel(const float *__restrict__ pred, size_t n_px, float4 *__restrict__ out) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= n_px) return;
+    out[p] = make_float4(expm1f(pred[p * 3]), expm1f(pred[p * 3 + 1]), expm1f(pred[p * 3 + 2]), 1.0f);
+}
+
+// Crops of the training set into a batch: sample k of the batch is the
+// (cw x ch) window at (ox[k], oy[k]) of image img[k] (all images iw x ih).
+__global__ __launch_bounds__(256) void crop_kernel(const bf16 *__restrict__ F, const float *__restrict__ noisy,
+                                                   const float *__restrict__ target, const int *__restrict__ sel,
+                                                   int iw, int ih, int cw, int chh, bf16 *__restrict__ bF,
+                                                   float *__restrict__ bnoisy, float *__restrict__ btarget) {
+    const int k = blockIdx.y;
+    const int img = sel[3 * k], ox = sel[3 * k + 1], oy = sel[3 * k + 2];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < cw * chh; i += gridDim.x * 256) {
+        const int x = i % cw, y = i / cw;
+        const size_t src = ((size_t)img * ih + oy + y) * iw + ox + x;
+        const size_t dst = (size_t)k * cw * chh + i;
+        const uint4 *s = reinterpret_cast<const uint4 *>(F + src * 16);
+        uint4 *d = reinterpret_cast<uint4 *>(bF + dst * 16);
+        d[0] = s[0];
+        d[1] = s[1];
+        for (int c = 0; c < 3; c++) {
+            bnoisy[dst * 3 + c] = noisy[src * 3 + c];
+            btarget[dst * 3 + c] = target[src * 3 + c];
+        }
+    }
+}
+
+// Adam on an FP32 parameter array, then the bf16 copies: forward layout
+// [tap][co][ci] (same as the master) and, when wt is given, the backward-data
+// layout wt[8 - tap][ci][co] (flipped taps, transposed channels).
+__global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ p, const float *__restrict__ g,
+                                                   float *__restrict__ m, float *__restrict__ v, size_t n, float lr,
+                                                   float b1, float b2, float c1, float c2, float eps,
+                                                   bf16 *__restrict__ wf, bf16 *__restrict__ wt, int cout, int cin) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] = pi;
+        if (wf) {
+            const bf16 h = f2bf(pi);
+            wf[i] = h;
+            if (wt) {
+                const int ci = (int)(i % cin), co = (int)((i / cin) % cout), tap = (int)(i / ((size_t)cin * cout));
+                wt[((size_t)(8 - tap) * cin + ci) * cout + co] = h;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void weights_bf16_kernel(const float *__restrict__ p, size_t n, bf16 *__restrict__ wf,
+                                                           bf16 *__restrict__ wt, int cout, int cin) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const bf16 h = f2bf(p[i]);
+        wf[i] = h;
+        if (wt) {
+            const int ci = (int)(i % cin), co = (int)((i / cin) % cout), tap = (int)(i / ((size_t)cin * cout));
+            wt[((size_t)(8 - tap) * cin + ci) * cout + co] = h;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void zero_kernel(float *__restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0f;
+}
+
+}  // namespace rtu
+
+// ─────────────────────────────── host side ───────────────────────────────
+
+namespace {
+
+using rt::ApiError;
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw ApiError(RT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T *dalloc(size_t count, const char *what) {
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), what);
+    return static_cast<T *>(p);
+}
+
+unsigned blocks_for(size_t n, unsigned cap = 8192) { return (unsigned)std::min<size_t>((n + 255) / 256, cap); }
+
+struct ConvLayer {
+    int cin = 0, cout = 0;
+    size_t nw() const { return (size_t)9 * cin * cout; }
+};
+// conv1..conv6 of the network described at the top of this file
+const ConvLayer kLayers[6] = {{16, 32}, {32, 32}, {32, 64}, {64, 64}, {96, 32}, {32, 32}};
+constexpr int kFeat = 16;
+
+// Activations (and their gradients) of one batch shape.
+struct Acts {
+    int N = 0, H = 0, W = 0;
+    size_t px() const { return (size_t)N * H * W; }
+    rtu::bf16 *F = nullptr, *A1 = nullptr, *CAT = nullptr, *P = nullptr, *B1 = nullptr, *B2 = nullptr, *D1 = nullptr,
+              *D2 = nullptr;
+    float *noisy = nullptr, *target = nullptr, *pred = nullptr;
+    // gradients
+    rtu::bf16 *gD2 = nullptr, *gD1 = nullptr, *gCAT = nullptr, *gB2 = nullptr, *gB1 = nullptr, *gP = nullptr,
+              *gA2 = nullptr, *gA1 = nullptr;
+    bool grads = false;
+    void release() {
+        void *all[] = {F, A1, CAT, P, B1, B2, D1, D2, noisy, target, pred, gD2, gD1, gCAT, gB2, gB1, gP, gA2, gA1};
+        for (void *p : all)
+            if (p) (void)hipFree(p);
+        *this = Acts{};
+    }
+    void ensure(int n, int h, int w, bool with_grads) {
+        if (n == N && h == H && w == W && (grads || !with_grads)) return;
+        release();
+        N = n;
+        H = h;
+        W = w;
+        grads = with_grads;
+        const size_t P1 = px(), P2 = P1 / 4;
+        F = dalloc<rtu::bf16>(P1 * 16, "unet F");
+        A1 = dalloc<rtu::bf16>(P1 * 32, "unet A1");
+        CAT = dalloc<rtu::bf16>(P1 * 96, "unet CAT");
+        P = dalloc<rtu::bf16>(P2 * 32, "unet P");
+        B1 = dalloc<rtu::bf16>(P2 * 64, "unet B1");
+        B2 = dalloc<rtu::bf16>(P2 * 64, "unet B2");
+        D1 = dalloc<rtu::bf16>(P1 * 32, "unet D1");
+        D2 = dalloc<rtu::bf16>(P1 * 32, "unet D2");
+        noisy = dalloc<float>(P1 * 3, "unet noisy");
+        target = dalloc<float>(P1 * 3, "unet target");
+        pred = dalloc<float>(P1 * 3, "unet pred");
+        if (with_grads) {
+            gD2 = dalloc<rtu::bf16>(P1 * 32, "unet gD2");
+            gD1 = dalloc<rtu::bf16>(P1 * 32, "unet gD1");
+            gCAT = dalloc<rtu::bf16>(P1 * 96, "unet gCAT");
+            gB2 = dalloc<rtu::bf16>(P2 * 64, "unet gB2");
+            gB1 = dalloc<rtu::bf16>(P2 * 64, "unet gB1");
+            gP = dalloc<rtu::bf16>(P2 * 32, "unet gP");
+            gA2 = dalloc<rtu::bf16>(P1 * 32, "unet gA2");
+            gA1 = dalloc<rtu::bf16>(P1 * 32, "unet gA1");
+        }
+    }
+};
+
+}  // namespace
+
+struct rt_unet {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // FP32 master parameters: conv weights [9][cout][cin] and biases per layer,
+    // then the output layer's w[3][32] and b[3]; one contiguous array.
+    size_t n_params = 0;
+    size_t w_off[6] = {}, b_off[6] = {}, ow_off = 0, ob_off = 0;
+    float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
+    rtu::bf16 *d_wf[6] = {}, *d_wt[6] = {};
+    uint64_t adam_t = 0;
+    Acts inf, train;
+    // training set
+    uint32_t ds_w = 0, ds_h = 0, ds_cap = 0;
+    rtu::bf16 *ds_F = nullptr;
+    float *ds_noisy = nullptr, *ds_target = nullptr;
+    std::vector<uint8_t> ds_has_in, ds_has_tgt;
+    int *d_sel = nullptr;
+    size_t sel_cap = 0;
+    double *d_loss = nullptr;
+    size_t loss_cap = 0;
+
+    ~rt_unet() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        inf.release();
+        train.release();
+        void *all[] = {d_p, d_g, d_m, d_v, ds_F, ds_noisy, ds_target, d_sel, d_loss};
+        for (void *p : all)
+            if (p) (void)hipFree(p);
+        for (int k = 0; k < 6; k++) {
+            if (d_wf[k]) (void)hipFree(d_wf[k]);
+            if (d_wt[k]) (void)hipFree(d_wt[k]);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void refresh_bf16() {
+        for (int k = 0; k < 6; k++) {
+            const ConvLayer &L = kLayers[k];
+            hipLaunchKernelGGL(rtu::weights_bf16_kernel, dim3(blocks_for(L.nw())), dim3(256), 0, stream,
+                               d_p + w_off[k], L.nw(), d_wf[k], d_wt[k], L.cout, L.cin);
+        }
+        hip_check(hipGetLastError(), "weights to bf16");
+    }
+
+    void conv(const rtu::bf16 *X, int xs, int k, bool bwd, rtu::bf16 *Y, int ys, int yoff, bool relu,
+              const rtu::bf16 *mask, int ms, int N, int H, int W) {
+        const ConvLayer &L = kLayers[k];
+        const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
+        const dim3 grid((W + 31) / 32, H, N * (cout / 32));
+        hipLaunchKernelGGL(rtu::conv3x3_kernel, grid, dim3(64), 0, stream, X, xs, cin, bwd ? d_wt[k] : d_wf[k],
+                           bwd ? nullptr : d_p + b_off[k], Y, ys, yoff, cout, relu ? 1 : 0, mask, ms, H, W);
+    }
+    void wgrad(const rtu::bf16 *G, int k, const rtu::bf16 *X, int xs, int N, int H, int W) {
+        const ConvLayer &L = kLayers[k];
+        const int segs = N * H * ((W + 31) / 32);
+        const dim3 grid((unsigned)std::min(segs, 96), 9, (L.cout / 32) * ((L.cin + 31) / 32));
+        hipLaunchKernelGGL(rtu::conv3x3_wgrad_kernel, grid, dim3(64), 0, stream, G, L.cout, L.cout, X, xs, L.cin,
+                           d_g + w_off[k], N, H, W);
+        hipLaunchKernelGGL(rtu::bias_grad_kernel, dim3(64), dim3(256), 0, stream, G, L.cout, L.cout,
+                           (size_t)N * H * W, d_g + b_off[k]);
+    }
+
+    // Forward pass over a.F / a.noisy; with `target` the loss goes to loss_slot
+    // and the output layer's backward pass fills a.gD2 and the output-layer grads.
+    void forward(Acts &a, bool with_target, double *loss_slot) {
+        const int N = a.N, H = a.H, W = a.W, H2 = H / 2, W2 = W / 2;
+        conv(a.F, 16, 0, false, a.A1, 32, 0, true, nullptr, 0, N, H, W);
+        conv(a.A1, 32, 1, false, a.CAT, 96, 0, true, nullptr, 0, N, H, W);
+        hipLaunchKernelGGL(rtu::pool_kernel, dim3(blocks_for(a.px() / 4 * 32)), dim3(256), 0, stream, a.CAT, 96, 32,
+                           a.P, N, H, W);
+        conv(a.P, 32, 2, false, a.B1, 64, 0, true, nullptr, 0, N, H2, W2);
+        conv(a.B1, 64, 3, false, a.B2, 64, 0, true, nullptr, 0, N, H2, W2);
+        hipLaunchKernelGGL(rtu::upsample_kernel, dim3(blocks_for(a.px() * 64)), dim3(256), 0, stream, a.B2, 64, a.CAT,
+                           96, 32, N, H, W);
+        conv(a.CAT, 96, 4, false, a.D1, 32, 0, true, nullptr, 0, N, H, W);
+        conv(a.D1, 32, 5, false, a.D2, 32, 0, true, nullptr, 0, N, H, W);
+        const float scale = 2.0f / (3.0f * (float)a.px());
+        hipLaunchKernelGGL(rtu::out_layer_kernel, dim3(blocks_for(a.px(), 1024)), dim3(256), 0, stream, a.D2, a.noisy,
+                           d_p + ow_off, d_p + ob_off, a.pred, with_target ? a.target : nullptr, a.px(), scale,
+                           a.gD2, d_g + ow_off, d_g + ob_off, loss_slot);
+        hip_check(hipGetLastError(), "unet forward");
+    }
+
+    void backward(Acts &a) {
+        const int N = a.N, H = a.H, W = a.W, H2 = H / 2, W2 = W / 2;
+        wgrad(a.gD2, 5, a.D1, 32, N, H, W);
+        conv(a.gD2, 32, 5, true, a.gD1, 32, 0, false, a.D1, 32, N, H, W);
+        wgrad(a.gD1, 4, a.CAT, 96, N, H, W);
+        conv(a.gD1, 32, 4, true, a.gCAT, 96, 0, false, a.CAT, 96, N, H, W);
+        hipLaunchKernelGGL(rtu::upsample_back_kernel, dim3(blocks_for(a.px() / 4 * 64)), dim3(256), 0, stream, a.gCAT,
+                           96, 32, 64, a.gB2, N, H, W);
+        wgrad(a.gB2, 3, a.B1, 64, N, H2, W2);
+        conv(a.gB2, 64, 3, true, a.gB1, 64, 0, false, a.B1, 64, N, H2, W2);
+        wgrad(a.gB1, 2, a.P, 32, N, H2, W2);
+        conv(a.gB1, 64, 2, true, a.gP, 32, 0, false, nullptr, 0, N, H2, W2);
+        hipLaunchKernelGGL(rtu::pool_back_kernel, dim3(blocks_for(a.px() * 32)), dim3(256), 0, stream, a.gCAT, 96,
+                           a.gP, a.CAT, 96, 32, a.gA2, N, H, W);
+        wgrad(a.gA2, 1, a.A1, 32, N, H, W);
+        conv(a.gA2, 32, 1, true, a.gA1, 32, 0, false, a.A1, 32, N, H, W);
+        wgrad(a.gA1, 0, a.F, 16, N, H, W);
+        hip_check(hipGetLastError(), "unet backward");
+    }
+
+    void adam(float lr) {
+        adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - std::pow(b1, (float)adam_t), c2 = 1.0f - std::pow(b2, (float)adam_t);
+        for (int k = 0; k < 6; k++) {
+            const ConvLayer &L = kLayers[k];
+            hipLaunchKernelGGL(rtu::adam_kernel, dim3(blocks_for(L.nw())), dim3(256), 0, stream, d_p + w_off[k],
+                               d_g + w_off[k], d_m + w_off[k], d_v + w_off[k], L.nw(), lr, b1, b2, c1, c2, 1e-8f,
+                               d_wf[k], d_wt[k], L.cout, L.cin);
+            hipLaunchKernelGGL(rtu::adam_kernel, dim3(1), dim3(256), 0, stream, d_p + b_off[k], d_g + b_off[k],
+                               d_m + b_off[k], d_v + b_off[k], (size_t)L.cout, lr, b1, b2, c1, c2, 1e-8f,
+                               (rtu::bf16 *)nullptr, (rtu::bf16 *)nullptr, 0, 0);
+        }
+        const size_t no = (size_t)3 * rtu::kOutC + 3;
+        hipLaunchKernelGGL(rtu::adam_kernel, dim3(1), dim3(256), 0, stream, d_p + ow_off, d_g + ow_off, d_m + ow_off,
+                           d_v + ow_off, no, lr, b1, b2, c1, c2, 1e-8f, (rtu::bf16 *)nullptr, (rtu::bf16 *)nullptr, 0,
+                           0);
+        hip_check(hipGetLastError(), "adam");
+    }
+
+    void features(Acts &a, const rtu::FrameViews &v) {
+        const size_t n = (size_t)v.width * v.height;
+        const float ff = std::log2((float)std::max(v.frames, 1u) + 1.0f) / 8.0f;
+        hipLaunchKernelGGL(rtu::features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                           reinterpret_cast<const float4 *>(v.accum), reinterpret_cast<const float4 *>(v.gbuf),
+                           reinterpret_cast<const float4 *>(v.albedo), ff, n, a.F, a.noisy);
+    }
+};
+
+namespace {
+
+rt_unet *create_unet(int device, uint64_t seed) {
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    auto u = std::make_unique<rt_unet>();
+    u->device = device;
+    hip_check(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking), "hipStreamCreate");
+    size_t off = 0;
+    for (int k = 0; k < 6; k++) {
+        u->w_off[k] = off;
+        off += kLayers[k].nw();
+        u->b_off[k] = off;
+        off += kLayers[k].cout;
+    }
+    u->ow_off = off;
+    off += (size_t)3 * rtu::kOutC;
+    u->ob_off = off;
+    off += 3;
+    u->n_params = off;
+    std::vector<float> p(off, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (int k = 0; k < 6; k++) {
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (9.0f * kLayers[k].cin)));
+        for (size_t i = 0; i < kLayers[k].nw(); i++) p[u->w_off[k] + i] = nd(rng);
+    }
+    // the output layer starts at zero: the untrained network returns its input
+    u->d_p = dalloc<float>(off, "unet params");
+    u->d_g = dalloc<float>(off, "unet grads");
+    u->d_m = dalloc<float>(off, "unet adam m");
+    u->d_v = dalloc<float>(off, "unet adam v");
+    hip_check(hipMemcpy(u->d_p, p.data(), off * 4, hipMemcpyHostToDevice), "upload params");
+    for (float *b : {u->d_g, u->d_m, u->d_v}) hip_check(hipMemset(b, 0, off * 4), "memset");
+    for (int k = 0; k < 6; k++) {
+        u->d_wf[k] = dalloc<rtu::bf16>(kLayers[k].nw(), "unet bf16 weights");
+        u->d_wt[k] = dalloc<rtu::bf16>(kLayers[k].nw(), "unet bf16 weights (backward)");
+    }
+    hip_check(hipDeviceSynchronize(), "sync");
+    u->refresh_bf16();
+    hip_check(hipStreamSynchronize(u->stream), "sync");
+    return u.release();
+}
+
+void check_views(const rtu::FrameViews &v) {
+    if (!v.accum || !v.gbuf || !v.albedo) throw ApiError(RT_ERR_INVALID, "frame views incomplete");
+    if (v.width % 2 || v.height % 2) throw ApiError(RT_ERR_INVALID, "the denoiser needs even width and height");
+}
+
+}  // namespace
+
+namespace rtu {
+
+void denoise(rt_unet *u, const FrameViews &v, void *out, hipStream_t stream) {
+    check_views(v);
+    // the network runs on the caller's stream
+    hipStream_t saved = u->stream;
+    u->stream = stream;
+    try {
+        u->inf.ensure(1, (int)v.height, (int)v.width, false);
+        u->features(u->inf, v);
+        u->forward(u->inf, false, nullptr);
+        const size_t n = (size_t)v.width * v.height;
+        hipLaunchKernelGGL(image_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, u->inf.pred, n,
+                           reinterpret_cast<float4 *>(out));
+        hip_check(hipGetLastError(), "unet image");
+    } catch (...) {
+        u->stream = saved;
+        throw;
+    }
+    u->stream = saved;
+}
+
+}  // namespace rtu
+
+extern "C" {
+
+int rt_unet_create(int device, uint64_t seed, rt_unet **out) {
+    return rt::api_call([&] {
+        if (!out) throw ApiError(RT_ERR_INVALID, "null argument");
+        *out = create_unet(device, seed);
+    });
+}
+
+int rt_unet_destroy(rt_unet *u) {
+    return rt::api_call([&] { delete u; });
+}
+
+int rt_unet_param_count(const rt_unet *u, size_t *n) {
+    return rt::api_call([&] {
+        if (!u || !n) throw ApiError(RT_ERR_INVALID, "null argument");
+        *n = u->n_params;
+    });
+}
+
+int rt_unet_get_params(rt_unet *u, float *out, size_t n) {
+    return rt::api_call([&] {
+        if (!u || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n < u->n_params) throw ApiError(RT_ERR_INVALID, "buffer smaller than the parameter count");
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        hip_check(hipMemcpy(out, u->d_p, u->n_params * 4, hipMemcpyDeviceToHost), "read params");
+    });
+}
+
+int rt_unet_set_params(rt_unet *u, const float *in, size_t n) {
+    return rt::api_call([&] {
+        if (!u || !in) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n != u->n_params) throw ApiError(RT_ERR_INVALID, "parameter count mismatch");
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        hip_check(hipMemcpy(u->d_p, in, n * 4, hipMemcpyHostToDevice), "write params");
+        u->refresh_bf16();
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+    });
+}
+
+int rt_unet_dataset_reset(rt_unet *u, uint32_t width, uint32_t height, uint32_t capacity) {
+    return rt::api_call([&] {
+        if (!u) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (width % 2 || height % 2 || width < 32 || height < 32 || capacity == 0)
+            throw ApiError(RT_ERR_INVALID, "dataset images must be even-sized, >= 32, capacity > 0");
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        for (void *p : {(void *)u->ds_F, (void *)u->ds_noisy, (void *)u->ds_target})
+            if (p) (void)hipFree(p);
+        const size_t px = (size_t)width * height * capacity;
+        u->ds_F = dalloc<rtu::bf16>(px * 16, "dataset features");
+        u->ds_noisy = dalloc<float>(px * 3, "dataset inputs");
+        u->ds_target = dalloc<float>(px * 3, "dataset targets");
+        u->ds_w = width;
+        u->ds_h = height;
+        u->ds_cap = capacity;
+        u->ds_has_in.assign(capacity, 0);
+        u->ds_has_tgt.assign(capacity, 0);
+    });
+}
+
+int rt_unet_dataset_set(rt_unet *u, uint32_t slot, int which, rt_renderer *r) {
+    return rt::api_call([&] {
+        if (!u || !r) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (slot >= u->ds_cap) throw ApiError(RT_ERR_INVALID, "slot out of range");
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        const rtu::FrameViews v = rtu::frame_views(r);
+        if (v.width != u->ds_w || v.height != u->ds_h) throw ApiError(RT_ERR_INVALID, "image size differs from the dataset");
+        const size_t n = (size_t)v.width * v.height, base = (size_t)slot * n;
+        if (which == 0) {
+            check_views(v);
+            const float ff = std::log2((float)std::max(v.frames, 1u) + 1.0f) / 8.0f;
+            hipLaunchKernelGGL(rtu::features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, u->stream,
+                               reinterpret_cast<const float4 *>(v.accum), reinterpret_cast<const float4 *>(v.gbuf),
+                               reinterpret_cast<const float4 *>(v.albedo), ff, n, u->ds_F + base * 16,
+                               u->ds_noisy + base * 3);
+            u->ds_has_in[slot] = 1;
+        } else {
+            hipLaunchKernelGGL(rtu::log_target_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, u->stream,
+                               reinterpret_cast<const float4 *>(v.accum), n, u->ds_target + base * 3);
+            u->ds_has_tgt[slot] = 1;
+        }
+        hip_check(hipGetLastError(), "dataset set");
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+    });
+}
+
+int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, float lr, uint64_t seed,
+                  float *losses) {
+    return rt::api_call([&] {
+        if (!u) throw ApiError(RT_ERR_INVALID, "null argument");
+        std::vector<uint32_t> ready;
+        for (uint32_t k = 0; k < u->ds_cap; k++)
+            if (u->ds_has_in[k] && u->ds_has_tgt[k]) ready.push_back(k);
+        if (ready.empty()) throw ApiError(RT_ERR_INVALID, "no dataset slot holds both an input and a target");
+        if (crop % 32 || crop < 32 || crop > u->ds_w || crop > u->ds_h || batch == 0 || batch > 256)
+            throw ApiError(RT_ERR_INVALID, "crop must be a multiple of 32 within the images; 1 <= batch <= 256");
+        u->train.ensure((int)batch, (int)crop, (int)crop, true);
+        if (u->sel_cap < 3 * batch) {
+            if (u->d_sel) hipFree(u->d_sel);
+            u->d_sel = dalloc<int>(3 * batch, "crop selection");
+            u->sel_cap = 3 * batch;
+        }
+        if (u->loss_cap < steps) {
+            if (u->d_loss) hipFree(u->d_loss);
+            u->d_loss = dalloc<double>(steps, "losses");
+            u->loss_cap = steps;
+        }
+        hip_check(hipMemsetAsync(u->d_loss, 0, 8ull * std::max(steps, 1u), u->stream), "clear losses");
+        std::mt19937_64 rng(seed);
+        std::vector<int> sel(3 * batch);
+        for (uint32_t s = 0; s < steps; s++) {
+            for (uint32_t k = 0; k < batch; k++) {
+                sel[3 * k] = (int)ready[rng() % ready.size()];
+                sel[3 * k + 1] = (int)(rng() % ((u->ds_w - crop) / 2 + 1)) * 2;
+                sel[3 * k + 2] = (int)(rng() % ((u->ds_h - crop) / 2 + 1)) * 2;
+            }
+            hip_check(hipMemcpyAsync(u->d_sel, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, u->stream), "sel");
+            hip_check(hipStreamSynchronize(u->stream), "sync");  // `sel` is reused next step
+            hipLaunchKernelGGL(rtu::crop_kernel, dim3(std::max(1u, crop * crop / 256), batch), dim3(256), 0,
+                               u->stream, u->ds_F, u->ds_noisy, u->ds_target, u->d_sel, (int)u->ds_w, (int)u->ds_h,
+                               (int)crop, (int)crop, u->train.F, u->train.noisy, u->train.target);
+            hipLaunchKernelGGL(rtu::zero_kernel, dim3(blocks_for(u->n_params)), dim3(256), 0, u->stream, u->d_g,
+                               u->n_params);
+            u->forward(u->train, true, u->d_loss + s);
+            u->backward(u->train);
+            u->adam(lr);
+        }
+        hip_check(hipStreamSynchronize(u->stream), "train sync");
+        if (losses && steps) {
+            std::vector<double> l(steps);
+            hip_check(hipMemcpy(l.data(), u->d_loss, 8ull * steps, hipMemcpyDeviceToHost), "read losses");
+            const double norm = 1.0 / (3.0 * (double)u->train.px());
+            for (uint32_t s = 0; s < steps; s++) losses[s] = (float)(l[s] * norm);
+        }
+    });
+}
+
+// Test hooks: one forward (and backward) pass on caller-supplied host tensors.
+int rt_unet_eval(rt_unet *u, uint32_t n, uint32_t h, uint32_t w, const uint16_t *features, const float *noisy,
+                 const float *target, float *pred, float *grads, double *loss) {
+    return rt::api_call([&] {
+        if (!u || !features || !noisy || !pred) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (h % 2 || w % 2 || n == 0) throw ApiError(RT_ERR_INVALID, "even sizes required");
+        Acts &a = u->train;
+        a.ensure((int)n, (int)h, (int)w, true);
+        const size_t px = (size_t)n * h * w;
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        hip_check(hipMemcpy(a.F, features, px * 16 * 2, hipMemcpyHostToDevice), "upload features");
+        hip_check(hipMemcpy(a.noisy, noisy, px * 12, hipMemcpyHostToDevice), "upload inputs");
+        if (target) hip_check(hipMemcpy(a.target, target, px * 12, hipMemcpyHostToDevice), "upload targets");
+        if (u->loss_cap < 1) {
+            u->d_loss = dalloc<double>(1, "loss");
+            u->loss_cap = 1;
+        }
+        hip_check(hipMemsetAsync(u->d_loss, 0, 8, u->stream), "clear loss");
+        hipLaunchKernelGGL(rtu::zero_kernel, dim3(blocks_for(u->n_params)), dim3(256), 0, u->stream, u->d_g,
+                           u->n_params);
+        u->forward(a, target != nullptr, u->d_loss);
+        if (target) u->backward(a);
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        hip_check(hipMemcpy(pred, a.pred, px * 12, hipMemcpyDeviceToHost), "read pred");
+        if (target && grads) hip_check(hipMemcpy(grads, u->d_g, u->n_params * 4, hipMemcpyDeviceToHost), "read grads");
+        if (target && loss) {
+            double l = 0;
+            hip_check(hipMemcpy(&l, u->d_loss, 8, hipMemcpyDeviceToHost), "read loss");
+            *loss = l / (3.0 * (double)px);
+        }
+    });
+}
+
+}  // extern "C"
