@@ -50,16 +50,18 @@ def build_reference(r, ref_frames: int) -> float:
     return time.perf_counter() - t0
 
 
-def run_method(r, name: str, denoise: bool, start_frame: int, max_frames: int):
+def run_method(r, name: str, denoise: bool, start_frame: int, max_frames: int, unet=None):
     """First frame count reaching the threshold, its timed back-to-back wall
-    time, and the relMSE curve at powers of two."""
+    time, and the relMSE curve at powers of two. With `unet` the displayed
+    image is the learned denoiser's output (run every frame)."""
     _set(r, enable_denoise=denoise, accumulate_frames=1 << 20)
+    r.set_denoiser(unet)
     r.reset_accumulation()
     r.set_sample_count(start_frame)
     curve, hit = {}, None
     for k in range(1, max_frames + 1):
         r.record_frame()
-        e = r.accum64_relmse(EPS)
+        e = r.accum64_relmse_display(EPS)
         if k & (k - 1) == 0:
             curve[k] = e
         if e <= THRESHOLD:
@@ -69,6 +71,7 @@ def run_method(r, name: str, denoise: bool, start_frame: int, max_frames: int):
     if hit is None:
         result.update(frames=None, relmse_at_end=e, frames_run=max_frames)
         _log(f"{name}: relMSE {e:.4f} after {max_frames} frames (threshold not reached)")
+        r.set_denoiser(None)
         return result
     k, e = hit
     r.reset_accumulation()
@@ -79,9 +82,10 @@ def run_method(r, name: str, denoise: bool, start_frame: int, max_frames: int):
         r.record_frame()
     r.synchronize()
     ms = (time.perf_counter() - t0) * 1e3
-    check = r.accum64_relmse(EPS)
+    check = r.accum64_relmse_display(EPS)
     result.update(frames=k, relmse=e, relmse_rerun=check, ms=round(ms, 3))
     _log(f"{name}: relMSE {e:.4f} after {k} frames, {ms:.1f} ms back to back")
+    r.set_denoiser(None)
     return result
 
 
@@ -94,7 +98,17 @@ def main(argv=None) -> int:
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--ref-frames", type=int, default=16384)
     ap.add_argument("--max-frames", type=int, default=4096)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-scenes", type=int, default=24)
+    ap.add_argument("--train-steps", type=int, default=600)
     args = ap.parse_args(argv)
+    unet, train_info = None, None
+    if not args.no_learned:
+        from .denoiser import train_denoiser
+        unet, losses, (data_s, train_s) = train_denoiser(args.train_scenes, args.train_steps, log=_log)
+        train_info = {"scenes": args.train_scenes, "steps": args.train_steps, "data_seconds": round(data_s, 2),
+                      "train_seconds": round(train_s, 2), "loss_first10": float(losses[:10].mean()),
+                      "loss_last10": float(losses[-10:].mean())}
     scene = rt.Scene.builtin(args.scene, 1)
     cam = rt.Scene.builtin_camera(args.scene, 1)
     cfg = rt.RenderConfig(width=args.width, height=args.height, spp=1, max_depth=args.depth,
@@ -111,7 +125,17 @@ def main(argv=None) -> int:
                "reference_seconds": round(ref_s, 2), "methods": []}
         out["methods"].append(run_method(r, "a-trous x2 + running mean", True, start, args.max_frames))
         out["methods"].append(run_method(r, "running mean (no filter)", False, start, args.max_frames))
+        if unet is not None:
+            out["training"] = train_info
+            out["methods"].append(run_method(r, "learned U-Net on the running mean", False, start,
+                                             args.max_frames, unet))
+            base = out["methods"][0]
+            learned = out["methods"][-1]
+            if base.get("ms") and learned.get("ms"):
+                out["speedup_vs_atrous"] = round(base["ms"] / learned["ms"], 3)
     print(json.dumps(out), flush=True)
+    if unet is not None:
+        unet.close()
     return 0
 
 
