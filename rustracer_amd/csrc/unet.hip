@@ -57,42 +57,64 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // or a mask (x > 0) of another tensor at the output position (the ReLU
 // derivative in the backward-data pass). X: [N][H][W][xs] with cin channels at
 // offset 0; Wt: [9][cout][cin]; Y: [N][H][W][ys], cout channels at offset yoff.
-// Grid: (ceil(W/32), H, N * cout/32).
-__global__ __launch_bounds__(64) void conv3x3_kernel(const bf16 *__restrict__ X, int xs, int cin,
-                                                     const bf16 *__restrict__ Wt, const float *__restrict__ bias,
-                                                     bf16 *__restrict__ Y, int ys, int yoff, int cout, int relu,
-                                                     const bf16 *__restrict__ mask, int ms, int H, int W) {
-    const int l = threadIdx.x;
-    const int x0 = blockIdx.x * 32;
-    const int y = blockIdx.y;
-    const int nco = cout / 32;
-    const int n = blockIdx.z / nco;
-    const int co0 = (blockIdx.z % nco) * 32;
+// A workgroup of four wavefronts takes 32 pixels of 4 consecutive rows; the
+// input rows y0-1 .. y0+4, pixels x0-1 .. x0+32 are staged once in LDS (pixel
+// stride cin + 8 to spread banks) and serve all 9 taps; each wavefront computes
+// its row for all NCO x 32 output channels, so every A fragment feeds NCO MFMAs.
+// Grid: (ceil(W/32), ceil(H/4), N).
+constexpr int kConvRows = 4;
+template <int NCO>
+__global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X, int xs, int cin,
+                                                      const bf16 *__restrict__ Wt, const float *__restrict__ bias,
+                                                      bf16 *__restrict__ Y, int ys, int yoff, int cout, int relu,
+                                                      const bf16 *__restrict__ mask, int ms, int H, int W) {
+    extern __shared__ __attribute__((aligned(16))) bf16 tile[];  // [kConvRows + 2][34][cin + 8]
+    const int l = threadIdx.x % 64, wv = threadIdx.x / 64;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * kConvRows, n = blockIdx.z;
+    const int ps = cin + 8;
+    const int groups = cin / 8;
+    for (int i = threadIdx.x; i < (kConvRows + 2) * 34 * groups; i += 256) {
+        const int g = i % groups, q = i / groups;
+        const int px = q % 34, row = q / 34;
+        const int yy = y0 - 1 + row, xx = x0 - 1 + px;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+            v = *reinterpret_cast<const uint4 *>(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
+        *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * ps + g * 8) = v;
+    }
+    __syncthreads();
     const int r = l % 32, kg = (l / 32) * 8;
-    floatx16 acc = {};
+    floatx16 acc[NCO];
+#pragma unroll
+    for (int j = 0; j < NCO; j++) acc[j] = floatx16{};
     for (int tap = 0; tap < 9; tap++) {
-        const int yy = y + tap / 3 - 1;
-        const int xx = x0 + r + tap % 3 - 1;
-        const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const bf16 *xp = X + ((size_t)(n * H + (ok ? yy : 0)) * W + (ok ? xx : 0)) * xs + kg;
-        const bf16 *wp = Wt + ((size_t)tap * cout + co0 + r) * cin + kg;
+        const bf16 *ap = tile + ((wv + tap / 3) * 34 + r + tap % 3) * ps + kg;
+        const bf16 *wp = Wt + ((size_t)tap * cout + r) * cin + kg;
         for (int c0 = 0; c0 < cin; c0 += 16) {
-            const bf16x8 a = ok ? load8(xp + c0) : zero8();
-            const bf16x8 b = load8(wp + c0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            const bf16x8 a = load8(ap + c0);
+#pragma unroll
+            for (int j = 0; j < NCO; j++) {
+                const bf16x8 b = load8(wp + (size_t)j * 32 * cin + c0);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+            }
         }
     }
-    const int co = co0 + r;
-    const float bv = bias ? bias[co] : 0.0f;
+    const int y = y0 + wv;
+    if (y >= H) return;
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        const int x = x0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
-        if (x >= W) continue;
-        const size_t p = (size_t)(n * H + y) * W + x;
-        float v = acc[e] + bv;
-        if (relu) v = fmaxf(v, 0.0f);
-        if (mask && !(bf2f(mask[p * ms + co]) > 0.0f)) v = 0.0f;
-        Y[p * ys + yoff + co] = f2bf(v);
+    for (int j = 0; j < NCO; j++) {
+        const int co = j * 32 + r;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int x = x0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
+            if (x >= W) continue;
+            const size_t p = (size_t)(n * H + y) * W + x;
+            float v = acc[j][e] + bv;
+            if (relu) v = fmaxf(v, 0.0f);
+            if (mask && !(bf2f(mask[p * ms + co]) > 0.0f)) v = 0.0f;
+            Y[p * ys + yoff + co] = f2bf(v);
+        }
     }
 }
 
@@ -178,68 +200,104 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__
     }
 }
 
-// 2x2 average pool of channels [0, c) of X (stride xs) into P (stride c), half resolution.
+// Elementwise layers work on 8-channel groups (16-byte loads and stores), one
+// thread per (pixel, group); grid.y = image row (n * rows + y), grid.x covers
+// the row's pixels x groups. All index math is 32-bit.
+__device__ __forceinline__ void unpack8(uint4 v, float f[8]) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        f[2 * i] = __uint_as_float(w[i] << 16);
+        f[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) w[i] = (uint32_t)f2bf(f[2 * i]) | ((uint32_t)f2bf(f[2 * i + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ uint4 ld16(const bf16 *p) { return *reinterpret_cast<const uint4 *>(p); }
+__device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
+
+// 2x2 average pool of channels [0, c) of X (stride xs) into P (stride c), half
+// resolution; grid.y = n * H/2 + y2.
 __global__ __launch_bounds__(256) void pool_kernel(const bf16 *__restrict__ X, int xs, int c, bf16 *__restrict__ P,
-                                                   int N, int H, int W) {
-    const int h2 = H / 2, w2 = W / 2;
-    const size_t total = (size_t)N * h2 * w2 * c;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-        const int ch = (int)(i % c);
-        const size_t q = i / c;
-        const int x = (int)(q % w2), y = (int)((q / w2) % h2), n = (int)(q / ((size_t)w2 * h2));
-        const size_t p00 = ((size_t)(n * H + 2 * y) * W + 2 * x) * xs + ch;
-        const float s = bf2f(X[p00]) + bf2f(X[p00 + xs]) + bf2f(X[p00 + (size_t)W * xs]) + bf2f(X[p00 + (size_t)W * xs + xs]);
-        P[i] = f2bf(0.25f * s);
-    }
+                                                   int H, int W) {
+    const int g = c / 8, w2 = W / 2, h2 = H / 2;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= w2 * g) return;
+    const int x = t / g, cg = (t % g) * 8;
+    const int n = blockIdx.y / h2, y = blockIdx.y % h2;
+    const uint32_t p00 = ((uint32_t)(n * H + 2 * y) * W + 2 * x) * xs + cg;
+    float a[8], b[8], d[8], e[8], o[8];
+    unpack8(ld16(X + p00), a);
+    unpack8(ld16(X + p00 + xs), b);
+    unpack8(ld16(X + p00 + (uint32_t)W * xs), d);
+    unpack8(ld16(X + p00 + (uint32_t)W * xs + xs), e);
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = 0.25f * (((a[i] + b[i]) + d[i]) + e[i]);
+    st16(P + ((uint32_t)blockIdx.y * w2 + x) * c + cg, pack8(o));
 }
 
-// Nearest 2x upsample of B (stride c, half resolution) into channels [off, off+c) of Y (stride ys).
+// Nearest 2x upsample of B (stride c, half resolution) into channels [off, off+c)
+// of Y (stride ys); grid.y = n * H + y.
 __global__ __launch_bounds__(256) void upsample_kernel(const bf16 *__restrict__ B, int c, bf16 *__restrict__ Y, int ys,
-                                                       int off, int N, int H, int W) {
-    const size_t total = (size_t)N * H * W * c;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-        const int ch = (int)(i % c);
-        const size_t p = i / c;
-        const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
-        Y[p * ys + off + ch] = B[((size_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * c + ch];
-    }
+                                                       int off, int H, int W) {
+    const int g = c / 8;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= W * g) return;
+    const int x = t / g, cg = (t % g) * 8;
+    const int n = blockIdx.y / H, y = blockIdx.y % H;
+    const uint4 v = ld16(B + ((uint32_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * c + cg);
+    st16(Y + ((uint32_t)blockIdx.y * W + x) * ys + off + cg, v);
 }
 
-// Backward of the pool into A2's gradient: gA[p][ch] = (gCat[p][ch] + gP[p/2][ch] / 4) * (A[p][ch] > 0).
-// gCat already carries the mask; the pool branch gets it here.
+// Backward of the pool into A2's gradient: gA[p] = gCat[p] + (gP[p/2] / 4) * (A[p] > 0).
+// gCat already carries the mask; the pool branch gets it here. grid.y = n * H + y.
 __global__ __launch_bounds__(256) void pool_back_kernel(const bf16 *__restrict__ gcat, int cs, const bf16 *__restrict__ gp,
                                                         const bf16 *__restrict__ A, int as, int c, bf16 *__restrict__ ga,
-                                                        int N, int H, int W) {
-    const size_t total = (size_t)N * H * W * c;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-        const int ch = (int)(i % c);
-        const size_t p = i / c;
-        const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((size_t)W * H));
-        const float fromp = 0.25f * bf2f(gp[((size_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * c + ch]);
-        const float m = bf2f(A[p * as + ch]) > 0.0f ? 1.0f : 0.0f;
-        ga[i] = f2bf(bf2f(gcat[p * cs + ch]) + fromp * m);
-    }
+                                                        int H, int W) {
+    const int g = c / 8;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= W * g) return;
+    const int x = t / g, cg = (t % g) * 8;
+    const int n = blockIdx.y / H, y = blockIdx.y % H;
+    const uint32_t p = (uint32_t)blockIdx.y * W + x;
+    float gc[8], pv[8], av[8], o[8];
+    unpack8(ld16(gcat + p * cs + cg), gc);
+    unpack8(ld16(gp + ((uint32_t)(n * (H / 2) + y / 2) * (W / 2) + x / 2) * c + cg), pv);
+    unpack8(ld16(A + p * as + cg), av);
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = gc[i] + (0.25f * pv[i]) * (av[i] > 0.0f ? 1.0f : 0.0f);
+    st16(ga + p * c + cg, pack8(o));
 }
 
-// Backward of the nearest upsample: gB[q][ch] = sum of the 4 children's gY[p][off + ch].
+// Backward of the nearest upsample: gB[q] = sum of the 4 children's gY[p][off ..];
+// grid.y = n * H/2 + y2.
 __global__ __launch_bounds__(256) void upsample_back_kernel(const bf16 *__restrict__ gy, int ys, int off, int c,
-                                                            bf16 *__restrict__ gb, int N, int H, int W) {
-    const int h2 = H / 2, w2 = W / 2;
-    const size_t total = (size_t)N * h2 * w2 * c;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
-        const int ch = (int)(i % c);
-        const size_t q = i / c;
-        const int x = (int)(q % w2), y = (int)((q / w2) % h2), n = (int)(q / ((size_t)w2 * h2));
-        const size_t p00 = ((size_t)(n * H + 2 * y) * W + 2 * x) * ys + off + ch;
-        const float s = bf2f(gy[p00]) + bf2f(gy[p00 + ys]) + bf2f(gy[p00 + (size_t)W * ys]) + bf2f(gy[p00 + (size_t)W * ys + ys]);
-        gb[i] = f2bf(s);
-    }
+                                                            bf16 *__restrict__ gb, int H, int W) {
+    const int g = c / 8, w2 = W / 2, h2 = H / 2;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= w2 * g) return;
+    const int x = t / g, cg = (t % g) * 8;
+    const int n = blockIdx.y / h2, y = blockIdx.y % h2;
+    const uint32_t p00 = ((uint32_t)(n * H + 2 * y) * W + 2 * x) * ys + off + cg;
+    float a[8], b[8], d[8], e[8], o[8];
+    unpack8(ld16(gy + p00), a);
+    unpack8(ld16(gy + p00 + ys), b);
+    unpack8(ld16(gy + p00 + (uint32_t)W * ys), d);
+    unpack8(ld16(gy + p00 + (uint32_t)W * ys + ys), e);
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = ((a[i] + b[i]) + d[i]) + e[i];
+    st16(gb + ((uint32_t)blockIdx.y * w2 + x) * c + cg, pack8(o));
 }
 
 // Output layer (1x1, 32 -> 3, FP32) plus the residual: pred = noisy + b + W·D.
 // With a target: loss partial sums, and the backward pass: gD = (W^T dpred) * (D > 0)
 // as bf16, gW += dpred D^T, gb += dpred (block reductions, then atomics).
 constexpr int kOutC = 32;
+template <bool TRAIN>
 __global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__ D, const float *__restrict__ noisy,
                                                         const float *__restrict__ w, const float *__restrict__ b,
                                                         float *__restrict__ pred, const float *__restrict__ target,
@@ -251,8 +309,8 @@ __global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__
     for (int i = threadIdx.x; i < 3 * kOutC + 3; i += 256) gacc[i] = 0.0f;
     __syncthreads();
     float lsum = 0.0f;
-    float lw[3 * kOutC + 3];
-    if (target)
+    float lw[TRAIN ? 3 * kOutC + 3 : 1];
+    if (TRAIN)
         for (int i = 0; i < 3 * kOutC + 3; i++) lw[i] = 0.0f;
     for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < n_px; p += (size_t)gridDim.x * 256) {
         float d[kOutC];
@@ -272,7 +330,7 @@ __global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__
             o[c] = noisy[p * 3 + c] + s;
             pred[p * 3 + c] = o[c];
         }
-        if (target) {
+        if (TRAIN) {
             float g[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -301,7 +359,7 @@ __global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__
             }
         }
     }
-    if (!target) return;
+    if (!TRAIN) return;
     for (int i = 0; i < 3 * kOutC + 3; i++) atomicAdd(&gacc[i], lw[i]);
     red[threadIdx.x] = lsum;
     __syncthreads();
@@ -547,9 +605,18 @@ struct rt_unet {
               const rtu::bf16 *mask, int ms, int N, int H, int W) {
         const ConvLayer &L = kLayers[k];
         const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
-        const dim3 grid((W + 31) / 32, H, N * (cout / 32));
-        hipLaunchKernelGGL(rtu::conv3x3_kernel, grid, dim3(64), 0, stream, X, xs, cin, bwd ? d_wt[k] : d_wf[k],
-                           bwd ? nullptr : d_p + b_off[k], Y, ys, yoff, cout, relu ? 1 : 0, mask, ms, H, W);
+        const dim3 grid((W + 31) / 32, (H + rtu::kConvRows - 1) / rtu::kConvRows, N);
+        const size_t lds = (size_t)(rtu::kConvRows + 2) * 34 * (cin + 8) * 2;
+        const rtu::bf16 *w = bwd ? d_wt[k] : d_wf[k];
+        const float *b = bwd ? nullptr : d_p + b_off[k];
+#define RT_CONV(NCO)                                                                                        \
+    hipLaunchKernelGGL(rtu::conv3x3_kernel<NCO>, grid, dim3(256), lds, stream, X, xs, cin, w, b, Y, ys, yoff, \
+                       cout, relu ? 1 : 0, mask, ms, H, W)
+        if (cout == 32) RT_CONV(1);
+        else if (cout == 64) RT_CONV(2);
+        else if (cout == 96) RT_CONV(3);
+        else throw ApiError(RT_ERR_INVALID, "unsupported channel count");
+#undef RT_CONV
     }
     void wgrad(const rtu::bf16 *G, int k, const rtu::bf16 *X, int xs, int N, int H, int W) {
         const ConvLayer &L = kLayers[k];
@@ -567,18 +634,23 @@ struct rt_unet {
         const int N = a.N, H = a.H, W = a.W, H2 = H / 2, W2 = W / 2;
         conv(a.F, 16, 0, false, a.A1, 32, 0, true, nullptr, 0, N, H, W);
         conv(a.A1, 32, 1, false, a.CAT, 96, 0, true, nullptr, 0, N, H, W);
-        hipLaunchKernelGGL(rtu::pool_kernel, dim3(blocks_for(a.px() / 4 * 32)), dim3(256), 0, stream, a.CAT, 96, 32,
-                           a.P, N, H, W);
+        hipLaunchKernelGGL(rtu::pool_kernel, dim3((W2 * 4 + 255) / 256, N * H2), dim3(256), 0, stream, a.CAT, 96,
+                           32, a.P, H, W);
         conv(a.P, 32, 2, false, a.B1, 64, 0, true, nullptr, 0, N, H2, W2);
         conv(a.B1, 64, 3, false, a.B2, 64, 0, true, nullptr, 0, N, H2, W2);
-        hipLaunchKernelGGL(rtu::upsample_kernel, dim3(blocks_for(a.px() * 64)), dim3(256), 0, stream, a.B2, 64, a.CAT,
-                           96, 32, N, H, W);
+        hipLaunchKernelGGL(rtu::upsample_kernel, dim3((W * 8 + 255) / 256, N * H), dim3(256), 0, stream, a.B2, 64,
+                           a.CAT, 96, 32, H, W);
         conv(a.CAT, 96, 4, false, a.D1, 32, 0, true, nullptr, 0, N, H, W);
         conv(a.D1, 32, 5, false, a.D2, 32, 0, true, nullptr, 0, N, H, W);
         const float scale = 2.0f / (3.0f * (float)a.px());
-        hipLaunchKernelGGL(rtu::out_layer_kernel, dim3(blocks_for(a.px(), 1024)), dim3(256), 0, stream, a.D2, a.noisy,
-                           d_p + ow_off, d_p + ob_off, a.pred, with_target ? a.target : nullptr, a.px(), scale,
-                           a.gD2, d_g + ow_off, d_g + ob_off, loss_slot);
+        if (with_target)
+            hipLaunchKernelGGL(rtu::out_layer_kernel<true>, dim3(blocks_for(a.px(), 1024)), dim3(256), 0, stream, a.D2,
+                               a.noisy, d_p + ow_off, d_p + ob_off, a.pred, a.target, a.px(), scale, a.gD2,
+                               d_g + ow_off, d_g + ob_off, loss_slot);
+        else
+            hipLaunchKernelGGL(rtu::out_layer_kernel<false>, dim3(blocks_for(a.px())), dim3(256), 0, stream, a.D2,
+                               a.noisy, d_p + ow_off, d_p + ob_off, a.pred, (const float *)nullptr, a.px(), scale,
+                               (rtu::bf16 *)nullptr, (float *)nullptr, (float *)nullptr, (double *)nullptr);
         hip_check(hipGetLastError(), "unet forward");
     }
 
@@ -588,14 +660,14 @@ struct rt_unet {
         conv(a.gD2, 32, 5, true, a.gD1, 32, 0, false, a.D1, 32, N, H, W);
         wgrad(a.gD1, 4, a.CAT, 96, N, H, W);
         conv(a.gD1, 32, 4, true, a.gCAT, 96, 0, false, a.CAT, 96, N, H, W);
-        hipLaunchKernelGGL(rtu::upsample_back_kernel, dim3(blocks_for(a.px() / 4 * 64)), dim3(256), 0, stream, a.gCAT,
-                           96, 32, 64, a.gB2, N, H, W);
+        hipLaunchKernelGGL(rtu::upsample_back_kernel, dim3((W2 * 8 + 255) / 256, N * H2), dim3(256), 0, stream,
+                           a.gCAT, 96, 32, 64, a.gB2, H, W);
         wgrad(a.gB2, 3, a.B1, 64, N, H2, W2);
         conv(a.gB2, 64, 3, true, a.gB1, 64, 0, false, a.B1, 64, N, H2, W2);
         wgrad(a.gB1, 2, a.P, 32, N, H2, W2);
         conv(a.gB1, 64, 2, true, a.gP, 32, 0, false, nullptr, 0, N, H2, W2);
-        hipLaunchKernelGGL(rtu::pool_back_kernel, dim3(blocks_for(a.px() * 32)), dim3(256), 0, stream, a.gCAT, 96,
-                           a.gP, a.CAT, 96, 32, a.gA2, N, H, W);
+        hipLaunchKernelGGL(rtu::pool_back_kernel, dim3((W * 4 + 255) / 256, N * H), dim3(256), 0, stream, a.gCAT, 96,
+                           a.gP, a.CAT, 96, 32, a.gA2, H, W);
         wgrad(a.gA2, 1, a.A1, 32, N, H, W);
         conv(a.gA2, 32, 1, true, a.gA1, 32, 0, false, a.A1, 32, N, H, W);
         wgrad(a.gA1, 0, a.F, 16, N, H, W);
