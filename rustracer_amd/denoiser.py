@@ -18,6 +18,9 @@ from . import RenderConfig, Renderer, Scene
 from ._ffi import check, lib, ptr
 
 INPUT_FRAMES = (1, 2, 4, 8, 16, 32)
+# (cin, cout) of the six 3x3 convolutions: two at full resolution, two at half,
+# two at full after the skip concatenation (the layout of csrc/unet.hip)
+LAYER_SHAPES = ((16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (32, 32))
 
 
 class UNet:
