@@ -77,6 +77,8 @@ def main() -> int:
     ap.add_argument("--no-photons", action="store_true")
     ap.add_argument("--no-network", action="store_true", help="skip the learned denoiser's timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batch measurement")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 720p photon-map workload and the network training throughput")
     ap.add_argument("--convergence", action="store_true",
                     help="run the ms-to-relMSE study now (about two minutes) instead of reporting the committed one")
     args = ap.parse_args()
@@ -241,6 +243,10 @@ def main() -> int:
     r.close()
     if not args.no_batch:
         result["batch_multiview"] = batch_multiview(args)
+    if not args.no_extra:
+        result["photon_plain_720p"] = photon_plain_720p()
+        if not args.no_network:
+            result["network_training"] = network_training()
     result["convergence"] = convergence(args)
     print(json.dumps(result), flush=True)
     return 0
@@ -275,6 +281,61 @@ def batch_multiview(args) -> dict:
             "pixels": px, "samples": smp, "ms_per_launch": round(ms, 3),
             "msamples_per_s": round(smp / (ms * 1e-3) / 1e6, 2), "tuning": best,
             "autotune_best_ms": round(tuned_ms, 3)}
+
+
+def photon_plain_720p() -> dict:
+    """The photon-mapped plain scene at 1280x720 (displaced GGX-metal icosphere, glass
+    sphere, Lambert floor, area + point light): 1M photons, grid cell = radius = 0.05,
+    4 spp x 32 accumulated frames, depth 6, a-trous x2."""
+    import rustracer_amd as rt
+
+    scene = rt.Scene.builtin("photon_plain", 1)
+    cam = rt.Scene.builtin_camera("photon_plain", 1)
+    cfg = rt.RenderConfig(width=1280, height=720, spp=4, max_depth=6, enable_photon_map=True,
+                          photon_count=1_000_000, grid_cell_size=0.05, enable_denoise=True, accumulate_frames=32)
+    with rt.Renderer(cfg, scene, 0) as r:
+        r.update_camera(cam)
+        t0 = time.perf_counter()
+        r.rebuild_photons()
+        r.synchronize()
+        t_ph = time.perf_counter() - t0
+        r.render_frames(2)
+        r.synchronize()
+        r.reset_accumulation()
+        t0 = time.perf_counter()
+        r.render_frames(32)
+        r.synchronize()
+        el = time.perf_counter() - t0
+        gp = r.grid_params()
+    smp = 1280 * 720 * 4 * 32
+    log(f"photon_plain 720p: 32 frames x 4 spp in {el * 1e3:.1f} ms; photon rebuild {t_ph * 1e3:.1f} ms")
+    return {"workload": "photon-mapped plain scene 1280x720, 1M photons, cell = radius = 0.05, 4 spp x 32 frames, "
+                        "depth 6, a-trous x2", "msamples_per_s": round(smp / el / 1e6, 2),
+            "ms_per_frame": round(el / 32 * 1e3, 3), "photon_rebuild_ms": round(t_ph * 1e3, 2),
+            "grid_cells": gp.num_cells, "triangles": scene.counts()["triangles"]}
+
+
+def network_training() -> dict:
+    """Training throughput of the learned denoiser: Adam steps on batches of 32
+    crops of 256x256 (the training workload's batch shape) from a small
+    renderer-made dataset; forward + backward = 3x the forward FLOPs."""
+    from rustracer_amd.denoiser import UNet, build_dataset
+
+    net = UNet(0, 1)
+    build_dataset(net, n_scenes=2, size=256, target_frames=64, photons=50_000, first_seed=77)
+    net.train(steps=3, batch=32, crop=256, lr=1e-3, seed=1)
+    steps = 20
+    t0 = time.perf_counter()
+    net.train(steps=steps, batch=32, crop=256, lr=1e-3, seed=2)
+    el = time.perf_counter() - t0
+    net.close()
+    flops = 3.0 * network_flops(256, 256) * 32
+    tf = flops * steps / el / 1e12
+    log(f"network training: {el / steps * 1e3:.2f} ms/step (batch 32 x 256^2), {tf:.1f} TFLOP/s")
+    return {"bound": "mfma", "ms_per_step": round(el / steps * 1e3, 3), "batch": "32 x 256x256",
+            "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
+            "note": "wall time per Adam step incl. the crop gather and the host's per-step crop draw"}
 
 
 def network_flops(w: int, h: int) -> float:

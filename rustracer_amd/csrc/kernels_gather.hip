@@ -346,8 +346,17 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                     const int32_t cx = iclamp((int32_t)((uint32_t)bx + (uint32_t)(k - 1)), 0, gxmax);
                     metas[k] = A.grid_meta[(uint32_t)cx + row];
                 }
+                // Away from the x borders the three cells are consecutive and
+                // grid_meta is an exclusive scan, so their photons form one
+                // contiguous run in the same order: walk it as one range.
+                int n_ranges = 3;
+                if (bx >= 1 && bx <= gxmax - 1) {
+                    metas[0].y = metas[0].y + metas[1].y + metas[2].y;
+                    n_ranges = 1;
+                }
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
+                    if (k >= n_ranges) break;
                     const uint2 meta = metas[k];
                     for (uint32_t i = 0u; i < meta.y && !done; i += 4u) {
                         const uint32_t nb = min(meta.y - i, 4u);

@@ -53,6 +53,27 @@ __device__ __forceinline__ bf16x8 load8(const bf16 *p) {
 }
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u)); }
 
+// Elementwise layers work on 8-channel groups (16-byte loads and stores), one
+// thread per (pixel, group); grid.y = image row (n * rows + y), grid.x covers
+// the row's pixels x groups. All index math is 32-bit.
+__device__ __forceinline__ void unpack8(uint4 v, float f[8]) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        f[2 * i] = __uint_as_float(w[i] << 16);
+        f[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) w[i] = (uint32_t)f2bf(f[2 * i]) | ((uint32_t)f2bf(f[2 * i + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ uint4 ld16(const bf16 *p) { return *reinterpret_cast<const uint4 *>(p); }
+__device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
+
+
 // One 3x3 convolution (stride 1, zero padding), optionally with bias and ReLU,
 // or a mask (x > 0) of another tensor at the output position (the ReLU
 // derivative in the backward-data pass). X: [N][H][W][xs] with cin channels at
@@ -188,37 +209,33 @@ __global__ __launch_bounds__(64) void conv3x3_wgrad_kernel(const bf16 *__restric
     }
 }
 
-// Bias gradient: gb[c] += sum over pixels of G[p][c]. Grid-stride over pixels.
+// Bias gradient: gb[c] += sum over pixels of G[p][c]. Thread t of a block
+// takes channel group t % (c/8) (16-byte loads) for pixels t / (c/8) + k *
+// lanes, grid-strided; the block then reduces its lanes in LDS and adds one
+// value per channel. c is a multiple of 8 and at most 256.
 __global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ G, int gs, int c, size_t n_px,
                                                         float *__restrict__ gb) {
-    const int t = threadIdx.x % 32, q = threadIdx.x / 32;  // 8 pixel lanes x 32 channels
-    for (int c0 = 0; c0 < c; c0 += 32) {
+    __shared__ float part[256 * 8];
+    const int groups = c / 8, lanes = 256 / groups;
+    const int g = threadIdx.x % groups, lane = threadIdx.x / groups;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (lane < lanes)
+        for (size_t p = (size_t)blockIdx.x * lanes + lane; p < n_px; p += (size_t)gridDim.x * lanes) {
+            float v[8];
+            unpack8(*reinterpret_cast<const uint4 *>(G + p * gs + g * 8), v);
+#pragma unroll
+            for (int i = 0; i < 8; i++) acc[i] += v[i];
+        }
+#pragma unroll
+    for (int i = 0; i < 8; i++) part[threadIdx.x * 8 + i] = acc[i];
+    __syncthreads();
+    if ((int)threadIdx.x < c) {
+        const int gg = threadIdx.x / 8, i = threadIdx.x % 8;
         float s = 0.0f;
-        for (size_t p = (size_t)blockIdx.x * 8 + q; p < n_px; p += (size_t)gridDim.x * 8)
-            if (c0 + t < c) s += bf2f(G[p * gs + c0 + t]);
-        if (c0 + t < c) atomicAdd(&gb[c0 + t], s);
+        for (int l = 0; l < lanes; l++) s += part[(l * groups + gg) * 8 + i];
+        atomicAdd(&gb[threadIdx.x], s);
     }
 }
-
-// Elementwise layers work on 8-channel groups (16-byte loads and stores), one
-// thread per (pixel, group); grid.y = image row (n * rows + y), grid.x covers
-// the row's pixels x groups. All index math is 32-bit.
-__device__ __forceinline__ void unpack8(uint4 v, float f[8]) {
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        f[2 * i] = __uint_as_float(w[i] << 16);
-        f[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
-    }
-}
-__device__ __forceinline__ uint4 pack8(const float f[8]) {
-    uint32_t w[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) w[i] = (uint32_t)f2bf(f[2 * i]) | ((uint32_t)f2bf(f[2 * i + 1]) << 16);
-    return make_uint4(w[0], w[1], w[2], w[3]);
-}
-__device__ __forceinline__ uint4 ld16(const bf16 *p) { return *reinterpret_cast<const uint4 *>(p); }
-__device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
 
 // 2x2 average pool of channels [0, c) of X (stride xs) into P (stride c), half
 // resolution; grid.y = n * H/2 + y2.
@@ -621,10 +638,10 @@ struct rt_unet {
     void wgrad(const rtu::bf16 *G, int k, const rtu::bf16 *X, int xs, int N, int H, int W) {
         const ConvLayer &L = kLayers[k];
         const int segs = N * H * ((W + 31) / 32);
-        const dim3 grid((unsigned)std::min(segs, 96), 9, (L.cout / 32) * ((L.cin + 31) / 32));
+        const dim3 grid((unsigned)std::min(segs, 384), 9, (L.cout / 32) * ((L.cin + 31) / 32));
         hipLaunchKernelGGL(rtu::conv3x3_wgrad_kernel, grid, dim3(64), 0, stream, G, L.cout, L.cout, X, xs, L.cin,
                            d_g + w_off[k], N, H, W);
-        hipLaunchKernelGGL(rtu::bias_grad_kernel, dim3(64), dim3(256), 0, stream, G, L.cout, L.cout,
+        hipLaunchKernelGGL(rtu::bias_grad_kernel, dim3(512), dim3(256), 0, stream, G, L.cout, L.cout,
                            (size_t)N * H * W, d_g + b_off[k]);
     }
 
@@ -873,10 +890,11 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
         if (crop % 32 || crop < 32 || crop > u->ds_w || crop > u->ds_h || batch == 0 || batch > 256)
             throw ApiError(RT_ERR_INVALID, "crop must be a multiple of 32 within the images; 1 <= batch <= 256");
         u->train.ensure((int)batch, (int)crop, (int)crop, true);
-        if (u->sel_cap < 3 * batch) {
-            if (u->d_sel) hipFree(u->d_sel);
-            u->d_sel = dalloc<int>(3 * batch, "crop selection");
-            u->sel_cap = 3 * batch;
+        const size_t sel_n = (size_t)3 * batch * std::max(steps, 1u);
+        if (u->sel_cap < sel_n) {
+            if (u->d_sel) (void)hipFree(u->d_sel);
+            u->d_sel = dalloc<int>(sel_n, "crop selection");
+            u->sel_cap = sel_n;
         }
         if (u->loss_cap < steps) {
             if (u->d_loss) hipFree(u->d_loss);
@@ -884,19 +902,21 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
             u->loss_cap = steps;
         }
         hip_check(hipMemsetAsync(u->d_loss, 0, 8ull * std::max(steps, 1u), u->stream), "clear losses");
+        // every step's crops drawn up front and uploaded once
         std::mt19937_64 rng(seed);
-        std::vector<int> sel(3 * batch);
+        std::vector<int> sel((size_t)3 * batch * std::max(steps, 1u));
+        for (size_t k = 0; k < (size_t)batch * steps; k++) {
+            sel[3 * k] = (int)ready[rng() % ready.size()];
+            sel[3 * k + 1] = (int)(rng() % ((u->ds_w - crop) / 2 + 1)) * 2;
+            sel[3 * k + 2] = (int)(rng() % ((u->ds_h - crop) / 2 + 1)) * 2;
+        }
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        hip_check(hipMemcpy(u->d_sel, sel.data(), sel.size() * 4, hipMemcpyHostToDevice), "upload crops");
         for (uint32_t s = 0; s < steps; s++) {
-            for (uint32_t k = 0; k < batch; k++) {
-                sel[3 * k] = (int)ready[rng() % ready.size()];
-                sel[3 * k + 1] = (int)(rng() % ((u->ds_w - crop) / 2 + 1)) * 2;
-                sel[3 * k + 2] = (int)(rng() % ((u->ds_h - crop) / 2 + 1)) * 2;
-            }
-            hip_check(hipMemcpyAsync(u->d_sel, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, u->stream), "sel");
-            hip_check(hipStreamSynchronize(u->stream), "sync");  // `sel` is reused next step
             hipLaunchKernelGGL(rtu::crop_kernel, dim3(std::max(1u, crop * crop / 256), batch), dim3(256), 0,
-                               u->stream, u->ds_F, u->ds_noisy, u->ds_target, u->d_sel, (int)u->ds_w, (int)u->ds_h,
-                               (int)crop, (int)crop, u->train.F, u->train.noisy, u->train.target);
+                               u->stream, u->ds_F, u->ds_noisy, u->ds_target, u->d_sel + (size_t)3 * batch * s,
+                               (int)u->ds_w, (int)u->ds_h, (int)crop, (int)crop, u->train.F, u->train.noisy,
+                               u->train.target);
             hipLaunchKernelGGL(rtu::zero_kernel, dim3(blocks_for(u->n_params)), dim3(256), 0, u->stream, u->d_g,
                                u->n_params);
             u->forward(u->train, true, u->d_loss + s);
