@@ -64,19 +64,19 @@ struct rt_renderer {
 
     float *d_accum = nullptr, *d_frame = nullptr, *d_ping = nullptr, *d_moments = nullptr, *d_gbuf = nullptr;
     uint32_t *d_rgba8 = nullptr;
-    // Frame pipelining: consecutive frames alternate between two output sets
-    // (frame, a-trous ping-pong, moments, G-buffer) and two gather streams, so
-    // the camera pass of frame k+1 can start while the last long paths of
-    // frame k are still running. The main stream waits for each frame's
+    // Frame pipelining: consecutive frames rotate through kSets output sets
+    // (frame, a-trous ping-pong, moments, G-buffer) and as many gather streams,
+    // so the camera pass of frame k+1 can start while the last long paths of
+    // earlier frames are still running. The main stream waits for each frame's
     // gather before its a-trous and composite passes (which stay in frame
-    // order), and a gather waits until the composite two frames back has
+    // order), and a gather waits until the composite kSets frames back has
     // released its output set. d_frame etc. name the latest frame's set.
-    float *set_frame[2] = {nullptr, nullptr}, *set_ping[2] = {nullptr, nullptr};
-    float *set_mom[2] = {nullptr, nullptr}, *set_gb[2] = {nullptr, nullptr};
-    hipStream_t gstream[2] = {nullptr, nullptr};
-    hipEvent_t ev_gdone[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
-    bool free_recorded[2] = {false, false};
-    uint32_t parity = 0;
+    static constexpr int kSets = 2;  // measured: 3 sets (three gathers in flight) ran 10 % slower
+    float *set_frame[kSets] = {}, *set_ping[kSets] = {}, *set_mom[kSets] = {}, *set_gb[kSets] = {};
+    hipStream_t gstream[kSets] = {};
+    hipEvent_t ev_gdone[kSets] = {}, ev_free[kSets] = {};
+    bool free_recorded[kSets] = {};
+    uint32_t next_set = 0;
 
     rt_grid_params gp{};
     uint32_t meta_cells = 0;
@@ -128,17 +128,17 @@ struct rt_renderer {
 
     ~rt_renderer() {
         if (stream) hipStreamSynchronize(stream);
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < kSets; k++) {
             if (gstream[k]) hipStreamSynchronize(gstream[k]);
             if (ev_gdone[k]) hipEventDestroy(ev_gdone[k]);
             if (ev_free[k]) hipEventDestroy(ev_free[k]);
             if (gstream[k]) hipStreamDestroy(gstream[k]);
-            // set 0 is freed through d_frame etc. only when it is the current set
         }
+        // once the extra sets exist, every set (d_frame etc. name one of them) is freed here
         if (set_frame[1]) {
-            for (float *b : {set_frame[0], set_ping[0], set_mom[0], set_gb[0], set_frame[1], set_ping[1], set_mom[1],
-                             set_gb[1]})
-                if (b) hipFree(b);
+            for (int k = 0; k < kSets; k++)
+                for (float *b : {set_frame[k], set_ping[k], set_mom[k], set_gb[k]})
+                    if (b) hipFree(b);
             d_frame = d_ping = d_moments = d_gbuf = nullptr;
         }
         for (auto &p : pending) {
@@ -283,8 +283,8 @@ struct rt_renderer {
         if (cfg.enable_photon_map && !photons_built)
             throw ApiError(RT_ERR_INVALID, "photon map enabled but rt_rebuild_photons has not run");
         ensure_path_queues();
-        const uint32_t si = parity;
-        parity ^= 1u;
+        const uint32_t si = next_set;
+        next_set = (next_set + 1u) % (uint32_t)kSets;
         d_frame = set_frame[si];
         d_ping = set_ping[si];
         d_moments = set_mom[si];
@@ -737,14 +737,17 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->set_ping[0] = r->d_ping;
     r->set_mom[0] = r->d_moments;
     r->set_gb[0] = r->d_gbuf;
-    r->set_frame[1] = dev_alloc<float>(npx * 4, "frame (second set)");
-    r->set_ping[1] = dev_alloc<float>(npx * 4, "denoise pingpong (second set)");
-    r->set_mom[1] = dev_alloc<float>(npx * 4, "frame moments (second set)");
-    r->set_gb[1] = dev_alloc<float>(npx * 4, "gbuffer (second set)");
-    for (float *b : {r->d_accum, r->d_frame, r->d_ping, r->d_moments, r->d_gbuf, r->set_frame[1], r->set_ping[1],
-                     r->set_mom[1], r->set_gb[1]})
-        hip_check(hipMemset(b, 0, npx * 16), "memset");
-    for (int k = 0; k < 2; k++) {
+    for (int k = 1; k < rt_renderer::kSets; k++) {
+        r->set_frame[k] = dev_alloc<float>(npx * 4, "frame (extra set)");
+        r->set_ping[k] = dev_alloc<float>(npx * 4, "denoise pingpong (extra set)");
+        r->set_mom[k] = dev_alloc<float>(npx * 4, "frame moments (extra set)");
+        r->set_gb[k] = dev_alloc<float>(npx * 4, "gbuffer (extra set)");
+    }
+    hip_check(hipMemset(r->d_accum, 0, npx * 16), "memset");
+    for (int k = 0; k < rt_renderer::kSets; k++)
+        for (float *b : {r->set_frame[k], r->set_ping[k], r->set_mom[k], r->set_gb[k]})
+            hip_check(hipMemset(b, 0, npx * 16), "memset");
+    for (int k = 0; k < rt_renderer::kSets; k++) {
         hip_check(hipStreamCreateWithFlags(&r->gstream[k], hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreateWithFlags(&r->ev_gdone[k], hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&r->ev_free[k], hipEventDisableTiming), "hipEventCreate");
