@@ -46,8 +46,8 @@ struct GatherLaunch {
     // path queues: when px_state is non-null the frame runs as a primary pass
     // plus one pass per depth (see gather_wavefront_bytes for the sizes)
     void *px_state;          // 5 * n_px float4
-    void *queue[2];          // 3 * n_px float4 each
-    uint32_t *path_count;    // max_depth + 1 counters
+    void *queue[2];          // [1]: 3 * n_px float4, the paths entering depth 1; [0] unused
+    uint32_t *path_count;    // max_depth + 1 counters ([0] tail work counter, [1] queued paths)
 };
 // Device bytes of the path-queue state for n_px pixels.
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth);

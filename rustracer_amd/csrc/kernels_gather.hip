@@ -36,7 +36,7 @@ struct GatherArgs {
     unsigned long long *counters;
     // path-queue (wavefront) state; unused by the one-pass kernels
     float4 *px_state;       // kPixelPlanes planes of n_px float4
-    float4 *queue[2];       // kQueuePlanes planes of n_px float4 each (paths of even / odd depth)
+    float4 *queue[2];       // [1]: kQueuePlanes planes of n_px float4 (paths entering depth 1); [0] unused
     uint32_t *path_count;   // paths queued per depth [max_depth + 1]
     uint32_t n_px;
     uint32_t sample;        // sample index of this pass
@@ -579,10 +579,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
 
 // ─────────────────────────── path queues (wavefront) ───────────────────────────
 //
-// The same per-pixel computation split at bounce boundaries. A primary pass
+// The same per-pixel computation split after the first bounce. A primary pass
 // runs sample s of every pixel through its first bounce; paths that go on are
-// appended to a queue (wave-aggregated atomics), and one pass per depth takes
-// the queue of depth b and writes that of depth b + 1. Every pass runs full
+// appended to a queue (wave-aggregated atomics), and a tail pass runs each
+// queued path through its remaining bounces. The tail pass runs full
 // wavefronts of live paths, so the few long paths (glass, metal chains) no
 // longer hold a wavefront of finished pixels. Each pixel's RNG stream, its
 // accumulators and every arithmetic step are those of gather_pixel, in the
@@ -710,10 +710,13 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     }
 }
 
-// Depth b >= 1: the paths queued by the previous pass, grid-stride over the
-// count the previous pass left in path_count[b].
+// The paths that survive the primary pass, compacted: each lane runs one path
+// through all its remaining bounces (depth 1 .. max_depth - 1) and closes its
+// sample. Wavefronts take 64 paths at a time from an atomic counter
+// (path_count[0]), so the long glass/metal chains no longer keep wavefronts of
+// finished pixels resident.
 template <int MODE, bool EARLY, bool COUNT>
-__global__ __launch_bounds__(64) void gather_bounce_kernel(GatherArgs A, uint32_t b) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gather_tail_kernel(GatherArgs A) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
@@ -722,39 +725,32 @@ __global__ __launch_bounds__(64) void gather_bounce_kernel(GatherArgs A, uint32_
     stk.stride = 64u;
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
-    const uint32_t count = __builtin_amdgcn_readfirstlane(A.path_count[b]);
-    const float4 *qin = A.queue[b & 1u];
-    float4 *qout = A.queue[(b + 1u) & 1u];
+    const uint32_t count = __builtin_amdgcn_readfirstlane(A.path_count[1]);
+    const float4 *qin = A.queue[1];
     const uint32_t spp = max(1u, A.ip.spp);
     const Env env = find_env(A);
-    for (uint32_t base = blockIdx.x * 64u; base < count; base += gridDim.x * 64u) {
+    for (;;) {
+        uint32_t base = 0u;
+        if (threadIdx.x == 0) base = atomicAdd(&A.path_count[0], 64u);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+        if (base >= count) break;
         const uint32_t i = base + threadIdx.x;
-        bool push = false;
-        F3 ro, rd, thr;
-        Rng rng{0u};
-        uint32_t idx = 0u;
         if (i < count) {
             const float4 q0 = qin[i], q1 = qin[A.n_px + i], q2 = qin[2u * A.n_px + i];
-            ro = xyz(q0);
-            idx = __float_as_uint(q0.w);
-            rd = xyz(q1);
-            rng.state = __float_as_uint(q1.w);
-            thr = xyz(q2);
+            F3 ro = xyz(q0);
+            const uint32_t idx = __float_as_uint(q0.w);
+            F3 rd = xyz(q1);
+            Rng rng{__float_as_uint(q1.w)};
+            F3 thr = xyz(q2);
             Acc acc;
             FirstHit fh{f3(0.0f, 1.0f, 0.0f), 1e30f};  // only depth 0 touches it
             uint32_t unused;
             load_running(A, idx, acc, unused);
-            const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads);
-            if (cont && b + 1u < A.max_depth) {
-                push = true;
-                store_running(A, idx, acc, 0u);
-            } else {
-                load_samples(A, idx, acc, fh);
-                end_path(A, idx, acc, fh, rng.state, spp);
-            }
+            for (uint32_t b = 1u; b < A.max_depth; b++)
+                if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads)) break;
+            load_samples(A, idx, acc, fh);
+            end_path(A, idx, acc, fh, rng.state, spp);
         }
-        const uint32_t slot = wave_append(push, &A.path_count[b + 1u]);
-        if (push) queue_put(qout, A.n_px, slot, ro, rd, thr, idx, rng.state);
     }
     if (COUNT) {
         atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
@@ -918,19 +914,19 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
 }
 
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth) {
-    return (size_t)n_px * 16u * (rtd::kPixelPlanes + 2u * rtd::kQueuePlanes) + 4u * ((size_t)max_depth + 1u);
+    return (size_t)n_px * 16u * (rtd::kPixelPlanes + rtd::kQueuePlanes) + 4u * ((size_t)max_depth + 1u);
 }
 
 template <int M, bool E, bool C>
 static void launch_wavefront_passes(rtd::GatherArgs A, hipStream_t s) {
     const uint32_t spp = A.ip.spp > 0u ? A.ip.spp : 1u;
-    const uint32_t bounce_wgs = std::min<uint32_t>((A.n_px + 63u) / 64u, 4096u);
+    const uint32_t tail_wgs = std::min<uint32_t>((A.n_px + 63u) / 64u, 2048u);
     for (uint32_t k = 0; k < spp; k++) {
         A.sample = k;
         launch_fill_u32(A.path_count, 0u, (size_t)A.max_depth + 1u, s);
         hipLaunchKernelGGL((rtd::gather_primary_kernel<M, E, C>), dim3(A.n_tiles), dim3(64), 0, s, A);
-        for (uint32_t b = 1; b < A.max_depth; b++)
-            hipLaunchKernelGGL((rtd::gather_bounce_kernel<M, E, C>), dim3(bounce_wgs), dim3(64), 0, s, A, b);
+        if (A.max_depth > 1u)
+            hipLaunchKernelGGL((rtd::gather_tail_kernel<M, E, C>), dim3(tail_wgs), dim3(64), 0, s, A);
     }
 }
 

@@ -249,9 +249,9 @@ struct rt_renderer {
             char *w = static_cast<char *>(d_wave);
             const size_t plane = n_px() * 16u;
             g.px_state = w;
-            g.queue[0] = w + 5u * plane;
-            g.queue[1] = w + 8u * plane;
-            g.path_count = reinterpret_cast<uint32_t *>(w + 11u * plane);
+            g.queue[0] = nullptr;
+            g.queue[1] = w + 5u * plane;
+            g.path_count = reinterpret_cast<uint32_t *>(w + 8u * plane);
         }
         return g;
     }
