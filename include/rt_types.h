@@ -39,6 +39,17 @@ extern "C" {
 #define RT_TEX_NONE 0xFFFFFFFFu
 #define RT_MAT_FLAG_ALPHA_CUTOUT 1u
 
+/* Texture roles (lib.rs:502-507) and one role's rgba8 array (device or host). */
+#define RT_TEX_ROLE_ALBEDO 0u
+#define RT_TEX_ROLE_MR 1u
+#define RT_TEX_ROLE_EMISSIVE 2u
+#define RT_TEX_ROLE_NORMAL 3u
+#define RT_TEX_ROLES 4u
+typedef struct rt_tex_array {
+    const uint8_t *texels; /* layers * height * width * 4 bytes, row-major */
+    uint32_t width, height, layers, _pad;
+} rt_tex_array;
+
 /* LightGpu.kind values, scene.rs:147 and lib.rs:1231-1282 */
 #define RT_LIGHT_POINT 0u
 #define RT_LIGHT_DIRECTIONAL 1u

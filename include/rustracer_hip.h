@@ -145,6 +145,15 @@ int rt_scene_bounds(const rt_scene *scene, float bmin[3], float bmax[3]);
 /* Scene::flatten_triangles (scene.rs:213-273). */
 int rt_scene_flatten_triangles(const rt_scene *scene, rt_triangle *out, uint32_t capacity);
 int rt_scene_flatten_materials(const rt_scene *scene, rt_material *out, uint32_t capacity);
+/* Scene::tex_albedo / tex_mr / tex_emissive / tex_normal (rustracer-core
+ * scene.rs, texture.rs): append a width x height RGBA float texture to a role
+ * (RT_TEX_ROLE_*); *layer receives its layer index (the material's tex_* value). */
+int rt_scene_add_texture(rt_scene *scene, uint32_t role, uint32_t width, uint32_t height, const float *rgba,
+                         uint32_t *layer);
+/* The role's rgba8 array as the renderer uploads it (lib.rs:1286-1358); with
+ * texels = NULL only the sizes are returned. */
+int rt_scene_texture_array(const rt_scene *scene, uint32_t role, uint8_t *texels, size_t capacity, uint32_t *width,
+                           uint32_t *height, uint32_t *layers);
 /* light_to_gpu over the scene's lights (lib.rs:1231-1282). */
 int rt_scene_lights_gpu(const rt_scene *scene, rt_light *out, uint32_t capacity);
 int rt_light_to_gpu(const rt_light_desc *light, rt_light *out);
@@ -190,6 +199,11 @@ int rt_renderer_create_raw(const rt_config *cfg, const rt_triangle *tris, uint32
                            const rt_light *lights, uint32_t n_lights, const float *bounds_min,
                            const float *bounds_max, int device, rt_renderer **out);
 void rt_renderer_destroy(rt_renderer *r);
+/* The texture array of one role (RT_TEX_ROLE_*), rgba8 texels layers x height x
+ * width, as create_role_array uploads it (lib.rs:1286-1358); for renderers made
+ * with rt_renderer_create_raw (rt_renderer_create takes the scene's textures). */
+int rt_renderer_set_texture_array(rt_renderer *r, uint32_t role, uint32_t width, uint32_t height, uint32_t layers,
+                                  const uint8_t *texels);
 
 /* record_frame + queue.submit (lib.rs:666-851): gather, a-trous, composite,
  * tonemap. Asynchronous; rt_synchronize waits (device.poll(Wait)). */

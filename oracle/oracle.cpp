@@ -267,17 +267,50 @@ Hit trace_bvh(const or_scene *s, V3 ro, V3 rd, or_stats *st) {
     return hit;
 }
 
-// gather.wgsl:91-103. Alpha-cutout needs a texture; the placeholder texel has
-// alpha 0, so a cutout material with a texture layer lets the ray through.
+// textureSampleLevel(role, sampler, uv, i32(layer), 0.0) (include/rt_texture.h)
+struct V4 {
+    float x, y, z, w;
+    V3 rgb() const { return v3(x, y, z); }
+};
+inline V4 tex(const or_scene *s, uint32_t role, V2 uv, uint32_t layer) {
+    float o[4];
+    rt_tex_sample(&s->tex[role], uv.x, uv.y, layer, o);
+    return V4{o[0], o[1], o[2], o[3]};
+}
+
+// gather.wgsl:91-103: the closest hit decides; a transparent texel of an
+// alpha-cutout material lets the shadow ray through.
 bool shadowed(Ctx &c, V3 origin, V3 dir, float max_dist) {
     const Hit sh = trace_bvh(c.s, origin, dir, &c.st);
     if (sh.t >= max_dist - 0.001f) return false;
     const rt_triangle &bt = c.s->tris[sh.prim_id];
     const rt_material &bm = c.s->mats[bt.material_id];
     if ((bm.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && bm.tex_albedo != RT_TEX_NONE) {
-        return false;  // placeholder alpha 0 < 0.5
+        const V2 uv = barycentric(bt, origin, dir, sh.t).uv;
+        const float alpha = tex(c.s, RT_TEX_ROLE_ALBEDO, uv, bm.tex_albedo).w;
+        if (alpha < 0.5f) return false;
     }
     return true;
+}
+
+// gather.wgsl:119-141
+struct MatParams {
+    V3 albedo;
+    float rough, metal;
+};
+inline MatParams eval_material(const or_scene *s, const rt_material &mat, V2 uv) {
+    MatParams p{v3(mat.albedo[0], mat.albedo[1], mat.albedo[2]), mat.roughness, mat.metallic};
+    if (mat.tex_albedo != RT_TEX_NONE) p.albedo = p.albedo * tex(s, RT_TEX_ROLE_ALBEDO, uv, mat.tex_albedo).rgb();
+    if (mat.tex_mr != RT_TEX_NONE) {
+        const V4 mr = tex(s, RT_TEX_ROLE_MR, uv, mat.tex_mr);
+        p.rough = wclamp(mat.roughness * mr.y, 0.0f, 1.0f);
+        p.metal = wclamp(mat.metallic * mr.z, 0.0f, 1.0f);
+    }
+    return p;
+}
+inline V3 emissive_at(const or_scene *s, const rt_material &mat, V2 uv) {
+    if (mat.tex_emissive != RT_TEX_NONE) return L3(mat.emissive) * tex(s, RT_TEX_ROLE_EMISSIVE, uv, mat.tex_emissive).rgb();
+    return L3(mat.emissive);
 }
 
 // gather.wgsl:107-111
@@ -325,7 +358,10 @@ LightSample direct_light(Ctx &c, V3 hp, V3 sn, Rng *rng) {
         const float v = u2 * su;
         const float w = 1.0f - u - v;
         const V3 lp = L3(tri.v0) * w + L3(tri.v1) * u + L3(tri.v2) * v;
-        V3 le = lcolor;  // emissive texture: TEX_NONE in every supported material
+        const rt_material &lmat = c.s->mats[tri.material_id];
+        const V2 luv = L2(tri.uv0) * w + L2(tri.uv1) * u + L2(tri.uv2) * v;
+        V3 le = lcolor;
+        if (lmat.tex_emissive != RT_TEX_NONE) le = le * tex(c.s, RT_TEX_ROLE_EMISSIVE, luv, lmat.tex_emissive).rgb();
         le = le * L3(c.ip->light_emission);
         const V3 to = lp - hp;
         const float r2 = dot(to, to);
@@ -469,26 +505,33 @@ void gather_pixel(Ctx &c, uint32_t px, uint32_t py, float *frame, float *moments
             const V2 uv_hit = bb.uv;
 
             if ((mat.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && mat.tex_albedo != RT_TEX_NONE) {
-                // placeholder texel alpha 0 < 0.5: skip the surface (gather.wgsl:318-323)
-                ro = hp + rd * 0.002f;
-                continue;
+                if (tex(c.s, RT_TEX_ROLE_ALBEDO, uv_hit, mat.tex_albedo).w < 0.5f) {  // gather.wgsl:318-323
+                    ro = hp + rd * 0.002f;
+                    continue;
+                }
             }
 
             const V3 gn = get_triangle_normal(tr);
             const bool ff = dot(rd, gn) < 0.0f;
             const V3 sn_geom = smooth_normal(tr, bb, gn);
             const V3 sn = ff ? sn_geom : -sn_geom;
-            const V3 shn = sn;  // normal map: TEX_NONE in every supported material
+            V3 shn = sn;
+            if (mat.tex_normal != RT_TEX_NONE) {  // gather.wgsl:335-340
+                const V3 nm = tex(c.s, RT_TEX_ROLE_NORMAL, uv_hit, mat.tex_normal).rgb() * 2.0f - splat(1.0f);
+                const TanFrame fr = tangent_frame(tr, sn);
+                shn = normalize(fr.t * nm.x + fr.b * nm.y + sn * nm.z);
+            }
 
             if (b == 0u && hit.t < first_depth) {
                 first_normal = sn;
                 first_depth = hit.t;
             }
 
-            const V3 albedo = v3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
-            const float rough = mat.roughness;
-            const float metal = mat.metallic;
-            const V3 emissive = L3(mat.emissive);
+            const MatParams params = eval_material(c.s, mat, uv_hit);
+            const V3 albedo = params.albedo;
+            const float rough = params.rough;
+            const float metal = params.metal;
+            const V3 emissive = emissive_at(c.s, mat, uv_hit);
 
             add_radiance(s, acc, thr * emissive);
 
@@ -500,8 +543,14 @@ void gather_pixel(Ctx &c, uint32_t px, uint32_t py, float *frame, float *moments
                     case 3u: acc.col = shn * 0.5f + splat(0.5f); break;
                     case 4u: acc.col = gn * 0.5f + splat(0.5f); break;
                     case 5u: acc.col = emissive; break;
-                    case 8u: case 9u: case 11u: case 18u: acc.col = splat(0.0f); break;
-                    case 12u: acc.col = emissive; break;
+                    case 8u: case 9u: {  // screen-space samples of layer 0
+                        const V2 puv = V2{(float)px / (float)ip.width, (float)py / (float)ip.height};
+                        acc.col = tex(c.s, debug_view == 8u ? RT_TEX_ROLE_EMISSIVE : RT_TEX_ROLE_ALBEDO, puv, 0u).rgb();
+                        break;
+                    }
+                    case 11u: acc.col = tex(c.s, RT_TEX_ROLE_EMISSIVE, uv_hit, 0u).rgb(); break;
+                    case 12u: acc.col = L3(mat.emissive); break;
+                    case 18u: acc.col = tex(c.s, RT_TEX_ROLE_ALBEDO, uv_hit, mat.tex_albedo).rgb(); break;
                     case 13u: acc.col = v3((float)mat.tex_emissive / 65535.0f, 0.0f, 0.0f); break;
                     case 17u:
                         acc.col = v3(uv_hit.x - std::floor(uv_hit.x), uv_hit.y - std::floor(uv_hit.y), 0.0f);
@@ -674,12 +723,8 @@ void gather_pixel(Ctx &c, uint32_t px, uint32_t py, float *frame, float *moments
 }
 
 bool materials_supported(const or_scene *s) {
-    for (uint32_t i = 0; i < s->n_mats; i++) {
-        const rt_material &m = s->mats[i];
-        if (m.tex_albedo != RT_TEX_NONE || m.tex_mr != RT_TEX_NONE || m.tex_emissive != RT_TEX_NONE ||
-            m.tex_normal != RT_TEX_NONE)
-            return false;
-    }
+    for (uint32_t r = 0; r < RT_TEX_ROLES; r++)
+        if (!s->tex[r].texels || s->tex[r].width == 0 || s->tex[r].height == 0 || s->tex[r].layers == 0) return false;
     return s->n_lights > 0;
 }
 
@@ -764,11 +809,18 @@ void photon_one(const or_scene *s, const rt_grid_params *gp, uint32_t photon_ind
         const bool front_face = dot(dir, normal) < 0.0f;
         const V3 sn_geom = smooth_normal(tri, bb, normal);
         const V3 face_normal = front_face ? sn_geom : -sn_geom;
-        const V3 shn = face_normal;  // normal map: TEX_NONE
+        V3 shn = face_normal;
+        if (mat.tex_normal != RT_TEX_NONE) {  // photon_trace.wgsl:193-198
+            const V3 nm = tex(s, RT_TEX_ROLE_NORMAL, bb.uv, mat.tex_normal).rgb() * 2.0f - splat(1.0f);
+            const TanFrame fr = tangent_frame(tri, face_normal);
+            shn = normalize(fr.t * nm.x + fr.b * nm.y + face_normal * nm.z);
+        }
 
         if ((mat.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && mat.tex_albedo != RT_TEX_NONE) {
-            origin = hit_pos + dir * 0.002f;  // placeholder alpha 0 < 0.5
-            continue;
+            if (tex(s, RT_TEX_ROLE_ALBEDO, bb.uv, mat.tex_albedo).w < 0.5f) {  // photon_trace.wgsl:200-206
+                origin = hit_pos + dir * 0.002f;
+                continue;
+            }
         }
         if (mat.tex_emissive != RT_TEX_NONE || length(L3(mat.emissive)) > 1e-4f || mat.kind == 3u) return;
 
@@ -793,7 +845,9 @@ void photon_one(const or_scene *s, const rt_grid_params *gp, uint32_t photon_ind
         ph.incident[0] = -dir.x; ph.incident[1] = -dir.y; ph.incident[2] = -dir.z;
         ph.normal[0] = shn.x; ph.normal[1] = shn.y; ph.normal[2] = shn.z;
 
-        const V3 albedo = v3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
+        // photon_trace.wgsl:84-89 surface_albedo: the texture alone when present
+        const V3 albedo = mat.tex_albedo != RT_TEX_NONE ? tex(s, RT_TEX_ROLE_ALBEDO, bb.uv, mat.tex_albedo).rgb()
+                                                        : v3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
         const V3 f0 = mix(splat(0.04f), albedo, splat(mat.metallic));
         const float ct = std::fabs(dot(dir, shn));
         const V3 fr = fresnel_schlick(ct, f0);
@@ -828,6 +882,11 @@ inline uint32_t photon_cell_u(const rt_grid_params *gp, const float *pos) {
 extern "C" {
 
 uint32_t or_rng_init(uint32_t seed) { return rng_init(seed).state; }
+
+/* textureSampleLevel of one texture array (include/rt_texture.h). */
+void or_tex_sample(const rt_tex_array *t, float u, float v, uint32_t layer, float *out) {
+    rt_tex_sample(t, u, v, layer, out);
+}
 uint32_t or_rng_next(uint32_t *state) {
     Rng r{*state, 1u};
     const uint32_t w = rng_next(&r);

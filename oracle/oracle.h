@@ -12,15 +12,16 @@
  * (dot = (x*x' + y*y') + z*z', select(f, t, c), mix(a, b, t) = a*(1-t) + b*t)
  * and the implementation-defined builtins fixed in include/rt_wgsl_math.h.
  *
- * Not restated (parity unpinned, outside the configs this backend serves):
- * texture sampling (every material must have tex_* == TEX_NONE; a sampled
- * placeholder texel is (0,0,0,0) as in the reference's empty role arrays).
+ * Texture sampling follows include/rt_texture.h (shared with the kernels):
+ * the reference's bilinear/repeat sampler with its implementation-defined
+ * weights fixed there.
  */
 #ifndef RT_ORACLE_H
 #define RT_ORACLE_H
 
 #include <stdint.h>
 
+#include "../include/rt_texture.h"
 #include "../include/rt_types.h"
 
 #ifdef __cplusplus
@@ -38,6 +39,7 @@ typedef struct or_scene {
     uint32_t n_mats;
     const rt_light *lights; /* at least one record (the reference binds a zeroed one when empty) */
     uint32_t n_lights;
+    rt_tex_array tex[RT_TEX_ROLES]; /* albedo, metallic-roughness, emissive, normal (texels non-null) */
 } or_scene;
 
 /* Photon-map inputs of the gather pass; grid_meta holds [start, count] per cell. */
@@ -67,6 +69,8 @@ int or_gather(const or_scene *s, const rt_camera_uniform *cam, const rt_image_pa
 
 int or_photon_trace(const or_scene *s, const rt_grid_params *gp, uint32_t n_photons,
                     uint32_t max_depth, rt_photon *photons, int n_threads);
+
+void or_tex_sample(const rt_tex_array *t, float u, float v, uint32_t layer, float *out);
 
 int or_photon_grid(const rt_photon *photons, uint32_t n_slots, const rt_grid_params *gp,
                    uint32_t meta_cells, uint32_t *grid_meta, rt_photon *sorted,

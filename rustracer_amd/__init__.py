@@ -139,6 +139,25 @@ class Scene:
         check(lib().rt_scene_lights_gpu(self._h, ptr(out), len(out)), "rt_scene_lights_gpu")
         return out
 
+    def add_texture(self, role: int, rgba: np.ndarray) -> int:
+        """Append an (h, w, 4) float RGBA texture to a role (0 albedo, 1 metallic-roughness,
+        2 emissive, 3 normal); returns its layer index (the material's tex_* value)."""
+        a = np.ascontiguousarray(rgba, np.float32)
+        layer = C.c_uint32()
+        check(lib().rt_scene_add_texture(self._h, role, a.shape[1], a.shape[0], ptr(a), C.byref(layer)),
+              "rt_scene_add_texture")
+        return layer.value
+
+    def texture_array(self, role: int) -> np.ndarray:
+        """The role's rgba8 array as uploaded: (layers, h, w, 4) uint8."""
+        w, h, n = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().rt_scene_texture_array(self._h, role, None, 0, C.byref(w), C.byref(h), C.byref(n)),
+              "rt_scene_texture_array")
+        out = np.zeros((n.value, h.value, w.value, 4), np.uint8)
+        check(lib().rt_scene_texture_array(self._h, role, ptr(out), out.size, C.byref(w), C.byref(h), C.byref(n)),
+              "rt_scene_texture_array")
+        return out
+
     def build_bvh(self):
         nn, nprim = C.c_uint32(), C.c_uint32()
         check(lib().rt_scene_build_bvh(self._h, None, 0, C.byref(nn), None, 0, C.byref(nprim)), "rt_scene_build_bvh")
@@ -199,7 +218,7 @@ class Renderer:
         if arrays is None:
             check(lib().rt_renderer_create(C.byref(config), scene.handle, device, C.byref(h)), "rt_renderer_create")
         else:
-            tris, nodes, prims, mats, lights, bounds = arrays
+            tris, nodes, prims, mats, lights, bounds = arrays[:6]
             tris = np.ascontiguousarray(tris, TRIANGLE)
             nodes = np.ascontiguousarray(nodes, BVH_NODE)
             prims = np.ascontiguousarray(prims, np.uint32)
@@ -216,6 +235,16 @@ class Renderer:
         self._h = h
         cfg = self.config
         self.width, self.height = cfg.width, cfg.height
+        if arrays is not None and len(arrays) > 6 and arrays[6] is not None:
+            for role, t in enumerate(arrays[6]):
+                if t is not None:
+                    self.set_texture_array(role, t)
+
+    def set_texture_array(self, role: int, texels: np.ndarray) -> None:
+        """Upload a role's rgba8 array (layers, h, w, 4) uint8 (renderers made from raw arrays)."""
+        t = np.ascontiguousarray(texels, np.uint8)
+        check(lib().rt_renderer_set_texture_array(self._h, role, t.shape[2], t.shape[1], t.shape[0], ptr(t)),
+              "rt_renderer_set_texture_array")
 
     def close(self):
         if getattr(self, "_h", None):

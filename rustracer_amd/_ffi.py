@@ -195,6 +195,9 @@ SIGNATURES = {
     "rt_trace_closest": (_i, [_vp, _vp, _u32, _vp]),
     "rt_trace_closest_device": (_i, [_vp, _vp, _u32, _vp]),
     "rt_traversal_info": (_i, [_vp, _vp, _vp, _vp]),
+    "rt_renderer_set_texture_array": (_i, [_vp, _u32, _u32, _u32, _u32, _vp]),
+    "rt_scene_add_texture": (_i, [_vp, _u32, _u32, _u32, _vp, _P(_u32)]),
+    "rt_scene_texture_array": (_i, [_vp, _u32, _vp, C.c_size_t, _P(_u32), _P(_u32), _P(_u32)]),
     "rt_build_photon_grid_from": (_i, [_vp, _vp, _u32]),
     "rt_get_pass_stats": (_i, [_vp, _P(PassStats)]),
     "rt_reset_pass_stats": (_i, [_vp]),

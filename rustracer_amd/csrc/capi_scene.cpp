@@ -142,6 +142,37 @@ int rt_scene_flatten_materials(const rt_scene *s, rt_material *out, uint32_t cap
     });
 }
 
+int rt_scene_add_texture(rt_scene *s, uint32_t role, uint32_t width, uint32_t height, const float *rgba,
+                         uint32_t *layer) {
+    return api_call([&] {
+        if (!s || !rgba) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (role >= RT_TEX_ROLES) throw ApiError(RT_ERR_INVALID, "texture role out of range");
+        if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26))
+            throw ApiError(RT_ERR_INVALID, "texture size must be non-zero and at most 2^26 texels");
+        rt::Texture t;
+        t.width = width;
+        t.height = height;
+        t.rgba.assign(rgba, rgba + (size_t)width * height * 4);
+        s->scene.textures[role].push_back(std::move(t));
+        if (layer) *layer = (uint32_t)(s->scene.textures[role].size() - 1);
+    });
+}
+
+int rt_scene_texture_array(const rt_scene *s, uint32_t role, uint8_t *texels, size_t capacity, uint32_t *width,
+                           uint32_t *height, uint32_t *layers) {
+    return api_call([&] {
+        if (!s) throw ApiError(RT_ERR_INVALID, "null scene");
+        if (role >= RT_TEX_ROLES) throw ApiError(RT_ERR_INVALID, "texture role out of range");
+        const rt::TextureArray a = rt::build_texture_array(s->scene.textures[role]);
+        if (width) *width = a.width;
+        if (height) *height = a.height;
+        if (layers) *layers = a.layers;
+        if (!texels) return;
+        if (capacity < a.texels.size()) throw ApiError(RT_ERR_INVALID, "capacity below width*height*layers*4");
+        std::memcpy(texels, a.texels.data(), a.texels.size());
+    });
+}
+
 int rt_scene_lights_gpu(const rt_scene *s, rt_light *out, uint32_t cap) {
     return api_call([&] {
         if (!s || !out) throw ApiError(RT_ERR_INVALID, "null argument");

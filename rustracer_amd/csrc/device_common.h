@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/rt_texture.h"
 #include "../../include/rt_types.h"
 #include "../../include/rt_wgsl_math.h"
 
@@ -104,6 +105,17 @@ struct Bary {
     F2 uv;
     float u, v;
 };
+
+// textureSampleLevel(role, sampler, uv, i32(layer), 0.0), include/rt_texture.h.
+struct F4s {
+    float x, y, z, w;
+};
+__device__ __forceinline__ F4s tex_sample(const rt_tex_array &t, F2 uv, uint32_t layer) {
+    float o[4];
+    rt_tex_sample(&t, uv.x, uv.y, layer, o);
+    return F4s{o[0], o[1], o[2], o[3]};
+}
+__device__ __forceinline__ F3 rgb(F4s c) { return F3{c.x, c.y, c.z}; }
 // common.wgsl:178-192
 __device__ __forceinline__ Bary barycentric(const rt_triangle &tri, F3 ro, F3 rd, float t) {
     const F3 v0 = ld3(tri.v0);

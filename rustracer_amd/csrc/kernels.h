@@ -45,6 +45,7 @@ struct GatherLaunch {
     unsigned long long *counters;  // non-null: count traffic (4 counters)
     // path queues: when px_state is non-null the frame runs as a primary pass
     // plus one pass per depth (see gather_wavefront_bytes for the sizes)
+    rt_tex_array tex[RT_TEX_ROLES];  // texture arrays (device texels)
     void *px_state;          // 5 * n_px float4
     void *queue[2];          // [1]: 3 * n_px float4, the paths entering depth 1; [0] unused
     uint32_t *path_count;    // max_depth + 1 counters ([0] tail work counter, [1] queued paths)
@@ -88,6 +89,7 @@ struct PhotonLaunch {
     uint32_t max_depth;  // slots per photon
     uint32_t mode;
     rt_photon *photons;
+    rt_tex_array tex[RT_TEX_ROLES];
 };
 hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s);
 

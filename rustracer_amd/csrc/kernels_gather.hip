@@ -40,6 +40,7 @@ struct GatherArgs {
     uint32_t *path_count;   // paths queued per depth [max_depth + 1]
     uint32_t n_px;
     uint32_t sample;        // sample index of this pass
+    rt_tex_array tex[RT_TEX_ROLES];  // albedo, metallic-roughness, emissive, normal
 };
 
 struct LightSample {
@@ -57,16 +58,53 @@ struct Tracer {
         if (COUNT) queries++;
         return trace_mode<MODE, false, COUNT>(ts, ro, rd, kInf, stk, cnt);
     }
-    // gather.wgsl:91-103 without alpha-cutout materials (no textures here):
-    // occluded iff the closest hit is nearer than max_dist - 0.001. EARLY
-    // stops the same traversal as soon as a hit that near is found.
-    __device__ bool shadowed(F3 origin, F3 dir, float max_dist) {
+    // gather.wgsl:92-93: is the closest hit nearer than max_dist - 0.001? EARLY
+    // stops the same traversal as soon as a hit that near is found (then `h`
+    // is that hit, not necessarily the closest).
+    __device__ bool shadowed(F3 origin, F3 dir, float max_dist, Hit &h) {
         if (COUNT) queries++;
         const float lim = max_dist - 0.001f;
-        const Hit h = trace_mode<MODE, EARLY, COUNT>(ts, origin, dir, lim, stk, cnt);
+        h = trace_mode<MODE, EARLY, COUNT>(ts, origin, dir, lim, stk, cnt);
         return !(h.t >= lim);
     }
 };
+
+// gather.wgsl:91-103 with alpha-cutout: the closest hit decides, and a
+// transparent texel of a cutout material lets the shadow ray through. Scenes
+// with cutout materials run with EARLY = false (the closest hit is needed).
+template <class T>
+__device__ bool occluded(const GatherArgs &A, T &tr, F3 origin, F3 dir, float max_dist) {
+    Hit h;
+    if (!tr.shadowed(origin, dir, max_dist, h)) return false;
+    const rt_triangle &bt = A.tris[h.prim];
+    const rt_material &bm = A.mats[bt.material_id];
+    if ((bm.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && bm.tex_albedo != RT_TEX_NONE) {
+        const F2 uv = barycentric(bt, origin, dir, h.t).uv;
+        if (tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], uv, bm.tex_albedo).w < 0.5f) return false;
+    }
+    return true;
+}
+
+// gather.wgsl:119-141
+struct MatParams {
+    F3 albedo;
+    float rough, metal;
+};
+__device__ __forceinline__ MatParams eval_material(const GatherArgs &A, const rt_material &mat, F2 uv) {
+    MatParams p{f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]), mat.roughness, mat.metallic};
+    if (mat.tex_albedo != RT_TEX_NONE) p.albedo = p.albedo * rgb(tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], uv, mat.tex_albedo));
+    if (mat.tex_mr != RT_TEX_NONE) {
+        const F4s mr = tex_sample(A.tex[RT_TEX_ROLE_MR], uv, mat.tex_mr);
+        p.rough = wclamp(mat.roughness * mr.y, 0.0f, 1.0f);
+        p.metal = wclamp(mat.metallic * mr.z, 0.0f, 1.0f);
+    }
+    return p;
+}
+__device__ __forceinline__ F3 emissive_at(const GatherArgs &A, const rt_material &mat, F2 uv) {
+    if (mat.tex_emissive != RT_TEX_NONE)
+        return ld3(mat.emissive) * rgb(tex_sample(A.tex[RT_TEX_ROLE_EMISSIVE], uv, mat.tex_emissive));
+    return ld3(mat.emissive);
+}
 
 // gather.wgsl:107-111
 __device__ __forceinline__ F3 sky_color(F3 dir, F3 zenith, F3 horizon, float intensity) {
@@ -88,14 +126,14 @@ __device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rn
         const float r = wgsl_sqrt(r2);
         const F3 ld = to / r;
         if (dot(sn, ld) <= 0.0f) return LightSample{fs(0.0f), ld};
-        if (tr.shadowed(hp + sn * 0.01f, ld, r - 0.001f)) return LightSample{fs(0.0f), ld};
+        if (occluded(A, tr, hp + sn * 0.01f, ld, r - 0.001f)) return LightSample{fs(0.0f), ld};
         const F3 ple = ld3(A.ip.point_light_emission);
         return LightSample{lcolor * l.intensity * ple * wmax(dot(sn, ld), 0.0f) / (r2 + 0.01f) * (float)n, ld};
     } else if (l.kind == 1u) {
         const F3 ld = normalize(ldata);
         const float ndl = dot(sn, ld);
         if (ndl <= 0.0f) return LightSample{fs(0.0f), ld};
-        if (tr.shadowed(hp + sn * 0.01f, ld, 1e30f)) return LightSample{fs(0.0f), ld};
+        if (occluded(A, tr, hp + sn * 0.01f, ld, 1e30f)) return LightSample{fs(0.0f), ld};
         return LightSample{lcolor * l.intensity * ndl * (float)n, ld};
     } else if (l.kind == 2u) {
         uint32_t ti = wgsl_u32(l.data[0]);
@@ -109,7 +147,13 @@ __device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rn
         const float w = 1.0f - u - v;
         const F3 v0 = ld3(tri.v0), v1 = ld3(tri.v1), v2 = ld3(tri.v2);
         const F3 lp = v0 * w + v1 * u + v2 * v;
-        const F3 le = lcolor * ld3(A.ip.light_emission);
+        // gather.wgsl:180-189: the emissive texture at the sampled point tints the light
+        const rt_material &lmat = A.mats[tri.material_id];
+        const F2 luv = F2{(tri.uv0[0] * w + tri.uv1[0] * u) + tri.uv2[0] * v,
+                          (tri.uv0[1] * w + tri.uv1[1] * u) + tri.uv2[1] * v};
+        F3 le = lcolor;
+        if (lmat.tex_emissive != RT_TEX_NONE) le = le * rgb(tex_sample(A.tex[RT_TEX_ROLE_EMISSIVE], luv, lmat.tex_emissive));
+        le = le * ld3(A.ip.light_emission);
         const F3 to = lp - hp;
         const float r2 = dot(to, to);
         const float r = wgsl_sqrt(r2);
@@ -119,7 +163,7 @@ __device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rn
         const F3 ln = normalize(cross(v1 - v0, v2 - v0));
         const float ndl_l = dot(ln, -ld);
         if (ndl_l <= 0.0f) return LightSample{fs(0.0f), ld};
-        if (tr.shadowed(hp + sn * 0.01f, ld, r - 0.001f)) return LightSample{fs(0.0f), ld};
+        if (occluded(A, tr, hp + sn * 0.01f, ld, r - 0.001f)) return LightSample{fs(0.0f), ld};
         const float area = 0.5f * length(cross(v1 - v0, v2 - v0));
         return LightSample{le * l.intensity * ndl * ndl_l * area * (float)n / (r2 + 0.01f), ld};
     }
@@ -234,7 +278,8 @@ struct FirstHit {
 // (ro, rd and thr then hold the next ray), false when it ends.
 template <int MODE, bool EARLY, bool COUNT>
 __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, COUNT> &tr, uint32_t b, F3 &ro,
-                       F3 &rd, F3 &thr, Rng &rng, Acc &acc, FirstHit &fh, uint32_t &photon_reads) {
+                       F3 &rd, F3 &thr, Rng &rng, Acc &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
+                       uint32_t py) {
     const rt_image_params &ip = A.ip;
     const uint32_t debug_view = ip._pad[0];
     const Hit hit = tr.closest(ro, rd);
@@ -246,30 +291,52 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
     const rt_material &mat = A.mats[trg.material_id];
     const F3 hp = ro + rd * hit.t;
     const Bary bb = barycentric(trg, ro, rd, hit.t);
+    // gather.wgsl:315-323: a transparent texel of a cutout material is skipped
+    // (the path goes on from just behind it; the bounce count still advances)
+    if ((mat.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && mat.tex_albedo != RT_TEX_NONE) {
+        if (tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], bb.uv, mat.tex_albedo).w < 0.5f) {
+            ro = hp + rd * 0.002f;
+            return true;
+        }
+    }
     const F3 gn = get_triangle_normal(trg);
     const bool ff = dot(rd, gn) < 0.0f;
     const F3 sn_geom = smooth_normal(trg, bb, gn);
     const F3 sn = ff ? sn_geom : -sn_geom;
-    const F3 shn = sn;
+    F3 shn = sn;
+    if (mat.tex_normal != RT_TEX_NONE) {  // gather.wgsl:334-340
+        const F3 nm = rgb(tex_sample(A.tex[RT_TEX_ROLE_NORMAL], bb.uv, mat.tex_normal)) * 2.0f - fs(1.0f);
+        const TanFrame frm = tangent_frame(trg, sn);
+        shn = normalize(frm.t * nm.x + frm.b * nm.y + sn * nm.z);
+    }
 
     if (b == 0u && hit.t < fh.depth) {
         fh.normal = sn;
         fh.depth = hit.t;
     }
-    const F3 albedo = f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
-    const float rough = mat.roughness;
-    const float metal = mat.metallic;
-    const F3 emissive = ld3(mat.emissive);
+    const MatParams params = eval_material(A, mat, bb.uv);
+    const F3 albedo = params.albedo;
+    const float rough = params.rough;
+    const float metal = params.metal;
+    const F3 emissive = emissive_at(A, mat, bb.uv);
     acc.add(thr * emissive);
 
     if (debug_view != 0u) {
         bool stop = true;
         switch (debug_view) {
             case 1u: acc.col = f3(bb.uv.x, bb.uv.y, 0.0f); break;
-            case 2u: case 3u: acc.col = sn * 0.5f + fs(0.5f); break;
+            case 2u: acc.col = sn * 0.5f + fs(0.5f); break;
+            case 3u: acc.col = shn * 0.5f + fs(0.5f); break;
             case 4u: acc.col = gn * 0.5f + fs(0.5f); break;
-            case 5u: case 12u: acc.col = emissive; break;
-            case 8u: case 9u: case 11u: case 18u: acc.col = fs(0.0f); break;
+            case 5u: acc.col = emissive; break;
+            case 12u: acc.col = ld3(mat.emissive); break;
+            case 8u: case 9u: {  // screen-space samples of layer 0 (gather.wgsl:365-376)
+                const F2 puv = F2{(float)px / (float)ip.width, (float)py / (float)ip.height};
+                acc.col = rgb(tex_sample(A.tex[debug_view == 8u ? RT_TEX_ROLE_EMISSIVE : RT_TEX_ROLE_ALBEDO], puv, 0u));
+                break;
+            }
+            case 11u: acc.col = rgb(tex_sample(A.tex[RT_TEX_ROLE_EMISSIVE], bb.uv, 0u)); break;
+            case 18u: acc.col = rgb(tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], bb.uv, mat.tex_albedo)); break;
             case 13u: acc.col = f3((float)mat.tex_emissive / 65535.0f, 0.0f, 0.0f); break;
             case 17u: acc.col = f3(bb.uv.x - floorf(bb.uv.x), bb.uv.y - floorf(bb.uv.y), 0.0f); break;
             case 19u: acc.col = tangent_frame(trg, sn).t * 0.5f + fs(0.5f); break;
@@ -540,7 +607,7 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
         camera_ray(V, px, py, rng, ro, rd);
         F3 thr = fs(1.0f);
         for (uint32_t b = 0u; b < A.max_depth; b++)
-            if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads)) break;
+            if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads, px, py)) break;
         acc.end_sample();
     }
     finalize(A, acc, spp, fh, out_idx);
@@ -691,7 +758,7 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
         camera_ray(V, px, py, rng, ro, rd);
         thr = fs(1.0f);
         const Env env = find_env(A);
-        const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, 0u, ro, rd, thr, rng, acc, fh, photon_reads);
+        const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, 0u, ro, rd, thr, rng, acc, fh, photon_reads, px, py);
         if (cont && A.max_depth > 1u) {
             push = true;
             store_running(A, idx, acc, 0u);
@@ -747,7 +814,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
             uint32_t unused;
             load_running(A, idx, acc, unused);
             for (uint32_t b = 1u; b < A.max_depth; b++)
-                if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads)) break;
+                if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads,
+                                                idx % A.ip.width, idx / A.ip.width))
+                    break;
             load_samples(A, idx, acc, fh);
             end_path(A, idx, acc, fh, rng.state, spp);
         }
@@ -910,6 +979,7 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
     A.path_count = g.path_count;
     A.n_px = g.ip.width * g.ip.height;
     A.sample = 0u;
+    for (uint32_t k = 0; k < RT_TEX_ROLES; k++) A.tex[k] = g.tex[k];
     return A;
 }
 

@@ -30,6 +30,7 @@ struct PhotonArgs {
     uint32_t n_photons;
     uint32_t max_depth;
     rt_photon *photons;
+    rt_tex_array tex[RT_TEX_ROLES];
 };
 
 // photon_trace.wgsl:91-97
@@ -123,7 +124,19 @@ __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
         const bool front_face = dot(dir, normal) < 0.0f;
         const F3 sn_geom = smooth_normal(tri, bb, normal);
         const F3 face_normal = front_face ? sn_geom : -sn_geom;
-        const F3 shn = face_normal;
+        F3 shn = face_normal;
+        if (mat.tex_normal != RT_TEX_NONE) {  // photon_trace.wgsl:193-198
+            const F3 nm = rgb(tex_sample(A.tex[RT_TEX_ROLE_NORMAL], bb.uv, mat.tex_normal)) * 2.0f - fs(1.0f);
+            const TanFrame frm = tangent_frame(tri, face_normal);
+            shn = normalize(frm.t * nm.x + frm.b * nm.y + face_normal * nm.z);
+        }
+        // photon_trace.wgsl:200-206: pass through a transparent texel of a cutout material
+        if ((mat.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && mat.tex_albedo != RT_TEX_NONE) {
+            if (tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], bb.uv, mat.tex_albedo).w < 0.5f) {
+                origin = hit_pos + dir * 0.002f;
+                continue;
+            }
+        }
         if (mat.tex_emissive != RT_TEX_NONE || length(ld3(mat.emissive)) > 1e-4f || mat.kind == 3u) return;
         if (mat.kind == 2u) {
             const F3 f0 = fs(wgsl_pow((mat.ior - 1.0f) / (mat.ior + 1.0f), 2.0f));
@@ -145,7 +158,9 @@ __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
         rec[2] = make_float4(-dir.x, -dir.y, -dir.z, 0.0f);
         rec[3] = make_float4(shn.x, shn.y, shn.z, 0.0f);
 
-        const F3 albedo = f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
+        // photon_trace.wgsl:84-89 surface_albedo: the texture alone when present
+        const F3 albedo = mat.tex_albedo != RT_TEX_NONE ? rgb(tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], bb.uv, mat.tex_albedo))
+                                                        : f3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
         const F3 f0 = mix(fs(0.04f), albedo, fs(mat.metallic));
         const float ct = __builtin_fabsf(dot(dir, shn));
         const F3 fr = fresnel_schlick(ct, f0);
@@ -382,6 +397,7 @@ hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s) {
     A.n_photons = p.n_photons;
     A.max_depth = p.max_depth;
     A.photons = p.photons;
+    for (uint32_t k = 0; k < RT_TEX_ROLES; k++) A.tex[k] = p.tex[k];
     const dim3 grid((p.n_photons + 63u) / 64u), block(64);
     if (p.mode == 0) hipLaunchKernelGGL((rtd::photon_trace_kernel<0>), grid, block, 0, s, A);
     else if (p.mode == 1) hipLaunchKernelGGL((rtd::photon_trace_kernel<1>), grid, block, 0, s, A);

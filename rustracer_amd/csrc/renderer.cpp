@@ -99,6 +99,10 @@ struct rt_renderer {
     double *d_relmse = nullptr;  // reduction scratch of rt_accum64_relmse
     void *d_wave = nullptr;  // path-queue state of the camera pass
     size_t wave_cap = 0;
+    // material texture arrays (lib.rs:1286-1358), one per role
+    uint8_t *d_tex[RT_TEX_ROLES] = {};
+    rt_tex_array tex[RT_TEX_ROLES] = {};
+    bool has_cutout = false;  // an alpha-cutout material with an albedo texture
     // learned denoiser: attached network, its output image, the albedo guide
     rt_unet *unet = nullptr;
     float *d_display = nullptr;
@@ -148,7 +152,8 @@ struct rt_renderer {
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
-                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo};
+                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo,
+                        d_tex[0], d_tex[1], d_tex[2], d_tex[3]};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -240,8 +245,10 @@ struct rt_renderer {
         g.gbuffer = gbuf;
         g.max_depth = cfg.max_depth;
         g.mode = traversal_mode();
-        g.early_shadow = (cfg.flags & RT_FLAG_CLOSEST_SHADOW) ? 0u : 1u;
+        // alpha-cutout shadows need the closest hit (gather.wgsl:91-103)
+        g.early_shadow = ((cfg.flags & RT_FLAG_CLOSEST_SHADOW) || has_cutout) ? 0u : 1u;
         g.counters = nullptr;
+        for (uint32_t k = 0; k < RT_TEX_ROLES; k++) g.tex[k] = tex[k];
         g.px_state = nullptr;
         g.queue[0] = g.queue[1] = nullptr;
         g.path_count = nullptr;
@@ -415,6 +422,7 @@ struct rt_renderer {
         p.gp = gp;
         p.n_photons = traced_photons();
         p.max_depth = cfg.max_depth;
+        for (uint32_t k = 0; k < RT_TEX_ROLES; k++) p.tex[k] = tex[k];
         p.mode = traversal_mode();
         p.photons = d_photons;
         hipEvent_t ev;
@@ -683,17 +691,23 @@ static void build_trace_layouts(rt_renderer &r, const rt_triangle *tris, const r
     hip_check(hipMemcpy(r.d_w4nodes, w4.data(), w4.size() * 4, hipMemcpyHostToDevice), "upload 4-wide nodes");
 }
 
+static void upload_texture_array(rt_renderer &r, uint32_t role, const rt::TextureArray &a) {
+    if (a.width == 0 || a.height == 0 || a.layers == 0 ||
+        a.texels.size() != (size_t)a.width * a.height * a.layers * 4)
+        throw ApiError(RT_ERR_INVALID, "texture array size mismatch");
+    uint8_t *d = dev_alloc<uint8_t>(a.texels.size(), "texture array");
+    hip_check(hipMemcpy(d, a.texels.data(), a.texels.size(), hipMemcpyHostToDevice), "upload texture array");
+    if (r.d_tex[role]) hip_check(hipFree(r.d_tex[role]), "free texture array");
+    r.d_tex[role] = d;
+    r.tex[role] = rt_tex_array{d, a.width, a.height, a.layers, 0u};
+}
+
 static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *tris, uint32_t n_tris,
                                     const rt_bvh_node *nodes, uint32_t n_nodes, const uint32_t *prims, uint32_t n_prims,
                                     const rt_material *mats, uint32_t n_mats, const rt_light *lights, uint32_t n_lights,
-                                    const float *bmin, const float *bmax, int device) {
+                                    const float *bmin, const float *bmax, int device,
+                                    const rt::TextureArray *textures) {
     validate_config(cfg_in);
-    for (uint32_t i = 0; i < n_mats; i++) {
-        const rt_material &m = mats[i];
-        if (m.tex_albedo != RT_TEX_NONE || m.tex_mr != RT_TEX_NONE || m.tex_emissive != RT_TEX_NONE ||
-            m.tex_normal != RT_TEX_NONE)
-            throw ApiError(RT_ERR_UNSUPPORTED, "textured materials are not supported by this backend");
-    }
     for (uint32_t i = 0; i < n_tris; i++)
         if (tris[i].material_id >= std::max(n_mats, 1u))
             throw ApiError(RT_ERR_INVALID, "triangle material id out of range");
@@ -708,6 +722,8 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->n_prims = n_prims;
     r->n_mats = n_mats;
     r->n_lights = std::max(n_lights, 1u);
+    for (uint32_t i = 0; i < n_mats; i++)
+        if ((mats[i].flags & RT_MAT_FLAG_ALPHA_CUTOUT) && mats[i].tex_albedo != RT_TEX_NONE) r->has_cutout = true;
 
     // scene upload (lib.rs:234-271); empty arrays bind one zeroed record (buffer.rs:19-34)
     r->d_tris = dev_alloc<rt_triangle>(n_tris, "triangles");
@@ -773,6 +789,10 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->d_nvalid = dev_alloc<uint32_t>(1, "n_valid");
     r->d_counters = dev_alloc<unsigned long long>(4, "counters");
 
+    // texture arrays: the scene's, else the reference's 1x1 placeholder per role
+    for (uint32_t k = 0; k < RT_TEX_ROLES; k++)
+        upload_texture_array(*r, k, textures ? textures[k] : rt::TextureArray{});
+
     // default camera (lib.rs:266-270)
     r->cam = rt::camera_to_uniform(rt::camera_default(), r->cfg.width, r->cfg.height);
     hip_check(hipDeviceSynchronize(), "upload sync");
@@ -829,7 +849,25 @@ int rt_renderer_create_raw(const rt_config *cfg, const rt_triangle *tris, uint32
             (n_lights && !lights))
             throw ApiError(RT_ERR_INVALID, "null array with non-zero count");
         *out = create_renderer(*cfg, tris, n_tris, nodes, n_nodes, prims, n_prims, mats, n_mats, lights, n_lights,
-                               bounds_min, bounds_max, device);
+                               bounds_min, bounds_max, device, nullptr);
+    });
+}
+
+int rt_renderer_set_texture_array(rt_renderer *r, uint32_t role, uint32_t width, uint32_t height, uint32_t layers,
+                                  const uint8_t *texels) {
+    return api_call([&] {
+        if (!r || !texels) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (role >= RT_TEX_ROLES) throw ApiError(RT_ERR_INVALID, "texture role out of range");
+        if (width == 0 || height == 0 || layers == 0 || (uint64_t)width * height * layers > (1ull << 28))
+            throw ApiError(RT_ERR_INVALID, "texture array size out of range");
+        hip_check(hipStreamSynchronize(r->stream), "sync");
+        rt::TextureArray a;
+        a.width = width;
+        a.height = height;
+        a.layers = layers;
+        a.texels.assign(texels, texels + (size_t)width * height * layers * 4);
+        upload_texture_array(*r, role, a);
+        r->albedo_ok = false;
     });
 }
 
@@ -843,10 +881,12 @@ int rt_renderer_create(const rt_config *cfg, const rt_scene *scene, int device, 
         rt::Vec3 bmin, bmax;
         const bool hb = s.bounds(bmin, bmax);
         const float lo[3] = {bmin.x, bmin.y, bmin.z}, hi[3] = {bmax.x, bmax.y, bmax.z};
+        rt::TextureArray arrays[RT_TEX_ROLES];
+        for (uint32_t k = 0; k < RT_TEX_ROLES; k++) arrays[k] = rt::build_texture_array(s.textures[k]);
         *out = create_renderer(*cfg, tris.data(), (uint32_t)tris.size(), bvh.nodes.data(), (uint32_t)bvh.nodes.size(),
                                bvh.prim_indices.data(), (uint32_t)bvh.prim_indices.size(), s.materials.data(),
                                (uint32_t)s.materials.size(), lights.data(), (uint32_t)lights.size(),
-                               hb ? lo : nullptr, hb ? hi : nullptr, device);
+                               hb ? lo : nullptr, hb ? hi : nullptr, device, arrays);
     });
 }
 

@@ -9,6 +9,7 @@
 #include "scene.h"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <stdexcept>
 
@@ -588,6 +589,85 @@ static std::unique_ptr<Scene> scene_cornell_box() {
     return s;
 }
 
+// The Cornell box of scene_cornell_box with every texture role in use, for
+// parity of the texture branches: a checker albedo on the floor (uv 0..3, so
+// the sampler's repeat wraps), a tangent-space normal map on the back wall,
+// a metallic-roughness map on a box, an emissive map on the ceiling light (and
+// on its area-light samples), and an alpha-cutout quad whose albedo layer is
+// smaller than the array (the reference leaves the rest of the layer zero,
+// i.e. transparent).
+static std::unique_ptr<Scene> scene_textured() {
+    auto s = scene_cornell_box();
+    auto tex = [](uint32_t w, uint32_t h, auto f) {
+        Texture t;
+        t.width = w;
+        t.height = h;
+        t.rgba.resize((size_t)w * h * 4);
+        for (uint32_t y = 0; y < h; y++)
+            for (uint32_t x = 0; x < w; x++) {
+                const std::array<float, 4> c = f(x, y);
+                for (int k = 0; k < 4; k++) t.rgba[((size_t)y * w + x) * 4 + k] = c[k];
+            }
+        return t;
+    };
+    // albedo layer 0: 8x8-texel checker; layer 1: a 16x16 disc mask (alpha)
+    s->textures[0].push_back(tex(64, 64, [](uint32_t x, uint32_t y) {
+        const bool a = ((x / 8) + (y / 8)) % 2 == 0;
+        return std::array<float, 4>{a ? 0.9f : 0.25f, a ? 0.85f : 0.3f, a ? 0.8f : 0.6f, 1.0f};
+    }));
+    s->textures[0].push_back(tex(16, 16, [](uint32_t x, uint32_t y) {
+        const float dx = (float)x - 7.5f, dy = (float)y - 7.5f;
+        const float r = std::sqrt(dx * dx + dy * dy) / 8.0f;
+        return std::array<float, 4>{0.9f, 0.4f + 0.3f * r, 0.2f, r < 0.8f ? 1.0f : 0.0f};
+    }));
+    // metallic-roughness: G = roughness ramp, B = metallic stripes
+    s->textures[1].push_back(tex(32, 32, [](uint32_t x, uint32_t y) {
+        return std::array<float, 4>{0.0f, (float)x / 31.0f, (y / 4) % 2 ? 1.0f : 0.1f, 1.0f};
+    }));
+    // emissive: warm centre fading to the rim
+    s->textures[2].push_back(tex(16, 16, [](uint32_t x, uint32_t y) {
+        const float dx = ((float)x - 7.5f) / 8.0f, dy = ((float)y - 7.5f) / 8.0f;
+        const float v = std::max(0.0f, 1.0f - 0.7f * (dx * dx + dy * dy));
+        return std::array<float, 4>{v, 0.9f * v, 0.7f * v, 1.0f};
+    }));
+    // normal map: sinusoidal bumps (tangent space, encoded n * 0.5 + 0.5)
+    s->textures[3].push_back(tex(32, 32, [](uint32_t x, uint32_t y) {
+        const float a = 0.5f * std::sin((float)x * 0.785398f), b = 0.5f * std::cos((float)y * 0.785398f);
+        const float inv = 1.0f / std::sqrt(1.0f + a * a + b * b);
+        return std::array<float, 4>{a * inv * 0.5f + 0.5f, b * inv * 0.5f + 0.5f, inv * 0.5f + 0.5f, 1.0f};
+    }));
+    // floor: checker, uv 0..3
+    const uint32_t floor_mat = (uint32_t)s->materials.size();
+    rt_material fm = material_lambertian({1.0f, 1.0f, 1.0f});
+    fm.tex_albedo = 0;
+    s->materials.push_back(fm);
+    s->meshes[0].material_id = floor_mat;
+    for (auto &uv : s->meshes[0].texcoords) uv = Vec2(uv.x * 3.0f, uv.y * 3.0f);
+    // back wall: normal map
+    rt_material wm = material_lambertian({0.73f, 0.73f, 0.73f});
+    wm.tex_normal = 0;
+    s->materials.push_back(wm);
+    s->meshes[2].material_id = floor_mat + 1;
+    // ceiling light: emissive map (the area lights keep their triangles 10, 11)
+    rt_material lm = material_emissive({25.0f, 20.0f, 14.0f}, 1.0f);
+    lm.tex_emissive = 0;
+    s->materials.push_back(lm);
+    s->meshes[5].material_id = floor_mat + 2;
+    // tall box: metallic-roughness map
+    rt_material mm = material_metal({0.9f, 0.8f, 0.7f}, 0.8f);
+    mm.metallic = 1.0f;
+    mm.tex_mr = 0;
+    s->materials.push_back(mm);
+    for (int f = 0; f < 6; f++) s->meshes[6 + f].material_id = floor_mat + 3;
+    // alpha-cutout quad standing in the room, albedo layer 1
+    rt_material cm = material_lambertian({1.0f, 1.0f, 1.0f});
+    cm.tex_albedo = 1;
+    cm.flags = RT_MAT_FLAG_ALPHA_CUTOUT;
+    s->materials.push_back(cm);
+    add_quad(*s, {-0.9f, -1.0f, 0.3f}, {0.1f, -1.0f, 0.3f}, {0.1f, 0.8f, 0.3f}, {-0.9f, 0.8f, 0.3f}, floor_mat + 4);
+    return s;
+}
+
 static std::unique_ptr<Scene> scene_dark_room() {
     // Scene::dark_room, scene.rs:386-460
     auto s = std::make_unique<Scene>();
@@ -811,8 +891,31 @@ static std::unique_ptr<Scene> scene_training(uint64_t seed) {
     return s;
 }
 
+TextureArray build_texture_array(const std::vector<Texture> &textures) {
+    TextureArray a;
+    if (textures.empty()) return a;
+    a.width = std::max(textures[0].width, 1u);
+    a.height = std::max(textures[0].height, 1u);
+    a.layers = (uint32_t)textures.size();
+    a.texels.assign((size_t)a.width * a.height * a.layers * 4, 0);
+    for (uint32_t l = 0; l < a.layers; l++) {
+        const Texture &t = textures[l];
+        const uint32_t w = std::min(t.width, a.width), h = std::min(t.height, a.height);
+        for (uint32_t y = 0; y < h; y++)
+            for (uint32_t x = 0; x < w; x++)
+                for (int c = 0; c < 4; c++) {
+                    const float v = t.rgba[((size_t)y * t.width + x) * 4 + c];
+                    // Rust: clamp keeps NaN and `NaN as u8` is 0
+                    const float q = (v == v) ? std::min(std::max(v, 0.0f), 1.0f) * 255.0f : 0.0f;
+                    a.texels[(((size_t)l * a.height + y) * a.width + x) * 4 + c] = (uint8_t)q;
+                }
+    }
+    return a;
+}
+
 std::unique_ptr<Scene> builtin_scene(const std::string &name, uint64_t seed) {
     if (name == "cornell_box") return scene_cornell_box();
+    if (name == "textured") return scene_textured();
     if (name == "dark_room") return scene_dark_room();
     if (name == "smoke") return scene_smoke();
     if (name == "cornell36") return scene_cornell36();
@@ -911,7 +1014,7 @@ bool builtin_camera(const std::string &name, uint64_t seed, uint32_t index, rt_c
         S(c.up, {0.f, 1.f, 0.f});
         c.fov_degrees = fov;
     };
-    if (name == "cornell_box") set({0.0f, 0.5f, 3.0f}, {0, 0, 0}, 50.0f);  // cornell.rs:32-38
+    if (name == "cornell_box" || name == "textured") set({0.0f, 0.5f, 3.0f}, {0, 0, 0}, 50.0f);  // cornell.rs:32-38
     else if (name == "cornell36") set({0.5f, 0.5f, 2.4f}, {0.5f, 0.5f, 0.0f}, 40.0f);
     else if (name == "photon_plain") set({0.0f, 1.3f, 3.4f}, {0.0f, 0.45f, 0.0f}, 45.0f);
     else if (name == "fullframe") set({-7.5f, 6.0f, 9.5f}, {0.5f, 0.0f, 0.0f}, 55.0f);

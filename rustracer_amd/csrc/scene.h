@@ -50,10 +50,26 @@ BVH build_bvh(const std::vector<Vec3> &centroids, const std::vector<AABB> &boxes
 // Depth of the deepest leaf (root = 1), and whether every node is reachable once.
 uint32_t bvh_depth(const BVH &bvh);
 
+// rustracer-core/src/texture.rs:5-9: RGBA linear floats.
+struct Texture {
+    uint32_t width = 0, height = 0;
+    std::vector<float> rgba;  // width * height * 4
+};
+
+// One role's texture array as lib.rs:1286-1358 uploads it: every layer has the
+// size of the first texture, a texture fills its top-left corner, texels are
+// (u8)(clamp(c, 0, 1) * 255); an empty role is one 1x1 zero layer.
+struct TextureArray {
+    uint32_t width = 1, height = 1, layers = 1;
+    std::vector<uint8_t> texels = std::vector<uint8_t>(4, 0);
+};
+TextureArray build_texture_array(const std::vector<Texture> &textures);
+
 struct Scene {
     std::vector<Mesh> meshes;
     std::vector<rt_material> materials;
     std::vector<rt_light_desc> lights;
+    std::vector<Texture> textures[4];  // albedo, metallic-roughness, emissive, normal
 
     size_t triangle_count() const;
     bool bounds(Vec3 &bmin, Vec3 &bmax) const;

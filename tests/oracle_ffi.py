@@ -17,10 +17,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 
 
+class TexArray(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("layers", C.c_uint32),
+                ("_pad", C.c_uint32)]
+
+
 class OrScene(C.Structure):
     _fields_ = [("tris", C.c_void_p), ("n_tris", C.c_uint32), ("nodes", C.c_void_p), ("n_nodes", C.c_uint32),
                 ("prims", C.c_void_p), ("n_prims", C.c_uint32), ("mats", C.c_void_p), ("n_mats", C.c_uint32),
-                ("lights", C.c_void_p), ("n_lights", C.c_uint32)]
+                ("lights", C.c_void_p), ("n_lights", C.c_uint32), ("tex", TexArray * 4)]
 
 
 class OrPhotonMap(C.Structure):
@@ -69,6 +74,18 @@ def lib():
     return _lib
 
 
+def tex_sample(texels: np.ndarray, u: float, v: float, layer: int) -> np.ndarray:
+    """The oracle's textureSampleLevel of a (layers, h, w, 4) uint8 array."""
+    t = np.ascontiguousarray(texels, np.uint8)
+    ta = TexArray(t.ctypes.data, t.shape[2], t.shape[1], t.shape[0], 0)
+    out = np.zeros(4, np.float32)
+    L = lib()
+    L.or_tex_sample.argtypes = [C.POINTER(TexArray), C.c_float, C.c_float, C.c_uint32, C.c_void_p]
+    L.or_tex_sample.restype = None
+    L.or_tex_sample(C.byref(ta), u, v, layer & 0xFFFFFFFF, out.ctypes.data)
+    return out
+
+
 def _p(a):
     return None if a is None else a.ctypes.data
 
@@ -76,7 +93,7 @@ def _p(a):
 class OracleScene:
     """Keeps the arrays alive and exposes the or_scene view of them."""
 
-    def __init__(self, tris, nodes, prims, mats, lights):
+    def __init__(self, tris, nodes, prims, mats, lights, textures=None):
         self.tris = np.ascontiguousarray(tris, TRIANGLE)
         self.nodes = np.ascontiguousarray(nodes, BVH_NODE)
         self.prims = np.ascontiguousarray(prims, np.uint32)
@@ -87,11 +104,22 @@ class OracleScene:
         self.lights = lights
         self.s = OrScene(_p(self.tris), len(self.tris), _p(self.nodes), len(self.nodes), _p(self.prims),
                          len(self.prims), _p(self.mats), len(self.mats), _p(self.lights), len(self.lights))
+        # texture arrays per role: (texels uint8 [layers, h, w, 4]); empty roles are
+        # the reference's 1x1 zero placeholder
+        self.textures = []
+        for k in range(4):
+            t = None if textures is None else textures[k]
+            if t is None:
+                t = np.zeros((1, 1, 1, 4), np.uint8)
+            t = np.ascontiguousarray(t, np.uint8)
+            self.textures.append(t)
+            self.s.tex[k] = TexArray(_p(t), t.shape[2], t.shape[1], t.shape[0], 0)
 
     @classmethod
     def from_scene(cls, scene):
         nodes, prims = scene.build_bvh()
-        return cls(scene.flatten_triangles(), nodes, prims, scene.materials(), scene.lights_gpu())
+        return cls(scene.flatten_triangles(), nodes, prims, scene.materials(), scene.lights_gpu(),
+                   [scene.texture_array(k) for k in range(4)])
 
     def trace(self, rays, stats=False):
         r = np.ascontiguousarray(rays, RAY)

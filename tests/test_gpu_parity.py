@@ -366,3 +366,71 @@ def test_accum64_relmse_matches_numpy():
     want = np.mean((x[..., :3] - ref[..., :3]) ** 2 / (ref[..., :3] ** 2 + eps))
     assert abs(got - want) <= 1e-12 * max(abs(want), 1e-30), (got, want)
     assert got > 0
+
+
+@pytest.mark.parametrize("flags", [0, _ffi.FLAG_CLOSEST_SHADOW, _ffi.FLAG_REFERENCE_TRAVERSAL,
+                                   _ffi.FLAG_BINARY_RECORDS, _ffi.FLAG_PATH_QUEUES])
+def test_textured_gather_bit_exact(flags):
+    """Every texture role (albedo, metallic-roughness, emissive incl. area-light
+    samples, normal map, alpha cutout on camera and shadow rays) against the oracle."""
+    scene = rt.Scene.builtin("textured")
+    cam = rt.Scene.builtin_camera("textured")
+    w, h = 96, 72
+    print(f"textured gather flags={flags}", flush=True)
+    with _renderer(scene, width=w, height=h, spp=4, max_depth=6, enable_denoise=False, flags=flags) as r:
+        r.update_camera(cam)
+        r.render_frames(2)
+        got = (r.read_frame(), r.read_moments(), r.read_gbuffer())
+        cfg = r.config
+    ref = orc.OracleScene.from_scene(scene).gather(rt.camera_to_uniform(cam, w, h), image_params(cfg, 2))
+    for name, a, b in zip(("frame", "moments", "gbuffer"), got, ref[:3]):
+        _check_equal(name, a, b)
+
+
+@pytest.mark.parametrize("view", [3, 5, 8, 9, 11, 12, 18])
+def test_textured_debug_views_bit_exact(view):
+    """spp 16: at spp <= 8 the firefly clamp rescales the colour by
+    clamped/total luminance, and a debug view adds no luminance, so the
+    reference shows it black (gather.wgsl:548-576)."""
+    scene = rt.Scene.builtin("textured")
+    cam = rt.Scene.builtin_camera("textured")
+    with _renderer(scene, width=40, height=30, spp=16, debug_view=view, enable_denoise=False) as r:
+        r.update_camera(cam)
+        r.record_frame()
+        got = r.read_frame()
+        cfg = r.config
+    ref = orc.OracleScene.from_scene(scene).gather(rt.camera_to_uniform(cam, 40, 30), image_params(cfg, 1))
+    _check_equal("frame", got, ref[0])
+    if view not in (5, 12):  # the emissive views: the ceiling light is outside this camera's frame
+        assert np.abs(got[..., :3]).sum() > 0
+
+
+def test_textured_photons_and_raw_upload_bit_exact():
+    """Photon trace with textures (albedo alone, normal map, cutout) and grid,
+    and a renderer built from raw arrays plus uploaded texture arrays."""
+    scene = rt.Scene.builtin("textured")
+    cfg = dict(width=48, height=32, spp=2, enable_denoise=False, enable_photon_map=True, photon_count=4096 + 33)
+    with _renderer(scene, **cfg) as r:
+        r.rebuild_photons()
+        photons = r.read_photons()
+        meta = r.read_grid_meta()
+        gp = r.grid_params()
+        r.update_camera(rt.Scene.builtin_camera("textured"))
+        r.record_frame()
+        frame = r.read_frame()
+    o = orc.OracleScene.from_scene(scene)
+    n = cfg["photon_count"]
+    ref_ph = o.photon_trace(gp, (n // 256) * 256, 6, n_slots=gp.photon_slots)
+    _check_equal("photons", photons.view(np.float32), ref_ph.view(np.float32))
+    n_proc = min(max(n * 6 // 256, 1) * 256, gp.photon_slots)
+    ref_meta, _ = orc.photon_grid(ref_ph, n_proc, gp, len(meta))
+    assert np.array_equal(meta, ref_meta), first_diff(meta, ref_meta)
+    nodes, prims = scene.build_bvh()
+    lo, hi = scene.bounds()
+    arrays = (scene.flatten_triangles(), nodes, prims, scene.materials(), scene.lights_gpu(), (lo, hi),
+              [scene.texture_array(k) for k in range(4)])
+    with rt.Renderer(rt.RenderConfig(**cfg), None, 0, arrays=arrays) as r2:
+        r2.rebuild_photons()
+        r2.update_camera(rt.Scene.builtin_camera("textured"))
+        r2.record_frame()
+        _check_equal("raw-upload frame", r2.read_frame(), frame)
