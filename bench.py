@@ -74,6 +74,9 @@ def main() -> int:
                     help="binary child-box records instead of the 4-wide records (both exact)")
     ap.add_argument("--path-queues", action="store_true",
                     help="camera pass as path-queue passes instead of one kernel per frame (same results)")
+    ap.add_argument("--tile-launch", action="store_true",
+                    help="camera pass as one workgroup per 8x8 tile instead of resident wavefronts whose lanes "
+                         "take new pixels (same results)")
     ap.add_argument("--no-photons", action="store_true")
     ap.add_argument("--no-network", action="store_true", help="skip the learned denoiser's timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batch measurement")
@@ -101,7 +104,8 @@ def main() -> int:
     flags = ((_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0) |
              (_ffi.FLAG_NEAREST_FIRST if args.nearest_first else 0) |
              (_ffi.FLAG_BINARY_RECORDS if args.binary_records else 0) |
-             (_ffi.FLAG_PATH_QUEUES if args.path_queues else 0))
+             (_ffi.FLAG_PATH_QUEUES if args.path_queues else 0) |
+             (_ffi.FLAG_TILE_LAUNCH if args.tile_launch else 0))
     photons_on = not args.no_photons
     cfg = rt.RenderConfig(width=args.width, height=args.height, spp=1, max_depth=args.depth,
                           enable_photon_map=photons_on, photon_count=args.photons, enable_denoise=True,
@@ -183,7 +187,9 @@ def main() -> int:
                    "width": w, "height": h, "spp": 1, "depth": args.depth, "photons": args.photons,
                    "triangles": counts["triangles"],
                    "traversal": TRAVERSALS[tinfo["mode"]], "bvh_depth": tinfo["bvh_depth"],
-                   "camera_pass": "path queues (primary + one pass per depth)" if args.path_queues else "one kernel",
+                   "camera_pass": ("path queues (primary pass + compacted tail pass)" if args.path_queues else
+                                   "one workgroup per tile" if args.tile_launch else
+                                   "persistent lanes taking new pixels"),
                    "photon_map": photons_on},
         "mrays_per_s": round(mrays, 2),
         "pass_ms_per_frame": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},

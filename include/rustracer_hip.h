@@ -84,6 +84,10 @@ typedef struct rt_config {
  * depth over the compacted live paths) instead of the default one kernel per
  * frame (same results). */
 #define RT_FLAG_PATH_QUEUES 32u
+/* Run the camera pass as one workgroup per 8x8 tile, each lane running one
+ * pixel to the end, instead of the default resident wavefronts whose lanes take
+ * a new pixel whenever theirs is done (same results). */
+#define RT_FLAG_TILE_LAUNCH 64u
 
 /* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
 int rt_config_default(rt_config *cfg);
@@ -282,8 +286,9 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out);
  * renders one camera per dispatch: each view is the gather of gather.wgsl with
  * its own camera, resolution, spp and seed frame (every other parameter comes
  * from the renderer's config), and its pixels are bit-identical to a
- * single-view render with those values. Persistent workgroups pull tiles from a
- * global queue (most expensive first). */
+ * single-view render with those values. Persistent workgroups pull work from a
+ * global queue of tiles (most expensive first): whole tiles per wavefront, or
+ * pixel slots per lane, each lane taking the next pixel when its own is done. */
 typedef struct rt_view {
     rt_camera_uniform camera; /* Camera::to_uniform at this view's resolution */
     uint32_t width;
@@ -295,13 +300,16 @@ typedef struct rt_view {
 typedef struct rt_batch_tuning {
     uint32_t tile_w, tile_h;  /* pixels per tile; tile_w * tile_h a multiple of 64 */
     uint32_t waves_per_cu;    /* persistent workgroups (one wavefront each) per CU, 1..20 */
-    uint32_t tiles_per_fetch; /* tiles claimed per queue access */
+    uint32_t tiles_per_fetch; /* tiles claimed per queue access (lanes = 0) */
+    uint32_t lanes;           /* 1: lanes take pixel slots as they finish; 0: wavefronts claim whole tiles */
 } rt_batch_tuning;
 
 int rt_batch_set_views(rt_renderer *r, const rt_view *views, uint32_t n);
-/* One launch over every view; NULL uses the current tuning (default 8x8, 8 waves/CU, 1 tile). */
+/* One launch over every view; NULL uses the current tuning (default 8x8 tiles, 16 waves/CU,
+ * lanes taking pixel slots). */
 int rt_render_batch(rt_renderer *r, const rt_batch_tuning *tuning);
-/* Sweep tile shape, waves per CU and queue granularity on the current views,
+/* Sweep tile shape, waves per CU and queue granularity (pixel slots per lane, or
+ * 1 or 4 tiles per wavefront) on the current views,
  * keep the fastest (mean of `repeats` timed launches after one warm-up). */
 int rt_batch_autotune(rt_renderer *r, uint32_t repeats, rt_batch_tuning *best, double *best_ms);
 int rt_batch_get_tuning(const rt_renderer *r, rt_batch_tuning *out);

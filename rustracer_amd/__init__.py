@@ -408,10 +408,11 @@ class Renderer:
             self._views.append((w, h))
         check(lib().rt_batch_set_views(self._h, arr, len(views)), "rt_batch_set_views")
 
-    def render_batch(self, tile_w=None, tile_h=8, waves_per_cu=8, tiles_per_fetch=1) -> None:
+    def render_batch(self, tile_w=None, tile_h=8, waves_per_cu=16, tiles_per_fetch=1, lanes=1) -> None:
+        """One launch over all views; tile_w None keeps the current tuning."""
         t = None
         if tile_w is not None:
-            t = C.byref(_ffi.BatchTuning(tile_w, tile_h, waves_per_cu, tiles_per_fetch))
+            t = C.byref(_ffi.BatchTuning(tile_w, tile_h, waves_per_cu, tiles_per_fetch, lanes))
         check(lib().rt_render_batch(self._h, t), "rt_render_batch")
 
     def batch_autotune(self, repeats: int = 1):

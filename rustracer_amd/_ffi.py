@@ -45,6 +45,7 @@ RT_OK, RT_ERR_INVALID, RT_ERR_UNSUPPORTED, RT_ERR_HIP, RT_ERR_NO_DEVICE, RT_ERR_
 FLAG_PROFILE, FLAG_CLOSEST_SHADOW, FLAG_REFERENCE_TRAVERSAL, FLAG_NEAREST_FIRST = 1, 2, 4, 8
 FLAG_BINARY_RECORDS = 16
 FLAG_PATH_QUEUES = 32
+FLAG_TILE_LAUNCH = 64
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
               "accum64", "batch", "network")
 
@@ -117,7 +118,7 @@ class View(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("waves_per_cu", C.c_uint32),
-                ("tiles_per_fetch", C.c_uint32)]
+                ("tiles_per_fetch", C.c_uint32), ("lanes", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

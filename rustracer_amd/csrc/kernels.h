@@ -49,6 +49,11 @@ struct GatherLaunch {
     void *px_state;          // 5 * n_px float4
     void *queue[2];          // [1]: 3 * n_px float4, the paths entering depth 1; [0] unused
     uint32_t *path_count;    // max_depth + 1 counters ([0] tail work counter, [1] queued paths)
+    // persistent form: when non-null (and px_state is null) the frame runs as
+    // resident wavefronts whose lanes take pixels from 8 band counters here
+    uint32_t *regen;
+    uint32_t regen_blocks;   // cap on its workgroups (0 = all that are resident at once)
+    uint32_t refill_min;     // a wavefront takes new pixels once this many of its lanes have none
 };
 // Device bytes of the path-queue state for n_px pixels.
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth);
@@ -71,6 +76,7 @@ struct BatchLaunch {
     uint32_t tiles_per_fetch;
     uint32_t n_workgroups;     // persistent workgroups (one wavefront each)
     uint32_t *counter;
+    uint32_t lanes;            // 1: lanes take pixel slots one at a time; 0: wavefronts claim tiles
 };
 // g.frame / g.moments / g.gbuffer point at the batch outputs (sum of view pixels).
 hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipStream_t s);

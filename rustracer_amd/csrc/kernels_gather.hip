@@ -41,6 +41,8 @@ struct GatherArgs {
     uint32_t n_px;
     uint32_t sample;        // sample index of this pass
     rt_tex_array tex[RT_TEX_ROLES];  // albedo, metallic-roughness, emissive, normal
+    uint32_t *regen;        // persistent kernel: kBands pixel counters, zero at launch
+    uint32_t refill_min;    // persistent kernel: lanes without a pixel before a wavefront takes new ones
 };
 
 struct LightSample {
@@ -644,6 +646,107 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     }
 }
 
+// ─────────────────────────── persistent lanes, pixels handed out ───────────────────────────
+//
+// As many one-wavefront workgroups as fit on the GPU at once. Each lane owns
+// one pixel at a time and runs its samples one bounce per loop iteration; a
+// lane whose pixel is done takes the next one from a shared counter, so the
+// few long glass/metal paths of a frame no longer hold wavefronts of finished
+// pixels resident. The pixels (in 8x8 tile order) are cut into kBands
+// contiguous bands, one per XCD (workgroup b runs on XCD b % 8): a wavefront
+// takes pixels from its own XCD's band first, then from the following bands,
+// so neighbouring pixels share an L2. Every pixel runs exactly the steps of
+// gather_pixel (same RNG stream, same accumulation order); only the lane that
+// runs it and the moment it starts change.
+constexpr uint32_t kBands = 8;
+
+template <int MODE, bool EARLY, bool COUNT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gather_persistent_kernel(GatherArgs A) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    uint32_t spill_words[kSpillAlloc];
+    LaneStack stk;
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
+    stk.stride = 64u;
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
+    uint32_t photon_reads = 0u;
+    const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
+    const uint32_t spp = max(1u, V.spp);
+    const Env env = find_env(A);
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t n_slots = A.n_tiles * 64u;
+    const uint32_t band_len = (A.n_tiles + kBands - 1u) / kBands * 64u;
+    uint32_t band = blockIdx.x % kBands, bands_left = kBands;
+    bool live = false;
+    uint32_t px = 0u, py = 0u, s = 0u, b = 0u;
+    Rng rng{0u};
+    Acc acc;
+    FirstHit fh;
+    acc_clear(acc, fh);
+    F3 ro = fs(0.0f), rd = fs(0.0f), thr = fs(0.0f);
+    for (;;) {
+        // lanes without a pixel take the next slots of the current band; a
+        // slot past the image (edge tiles) is skipped by taking another
+        while (bands_left > 0u) {
+            const unsigned long long need = __ballot(!live);
+            if (need == 0ull) break;
+            if ((uint32_t)__popcll(need) < A.refill_min && __ballot(live) != 0ull) break;
+            const uint32_t n_need = (uint32_t)__popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            uint32_t base = 0u;
+            if (lane == (uint32_t)leader) base = atomicAdd(&A.regen[band], n_need);
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, leader, 64));
+            const uint32_t band0 = band * band_len;
+            const uint32_t len = band0 < n_slots ? min(band_len, n_slots - band0) : 0u;
+            if (!live) {
+                const uint32_t k = base + (uint32_t)__popcll(need & below);
+                if (k < len) {
+                    const uint32_t slot = band0 + k, tile = slot >> 6, l = slot & 63u;
+                    px = (tile % A.tiles_x) * 8u + (l & 7u);
+                    py = (tile / A.tiles_x) * 8u + (l >> 3);
+                    if (px < A.ip.width && py < A.ip.height) {
+                        live = true;
+                        rng = pixel_rng(V, px, py);
+                        acc_clear(acc, fh);
+                        s = 0u;
+                        b = 0u;
+                        camera_ray(V, px, py, rng, ro, rd);
+                        thr = fs(1.0f);
+                    }
+                }
+            }
+            if (base + n_need >= len) {
+                band = (band + 1u) % kBands;
+                bands_left--;
+            }
+        }
+        if (__ballot(live) == 0ull) break;
+        if (live) {
+            const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads, px, py);
+            b++;
+            if (!cont || b >= A.max_depth) {
+                acc.end_sample();
+                s++;
+                if (s >= spp) {
+                    finalize(A, acc, spp, fh, (size_t)py * A.ip.width + px);
+                    live = false;
+                } else {
+                    b = 0u;
+                    camera_ray(V, px, py, rng, ro, rd);
+                    thr = fs(1.0f);
+                }
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
+        atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
+        atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
+        atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+    }
+}
+
 // ─────────────────────────── path queues (wavefront) ───────────────────────────
 //
 // The same per-pixel computation split after the first bounce. A primary pass
@@ -887,6 +990,89 @@ __global__ __launch_bounds__(64) void gather_batch_kernel(BatchArgs B) {
     }
 }
 
+// The same queue handed out one pixel slot at a time: a slot is (tile of the
+// cost-sorted list, pixel inside the tile), and each lane takes the next slot
+// when its pixel is done, as gather_persistent_kernel does within one view.
+// Each lane reads its own view's record when a pixel or sample starts.
+template <int MODE, bool EARLY>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gather_batch_lanes_kernel(BatchArgs B) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    uint32_t spill_words[kSpillAlloc];
+    LaneStack stk;
+    stk.spill = (priv_u32 *)spill_words;
+    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
+    stk.stride = 64u;
+    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{0u, 0u}, 0u};
+    uint32_t photon_reads = 0u;
+    const Env env = find_env(B.g);
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t tile_px = B.tile_w * B.tile_h;
+    const uint32_t n_slots = B.n_tiles * tile_px;  // < 2^32, checked by the host
+    bool more = true, live = false;
+    uint32_t vi = 0u, px = 0u, py = 0u, s = 0u, b = 0u;
+    Rng rng{0u};
+    Acc acc;
+    FirstHit fh;
+    acc_clear(acc, fh);
+    F3 ro = fs(0.0f), rd = fs(0.0f), thr = fs(0.0f);
+    for (;;) {
+        while (more) {
+            const unsigned long long need = __ballot(!live);
+            if (need == 0ull) break;
+            const uint32_t n_need = (uint32_t)__popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            uint32_t base = 0u;
+            if (lane == (uint32_t)leader) base = atomicAdd(B.counter, n_need);
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, leader, 64));
+            if (!live) {
+                const uint32_t k = base + (uint32_t)__popcll(need & below);
+                if (k < n_slots) {
+                    const uint32_t t = k / tile_px, j = k % tile_px;
+                    const uint2 te = B.tiles[t];
+                    const BatchView &bv = B.views[te.x];
+                    const uint32_t tx = te.y % bv.tiles_x, ty = te.y / bv.tiles_x;
+                    px = tx * B.tile_w + j % B.tile_w;
+                    py = ty * B.tile_h + j / B.tile_w;
+                    if (px < bv.width && py < bv.height) {
+                        live = true;
+                        vi = te.x;
+                        const ViewParams V{bv.cam, bv.width, bv.height, bv.spp, bv.frame};
+                        rng = pixel_rng(V, px, py);
+                        acc_clear(acc, fh);
+                        s = 0u;
+                        b = 0u;
+                        camera_ray(V, px, py, rng, ro, rd);
+                        thr = fs(1.0f);
+                    }
+                }
+            }
+            if (base + n_need >= n_slots) more = false;
+        }
+        if (__ballot(live) == 0ull) break;
+        if (live) {
+            const bool cont =
+                bounce<MODE, EARLY, false>(B.g, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads, px, py);
+            b++;
+            if (!cont || b >= B.g.max_depth) {
+                acc.end_sample();
+                s++;
+                const BatchView &bv = B.views[vi];
+                const uint32_t spp = max(1u, bv.spp);
+                if (s >= spp) {
+                    finalize(B.g, acc, spp, fh, bv.pixel_offset + (size_t)py * bv.width + px);
+                    live = false;
+                } else {
+                    const ViewParams V{bv.cam, bv.width, bv.height, bv.spp, bv.frame};
+                    b = 0u;
+                    camera_ray(V, px, py, rng, ro, rd);
+                    thr = fs(1.0f);
+                }
+            }
+        }
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(64) void trace_rays_kernel(TraceScene ts, const rt_ray *rays, uint32_t n, rt_hit *out) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
@@ -980,7 +1166,39 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
     A.n_px = g.ip.width * g.ip.height;
     A.sample = 0u;
     for (uint32_t k = 0; k < RT_TEX_ROLES; k++) A.tex[k] = g.tex[k];
+    A.regen = g.regen;
+    A.refill_min = std::max(g.refill_min, 1u);
     return A;
+}
+
+// Workgroups of the persistent gather that are resident at once on the
+// current device (one 64-lane wavefront each).
+template <int M, bool E, bool C>
+static uint32_t persistent_blocks() {
+    static uint32_t blocks = 0;
+    if (blocks == 0u) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtd::gather_persistent_kernel<M, E, C>, 64, 0) !=
+                hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 0u;
+        blocks = (uint32_t)std::max(per_cu, 1) * (uint32_t)std::max(cus, 1);
+    }
+    return blocks;
+}
+
+template <int M, bool E, bool C>
+static hipError_t launch_persistent(const rtd::GatherArgs &A, uint32_t max_blocks, hipStream_t s) {
+    uint32_t nb = persistent_blocks<M, E, C>();
+    if (nb == 0u) return hipErrorInvalidValue;
+    if (max_blocks) nb = std::min(nb, max_blocks);
+    // more workgroups than pixel slots would only start and leave
+    nb = std::min(nb, std::max(A.n_tiles, 1u));
+    const hipError_t e = launch_fill_u32(A.regen, 0u, rtd::kBands, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((rtd::gather_persistent_kernel<M, E, C>), dim3(nb), dim3(64), 0, s, A);
+    return hipSuccess;
 }
 
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth) {
@@ -1006,7 +1224,10 @@ hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
 #define RT_GATHER(M, E, C)                                                                    \
     do {                                                                                      \
         if (g.px_state) launch_wavefront_passes<M, E, C>(A, s);                              \
-        else hipLaunchKernelGGL((rtd::gather_kernel<M, E, C>), grid, block, 0, s, A);        \
+        else if (g.regen) {                                                                   \
+            const hipError_t e = launch_persistent<M, E, C>(A, g.regen_blocks, s);            \
+            if (e != hipSuccess) return e;                                                    \
+        } else hipLaunchKernelGGL((rtd::gather_kernel<M, E, C>), grid, block, 0, s, A);      \
     } while (0)
 #define RT_GATHER_M(M)                        \
     if (g.early_shadow) {                     \
@@ -1040,7 +1261,11 @@ hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipS
     hipError_t e = launch_fill_u32(b.counter, 0u, 1, s);
     if (e != hipSuccess) return e;
     const dim3 grid(b.n_workgroups), block(64);
-#define RT_BATCH(M, E) hipLaunchKernelGGL((rtd::gather_batch_kernel<M, E>), grid, block, 0, s, B)
+#define RT_BATCH(M, E)                                                                          \
+    do {                                                                                        \
+        if (b.lanes) hipLaunchKernelGGL((rtd::gather_batch_lanes_kernel<M, E>), grid, block, 0, s, B); \
+        else hipLaunchKernelGGL((rtd::gather_batch_kernel<M, E>), grid, block, 0, s, B);        \
+    } while (0)
     if (g.mode == 0) { if (g.early_shadow) RT_BATCH(0, true); else RT_BATCH(0, false); }
     else if (g.mode == 1) { if (g.early_shadow) RT_BATCH(1, true); else RT_BATCH(1, false); }
     else if (g.mode == 2) { if (g.early_shadow) RT_BATCH(2, true); else RT_BATCH(2, false); }

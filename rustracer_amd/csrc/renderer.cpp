@@ -95,6 +95,9 @@ struct rt_renderer {
     uint32_t frames_since_reset = 0;
 
     unsigned long long *d_counters = nullptr;
+    // pixel counters of the persistent camera pass: one block of 8 per output
+    // set (gathers of two frames run at once) and one for the counting pass
+    uint32_t *d_regen = nullptr;
     double *d_acc64 = nullptr;
     double *d_relmse = nullptr;  // reduction scratch of rt_accum64_relmse
     void *d_wave = nullptr;  // path-queue state of the camera pass
@@ -122,7 +125,7 @@ struct rt_renderer {
     size_t tiles_cap = 0;
     uint32_t n_tiles = 0, tiles_w = 0, tiles_h = 0;
     uint32_t *d_counter = nullptr;
-    rt_batch_tuning tuning{8u, 8u, 8u, 1u};
+    rt_batch_tuning tuning{8u, 8u, 16u, 1u, 1u};
     int n_cu = 0;
 
     std::vector<PendingTiming> pending;
@@ -151,7 +154,7 @@ struct rt_renderer {
         }
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
-                        d_temp,  d_nvalid, d_counters, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
+                        d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
                         d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo,
                         d_tex[0], d_tex[1], d_tex[2], d_tex[3]};
         for (void *b : bufs)
@@ -226,7 +229,7 @@ struct rt_renderer {
         return ip;
     }
 
-    rtk::GatherLaunch gather_launch(float alpha, float *frame, float *moments, float *gbuf) const {
+    rtk::GatherLaunch gather_launch(float alpha, float *frame, float *moments, float *gbuf, uint32_t set) const {
         rtk::GatherLaunch g;
         g.trace = trace_inputs();
         g.tris = d_tris;
@@ -260,6 +263,12 @@ struct rt_renderer {
             g.queue[1] = w + 5u * plane;
             g.path_count = reinterpret_cast<uint32_t *>(w + 8u * plane);
         }
+        // measured on the 1080p workload: all resident workgroups, refill as
+        // soon as one lane is free (thresholds 8..32 and 3/4 or 1/2 of the
+        // workgroups were no faster)
+        g.regen = (cfg.flags & RT_FLAG_TILE_LAUNCH) ? nullptr : d_regen + 8u * set;
+        g.regen_blocks = 0u;
+        g.refill_min = 1u;
         return g;
     }
 
@@ -301,7 +310,7 @@ struct rt_renderer {
         if (gs != stream && free_recorded[si]) hip_check(hipStreamWaitEvent(gs, ev_free[si], 0), "wait set");
         hipEvent_t ev;
         begin(RT_PASS_GATHER, ev, gs);
-        hip_check(rtk::launch_gather(gather_launch(alpha, d_frame, d_moments, d_gbuf), gs), "gather");
+        hip_check(rtk::launch_gather(gather_launch(alpha, d_frame, d_moments, d_gbuf, si), gs), "gather");
         end(RT_PASS_GATHER, ev, gs);
         if (gs != stream) {
             hip_check(hipEventRecord(ev_gdone[si], gs), "record gather");
@@ -788,6 +797,7 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->d_temp = dev_alloc<char>(r->temp_bytes, "grid temp");
     r->d_nvalid = dev_alloc<uint32_t>(1, "n_valid");
     r->d_counters = dev_alloc<unsigned long long>(4, "counters");
+    r->d_regen = dev_alloc<uint32_t>(8u * (rt_renderer::kSets + 1u), "pixel counters");
 
     // texture arrays: the scene's, else the reference's 1x1 placeholder per role
     for (uint32_t k = 0; k < RT_TEX_ROLES; k++)
@@ -1148,7 +1158,7 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         r->ensure_path_queues();
         const uint32_t saved = r->sample_counter;
         r->sample_counter = saved + 1u;  // the frame the next rt_render_frame would trace
-        rtk::GatherLaunch g = r->gather_launch(1.0f, r->d_ping, r->d_ping, r->d_ping);
+        rtk::GatherLaunch g = r->gather_launch(1.0f, r->d_ping, r->d_ping, r->d_ping, (uint32_t)rt_renderer::kSets);
         r->sample_counter = saved;
         g.counters = r->d_counters;
         hip_check(rtk::launch_gather(g, r->stream), "gather (counting)");
@@ -1215,6 +1225,7 @@ static void batch_validate(const rt_batch_tuning &t) {
     if (t.waves_per_cu == 0 || t.waves_per_cu > 20)
         throw ApiError(RT_ERR_INVALID, "waves_per_cu must be in 1..20 (8 KB of LDS stack per wavefront)");
     if (t.tiles_per_fetch == 0 || t.tiles_per_fetch > 64) throw ApiError(RT_ERR_INVALID, "tiles_per_fetch in 1..64");
+    if (t.lanes > 1u) throw ApiError(RT_ERR_INVALID, "lanes must be 0 or 1");
 }
 
 static void batch_launch(rt_renderer &r, const rt_batch_tuning &t) {
@@ -1223,7 +1234,7 @@ static void batch_launch(rt_renderer &r, const rt_batch_tuning &t) {
         throw ApiError(RT_ERR_INVALID, "photon map enabled but rt_rebuild_photons has not run");
     batch_validate(t);
     if (t.tile_w != r.tiles_w || t.tile_h != r.tiles_h) batch_build_tiles(r, t.tile_w, t.tile_h);
-    rtk::GatherLaunch g = r.gather_launch(1.0f, r.d_bframe, r.d_bmom, r.d_bgb);
+    rtk::GatherLaunch g = r.gather_launch(1.0f, r.d_bframe, r.d_bmom, r.d_bgb, (uint32_t)rt_renderer::kSets);
     g.px_state = nullptr;  // the batch kernel is one persistent pass
     rtk::BatchLaunch b;
     b.views = r.d_views;
@@ -1234,6 +1245,9 @@ static void batch_launch(rt_renderer &r, const rt_batch_tuning &t) {
     b.tiles_per_fetch = t.tiles_per_fetch;
     b.n_workgroups = (uint32_t)r.n_cu * t.waves_per_cu;
     b.counter = r.d_counter;
+    b.lanes = t.lanes;
+    if (t.lanes && (uint64_t)r.n_tiles * t.tile_w * t.tile_h >= (1ull << 32))
+        throw ApiError(RT_ERR_CAPACITY, "batch has 2^32 or more pixel slots");
     hipEvent_t ev;
     r.begin(RT_PASS_BATCH, ev);
     hip_check(rtk::launch_gather_batch(g, b, r.stream), "gather batch");
@@ -1290,7 +1304,8 @@ extern "C" int rt_batch_autotune(rt_renderer *r, uint32_t repeats, rt_batch_tuni
         repeats = std::max(repeats, 1u);
         const uint32_t shapes[][2] = {{8, 8}, {16, 4}, {16, 16}, {32, 8}};
         const uint32_t waves[] = {4, 8, 12, 16};
-        const uint32_t fetch[] = {1, 4};
+        // (lanes, tiles per fetch): pixel slots per lane, or 1 or 4 tiles per wavefront
+        const uint32_t forms[][2] = {{1, 1}, {0, 1}, {0, 4}};
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
@@ -1299,8 +1314,8 @@ extern "C" int rt_batch_autotune(rt_renderer *r, uint32_t repeats, rt_batch_tuni
         try {
             for (auto &sh : shapes) {
                 for (uint32_t w : waves) {
-                    for (uint32_t f : fetch) {
-                        const rt_batch_tuning t{sh[0], sh[1], w, f};
+                    for (auto &f : forms) {
+                        const rt_batch_tuning t{sh[0], sh[1], w, f[1], f[0]};
                         batch_launch(*r, t);  // warm-up (also builds the tile table)
                         hip_check(hipEventRecord(e0, r->stream), "event");
                         for (uint32_t k = 0; k < repeats; k++) batch_launch(*r, t);

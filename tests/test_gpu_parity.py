@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 FLAG_VARIANTS = [0, _ffi.FLAG_REFERENCE_TRAVERSAL, _ffi.FLAG_CLOSEST_SHADOW,
                  _ffi.FLAG_REFERENCE_TRAVERSAL | _ffi.FLAG_CLOSEST_SHADOW, _ffi.FLAG_BINARY_RECORDS,
-                 _ffi.FLAG_PATH_QUEUES]
+                 _ffi.FLAG_PATH_QUEUES, _ffi.FLAG_TILE_LAUNCH]
 # traversal each flag set selects: 3 = 4-wide records, 1 = binary records, 0 = BVH2 nodes
 EXPECTED_MODE = {0: 3, _ffi.FLAG_BINARY_RECORDS: 1, _ffi.FLAG_REFERENCE_TRAVERSAL: 0}
 
@@ -247,7 +247,7 @@ def _view_params(cfg, w, h, spp, frame):
     return ip
 
 
-@pytest.mark.parametrize("tuning", [None, (16, 16, 4, 4), (16, 4, 12, 2)])
+@pytest.mark.parametrize("tuning", [None, (16, 16, 4, 4, 0), (16, 4, 12, 2, 0), (32, 8, 8, 1, 1), (8, 8, 3, 1, 1)])
 def test_batch_views_bit_exact(tuning):
     """Many views of uneven resolution and spp in one persistent launch: every
     view equals the oracle's gather for that camera, resolution, spp and seed."""
@@ -290,15 +290,19 @@ def test_batch_autotune_and_totals():
             assert np.array_equal(r.read_batch_view(k)[0], first[k])
 
 
+@pytest.mark.parametrize("other", [_ffi.FLAG_PATH_QUEUES, _ffi.FLAG_TILE_LAUNCH])
 @pytest.mark.parametrize("scene_name,spp,photons", [("cornell_box", 3, False), ("photon_plain", 2, True)])
-def test_path_queue_and_megakernel_same_work(scene_name, spp, photons):
-    """The path-queue passes and the one-pass kernel run the same per-pixel
-    work: identical outputs and identical traversal/photon counters."""
+def test_camera_pass_forms_same_work(scene_name, spp, photons, other):
+    """The path-queue passes, the one-workgroup-per-tile kernel and the default
+    persistent kernel run the same per-pixel work: identical outputs and
+    identical traversal/photon counters."""
     scene = rt.Scene.builtin(scene_name, 4)
     cam = rt.Scene.builtin_camera(scene_name, 4)
     out = {}
-    for flags in (0, _ffi.FLAG_PATH_QUEUES):
-        cfg = dict(width=96, height=64, spp=spp, max_depth=6, enable_denoise=True, accumulate_frames=4,
+    print(f"camera pass forms {scene_name} spp={spp} flags={other}", flush=True)
+    for flags in (0, other):
+        # ragged size: edge tiles hold slots outside the image
+        cfg = dict(width=97, height=61, spp=spp, max_depth=6, enable_denoise=True, accumulate_frames=4,
                    enable_photon_map=photons, photon_count=20000, flags=flags)
         with _renderer(scene, **cfg) as r:
             r.update_camera(cam)
@@ -308,23 +312,25 @@ def test_path_queue_and_megakernel_same_work(scene_name, spp, photons):
             cnt = r.count_gather_traffic()
             out[flags] = (r.read_frame(), r.read_moments(), r.read_gbuffer(), r.read_accumulation(),
                           r.read_output_rgba8(), cnt)
-    a, b = out[0], out[_ffi.FLAG_PATH_QUEUES]
+    a, b = out[0], out[other]
     for name, x, y in zip(("frame", "moments", "gbuffer", "accumulation", "rgba8"), a[:5], b[:5]):
         _check_equal(name, x, y)
     assert a[5] == b[5], (a[5], b[5])
-    assert a[5]["ray_queries"] > 96 * 64 * spp
+    assert a[5]["ray_queries"] > 97 * 61 * spp
 
 
-def test_back_to_back_frames_match_one_at_a_time():
+@pytest.mark.parametrize("flags", [0, _ffi.FLAG_TILE_LAUNCH])
+def test_back_to_back_frames_match_one_at_a_time(flags):
     """Frames issued back to back (consecutive gathers overlap on two streams,
     each frame with its own output set) give the same accumulation, output and
     last-frame buffers as frames run one at a time with a sync in between."""
     scene = rt.Scene.builtin("multiview", 2)
     cam = rt.Scene.builtin_camera("multiview", 2, 5)
     res = []
+    print(f"back to back flags={flags}", flush=True)
     for sync_each in (False, True):
         cfg = dict(width=640, height=360, spp=1, max_depth=8, enable_denoise=True, accumulate_frames=16,
-                   enable_photon_map=True, photon_count=100000)
+                   enable_photon_map=True, photon_count=100000, flags=flags)
         with _renderer(scene, **cfg) as r:
             r.update_camera(cam)
             r.rebuild_photons()
@@ -369,7 +375,7 @@ def test_accum64_relmse_matches_numpy():
 
 
 @pytest.mark.parametrize("flags", [0, _ffi.FLAG_CLOSEST_SHADOW, _ffi.FLAG_REFERENCE_TRAVERSAL,
-                                   _ffi.FLAG_BINARY_RECORDS, _ffi.FLAG_PATH_QUEUES])
+                                   _ffi.FLAG_BINARY_RECORDS, _ffi.FLAG_PATH_QUEUES, _ffi.FLAG_TILE_LAUNCH])
 def test_textured_gather_bit_exact(flags):
     """Every texture role (albedo, metallic-roughness, emissive incl. area-light
     samples, normal map, alpha cutout on camera and shadow rays) against the oracle."""
