@@ -234,7 +234,8 @@ def main() -> int:
     default_run = (w, h, args.depth, args.photons, photons_on) == (1920, 1080, 8, 1_000_000, True)
     if default_run and os.path.exists(traffic_file):
         t = json.load(open(traffic_file))
-        if t.get("traversal_mode") == tinfo["mode"]:
+        form = "path_queues" if args.path_queues else "tile_launch" if args.tile_launch else "persistent"
+        if t.get("traversal_mode") == tinfo["mode"] and t.get("camera_pass") == form:
             result["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
             result["roofline"]["traffic_source"] = t["source"]
 

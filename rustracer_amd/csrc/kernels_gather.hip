@@ -659,9 +659,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
 // gather_pixel (same RNG stream, same accumulation order); only the lane that
 // runs it and the moment it starts change.
 constexpr uint32_t kBands = 8;
+#ifndef RT_PERSISTENT_WAVES
+#define RT_PERSISTENT_WAVES 4  // wavefronts per SIMD the register budget is cut for
+#endif
 
 template <int MODE, bool EARLY, bool COUNT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gather_persistent_kernel(GatherArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTENT_WAVES))) void gather_persistent_kernel(
+    GatherArgs A) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
