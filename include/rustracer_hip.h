@@ -80,9 +80,10 @@ typedef struct rt_config {
 /* Use the binary child-box records instead of the default 4-wide records
  * (both exact; for measurement). */
 #define RT_FLAG_BINARY_RECORDS 16u
-/* Run the camera pass as path-queue passes (a primary pass, then one pass per
- * depth over the compacted live paths) instead of the default one kernel per
- * frame (same results). */
+/* Run the camera pass as path-queue passes (per sample a primary pass, then one
+ * pass over the compacted live paths, sorted by the kind of their first
+ * scattering: diffuse, or specular/dielectric) instead of the default
+ * persistent kernel (same results). */
 #define RT_FLAG_PATH_QUEUES 32u
 /* Run the camera pass as one workgroup per 8x8 tile, each lane running one
  * pixel to the end, instead of the default resident wavefronts whose lanes take

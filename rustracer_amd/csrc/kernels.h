@@ -43,12 +43,12 @@ struct GatherLaunch {
     uint32_t mode;           // 0 reference nodes, 1 child-box reference order, 2 child-box nearest first, 3 4-wide
     uint32_t early_shadow;   // shadow rays stop at the first hit nearer than the light
     unsigned long long *counters;  // non-null: count traffic (4 counters)
-    // path queues: when px_state is non-null the frame runs as a primary pass
-    // plus one pass per depth (see gather_wavefront_bytes for the sizes)
+    // path queues: when px_state is non-null each sample runs as a primary pass
+    // plus a tail pass over the queued paths (gather_wavefront_bytes: sizes)
     rt_tex_array tex[RT_TEX_ROLES];  // texture arrays (device texels)
     void *px_state;          // 5 * n_px float4
     void *queue[2];          // [1]: 3 * n_px float4, the paths entering depth 1; [0] unused
-    uint32_t *path_count;    // max_depth + 1 counters ([0] tail work counter, [1] queued paths)
+    uint32_t *path_count;    // 3 counters: [0] tail work, [1] diffuse paths queued, [2] smooth paths queued
     // persistent form: when non-null (and px_state is null) the frame runs as
     // resident wavefronts whose lanes take pixels from 8 band counters here
     uint32_t *regen;

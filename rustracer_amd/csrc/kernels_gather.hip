@@ -37,7 +37,7 @@ struct GatherArgs {
     // path-queue (wavefront) state; unused by the one-pass kernels
     float4 *px_state;       // kPixelPlanes planes of n_px float4
     float4 *queue[2];       // [1]: kQueuePlanes planes of n_px float4 (paths entering depth 1); [0] unused
-    uint32_t *path_count;   // paths queued per depth [max_depth + 1]
+    uint32_t *path_count;   // [0] tail work counter, [1] diffuse paths queued, [2] smooth paths queued
     uint32_t n_px;
     uint32_t sample;        // sample index of this pass
     rt_tex_array tex[RT_TEX_ROLES];  // albedo, metallic-roughness, emissive, normal
@@ -277,11 +277,14 @@ struct FirstHit {
 
 // One iteration of the bounce loop of gather.wgsl:298-537 at depth b: the
 // closest hit of (ro, rd) and its shading. Returns true when the path goes on
-// (ro, rd and thr then hold the next ray), false when it ends.
+// (ro, rd and thr then hold the next ray), false when it ends. When `smooth` is
+// given it is set to whether the next ray was scattered by a specular lobe or
+// a dielectric (rather than diffusely, or passed through a cutout texel).
 template <int MODE, bool EARLY, bool COUNT>
 __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, COUNT> &tr, uint32_t b, F3 &ro,
                        F3 &rd, F3 &thr, Rng &rng, Acc &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
-                       uint32_t py) {
+                       uint32_t py, bool *smooth = nullptr) {
+    if (smooth) *smooth = false;
     const rt_image_params &ip = A.ip;
     const uint32_t debug_view = ip._pad[0];
     const Hit hit = tr.closest(ro, rd);
@@ -350,6 +353,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
     }
 
     if (mat.kind == 2u) {
+        if (smooth) *smooth = true;
         const F3 f0 = fs(wgsl_pow((mat.ior - 1.0f) / (mat.ior + 1.0f), 2.0f));
         const float ct = __builtin_fabsf(dot(rd, shn));
         const float fr = fresnel_schlick(ct, f0).x;
@@ -478,6 +482,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
     }
 
     if (do_spec) {
+        if (smooth) *smooth = true;
         rd = sample_spec_dir(rd, shn, a, rng);
         ro = hp + shn * 0.001f;
         const F3 h = normalize(wo + rd);
@@ -764,12 +769,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
 // (registers there, HBM here). Samples run one after another, as the RNG
 // stream requires.
 //
+// The queue is sorted by material between the two passes, with two keys: the
+// paths whose first bounce was diffuse fill it from the front, those that left
+// a specular lobe or a dielectric fill it from the back. The tail pass takes
+// them in that order, so its wavefronts hold paths of one kind: the long
+// glass/metal chains run together instead of one per wavefront of short
+// diffuse paths.
+//
 // Pixel state (SoA planes of n_px float4):
 //   0: col.xyz, lum_sum   1: lum2_sum, cur, rng state, 0
 //   2: sl[0..3]           3: sl[4..7]           4: first normal.xyz, first depth
 // Queue entry (SoA planes): 0: ro.xyz, pixel   1: rd.xyz, rng state   2: thr.xyz, 0
 constexpr uint32_t kPixelPlanes = 5;
 constexpr uint32_t kQueuePlanes = 3;
+constexpr uint32_t kPathCounters = 3;
 
 __device__ __forceinline__ void store_running(const GatherArgs &A, uint32_t i, const Acc &acc, uint32_t rng) {
     A.px_state[i] = make_float4(acc.col.x, acc.col.y, acc.col.z, acc.lum_sum);
@@ -849,7 +862,7 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
     const uint32_t spp = max(1u, V.spp);
     const uint32_t idx = py * A.ip.width + px;
-    bool push = false;
+    bool push = false, smooth = false;
     F3 ro, rd, thr;
     Rng rng{0u};
     if (px < A.ip.width && py < A.ip.height) {
@@ -865,7 +878,8 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
         camera_ray(V, px, py, rng, ro, rd);
         thr = fs(1.0f);
         const Env env = find_env(A);
-        const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, 0u, ro, rd, thr, rng, acc, fh, photon_reads, px, py);
+        const bool cont =
+            bounce<MODE, EARLY, COUNT>(A, env, tr, 0u, ro, rd, thr, rng, acc, fh, photon_reads, px, py, &smooth);
         if (cont && A.max_depth > 1u) {
             push = true;
             store_running(A, idx, acc, 0u);
@@ -874,8 +888,9 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
             end_path(A, idx, acc, fh, rng.state, spp);
         }
     }
-    const uint32_t slot = wave_append(push, &A.path_count[1]);
-    if (push) queue_put(A.queue[1], A.n_px, slot, ro, rd, thr, idx, rng.state);
+    const uint32_t front = wave_append(push && !smooth, &A.path_count[1]);
+    const uint32_t back = wave_append(push && smooth, &A.path_count[2]);
+    if (push) queue_put(A.queue[1], A.n_px, smooth ? A.n_px - 1u - back : front, ro, rd, thr, idx, rng.state);
     if (COUNT) {
         atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
@@ -899,7 +914,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     stk.stride = 64u;
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
-    const uint32_t count = __builtin_amdgcn_readfirstlane(A.path_count[1]);
+    const uint32_t n_front = __builtin_amdgcn_readfirstlane(A.path_count[1]);
+    const uint32_t count = n_front + __builtin_amdgcn_readfirstlane(A.path_count[2]);
     const float4 *qin = A.queue[1];
     const uint32_t spp = max(1u, A.ip.spp);
     const Env env = find_env(A);
@@ -908,8 +924,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
         if (threadIdx.x == 0) base = atomicAdd(&A.path_count[0], 64u);
         base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
         if (base >= count) break;
-        const uint32_t i = base + threadIdx.x;
-        if (i < count) {
+        const uint32_t w = base + threadIdx.x;
+        if (w < count) {
+            const uint32_t i = w < n_front ? w : A.n_px - 1u - (w - n_front);  // diffuse first, then smooth
             const float4 q0 = qin[i], q1 = qin[A.n_px + i], q2 = qin[2u * A.n_px + i];
             F3 ro = xyz(q0);
             const uint32_t idx = __float_as_uint(q0.w);
@@ -1206,7 +1223,8 @@ static hipError_t launch_persistent(const rtd::GatherArgs &A, uint32_t max_block
 }
 
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth) {
-    return (size_t)n_px * 16u * (rtd::kPixelPlanes + rtd::kQueuePlanes) + 4u * ((size_t)max_depth + 1u);
+    (void)max_depth;
+    return (size_t)n_px * 16u * (rtd::kPixelPlanes + rtd::kQueuePlanes) + 4u * rtd::kPathCounters;
 }
 
 template <int M, bool E, bool C>
@@ -1215,7 +1233,7 @@ static void launch_wavefront_passes(rtd::GatherArgs A, hipStream_t s) {
     const uint32_t tail_wgs = std::min<uint32_t>((A.n_px + 63u) / 64u, 2048u);
     for (uint32_t k = 0; k < spp; k++) {
         A.sample = k;
-        launch_fill_u32(A.path_count, 0u, (size_t)A.max_depth + 1u, s);
+        (void)launch_fill_u32(A.path_count, 0u, rtd::kPathCounters, s);
         hipLaunchKernelGGL((rtd::gather_primary_kernel<M, E, C>), dim3(A.n_tiles), dim3(64), 0, s, A);
         if (A.max_depth > 1u)
             hipLaunchKernelGGL((rtd::gather_tail_kernel<M, E, C>), dim3(tail_wgs), dim3(64), 0, s, A);
