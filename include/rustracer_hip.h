@@ -89,6 +89,12 @@ typedef struct rt_config {
  * pixel to the end, instead of the default resident wavefronts whose lanes take
  * a new pixel whenever theirs is done (same results). */
 #define RT_FLAG_TILE_LAUNCH 64u
+/* Replay each frame from hipGraphs captured once per output set: the gather
+ * pass, and the a-trous + composite + tonemap passes. The camera, seed frame
+ * and accumulation weight are written to device memory ahead of each replay;
+ * a graph is captured again when anything else it was made from changes
+ * (config, photon map, textures, buffers). Same results. */
+#define RT_FLAG_GRAPH 128u
 
 /* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
 int rt_config_default(rt_config *cfg);
@@ -251,6 +257,10 @@ int rt_trace_closest_device(rt_renderer *r, const rt_ray *d_rays, uint32_t n, rt
  * and the number of 4-wide records (0 when that layout is unavailable). No
  * reference counterpart: it reports this backend's derived layouts. */
 int rt_traversal_info(const rt_renderer *r, uint32_t *mode, uint32_t *bvh_depth, uint32_t *n_records4);
+/* Number of frame graphs captured so far (RT_FLAG_GRAPH): two per output set
+ * at first use, then one more each time a captured pass's inputs change. No
+ * counterpart in the reference. */
+int rt_frame_graph_captures(const rt_renderer *r, uint32_t *captures);
 /* Build the photon grid from a given photon array (count, scan, stable
  * scatter: photon_count.wgsl, lib.rs:987-1007, photon_scatter.wgsl) into the
  * renderer's grid, replacing traced photons. n_slots <= photon_count*max_depth. */

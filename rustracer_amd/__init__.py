@@ -388,6 +388,11 @@ class Renderer:
         check(lib().rt_get_pass_stats(self._h, C.byref(s)), "rt_get_pass_stats")
         return {name: (s.ms[i], s.launches[i]) for i, name in enumerate(_ffi.PASS_NAMES)}
 
+    def frame_graph_captures(self) -> int:
+        n = C.c_uint32()
+        check(lib().rt_frame_graph_captures(self._h, C.byref(n)), "rt_frame_graph_captures")
+        return n.value
+
     def reset_pass_stats(self) -> None:
         check(lib().rt_reset_pass_stats(self._h), "rt_reset_pass_stats")
 

@@ -46,6 +46,7 @@ FLAG_PROFILE, FLAG_CLOSEST_SHADOW, FLAG_REFERENCE_TRAVERSAL, FLAG_NEAREST_FIRST 
 FLAG_BINARY_RECORDS = 16
 FLAG_PATH_QUEUES = 32
 FLAG_TILE_LAUNCH = 64
+FLAG_GRAPH = 128
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
               "accum64", "batch", "network")
 
@@ -196,6 +197,7 @@ SIGNATURES = {
     "rt_trace_closest": (_i, [_vp, _vp, _u32, _vp]),
     "rt_trace_closest_device": (_i, [_vp, _vp, _u32, _vp]),
     "rt_traversal_info": (_i, [_vp, _vp, _vp, _vp]),
+    "rt_frame_graph_captures": (_i, [_vp, _vp]),
     "rt_renderer_set_texture_array": (_i, [_vp, _u32, _u32, _u32, _u32, _vp]),
     "rt_scene_add_texture": (_i, [_vp, _u32, _u32, _u32, _vp, _P(_u32)]),
     "rt_scene_texture_array": (_i, [_vp, _u32, _vp, C.c_size_t, _P(_u32), _P(_u32), _P(_u32)]),

@@ -11,6 +11,19 @@ namespace rtk {
 // Per-lane traversal stack capacity in 32-bit words (LDS rows + private spill).
 constexpr uint32_t kTraversalStackWords = 96;
 
+// The values of one frame that change from frame to frame, kept in device
+// memory for replayed frame graphs: kernels given a FrameState read the camera,
+// the seed frame and the accumulation weight from it instead of their
+// arguments, so one captured graph serves every frame.
+struct FrameState {
+    rt_camera_uniform cam;
+    uint32_t frame;  // ImageParams.frame (the seed, gather.wgsl:262)
+    float alpha;     // accumulation weight of the frame (lib.rs:679-684)
+    uint32_t pad[2];
+};
+static_assert(sizeof(FrameState) == 112, "FrameState layout");
+hipError_t launch_set_frame_state(FrameState *dst, const FrameState &v, hipStream_t s);
+
 // Traversal inputs shared by every tracing kernel (device pointers).
 struct TraceInputs {
     const rt_bvh_node *nodes;
@@ -54,6 +67,7 @@ struct GatherLaunch {
     uint32_t *regen;
     uint32_t regen_blocks;   // cap on its workgroups (0 = all that are resident at once)
     uint32_t refill_min;     // a wavefront takes new pixels once this many of its lanes have none
+    const FrameState *fstate;  // non-null: camera and seed frame come from here
 };
 // Device bytes of the path-queue state for n_px pixels.
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth);
@@ -122,11 +136,14 @@ struct DenoiseLaunch {
     const void *gbuffer;
     const void *moments;
     rt_denoise_params p;
+    const FrameState *fstate;  // non-null: alpha comes from here
 };
 hipError_t launch_denoise(const DenoiseLaunch &d, hipStream_t s);
 
+// fstate non-null: alpha comes from fstate->alpha
 hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rgba8, uint32_t w, uint32_t h,
-                                    float alpha, const rt_tonemap_uniform &tm, const void *gbuffer, hipStream_t s);
+                                    float alpha, const rt_tonemap_uniform &tm, const void *gbuffer,
+                                    const FrameState *fstate, hipStream_t s);
 hipError_t launch_tonemap(const void *img, uint32_t *rgba8, uint32_t w, uint32_t h, const rt_tonemap_uniform &tm,
                           const void *gbuffer, hipStream_t s);
 // First-hit albedo of the pixel-centre camera ray (rgb; w = 1 on a hit, 0 on a miss).

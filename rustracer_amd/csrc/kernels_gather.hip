@@ -43,6 +43,7 @@ struct GatherArgs {
     rt_tex_array tex[RT_TEX_ROLES];  // albedo, metallic-roughness, emissive, normal
     uint32_t *regen;        // persistent kernel: kBands pixel counters, zero at launch
     uint32_t refill_min;    // persistent kernel: lanes without a pixel before a wavefront takes new ones
+    const rtk::FrameState *fstate;  // non-null: camera and seed frame of a replayed frame graph
 };
 
 struct LightSample {
@@ -249,6 +250,13 @@ struct ViewParams {
     rt_camera_uniform cam;
     uint32_t width, height, spp, frame;
 };
+
+// The frame's camera and seed: from the arguments, or from the device-resident
+// frame state of a replayed frame graph.
+__device__ __forceinline__ ViewParams frame_view(const GatherArgs &A) {
+    if (A.fstate) return ViewParams{A.fstate->cam, A.ip.width, A.ip.height, A.ip.spp, A.fstate->frame};
+    return ViewParams{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
+}
 
 // The environment light (gather.wgsl:274-288): the first kind-3 light, else
 // the shader's default sky.
@@ -640,7 +648,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     stk.stride = 64u;
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
-    const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
+    const ViewParams V = frame_view(A);
     if (px < A.ip.width && py < A.ip.height)
         gather_pixel<MODE, EARLY, COUNT>(A, V, tr, px, py, (size_t)py * A.ip.width + px, photon_reads);
     if (COUNT) {
@@ -679,7 +687,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
     stk.stride = 64u;
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
-    const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
+    const ViewParams V = frame_view(A);
     const uint32_t spp = max(1u, V.spp);
     const Env env = find_env(A);
     const uint32_t lane = __lane_id();
@@ -859,7 +867,7 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     stk.stride = 64u;
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
     uint32_t photon_reads = 0u;
-    const ViewParams V{A.cam, A.ip.width, A.ip.height, A.ip.spp, A.ip.frame};
+    const ViewParams V = frame_view(A);
     const uint32_t spp = max(1u, V.spp);
     const uint32_t idx = py * A.ip.width + px;
     bool push = false, smooth = false;
@@ -1189,6 +1197,7 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
     for (uint32_t k = 0; k < RT_TEX_ROLES; k++) A.tex[k] = g.tex[k];
     A.regen = g.regen;
     A.refill_min = std::max(g.refill_min, 1u);
+    A.fstate = g.fstate;
     return A;
 }
 

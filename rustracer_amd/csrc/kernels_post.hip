@@ -20,7 +20,8 @@ __device__ __forceinline__ float luma_of(F3 c) { return dot(c, f3(0.2126f, 0.715
 template <int STEP>
 __global__ __launch_bounds__(256) void denoise_kernel(const float4 *__restrict__ in, float4 *__restrict__ out,
                                                       const float4 *__restrict__ gbuf, const float4 *__restrict__ mom,
-                                                      rt_denoise_params p) {
+                                                      rt_denoise_params p, const rtk::FrameState *fstate) {
+    if (fstate) p.alpha = fstate->alpha;
     constexpr int H = 2 * STEP;             // halo radius of the 5x5 taps
     constexpr int T = kDnTile + 2 * H;      // staged tile side
     __shared__ float4 s_col[T * T];         // rgb + luminance
@@ -103,7 +104,9 @@ __global__ __launch_bounds__(256) void denoise_kernel(const float4 *__restrict__
 // Generic-step fallback (steps > 4): the same arithmetic reading global memory.
 __global__ __launch_bounds__(256) void denoise_global_kernel(const float4 *__restrict__ in, float4 *__restrict__ out,
                                                              const float4 *__restrict__ gbuf,
-                                                             const float4 *__restrict__ mom, rt_denoise_params p) {
+                                                             const float4 *__restrict__ mom, rt_denoise_params p,
+                                                             const rtk::FrameState *fstate) {
+    if (fstate) p.alpha = fstate->alpha;
     const int W = (int)p.width, Hh = (int)p.height;
     const int px = (int)(blockIdx.x * kDnTile + threadIdx.x % kDnTile);
     const int py = (int)(blockIdx.y * kDnTile + threadIdx.x / kDnTile);
@@ -187,9 +190,11 @@ __device__ __forceinline__ uint32_t tonemap_pixel(float4 a, uint32_t i, uint32_t
 __global__ __launch_bounds__(256) void composite_tonemap_kernel(const float4 *__restrict__ frame,
                                                                 float4 *__restrict__ accum, uint32_t *__restrict__ rgba8,
                                                                 uint32_t n_px, uint32_t width, float alpha,
-                                                                rt_tonemap_uniform tm, const float4 *__restrict__ gbuf) {
+                                                                rt_tonemap_uniform tm, const float4 *__restrict__ gbuf,
+                                                                const rtk::FrameState *fstate) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n_px) return;
+    if (fstate) alpha = fstate->alpha;
     const float4 fresh = frame[i];
     const float4 prev = accum[i];
     const float om = 1.0f - alpha;
@@ -197,6 +202,11 @@ __global__ __launch_bounds__(256) void composite_tonemap_kernel(const float4 *__
                                  prev.z * om + fresh.z * alpha, prev.w * om + fresh.w * alpha);
     accum[i] = a;
     rgba8[i] = tonemap_pixel(a, i, width, tm, gbuf);
+}
+
+// One lane stores a frame graph's per-frame values (vector store).
+__global__ void set_frame_state_kernel(rtk::FrameState *dst, rtk::FrameState v) {
+    if (threadIdx.x == 0) *dst = v;
 }
 
 // The tonemap alone, of another displayed image (the learned denoiser's output).
@@ -310,21 +320,27 @@ hipError_t launch_denoise(const DenoiseLaunch &d, hipStream_t s) {
     const float4 *gb = reinterpret_cast<const float4 *>(d.gbuffer);
     const float4 *mo = reinterpret_cast<const float4 *>(d.moments);
     switch (d.p.step_size) {
-        case 1: hipLaunchKernelGGL((rtd::denoise_kernel<1>), grid, dim3(256), 0, s, in, out, gb, mo, d.p); break;
-        case 2: hipLaunchKernelGGL((rtd::denoise_kernel<2>), grid, dim3(256), 0, s, in, out, gb, mo, d.p); break;
-        case 4: hipLaunchKernelGGL((rtd::denoise_kernel<4>), grid, dim3(256), 0, s, in, out, gb, mo, d.p); break;
-        default: hipLaunchKernelGGL(rtd::denoise_global_kernel, grid, dim3(256), 0, s, in, out, gb, mo, d.p); break;
+        case 1: hipLaunchKernelGGL((rtd::denoise_kernel<1>), grid, dim3(256), 0, s, in, out, gb, mo, d.p, d.fstate); break;
+        case 2: hipLaunchKernelGGL((rtd::denoise_kernel<2>), grid, dim3(256), 0, s, in, out, gb, mo, d.p, d.fstate); break;
+        case 4: hipLaunchKernelGGL((rtd::denoise_kernel<4>), grid, dim3(256), 0, s, in, out, gb, mo, d.p, d.fstate); break;
+        default: hipLaunchKernelGGL(rtd::denoise_global_kernel, grid, dim3(256), 0, s, in, out, gb, mo, d.p, d.fstate); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rgba8, uint32_t w, uint32_t h,
-                                    float alpha, const rt_tonemap_uniform &tm, const void *gbuffer, hipStream_t s) {
+                                    float alpha, const rt_tonemap_uniform &tm, const void *gbuffer,
+                                    const FrameState *fstate, hipStream_t s) {
     const uint32_t n = w * h;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(rtd::composite_tonemap_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                        reinterpret_cast<const float4 *>(frame), reinterpret_cast<float4 *>(accum), rgba8, n, w, alpha,
-                       tm, reinterpret_cast<const float4 *>(gbuffer));
+                       tm, reinterpret_cast<const float4 *>(gbuffer), fstate);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_frame_state(FrameState *dst, const FrameState &v, hipStream_t s) {
+    hipLaunchKernelGGL(rtd::set_frame_state_kernel, dim3(1), dim3(64), 0, s, dst, v);
     return hipGetLastError();
 }
 

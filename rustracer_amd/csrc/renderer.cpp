@@ -42,6 +42,14 @@ struct PendingTiming {
 
 }  // namespace
 
+// What a captured post-pass graph (a-trous, composite, tonemap) was made from:
+// a replay is valid while every field is unchanged.
+struct PostKey {
+    const void *frame, *ping, *moments, *gbuf, *accum, *rgba8;
+    uint32_t width, height, denoise, passes;
+    rt_tonemap_uniform tm;
+};
+
 struct rt_renderer {
     rt_config cfg{};
     int device = 0;
@@ -131,10 +139,28 @@ struct rt_renderer {
     std::vector<PendingTiming> pending;
     rt_pass_stats stats{};
 
+    // Frame graphs (RT_FLAG_GRAPH): per output set, the gather pass and the post
+    // passes captured once into hipGraphs and replayed every frame. The values
+    // that change per frame (camera, seed frame, accumulation weight) are stored
+    // into d_fstate[set] by a one-lane kernel ahead of each replay; a graph is
+    // captured again only when what it was made from changes (gkey / pkey).
+    rtk::FrameState *d_fstate = nullptr;
+    hipStream_t cstream = nullptr;  // capture stream (nothing runs on it)
+    hipGraphExec_t gexec[kSets] = {}, pexec[kSets] = {};
+    rtk::GatherLaunch gkey[kSets] = {};
+    PostKey pkey[kSets] = {};
+    uint32_t graph_captures = 0;
+
     size_t n_px() const { return (size_t)cfg.width * cfg.height; }
 
     ~rt_renderer() {
         if (stream) hipStreamSynchronize(stream);
+        for (int k = 0; k < kSets; k++) {
+            if (gstream[k]) hipStreamSynchronize(gstream[k]);
+            if (gexec[k]) hipGraphExecDestroy(gexec[k]);
+            if (pexec[k]) hipGraphExecDestroy(pexec[k]);
+        }
+        if (cstream) hipStreamDestroy(cstream);
         for (int k = 0; k < kSets; k++) {
             if (gstream[k]) hipStreamSynchronize(gstream[k]);
             if (ev_gdone[k]) hipEventDestroy(ev_gdone[k]);
@@ -155,7 +181,7 @@ struct rt_renderer {
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
-                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo,
+                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo, d_fstate,
                         d_tex[0], d_tex[1], d_tex[2], d_tex[3]};
         for (void *b : bufs)
             if (b) hipFree(b);
@@ -231,6 +257,7 @@ struct rt_renderer {
 
     rtk::GatherLaunch gather_launch(float alpha, float *frame, float *moments, float *gbuf, uint32_t set) const {
         rtk::GatherLaunch g;
+        std::memset(&g, 0, sizeof(g));  // padding too: captured launches are compared bytewise
         g.trace = trace_inputs();
         g.tris = d_tris;
         g.mats = d_mats;
@@ -308,14 +335,65 @@ struct rt_renderer {
         // the path-queue state is shared by all frames: those gathers stay on the main stream
         hipStream_t gs = use_path_queues() ? stream : gstream[si];
         if (gs != stream && free_recorded[si]) hip_check(hipStreamWaitEvent(gs, ev_free[si], 0), "wait set");
+        const bool graphs = (cfg.flags & RT_FLAG_GRAPH) != 0;
+        rtk::GatherLaunch gl = gather_launch(alpha, d_frame, d_moments, d_gbuf, si);
         hipEvent_t ev;
-        begin(RT_PASS_GATHER, ev, gs);
-        hip_check(rtk::launch_gather(gather_launch(alpha, d_frame, d_moments, d_gbuf, si), gs), "gather");
-        end(RT_PASS_GATHER, ev, gs);
+        if (graphs) {
+            ensure_fstate();
+            gl.fstate = d_fstate + si;
+            ensure_gather_graph(si, gl);
+            rtk::FrameState fsv;
+            std::memset(&fsv, 0, sizeof(fsv));
+            fsv.cam = cam;
+            fsv.frame = sample_counter;
+            fsv.alpha = alpha;
+            hip_check(rtk::launch_set_frame_state(d_fstate + si, fsv, gs), "frame state");
+            begin(RT_PASS_GATHER, ev, gs);
+            hip_check(hipGraphLaunch(gexec[si], gs), "gather graph");
+            end(RT_PASS_GATHER, ev, gs);
+        } else {
+            begin(RT_PASS_GATHER, ev, gs);
+            hip_check(rtk::launch_gather(gl, gs), "gather");
+            end(RT_PASS_GATHER, ev, gs);
+        }
         if (gs != stream) {
             hip_check(hipEventRecord(ev_gdone[si], gs), "record gather");
             hip_check(hipStreamWaitEvent(stream, ev_gdone[si], 0), "wait gather");
         }
+        if (graphs && !unet) {
+            ensure_post_graph(si);
+            begin(RT_PASS_COMPOSITE_TONEMAP, ev);  // a-trous passes included
+            hip_check(hipGraphLaunch(pexec[si], stream), "post graph");
+            end(RT_PASS_COMPOSITE_TONEMAP, ev);
+        } else {
+            post_passes(alpha, nullptr, stream, true);
+        }
+        if (unet) {
+            begin(RT_PASS_NETWORK, ev);
+            ensure_albedo();
+            rtu::denoise(unet, frame_views(), d_display, stream);
+            hip_check(rtk::launch_tonemap(d_display, d_rgba8, cfg.width, cfg.height, tonemap_uniform(), d_gbuf, stream),
+                      "tonemap");
+            end(RT_PASS_NETWORK, ev);
+        }
+        hip_check(hipEventRecord(ev_free[si], stream), "record set release");
+        free_recorded[si] = true;
+    }
+
+    rt_tonemap_uniform tonemap_uniform() const {
+        rt_tonemap_uniform tm;
+        std::memset(&tm, 0, sizeof(tm));
+        tm.exposure = cfg.exposure;
+        tm.gamma = cfg.gamma;
+        tm.width = cfg.width;
+        tm.glow = 1.0f;
+        return tm;
+    }
+
+    // a-trous passes, then composite + tonemap, of the current output set on
+    // stream s; with fs, alpha comes from the frame state.
+    void post_passes(float alpha, const rtk::FrameState *fs, hipStream_t s, bool timed) {
+        hipEvent_t ev = nullptr;
         float *result = d_frame;
         if (cfg.enable_denoise) {
             float *src = d_frame, *dst = d_ping;
@@ -333,33 +411,90 @@ struct rt_renderer {
                 d.p.phi_depth = 0.05f;
                 d.p.alpha = alpha;
                 d.p._pad = 0;
-                begin(RT_PASS_DENOISE, ev);
-                hip_check(rtk::launch_denoise(d, stream), "denoise");
-                end(RT_PASS_DENOISE, ev);
+                d.fstate = fs;
+                if (timed) begin(RT_PASS_DENOISE, ev, s);
+                hip_check(rtk::launch_denoise(d, s), "denoise");
+                if (timed) end(RT_PASS_DENOISE, ev, s);
                 std::swap(src, dst);
             }
             result = src;
         }
-        rt_tonemap_uniform tm;
-        std::memset(&tm, 0, sizeof(tm));
-        tm.exposure = cfg.exposure;
-        tm.gamma = cfg.gamma;
-        tm.width = cfg.width;
-        tm.glow = 1.0f;
-        begin(RT_PASS_COMPOSITE_TONEMAP, ev);
-        hip_check(rtk::launch_composite_tonemap(result, d_accum, d_rgba8, cfg.width, cfg.height, alpha, tm, d_gbuf,
-                                                stream),
+        if (timed) begin(RT_PASS_COMPOSITE_TONEMAP, ev, s);
+        hip_check(rtk::launch_composite_tonemap(result, d_accum, d_rgba8, cfg.width, cfg.height, alpha,
+                                                tonemap_uniform(), d_gbuf, fs, s),
                   "composite_tonemap");
-        end(RT_PASS_COMPOSITE_TONEMAP, ev);
-        if (unet) {
-            begin(RT_PASS_NETWORK, ev);
-            ensure_albedo();
-            rtu::denoise(unet, frame_views(), d_display, stream);
-            hip_check(rtk::launch_tonemap(d_display, d_rgba8, cfg.width, cfg.height, tm, d_gbuf, stream), "tonemap");
-            end(RT_PASS_NETWORK, ev);
+        if (timed) end(RT_PASS_COMPOSITE_TONEMAP, ev, s);
+    }
+
+    void sync_streams() {
+        hip_check(hipStreamSynchronize(stream), "sync");
+        for (int k = 0; k < kSets; k++) hip_check(hipStreamSynchronize(gstream[k]), "sync");
+    }
+
+    // Records f(s) on the capture stream into an instantiated graph.
+    template <class F>
+    hipGraphExec_t capture(F &&f) {
+        if (!cstream) hip_check(hipStreamCreateWithFlags(&cstream, hipStreamNonBlocking), "capture stream");
+        hip_check(hipStreamBeginCapture(cstream, hipStreamCaptureModeRelaxed), "begin capture");
+        hipGraph_t g = nullptr;
+        try {
+            f(cstream);
+        } catch (...) {
+            (void)hipStreamEndCapture(cstream, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
         }
-        hip_check(hipEventRecord(ev_free[si], stream), "record set release");
-        free_recorded[si] = true;
+        hip_check(hipStreamEndCapture(cstream, &g), "end capture");
+        hipGraphExec_t e = nullptr;
+        const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        hip_check(err, "instantiate graph");
+        graph_captures++;
+        return e;
+    }
+
+    void ensure_fstate() {
+        if (!d_fstate) {
+            hip_check(hipMalloc(&d_fstate, sizeof(rtk::FrameState) * kSets), "hipMalloc frame state");
+            hip_check(hipMemset(d_fstate, 0, sizeof(rtk::FrameState) * kSets), "memset");
+        }
+    }
+
+    // The gather graph of set si for launch gl (per-frame fields are ignored:
+    // they come from the frame state).
+    void ensure_gather_graph(uint32_t si, const rtk::GatherLaunch &gl) {
+        rtk::GatherLaunch key = gl;
+        std::memset(&key.cam, 0, sizeof(key.cam));
+        key.ip.frame = 0;
+        key.ip.accumulate_alpha = 0.0f;
+        if (gexec[si] && std::memcmp(&key, &gkey[si], sizeof(key)) == 0) return;
+        sync_streams();
+        if (gexec[si]) hip_check(hipGraphExecDestroy(gexec[si]), "destroy graph");
+        gexec[si] = nullptr;
+        gexec[si] = capture([&](hipStream_t s) { hip_check(rtk::launch_gather(gl, s), "gather"); });
+        gkey[si] = key;
+    }
+
+    void ensure_post_graph(uint32_t si) {
+        PostKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.frame = d_frame;
+        key.ping = d_ping;
+        key.moments = d_moments;
+        key.gbuf = d_gbuf;
+        key.accum = d_accum;
+        key.rgba8 = d_rgba8;
+        key.width = cfg.width;
+        key.height = cfg.height;
+        key.denoise = cfg.enable_denoise;
+        key.passes = cfg.denoise_passes;
+        key.tm = tonemap_uniform();
+        if (pexec[si] && std::memcmp(&key, &pkey[si], sizeof(key)) == 0) return;
+        sync_streams();
+        if (pexec[si]) hip_check(hipGraphExecDestroy(pexec[si]), "destroy graph");
+        pexec[si] = nullptr;
+        pexec[si] = capture([&](hipStream_t s) { post_passes(0.0f, d_fstate + si, s, false); });
+        pkey[si] = key;
     }
 
     // First-hit albedo of the pixel-centre rays, recomputed after a camera change.
@@ -1069,6 +1204,13 @@ int rt_read_grid_meta(rt_renderer *r, uint32_t *meta, size_t n, uint32_t *n_cell
         if (!meta) return;
         if (n < 2ull * r->meta_cells) throw ApiError(RT_ERR_INVALID, "capacity below 2*cells");
         read_dev(r, meta, r->d_meta, 8ull * r->meta_cells);
+    });
+}
+
+int rt_frame_graph_captures(const rt_renderer *r, uint32_t *captures) {
+    return api_call([&] {
+        if (!r || !captures) throw ApiError(RT_ERR_INVALID, "null argument");
+        *captures = r->graph_captures;
     });
 }
 
