@@ -167,6 +167,16 @@ int rt_scene_texture_array(const rt_scene *scene, uint32_t role, uint8_t *texels
                            uint32_t *height, uint32_t *layers);
 /* light_to_gpu over the scene's lights (lib.rs:1231-1282). */
 int rt_scene_lights_gpu(const rt_scene *scene, rt_light *out, uint32_t capacity);
+/* The scene's light descriptions; *n = their count (out = NULL: count only). */
+int rt_scene_light_descs(const rt_scene *scene, rt_light_desc *out, uint32_t capacity, uint32_t *n);
+/* Parse a scene.toml descriptor and apply it to the scene, as
+ * parse_descriptor + apply_descriptor do (rustracer-core/src/config.rs:62-174):
+ * [environment] replaces every environment light, [[lights]] of type point,
+ * directional or area are appended (other types skipped), [[materials]]
+ * overrides the material at `index`, or every material without one (indices
+ * out of range skipped). A malformed file or a field of the wrong type returns
+ * RT_ERR_INVALID and leaves the scene unchanged. */
+int rt_scene_apply_descriptor(rt_scene *scene, const char *toml_text);
 int rt_light_to_gpu(const rt_light_desc *light, rt_light *out);
 /* Scene::build_bvh (scene.rs:276-286) -> BVHBuilder::build (bvh.rs:135-151):
  * binned SAH (12 bins, <= 4 prims per leaf), root at node 0. Call with NULL

@@ -134,6 +134,23 @@ class Scene:
         check(lib().rt_scene_flatten_materials(self._h, ptr(out), len(out)), "rt_scene_flatten_materials")
         return out
 
+    def light_descs(self) -> list:
+        """The scene's lights as dicts (kind, vec_a, vec_b, color, intensity, triangle_index)."""
+        n = C.c_uint32()
+        check(lib().rt_scene_light_descs(self._h, None, 0, C.byref(n)), "rt_scene_light_descs")
+        arr = (LightDesc * max(n.value, 1))()
+        check(lib().rt_scene_light_descs(self._h, arr, n.value, C.byref(n)), "rt_scene_light_descs")
+        return [dict(kind=d.kind, vec_a=tuple(d.vec_a), vec_b=tuple(d.vec_b), color=tuple(d.color),
+                     intensity=d.intensity, triangle_index=d.triangle_index) for d in arr[:n.value]]
+
+    def apply_descriptor(self, toml_text: str) -> None:
+        """Apply a scene.toml descriptor (rustracer-core/src/config.rs:62-174)."""
+        check(lib().rt_scene_apply_descriptor(self._h, toml_text.encode()), "rt_scene_apply_descriptor")
+
+    def apply_descriptor_file(self, path: str) -> None:
+        with open(path, encoding="utf-8") as f:
+            self.apply_descriptor(f.read())
+
     def lights_gpu(self) -> np.ndarray:
         out = np.zeros(self.counts()["lights"], LIGHT)
         check(lib().rt_scene_lights_gpu(self._h, ptr(out), len(out)), "rt_scene_lights_gpu")

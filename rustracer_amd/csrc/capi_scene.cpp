@@ -173,6 +173,25 @@ int rt_scene_texture_array(const rt_scene *s, uint32_t role, uint8_t *texels, si
     });
 }
 
+int rt_scene_apply_descriptor(rt_scene *s, const char *toml_text) {
+    return api_call([&] {
+        if (!s || !toml_text) throw ApiError(RT_ERR_INVALID, "null argument");
+        rt::apply_descriptor(s->scene, toml_text);
+    });
+}
+
+int rt_scene_light_descs(const rt_scene *s, rt_light_desc *out, uint32_t cap, uint32_t *n) {
+    return api_call([&] {
+        if (!s) throw ApiError(RT_ERR_INVALID, "null scene");
+        const auto &l = s->scene.lights;
+        if (n) *n = (uint32_t)l.size();
+        if (out) {
+            if (cap < l.size()) throw ApiError(RT_ERR_INVALID, "capacity below light count");
+            if (!l.empty()) std::memcpy(out, l.data(), l.size() * sizeof(rt_light_desc));
+        }
+    });
+}
+
 int rt_scene_lights_gpu(const rt_scene *s, rt_light *out, uint32_t cap) {
     return api_call([&] {
         if (!s || !out) throw ApiError(RT_ERR_INVALID, "null argument");

@@ -90,6 +90,11 @@ rt_material material_emissive(Vec3 color, float intensity);
 void add_quad(Scene &s, Vec3 v0, Vec3 v1, Vec3 v2, Vec3 v3, uint32_t material);
 void add_cube(Scene &s, Vec3 lo, Vec3 hi, uint32_t material);
 
+// rustracer-core/src/config.rs:62-174: parse a scene.toml descriptor and apply
+// it (environment override, extra lights, material overrides). Throws
+// ApiError(RT_ERR_INVALID) on a malformed file, leaving the scene unchanged.
+void apply_descriptor(Scene &scene, const std::string &toml);
+
 std::unique_ptr<Scene> builtin_scene(const std::string &name, uint64_t seed);
 bool builtin_camera(const std::string &name, uint64_t seed, uint32_t index, rt_camera &out);
 

@@ -897,7 +897,7 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
             u->sel_cap = sel_n;
         }
         if (u->loss_cap < steps) {
-            if (u->d_loss) hipFree(u->d_loss);
+            if (u->d_loss) (void)hipFree(u->d_loss);
             u->d_loss = dalloc<double>(steps, "losses");
             u->loss_cap = steps;
         }
