@@ -267,6 +267,19 @@ int rt_trace_closest_device(rt_renderer *r, const rt_ray *d_rays, uint32_t n, rt
  * and the number of 4-wide records (0 when that layout is unavailable). No
  * reference counterpart: it reports this backend's derived layouts. */
 int rt_traversal_info(const rt_renderer *r, uint32_t *mode, uint32_t *bvh_depth, uint32_t *n_records4);
+/* The light gizmo of the tonemap pass (pub light_marker / light_glow,
+ * lib.rs:201-203, drawn by tonemap.wgsl:32-47): a warm ball of world radius
+ * `radius` at `position`, hidden behind nearer geometry. position = NULL hides
+ * it (light_marker = None). Projected every frame with project_marker
+ * (lib.rs:865-885) from the Camera given to rt_update_camera; a camera given
+ * only as a uniform (rt_update_camera_uniform) hides it. */
+int rt_set_light_marker(rt_renderer *r, const float *position, float radius);
+int rt_set_light_glow(rt_renderer *r, float glow);
+/* Renderer::project_marker for a camera and image size: out = (centre x, y in
+ * pixels, radius in pixels, camera distance or 0 when hidden). */
+int rt_project_marker(const rt_camera *cam, uint32_t width, uint32_t height, const float position[3], float radius,
+                      float out[4]);
+
 /* Number of frame graphs captured so far (RT_FLAG_GRAPH): two per output set
  * at first use, then one more each time a captured pass's inputs change. No
  * counterpart in the reference. */

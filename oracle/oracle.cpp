@@ -1088,6 +1088,42 @@ int or_composite(const float *frame, float *accum, uint32_t w, uint32_t h, float
     return 0;
 }
 
+// lib.rs:865-885, in glam's f32 arithmetic: normalize_or_zero(v) = v * (1 / |v|)
+// unless |v| is 0 or not finite; cross and dot in component order.
+static V3 nz(V3 v) {
+    const float len = std::sqrt((v.x * v.x + v.y * v.y) + v.z * v.z);
+    const float r = 1.0f / len;
+    if (!(std::isfinite(r) && r > 0.0f)) return v3(0.0f, 0.0f, 0.0f);
+    return v * r;
+}
+static V3 crs(V3 a, V3 b) { return v3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y); }
+static float dt(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+int or_project_marker(const float cam_position[3], const float cam_look_at[3], const float cam_up[3],
+                      float fov_degrees, float near_plane, uint32_t width, uint32_t height, const float pos[3],
+                      float radius, float out[4]) {
+    const V3 position = L3(cam_position), look = L3(cam_look_at), up0 = L3(cam_up);
+    const V3 forward = nz(look - position);
+    const V3 right = nz(crs(forward, up0));
+    const V3 up = crs(right, forward);
+    const V3 v = L3(pos) - position;
+    const float z = dt(v, forward);
+    out[0] = out[1] = out[2] = out[3] = 0.0f;
+    if (z <= std::fmax(near_plane, 1e-4f)) return 0;
+    const float fov_scale = std::tan(fov_degrees * (3.14159265358979323846f / 180.0f) * 0.5f);
+    const float aspect = (float)width / (float)height;
+    const float x_ndc = dt(v, right) / (z * fov_scale * aspect);
+    const float y_ndc = dt(v, up) / (z * fov_scale);
+    const float visible =
+        (std::fabs(x_ndc) <= 1.5f && std::fabs(y_ndc) <= 1.5f) ? std::sqrt(dt(v, v)) : 0.0f;
+    out[0] = (x_ndc * 0.5f + 0.5f) * (float)width;
+    out[1] = (0.5f - y_ndc * 0.5f) * (float)height;
+    const float radius_px = (radius / (z * fov_scale)) * ((float)height * 0.5f);
+    out[2] = std::fmax(radius_px, 1.5f);
+    out[3] = visible;
+    return 0;
+}
+
 // tonemap.wgsl:19-50, then the rgba8unorm store (round to nearest, half up).
 int or_tonemap(const float *accum, const float *gbuffer, const rt_tonemap_uniform *p, uint32_t w, uint32_t h,
                uint8_t *rgba) {

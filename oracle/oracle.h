@@ -82,6 +82,11 @@ int or_composite(const float *frame, float *accum, uint32_t w, uint32_t h, float
 int or_tonemap(const float *accum, const float *gbuffer, const rt_tonemap_uniform *p,
                uint32_t w, uint32_t h, uint8_t *rgba);
 int or_tonemap_cpu(const float *accum, uint64_t n_px, uint8_t *rgba);
+/* Renderer::project_marker (lib.rs:865-885): the light gizmo's pixel centre,
+ * pixel radius and camera distance (0 = hidden) for the tonemap pass. */
+int or_project_marker(const float cam_position[3], const float cam_look_at[3], const float cam_up[3],
+                      float fov_degrees, float near_plane, uint32_t width, uint32_t height, const float pos[3],
+                      float radius, float out[4]);
 
 /* PCG primitives, exposed for the known-answer tests. */
 uint32_t or_rng_init(uint32_t seed);

@@ -47,6 +47,14 @@ def make_material(kind: int, albedo=(0.8, 0.8, 0.8), roughness=0.5, metallic=0.0
     return m
 
 
+def project_marker(cam: Camera, width: int, height: int, position, radius: float) -> np.ndarray:
+    """Renderer::project_marker (lib.rs:865-885): (x, y, radius in pixels, distance or 0)."""
+    p = np.ascontiguousarray(position, np.float32)
+    out = np.zeros(4, np.float32)
+    check(lib().rt_project_marker(C.byref(cam), width, height, ptr(p), radius, ptr(out)), "rt_project_marker")
+    return out
+
+
 def camera(position, look_at, up=(0.0, 1.0, 0.0), fov_degrees=60.0, near_plane=0.1) -> Camera:
     c = Camera()
     check(lib().rt_camera_default(C.byref(c)), "rt_camera_default")
@@ -404,6 +412,14 @@ class Renderer:
         s = _ffi.PassStats()
         check(lib().rt_get_pass_stats(self._h, C.byref(s)), "rt_get_pass_stats")
         return {name: (s.ms[i], s.launches[i]) for i, name in enumerate(_ffi.PASS_NAMES)}
+
+    def set_light_marker(self, position=None, radius: float = 0.0) -> None:
+        """Draw the light gizmo at `position` (world units), or hide it with None."""
+        p = None if position is None else np.ascontiguousarray(position, np.float32)
+        check(lib().rt_set_light_marker(self._h, ptr(p), radius), "rt_set_light_marker")
+
+    def set_light_glow(self, glow: float) -> None:
+        check(lib().rt_set_light_glow(self._h, glow), "rt_set_light_glow")
 
     def frame_graph_captures(self) -> int:
         n = C.c_uint32()

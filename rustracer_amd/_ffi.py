@@ -199,6 +199,9 @@ SIGNATURES = {
     "rt_trace_closest": (_i, [_vp, _vp, _u32, _vp]),
     "rt_trace_closest_device": (_i, [_vp, _vp, _u32, _vp]),
     "rt_traversal_info": (_i, [_vp, _vp, _vp, _vp]),
+    "rt_set_light_marker": (_i, [_vp, _vp, C.c_float]),
+    "rt_set_light_glow": (_i, [_vp, C.c_float]),
+    "rt_project_marker": (_i, [_P(Camera), C.c_uint32, C.c_uint32, _vp, C.c_float, _vp]),
     "rt_frame_graph_captures": (_i, [_vp, _vp]),
     "rt_renderer_set_texture_array": (_i, [_vp, _u32, _u32, _u32, _u32, _vp]),
     "rt_scene_add_texture": (_i, [_vp, _u32, _u32, _u32, _vp, _P(_u32)]),

@@ -99,6 +99,14 @@ struct rt_renderer {
     bool photons_built = false;
 
     rt_camera_uniform cam{};
+    // light gizmo (lib.rs:201-203, 704-707): drawn by the tonemap pass when a
+    // marker is set and the camera came as a Camera (project_marker needs it)
+    rt_camera host_cam{};
+    bool host_cam_ok = false;
+    bool marker_on = false;
+    float marker_pos[3] = {0.f, 0.f, 0.f};
+    float marker_radius = 0.f;
+    float light_glow = 1.0f;
     uint32_t sample_counter = 0;
     uint32_t frames_since_reset = 0;
 
@@ -386,7 +394,10 @@ struct rt_renderer {
         tm.exposure = cfg.exposure;
         tm.gamma = cfg.gamma;
         tm.width = cfg.width;
-        tm.glow = 1.0f;
+        tm.glow = light_glow;
+        if (marker_on && host_cam_ok)
+            rt::project_marker(host_cam, cfg.width, cfg.height, rt::Vec3{marker_pos[0], marker_pos[1], marker_pos[2]},
+                               marker_radius, tm.marker);
         return tm;
     }
 
@@ -1063,6 +1074,8 @@ int rt_update_camera(rt_renderer *r, const rt_camera *cam) {
     return api_call([&] {
         if (!r || !cam) throw ApiError(RT_ERR_INVALID, "null argument");
         r->cam = rt::camera_to_uniform(*cam, r->cfg.width, r->cfg.height);
+        r->host_cam = *cam;
+        r->host_cam_ok = true;
         r->albedo_ok = false;
     });
 }
@@ -1071,6 +1084,7 @@ int rt_update_camera_uniform(rt_renderer *r, const rt_camera_uniform *u) {
     return api_call([&] {
         if (!r || !u) throw ApiError(RT_ERR_INVALID, "null argument");
         r->cam = *u;
+        r->host_cam_ok = false;
         r->albedo_ok = false;
     });
 }
@@ -1204,6 +1218,33 @@ int rt_read_grid_meta(rt_renderer *r, uint32_t *meta, size_t n, uint32_t *n_cell
         if (!meta) return;
         if (n < 2ull * r->meta_cells) throw ApiError(RT_ERR_INVALID, "capacity below 2*cells");
         read_dev(r, meta, r->d_meta, 8ull * r->meta_cells);
+    });
+}
+
+int rt_set_light_marker(rt_renderer *r, const float *position, float radius) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        r->marker_on = position != nullptr;
+        if (position) {
+            std::memcpy(r->marker_pos, position, sizeof(r->marker_pos));
+            r->marker_radius = radius;
+        }
+    });
+}
+
+int rt_set_light_glow(rt_renderer *r, float glow) {
+    return api_call([&] {
+        if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
+        r->light_glow = glow;
+    });
+}
+
+int rt_project_marker(const rt_camera *cam, uint32_t width, uint32_t height, const float position[3], float radius,
+                      float out[4]) {
+    return api_call([&] {
+        if (!cam || !position || !out) throw ApiError(RT_ERR_INVALID, "null argument");
+        if (width == 0 || height == 0) throw ApiError(RT_ERR_INVALID, "zero image size");
+        rt::project_marker(*cam, width, height, rt::Vec3{position[0], position[1], position[2]}, radius, out);
     });
 }
 

@@ -997,6 +997,27 @@ void camera_zoom(rt_camera &c, float delta) {
     S(c.position, V(c.look_at) - forward * nd);
 }
 
+void project_marker(const rt_camera &c, uint32_t w, uint32_t h, Vec3 pos, float radius, float out[4]) {
+    const Vec3 position{c.position[0], c.position[1], c.position[2]};
+    const Vec3 look{c.look_at[0], c.look_at[1], c.look_at[2]};
+    const Vec3 forward = normalize_or_zero(look - position);
+    const Vec3 right = normalize_or_zero(cross(forward, Vec3{c.up[0], c.up[1], c.up[2]}));
+    const Vec3 up = cross(right, forward);
+    const Vec3 v = pos - position;
+    const float z = dot(v, forward);
+    out[0] = out[1] = out[2] = out[3] = 0.0f;
+    if (z <= std::fmax(c.near_plane, 1e-4f)) return;  // behind the camera: hidden
+    const float fov_scale = std::tan(to_radians(c.fov_degrees) * 0.5f);
+    const float aspect = (float)w / (float)h;
+    const float x_ndc = dot(v, right) / (z * fov_scale * aspect);
+    const float y_ndc = dot(v, up) / (z * fov_scale);
+    // off screen it stays flagged only within 1.5 NDC; the tonemap clips by pixel anyway
+    out[3] = (std::fabs(x_ndc) <= 1.5f && std::fabs(y_ndc) <= 1.5f) ? length(v) : 0.0f;
+    out[0] = (x_ndc * 0.5f + 0.5f) * (float)w;
+    out[1] = (0.5f - y_ndc * 0.5f) * (float)h;
+    out[2] = std::fmax((radius / (z * fov_scale)) * ((float)h * 0.5f), 1.5f);
+}
+
 void camera_pan(rt_camera &c, float dx, float dy) {
     const Vec3 forward = normalize(V(c.look_at) - V(c.position));
     const Vec3 right = normalize(cross(forward, V(c.up)));

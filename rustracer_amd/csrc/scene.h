@@ -105,5 +105,8 @@ rt_camera camera_frame_scene(const Scene &s);
 void camera_orbit(rt_camera &c, float yaw, float pitch);
 void camera_zoom(rt_camera &c, float delta);
 void camera_pan(rt_camera &c, float dx, float dy);
+// lib.rs:865-885 Renderer::project_marker: pixel centre, pixel radius and camera
+// distance (0 = hidden) of the light gizmo at `pos` for a w x h image.
+void project_marker(const rt_camera &c, uint32_t w, uint32_t h, Vec3 pos, float radius, float out[4]);
 
 }  // namespace rt

@@ -58,6 +58,7 @@ def lib():
         L.or_composite.argtypes = [vp, vp, u32, u32, C.c_float]
         L.or_tonemap.argtypes = [vp, vp, C.POINTER(ToneMapUniform), u32, u32, vp]
         L.or_tonemap_cpu.argtypes = [vp, C.c_uint64, vp]
+        L.or_project_marker.argtypes = [vp, vp, vp, C.c_float, C.c_float, u32, u32, vp, C.c_float, vp]
         L.or_rng_init.argtypes = [u32]
         L.or_rng_init.restype = u32
         L.or_rng_next.argtypes = [C.POINTER(u32)]
@@ -68,7 +69,7 @@ def lib():
         L.or_intersect_triangle.restype = C.c_float
         L.or_math_eval.argtypes = [C.c_int, vp, vp, vp, u32]
         for fn in (L.or_trace_closest, L.or_trace_brute, L.or_gather, L.or_photon_trace, L.or_photon_grid,
-                   L.or_denoise_pass, L.or_composite, L.or_tonemap, L.or_tonemap_cpu):
+                   L.or_denoise_pass, L.or_composite, L.or_tonemap, L.or_tonemap_cpu, L.or_project_marker):
             fn.restype = C.c_int
         _lib = L
     return _lib
@@ -188,6 +189,15 @@ def tonemap(accum, gbuf, tm):
     out = np.zeros((h, w, 4), np.uint8)
     assert lib().or_tonemap(_p(np.ascontiguousarray(accum, np.float32)), _p(np.ascontiguousarray(gbuf, np.float32)),
                             C.byref(tm), w, h, _p(out)) == 0
+    return out
+
+
+def project_marker(cam, w, h, pos, radius):
+    p = np.ascontiguousarray(pos, np.float32)
+    out = np.zeros(4, np.float32)
+    f3 = [np.ascontiguousarray(list(v), np.float32) for v in (cam.position, cam.look_at, cam.up)]
+    assert lib().or_project_marker(_p(f3[0]), _p(f3[1]), _p(f3[2]), cam.fov_degrees, cam.near_plane, w, h, _p(p),
+                                   radius, _p(out)) == 0
     return out
 
 

@@ -487,3 +487,35 @@ def test_frame_graphs_replay_bit_exact(extra):
             _check_equal(f"read {k} {name}", p, q)
     # 2 sets x (gather + post) at first use; the exposure change re-captures both post graphs
     assert captures == 6, captures
+
+
+@pytest.mark.parametrize("glow", [1.0, 2.5])
+def test_light_marker_tonemap_bit_exact(glow):
+    """The light gizmo drawn by the tonemap pass (tonemap.wgsl:32-47) at the
+    position project_marker gives (lib.rs:865-885): the output equals the
+    oracle's tonemap with the oracle's projection, the gizmo changes pixels,
+    and hiding it restores the plain tonemap."""
+    scene = rt.Scene.builtin("cornell_box")
+    cam = rt.Scene.builtin_camera("cornell_box")
+    w, h = 64, 48
+    pos = tuple(0.7 * np.float32(a) + 0.3 * np.float32(b) for a, b in zip(cam.position, cam.look_at))
+    print(f"light marker glow={glow}", flush=True)
+    with _renderer(scene, width=w, height=h, spp=2, enable_denoise=False) as r:
+        r.update_camera(cam)
+        r.set_light_marker(pos, 0.08)
+        r.set_light_glow(glow)
+        r.render_frames(2)
+        acc, gb, rgba = r.read_accumulation(), r.read_gbuffer(), r.read_output_rgba8()
+        cfg = r.config
+        r.set_light_marker(None)
+        r.record_frame()
+        acc2, gb2, rgba2 = r.read_accumulation(), r.read_gbuffer(), r.read_output_rgba8()
+    marker = orc.project_marker(cam, w, h, pos, 0.08)
+    assert marker[3] > 0, "the test point must be in view"
+    tm = _ffi.ToneMapUniform(cfg.exposure, cfg.gamma, w, glow, tuple(float(x) for x in marker))
+    ref = orc.tonemap(acc, gb, tm)
+    assert np.array_equal(rgba, ref), first_diff(rgba, ref)
+    plain = orc.tonemap(acc, gb, _ffi.ToneMapUniform(cfg.exposure, cfg.gamma, w, glow, (0.0, 0.0, 0.0, 0.0)))
+    assert (rgba != plain).any()
+    ref2 = orc.tonemap(acc2, gb2, _ffi.ToneMapUniform(cfg.exposure, cfg.gamma, w, glow, (0.0, 0.0, 0.0, 0.0)))
+    assert np.array_equal(rgba2, ref2)
