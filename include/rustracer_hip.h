@@ -177,6 +177,9 @@ int rt_scene_light_descs(const rt_scene *scene, rt_light_desc *out, uint32_t cap
  * out of range skipped). A malformed file or a field of the wrong type returns
  * RT_ERR_INVALID and leaves the scene unchanged. */
 int rt_scene_apply_descriptor(rt_scene *scene, const char *toml_text);
+/* The glTF loader's sRGB decode (loader.rs:28-34) in place, in f32 with the C
+ * library's powf: c <= 0.04045 ? c / 12.92 : ((c + 0.055) / 1.055)^2.4. */
+int rt_srgb_to_linear(float *values, size_t n);
 int rt_light_to_gpu(const rt_light_desc *light, rt_light *out);
 /* Scene::build_bvh (scene.rs:276-286) -> BVHBuilder::build (bvh.rs:135-151):
  * binned SAH (12 bins, <= 4 prims per leaf), root at node 0. Call with NULL

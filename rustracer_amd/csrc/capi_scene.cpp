@@ -1,5 +1,6 @@
 // capi_scene.cpp — C ABI over the host scene model, BVH builder and camera
 // (rustracer-core), plus the error channel. Host-only: no HIP calls.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -170,6 +171,18 @@ int rt_scene_texture_array(const rt_scene *s, uint32_t role, uint8_t *texels, si
         if (!texels) return;
         if (capacity < a.texels.size()) throw ApiError(RT_ERR_INVALID, "capacity below width*height*layers*4");
         std::memcpy(texels, a.texels.data(), a.texels.size());
+    });
+}
+
+int rt_srgb_to_linear(float *values, size_t n) {
+    return api_call([&] {
+        if (n && !values) throw ApiError(RT_ERR_INVALID, "null argument");
+        // loader.rs:28-34 in f32; std::pow(float, float) is the C library's powf,
+        // which Rust's f32::powf also calls
+        for (size_t i = 0; i < n; i++) {
+            const float c = values[i];
+            values[i] = c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f);
+        }
     });
 }
 

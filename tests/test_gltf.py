@@ -1,0 +1,323 @@
+"""glTF loading (rustracer-core/src/loader.rs) into a Scene.
+
+The first five tests restate the reference's own unit tests (loader.rs:651-749).
+The rest load small files built here (JSON with data-URI buffers, GLB, PNG
+textures) and pin each rule of load_gltf / bake_primitive against values
+that follow from the file directly: identity and axis-aligned transforms,
+which are exact in f32; triangulation and winding; smooth normals; material
+defaults; role compaction; area lights; the default sky.
+"""
+import base64
+import io
+import json
+import struct
+
+import numpy as np
+import pytest
+from PIL import Image
+
+import rustracer_amd as rt
+from rustracer_amd import _ffi
+from rustracer_amd import gltf as G
+
+F32 = np.float32
+TEX_NONE = _ffi.TEX_NONE
+
+
+# ─────────────────────────── the reference's unit tests ───────────────────────────
+
+def tex(w, h, color):
+    return G.Texture(np.tile(np.asarray(color, F32), (h, w, 1)))
+
+
+def test_srgb_decode_matches_gltf_spec():
+    v = G.srgb_to_linear(np.array([0.0, 0.04045, 1.0, 0.5], F32))
+    assert abs(v[0] - 0.0) < 1e-6
+    assert abs(v[1] - 0.0031308) < 1e-4  # the spec's crossover point
+    assert abs(v[2] - 1.0) < 1e-6
+    assert abs(v[3] - 0.21404114) < 1e-5
+    # and both branches in f32: c / 12.92 below the crossover
+    assert v[1] == F32(F32(0.04045) / F32(12.92))
+
+
+def test_build_role_compacts_dedups_and_decodes_srgb():
+    master = [tex(2, 2, [0.5, 0.5, 0.5, 1.0]), tex(2, 2, [1.0, 1.0, 1.0, 1.0])]
+    role, mapping = G.build_role(master, [0, 1, 0], True)  # 0 used twice
+    assert len(role) == 2
+    assert abs(role[0].data[0, 0, 0] - 0.21404114) < 1e-5
+    assert role[0].data[0, 0, 3] == 1.0  # alpha is not decoded
+    assert mapping == [0, 1]
+
+
+def test_build_role_drops_mixed_sizes():
+    master = [tex(2, 2, [0.5, 0.5, 0.5, 1.0]), tex(4, 4, [0.5, 0.5, 0.5, 1.0])]
+    role, mapping = G.build_role(master, [0, 1], False)
+    assert len(role) == 1 and mapping == [0, None]
+
+
+def test_build_role_scaled_bakes_normal_scale():
+    master = [tex(2, 2, [1.0, 0.5, 0.0, 1.0])]
+    role, mapping = G.build_role(master, [0], False, scales=[2.0])
+    assert mapping == [0]
+    assert tuple(role[0].data[0, 0]) == (2.0, 1.0, 0.0, 1.0)
+
+
+def test_triangle_emits_samples_the_emissive_texture():
+    bright = G.Texture(np.array([[[1.0, 0.5, 0.2, 1.0], [0, 0, 0, 1]], [[0, 0, 0, 1], [0, 0, 0, 1]]], F32))
+    mat = np.zeros(1, _ffi.MATERIAL)[0]
+    mat["emissive"] = (1.0, 1.0, 1.0)
+    mat["tex_emissive"] = 0
+    mesh = G._Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], F32), np.tile(F32([0, 0, 1]), (3, 1)),
+                   np.zeros((3, 2), F32), np.array([0, 1, 2], np.uint32), 0)
+    assert G._triangle_emits(mesh, 0, mat, [bright])
+    assert not G._triangle_emits(mesh, 0, mat, [G.Texture(np.zeros((2, 2, 4), F32))])
+
+
+# ─────────────────────────── a small glTF writer ───────────────────────────
+
+class Builder:
+    def __init__(self):
+        self.bin = bytearray()
+        self.doc = {"asset": {"version": "2.0"}, "buffers": [], "bufferViews": [], "accessors": [], "meshes": [],
+                    "nodes": [], "scenes": [{"nodes": []}], "scene": 0}
+
+    def _view(self, data: bytes) -> int:
+        while len(self.bin) % 4:
+            self.bin.append(0)
+        self.doc["bufferViews"].append({"buffer": 0, "byteOffset": len(self.bin), "byteLength": len(data)})
+        self.bin += data
+        return len(self.doc["bufferViews"]) - 1
+
+    def accessor(self, arr, ctype, kind) -> int:
+        arr = np.ascontiguousarray(arr)
+        view = self._view(arr.tobytes())
+        count = arr.shape[0] if arr.ndim > 1 else arr.size
+        self.doc["accessors"].append({"bufferView": view, "componentType": ctype, "count": int(count), "type": kind})
+        return len(self.doc["accessors"]) - 1
+
+    def primitive(self, positions, indices=None, normals=None, uvs=None, mode=None, material=None, index_type=5125):
+        p = {"attributes": {"POSITION": self.accessor(np.asarray(positions, F32), 5126, "VEC3")}}
+        if normals is not None:
+            p["attributes"]["NORMAL"] = self.accessor(np.asarray(normals, F32), 5126, "VEC3")
+        if uvs is not None:
+            p["attributes"]["TEXCOORD_0"] = self.accessor(np.asarray(uvs, F32), 5126, "VEC2")
+        if indices is not None:
+            dt = {5121: np.uint8, 5123: np.uint16, 5125: np.uint32}[index_type]
+            p["indices"] = self.accessor(np.asarray(indices, dt), index_type, "SCALAR")
+        if mode is not None:
+            p["mode"] = mode
+        if material is not None:
+            p["material"] = material
+        return p
+
+    def mesh(self, *prims) -> int:
+        self.doc["meshes"].append({"primitives": list(prims)})
+        return len(self.doc["meshes"]) - 1
+
+    def node(self, root=True, **kw) -> int:
+        self.doc["nodes"].append(kw)
+        i = len(self.doc["nodes"]) - 1
+        if root:
+            self.doc["scenes"][0]["nodes"].append(i)
+        return i
+
+    def png_texture(self, rgba_u8) -> int:
+        buf = io.BytesIO()
+        Image.fromarray(np.asarray(rgba_u8, np.uint8)).save(buf, format="PNG")
+        view = self._view(buf.getvalue())
+        self.doc.setdefault("images", []).append({"bufferView": view, "mimeType": "image/png"})
+        self.doc.setdefault("textures", []).append({"source": len(self.doc["images"]) - 1})
+        return len(self.doc["textures"]) - 1
+
+    def material(self, **m) -> int:
+        self.doc.setdefault("materials", []).append(m)
+        return len(self.doc["materials"]) - 1
+
+    def save(self, path, glb=False):
+        doc = json.loads(json.dumps(self.doc))
+        if glb:
+            doc["buffers"] = [{"byteLength": len(self.bin)}]
+            js = json.dumps(doc).encode()
+            js += b" " * (-len(js) % 4)
+            bn = bytes(self.bin) + b"\0" * (-len(self.bin) % 4)
+            body = struct.pack("<II", len(js), 0x4E4F534A) + js + struct.pack("<II", len(bn), 0x004E4942) + bn
+            open(path, "wb").write(struct.pack("<III", 0x46546C67, 2, 12 + len(body)) + body)
+        else:
+            uri = "data:application/octet-stream;base64," + base64.b64encode(bytes(self.bin)).decode()
+            doc["buffers"] = [{"byteLength": len(self.bin), "uri": uri}]
+            open(path, "w").write(json.dumps(doc))
+        return str(path)
+
+
+TRI = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+UP = [[0, 0, 1]] * 3
+
+
+def tris_of(scene):
+    t = scene.flatten_triangles()
+    return np.stack([t["v0"], t["v1"], t["v2"]], axis=1), np.stack([t["n0"], t["n1"], t["n2"]], axis=1), t
+
+
+# ─────────────────────────── loading rules ───────────────────────────
+
+def test_single_triangle_defaults(tmp_path):
+    b = Builder()
+    b.node(mesh=b.mesh(b.primitive(TRI, [0, 1, 2], normals=UP)))
+    s = G.load_gltf(b.save(tmp_path / "t.gltf"))
+    v, n, t = tris_of(s)
+    assert np.array_equal(v[0], np.asarray(TRI, F32)) and np.array_equal(n[0], np.asarray(UP, F32))
+    m = s.materials()
+    # no materials in the file: Material::lambertian(0.8) (loader.rs:350-359)
+    assert len(m) == 1 and m["kind"][0] == 0 and tuple(m["albedo"][0]) == (F32(0.8),) * 3 + (1.0,)
+    assert m["roughness"][0] == 1.0 and m["metallic"][0] == 0.0 and m["ior"][0] == 1.5
+    (env,) = s.light_descs()
+    assert env["kind"] == _ffi.LIGHT_ENVIRONMENT and env["intensity"] == 1.0
+    assert env["vec_a"] == tuple(float(F32(x)) for x in G.SKY_ZENITH)
+
+
+@pytest.mark.parametrize("index_type", [5121, 5123, 5125])
+def test_strips_fans_and_implicit_indices(tmp_path, index_type):
+    quad = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]]
+    b = Builder()
+    b.node(mesh=b.mesh(b.primitive(quad, [0, 1, 2, 3], mode=5, index_type=index_type),  # strip
+                       b.primitive(quad, [0, 1, 3, 2], mode=6, index_type=index_type),  # fan
+                       b.primitive(quad[:3]),                                          # no indices
+                       b.primitive(quad, [0, 1], mode=1)))                             # lines: skipped
+    s = G.load_gltf(b.save(tmp_path / "s.gltf"))
+    v, _, _ = tris_of(s)
+    q = np.asarray(quad, F32)
+    want = [q[[0, 1, 2]], q[[1, 3, 2]],   # strip: the second triangle winds the other way
+            q[[0, 1, 3]], q[[0, 3, 2]],   # fan around vertex 0
+            q[[0, 1, 2]]]
+    assert len(v) == len(want)
+    for a, w in zip(v, want):
+        assert np.array_equal(a, w)
+
+
+def test_smooth_normals_when_none_are_authored(tmp_path):
+    b = Builder()
+    # two triangles of a tent: the shared edge gets the normalized sum
+    pos = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    b.node(mesh=b.mesh(b.primitive(pos, [0, 1, 2, 0, 2, 3])))
+    s = G.load_gltf(b.save(tmp_path / "n.gltf"))
+    _, n, _ = tris_of(s)
+    z, x = np.array([0, 0, 1], F32), np.array([1, 0, 0], F32)
+
+    def normalize(v):  # glam: v * (1 / sqrt((x*x + y*y) + z*z))
+        d = F32(F32(v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])
+        return v * (F32(1.0) / np.sqrt(d))
+
+    # the smooth normal, then normalized once more after the (identity) world transform
+    shared = normalize(normalize(z + x))
+    assert np.array_equal(n[0][0], shared) and np.array_equal(n[0][1], z) and np.array_equal(n[1][2], x)
+
+
+def test_node_hierarchy_and_mirroring(tmp_path):
+    b = Builder()
+    mesh = b.mesh(b.primitive(TRI, [0, 1, 2], normals=[[1, 0, 0]] * 3))
+    child = b.node(root=False, mesh=mesh, scale=[2.0, 2.0, 2.0])
+    mirror = b.node(root=False, mesh=mesh, scale=[-1.0, 1.0, 1.0])
+    b.node(children=[child, mirror], translation=[1.0, 2.0, 3.0])
+    b.node(mesh=mesh, matrix=[4, 0, 0, 0, 0, 4, 0, 0, 0, 0, 4, 0, 0.5, 0, 0, 1])
+    s = G.load_gltf(b.save(tmp_path / "h.gltf"))
+    v, n, _ = tris_of(s)
+    p = np.asarray(TRI, F32)
+    assert np.array_equal(v[0], p * 2 + F32([1, 2, 3]))
+    # mirrored (det < 0): positions mirrored, winding flipped, normal through the inverse transpose
+    assert np.array_equal(v[1], (p * F32([-1, 1, 1]) + F32([1, 2, 3]))[[0, 2, 1]])
+    assert np.array_equal(n[1], np.tile(F32([-1, 0, 0]), (3, 1)))
+    assert np.array_equal(v[2], p * 4 + F32([0.5, 0, 0]))
+    assert np.array_equal(n[0], np.tile(F32([1, 0, 0]), (3, 1)))
+
+
+def test_materials_defaults_alpha_modes_and_missing_material(tmp_path):
+    b = Builder()
+    m0 = b.material(pbrMetallicRoughness={"baseColorFactor": [0.1, 0.2, 0.3, 0.5], "roughnessFactor": -1.0,
+                                          "metallicFactor": -2.0}, alphaMode="MASK")
+    m1 = b.material(alphaMode="BLEND", emissiveFactor=[0.0, 2.0, 0.0])
+    b.node(mesh=b.mesh(b.primitive(TRI, [0, 1, 2], normals=UP, material=m0),
+                       b.primitive(TRI, [0, 1, 2], normals=UP)))       # no material: the last one
+    s = G.load_gltf(b.save(tmp_path / "m.gltf"))
+    m = s.materials()
+    assert tuple(m["albedo"][0]) == (F32(0.1), F32(0.2), F32(0.3), 1.0)
+    assert m["roughness"][0] == 1.0 and m["metallic"][0] == 0.0 and m["flags"][0] == 1 and m["kind"][0] == 0
+    # absent factors take glTF's defaults: roughness 1, metallic 1; BLEND renders as glass
+    assert m["roughness"][1] == 1.0 and m["metallic"][1] == 1.0 and m["kind"][1] == _ffi.MAT_DIELECTRIC
+    _, _, t = tris_of(s)
+    assert list(t["material_id"]) == [0, 1]
+    # the emissive material makes its triangle an area light (triangle 1), plus the sky
+    lights = s.light_descs()
+    assert [l["kind"] for l in lights] == [_ffi.LIGHT_AREA, _ffi.LIGHT_ENVIRONMENT]
+    assert lights[0]["triangle_index"] == 1 and lights[0]["color"] == (0.0, 2.0, 0.0)
+
+
+def test_textures_roles_and_emissive_lights(tmp_path):
+    b = Builder()
+    rng = np.random.default_rng(3)
+    albedo = rng.integers(0, 256, (4, 4, 4), dtype=np.uint8)
+    mr = rng.integers(0, 256, (4, 4, 3), dtype=np.uint8)
+    normal = rng.integers(0, 256, (4, 4, 4), dtype=np.uint8)
+    small = rng.integers(0, 256, (2, 2, 4), dtype=np.uint8)
+    emis = np.zeros((4, 4, 4), np.uint8)
+    emis[..., 3] = 255
+    emis[:2, :2, :3] = 255  # only the top-left quarter glows
+    t_alb, t_mr, t_nrm, t_small, t_emi = (b.png_texture(x) for x in (albedo, mr, normal, small, emis))
+    m0 = b.material(pbrMetallicRoughness={"baseColorTexture": {"index": t_alb},
+                                          "metallicRoughnessTexture": {"index": t_mr}},
+                    normalTexture={"index": t_nrm, "scale": 2.0}, emissiveTexture={"index": t_emi})
+    m1 = b.material(pbrMetallicRoughness={"baseColorTexture": {"index": t_small}})  # 2x2: dropped
+    uv_hot = [[0.1, 0.1]] * 3
+    uv_cold = [[0.8, 0.8]] * 3
+    b.node(mesh=b.mesh(b.primitive(TRI, [0, 1, 2], normals=UP, uvs=uv_hot, material=m0),
+                       b.primitive(TRI, [0, 1, 2], normals=UP, uvs=uv_cold, material=m0),
+                       b.primitive(TRI, [0, 1, 2], normals=UP, material=m1)))
+    s = G.load_gltf(b.save(tmp_path / "tex.gltf"))
+    m = s.materials()
+    assert (m["tex_albedo"][0], m["tex_mr"][0], m["tex_normal"][0], m["tex_emissive"][0]) == (0, 0, 0, 0)
+    assert m["tex_albedo"][1] == TEX_NONE, "a texture of another size is dropped from the role"
+    # an emissive texture with a zero factor glows with the texture as is
+    assert tuple(m["emissive"][0]) == (1.0, 1.0, 1.0)
+    # role arrays as uploaded: sRGB-decoded albedo, linear MR (alpha 1), normal x/y scaled by 2
+    a8 = s.texture_array(0)[0]
+    lin = G.srgb_to_linear(albedo[..., :3].astype(F32) / F32(255.0))
+    assert np.array_equal(a8[..., :3], (np.clip(lin, 0, 1) * F32(255.0)).astype(np.uint8))
+    assert np.array_equal(a8[..., 3], albedo[..., 3])
+    mr8 = s.texture_array(1)[0]
+    assert np.array_equal(mr8[..., :3], mr) and (mr8[..., 3] == 255).all()
+    n8 = s.texture_array(3)[0]
+    scaled = (normal[..., :2].astype(F32) / F32(255.0)) * F32(2.0)
+    assert np.array_equal(n8[..., :2], (np.clip(scaled, 0, 1) * F32(255.0)).astype(np.uint8))
+    # only the triangle whose centroid samples the glowing quarter is a light
+    lights = s.light_descs()
+    assert [l["triangle_index"] for l in lights if l["kind"] == _ffi.LIGHT_AREA] == [0]
+
+
+def test_glb_equals_gltf(tmp_path):
+    b = Builder()
+    t = b.png_texture(np.full((2, 2, 4), 200, np.uint8))
+    mat = b.material(pbrMetallicRoughness={"baseColorTexture": {"index": t}, "baseColorFactor": [1, 0.5, 0.25, 1]})
+    b.node(mesh=b.mesh(b.primitive([[0, 0, 0], [2, 0, 0], [0, 2, 0], [2, 2, 1]], [0, 1, 2, 2, 1, 3],
+                                   uvs=[[0, 0], [1, 0], [0, 1], [1, 1]], material=mat)), translation=[0, 0, -1])
+    a = G.load_gltf(b.save(tmp_path / "x.gltf"))
+    g = G.load_gltf(b.save(tmp_path / "x.glb", glb=True))
+    assert np.array_equal(a.flatten_triangles(), g.flatten_triangles())
+    assert np.array_equal(a.materials(), g.materials())
+    assert np.array_equal(a.texture_array(0), g.texture_array(0))
+    assert a.light_descs() == g.light_descs()
+
+
+def test_index_out_of_range_is_an_error(tmp_path):
+    b = Builder()
+    b.node(mesh=b.mesh(b.primitive(TRI, [0, 1, 7])))
+    with pytest.raises(G.GltfError):
+        G.load_gltf(b.save(tmp_path / "bad.gltf"))
+
+
+def test_loaded_scene_renders_through_the_renderer_api(tmp_path):
+    """The loaded scene goes through the same upload path as any scene (BVH
+    build, flattened records, role arrays): a renderer can be created on it."""
+    b = Builder()
+    b.node(mesh=b.mesh(b.primitive([[-1, 0, -1], [1, 0, -1], [-1, 0, 1], [1, 0, 1]], [0, 2, 1, 1, 2, 3])))
+    s = G.load_gltf(b.save(tmp_path / "floor.gltf"))
+    nodes, prims = s.build_bvh()
+    assert len(prims) == 2 and len(nodes) >= 1
