@@ -177,6 +177,9 @@ int rt_scene_light_descs(const rt_scene *scene, rt_light_desc *out, uint32_t cap
  * out of range skipped). A malformed file or a field of the wrong type returns
  * RT_ERR_INVALID and leaves the scene unchanged. */
 int rt_scene_apply_descriptor(rt_scene *scene, const char *toml_text);
+/* The PNG tonemap of render_headless (examples/common/mod.rs:37-47) over n_px
+ * float4 pixels: rgb -> u8 of (v / (1 + v))^(1/2.2) * 255, truncated; alpha 255. */
+int rt_tonemap_cpu(const float *accum_rgba, size_t n_px, uint8_t *rgba8);
 /* The glTF loader's sRGB decode (loader.rs:28-34) in place, in f32 with the C
  * library's powf: c <= 0.04045 ? c / 12.92 : ((c + 0.055) / 1.055)^2.4. */
 int rt_srgb_to_linear(float *values, size_t n);

@@ -158,6 +158,7 @@ SIGNATURES = {
     "rt_scene_flatten_materials": (_i, [_vp, _vp, _u32]),
     "rt_scene_lights_gpu": (_i, [_vp, _vp, _u32]),
     "rt_scene_light_descs": (_i, [_vp, _vp, C.c_uint32, _vp]),
+    "rt_tonemap_cpu": (_i, [_vp, C.c_size_t, _vp]),
     "rt_srgb_to_linear": (_i, [_vp, C.c_size_t]),
     "rt_scene_apply_descriptor": (_i, [_vp, C.c_char_p]),
     "rt_light_to_gpu": (_i, [_P(LightDesc), _vp]),

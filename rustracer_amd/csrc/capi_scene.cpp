@@ -174,6 +174,25 @@ int rt_scene_texture_array(const rt_scene *s, uint32_t role, uint8_t *texels, si
     });
 }
 
+int rt_tonemap_cpu(const float *accum_rgba, size_t n_px, uint8_t *rgba8) {
+    return api_call([&] {
+        if (n_px && (!accum_rgba || !rgba8)) throw ApiError(RT_ERR_INVALID, "null argument");
+        // examples/common/mod.rs:37-47: Reinhard without exposure, clamp (NaN kept),
+        // powf(1/2.2) * 255 and a saturating cast to u8 (NaN -> 0); alpha 255
+        const float e = 1.0f / 2.2f;
+        for (size_t i = 0; i < n_px; i++) {
+            for (int k = 0; k < 3; k++) {
+                const float v = accum_rgba[4 * i + k];
+                float m = v / (1.0f + v);
+                if (m == m) m = std::fmin(std::fmax(m, 0.0f), 1.0f);
+                const float o = std::pow(m, e) * 255.0f;
+                rgba8[4 * i + k] = (o != o || o <= 0.0f) ? 0 : (o >= 255.0f ? 255 : (uint8_t)o);
+            }
+            rgba8[4 * i + 3] = 255;
+        }
+    });
+}
+
 int rt_srgb_to_linear(float *values, size_t n) {
     return api_call([&] {
         if (n && !values) throw ApiError(RT_ERR_INVALID, "null argument");

@@ -6,7 +6,8 @@
 //
 // <scene> names a built-in procedural scene (cornell_box, dark_room, smoke,
 // cornell36, photon_plain, multiview, fullframe, training), optionally with
-// ":seed" appended; glTF loading is not part of this backend. Defaults,
+// ":seed" appended; glTF files go through `python -m rustracer_amd.render_headless`
+// (same arguments), whose loader is Python. Defaults,
 // config and camera follow render_headless.rs:18-114 (photon_count 50 000,
 // app_like = spp 4 + accumulation 32 + denoise, accum defaults to frames,
 // Camera::frame_scene then zoom/orbit). The result is the accumulation buffer,
@@ -93,16 +94,6 @@ bool write_png(const char *path, const uint8_t *rgba, uint32_t w, uint32_t h) {
     return ok;
 }
 
-// examples/common/mod.rs:37-47
-uint8_t cpu_tonemap(float v) {
-    float m = v / (1.0f + v);
-    m = (m != m) ? m : std::fmin(std::fmax(m, 0.0f), 1.0f);
-    const float o = std::pow(m, 1.0f / 2.2f) * 255.0f;
-    if (o != o || o <= 0.0f) return 0;
-    if (o >= 255.0f) return 255;
-    return (uint8_t)o;
-}
-
 bool is_one(int argc, char **argv, int i) { return argc > i && std::strcmp(argv[i], "1") == 0; }
 
 int fail(const char *what) {
@@ -139,6 +130,15 @@ int main(int argc, char **argv) {
     const float zoom = arg_f(12, 0.0f), yaw = arg_f(13, 0.0f), pitch = arg_f(14, 0.0f);
     const bool raw_save = is_one(argc, argv, 15);
 
+    const auto ends_with = [&](const char *sfx) {
+        const size_t n = std::strlen(sfx);
+        return scene_arg.size() >= n && scene_arg.compare(scene_arg.size() - n, n, sfx) == 0;
+    };
+    if (ends_with(".gltf") || ends_with(".glb")) {
+        std::fprintf(stderr, "render_headless: glTF files load through Python: "
+                             "python -m rustracer_amd.render_headless %s ...\n", argv[1]);
+        return 2;
+    }
     rt_scene *scene = nullptr;
     if (rt_scene_builtin(scene_arg.c_str(), seed, &scene) != RT_OK) return fail("scene");
     uint32_t nt = 0, nm = 0, nl = 0, nmesh = 0;
@@ -175,10 +175,7 @@ int main(int argc, char **argv) {
         std::printf("saved raw %s\n", path.c_str());
     } else {
         std::vector<uint8_t> px((size_t)w * h * 4);
-        for (size_t i = 0; i < (size_t)w * h; i++) {
-            for (int k = 0; k < 3; k++) px[4 * i + k] = cpu_tonemap(acc[4 * i + k]);
-            px[4 * i + 3] = 255;
-        }
+        if (rt_tonemap_cpu(acc.data(), (size_t)w * h, px.data()) != RT_OK) return fail("tonemap");
         if (!write_png(out.c_str(), px.data(), w, h)) return fail("write png");
         std::printf("saved %s\n", out.c_str());
     }

@@ -6,6 +6,7 @@ rgba8) must match exactly; float outputs are compared bit for bit as well
 with the same builtin definitions and no FMA contraction.
 """
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -563,3 +564,40 @@ def test_gltf_scene_gather_bit_exact(tmp_path):
     for name, a, bb in zip(("frame", "moments", "gbuffer"), got, ref[:3]):
         _check_equal(name, a, bb)
     assert got[0][..., :3].max() > 0
+
+
+def test_python_render_headless_on_gltf(tmp_path):
+    """`python -m rustracer_amd.render_headless file.gltf ...` writes the PNG of
+    the accumulation that the same configuration renders in this process."""
+    import subprocess
+    import sys
+
+    from PIL import Image
+
+    from rustracer_amd import gltf as G
+    from rustracer_amd.render_headless import tonemap_cpu
+
+    from .test_gltf import Builder
+
+    b = Builder()
+    quad = [[-1, 0, -1], [1, 0, -1], [-1, 0, 1], [1, 0, 1]]
+    mat = b.material(pbrMetallicRoughness={"baseColorFactor": [0.7, 0.5, 0.3, 1], "metallicFactor": 0.0})
+    b.node(mesh=b.mesh(b.primitive(quad, [0, 2, 1, 1, 2, 3], material=mat)))
+    b.node(mesh=b.mesh(b.primitive([[0, 0, 0], [0.5, 0, 0], [0, 0.7, 0]], [0, 1, 2], material=mat)))
+    path = b.save(tmp_path / "cli.gltf")
+    out = str(tmp_path / "cli.png")
+    print("python render_headless", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, "-m", "rustracer_amd.render_headless", path, "3", out, "0", "0", "0", "2",
+                          "3", "40", "30"], cwd=root, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    png = np.asarray(Image.open(out))
+    s = G.load_gltf(path)
+    cam = rt.Camera()
+    _ffi.lib().rt_camera_frame_scene(s.handle, ctypes.byref(cam))
+    with _renderer(s, width=40, height=30, spp=2, accumulate_frames=3, enable_denoise=False,
+                   photon_count=50000) as r:
+        r.update_camera(cam)
+        r.render_frames(3)
+        want = tonemap_cpu(r.read_accumulation())
+    assert np.array_equal(png, want)

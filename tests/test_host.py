@@ -98,3 +98,42 @@ def test_render_headless_cli_usage_and_no_device():
     assert r.returncode == 2 and "usage" in r.stderr
     r = subprocess.run([exe, "no_such_scene"], capture_output=True, text=True)
     assert r.returncode == 1 and "unknown built-in scene" in r.stderr
+
+
+def test_render_headless_cli_points_gltf_to_python():
+    exe = os.path.join(ROOT, "rustracer_amd", "render_headless")
+    r = subprocess.run([exe, "scene.gltf"], capture_output=True, text=True)
+    assert r.returncode == 2 and "rustracer_amd.render_headless" in r.stderr
+
+
+def test_cpu_tonemap_matches_oracle(rt):
+    """The PNG tonemap of render_headless (examples/common/mod.rs:37-47) in the
+    library equals the oracle's, including NaN, infinities and negatives."""
+    import numpy as np
+
+    from rustracer_amd.render_headless import tonemap_cpu
+
+    from . import oracle_ffi as orc
+
+    rng = np.random.default_rng(1)
+    a = (rng.standard_normal((64, 48, 4)) * 5).astype(np.float32)
+    a[0, 0, 0], a[0, 1, 1], a[0, 2, 2], a[1, 0, 0] = np.nan, np.inf, -np.inf, -1.0
+    a[1, 1, :3] = (1e-30, 1e30, 0.5)
+    assert np.array_equal(tonemap_cpu(a), orc.tonemap_cpu(a))
+
+
+def test_python_cli_loads_gltf_with_sibling_descriptor(rt, tmp_path):
+    """render_headless.rs:67-76: the glTF's sibling scene.toml is applied."""
+    from rustracer_amd.render_headless import load_scene
+
+    from .test_gltf import TRI, UP, Builder
+
+    b = Builder()
+    b.node(mesh=b.mesh(b.primitive(TRI, [0, 1, 2], normals=UP)))
+    path = b.save(tmp_path / "s.gltf")
+    plain = load_scene(path).light_descs()
+    (tmp_path / "scene.toml").write_text('[environment]\nintensity = 2.5\n[[lights]]\ntype = "point"\nintensity = 3\n')
+    lights = load_scene(path).light_descs()
+    assert len(plain) == 1 and plain[0]["intensity"] == 1.0
+    assert [l["kind"] for l in lights] == [3, 0] and lights[0]["intensity"] == 2.5
+    assert load_scene("cornell_box:3").counts()["triangles"] == rt.Scene.builtin("cornell_box", 3).counts()["triangles"]
