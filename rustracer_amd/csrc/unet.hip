@@ -13,12 +13,12 @@
 //   pred = log1p(noisy) + conv1x1(D2, 32->3) + b   (FP32), image = expm1(pred)
 // Loss: mean over pixels and rgb of (pred - log1p(target))^2.
 //
-// Convolution as implicit GEMM on v_mfma_f32_32x32x16_bf16 (one wavefront per
-// workgroup): M = 32 consecutive pixels of one row, N = 32 output channels,
-// K = 9 taps x input channels in steps of 16. Lane layouts (checked on the
-// device): A lane l holds row l%32, k = (l/32)*8 .. +7; B lane l holds column
-// l%32, the same k; accumulator element e of lane l is row (e/4)*8 + (l/32)*4 +
-// e%4, column l%32.
+// Convolution as implicit GEMM on v_mfma_f32_32x32x16_bf16: for the forward
+// and backward-data passes M = 32 output channels, N = 32 consecutive pixels of
+// one row, K = 9 taps x input channels in steps of 16. Lane layouts (checked on
+// the device): A lane l holds row l%32, k = (l/32)*8 .. +7; B lane l holds
+// column l%32, the same k; accumulator element e of lane l is row (e/4)*8 +
+// (l/32)*4 + e%4, column l%32.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -76,65 +76,130 @@ __device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4
 
 // One 3x3 convolution (stride 1, zero padding), optionally with bias and ReLU,
 // or a mask (x > 0) of another tensor at the output position (the ReLU
-// derivative in the backward-data pass). X: [N][H][W][xs] with cin channels at
-// offset 0; Wt: [9][cout][cin]; Y: [N][H][W][ys], cout channels at offset yoff.
-// A workgroup of four wavefronts takes 32 pixels of 4 consecutive rows; the
-// input rows y0-1 .. y0+4, pixels x0-1 .. x0+32 are staged once in LDS (pixel
-// stride cin + 8 to spread banks) and serve all 9 taps; each wavefront computes
-// its row for all NCO x 32 output channels, so every A fragment feeds NCO MFMAs.
-// Grid: (ceil(W/32), ceil(H/4), N).
-constexpr int kConvRows = 4;
-template <int NCO>
-__global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X, int xs, int cin,
+// derivative in the backward-data pass). X: [N][H][W][xs] with CIN channels at
+// offset 0; Wt: [9][cout][CIN]; Y: [N][H][W][ys], cout = 32 * NCO channels at
+// offset yoff.
+//
+// A workgroup of four wavefronts takes 32 pixels of kConvRows consecutive rows;
+// the input rows y0-1 .. y0+kConvRows, pixels x0-1 .. x0+32 are staged once in
+// LDS (pixel stride CIN + 8 to spread banks) and serve all 9 taps. Each
+// wavefront computes kConvRowsPerWave rows for all output channels. The
+// weights are the MFMA's A operand (row = output channel) and the pixels its B
+// operand (column = pixel), so every weight fragment, loaded once per tap and
+// input-channel step, feeds kConvRowsPerWave MFMAs, and each lane ends with 4
+// consecutive output channels of one pixel per accumulator group: the
+// epilogue stores 8 bytes at a time. Grid: (ceil(W/32), ceil(H/kConvRows), N).
+//
+// The inference pass folds the pool and the upsample into the staging of the
+// layers that read them (the training pass keeps P and CAT for its backward):
+// SRC_POOL reads X at twice the output resolution and stages the 2x2 average
+// of pool_kernel; SRC_CONCAT stages channels [0, c1) from X and [c1, CIN) from
+// X2 (half resolution, stride x2s) at (y/2, x/2), as upsample_kernel writes
+// them into CAT. The staged values are bit-identical to those kernels' outputs.
+constexpr int kConvRowsPerWave = 2;
+constexpr int kConvRows = 4 * kConvRowsPerWave;
+enum ConvSrc { SRC_PLAIN = 0, SRC_POOL = 1, SRC_CONCAT = 2 };
+template <int CIN, int NCO, int SRC = SRC_PLAIN>
+__global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X, int xs,
                                                       const bf16 *__restrict__ Wt, const float *__restrict__ bias,
-                                                      bf16 *__restrict__ Y, int ys, int yoff, int cout, int relu,
-                                                      const bf16 *__restrict__ mask, int ms, int H, int W) {
-    extern __shared__ __attribute__((aligned(16))) bf16 tile[];  // [kConvRows + 2][34][cin + 8]
+                                                      bf16 *__restrict__ Y, int ys, int yoff, int relu,
+                                                      const bf16 *__restrict__ mask, int ms, int H, int W,
+                                                      const bf16 *__restrict__ X2 = nullptr, int x2s = 0,
+                                                      int c1 = 0) {
+    constexpr int R = kConvRowsPerWave;
+    constexpr int PS = CIN + 8;  // LDS pixel stride (elements)
+    constexpr int GROUPS = CIN / 8;
+    constexpr int COUT = 32 * NCO;
+    extern __shared__ __attribute__((aligned(16))) bf16 tile[];  // [kConvRows + 2][34][PS]
     const int l = threadIdx.x % 64, wv = threadIdx.x / 64;
     const int x0 = blockIdx.x * 32, y0 = blockIdx.y * kConvRows, n = blockIdx.z;
-    const int ps = cin + 8;
-    const int groups = cin / 8;
-    for (int i = threadIdx.x; i < (kConvRows + 2) * 34 * groups; i += 256) {
-        const int g = i % groups, q = i / groups;
+    for (int i = threadIdx.x; i < (kConvRows + 2) * 34 * GROUPS; i += 256) {
+        const int g = i % GROUPS, q = i / GROUPS;
         const int px = q % 34, row = q / 34;
         const int yy = y0 - 1 + row, xx = x0 - 1 + px;
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-            v = *reinterpret_cast<const uint4 *>(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
-        *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * ps + g * 8) = v;
-    }
-    __syncthreads();
-    const int r = l % 32, kg = (l / 32) * 8;
-    floatx16 acc[NCO];
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            if (SRC == SRC_POOL) {
+                const size_t p00 = ((size_t)(n * 2 * H + 2 * yy) * (2 * W) + 2 * xx) * xs + g * 8;
+                const size_t down = (size_t)(2 * W) * xs;
+                float a[8], b[8], d[8], e[8], o[8];
+                unpack8(ld16(X + p00), a);
+                unpack8(ld16(X + p00 + xs), b);
+                unpack8(ld16(X + p00 + down), d);
+                unpack8(ld16(X + p00 + down + xs), e);
 #pragma unroll
-    for (int j = 0; j < NCO; j++) acc[j] = floatx16{};
-    for (int tap = 0; tap < 9; tap++) {
-        const bf16 *ap = tile + ((wv + tap / 3) * 34 + r + tap % 3) * ps + kg;
-        const bf16 *wp = Wt + ((size_t)tap * cout + r) * cin + kg;
-        for (int c0 = 0; c0 < cin; c0 += 16) {
-            const bf16x8 a = load8(ap + c0);
-#pragma unroll
-            for (int j = 0; j < NCO; j++) {
-                const bf16x8 b = load8(wp + (size_t)j * 32 * cin + c0);
-                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+                for (int k = 0; k < 8; k++) o[k] = 0.25f * (((a[k] + b[k]) + d[k]) + e[k]);
+                v = pack8(o);
+            } else if (SRC == SRC_CONCAT && g * 8 >= c1) {
+                v = ld16(X2 + ((size_t)(n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * x2s + (g * 8 - c1));
+            } else {
+                v = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
             }
         }
+        *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * PS + g * 8) = v;
     }
-    const int y = y0 + wv;
-    if (y >= H) return;
+    __syncthreads();
+    const int c = l % 32, kg = (l / 32) * 8;
+    floatx16 acc[R][NCO];
 #pragma unroll
-    for (int j = 0; j < NCO; j++) {
-        const int co = j * 32 + r;
-        const float bv = bias ? bias[co] : 0.0f;
+    for (int r = 0; r < R; r++)
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int x = x0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
-            if (x >= W) continue;
-            const size_t p = (size_t)(n * H + y) * W + x;
-            float v = acc[j][e] + bv;
-            if (relu) v = fmaxf(v, 0.0f);
-            if (mask && !(bf2f(mask[p * ms + co]) > 0.0f)) v = 0.0f;
-            Y[p * ys + yoff + co] = f2bf(v);
+        for (int j = 0; j < NCO; j++) acc[r][j] = floatx16{};
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+        const bf16 *wp = Wt + ((size_t)tap * COUT + c) * CIN + kg;
+#pragma unroll
+        for (int c0 = 0; c0 < CIN; c0 += 16) {
+            bf16x8 a[NCO], b[R];
+#pragma unroll
+            for (int j = 0; j < NCO; j++) a[j] = load8(wp + (size_t)j * 32 * CIN + c0);
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                b[r] = load8(tile + ((wv * R + r + tap / 3) * 34 + c + tap % 3) * PS + kg + c0);
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int j = 0; j < NCO; j++)
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[r], acc[r][j], 0, 0, 0);
+        }
+    }
+    const int x = x0 + c;
+    if (x >= W) return;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wv * R + r;
+        if (y >= H) break;
+        const size_t p = (size_t)(n * H + y) * W + x;
+#pragma unroll
+        for (int j = 0; j < NCO; j++) {
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                // accumulator elements 4g .. 4g+3: output channels co .. co+3 of pixel x
+                const int co = j * 32 + g * 8 + (l / 32) * 4;
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[e] = acc[r][j][4 * g + e];
+                if (bias) {
+                    const float4 bv = *reinterpret_cast<const float4 *>(bias + co);
+                    v[0] += bv.x;
+                    v[1] += bv.y;
+                    v[2] += bv.z;
+                    v[3] += bv.w;
+                }
+                if (relu)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.0f);
+                if (mask) {
+                    const uint2 mv = *reinterpret_cast<const uint2 *>(mask + p * ms + co);
+                    if (!(__uint_as_float(mv.x << 16) > 0.0f)) v[0] = 0.0f;
+                    if (!(__uint_as_float(mv.x & 0xFFFF0000u) > 0.0f)) v[1] = 0.0f;
+                    if (!(__uint_as_float(mv.y << 16) > 0.0f)) v[2] = 0.0f;
+                    if (!(__uint_as_float(mv.y & 0xFFFF0000u) > 0.0f)) v[3] = 0.0f;
+                }
+                const uint2 o = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                           (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+                *reinterpret_cast<uint2 *>(Y + p * ys + yoff + co) = o;
+            }
         }
     }
 }
@@ -618,22 +683,49 @@ struct rt_unet {
         hip_check(hipGetLastError(), "weights to bf16");
     }
 
+    // src: rtu::SRC_POOL (X at 2H x 2W, pooled while staged) or rtu::SRC_CONCAT
+    // (channels from c1 on read from X2, half resolution) for the inference pass.
     void conv(const rtu::bf16 *X, int xs, int k, bool bwd, rtu::bf16 *Y, int ys, int yoff, bool relu,
-              const rtu::bf16 *mask, int ms, int N, int H, int W) {
+              const rtu::bf16 *mask, int ms, int N, int H, int W, int src = rtu::SRC_PLAIN,
+              const rtu::bf16 *X2 = nullptr, int x2s = 0, int c1 = 0) {
         const ConvLayer &L = kLayers[k];
         const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
         const dim3 grid((W + 31) / 32, (H + rtu::kConvRows - 1) / rtu::kConvRows, N);
         const size_t lds = (size_t)(rtu::kConvRows + 2) * 34 * (cin + 8) * 2;
         const rtu::bf16 *w = bwd ? d_wt[k] : d_wf[k];
         const float *b = bwd ? nullptr : d_p + b_off[k];
-#define RT_CONV(NCO)                                                                                        \
-    hipLaunchKernelGGL(rtu::conv3x3_kernel<NCO>, grid, dim3(256), lds, stream, X, xs, cin, w, b, Y, ys, yoff, \
-                       cout, relu ? 1 : 0, mask, ms, H, W)
-        if (cout == 32) RT_CONV(1);
-        else if (cout == 64) RT_CONV(2);
-        else if (cout == 96) RT_CONV(3);
+        // the channel pairs of the six layers, forward (cin, cout) and backward (cout, cin)
+#define RT_CONV_SRC(CI, NCO, SRC)                                                                                \
+    do {                                                                                                         \
+        if (lds > 65536) { /* above the default dynamic LDS limit: raise it once per instance */              \
+            static bool raised = false;                                                                          \
+            if (!raised) {                                                                                       \
+                hip_check(hipFuncSetAttribute((const void *)rtu::conv3x3_kernel<CI, NCO, SRC>,                   \
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),             \
+                          "conv LDS size");                                                                      \
+                raised = true;                                                                                   \
+            }                                                                                                    \
+        }                                                                                                        \
+        hipLaunchKernelGGL((rtu::conv3x3_kernel<CI, NCO, SRC>), grid, dim3(256), lds, stream, X, xs, w, b, Y,   \
+                           ys, yoff, relu ? 1 : 0, mask, ms, H, W, X2, x2s, c1);                                 \
+    } while (0)
+#define RT_CONV(CI, NCO) RT_CONV_SRC(CI, NCO, rtu::SRC_PLAIN)
+        if (src == rtu::SRC_POOL) {
+            if (cin == 32 && cout == 64) RT_CONV_SRC(32, 2, rtu::SRC_POOL);
+            else throw ApiError(RT_ERR_INVALID, "unsupported pooled convolution");
+        } else if (src == rtu::SRC_CONCAT) {
+            if (cin == 96 && cout == 32) RT_CONV_SRC(96, 1, rtu::SRC_CONCAT);
+            else throw ApiError(RT_ERR_INVALID, "unsupported concatenated convolution");
+        } else if (cin == 16 && cout == 32) RT_CONV(16, 1);
+        else if (cin == 32 && cout == 32) RT_CONV(32, 1);
+        else if (cin == 32 && cout == 64) RT_CONV(32, 2);
+        else if (cin == 64 && cout == 64) RT_CONV(64, 2);
+        else if (cin == 96 && cout == 32) RT_CONV(96, 1);
+        else if (cin == 32 && cout == 96) RT_CONV(32, 3);
+        else if (cin == 64 && cout == 32) RT_CONV(64, 1);
         else throw ApiError(RT_ERR_INVALID, "unsupported channel count");
 #undef RT_CONV
+#undef RT_CONV_SRC
     }
     void wgrad(const rtu::bf16 *G, int k, const rtu::bf16 *X, int xs, int N, int H, int W) {
         const ConvLayer &L = kLayers[k];
@@ -651,13 +743,21 @@ struct rt_unet {
         const int N = a.N, H = a.H, W = a.W, H2 = H / 2, W2 = W / 2;
         conv(a.F, 16, 0, false, a.A1, 32, 0, true, nullptr, 0, N, H, W);
         conv(a.A1, 32, 1, false, a.CAT, 96, 0, true, nullptr, 0, N, H, W);
-        hipLaunchKernelGGL(rtu::pool_kernel, dim3((W2 * 4 + 255) / 256, N * H2), dim3(256), 0, stream, a.CAT, 96,
-                           32, a.P, H, W);
-        conv(a.P, 32, 2, false, a.B1, 64, 0, true, nullptr, 0, N, H2, W2);
+        if (with_target) {  // training: P and CAT[32:96] are kept for the backward pass
+            hipLaunchKernelGGL(rtu::pool_kernel, dim3((W2 * 4 + 255) / 256, N * H2), dim3(256), 0, stream, a.CAT,
+                               96, 32, a.P, H, W);
+            conv(a.P, 32, 2, false, a.B1, 64, 0, true, nullptr, 0, N, H2, W2);
+        } else {  // inference: pooled while staged
+            conv(a.CAT, 96, 2, false, a.B1, 64, 0, true, nullptr, 0, N, H2, W2, rtu::SRC_POOL);
+        }
         conv(a.B1, 64, 3, false, a.B2, 64, 0, true, nullptr, 0, N, H2, W2);
-        hipLaunchKernelGGL(rtu::upsample_kernel, dim3((W * 8 + 255) / 256, N * H), dim3(256), 0, stream, a.B2, 64,
-                           a.CAT, 96, 32, H, W);
-        conv(a.CAT, 96, 4, false, a.D1, 32, 0, true, nullptr, 0, N, H, W);
+        if (with_target) {
+            hipLaunchKernelGGL(rtu::upsample_kernel, dim3((W * 8 + 255) / 256, N * H), dim3(256), 0, stream, a.B2,
+                               64, a.CAT, 96, 32, H, W);
+            conv(a.CAT, 96, 4, false, a.D1, 32, 0, true, nullptr, 0, N, H, W);
+        } else {  // inference: B2 upsampled and concatenated while staged
+            conv(a.CAT, 96, 4, false, a.D1, 32, 0, true, nullptr, 0, N, H, W, rtu::SRC_CONCAT, a.B2, 64, 32);
+        }
         conv(a.D1, 32, 5, false, a.D2, 32, 0, true, nullptr, 0, N, H, W);
         const float scale = 2.0f / (3.0f * (float)a.px());
         if (with_target)

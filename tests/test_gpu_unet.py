@@ -113,3 +113,18 @@ def test_training_lowers_the_loss_and_the_held_out_error():
     assert np.isfinite(disp).all()
     assert learned < plain
     net.close()
+
+
+@pytest.mark.parametrize("n,h,w", [(1, 40, 72), (2, 34, 66)])
+def test_inference_fused_staging_equals_training_forward(n, h, w):
+    """The inference pass stages the pool and the upsample inside the
+    convolutions; its prediction equals the training forward's (separate pool
+    and upsample kernels) bit for bit, including ragged tiles."""
+    print(f"unet fused forward n={n} h={h} w={w}", flush=True)
+    p = _params(4)
+    fb, x, t = _inputs(5, n, h, w)
+    net = UNet(0, 1)
+    net.set_params(p)
+    pred_train, _, _ = net.eval(fb, x, t)
+    pred_infer = net.eval(fb, x, None)
+    assert np.array_equal(pred_train.view(np.uint32), pred_infer.view(np.uint32))
