@@ -448,11 +448,64 @@ inline I3 photon_cell_i(const rt_grid_params *gp, V3 pos) {
               wgsl_i32((pos.y - gp->room_min_y) / gp->cell_size),
               wgsl_i32((pos.z - gp->room_min_z) / gp->cell_size)};
 }
+
 inline uint32_t cell_index(const rt_grid_params *gp, I3 c) {
     return (uint32_t)c.x + (uint32_t)c.y * gp->grid_x + (uint32_t)c.z * gp->grid_x * gp->grid_y;
 }
 inline int32_t iclamp(int32_t v, int32_t lo, int32_t hi) { return std::min(std::max(v, lo), hi); }
 inline int32_t iadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+// gather.wgsl:473-514: the photons of the 27 cells around hp (indices clamped
+// into the grid, so border cells repeat), in cell order and stored order; at
+// most 384 candidates are read and the search stops at the 64th photon within
+// the radius. Returns sum(power * max(n . -incident, 0)) over those photons and
+// their count; *reads counts the candidates read.
+static V3 photon_estimate(const or_photon_map *pm, V3 hp, V3 shn, float radius, uint32_t *hits_out,
+                          uint64_t *reads) {
+    const rt_grid_params *gp = pm->gp;
+    const float r2 = radius * radius;
+    V3 ph_sum = splat(0.0f);
+    uint32_t ph_hits = 0u;
+    uint32_t budget = 384u;
+    bool done = false;
+    const I3 base = photon_cell_i(gp, hp);
+    const int32_t gxmax = (int32_t)gp->grid_x - 1;
+    const int32_t gymax = (int32_t)gp->grid_y - 1;
+    const int32_t gzmax = (int32_t)gp->grid_z - 1;
+    for (int32_t dz = -1; dz <= 1 && !done; dz++) {
+        for (int32_t dy = -1; dy <= 1 && !done; dy++) {
+            for (int32_t dx = -1; dx <= 1 && !done; dx++) {
+                const I3 cc{iclamp(iadd(base.x, dx), 0, gxmax), iclamp(iadd(base.y, dy), 0, gymax),
+                            iclamp(iadd(base.z, dz), 0, gzmax)};
+                const uint32_t cell = cell_index(gp, cc);
+                const uint32_t start = pm->grid_meta[2 * cell];
+                const uint32_t count = pm->grid_meta[2 * cell + 1];
+                for (uint32_t i = 0u; i < count && !done; i++) {
+                    if (budget == 0u) {
+                        done = true;
+                        break;
+                    }
+                    budget -= 1u;
+                    const rt_photon &ph = pm->sorted[start + i];
+                    (*reads)++;
+                    const V3 pp = L3(ph.position);
+                    const float d2 = dot(pp - hp, pp - hp);
+                    if (d2 < r2) {
+                        ph_sum = ph_sum + L3(ph.power) * wmax(dot(shn, -L3(ph.incident)), 0.0f);
+                        ph_hits += 1u;
+                        if (ph_hits >= 64u) {
+                            done = true;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    *hits_out = ph_hits;
+    return ph_sum;
+}
+
 
 // gather.wgsl:258-594, one invocation.
 void gather_pixel(Ctx &c, uint32_t px, uint32_t py, float *frame, float *moments, float *gbuffer) {
@@ -603,46 +656,10 @@ void gather_pixel(Ctx &c, uint32_t px, uint32_t py, float *frame, float *moments
             }
 
             if (ip.use_photons == 1u && !do_spec) {
-                const rt_grid_params *gp = c.pm->gp;
-                const float r2 = ip.photon_radius * ip.photon_radius;
-                V3 ph_sum = splat(0.0f);
                 uint32_t ph_hits = 0u;
-                uint32_t budget = 384u;
-                bool done = false;
-                const I3 base = photon_cell_i(gp, hp);
-                const int32_t gxmax = (int32_t)gp->grid_x - 1;
-                const int32_t gymax = (int32_t)gp->grid_y - 1;
-                const int32_t gzmax = (int32_t)gp->grid_z - 1;
-                for (int32_t dz = -1; dz <= 1 && !done; dz++) {
-                    for (int32_t dy = -1; dy <= 1 && !done; dy++) {
-                        for (int32_t dx = -1; dx <= 1 && !done; dx++) {
-                            const I3 cc{iclamp(iadd(base.x, dx), 0, gxmax), iclamp(iadd(base.y, dy), 0, gymax),
-                                        iclamp(iadd(base.z, dz), 0, gzmax)};
-                            const uint32_t cell = cell_index(gp, cc);
-                            const uint32_t start = c.pm->grid_meta[2 * cell];
-                            const uint32_t count = c.pm->grid_meta[2 * cell + 1];
-                            for (uint32_t i = 0u; i < count && !done; i++) {
-                                if (budget == 0u) {
-                                    done = true;
-                                    break;
-                                }
-                                budget -= 1u;
-                                const rt_photon &ph = c.pm->sorted[start + i];
-                                c.st.photon_reads++;
-                                const V3 pp = L3(ph.position);
-                                const float d2 = dot(pp - hp, pp - hp);
-                                if (d2 < r2) {
-                                    ph_sum = ph_sum + L3(ph.power) * wmax(dot(shn, -L3(ph.incident)), 0.0f);
-                                    ph_hits += 1u;
-                                    if (ph_hits >= 64u) {
-                                        done = true;
-                                        break;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
+                uint64_t reads = 0u;
+                const V3 ph_sum = photon_estimate(c.pm, hp, shn, ip.photon_radius, &ph_hits, &reads);
+                c.st.photon_reads += reads;
                 if (ph_hits > 0u) {
                     add_radiance(s, acc,
                                  thr * dscale * ph_sum * ip.photon_scale /
@@ -1172,6 +1189,18 @@ int or_tonemap_cpu(const float *accum, uint64_t n_px, uint8_t *rgba) {
         }
         rgba[4 * i + 3] = 255;
     }
+    return 0;
+}
+
+
+int or_photon_estimate(const or_photon_map *pm, const float hp[3], const float n[3], float radius, float sum[3],
+                       uint32_t *hits, uint64_t *reads) {
+    uint64_t r = 0;
+    const V3 v = photon_estimate(pm, L3(hp), L3(n), radius, hits, &r);
+    sum[0] = v.x;
+    sum[1] = v.y;
+    sum[2] = v.z;
+    if (reads) *reads = r;
     return 0;
 }
 

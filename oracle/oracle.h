@@ -82,6 +82,10 @@ int or_composite(const float *frame, float *accum, uint32_t w, uint32_t h, float
 int or_tonemap(const float *accum, const float *gbuffer, const rt_tonemap_uniform *p,
                uint32_t w, uint32_t h, uint8_t *rgba);
 int or_tonemap_cpu(const float *accum, uint64_t n_px, uint8_t *rgba);
+/* The photon radius gather of gather.wgsl:473-514 at one point: sum of
+ * power * max(n . -incident, 0) and the photon count, as or_gather uses it. */
+int or_photon_estimate(const or_photon_map *pm, const float hp[3], const float n[3], float radius, float sum[3],
+                       uint32_t *hits, uint64_t *reads);
 /* Renderer::project_marker (lib.rs:865-885): the light gizmo's pixel centre,
  * pixel radius and camera distance (0 = hidden) for the tonemap pass. */
 int or_project_marker(const float cam_position[3], const float cam_look_at[3], const float cam_up[3],

@@ -59,6 +59,8 @@ def lib():
         L.or_tonemap.argtypes = [vp, vp, C.POINTER(ToneMapUniform), u32, u32, vp]
         L.or_tonemap_cpu.argtypes = [vp, C.c_uint64, vp]
         L.or_project_marker.argtypes = [vp, vp, vp, C.c_float, C.c_float, u32, u32, vp, C.c_float, vp]
+        L.or_photon_estimate.argtypes = [C.POINTER(OrPhotonMap), vp, vp, C.c_float, vp, C.POINTER(u32),
+                                         C.POINTER(C.c_uint64)]
         L.or_rng_init.argtypes = [u32]
         L.or_rng_init.restype = u32
         L.or_rng_next.argtypes = [C.POINTER(u32)]
@@ -69,7 +71,8 @@ def lib():
         L.or_intersect_triangle.restype = C.c_float
         L.or_math_eval.argtypes = [C.c_int, vp, vp, vp, u32]
         for fn in (L.or_trace_closest, L.or_trace_brute, L.or_gather, L.or_photon_trace, L.or_photon_grid,
-                   L.or_denoise_pass, L.or_composite, L.or_tonemap, L.or_tonemap_cpu, L.or_project_marker):
+                   L.or_denoise_pass, L.or_composite, L.or_tonemap, L.or_tonemap_cpu, L.or_project_marker,
+                   L.or_photon_estimate):
             fn.restype = C.c_int
         _lib = L
     return _lib
@@ -168,6 +171,18 @@ def photon_grid(photons, n_slots, gp, meta_cells):
     assert lib().or_photon_grid(_p(photons), n_slots, C.byref(gp), meta_cells, _p(meta), _p(sorted_ph),
                                 C.byref(n)) == 0
     return meta, sorted_ph[:n.value]
+
+
+def photon_estimate(meta, srt, gp, hp, n, radius):
+    """gather.wgsl:473-514 at one point: (sum rgb, hits, candidates read)."""
+    meta = np.ascontiguousarray(meta, np.uint32)
+    srt = np.ascontiguousarray(srt, PHOTON)
+    pm = OrPhotonMap(C.pointer(gp), _p(meta), len(meta), _p(srt), len(srt))
+    h, n3 = np.asarray(hp, np.float32), np.asarray(n, np.float32)
+    out = np.zeros(3, np.float32)
+    hits, reads = C.c_uint32(), C.c_uint64()
+    assert lib().or_photon_estimate(C.byref(pm), _p(h), _p(n3), radius, _p(out), C.byref(hits), C.byref(reads)) == 0
+    return out, hits.value, reads.value
 
 
 def denoise_pass(inp, gbuf, mom, params):

@@ -149,3 +149,98 @@ def test_nee_converges_to_closed_form(kind):
     z = np.abs(mean - expect) / np.maximum(se, 1e-6)
     assert np.median(z) < 1.5 and z.max() < 5.5, (np.median(z), z.max())
     assert abs(mean.sum() / expect.sum() - 1) < 0.01
+
+
+def test_dielectric_and_sky_on_bounce_misses():
+    """A lone glass pane under a uniform sky (zenith = horizon = c, intensity I),
+    16 spp (no firefly clamp). gather.wgsl:298-306,386-399: a primary miss is
+    black; at the pane a sample reflects with Schlick probability F(cos) and
+    then misses, adding c*I, or refracts (throughput * albedo) and misses,
+    adding albedo*c*I; nothing else lights the pane. So every pixel inside the
+    pane is (k*cI + (16-k)*albedo*cI)/16 for an integer k, the same k in every
+    channel, and the k are Binomial(16, F) with F from the pixel's angle."""
+    albedo = np.array([0.8, 0.6, 0.4])
+    glass = rt.make_material(2, albedo=tuple(albedo), roughness=0.0, metallic=0.0, ior=1.5)
+    s = _quad_scene(z_plane=-3.0, material=glass, half=1.5)
+    c, intensity = np.array([0.5, 0.7, 0.9]), 2.0
+    s.add_light(_ffi.LIGHT_ENVIRONMENT, vec_a=tuple(c), vec_b=tuple(c), intensity=intensity)
+    cam = rt.camera([0, 0, 0], [0, 0, -1], fov_degrees=60)
+    w, h, spp = 40, 30, 16
+    cfg = _cfg(w, h, spp)
+    frame, _, _, _ = orc.OracleScene.from_scene(s).gather(rt.camera_to_uniform(cam, w, h), image_params(cfg, 1),
+                                                          max_depth=4)
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+    fy = np.tan(np.radians(30))
+    fx = fy * w / h
+    inside = np.ones((h, w), bool)
+    outside = np.ones((h, w), bool)
+    for jx in (-0.5, 0.5):
+        for jy in (-0.5, 0.5):
+            x = 3 * ((xs + jx) * 2 / w - 1) * fx
+            y = 3 * (1 - (ys + jy) * 2 / h) * fy
+            inside &= (np.abs(x) < 1.5 - 1e-4) & (np.abs(y) < 1.5 - 1e-4)
+            outside &= (np.abs(x) > 1.5 + 1e-4) | (np.abs(y) > 1.5 + 1e-4)
+    assert inside.sum() > 100 and outside.sum() > 100
+    assert (frame[outside][:, :3] == 0).all(), "primary misses see black"
+    sky = c * intensity
+    f = frame[inside][:, :3].astype(np.float64)
+    k = (spp * f / sky - spp * albedo) / (1.0 - albedo)  # reflected samples, per channel
+    kr = np.round(k[:, 0])
+    assert np.abs(k - kr[:, None]).max() < 1e-3, "one integer k explains all three channels"
+    assert ((kr >= 0) & (kr <= spp)).all()
+    # expected reflections from Schlick's F at each pixel's mean ray (through (px, py))
+    x = (xs * 2 / w - 1) * fx
+    y = (1 - ys * 2 / h) * fy
+    cos = (1.0 / np.sqrt(x * x + y * y + 1.0))[inside]
+    f0 = ((1.5 - 1.0) / (1.5 + 1.0)) ** 2
+    fr = f0 + (1 - f0) * (1 - cos) ** 5
+    mean, var = (spp * fr).sum(), (spp * fr * (1 - fr)).sum()
+    assert abs(kr.sum() - mean) < 5 * np.sqrt(var), (kr.sum(), mean, np.sqrt(var))
+
+
+def test_sky_gradient_seen_through_reflection_and_refraction():
+    """gather.wgsl:107-111 sky_color = mix(horizon, zenith, sqrt(clamp(y/2 + 1/2))) * I,
+    seen by the reflected ray (y unchanged by the pane at z = const) and the
+    refracted ray (y scaled by 1/ior, Snell at a face of normal +z). Row sums of
+    the pane's pixels match the expectation with Schlick's F within 5 sigma,
+    and swapping zenith and horizon would not."""
+    albedo = np.array([0.9, 0.9, 0.9])
+    glass = rt.make_material(2, albedo=tuple(albedo), roughness=0.0, metallic=0.0, ior=1.5)
+    s = _quad_scene(z_plane=-3.0, material=glass, half=1.5)
+    zen, hor, intensity = np.array([0.1, 0.3, 1.0]), np.array([1.0, 0.8, 0.2]), 1.5
+    s.add_light(_ffi.LIGHT_ENVIRONMENT, vec_a=tuple(zen), vec_b=tuple(hor), intensity=intensity)
+    cam = rt.camera([0, 0, 0], [0, 0, -1], fov_degrees=60)
+    w, h, spp = 40, 30, 64
+    cfg = _cfg(w, h, spp)
+    frame, _, _, _ = orc.OracleScene.from_scene(s).gather(rt.camera_to_uniform(cam, w, h), image_params(cfg, 1),
+                                                          max_depth=4)
+
+    def sky(y, z_, h_):
+        t = np.clip(y * 0.5 + 0.5, 0.0, 1.0)
+        return (h_ * (1 - np.sqrt(t))[..., None] + z_ * np.sqrt(t)[..., None]) * intensity
+
+    fy = np.tan(np.radians(30))
+    fx = fy * w / h
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+    # the camera ray of pixel (px, py) passes through (px + jx, py + jy), jx, jy in [-1/2, 1/2)
+    x = (xs * 2 / w - 1) * fx
+    y = (1 - ys * 2 / h) * fy
+    inside = (np.abs(3 * x) < 1.4) & (np.abs(3 * y) < 1.4)
+    norm = np.sqrt(x * x + y * y + 1.0)
+    ry, cos = y / norm, 1.0 / norm
+    f0 = 0.04
+    fr = f0 + (1 - f0) * (1 - cos) ** 5
+    sr, st = sky(ry, zen, hor), albedo * sky(ry / 1.5, zen, hor)
+    expect = fr[..., None] * sr + (1 - fr[..., None]) * st
+    var = fr[..., None] * (1 - fr[..., None]) * (sr - st) ** 2 / spp + 1e-12
+    got = frame[..., :3].astype(np.float64)
+    swapped = (fr[..., None] * sky(ry, hor, zen) + (1 - fr[..., None]) * albedo * sky(ry / 1.5, hor, zen))
+    rows = [r for r in range(h) if inside[r].sum() > 5]
+    assert len(rows) > 10
+    for r in rows:
+        m = inside[r]
+        d = (got[r][m] - expect[r][m]).sum(0)
+        sd = np.sqrt(var[r][m].sum(0)) + 2e-4 * m.sum()  # + the sub-pixel spread of y
+        assert (np.abs(d) < 5 * sd).all(), (r, d, sd)
+    dsw = np.abs((got - swapped)[inside]).mean()
+    assert dsw > 20 * np.abs((got - expect)[inside]).mean()

@@ -103,3 +103,113 @@ def test_photon_trace_invariants():
     assert not np.isin(mats, [4, 6]).any()
     # deterministic: same seeds, same photons
     assert np.array_equal(ph.view(F), o.photon_trace(gp, 2048, 6).view(F))
+
+
+def _cell_of(gp, hp):
+    c = (np.asarray(hp, F) - F([gp.room_min_x, gp.room_min_y, gp.room_min_z])) / F(gp.cell_size)
+    return np.floor(c).astype(np.int64), c
+
+
+def _brute_estimate(meta, srt, gp, hp, n, radius):
+    """The radius search as a multiset over the 27 clamped neighbour cells (a
+    border cell counts once per offset that clamps onto it), in float64, with
+    no order: valid when neither the 384-candidate nor the 64-hit limit is hit."""
+    base, _ = _cell_of(gp, hp)
+    dims = np.array([gp.grid_x, gp.grid_y, gp.grid_z])
+    hits, total, cand, ambiguous = 0, np.zeros(3), 0, False
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                cc = np.clip(base + (dx, dy, dz), 0, dims - 1)
+                cell = cc[0] + cc[1] * dims[0] + cc[2] * dims[0] * dims[1]
+                start, count = meta[cell]
+                ph = srt[start:start + count]
+                cand += count
+                d2 = ((ph["position"].astype(np.float64) - np.asarray(hp, np.float64)) ** 2).sum(-1)
+                r2 = float(radius) ** 2
+                ambiguous |= bool((np.abs(d2 - r2) < 1e-5 * r2).any())
+                inside = d2 < r2
+                hits += int(inside.sum())
+                w = np.maximum(-(ph["incident"].astype(np.float64) @ np.asarray(n, np.float64)), 0.0)
+                total += (ph["power"].astype(np.float64) * w[:, None])[inside].sum(0)
+    return total, hits, cand, ambiguous
+
+
+def _random_photon_grid(rng, n, dims, cell, lo=(0.0, 0.0, 0.0)):
+    gp = _gp(lo, cell, dims, n)
+    ph = np.zeros(n, _ffi.PHOTON)
+    ext = np.array(dims, np.float64) * cell
+    ph["position"] = (np.array(lo) + rng.uniform(0, 1, (n, 3)) * ext).astype(F)
+    ph["power"] = rng.uniform(0.1, 2.0, (n, 3)).astype(F)
+    inc = rng.normal(size=(n, 3))
+    ph["incident"] = (inc / np.linalg.norm(inc, axis=1, keepdims=True)).astype(F)
+    meta, srt = orc.photon_grid(ph, n, gp, dims[0] * dims[1] * dims[2])
+    return gp, meta, srt
+
+
+@pytest.mark.parametrize("radius", [0.2, 0.4])
+def test_photon_estimate_equals_brute_force_without_truncation(radius):
+    """Interior and border points: the hit count is exactly the multiset count
+    of photons within the radius in the 27 clamped neighbour cells, and the sum
+    matches float64 brute force. With the radius above the cell size photons
+    within the radius outside the neighbourhood are not found (the shader only
+    visits 27 cells)."""
+    rng = np.random.default_rng(7)
+    gp, meta, srt = _random_photon_grid(rng, 600, (8, 8, 8), 0.25)
+    checked, missed_outside = 0, 0
+    for _ in range(400):
+        hp = rng.uniform(0.0, 2.0, 3).astype(F)
+        _, frac = _cell_of(gp, hp)
+        if (np.abs(frac - np.round(frac)) < 1e-3).any():
+            continue  # too close to a cell face for an independent cell index
+        n = rng.normal(size=3)
+        n = (n / np.linalg.norm(n)).astype(F)
+        want, whits, cand, ambiguous = _brute_estimate(meta, srt, gp, hp, n, radius)
+        if ambiguous or cand > 384 or whits >= 64:
+            continue
+        got, hits, reads = orc.photon_estimate(meta, srt, gp, hp, n, radius)
+        assert hits == whits and reads == cand
+        assert np.allclose(got, want, rtol=2e-5, atol=1e-5), (got, want)
+        d_all = np.sqrt(((srt["position"].astype(np.float64) - hp) ** 2).sum(-1))
+        missed_outside += int((d_all < radius).sum()) > whits
+        checked += 1
+    assert checked > 250
+    if radius > 0.25:
+        assert missed_outside > 0
+
+
+def test_photon_estimate_corner_cell_counts_each_clamped_offset():
+    """In the grid's corner cell, 8 of the 27 offsets clamp onto the cell
+    itself: a single photon within the radius there is counted 8 times."""
+    gp = _gp((0.0, 0.0, 0.0), 1.0, (3, 3, 3), 1)
+    ph = np.zeros(1, _ffi.PHOTON)
+    ph["position"] = (0.3, 0.3, 0.3)
+    ph["power"] = (1.0, 2.0, 3.0)
+    ph["incident"] = (0.0, -1.0, 0.0)
+    meta, srt = orc.photon_grid(ph, 1, gp, 27)
+    got, hits, reads = orc.photon_estimate(meta, srt, gp, (0.25, 0.25, 0.25), (0.0, 1.0, 0.0), 0.5)
+    assert hits == 8 and reads == 8
+    assert np.array_equal(got, F([8.0, 16.0, 24.0]))
+    # an edge cell (x = 0, y and z interior) is visited 2 times per (dy, dz): 18 of 27
+    ph["position"] = (0.3, 1.5, 1.5)
+    meta, srt = orc.photon_grid(ph, 1, gp, 27)
+    _, hits, _ = orc.photon_estimate(meta, srt, gp, (0.25, 1.5, 1.5), (0.0, 1.0, 0.0), 0.5)
+    assert hits == 2
+
+
+def test_photon_estimate_limits():
+    """The 384-candidate budget and the 64-hit stop (gather.wgsl:485-505)."""
+    gp = _gp((0.0, 0.0, 0.0), 1.0, (3, 3, 3), 1000)
+    ph = np.zeros(1000, _ffi.PHOTON)
+    ph["power"] = 1.0
+    ph["incident"] = (0.0, -1.0, 0.0)
+    # 500 far photons in the first visited cell (0,0,0), then 100 near ones in the centre cell
+    ph["position"][:500] = (0.05, 0.05, 0.05)
+    ph["position"][500:600] = (1.5, 1.5, 1.5)
+    meta, srt = orc.photon_grid(ph[:600], 600, gp, 27)
+    _, hits, reads = orc.photon_estimate(meta, srt, gp, (1.5, 1.5, 1.5), (0.0, 1.0, 0.0), 0.3)
+    assert reads == 384 and hits == 0, "the budget runs out on the far photons"
+    # only near photons: the search stops at the 64th hit
+    meta, srt = orc.photon_grid(ph[500:600], 100, gp, 27)
+    got, hits, reads = orc.photon_estimate(meta, srt, gp, (1.5, 1.5, 1.5), (0.0, 1.0, 0.0), 0.3)
+    assert hits == 64 and reads == 64 and np.array_equal(got, F([64.0, 64.0, 64.0]))
