@@ -273,6 +273,9 @@ int rt_trace_closest_device(rt_renderer *r, const rt_ray *d_rays, uint32_t n, rt
  * and the number of 4-wide records (0 when that layout is unavailable). No
  * reference counterpart: it reports this backend's derived layouts. */
 int rt_traversal_info(const rt_renderer *r, uint32_t *mode, uint32_t *bvh_depth, uint32_t *n_records4);
+/* Worst-case per-lane traversal stack words of the traversal in use, from the
+ * tree's shape (no counterpart in the reference). */
+int rt_traversal_stack_words(const rt_renderer *r, uint32_t *words);
 /* The light gizmo of the tonemap pass (pub light_marker / light_glow,
  * lib.rs:201-203, drawn by tonemap.wgsl:32-47): a warm ball of world radius
  * `radius` at `position`, hidden behind nearer geometry. position = NULL hides

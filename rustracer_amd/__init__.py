@@ -401,7 +401,9 @@ class Renderer:
         mode, depth, n4 = C.c_uint32(), C.c_uint32(), C.c_uint32()
         check(lib().rt_traversal_info(self._h, C.byref(mode), C.byref(depth), C.byref(n4)),
               "rt_traversal_info")
-        return {"mode": mode.value, "bvh_depth": depth.value, "records4": n4.value}
+        words = C.c_uint32()
+        check(lib().rt_traversal_stack_words(self._h, C.byref(words)), "rt_traversal_stack_words")
+        return {"mode": mode.value, "bvh_depth": depth.value, "records4": n4.value, "stack_words": words.value}
 
     def build_photon_grid_from(self, photons: np.ndarray) -> None:
         p = np.ascontiguousarray(photons, PHOTON)

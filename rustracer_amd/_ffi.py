@@ -200,6 +200,7 @@ SIGNATURES = {
     "rt_read_grid_meta": (_i, [_vp, _vp, C.c_size_t, _P(_u32)]),
     "rt_trace_closest": (_i, [_vp, _vp, _u32, _vp]),
     "rt_trace_closest_device": (_i, [_vp, _vp, _u32, _vp]),
+    "rt_traversal_stack_words": (_i, [_vp, _vp]),
     "rt_traversal_info": (_i, [_vp, _vp, _vp, _vp]),
     "rt_set_light_marker": (_i, [_vp, _vp, C.c_float]),
     "rt_set_light_glow": (_i, [_vp, C.c_float]),

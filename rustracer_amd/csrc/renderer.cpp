@@ -66,6 +66,7 @@ struct rt_renderer {
     float *d_w4nodes = nullptr;    // float4[8 * n_rec4]
     uint32_t n_rec4 = 0;
     bool wide4_ok = false;
+    uint32_t stack_words4 = 0;  // worst-case traversal stack words of the 4-wide records
     float root_min[4] = {0, 0, 0, 0}, root_max[4] = {0, 0, 0, 0};
     bool wide_ok = false;
     uint32_t bvh_depth = 0;
@@ -825,6 +826,7 @@ static void build_trace_layouts(rt_renderer &r, const rt_triangle *tris, const r
                 if (ref != 0xFFFFFFFFu && !(ref & 0x80000000u)) deeper = std::max(deeper, need[ref]);
             need[rec] = (uint32_t)kids_of[rec].size() - 1u + deeper;
         }
+        r.stack_words4 = n_rec > 0 ? 2u * need[0] : 0u;
         if (n_rec > 0 && 2u * need[0] > kTraversalStackWords) r.wide4_ok = false;
         w4.assign((size_t)std::max(n_rec, 1u) * 32, 0.0f);
         for (uint32_t rec = 0; rec < n_rec && r.wide4_ok; rec++) {
@@ -1261,6 +1263,14 @@ int rt_traversal_info(const rt_renderer *r, uint32_t *mode, uint32_t *bvh_depth,
         if (mode) *mode = r->traversal_mode();
         if (bvh_depth) *bvh_depth = r->bvh_depth;
         if (n_records4) *n_records4 = r->wide4_ok ? r->n_rec4 : 0u;
+    });
+}
+
+int rt_traversal_stack_words(const rt_renderer *r, uint32_t *words) {
+    return api_call([&] {
+        if (!r || !words) throw ApiError(RT_ERR_INVALID, "null argument");
+        const uint32_t m = r->traversal_mode();
+        *words = m == 3u ? r->stack_words4 : (m == 0u ? 126u : 2u * (r->bvh_depth > 0 ? r->bvh_depth - 1 : 0));
     });
 }
 
