@@ -435,14 +435,31 @@ __device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneS
         if (ref_is_leaf(cur)) {
             const uint32_t first = cur & 0x07FFFFFFu;
             const uint32_t count = ((cur >> 27) & 15u) + 1u;
-            for (uint32_t i = 0u; i < count; i++) {
+            // two triangles per step: both loads in flight before either test;
+            // the tests still run in slot order
+            for (uint32_t i = 0u; i < count; i += 2u) {
                 const uint32_t slot = first + i;
-                const float4 q0 = S.slot_tris[3 * slot], q1 = S.slot_tris[3 * slot + 1], q2 = S.slot_tris[3 * slot + 2];
-                if (COUNT) cnt.tris++;
-                const float t = mt_slot(ro, rd, xyz(q0), xyz(q1), xyz(q2));
+                const bool two = i + 1u < count;
+                const float4 *q = S.slot_tris + 3u * slot;
+                const float4 a0 = q[0], a1 = q[1], a2 = q[2];
+                float4 b0 = a0, b1 = a1, b2 = a2;
+                if (two) {
+                    b0 = q[3];
+                    b1 = q[4];
+                    b2 = q[5];
+                }
+                if (COUNT) cnt.tris += two ? 2u : 1u;
+                const float t = mt_slot(ro, rd, xyz(a0), xyz(a1), xyz(a2));
                 if (t < hit.t) {
                     hit.t = t;
-                    hit.prim = __float_as_uint(q0.w);
+                    hit.prim = __float_as_uint(a0.w);
+                }
+                if (two) {
+                    const float u = mt_slot(ro, rd, xyz(b0), xyz(b1), xyz(b2));
+                    if (u < hit.t) {
+                        hit.t = u;
+                        hit.prim = __float_as_uint(b0.w);
+                    }
                 }
             }
             if (EARLY_EXIT && hit.t < t_max) return hit;
