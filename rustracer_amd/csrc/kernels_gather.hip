@@ -221,13 +221,16 @@ __device__ __forceinline__ float luma_of(F3 c) { return dot(c, f3(0.2126f, 0.715
 // Per-pixel accumulators of gather.wgsl:36-42. The per-sample luma sums
 // (sample_lum[s] += l) build up in `cur` while sample s runs (the same additions
 // in the same order from 0) and are shifted into sl[] when the sample ends, so
-// after n samples they sit in order at sl[8-n .. 7]. Only static indices are
-// used: the array stays in registers.
-struct Acc {
+// after n samples they sit in order at sl[NS-n .. NS-1]. Only static indices
+// are used: the array stays in registers. NS = 8 holds every sample the clamp
+// looks at (spp <= 8); NS = 1 serves 1-spp passes with one register instead
+// of eight.
+template <int NS>
+struct AccT {
     F3 col;
     float lum_sum, lum2_sum;
     float cur;
-    float sl[8];
+    float sl[NS];
     __device__ __forceinline__ void add(F3 c) {
         col = col + c;
         const float l = luma_of(c);
@@ -237,11 +240,12 @@ struct Acc {
     }
     __device__ __forceinline__ void end_sample() {
 #pragma unroll
-        for (int k = 0; k < 7; k++) sl[k] = sl[k + 1];
-        sl[7] = cur;
+        for (int k = 0; k < NS - 1; k++) sl[k] = sl[k + 1];
+        sl[NS - 1] = cur;
         cur = 0.0f;
     }
 };
+using Acc = AccT<8>;
 
 __device__ __forceinline__ int32_t iclamp(int32_t v, int32_t lo, int32_t hi) { return min(max(v, lo), hi); }
 
@@ -283,14 +287,20 @@ struct FirstHit {
     float depth;
 };
 
+#ifndef RT_PHOTON_BATCH
+#define RT_PHOTON_BATCH 4
+#endif
+// photon candidates whose positions are loaded together in the radius gather
+constexpr uint32_t kPhotonBatch = RT_PHOTON_BATCH;
+
 // One iteration of the bounce loop of gather.wgsl:298-537 at depth b: the
 // closest hit of (ro, rd) and its shading. Returns true when the path goes on
 // (ro, rd and thr then hold the next ray), false when it ends. When `smooth` is
 // given it is set to whether the next ray was scattered by a specular lobe or
 // a dielectric (rather than diffusely, or passed through a cutout texel).
-template <int MODE, bool EARLY, bool COUNT>
+template <int MODE, bool EARLY, bool COUNT, class AccType>
 __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, COUNT> &tr, uint32_t b, F3 &ro,
-                       F3 &rd, F3 &thr, Rng &rng, Acc &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
+                       F3 &rd, F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
                        uint32_t py, bool *smooth = nullptr) {
     if (smooth) *smooth = false;
     const rt_image_params &ip = A.ip;
@@ -439,15 +449,15 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                 for (int k = 0; k < 3; k++) {
                     if (k >= n_ranges) break;
                     const uint2 meta = metas[k];
-                    for (uint32_t i = 0u; i < meta.y && !done; i += 4u) {
-                        const uint32_t nb = min(meta.y - i, 4u);
-                        float4 pp[4];
+                    for (uint32_t i = 0u; i < meta.y && !done; i += kPhotonBatch) {
+                        const uint32_t nb = min(meta.y - i, kPhotonBatch);
+                        float4 pp[kPhotonBatch];
 #pragma unroll
-                        for (uint32_t j = 0; j < 4u; j++)
+                        for (uint32_t j = 0; j < kPhotonBatch; j++)
                             pp[j] = j < nb ? A.sorted_pos[meta.x + i + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         uint32_t accept = 0u;
 #pragma unroll
-                        for (uint32_t j = 0; j < 4u; j++) {
+                        for (uint32_t j = 0; j < kPhotonBatch; j++) {
                             if (j < nb && !done) {
                                 if (budget == 0u) {
                                     done = true;
@@ -464,9 +474,9 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                                 }
                             }
                         }
-                        float4 pw[4], inc[4];
+                        float4 pw[kPhotonBatch], inc[kPhotonBatch];
 #pragma unroll
-                        for (uint32_t j = 0; j < 4u; j++) {
+                        for (uint32_t j = 0; j < kPhotonBatch; j++) {
                             if ((accept >> j) & 1u) {
                                 const rt_photon &ph = A.sorted[meta.x + i + j];
                                 pw[j] = *reinterpret_cast<const float4 *>(&ph.power[0]);
@@ -474,7 +484,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                             }
                         }
 #pragma unroll
-                        for (uint32_t j = 0; j < 4u; j++)
+                        for (uint32_t j = 0; j < kPhotonBatch; j++)
                             if ((accept >> j) & 1u) ph_sum = ph_sum + xyz(pw[j]) * wmax(dot(shn, -xyz(inc[j])), 0.0f);
                     }
                 }
@@ -524,28 +534,31 @@ __device__ __forceinline__ void camera_ray(const ViewParams &V, uint32_t px, uin
     rd = normalize(ld3(cam.forward) + ld3(cam.right) * ux + ld3(cam.up) * uy);
 }
 
-__device__ __forceinline__ void acc_clear(Acc &acc, FirstHit &fh) {
+template <int NS>
+__device__ __forceinline__ void acc_clear(AccT<NS> &acc, FirstHit &fh) {
     acc.col = fs(0.0f);
     acc.lum_sum = 0.0f;
     acc.lum2_sum = 0.0f;
     acc.cur = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc.sl[i] = 0.0f;
+    for (int i = 0; i < NS; i++) acc.sl[i] = 0.0f;
     fh.normal = f3(0.0f, 1.0f, 0.0f);
     fh.depth = 1e30f;
 }
 
 // gather.wgsl:540-595: firefly clamp over the samples, then the frame, moments
 // and G-buffer writes.
-__device__ void finalize(const GatherArgs &A, Acc &acc, uint32_t spp, const FirstHit &fh, size_t out_idx) {
+template <int NS>
+__device__ void finalize(const GatherArgs &A, AccT<NS> &acc, uint32_t spp, const FirstHit &fh, size_t out_idx) {
     // gather.wgsl:548-579 firefly sigma clamp (spp <= 8). The samples' lumas are
-    // at sl[w0 .. 7] (w0 = 8 - spp) in sample order; the insertion sort, median,
-    // spread and clamped sums run over that window with static indices.
+    // at sl[w0 .. NS-1] (w0 = NS - spp; spp <= NS whenever spp <= 8) in sample
+    // order; the insertion sort, median, spread and clamped sums run over that
+    // window with static indices.
     if (spp <= 8u) {
         float *sl = acc.sl;
-        const uint32_t w0 = 8u - spp;
+        const uint32_t w0 = (uint32_t)NS - spp;
 #pragma unroll
-        for (uint32_t i = 1u; i < 8u; i++) {
+        for (uint32_t i = 1u; i < (uint32_t)NS; i++) {
             if (i > w0) {
                 const float key = sl[i];
                 bool moving = true;
@@ -569,7 +582,7 @@ __device__ void finalize(const GatherArgs &A, Acc &acc, uint32_t spp, const Firs
         const uint32_t lo = w0 + (spp - 1u) / 2u, hi = w0 + spp / 2u;
         float slo = 0.0f, shi = 0.0f;
 #pragma unroll
-        for (uint32_t k = 0; k < 8u; k++) {
+        for (uint32_t k = 0; k < (uint32_t)NS; k++) {
             if (k == lo) slo = sl[k];
             if (k == hi) shi = sl[k];
         }
@@ -577,7 +590,7 @@ __device__ void finalize(const GatherArgs &A, Acc &acc, uint32_t spp, const Firs
         const uint32_t half = max(spp / 2u, 1u);
         float spread = 0.0f;
 #pragma unroll
-        for (uint32_t k = 0; k < 8u; k++) {
+        for (uint32_t k = 0; k < (uint32_t)NS; k++) {
             if (k >= w0 && k < w0 + half) {
                 const float d = sl[k] - median;
                 spread += d * d;
@@ -587,7 +600,7 @@ __device__ void finalize(const GatherArgs &A, Acc &acc, uint32_t spp, const Firs
         const float thresh = median + 3.0f * sigma;
         float cl = 0.0f, cl2 = 0.0f;
 #pragma unroll
-        for (uint32_t k = 0; k < 8u; k++) {
+        for (uint32_t k = 0; k < (uint32_t)NS; k++) {
             if (k >= w0) {
                 const float lc = wmin(sl[k], thresh);
                 cl += lc;
@@ -676,7 +689,7 @@ constexpr uint32_t kBands = 8;
 #define RT_PERSISTENT_WAVES 4  // wavefronts per SIMD the register budget is cut for
 #endif
 
-template <int MODE, bool EARLY, bool COUNT>
+template <int MODE, bool EARLY, bool COUNT, int NS = 8>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTENT_WAVES))) void gather_persistent_kernel(
     GatherArgs A) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
@@ -698,7 +711,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
     bool live = false;
     uint32_t px = 0u, py = 0u, s = 0u, b = 0u;
     Rng rng{0u};
-    Acc acc;
+    AccT<NS> acc;
     FirstHit fh;
     acc_clear(acc, fh);
     F3 ro = fs(0.0f), rd = fs(0.0f), thr = fs(0.0f);
@@ -1203,12 +1216,12 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
 
 // Workgroups of the persistent gather that are resident at once on the
 // current device (one 64-lane wavefront each).
-template <int M, bool E, bool C>
+template <int M, bool E, bool C, int NS>
 static uint32_t persistent_blocks() {
     static uint32_t blocks = 0;
     if (blocks == 0u) {
         int per_cu = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtd::gather_persistent_kernel<M, E, C>, 64, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtd::gather_persistent_kernel<M, E, C, NS>, 64, 0) !=
                 hipSuccess ||
             hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -1218,17 +1231,27 @@ static uint32_t persistent_blocks() {
     return blocks;
 }
 
-template <int M, bool E, bool C>
-static hipError_t launch_persistent(const rtd::GatherArgs &A, uint32_t max_blocks, hipStream_t s) {
-    uint32_t nb = persistent_blocks<M, E, C>();
+template <int M, bool E, bool C, int NS>
+static hipError_t launch_persistent_ns(const rtd::GatherArgs &A, uint32_t max_blocks, hipStream_t s) {
+    uint32_t nb = persistent_blocks<M, E, C, NS>();
     if (nb == 0u) return hipErrorInvalidValue;
     if (max_blocks) nb = std::min(nb, max_blocks);
     // more workgroups than pixel slots would only start and leave
     nb = std::min(nb, std::max(A.n_tiles, 1u));
     const hipError_t e = launch_fill_u32(A.regen, 0u, rtd::kBands, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((rtd::gather_persistent_kernel<M, E, C>), dim3(nb), dim3(64), 0, s, A);
+    hipLaunchKernelGGL((rtd::gather_persistent_kernel<M, E, C, NS>), dim3(nb), dim3(64), 0, s, A);
     return hipSuccess;
+}
+
+// 1-spp frames of the default traversal use the one-sample accumulator
+// (fewer live registers on the whole path); everything else the 8-sample one.
+template <int M, bool E, bool C>
+static hipError_t launch_persistent(const rtd::GatherArgs &A, uint32_t max_blocks, hipStream_t s) {
+    if constexpr (M == 3) {
+        if (A.ip.spp <= 1u) return launch_persistent_ns<M, E, C, 1>(A, max_blocks, s);
+    }
+    return launch_persistent_ns<M, E, C, 8>(A, max_blocks, s);
 }
 
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth) {
