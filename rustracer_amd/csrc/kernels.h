@@ -48,7 +48,7 @@ struct GatherLaunch {
     rt_grid_params gp;
     const uint32_t *grid_meta;  // [start, count] per cell
     const rt_photon *sorted_photons;
-    const void *sorted_pos;     // float4 position per sorted photon
+    const void *gather_rec;     // 3 float4 per sorted photon: the radius gather's record
     void *frame;    // float4 per pixel
     void *moments;  // float4 per pixel
     void *gbuffer;  // float4 per pixel
@@ -121,7 +121,7 @@ struct GridBuildLaunch {
     uint32_t meta_cells;   // >= num_cells
     uint32_t *grid_meta;   // 2 per cell
     rt_photon *sorted;
-    void *sorted_pos;      // float4 per sorted photon
+    void *gather_rec;      // 3 float4 per sorted photon (position, power, incident)
     // temporaries (device), sized by grid_build_temp_bytes
     void *temp;
     size_t temp_bytes;

@@ -26,7 +26,7 @@ struct GatherArgs {
     rt_grid_params gp;
     const uint2 *grid_meta;
     const rt_photon *sorted;
-    const float4 *sorted_pos;
+    const float4 *gather_rec;  // 3 per sorted photon: (pos.xyz, power.x), (power.yz, incident.xy), (incident.z)
     float4 *frame;
     float4 *moments;
     float4 *gbuffer;
@@ -290,7 +290,7 @@ struct FirstHit {
 #ifndef RT_PHOTON_BATCH
 #define RT_PHOTON_BATCH 4
 #endif
-// photon candidates whose positions are loaded together in the radius gather
+// photon candidates whose gather records are loaded together in the radius gather
 constexpr uint32_t kPhotonBatch = RT_PHOTON_BATCH;
 
 // One iteration of the bounce loop of gather.wgsl:298-537 at depth b: the
@@ -423,9 +423,8 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                       gzmax = (int32_t)gp.grid_z - 1;
         // Same visiting order, budget and stop rules as the shader's loops; the
         // loads are grouped to shorten the chain of dependent memory reads: the
-        // three cell records of a row together, then up to four candidates'
-        // positions, then the accepted candidates' power and incident
-        // direction. Sums are added in candidate order.
+        // three cell records of a row together, then the gather records of up
+        // to four candidates together. Sums are added in candidate order.
         for (int32_t dz = -1; dz <= 1 && !done; dz++) {
             for (int32_t dy = -1; dy <= 1 && !done; dy++) {
                 const int32_t cy = iclamp((int32_t)((uint32_t)by + (uint32_t)dy), 0, gymax);
@@ -450,12 +449,23 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                     if (k >= n_ranges) break;
                     const uint2 meta = metas[k];
                     for (uint32_t i = 0u; i < meta.y && !done; i += kPhotonBatch) {
+                        // one round trip per batch: position, power and incident
+                        // direction of each candidate come in one 48-byte record
                         const uint32_t nb = min(meta.y - i, kPhotonBatch);
-                        float4 pp[kPhotonBatch];
+                        float4 g0[kPhotonBatch], g1[kPhotonBatch];
+                        float g2[kPhotonBatch];
 #pragma unroll
-                        for (uint32_t j = 0; j < kPhotonBatch; j++)
-                            pp[j] = j < nb ? A.sorted_pos[meta.x + i + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                        uint32_t accept = 0u;
+                        for (uint32_t j = 0; j < kPhotonBatch; j++) {
+                            if (j < nb) {
+                                const float4 *gr = A.gather_rec + 3u * (meta.x + i + j);
+                                g0[j] = gr[0];
+                                g1[j] = gr[1];
+                                g2[j] = gr[2].x;
+                            } else {
+                                g0[j] = g1[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                                g2[j] = 0.0f;
+                            }
+                        }
 #pragma unroll
                         for (uint32_t j = 0; j < kPhotonBatch; j++) {
                             if (j < nb && !done) {
@@ -464,28 +474,18 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                                 } else {
                                     budget -= 1u;
                                     if (COUNT) photon_reads++;
-                                    const F3 d = xyz(pp[j]) - hp;
+                                    const F3 d = xyz(g0[j]) - hp;
                                     const float d2 = dot(d, d);
                                     if (d2 < r2) {
-                                        accept |= 1u << j;
+                                        const F3 pw = f3(g0[j].w, g1[j].x, g1[j].y);
+                                        const F3 inc = f3(g1[j].z, g1[j].w, g2[j]);
+                                        ph_sum = ph_sum + pw * wmax(dot(shn, -inc), 0.0f);
                                         ph_hits += 1u;
                                         if (ph_hits >= 64u) done = true;
                                     }
                                 }
                             }
                         }
-                        float4 pw[kPhotonBatch], inc[kPhotonBatch];
-#pragma unroll
-                        for (uint32_t j = 0; j < kPhotonBatch; j++) {
-                            if ((accept >> j) & 1u) {
-                                const rt_photon &ph = A.sorted[meta.x + i + j];
-                                pw[j] = *reinterpret_cast<const float4 *>(&ph.power[0]);
-                                inc[j] = *reinterpret_cast<const float4 *>(&ph.incident[0]);
-                            }
-                        }
-#pragma unroll
-                        for (uint32_t j = 0; j < kPhotonBatch; j++)
-                            if ((accept >> j) & 1u) ph_sum = ph_sum + xyz(pw[j]) * wmax(dot(shn, -xyz(inc[j])), 0.0f);
                     }
                 }
             }
@@ -1192,7 +1192,7 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
     A.gp = g.gp;
     A.grid_meta = reinterpret_cast<const uint2 *>(g.grid_meta);
     A.sorted = g.sorted_photons;
-    A.sorted_pos = reinterpret_cast<const float4 *>(g.sorted_pos);
+    A.gather_rec = reinterpret_cast<const float4 *>(g.gather_rec);
     A.frame = reinterpret_cast<float4 *>(g.frame);
     A.moments = reinterpret_cast<float4 *>(g.moments);
     A.gbuffer = reinterpret_cast<float4 *>(g.gbuffer);

@@ -356,11 +356,12 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *keys
 }
 
 // photon_scatter.wgsl:35-36 with the sorted order: copy the 64-byte records,
-// and the first 16 bytes (position, zero pad) once more into a dense array that
-// the radius gather scans (16 B per candidate instead of 64).
+// and write the nine floats the radius gather reads (position, power,
+// incident direction) once more as a dense 48-byte record, so one batch of
+// candidate loads brings everything the gather needs.
 __global__ __launch_bounds__(256) void grid_gather_kernel(const rt_photon *photons, const uint32_t *vals,
                                                           const uint32_t *n_valid, rt_photon *sorted,
-                                                          float4 *sorted_pos) {
+                                                          float4 *gather_rec) {
     // 4 lanes per photon, one float4 each
     const size_t t = (size_t)blockIdx.x * 256u + threadIdx.x;
     const size_t i = t >> 2;
@@ -370,7 +371,14 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const rt_photon *photo
     float4 *dst = reinterpret_cast<float4 *>(sorted + i);
     const float4 v = src[part];
     dst[part] = v;
-    if (part == 0u) sorted_pos[i] = v;
+    if (part == 0u) {
+        gather_rec[3 * i] = make_float4(v.x, v.y, v.z, src[1].x);
+    } else if (part == 1u) {
+        const float4 inc = src[2];
+        gather_rec[3 * i + 1] = make_float4(v.y, v.z, inc.x, inc.y);
+    } else if (part == 2u) {
+        gather_rec[3 * i + 2] = make_float4(v.z, 0.0f, 0.0f, 0.0f);
+    }
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t value, size_t n) {
@@ -476,7 +484,7 @@ hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
     }
     const size_t threads = (size_t)n * 4;
     hipLaunchKernelGGL(rtd::grid_gather_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g.photons,
-                       vin, g.n_valid_out, g.sorted, reinterpret_cast<float4 *>(g.sorted_pos));
+                       vin, g.n_valid_out, g.sorted, reinterpret_cast<float4 *>(g.gather_rec));
     return hipGetLastError();
 }
 

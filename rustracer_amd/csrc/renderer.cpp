@@ -91,7 +91,7 @@ struct rt_renderer {
     uint32_t meta_cells = 0;
     uint32_t photon_slots = 0;
     rt_photon *d_photons = nullptr, *d_sorted = nullptr;
-    float *d_sorted_pos = nullptr;  // float4 per sorted photon
+    float *d_gather_rec = nullptr;  // 3 float4 per sorted photon (radius-gather record)
     uint32_t *d_meta = nullptr;
     void *d_temp = nullptr;
     size_t temp_bytes = 0;
@@ -189,7 +189,7 @@ struct rt_renderer {
         }
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
-                        d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_sorted_pos, d_bframe, d_bmom, d_bgb,
+                        d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_gather_rec, d_bframe, d_bmom, d_bgb,
                         d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo, d_fstate,
                         d_tex[0], d_tex[1], d_tex[2], d_tex[3]};
         for (void *b : bufs)
@@ -278,7 +278,7 @@ struct rt_renderer {
         g.gp = gp;
         g.grid_meta = d_meta;
         g.sorted_photons = d_sorted;
-        g.sorted_pos = d_sorted_pos;
+        g.gather_rec = d_gather_rec;
         g.frame = frame;
         g.moments = moments;
         g.gbuffer = gbuf;
@@ -550,7 +550,7 @@ struct rt_renderer {
         g.meta_cells = meta_cells;
         g.grid_meta = d_meta;
         g.sorted = d_sorted;
-        g.sorted_pos = d_sorted_pos;
+        g.gather_rec = d_gather_rec;
         g.temp = d_temp;
         g.temp_bytes = temp_bytes;
         g.n_valid_out = d_nvalid;
@@ -935,8 +935,8 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->photon_slots = r->gp.photon_slots;
     r->d_photons = dev_alloc<rt_photon>(r->photon_slots, "photons");
     r->d_sorted = dev_alloc<rt_photon>(r->photon_slots, "sorted photons");
-    r->d_sorted_pos = dev_alloc<float>((size_t)r->photon_slots * 4, "sorted photon positions");
-    hip_check(hipMemset(r->d_sorted_pos, 0, 16ull * r->photon_slots), "memset");
+    r->d_gather_rec = dev_alloc<float>((size_t)r->photon_slots * 12, "photon gather records");
+    hip_check(hipMemset(r->d_gather_rec, 0, 48ull * r->photon_slots), "memset");
     r->d_meta = dev_alloc<uint32_t>((size_t)r->meta_cells * 2, "grid meta");
     hip_check(hipMemset(r->d_photons, 0, sizeof(rt_photon) * r->photon_slots), "memset");
     hip_check(hipMemset(r->d_sorted, 0, sizeof(rt_photon) * r->photon_slots), "memset");
