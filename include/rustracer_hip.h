@@ -235,6 +235,9 @@ int rt_renderer_set_texture_array(rt_renderer *r, uint32_t role, uint32_t width,
 /* record_frame + queue.submit (lib.rs:666-851): gather, a-trous, composite,
  * tonemap. Asynchronous; rt_synchronize waits (device.poll(Wait)). */
 int rt_render_frame(rt_renderer *r);
+/* n_frames record_frame calls in a row (the frame loop of render_headless.rs:
+ * 132-136): the same frames and results, with up to eight frames rendered by
+ * one camera-pass launch. */
 int rt_render_frames(rt_renderer *r, uint32_t n_frames);
 int rt_synchronize(rt_renderer *r);
 int rt_update_camera(rt_renderer *r, const rt_camera *cam);                 /* lib.rs:853-857 */

@@ -68,6 +68,11 @@ struct GatherLaunch {
     uint32_t regen_blocks;   // cap on its workgroups (0 = all that are resident at once)
     uint32_t refill_min;     // a wavefront takes new pixels once this many of its lanes have none
     const FrameState *fstate;  // non-null: camera and seed frame come from here
+    // persistent kernel: consecutive frames rendered by this one launch (0 or 1
+    // = one). Frame f of the launch has seed frame ip.frame + f and writes
+    // frame/moments/gbuffer + f * frame_stride float4 elements.
+    uint32_t n_frames;
+    uint32_t frame_stride;
 };
 // Device bytes of the path-queue state for n_px pixels.
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth);

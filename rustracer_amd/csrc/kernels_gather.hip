@@ -44,6 +44,8 @@ struct GatherArgs {
     uint32_t *regen;        // persistent kernel: kBands pixel counters, zero at launch
     uint32_t refill_min;    // persistent kernel: lanes without a pixel before a wavefront takes new ones
     const rtk::FrameState *fstate;  // non-null: camera and seed frame of a replayed frame graph
+    uint32_t n_frames;      // persistent kernel: consecutive frames of this launch (>= 1)
+    uint32_t frame_stride;  // float4 elements between the outputs of consecutive frames
 };
 
 struct LightSample {
@@ -520,9 +522,12 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
 }
 
 // gather.wgsl:262-263: the pixel's seed; gather.wgsl:291-296: the jittered camera ray of one sample.
-__device__ __forceinline__ Rng pixel_rng(const ViewParams &V, uint32_t px, uint32_t py) {
-    const uint32_t base_seed = py * V.width + px + V.frame * 2654435761u;
+__device__ __forceinline__ Rng pixel_rng(const ViewParams &V, uint32_t px, uint32_t py, uint32_t frame) {
+    const uint32_t base_seed = py * V.width + px + frame * 2654435761u;
     return rng_init(base_seed * 1664525u + 1013904223u);
+}
+__device__ __forceinline__ Rng pixel_rng(const ViewParams &V, uint32_t px, uint32_t py) {
+    return pixel_rng(V, px, py, V.frame);
 }
 __device__ __forceinline__ void camera_ray(const ViewParams &V, uint32_t px, uint32_t py, Rng &rng, F3 &ro, F3 &rd) {
     const rt_camera_uniform &cam = V.cam;
@@ -707,9 +712,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
     const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t n_slots = A.n_tiles * 64u;
     const uint32_t band_len = (A.n_tiles + kBands - 1u) / kBands * 64u;
+    const uint32_t n_frames = A.n_frames;
     uint32_t band = blockIdx.x % kBands, bands_left = kBands;
     bool live = false;
-    uint32_t px = 0u, py = 0u, s = 0u, b = 0u;
+    uint32_t px = 0u, py = 0u, s = 0u, b = 0u, fi = 0u;
     Rng rng{0u};
     AccT<NS> acc;
     FirstHit fh;
@@ -727,17 +733,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
             uint32_t base = 0u;
             if (lane == (uint32_t)leader) base = atomicAdd(&A.regen[band], n_need);
             base = __builtin_amdgcn_readfirstlane(__shfl(base, leader, 64));
+            // a band's counter runs over the band's slots of frame 0, then
+            // of frame 1, ... of the frames of this launch
             const uint32_t band0 = band * band_len;
             const uint32_t len = band0 < n_slots ? min(band_len, n_slots - band0) : 0u;
+            const uint32_t total = len * n_frames;
             if (!live) {
                 const uint32_t k = base + (uint32_t)__popcll(need & below);
-                if (k < len) {
-                    const uint32_t slot = band0 + k, tile = slot >> 6, l = slot & 63u;
+                if (k < total) {
+                    const uint32_t f = k / len;
+                    const uint32_t slot = band0 + (k - f * len), tile = slot >> 6, l = slot & 63u;
                     px = (tile % A.tiles_x) * 8u + (l & 7u);
                     py = (tile / A.tiles_x) * 8u + (l >> 3);
                     if (px < A.ip.width && py < A.ip.height) {
                         live = true;
-                        rng = pixel_rng(V, px, py);
+                        fi = f;
+                        rng = pixel_rng(V, px, py, V.frame + f);
                         acc_clear(acc, fh);
                         s = 0u;
                         b = 0u;
@@ -746,7 +757,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
                     }
                 }
             }
-            if (base + n_need >= len) {
+            if (base + n_need >= total) {
                 band = (band + 1u) % kBands;
                 bands_left--;
             }
@@ -759,7 +770,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
                 acc.end_sample();
                 s++;
                 if (s >= spp) {
-                    finalize(A, acc, spp, fh, (size_t)py * A.ip.width + px);
+                    finalize(A, acc, spp, fh, (size_t)fi * A.frame_stride + (size_t)py * A.ip.width + px);
                     live = false;
                 } else {
                     b = 0u;
@@ -1211,6 +1222,8 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
     A.regen = g.regen;
     A.refill_min = std::max(g.refill_min, 1u);
     A.fstate = g.fstate;
+    A.n_frames = std::max(g.n_frames, 1u);
+    A.frame_stride = g.frame_stride;
     return A;
 }
 

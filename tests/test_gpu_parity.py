@@ -348,6 +348,41 @@ def test_back_to_back_frames_match_one_at_a_time(flags):
     assert res[0][5] == res[1][5] == 5
 
 
+@pytest.mark.parametrize("spp,lead", [(1, 0), (1, 1), (2, 0)])
+def test_chunked_frames_match_frame_by_frame(spp, lead):
+    """rt_render_frames renders eight frames per camera-pass launch (the
+    launch's pixel counters run over the frames one after another) when the
+    next output set starts a group of eight, single frames otherwise. After
+    `lead` single frames, 19 frames rendered that way (lead 0: two launches of
+    eight, then three single frames; lead 1: seven single frames up to the
+    next group, one launch of eight, four single frames) give the same
+    accumulation, output and last-frame buffers as the same frames run one at
+    a time with a sync in between."""
+    scene = rt.Scene.builtin("multiview", 2)
+    cam = rt.Scene.builtin_camera("multiview", 2, 5)
+    res = []
+    print(f"chunked frames spp={spp} lead={lead}", flush=True)
+    for chunked in (True, False):
+        cfg = dict(width=321, height=187, spp=spp, max_depth=8, enable_denoise=True, accumulate_frames=16,
+                   enable_photon_map=True, photon_count=100000)
+        with _renderer(scene, **cfg) as r:
+            r.update_camera(cam)
+            r.rebuild_photons()
+            for _ in range(lead):
+                r.record_frame()
+            if chunked:
+                r.render_frames(19)
+            else:
+                for _ in range(19):
+                    r.record_frame()
+                    r.synchronize()
+            res.append((r.read_accumulation(), r.read_output_rgba8(), r.read_frame(), r.read_moments(),
+                        r.read_gbuffer(), r.sample_count()))
+    for name, a, b in zip(("accumulation", "rgba8", "frame", "moments", "gbuffer"), res[0][:5], res[1][:5]):
+        _check_equal(name, a, b)
+    assert res[0][5] == res[1][5] == 19 + lead
+
+
 def test_accum64_relmse_matches_numpy():
     """The GPU relMSE reduction equals a float64 numpy evaluation of the same
     formula on the read-back buffers."""
@@ -466,11 +501,11 @@ def test_frame_graphs_replay_bit_exact(extra):
         with _renderer(scene, **cfg) as r:
             r.update_camera(cam0)
             r.rebuild_photons()
-            for k in range(9):
+            for k in range(20):
                 if k == 3:
                     r.update_camera(cam1)
                     r.reset_accumulation_soft()
-                if k == 6:
+                if k == 17:
                     c = r.config
                     c.exposure = 1.7
                     r.config = c
@@ -482,12 +517,13 @@ def test_frame_graphs_replay_bit_exact(extra):
                 captures = r.frame_graph_captures()
         runs[flags] = seq
     a, b = runs[extra], runs[extra | _ffi.FLAG_GRAPH]
-    assert len(a) == len(b) == 6
+    assert len(a) == len(b) == 13
     for k, (x, y) in enumerate(zip(a, b)):
         for name, p, q in zip(("accumulation", "rgba8", "frame", "moments", "gbuffer"), x, y):
             _check_equal(f"read {k} {name}", p, q)
-    # 2 sets x (gather + post) at first use; the exposure change re-captures both post graphs
-    assert captures == 6, captures
+    # 16 output sets x (gather + post) at first use; frames 17-19 after the
+    # exposure change run on sets 1-3, whose post graphs are captured again
+    assert captures == 35, captures
 
 
 @pytest.mark.parametrize("glow", [1.0, 2.5])
