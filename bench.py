@@ -52,7 +52,8 @@ TRAVERSALS = {0: "reference BVH2 nodes", 1: "binary child-box records, reference
               2: "binary child-box records, nearest first", 3: "4-wide records, reference order"}
 
 
-PROFILE_FRAMES = 5     # frames timed one at a time for the per-pass kernel times
+PROFILE_FRAMES = 8     # frames of the per-pass kernel times (one rt_render_frames launch)
+CHUNK = 8              # frames per camera-pass launch of rt_render_frames (16 output sets)
 
 
 def log(msg: str) -> None:
@@ -83,6 +84,9 @@ def main() -> int:
     ap.add_argument("--no-photons", action="store_true")
     ap.add_argument("--record-frame", action="store_true",
                     help="time frames issued one rt_render_frame call at a time for the headline value")
+    ap.add_argument("--no-record-frame", action="store_true",
+                    help="issue no frame one call at a time (every camera-pass launch then renders eight "
+                         "frames when --steps is a multiple of eight; for profiler runs)")
     ap.add_argument("--no-network", action="store_true", help="skip the learned denoiser's timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batch measurement")
     ap.add_argument("--no-extra", action="store_true",
@@ -139,7 +143,11 @@ def main() -> int:
     set_profile(False)
     log(f"photon map rebuilt in {t_photons*1e3:.1f} ms")
 
+    issued = 0  # frames rendered so far: the output-set rotation is at issued mod 16
+
     def timed_frames(n: int, one_call: bool) -> float:
+        nonlocal issued
+        issued += n
         r.synchronize()
         t0 = time.perf_counter()
         if one_call:
@@ -150,22 +158,39 @@ def main() -> int:
         r.synchronize()
         return time.perf_counter() - t0
 
-    timed_frames(args.warmup, False)
+    if args.record_frame and args.no_record_frame:
+        log("--record-frame and --no-record-frame exclude each other")
+        return 2
+    if not args.no_record_frame:
+        timed_frames(args.warmup, False)
     timed_frames(args.warmup, True)
     log(f"warmup {args.warmup} frames done; timing {args.steps} frames")
 
     # timed region: frames back to back (consecutive camera passes overlap on
     # two streams; rt_render_frames also renders eight frames per launch)
     elapsed = timed_frames(args.steps, not args.record_frame)
-    elapsed_other = timed_frames(args.steps, args.record_frame)
-    log(f"{args.steps} frames in {elapsed * 1e3:.1f} ms ({elapsed_other * 1e3:.1f} ms the other way)")
+    elapsed_other = None if args.no_record_frame else timed_frames(args.steps, args.record_frame)
+    log(f"{args.steps} frames in {elapsed * 1e3:.1f} ms" +
+        ("" if elapsed_other is None else f" ({elapsed_other * 1e3:.1f} ms the other way)"))
 
-    # per-pass kernel times, each frame on its own (no overlap), for the rooflines
+    # per-pass kernel times, each launch on its own (no overlap), for the
+    # rooflines: frames issued as in the timed region, starting at a multiple of
+    # eight in the output-set rotation (so rt_render_frames makes one launch)
+    if not args.no_record_frame:
+        for _ in range((-issued) % CHUNK):
+            r.record_frame()
+            issued += 1
+    r.synchronize()
     set_profile(True)
     r.reset_pass_stats()
-    for _ in range(PROFILE_FRAMES):
-        r.record_frame()
+    if args.record_frame:
+        for _ in range(PROFILE_FRAMES):
+            r.record_frame()
+            r.synchronize()
+    else:
+        r.render_frames(PROFILE_FRAMES)
         r.synchronize()
+    issued += PROFILE_FRAMES
     stats = r.pass_stats()
     set_profile(False)
     w, h = args.width, args.height
@@ -177,8 +202,11 @@ def main() -> int:
     cnt = r.count_gather_traffic()
     gather_ms, gather_n = stats["gather"]
     gather_avg_ms = gather_ms / max(gather_n, 1)
-    alg_bytes = (cnt["nodes_visited"] * NODE_BOX_BYTES + cnt["tris_tested"] * TRI_BYTES +
-                 cnt["ray_queries"] * RAY_BYTES + cnt["photon_reads"] * PHOTON_BYTES + w * h * PIXEL_OUT_BYTES)
+    frames_per_launch = PROFILE_FRAMES // max(gather_n, 1)
+    # algorithmic bytes of one frame (counted on the next frame) x the frames of one launch
+    frame_bytes = (cnt["nodes_visited"] * NODE_BOX_BYTES + cnt["tris_tested"] * TRI_BYTES +
+                   cnt["ray_queries"] * RAY_BYTES + cnt["photon_reads"] * PHOTON_BYTES + w * h * PIXEL_OUT_BYTES)
+    alg_bytes = frame_bytes * frames_per_launch
     achieved = alg_bytes / (gather_avg_ms * 1e-3) / 1e9
     mrays = cnt["ray_queries"] * args.steps / elapsed / 1e6
 
@@ -206,11 +234,13 @@ def main() -> int:
                    "photon_map": photons_on,
                    "frames_issued": ("one rt_render_frame call per frame" if args.record_frame else
                                      "rt_render_frames (eight frames per camera-pass launch)")},
-        ("msamples_per_s_render_frames" if args.record_frame else "msamples_per_s_record_frame"):
-            round(w * h * args.steps / elapsed_other / 1e6, 3),
         "mrays_per_s": round(mrays, 2),
-        "pass_ms_per_frame": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},
-        "pass_ms_note": f"kernel times of {PROFILE_FRAMES} frames run one at a time (HIP events on each pass's stream)",
+        "pass_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},
+        "gather_frames_per_launch": frames_per_launch,
+        "gather_ms_per_frame": round(gather_ms / PROFILE_FRAMES, 4),
+        "pass_ms_note": (f"kernel times of {PROFILE_FRAMES} frames issued as in the timed region, each launch on "
+                         "its own (HIP events on each pass's stream); a-trous and composite launch once per "
+                         "frame (a-trous twice)"),
         "photon_rebuild_ms": round(t_photons * 1e3, 2),
         "photon_pass_ms": {k: round(v[0], 3) for k, v in photon_stats.items() if v[1]},
         "setup_s": {"scene": round(t_scene, 2), "create_bvh_upload": round(t_create, 2)},
@@ -218,6 +248,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "gather", "kernel_avg_ms": round(gather_avg_ms, 4),
+                     "frames_per_launch": frames_per_launch,
                      "algorithmic_bytes_per_launch": int(alg_bytes)},
         "cpu_baseline": None,
     }
@@ -252,9 +283,17 @@ def main() -> int:
     if default_run and os.path.exists(traffic_file):
         t = json.load(open(traffic_file))
         form = "path_queues" if args.path_queues else "tile_launch" if args.tile_launch else "persistent"
-        if t.get("traversal_mode") == tinfo["mode"] and t.get("camera_pass") == form:
+        if (t.get("traversal_mode") == tinfo["mode"] and t.get("camera_pass") == form and
+                t.get("frames_per_launch", 1) == frames_per_launch):
             result["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
             result["roofline"]["traffic_source"] = t["source"]
+            result["roofline"]["note"] = ("the algorithmic bytes are served from L2/MALL (HBM traffic is the "
+                                          "'traffic' figure); VALU issue bounds this kernel: "
+                                          "profiles/gather_fullframe_1080p_valu_counters.md")
+
+    if elapsed_other is not None:
+        key = "msamples_per_s_render_frames" if args.record_frame else "msamples_per_s_record_frame"
+        result[key] = round(w * h * args.steps / elapsed_other / 1e6, 3)
 
     copy_gbs, copy_ms = rt.hbm_copy_probe(0, 1 << 30, 20)
     result["roofline"]["measured_copy_gbs"] = round(copy_gbs, 1)
