@@ -115,6 +115,36 @@ def test_training_lowers_the_loss_and_the_held_out_error():
     net.close()
 
 
+def test_chunked_frames_with_denoiser_match_frame_by_frame():
+    """With the learned denoiser attached, frames rendered by rt_render_frames
+    (eight per camera-pass launch, the network run after each frame) give the
+    same network output, accumulation and rgba8 as the same frames issued one
+    at a time with a sync in between."""
+    net = UNet(0, 3)
+    net.set_params(_params(7))
+    scene = rt.Scene.builtin("cornell_box", 2)
+    cam = rt.Scene.builtin_camera("cornell_box", 2)
+    print("chunked frames with the learned denoiser", flush=True)
+    res = []
+    for chunked in (True, False):
+        cfg = rt.RenderConfig(width=96, height=72, spp=1, max_depth=6, enable_denoise=True, accumulate_frames=8)
+        with rt.Renderer(cfg, scene, 0) as r:
+            r.update_camera(cam)
+            r.set_denoiser(net)
+            if chunked:
+                r.render_frames(9)
+            else:
+                for _ in range(9):
+                    r.record_frame()
+                    r.synchronize()
+            res.append((r.read_display(), r.read_accumulation(), r.read_output_rgba8(), r.sample_count()))
+            r.set_denoiser(None)
+    net.close()
+    for name, a, b in zip(("display", "accumulation", "rgba8"), res[0][:3], res[1][:3]):
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+    assert res[0][3] == res[1][3] == 9
+
+
 @pytest.mark.parametrize("n,h,w", [(1, 40, 72), (2, 34, 66)])
 def test_inference_fused_staging_equals_training_forward(n, h, w):
     """The inference pass stages the pool and the upsample inside the
