@@ -78,7 +78,7 @@ def run_method(r, name: str, denoise: bool, start_frame: int, max_frames: int, u
     r.set_sample_count(start_frame)
     r.synchronize()
     t0 = time.perf_counter()
-    r.render_frames(k)  # as a frame loop issues them: eight frames per camera-pass launch
+    r.render_frames(k)  # as a frame loop issues them: up to eight frames per camera-pass launch
     r.synchronize()
     ms = (time.perf_counter() - t0) * 1e3
     check = r.accum64_relmse_display(EPS)

@@ -94,6 +94,7 @@ struct rt_renderer {
     hipEvent_t ev_gdone[kSets] = {}, ev_free[kSets] = {};
     bool free_recorded[kSets] = {};
     uint32_t next_set = 0;
+    uint32_t chunk_stream = 0;
 
     rt_grid_params gp{};
     uint32_t meta_cells = 0;
@@ -395,38 +396,42 @@ struct rt_renderer {
         free_recorded[si] = true;
     }
 
-    // Frames that rt_render_frames may render kChunk at a time: the persistent
-    // camera pass without frame graphs, and a chunk that starts at a chunk
-    // boundary of the output sets.
-    bool chunk_ready() const {
-        return (cfg.flags & (RT_FLAG_GRAPH | RT_FLAG_TILE_LAUNCH)) == 0 && !use_path_queues() &&
-               next_set % (uint32_t)kChunk == 0u;
+    // How many of the next n frames rt_render_frames renders with one camera
+    // pass launch: up to kChunk with the persistent camera pass (no frame
+    // graphs), as long as their output sets are consecutive (no wrap of the
+    // set rotation inside a launch); 1 = a frame on its own.
+    uint32_t chunk_frames(uint32_t n) const {
+        if ((cfg.flags & (RT_FLAG_GRAPH | RT_FLAG_TILE_LAUNCH)) != 0 || use_path_queues()) return 1u;
+        return std::max(1u, std::min({n, (uint32_t)kChunk, (uint32_t)kSets - next_set}));
     }
 
-    // kChunk consecutive frames with one camera pass launch: the same frames,
-    // seeds, output sets and post passes as kChunk render_frame calls (the
+    // nf (2..kChunk) consecutive frames with one camera pass launch: the same
+    // frames, seeds, output sets and post passes as nf render_frame calls (the
     // persistent kernel's band counters run over the chunk's frames one after
     // another), so the end of one frame's camera pass, where few long paths
     // are left, overlaps the start of the next frame's.
-    void render_chunk() {
+    void render_chunk(uint32_t nf) {
         if (cfg.enable_photon_map && !photons_built)
             throw ApiError(RT_ERR_INVALID, "photon map enabled but rt_rebuild_photons has not run");
+        if (nf < 2u || nf > (uint32_t)kChunk || next_set + nf > (uint32_t)kSets)
+            throw ApiError(RT_ERR_INVALID, "frame chunk out of range");
         const uint32_t s0 = next_set;
-        next_set = (next_set + (uint32_t)kChunk) % (uint32_t)kSets;
+        next_set = (next_set + nf) % (uint32_t)kSets;
         float alphas[kChunk];
         uint32_t seed0 = 0;
-        for (int f = 0; f < kChunk; f++) {
+        for (uint32_t f = 0; f < nf; f++) {
             sample_counter = sample_counter + 1u;
             frames_since_reset = frames_since_reset == 0xFFFFFFFFu ? frames_since_reset : frames_since_reset + 1u;
             alphas[f] = current_alpha();
             if (f == 0) seed0 = sample_counter;
         }
-        hipStream_t gs = gstream[(s0 / (uint32_t)kChunk) % (uint32_t)kStreams];
-        for (int f = 0; f < kChunk; f++)
+        hipStream_t gs = gstream[chunk_stream];  // consecutive chunks alternate between the gather streams
+        chunk_stream = (chunk_stream + 1u) % (uint32_t)kStreams;
+        for (uint32_t f = 0; f < nf; f++)
             if (free_recorded[s0 + f]) hip_check(hipStreamWaitEvent(gs, ev_free[s0 + f], 0), "wait set");
         rtk::GatherLaunch gl = gather_launch(alphas[0], set_frame[s0], set_mom[s0], set_gb[s0], s0);
         gl.ip.frame = seed0;
-        gl.n_frames = (uint32_t)kChunk;
+        gl.n_frames = nf;
         gl.frame_stride = (uint32_t)n_px();
         hipEvent_t ev;
         begin(RT_PASS_GATHER, ev, gs);
@@ -434,8 +439,8 @@ struct rt_renderer {
         end(RT_PASS_GATHER, ev, gs);
         hip_check(hipEventRecord(ev_gdone[s0], gs), "record gather");
         hip_check(hipStreamWaitEvent(stream, ev_gdone[s0], 0), "wait gather");
-        for (int f = 0; f < kChunk; f++) {
-            const uint32_t si = s0 + (uint32_t)f;
+        for (uint32_t f = 0; f < nf; f++) {
+            const uint32_t si = s0 + f;
             d_frame = set_frame[si];
             d_ping = set_ping[si];
             d_moments = set_mom[si];
@@ -1128,9 +1133,10 @@ int rt_render_frames(rt_renderer *r, uint32_t n) {
     return api_call([&] {
         if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
         while (n > 0) {
-            if (n >= (uint32_t)rt_renderer::kChunk && r->chunk_ready()) {
-                r->render_chunk();
-                n -= (uint32_t)rt_renderer::kChunk;
+            const uint32_t nf = r->chunk_frames(n);
+            if (nf >= 2u) {
+                r->render_chunk(nf);
+                n -= nf;
             } else {
                 r->render_frame();
                 n--;

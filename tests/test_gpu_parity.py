@@ -350,14 +350,12 @@ def test_back_to_back_frames_match_one_at_a_time(flags):
 
 @pytest.mark.parametrize("spp,lead", [(1, 0), (1, 1), (2, 0)])
 def test_chunked_frames_match_frame_by_frame(spp, lead):
-    """rt_render_frames renders eight frames per camera-pass launch (the
-    launch's pixel counters run over the frames one after another) when the
-    next output set starts a group of eight, single frames otherwise. After
-    `lead` single frames, 19 frames rendered that way (lead 0: two launches of
-    eight, then three single frames; lead 1: seven single frames up to the
-    next group, one launch of eight, four single frames) give the same
-    accumulation, output and last-frame buffers as the same frames run one at
-    a time with a sync in between."""
+    """rt_render_frames renders up to eight frames per camera-pass launch (the
+    launch's pixel counters run over the frames one after another), as many
+    as fit before the output-set rotation wraps. After `lead` single frames,
+    19 frames rendered that way (lead 0: launches of 8, 8 and 3 frames; lead 1:
+    8, 7 and 4) give the same accumulation, output and last-frame buffers as
+    the same frames run one at a time with a sync in between."""
     scene = rt.Scene.builtin("multiview", 2)
     cam = rt.Scene.builtin_camera("multiview", 2, 5)
     res = []
