@@ -479,14 +479,18 @@ __device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneS
                 float e, a;
                 slab(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), ro, inv, e, a);
                 tes[k] = e;
-                bool ok = r[k] != 0xFFFFFFFFu && e <= a && a > 0.0f && e < hit.t;
-                const uint32_t kb = (kill >> (3 * k)) & 7u;
-                if (kb != 0u) {
-                    if ((kb & 1u) && ro.x == bmnx[k]) ok = false;
-                    if ((kb & 2u) && ro.y == bmny[k]) ok = false;
-                    if ((kb & 4u) && ro.z == bmnz[k]) ok = false;
+                if (r[k] != 0xFFFFFFFFu && e <= a && a > 0.0f && e < hit.t) pass |= 1u << k;
+            }
+            // kill bits only matter for rays with a zero direction component
+            // (rare): one branch per record instead of one per child
+            if (kill != 0u) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t kb = (kill >> (3 * k)) & 7u;
+                    if (((kb & 1u) && ro.x == bmnx[k]) || ((kb & 2u) && ro.y == bmny[k]) ||
+                        ((kb & 4u) && ro.z == bmnz[k]))
+                        pass &= ~(1u << k);
                 }
-                if (ok) pass |= 1u << k;
             }
             if (COUNT) cnt.nodes += 4;
             if (pass != 0u) {
