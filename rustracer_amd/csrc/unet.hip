@@ -82,11 +82,14 @@ __device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4
 //
 // A workgroup of four wavefronts takes 32 pixels of kConvRows consecutive rows;
 // the input rows y0-1 .. y0+kConvRows, pixels x0-1 .. x0+32 are staged once in
-// LDS (pixel stride CIN + 8 to spread banks) and serve all 9 taps. Each
-// wavefront computes kConvRowsPerWave rows for all output channels. The
-// weights are the MFMA's A operand (row = output channel) and the pixels its B
-// operand (column = pixel), so every weight fragment, loaded once per tap and
-// input-channel step, feeds kConvRowsPerWave MFMAs, and each lane ends with 4
+// LDS (pixel stride CIN + 8 to spread banks) and serve all 9 taps; the layer's
+// weights are staged in LDS beside them (row stride CIN + 8), so the MFMA loop
+// waits on no global load. Both stagings issue up to eight 16-byte loads per
+// thread before writing any of them. Each wavefront computes
+// kConvRowsPerWave rows for all output channels. The weights are the MFMA's A
+// operand (row = output channel) and the pixels its B operand (column =
+// pixel), so every weight fragment, read once per tap and input-channel step,
+// feeds kConvRowsPerWave MFMAs, and each lane ends with 4
 // consecutive output channels of one pixel per accumulator group: the
 // epilogue stores 8 bytes at a time. Grid: (ceil(W/32), ceil(H/kConvRows), N).
 //
@@ -110,33 +113,90 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
     constexpr int PS = CIN + 8;  // LDS pixel stride (elements)
     constexpr int GROUPS = CIN / 8;
     constexpr int COUT = 32 * NCO;
-    extern __shared__ __attribute__((aligned(16))) bf16 tile[];  // [kConvRows + 2][34][PS]
+    constexpr int WPS = CIN + 8;  // LDS weight row stride (elements)
+    // dynamic LDS: the input tile [kConvRows + 2][34][PS], then the layer's
+    // weights [9][COUT][WPS] (staged once, so the MFMA loop reads only LDS)
+    extern __shared__ __attribute__((aligned(16))) bf16 tile[];
+    bf16 *wl = tile + (kConvRows + 2) * 34 * PS;
     const int l = threadIdx.x % 64, wv = threadIdx.x / 64;
     const int x0 = blockIdx.x * 32, y0 = blockIdx.y * kConvRows, n = blockIdx.z;
-    for (int i = threadIdx.x; i < (kConvRows + 2) * 34 * GROUPS; i += 256) {
-        const int g = i % GROUPS, q = i / GROUPS;
-        const int px = q % 34, row = q / 34;
-        const int yy = y0 - 1 + row, xx = x0 - 1 + px;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-            if (SRC == SRC_POOL) {
-                const size_t p00 = ((size_t)(n * 2 * H + 2 * yy) * (2 * W) + 2 * xx) * xs + g * 8;
-                const size_t down = (size_t)(2 * W) * xs;
-                float a[8], b[8], d[8], e[8], o[8];
-                unpack8(ld16(X + p00), a);
-                unpack8(ld16(X + p00 + xs), b);
-                unpack8(ld16(X + p00 + down), d);
-                unpack8(ld16(X + p00 + down + xs), e);
+    // Staging: each thread takes items threadIdx.x + 256 k; the loads of
+    // kStageBatch items are all issued before their values are combined and
+    // written to LDS, so their latencies overlap.
+    constexpr int ITEMS = (kConvRows + 2) * 34 * GROUPS;
+    constexpr int PER = (ITEMS + 255) / 256;
+    constexpr int BATCH = SRC == SRC_POOL ? 2 : 8;
 #pragma unroll
-                for (int k = 0; k < 8; k++) o[k] = 0.25f * (((a[k] + b[k]) + d[k]) + e[k]);
-                v = pack8(o);
-            } else if (SRC == SRC_CONCAT && g * 8 >= c1) {
-                v = ld16(X2 + ((size_t)(n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * x2s + (g * 8 - c1));
+    for (int k0 = 0; k0 < PER; k0 += BATCH) {
+        uint4 raw[BATCH][SRC == SRC_POOL ? 4 : 1];
+        bool in[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int i = threadIdx.x + (k0 + k) * 256;
+            const int g = i % GROUPS, q = i / GROUPS;
+            const int px = q % 34, row = q / 34;
+            const int yy = y0 - 1 + row, xx = x0 - 1 + px;
+            in[k] = k0 + k < PER && i < ITEMS && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            if (SRC == SRC_POOL) {
+                raw[k][0] = raw[k][1] = raw[k][2] = raw[k][3] = make_uint4(0u, 0u, 0u, 0u);
+                if (in[k]) {
+                    const size_t p00 = ((size_t)(n * 2 * H + 2 * yy) * (2 * W) + 2 * xx) * xs + g * 8;
+                    const size_t down = (size_t)(2 * W) * xs;
+                    raw[k][0] = ld16(X + p00);
+                    raw[k][1] = ld16(X + p00 + xs);
+                    raw[k][2] = ld16(X + p00 + down);
+                    raw[k][3] = ld16(X + p00 + down + xs);
+                }
             } else {
-                v = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
+                raw[k][0] = make_uint4(0u, 0u, 0u, 0u);
+                if (in[k]) {
+                    if (SRC == SRC_CONCAT && g * 8 >= c1)
+                        raw[k][0] = ld16(X2 + ((size_t)(n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * x2s + (g * 8 - c1));
+                    else
+                        raw[k][0] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
+                }
             }
         }
-        *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * PS + g * 8) = v;
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int i = threadIdx.x + (k0 + k) * 256;
+            if (k0 + k < PER && i < ITEMS) {
+                const int g = i % GROUPS, q = i / GROUPS;
+                const int px = q % 34, row = q / 34;
+                uint4 v = raw[k][0];
+                if (SRC == SRC_POOL && in[k]) {
+                    float a[8], b[8], d[8], e[8], o[8];
+                    unpack8(raw[k][0], a);
+                    unpack8(raw[k][1], b);
+                    unpack8(raw[k][2], d);
+                    unpack8(raw[k][3], e);
+#pragma unroll
+                    for (int c8 = 0; c8 < 8; c8++) o[c8] = 0.25f * (((a[c8] + b[c8]) + d[c8]) + e[c8]);
+                    v = pack8(o);
+                }
+                *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * PS + g * 8) = v;
+            }
+        }
+    }
+    {
+        constexpr int WITEMS = 9 * COUT * GROUPS;
+        constexpr int WPER = (WITEMS + 255) / 256;
+        constexpr int WB = 8;
+#pragma unroll
+        for (int k0 = 0; k0 < WPER; k0 += WB) {
+            uint4 v[WB];
+#pragma unroll
+            for (int k = 0; k < WB; k++) {
+                const int i = threadIdx.x + (k0 + k) * 256;
+                if (k0 + k < WPER && i < WITEMS) v[k] = ld16(Wt + (size_t)(i / GROUPS) * CIN + (i % GROUPS) * 8);
+            }
+#pragma unroll
+            for (int k = 0; k < WB; k++) {
+                const int i = threadIdx.x + (k0 + k) * 256;
+                if (k0 + k < WPER && i < WITEMS)
+                    *reinterpret_cast<uint4 *>(wl + (i / GROUPS) * WPS + (i % GROUPS) * 8) = v[k];
+            }
+        }
     }
     __syncthreads();
     const int c = l % 32, kg = (l / 32) * 8;
@@ -147,12 +207,12 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
         for (int j = 0; j < NCO; j++) acc[r][j] = floatx16{};
 #pragma unroll
     for (int tap = 0; tap < 9; tap++) {
-        const bf16 *wp = Wt + ((size_t)tap * COUT + c) * CIN + kg;
+        const bf16 *wp = wl + (tap * COUT + c) * WPS + kg;
 #pragma unroll
         for (int c0 = 0; c0 < CIN; c0 += 16) {
             bf16x8 a[NCO], b[R];
 #pragma unroll
-            for (int j = 0; j < NCO; j++) a[j] = load8(wp + (size_t)j * 32 * CIN + c0);
+            for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * WPS + c0);
 #pragma unroll
             for (int r = 0; r < R; r++)
                 b[r] = load8(tile + ((wv * R + r + tap / 3) * 34 + c + tap % 3) * PS + kg + c0);
@@ -691,7 +751,8 @@ struct rt_unet {
         const ConvLayer &L = kLayers[k];
         const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
         const dim3 grid((W + 31) / 32, (H + rtu::kConvRows - 1) / rtu::kConvRows, N);
-        const size_t lds = (size_t)(rtu::kConvRows + 2) * 34 * (cin + 8) * 2;
+        // the input tile, then the weights [9][cout][cin + 8] (bf16)
+        const size_t lds = (size_t)(rtu::kConvRows + 2) * 34 * (cin + 8) * 2 + (size_t)9 * cout * (cin + 8) * 2;
         const rtu::bf16 *w = bwd ? d_wt[k] : d_wf[k];
         const float *b = bwd ? nullptr : d_p + b_off[k];
         // the channel pairs of the six layers, forward (cin, cout) and backward (cout, cin)
