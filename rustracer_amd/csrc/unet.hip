@@ -91,7 +91,10 @@ __device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4
 // pixel), so every weight fragment, read once per tap and input-channel step,
 // feeds kConvRowsPerWave MFMAs, and each lane ends with 4
 // consecutive output channels of one pixel per accumulator group: the
-// epilogue stores 8 bytes at a time. Grid: (ceil(W/32), ceil(H/kConvRows), N).
+// epilogue stores 8 bytes at a time. The workgroups are persistent (as many as
+// are resident at once): each stages the weights once, then takes the tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... of the ceil(W/32) x
+// ceil(H/kConvRows) x N tiles.
 //
 // The inference pass folds the pool and the upsample into the staging of the
 // layers that read them (the training pass keeps P and CAT for its backward):
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
                                                       bf16 *__restrict__ Y, int ys, int yoff, int relu,
                                                       const bf16 *__restrict__ mask, int ms, int H, int W,
                                                       const bf16 *__restrict__ X2 = nullptr, int x2s = 0,
-                                                      int c1 = 0) {
+                                                      int c1 = 0, int N = 1) {
     constexpr int R = kConvRowsPerWave;
     constexpr int PS = CIN + 8;  // LDS pixel stride (elements)
     constexpr int GROUPS = CIN / 8;
@@ -119,65 +122,8 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
     extern __shared__ __attribute__((aligned(16))) bf16 tile[];
     bf16 *wl = tile + (kConvRows + 2) * 34 * PS;
     const int l = threadIdx.x % 64, wv = threadIdx.x / 64;
-    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * kConvRows, n = blockIdx.z;
-    // Staging: each thread takes items threadIdx.x + 256 k; the loads of
-    // kStageBatch items are all issued before their values are combined and
-    // written to LDS, so their latencies overlap.
-    constexpr int ITEMS = (kConvRows + 2) * 34 * GROUPS;
-    constexpr int PER = (ITEMS + 255) / 256;
-    constexpr int BATCH = SRC == SRC_POOL ? 2 : 8;
-#pragma unroll
-    for (int k0 = 0; k0 < PER; k0 += BATCH) {
-        uint4 raw[BATCH][SRC == SRC_POOL ? 4 : 1];
-        bool in[BATCH];
-#pragma unroll
-        for (int k = 0; k < BATCH; k++) {
-            const int i = threadIdx.x + (k0 + k) * 256;
-            const int g = i % GROUPS, q = i / GROUPS;
-            const int px = q % 34, row = q / 34;
-            const int yy = y0 - 1 + row, xx = x0 - 1 + px;
-            in[k] = k0 + k < PER && i < ITEMS && yy >= 0 && yy < H && xx >= 0 && xx < W;
-            if (SRC == SRC_POOL) {
-                raw[k][0] = raw[k][1] = raw[k][2] = raw[k][3] = make_uint4(0u, 0u, 0u, 0u);
-                if (in[k]) {
-                    const size_t p00 = ((size_t)(n * 2 * H + 2 * yy) * (2 * W) + 2 * xx) * xs + g * 8;
-                    const size_t down = (size_t)(2 * W) * xs;
-                    raw[k][0] = ld16(X + p00);
-                    raw[k][1] = ld16(X + p00 + xs);
-                    raw[k][2] = ld16(X + p00 + down);
-                    raw[k][3] = ld16(X + p00 + down + xs);
-                }
-            } else {
-                raw[k][0] = make_uint4(0u, 0u, 0u, 0u);
-                if (in[k]) {
-                    if (SRC == SRC_CONCAT && g * 8 >= c1)
-                        raw[k][0] = ld16(X2 + ((size_t)(n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * x2s + (g * 8 - c1));
-                    else
-                        raw[k][0] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < BATCH; k++) {
-            const int i = threadIdx.x + (k0 + k) * 256;
-            if (k0 + k < PER && i < ITEMS) {
-                const int g = i % GROUPS, q = i / GROUPS;
-                const int px = q % 34, row = q / 34;
-                uint4 v = raw[k][0];
-                if (SRC == SRC_POOL && in[k]) {
-                    float a[8], b[8], d[8], e[8], o[8];
-                    unpack8(raw[k][0], a);
-                    unpack8(raw[k][1], b);
-                    unpack8(raw[k][2], d);
-                    unpack8(raw[k][3], e);
-#pragma unroll
-                    for (int c8 = 0; c8 < 8; c8++) o[c8] = 0.25f * (((a[c8] + b[c8]) + d[c8]) + e[c8]);
-                    v = pack8(o);
-                }
-                *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * PS + g * 8) = v;
-            }
-        }
-    }
+    const int tiles_x = (W + 31) / 32, tiles_y = (H + kConvRows - 1) / kConvRows;
+    const int total = tiles_x * tiles_y * N;
     {
         constexpr int WITEMS = 9 * COUT * GROUPS;
         constexpr int WPER = (WITEMS + 255) / 256;
@@ -198,67 +144,135 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
             }
         }
     }
-    __syncthreads();
-    const int c = l % 32, kg = (l / 32) * 8;
-    floatx16 acc[R][NCO];
+    // Tile staging: thread items threadIdx.x + 256 k. The loads of the next
+    // tile are issued into registers before the current tile's MFMAs and
+    // written to LDS after them, so their latency overlaps the compute.
+    constexpr int ITEMS = (kConvRows + 2) * 34 * GROUPS;
+    constexpr int PER = (ITEMS + 255) / 256;
+    constexpr int NL = SRC == SRC_POOL ? 4 : 1;
+    uint4 raw[PER][NL];
+    auto fetch = [&](int t) {
+        const int bx = t % tiles_x, rest = t / tiles_x;
+        const int x0 = bx * 32, y0 = (rest % tiles_y) * kConvRows, n = rest / tiles_y;
 #pragma unroll
-    for (int r = 0; r < R; r++)
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + k * 256;
+            const int g = i % GROUPS, q = i / GROUPS;
+            const int px = q % 34, row = q / 34;
+            const int yy = y0 - 1 + row, xx = x0 - 1 + px;
 #pragma unroll
-        for (int j = 0; j < NCO; j++) acc[r][j] = floatx16{};
-#pragma unroll
-    for (int tap = 0; tap < 9; tap++) {
-        const bf16 *wp = wl + (tap * COUT + c) * WPS + kg;
-#pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 16) {
-            bf16x8 a[NCO], b[R];
-#pragma unroll
-            for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * WPS + c0);
-#pragma unroll
-            for (int r = 0; r < R; r++)
-                b[r] = load8(tile + ((wv * R + r + tap / 3) * 34 + c + tap % 3) * PS + kg + c0);
-#pragma unroll
-            for (int r = 0; r < R; r++)
-#pragma unroll
-                for (int j = 0; j < NCO; j++)
-                    acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[r], acc[r][j], 0, 0, 0);
+            for (int e = 0; e < NL; e++) raw[k][e] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < ITEMS && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                if (SRC == SRC_POOL) {
+                    const size_t p00 = ((size_t)(n * 2 * H + 2 * yy) * (2 * W) + 2 * xx) * xs + g * 8;
+                    const size_t down = (size_t)(2 * W) * xs;
+                    raw[k][0] = ld16(X + p00);
+                    raw[k][NL > 1 ? 1 : 0] = ld16(X + p00 + xs);
+                    raw[k][NL > 2 ? 2 : 0] = ld16(X + p00 + down);
+                    raw[k][NL > 3 ? 3 : 0] = ld16(X + p00 + down + xs);
+                } else if (SRC == SRC_CONCAT && g * 8 >= c1) {
+                    raw[k][0] = ld16(X2 + ((size_t)(n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * x2s + (g * 8 - c1));
+                } else {
+                    raw[k][0] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
+                }
+            }
         }
-    }
-    const int x = x0 + c;
-    if (x >= W) return;
+    };
+    auto commit = [&]() {
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int y = y0 + wv * R + r;
-        if (y >= H) break;
-        const size_t p = (size_t)(n * H + y) * W + x;
+        for (int k = 0; k < PER; k++) {
+            const int i = threadIdx.x + k * 256;
+            if (i < ITEMS) {
+                const int g = i % GROUPS, q = i / GROUPS;
+                const int px = q % 34, row = q / 34;
+                uint4 v = raw[k][0];
+                if (SRC == SRC_POOL) {
+                    // zero loads (outside the image) give the zero that the pool
+                    // of a padded position is: 0.25 * (0 + 0 + 0 + 0) = 0
+                    float a[8], b[8], d[8], e[8], o[8];
+                    unpack8(raw[k][0], a);
+                    unpack8(raw[k][NL > 1 ? 1 : 0], b);
+                    unpack8(raw[k][NL > 2 ? 2 : 0], d);
+                    unpack8(raw[k][NL > 3 ? 3 : 0], e);
 #pragma unroll
-        for (int j = 0; j < NCO; j++) {
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                // accumulator elements 4g .. 4g+3: output channels co .. co+3 of pixel x
-                const int co = j * 32 + g * 8 + (l / 32) * 4;
-                float v[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) v[e] = acc[r][j][4 * g + e];
-                if (bias) {
-                    const float4 bv = *reinterpret_cast<const float4 *>(bias + co);
-                    v[0] += bv.x;
-                    v[1] += bv.y;
-                    v[2] += bv.z;
-                    v[3] += bv.w;
+                    for (int c8 = 0; c8 < 8; c8++) o[c8] = 0.25f * (((a[c8] + b[c8]) + d[c8]) + e[c8]);
+                    v = pack8(o);
                 }
-                if (relu)
+                *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * PS + g * 8) = v;
+            }
+        }
+    };
+    int t = blockIdx.x;
+    if (t < total) fetch(t);
+    for (; t < total; t += gridDim.x) {
+        __syncthreads();  // every wavefront is done with the previous tile (and the weights are in)
+        commit();
+        __syncthreads();
+        if (t + (int)gridDim.x < total) fetch(t + (int)gridDim.x);
+        const int bx = t % tiles_x, rest = t / tiles_x;
+        const int x0 = bx * 32, y0 = (rest % tiles_y) * kConvRows, n = rest / tiles_y;
+        const int c = l % 32, kg = (l / 32) * 8;
+        floatx16 acc[R][NCO];
 #pragma unroll
-                    for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.0f);
-                if (mask) {
-                    const uint2 mv = *reinterpret_cast<const uint2 *>(mask + p * ms + co);
-                    if (!(__uint_as_float(mv.x << 16) > 0.0f)) v[0] = 0.0f;
-                    if (!(__uint_as_float(mv.x & 0xFFFF0000u) > 0.0f)) v[1] = 0.0f;
-                    if (!(__uint_as_float(mv.y << 16) > 0.0f)) v[2] = 0.0f;
-                    if (!(__uint_as_float(mv.y & 0xFFFF0000u) > 0.0f)) v[3] = 0.0f;
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int j = 0; j < NCO; j++) acc[r][j] = floatx16{};
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const bf16 *wp = wl + (tap * COUT + c) * WPS + kg;
+#pragma unroll
+            for (int c0 = 0; c0 < CIN; c0 += 16) {
+                bf16x8 a[NCO], b[R];
+#pragma unroll
+                for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * WPS + c0);
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    b[r] = load8(tile + ((wv * R + r + tap / 3) * 34 + c + tap % 3) * PS + kg + c0);
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int j = 0; j < NCO; j++)
+                        acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[r], acc[r][j], 0, 0, 0);
+            }
+        }
+        const int x = x0 + c;
+        if (x < W) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int y = y0 + wv * R + r;
+                if (y >= H) break;
+                const size_t p = (size_t)(n * H + y) * W + x;
+#pragma unroll
+                for (int j = 0; j < NCO; j++) {
+#pragma unroll
+                    for (int g = 0; g < 4; g++) {
+                        // accumulator elements 4g .. 4g+3: output channels co .. co+3 of pixel x
+                        const int co = j * 32 + g * 8 + (l / 32) * 4;
+                        float v[4];
+#pragma unroll
+                        for (int e = 0; e < 4; e++) v[e] = acc[r][j][4 * g + e];
+                        if (bias) {
+                            const float4 bv = *reinterpret_cast<const float4 *>(bias + co);
+                            v[0] += bv.x;
+                            v[1] += bv.y;
+                            v[2] += bv.z;
+                            v[3] += bv.w;
+                        }
+                        if (relu)
+#pragma unroll
+                            for (int e = 0; e < 4; e++) v[e] = fmaxf(v[e], 0.0f);
+                        if (mask) {
+                            const uint2 mv = *reinterpret_cast<const uint2 *>(mask + p * ms + co);
+                            if (!(__uint_as_float(mv.x << 16) > 0.0f)) v[0] = 0.0f;
+                            if (!(__uint_as_float(mv.x & 0xFFFF0000u) > 0.0f)) v[1] = 0.0f;
+                            if (!(__uint_as_float(mv.y << 16) > 0.0f)) v[2] = 0.0f;
+                            if (!(__uint_as_float(mv.y & 0xFFFF0000u) > 0.0f)) v[3] = 0.0f;
+                        }
+                        const uint2 o = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                                   (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+                        *reinterpret_cast<uint2 *>(Y + p * ys + yoff + co) = o;
+                    }
                 }
-                const uint2 o = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
-                                           (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
-                *reinterpret_cast<uint2 *>(Y + p * ys + yoff + co) = o;
             }
         }
     }
@@ -640,6 +654,14 @@ T *dalloc(size_t count, const char *what) {
 
 unsigned blocks_for(size_t n, unsigned cap = 8192) { return (unsigned)std::min<size_t>((n + 255) / 256, cap); }
 
+// compute units of the current device
+int cu_count() {
+    int dev = 0, cus = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    return std::max(cus, 1);
+}
+
 struct ConvLayer {
     int cin = 0, cout = 0;
     size_t nw() const { return (size_t)9 * cin * cout; }
@@ -750,7 +772,7 @@ struct rt_unet {
               const rtu::bf16 *X2 = nullptr, int x2s = 0, int c1 = 0) {
         const ConvLayer &L = kLayers[k];
         const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
-        const dim3 grid((W + 31) / 32, (H + rtu::kConvRows - 1) / rtu::kConvRows, N);
+        const int tiles = ((W + 31) / 32) * ((H + rtu::kConvRows - 1) / rtu::kConvRows) * N;
         // the input tile, then the weights [9][cout][cin + 8] (bf16)
         const size_t lds = (size_t)(rtu::kConvRows + 2) * 34 * (cin + 8) * 2 + (size_t)9 * cout * (cin + 8) * 2;
         const rtu::bf16 *w = bwd ? d_wt[k] : d_wf[k];
@@ -758,17 +780,22 @@ struct rt_unet {
         // the channel pairs of the six layers, forward (cin, cout) and backward (cout, cin)
 #define RT_CONV_SRC(CI, NCO, SRC)                                                                                \
     do {                                                                                                         \
-        if (lds > 65536) { /* above the default dynamic LDS limit: raise it once per instance */              \
-            static bool raised = false;                                                                          \
-            if (!raised) {                                                                                       \
+        /* once per instance: raise the dynamic LDS limit (above 64 KB) and size the persistent grid */          \
+        static int resident = 0;                                                                                 \
+        if (resident == 0) {                                                                                     \
+            if (lds > 65536)                                                                                     \
                 hip_check(hipFuncSetAttribute((const void *)rtu::conv3x3_kernel<CI, NCO, SRC>,                   \
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),             \
                           "conv LDS size");                                                                      \
-                raised = true;                                                                                   \
-            }                                                                                                    \
+            int per_cu = 0;                                                                                      \
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtu::conv3x3_kernel<CI, NCO, SRC>,   \
+                                                                   256, lds),                                    \
+                      "conv occupancy");                                                                         \
+            resident = std::max(per_cu, 1) * cu_count();                                                         \
         }                                                                                                        \
+        const dim3 grid((unsigned)std::max(1, std::min(tiles, resident)));                                       \
         hipLaunchKernelGGL((rtu::conv3x3_kernel<CI, NCO, SRC>), grid, dim3(256), lds, stream, X, xs, w, b, Y,   \
-                           ys, yoff, relu ? 1 : 0, mask, ms, H, W, X2, x2s, c1);                                 \
+                           ys, yoff, relu ? 1 : 0, mask, ms, H, W, X2, x2s, c1, N);                              \
     } while (0)
 #define RT_CONV(CI, NCO) RT_CONV_SRC(CI, NCO, rtu::SRC_PLAIN)
         if (src == rtu::SRC_POOL) {
