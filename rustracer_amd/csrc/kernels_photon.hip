@@ -190,27 +190,118 @@ constexpr uint32_t kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kRadixBits = 8;
 constexpr uint32_t kRadix = 1u << kRadixBits;
 
-// photon_count.wgsl:18-33: u32 cell of the photon, or `invalid` when the slot is
-// empty (power.r <= 0) or the cell index is outside the grid.
+__device__ __forceinline__ uint32_t wave_reduce_add(uint32_t v) {
+#pragma unroll
+    for (uint32_t o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// photon_count.wgsl:18-33: u32 cell of the photon, or `num_cells` (invalid)
+// when the slot is empty (power.r <= 0) or the cell index is outside the grid.
+// One workgroup covers a tile of kSortTile consecutive slots, reads them in
+// kSortItems rounds of 256 and writes the key of every slot plus the tile's
+// count of valid slots. No atomics: the per-cell counts come from the sorted
+// keys afterwards, so hot cells (many photons in one cell) cost nothing extra.
 __global__ __launch_bounds__(256) void grid_keys_kernel(const rt_photon *photons, uint32_t n_slots, rt_grid_params gp,
-                                                         uint32_t *keys, uint32_t *vals, uint32_t *counts) {
-    const uint32_t slot = blockIdx.x * 256u + threadIdx.x;
-    if (slot >= n_slots) return;
-    const float4 pos = *reinterpret_cast<const float4 *>(&photons[slot].position[0]);
-    const float pr = photons[slot].power[0];
-    uint32_t key = gp.num_cells;
-    if (pr > 0.0f) {
-        const uint32_t gx = wgsl_u32((pos.x - gp.room_min_x) / gp.cell_size);
-        const uint32_t gy = wgsl_u32((pos.y - gp.room_min_y) / gp.cell_size);
-        const uint32_t gz = wgsl_u32((pos.z - gp.room_min_z) / gp.cell_size);
-        const uint32_t cell = gx + gy * gp.grid_x + gz * gp.grid_x * gp.grid_y;
-        if (cell < gp.num_cells) {
-            key = cell;
-            atomicAdd(&counts[cell], 1u);
+                                                         uint32_t *keys, uint32_t *tile_counts) {
+    __shared__ uint32_t lds_wave[4];
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    float4 pos[kSortItems];
+    float pr[kSortItems];
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        const size_t slot = base + (size_t)r * kSortThreads + threadIdx.x;
+        pos[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pr[r] = 0.0f;
+        if (slot < n_slots) {
+            pos[r] = *reinterpret_cast<const float4 *>(&photons[slot].position[0]);
+            pr[r] = photons[slot].power[0];
         }
     }
-    keys[slot] = key;
-    vals[slot] = slot;
+    uint32_t valid = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        const size_t slot = base + (size_t)r * kSortThreads + threadIdx.x;
+        uint32_t key = gp.num_cells;
+        if (pr[r] > 0.0f) {
+            const uint32_t gx = wgsl_u32((pos[r].x - gp.room_min_x) / gp.cell_size);
+            const uint32_t gy = wgsl_u32((pos[r].y - gp.room_min_y) / gp.cell_size);
+            const uint32_t gz = wgsl_u32((pos[r].z - gp.room_min_z) / gp.cell_size);
+            const uint32_t cell = gx + gy * gp.grid_x + gz * gp.grid_x * gp.grid_y;
+            if (cell < gp.num_cells) key = cell;
+        }
+        if (slot < n_slots) {
+            keys[slot] = key;
+            valid += key < gp.num_cells ? 1u : 0u;
+        }
+    }
+    const uint32_t wave_sum = wave_reduce_add(valid);
+    if ((threadIdx.x & 63u) == 0u) lds_wave[threadIdx.x >> 6] = wave_sum;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = lds_wave[0] + lds_wave[1] + lds_wave[2] + lds_wave[3];
+}
+
+// Stream compaction that keeps slot order: the valid (key, slot) pairs of each
+// tile go to tile_offsets[tile] + their rank among the tile's valid slots.
+__global__ __launch_bounds__(256) void grid_compact_kernel(const uint32_t *keys, uint32_t n_slots, uint32_t num_cells,
+                                                            const uint32_t *tile_offsets, uint32_t *keys_out,
+                                                            uint32_t *vals_out) {
+    __shared__ uint32_t wave_tot[kSortItems][4];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    uint32_t key[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        const size_t i = base + (size_t)r * kSortThreads + threadIdx.x;
+        key[r] = i < n_slots ? keys[i] : num_cells;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        const uint64_t mask = __ballot(key[r] < num_cells);
+        rank[r] = (uint32_t)__popcll(mask & lt_mask);
+        if (lane == 0u) wave_tot[r][wave] = (uint32_t)__popcll(mask);
+    }
+    __syncthreads();
+    uint32_t run = tile_offsets[blockIdx.x];
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        uint32_t off = run;
+        for (uint32_t w = 0; w < wave; w++) off += wave_tot[r][w];
+        if (key[r] < num_cells) {
+            keys_out[off + rank[r]] = key[r];
+            vals_out[off + rank[r]] = (uint32_t)(base + (size_t)r * kSortThreads + threadIdx.x);
+        }
+        run += wave_tot[r][0] + wave_tot[r][1] + wave_tot[r][2] + wave_tot[r][3];
+    }
+}
+
+// grid_meta[c] = [start, count] (the reference's exclusive scan of the cell
+// counts, lib.rs:987-998) read off the sorted keys: start = number of keys
+// below c, count = number of keys equal to c. Cells past num_cells get
+// [n_valid, 0], as the scan gives them.
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+    uint32_t lo = 0, len = n;
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        if (a[lo + half] < v) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void grid_meta_kernel(const uint32_t *sorted_keys, const uint32_t *n_valid,
+                                                         uint32_t meta_cells, uint2 *meta) {
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= meta_cells) return;
+    const uint32_t n = *n_valid;
+    const uint32_t lo = lower_bound_u32(sorted_keys, n, c);
+    const uint32_t hi = lo + lower_bound_u32(sorted_keys + lo, n - lo, c + 1u);
+    meta[c] = make_uint2(lo, hi - lo);
 }
 
 // Block-wide exclusive scan of one value per thread (256 threads = 4 waves).
@@ -297,9 +388,10 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(const uint32_t *in, uin
 
 // Radix pass, step 1: digit histogram of each tile, stored digit-major
 // (hist[d * n_tiles + tile]) so one exclusive scan gives every (digit, tile) base.
-__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *keys, uint32_t n, uint32_t shift,
+__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *keys, const uint32_t *n_dev, uint32_t shift,
                                                          uint32_t n_tiles, uint32_t *hist) {
     __shared__ uint32_t h[kRadix];
+    const uint32_t n = *n_dev;
     h[threadIdx.x] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kSortTile;
@@ -316,14 +408,16 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *keys, u
 // equal digits with 8 ballots (bit-by-bit match), wavefronts are ordered
 // through LDS, and a running per-digit offset carries the order across rounds.
 __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *keys_in, const uint32_t *vals_in,
-                                                            uint32_t n, uint32_t shift, uint32_t n_tiles,
+                                                            const uint32_t *n_dev, uint32_t shift, uint32_t n_tiles,
                                                             const uint32_t *hist_scanned, uint32_t *keys_out,
                                                             uint32_t *vals_out) {
     __shared__ uint32_t digit_base[kRadix];        // global base + items already placed
     __shared__ uint32_t wave_counts[4][kRadix];    // per-round, per-wave digit counts
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    digit_base[threadIdx.x] = hist_scanned[(size_t)threadIdx.x * n_tiles + blockIdx.x];
+    const uint32_t n = *n_dev;
     const size_t base = (size_t)blockIdx.x * kSortTile;
+    if (base >= n) return;  // tile past the valid count: nothing to place (uniform per workgroup)
+    digit_base[threadIdx.x] = hist_scanned[(size_t)threadIdx.x * n_tiles + blockIdx.x];
     const uint64_t lt_mask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
     for (uint32_t r = 0; r < kSortItems; r++) {
         for (uint32_t w = 0; w < 4u; w++) wave_counts[w][threadIdx.x] = 0u;
@@ -424,27 +518,31 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static uint32_t tiles_of(size_t n) { return (uint32_t)((n + rtd::kSortTile - 1) / rtd::kSortTile); }
 
 size_t grid_build_temp_bytes(uint32_t n_slots, uint32_t meta_cells) {
+    (void)meta_cells;
     const size_t n = n_slots;
     const uint32_t t = std::max(1u, tiles_of(n));
-    const uint32_t tc = std::max(1u, tiles_of(meta_cells));
     const size_t hist = (size_t)rtd::kRadix * t;
     const uint32_t th = std::max(1u, tiles_of(hist));
-    return 4 * align256(n * 4) +          // keys/vals x 2
-           align256((size_t)meta_cells * 4) +  // counts
-           align256(hist * 4) +           // histograms
-           align256((size_t)std::max(tc, th) * 4) + 256;  // tile sums + total
+    return 4 * align256(n * 4) +                       // keys/vals x 2
+           align256(hist * 4) +                        // histograms
+           align256((size_t)t * 4) +                   // valid slots per tile
+           align256((size_t)std::max(t, th) * 4) + 256;  // scan tile sums
 }
 
-// Exclusive scan of `in[0..n)` into out (or into meta pairs).
-static hipError_t scan_u32(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *meta, uint32_t *tile_sums,
-                           uint32_t *total, hipStream_t s) {
+// Exclusive scan of `in[0..n)` into out.
+static hipError_t scan_u32(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *tile_sums, hipStream_t s) {
     const uint32_t tiles = std::max(1u, tiles_of(n));
     hipLaunchKernelGGL(rtd::scan_reduce_kernel, dim3(tiles), dim3(256), 0, s, in, n, tile_sums);
-    hipLaunchKernelGGL(rtd::scan_tiles_kernel, dim3(1), dim3(256), 0, s, tile_sums, tiles, total);
-    hipLaunchKernelGGL(rtd::scan_apply_kernel, dim3(tiles), dim3(256), 0, s, in, n, tile_sums, out, meta);
+    hipLaunchKernelGGL(rtd::scan_tiles_kernel, dim3(1), dim3(256), 0, s, tile_sums, tiles, nullptr);
+    hipLaunchKernelGGL(rtd::scan_apply_kernel, dim3(tiles), dim3(256), 0, s, in, n, tile_sums, out, nullptr);
     return hipGetLastError();
 }
 
+// photon_count + CPU scan + photon_scatter (photon_count.wgsl, lib.rs:987-998,
+// photon_scatter.wgsl) as one device pipeline: keys of every slot, compaction
+// of the valid slots in slot order, a stable LSD radix sort of those pairs
+// (so photons of one cell stay in ascending slot order), grid_meta from the
+// sorted keys, then the record copy in sorted order.
 hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
     const uint32_t n = g.n_slots;
     const uint32_t tiles = std::max(1u, tiles_of(n));
@@ -454,34 +552,45 @@ hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
     uint32_t *vals_a = reinterpret_cast<uint32_t *>(p); p += align256((size_t)n * 4);
     uint32_t *keys_b = reinterpret_cast<uint32_t *>(p); p += align256((size_t)n * 4);
     uint32_t *vals_b = reinterpret_cast<uint32_t *>(p); p += align256((size_t)n * 4);
-    uint32_t *counts = reinterpret_cast<uint32_t *>(p); p += align256((size_t)g.meta_cells * 4);
     uint32_t *hist = reinterpret_cast<uint32_t *>(p); p += align256(hist_n * 4);
+    uint32_t *tile_counts = reinterpret_cast<uint32_t *>(p); p += align256((size_t)tiles * 4);
     uint32_t *tile_sums = reinterpret_cast<uint32_t *>(p);
     hipError_t e;
-    if ((e = launch_fill_u32(counts, 0u, g.meta_cells, s)) != hipSuccess) return e;
-    if (n > 0) {
-        hipLaunchKernelGGL(rtd::grid_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g.photons, n, g.gp, keys_a,
-                           vals_a, counts);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+    const unsigned meta_blocks = (unsigned)((g.meta_cells + 255u) / 256u);
+    if (n == 0) {
+        if ((e = launch_fill_u32(g.n_valid_out, 0u, 1, s)) != hipSuccess) return e;
+        if (meta_blocks)
+            hipLaunchKernelGGL(rtd::grid_meta_kernel, dim3(meta_blocks), dim3(256), 0, s, keys_a, g.n_valid_out,
+                               g.meta_cells, reinterpret_cast<uint2 *>(g.grid_meta));
+        return hipGetLastError();
     }
-    // grid_meta = exclusive scan of the counts; n_valid = their total
-    if ((e = scan_u32(counts, g.meta_cells, nullptr, g.grid_meta, tile_sums, g.n_valid_out, s)) != hipSuccess) return e;
-    if (n == 0) return hipSuccess;
-    // key bits needed for values 0..num_cells (num_cells = invalid)
+    // keys of all slots (into keys_b, free until the first radix pass) and valid counts per tile
+    hipLaunchKernelGGL(rtd::grid_keys_kernel, dim3(tiles), dim3(256), 0, s, g.photons, n, g.gp, keys_b, tile_counts);
+    // tile offsets of the compacted list; n_valid = their total
+    hipLaunchKernelGGL(rtd::scan_tiles_kernel, dim3(1), dim3(256), 0, s, tile_counts, tiles, g.n_valid_out);
+    hipLaunchKernelGGL(rtd::grid_compact_kernel, dim3(tiles), dim3(256), 0, s, keys_b, n, g.gp.num_cells, tile_counts,
+                       keys_a, vals_a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // key bits needed for the valid keys 0..num_cells-1
     uint32_t bits = 0;
-    while (bits < 32 && (1ull << bits) <= (uint64_t)g.gp.num_cells) bits++;
+    const uint64_t max_key = g.gp.num_cells > 0 ? (uint64_t)g.gp.num_cells - 1 : 0;
+    while (bits < 32 && (1ull << bits) <= max_key) bits++;
     const uint32_t passes = std::max(1u, (bits + rtd::kRadixBits - 1) / rtd::kRadixBits);
     uint32_t *kin = keys_a, *vin = vals_a, *kout = keys_b, *vout = vals_b;
     for (uint32_t pass = 0; pass < passes; pass++) {
         const uint32_t shift = pass * rtd::kRadixBits;
-        hipLaunchKernelGGL(rtd::radix_hist_kernel, dim3(tiles), dim3(256), 0, s, kin, n, shift, tiles, hist);
-        if ((e = scan_u32(hist, (uint32_t)hist_n, hist, nullptr, tile_sums, nullptr, s)) != hipSuccess) return e;
-        hipLaunchKernelGGL(rtd::radix_scatter_kernel, dim3(tiles), dim3(256), 0, s, kin, vin, n, shift, tiles, hist,
-                           kout, vout);
+        hipLaunchKernelGGL(rtd::radix_hist_kernel, dim3(tiles), dim3(256), 0, s, kin, g.n_valid_out, shift, tiles,
+                           hist);
+        if ((e = scan_u32(hist, (uint32_t)hist_n, hist, tile_sums, s)) != hipSuccess) return e;
+        hipLaunchKernelGGL(rtd::radix_scatter_kernel, dim3(tiles), dim3(256), 0, s, kin, vin, g.n_valid_out, shift,
+                           tiles, hist, kout, vout);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
+    if (meta_blocks)
+        hipLaunchKernelGGL(rtd::grid_meta_kernel, dim3(meta_blocks), dim3(256), 0, s, kin, g.n_valid_out, g.meta_cells,
+                           reinterpret_cast<uint2 *>(g.grid_meta));
     const size_t threads = (size_t)n * 4;
     hipLaunchKernelGGL(rtd::grid_gather_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g.photons,
                        vin, g.n_valid_out, g.sorted, reinterpret_cast<float4 *>(g.gather_rec));
