@@ -114,11 +114,12 @@ struct PhotonLaunch {
     uint32_t max_depth;  // slots per photon
     uint32_t mode;
     rt_photon *photons;
+    uint32_t *keys = nullptr;  // n_photons * max_depth cell keys, pre-filled with num_cells; may be null
     rt_tex_array tex[RT_TEX_ROLES];
 };
 hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s);
 
-// Grid build: keys + counts, scan, stable radix sort of (key, slot), gather.
+// Grid build: keys, compaction of the valid slots, stable radix sort of (key, slot), grid_meta, gather.
 struct GridBuildLaunch {
     const rt_photon *photons;
     uint32_t n_slots;      // slots processed
@@ -131,6 +132,7 @@ struct GridBuildLaunch {
     void *temp;
     size_t temp_bytes;
     uint32_t *n_valid_out; // device u32
+    const uint32_t *keys_in = nullptr;  // cell key per slot from the photon trace; null: computed from photons
 };
 size_t grid_build_temp_bytes(uint32_t n_slots, uint32_t meta_cells);
 hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s);

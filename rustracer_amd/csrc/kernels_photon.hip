@@ -6,12 +6,15 @@
 // and scatters with an atomic cursor, so photons inside a cell land in arrival
 // order. Here the whole build stays on the GPU and is stable (photons in a
 // cell keep ascending slot order):
-//   1. keys:    cell key per slot (invalid -> num_cells) + per-cell counts
-//   2. scan:    exclusive scan of the counts -> grid_meta [start, count]
-//   3. sort:    LSD radix sort of (key, slot), 8-bit digits; each pass builds
+//   1. keys:    cell key per slot (invalid -> num_cells), written by the photon
+//               trace as it stores each record (or computed from an uploaded
+//               photon array), and the count of valid slots per 2048-slot tile
+//   2. compact: the valid (key, slot) pairs in slot order
+//   3. sort:    LSD radix sort of those pairs, 8-bit digits; each pass builds
 //               per-workgroup digit histograms in LDS, scans them, and ranks
 //               equal digits inside a wavefront with ballots
-//   4. gather:  sorted_photons[i] = photons[slot of i]
+//   4. meta:    grid_meta [start, count] per cell by binary search of the keys
+//   5. gather:  sorted_photons[i] = photons[slot of i]
 #include "device_common.h"
 #include "kernels.h"
 
@@ -30,8 +33,22 @@ struct PhotonArgs {
     uint32_t n_photons;
     uint32_t max_depth;
     rt_photon *photons;
+    uint32_t *keys;  // cell key per slot, or null
     rt_tex_array tex[RT_TEX_ROLES];
 };
+
+// photon_count.wgsl:18-33: the u32 cell of a stored photon, or num_cells when
+// the slot counts as empty (power.r <= 0) or the cell is outside the grid.
+// u32(f32) saturates, so end-of-axis photons alias into the next row's x = 0.
+__device__ __forceinline__ uint32_t photon_cell_key(float x, float y, float z, float power_r,
+                                                    const rt_grid_params &gp) {
+    if (!(power_r > 0.0f)) return gp.num_cells;
+    const uint32_t gx = wgsl_u32((x - gp.room_min_x) / gp.cell_size);
+    const uint32_t gy = wgsl_u32((y - gp.room_min_y) / gp.cell_size);
+    const uint32_t gz = wgsl_u32((z - gp.room_min_z) / gp.cell_size);
+    const uint32_t cell = gx + gy * gp.grid_x + gz * gp.grid_x * gp.grid_y;
+    return cell < gp.num_cells ? cell : gp.num_cells;
+}
 
 // photon_trace.wgsl:91-97
 __device__ __forceinline__ F3 bounce_dir(F3 rd, F3 n, float rough, Rng &rng) {
@@ -152,11 +169,14 @@ __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
             }
             continue;
         }
-        float4 *rec = reinterpret_cast<float4 *>(A.photons + (size_t)photon_index * A.max_depth + bounce);
+        const size_t slot = (size_t)photon_index * A.max_depth + bounce;
+        float4 *rec = reinterpret_cast<float4 *>(A.photons + slot);
         rec[0] = make_float4(hit_pos.x, hit_pos.y, hit_pos.z, 0.0f);
         rec[1] = make_float4(power.x, power.y, power.z, 0.0f);
         rec[2] = make_float4(-dir.x, -dir.y, -dir.z, 0.0f);
         rec[3] = make_float4(shn.x, shn.y, shn.z, 0.0f);
+        // the grid key of the record just stored, from the same floats
+        if (A.keys) A.keys[slot] = photon_cell_key(hit_pos.x, hit_pos.y, hit_pos.z, power.x, A.gp);
 
         // photon_trace.wgsl:84-89 surface_albedo: the texture alone when present
         const F3 albedo = mat.tex_albedo != RT_TEX_NONE ? rgb(tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], bb.uv, mat.tex_albedo))
@@ -196,9 +216,7 @@ __device__ __forceinline__ uint32_t wave_reduce_add(uint32_t v) {
     return v;
 }
 
-// photon_count.wgsl:18-33: u32 cell of the photon, or `num_cells` (invalid)
-// when the slot is empty (power.r <= 0) or the cell index is outside the grid.
-// One workgroup covers a tile of kSortTile consecutive slots, reads them in
+// Keys of an uploaded photon array (photon_count.wgsl:18-33). One workgroup covers a tile of kSortTile consecutive slots, reads them in
 // kSortItems rounds of 256 and writes the key of every slot plus the tile's
 // count of valid slots. No atomics: the per-cell counts come from the sorted
 // keys afterwards, so hot cells (many photons in one cell) cost nothing extra.
@@ -222,18 +240,28 @@ __global__ __launch_bounds__(256) void grid_keys_kernel(const rt_photon *photons
 #pragma unroll
     for (uint32_t r = 0; r < kSortItems; r++) {
         const size_t slot = base + (size_t)r * kSortThreads + threadIdx.x;
-        uint32_t key = gp.num_cells;
-        if (pr[r] > 0.0f) {
-            const uint32_t gx = wgsl_u32((pos[r].x - gp.room_min_x) / gp.cell_size);
-            const uint32_t gy = wgsl_u32((pos[r].y - gp.room_min_y) / gp.cell_size);
-            const uint32_t gz = wgsl_u32((pos[r].z - gp.room_min_z) / gp.cell_size);
-            const uint32_t cell = gx + gy * gp.grid_x + gz * gp.grid_x * gp.grid_y;
-            if (cell < gp.num_cells) key = cell;
-        }
+        const uint32_t key = photon_cell_key(pos[r].x, pos[r].y, pos[r].z, pr[r], gp);
         if (slot < n_slots) {
             keys[slot] = key;
             valid += key < gp.num_cells ? 1u : 0u;
         }
+    }
+    const uint32_t wave_sum = wave_reduce_add(valid);
+    if ((threadIdx.x & 63u) == 0u) lds_wave[threadIdx.x >> 6] = wave_sum;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = lds_wave[0] + lds_wave[1] + lds_wave[2] + lds_wave[3];
+}
+
+// Valid slots per tile of keys written by the photon trace.
+__global__ __launch_bounds__(256) void grid_tile_count_kernel(const uint32_t *keys, uint32_t n_slots,
+                                                               uint32_t num_cells, uint32_t *tile_counts) {
+    __shared__ uint32_t lds_wave[4];
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    uint32_t valid = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kSortItems; r++) {
+        const size_t i = base + (size_t)r * kSortThreads + threadIdx.x;
+        if (i < n_slots) valid += keys[i] < num_cells ? 1u : 0u;
     }
     const uint32_t wave_sum = wave_reduce_add(valid);
     if ((threadIdx.x & 63u) == 0u) lds_wave[threadIdx.x >> 6] = wave_sum;
@@ -499,6 +527,7 @@ hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s) {
     A.n_photons = p.n_photons;
     A.max_depth = p.max_depth;
     A.photons = p.photons;
+    A.keys = p.keys;
     for (uint32_t k = 0; k < RT_TEX_ROLES; k++) A.tex[k] = p.tex[k];
     const dim3 grid((p.n_photons + 63u) / 64u), block(64);
     if (p.mode == 0) hipLaunchKernelGGL((rtd::photon_trace_kernel<0>), grid, block, 0, s, A);
@@ -564,11 +593,20 @@ hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
                                g.meta_cells, reinterpret_cast<uint2 *>(g.grid_meta));
         return hipGetLastError();
     }
-    // keys of all slots (into keys_b, free until the first radix pass) and valid counts per tile
-    hipLaunchKernelGGL(rtd::grid_keys_kernel, dim3(tiles), dim3(256), 0, s, g.photons, n, g.gp, keys_b, tile_counts);
+    // keys of all slots (from the trace, or computed into keys_b, which is
+    // free until the first radix pass) and the valid slots per tile
+    const uint32_t *keys = g.keys_in;
+    if (keys) {
+        hipLaunchKernelGGL(rtd::grid_tile_count_kernel, dim3(tiles), dim3(256), 0, s, keys, n, g.gp.num_cells,
+                           tile_counts);
+    } else {
+        hipLaunchKernelGGL(rtd::grid_keys_kernel, dim3(tiles), dim3(256), 0, s, g.photons, n, g.gp, keys_b,
+                           tile_counts);
+        keys = keys_b;
+    }
     // tile offsets of the compacted list; n_valid = their total
     hipLaunchKernelGGL(rtd::scan_tiles_kernel, dim3(1), dim3(256), 0, s, tile_counts, tiles, g.n_valid_out);
-    hipLaunchKernelGGL(rtd::grid_compact_kernel, dim3(tiles), dim3(256), 0, s, keys_b, n, g.gp.num_cells, tile_counts,
+    hipLaunchKernelGGL(rtd::grid_compact_kernel, dim3(tiles), dim3(256), 0, s, keys, n, g.gp.num_cells, tile_counts,
                        keys_a, vals_a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // key bits needed for the valid keys 0..num_cells-1

@@ -100,6 +100,7 @@ struct rt_renderer {
     uint32_t meta_cells = 0;
     uint32_t photon_slots = 0;
     rt_photon *d_photons = nullptr, *d_sorted = nullptr;
+    uint32_t *d_keys = nullptr;     // grid cell key per photon slot, written by the photon trace
     float *d_gather_rec = nullptr;  // 3 float4 per sorted photon (radius-gather record)
     uint32_t *d_meta = nullptr;
     void *d_temp = nullptr;
@@ -198,7 +199,7 @@ struct rt_renderer {
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_gather_rec, d_bframe, d_bmom, d_bgb,
                         d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo, d_fstate,
-                        d_tex[0], d_tex[1], d_tex[2], d_tex[3]};
+                        d_tex[0], d_tex[1], d_tex[2], d_tex[3], d_keys};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -613,8 +614,9 @@ struct rt_renderer {
         return (uint32_t)std::min<uint64_t>(d, photon_slots);
     }
 
-    void grid_build(uint32_t n_slots) {
+    void grid_build(uint32_t n_slots, const uint32_t *keys_in) {
         rtk::GridBuildLaunch g;
+        g.keys_in = keys_in;
         g.photons = d_photons;
         g.n_slots = n_slots;
         g.gp = gp;
@@ -639,6 +641,7 @@ struct rt_renderer {
     void rebuild_photons() {
         if (!cfg.enable_photon_map) throw ApiError(RT_ERR_INVALID, "photon map disabled in the config");
         hip_check(hipMemsetAsync(d_photons, 0, (size_t)photon_slots * sizeof(rt_photon), stream), "clear photons");
+        hip_check(rtk::launch_fill_u32(d_keys, gp.num_cells, photon_slots, stream), "clear photon keys");
         rtk::PhotonLaunch p;
         p.trace = trace_inputs();
         p.tris = d_tris;
@@ -652,11 +655,12 @@ struct rt_renderer {
         for (uint32_t k = 0; k < RT_TEX_ROLES; k++) p.tex[k] = tex[k];
         p.mode = traversal_mode();
         p.photons = d_photons;
+        p.keys = d_keys;
         hipEvent_t ev;
         begin(RT_PASS_PHOTON_TRACE, ev);
         hip_check(rtk::launch_photon_trace(p, stream), "photon trace");
         end(RT_PASS_PHOTON_TRACE, ev);
-        grid_build(processed_slots());
+        grid_build(processed_slots(), d_keys);
     }
 };
 
@@ -1007,6 +1011,7 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->photon_slots = r->gp.photon_slots;
     r->d_photons = dev_alloc<rt_photon>(r->photon_slots, "photons");
     r->d_sorted = dev_alloc<rt_photon>(r->photon_slots, "sorted photons");
+    r->d_keys = dev_alloc<uint32_t>(r->photon_slots, "photon keys");
     r->d_gather_rec = dev_alloc<float>((size_t)r->photon_slots * 12, "photon gather records");
     hip_check(hipMemset(r->d_gather_rec, 0, 48ull * r->photon_slots), "memset");
     r->d_meta = dev_alloc<uint32_t>((size_t)r->meta_cells * 2, "grid meta");
@@ -1401,7 +1406,7 @@ int rt_build_photon_grid_from(rt_renderer *r, const rt_photon *photons, uint32_t
         hip_check(hipMemset(r->d_photons, 0, sizeof(rt_photon) * r->photon_slots), "clear photons");
         if (n_slots)
             hip_check(hipMemcpy(r->d_photons, photons, sizeof(rt_photon) * n_slots, hipMemcpyHostToDevice), "upload");
-        r->grid_build(n_slots);
+        r->grid_build(n_slots, nullptr);
     });
 }
 
