@@ -484,22 +484,24 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *keys
 __global__ __launch_bounds__(256) void grid_gather_kernel(const rt_photon *photons, const uint32_t *vals,
                                                           const uint32_t *n_valid, rt_photon *sorted,
                                                           float4 *gather_rec) {
-    // 4 lanes per photon, one float4 each
-    const size_t t = (size_t)blockIdx.x * 256u + threadIdx.x;
-    const size_t i = t >> 2;
-    const uint32_t part = (uint32_t)(t & 3u);
-    if (i >= *n_valid) return;
-    const float4 *src = reinterpret_cast<const float4 *>(photons + vals[i]);
-    float4 *dst = reinterpret_cast<float4 *>(sorted + i);
-    const float4 v = src[part];
-    dst[part] = v;
-    if (part == 0u) {
-        gather_rec[3 * i] = make_float4(v.x, v.y, v.z, src[1].x);
-    } else if (part == 1u) {
-        const float4 inc = src[2];
-        gather_rec[3 * i + 1] = make_float4(v.y, v.z, inc.x, inc.y);
-    } else if (part == 2u) {
-        gather_rec[3 * i + 2] = make_float4(v.z, 0.0f, 0.0f, 0.0f);
+    // 4 lanes per photon, one float4 each; a grid-stride loop over the stored
+    // photons only (the launch is sized for the slot count, which is far larger)
+    const size_t total = (size_t)*n_valid * 4u;
+    for (size_t t = (size_t)blockIdx.x * 256u + threadIdx.x; t < total; t += (size_t)gridDim.x * 256u) {
+        const size_t i = t >> 2;
+        const uint32_t part = (uint32_t)(t & 3u);
+        const float4 *src = reinterpret_cast<const float4 *>(photons + vals[i]);
+        float4 *dst = reinterpret_cast<float4 *>(sorted + i);
+        const float4 v = src[part];
+        dst[part] = v;
+        if (part == 0u) {
+            gather_rec[3 * i] = make_float4(v.x, v.y, v.z, src[1].x);
+        } else if (part == 1u) {
+            const float4 inc = src[2];
+            gather_rec[3 * i + 1] = make_float4(v.y, v.z, inc.x, inc.y);
+        } else if (part == 2u) {
+            gather_rec[3 * i + 2] = make_float4(v.z, 0.0f, 0.0f, 0.0f);
+        }
     }
 }
 
@@ -630,7 +632,8 @@ hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
         hipLaunchKernelGGL(rtd::grid_meta_kernel, dim3(meta_blocks), dim3(256), 0, s, kin, g.n_valid_out, g.meta_cells,
                            reinterpret_cast<uint2 *>(g.grid_meta));
     const size_t threads = (size_t)n * 4;
-    hipLaunchKernelGGL(rtd::grid_gather_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, g.photons,
+    const unsigned gather_blocks = (unsigned)std::min<size_t>((threads + 255) / 256, 2048);
+    hipLaunchKernelGGL(rtd::grid_gather_kernel, dim3(gather_blocks), dim3(256), 0, s, g.photons,
                        vin, g.n_valid_out, g.sorted, reinterpret_cast<float4 *>(g.gather_rec));
     return hipGetLastError();
 }
