@@ -159,6 +159,34 @@ def test_photon_trace_and_grid_bit_exact(cell):
     _check_equal("sorted photons", srt.view(np.float32), ref_sorted.view(np.float32))
 
 
+def test_traced_keys_equal_keys_from_records():
+    """The photon trace writes each stored photon's cell key as it stores the
+    record; the grid built from those keys must equal the grid built again from
+    the read-back photon array (keys computed from the records). 300k photons,
+    1.8M slots, on the 0.05 grid; then an array with no valid photon and an
+    empty array (the build's zero paths)."""
+    scene, cfg = _photon_setup(300_000, 0.05)
+    print("traced keys vs record keys, 300k photons", flush=True)
+    with _renderer(scene, **cfg) as r:
+        r.rebuild_photons()
+        meta_t, srt_t = r.read_grid_meta(), r.read_sorted_photons()
+        ph = r.read_photons()
+        r.build_photon_grid_from(ph)
+        meta_u, srt_u = r.read_grid_meta(), r.read_sorted_photons()
+        assert len(srt_t) > 10_000
+        assert np.array_equal(meta_t, meta_u), first_diff(meta_t, meta_u)
+        _check_equal("sorted photons", srt_t.view(np.float32), srt_u.view(np.float32))
+        dark = ph.copy()
+        dark["power"][:, 0] = 0.0
+        r.build_photon_grid_from(dark)
+        meta_d = r.read_grid_meta()
+        assert len(r.read_sorted_photons()) == 0
+        assert not meta_d.any()
+        r.build_photon_grid_from(ph[:0])
+        assert len(r.read_sorted_photons()) == 0
+        assert not r.read_grid_meta().any()
+
+
 def test_grid_build_large_matches_stable_sort():
     """1.5M synthetic photon slots (a quarter empty) on a 0.05 grid: offsets and
     order equal numpy's stable argsort of the cell keys."""
