@@ -35,7 +35,9 @@ METRIC = "Msamples/s @1080p; ms to relMSE≤0.01: batched+learned vs plain à-tr
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
-# Bytes per unit of work of the traversal layouts (roofline section).
+# Algorithmic bytes per unit of work of the traversal layouts: a frame's bytes are
+# 32 per child box tested + 48 per triangle tested + 32 per ray query (ray + hit)
+# + 16 per photon candidate + 48 of outputs per pixel.
 NODE_BOX_BYTES = 32    # one child box: min/max (binary record 64 B / 2, 4-wide record 128 B / 4)
 TRI_BYTES = 48         # slot triangle: v0, v1-v0, v2-v0 (3 x float4)
 RAY_BYTES = 32         # 24 B ray + 8 B hit
@@ -96,7 +98,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, not built)", file=sys.stderr)
+              "(sharding pixels, views or photon indices across GPUs is designed, not built)", file=sys.stderr)
         return 2
 
     import numpy as np
