@@ -13,6 +13,7 @@
 // Camera::frame_scene then zoom/orbit). The result is the accumulation buffer,
 // saved as a PNG with the CPU tonemap of examples/common/mod.rs:37-47 or, with
 // raw_save, as raw little-endian f32 RGBA in <out>.bin.
+#include <cctype>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -96,6 +97,58 @@ bool write_png(const char *path, const uint8_t *rgba, uint32_t w, uint32_t h) {
 
 bool is_one(int argc, char **argv, int i) { return argc > i && std::strcmp(argv[i], "1") == 0; }
 
+// Rust's `str::parse::<u32>()`: an optional '+', then decimal digits only, no
+// whitespace, at most u32::MAX. Anything else fails, and the argument keeps its
+// default (render_headless.rs:21-24 `.parse().ok().unwrap_or(default)`).
+bool parse_u32(const char *s, uint32_t *out) {
+    if (*s == '+') s++;
+    if (!*s) return false;
+    uint64_t v = 0;
+    for (; *s; s++) {
+        if (*s < '0' || *s > '9') return false;
+        v = v * 10 + (uint64_t)(*s - '0');
+        if (v > 0xFFFFFFFFull) return false;
+    }
+    *out = (uint32_t)v;
+    return true;
+}
+
+bool ieq(const char *a, const char *b) {
+    for (; *a && *b; a++, b++)
+        if (std::tolower((unsigned char)*a) != *b) return false;
+    return *a == *b;
+}
+
+// Rust's `str::parse::<f32>()` grammar: [+-] ( inf | infinity | nan | Number ),
+// Number = ( digits | digits '.' digits? | '.' digits ) ( [eE] [+-]? digits )?,
+// case-insensitive keywords, no whitespace, no hex. A string of that form is
+// then converted by strtof, which rounds to nearest as Rust does.
+bool parse_f32(const char *s, float *out) {
+    const char *p = s;
+    if (*p == '+' || *p == '-') p++;
+    if (ieq(p, "inf") || ieq(p, "infinity") || ieq(p, "nan")) {
+        *out = std::strtof(s, nullptr);
+        return true;
+    }
+    int int_digits = 0, frac_digits = 0;
+    while (*p >= '0' && *p <= '9') p++, int_digits++;
+    if (*p == '.') {
+        p++;
+        while (*p >= '0' && *p <= '9') p++, frac_digits++;
+    }
+    if (int_digits + frac_digits == 0) return false;
+    if (*p == 'e' || *p == 'E') {
+        p++;
+        if (*p == '+' || *p == '-') p++;
+        int exp_digits = 0;
+        while (*p >= '0' && *p <= '9') p++, exp_digits++;
+        if (exp_digits == 0) return false;
+    }
+    if (*p) return false;
+    *out = std::strtof(s, nullptr);
+    return true;
+}
+
 int fail(const char *what) {
     std::fprintf(stderr, "render_headless: %s: %s\n", what, rt_last_error());
     return 1;
@@ -109,8 +162,14 @@ int main(int argc, char **argv) {
                              "[spp] [accum] [w] [h] [debug_view] [zoom] [yaw] [pitch] [raw_save]\n");
         return 2;
     }
-    auto arg_u = [&](int i, uint32_t d) { return argc > i ? (uint32_t)std::strtoul(argv[i], nullptr, 10) : d; };
-    auto arg_f = [&](int i, float d) { return argc > i ? std::strtof(argv[i], nullptr) : d; };
+    auto arg_u = [&](int i, uint32_t d) {
+        uint32_t v;
+        return argc > i && parse_u32(argv[i], &v) ? v : d;
+    };
+    auto arg_f = [&](int i, float d) {
+        float v;
+        return argc > i && parse_f32(argv[i], &v) ? v : d;
+    };
     std::string scene_arg = argv[1];
     uint64_t seed = 1;
     const size_t colon = scene_arg.find(':');
@@ -129,6 +188,14 @@ int main(int argc, char **argv) {
     const uint32_t debug_view = arg_u(11, 0);
     const float zoom = arg_f(12, 0.0f), yaw = arg_f(13, 0.0f), pitch = arg_f(14, 0.0f);
     const bool raw_save = is_one(argc, argv, 15);
+    if (std::getenv("RT_HEADLESS_PRINT_ARGS")) {
+        // the parsed argument set, printed before anything touches the GPU
+        std::printf("scene=%s seed=%llu frames=%u out=%s denoise=%d app_like=%d photons=%d spp=%u accum=%u w=%u "
+                    "h=%u debug_view=%u zoom=%.9g yaw=%.9g pitch=%.9g raw_save=%d\n",
+                    scene_arg.c_str(), (unsigned long long)seed, frames, out.c_str(), denoise, app_like, photons,
+                    spp, accum, w, h, debug_view, (double)zoom, (double)yaw, (double)pitch, raw_save);
+        return 0;
+    }
 
     const auto ends_with = [&](const char *sfx) {
         const size_t n = std::strlen(sfx);

@@ -1355,8 +1355,12 @@ int rt_traversal_info(const rt_renderer *r, uint32_t *mode, uint32_t *bvh_depth,
 int rt_traversal_stack_words(const rt_renderer *r, uint32_t *words) {
     return api_call([&] {
         if (!r || !words) throw ApiError(RT_ERR_INVALID, "null argument");
+        // a deferred child is one word (reference nodes, at most 63 pushed) or two
+        // (child-box records: index and entry distance); a walk defers at most one
+        // child per level below the root
         const uint32_t m = r->traversal_mode();
-        *words = m == 3u ? r->stack_words4 : (m == 0u ? 126u : 2u * (r->bvh_depth > 0 ? r->bvh_depth - 1 : 0));
+        const uint32_t deferred = r->bvh_depth > 0 ? r->bvh_depth - 1 : 0;
+        *words = m == 3u ? r->stack_words4 : (m == 0u ? std::min(63u, deferred) : 2u * deferred);
     });
 }
 

@@ -763,7 +763,7 @@ static std::unique_ptr<Scene> scene_photon_plain(uint64_t seed) {
     add_geodesic(*s, {0.6f, 0.4f, 0.25f}, 0.4f, 20, [](Vec3) { return 0.0f; }, std::vector<uint32_t>{glass});
     const uint32_t floor = add_material(*s, material_lambertian({0.7f, 0.7f, 0.68f}));
     add_quad(*s, {-2, 0, 2}, {2, 0, 2}, {2, 0, -2}, {-2, 0, -2}, floor);
-    add_area_light_quad(*s, {-0.4f, 2.2f, -0.4f}, {-0.4f, 2.2f, 0.4f}, {0.4f, 2.2f, 0.4f}, {0.4f, 2.2f, -0.4f},
+    add_area_light_quad(*s, {-0.4f, 2.2f, -0.4f}, {0.4f, 2.2f, -0.4f}, {0.4f, 2.2f, 0.4f}, {-0.4f, 2.2f, 0.4f},
                         {15.0f, 15.0f, 15.0f});
     add_point_light(*s, {1.2f, 1.5f, 1.0f}, {1.0f, 0.95f, 0.9f}, 6.0f);
     return s;
@@ -775,9 +775,16 @@ static std::unique_ptr<Scene> scene_multiview(uint64_t seed) {
     auto s = std::make_unique<Scene>();
     HostRng rng(seed);
     for (int i = 0; i < 70; i++) {
-        Mesh m = i < 30 ? make_torus(rng.uniform(0.25f, 0.45f), rng.uniform(0.06f, 0.14f), 48, 24, 0.002f, &rng)
-                        : make_bevelled_box({rng.uniform(0.15f, 0.35f), rng.uniform(0.15f, 0.35f), rng.uniform(0.15f, 0.35f)},
-                                            0.05f, 9, 0.002f, &rng);
+        // draws are named one per statement: the order of argument evaluation is unspecified
+        Mesh m;
+        if (i < 30) {
+            const float R = rng.uniform(0.25f, 0.45f);
+            const float r = rng.uniform(0.06f, 0.14f);
+            m = make_torus(R, r, 48, 24, 0.002f, &rng);
+        } else {
+            const Vec3 half{rng.uniform(0.15f, 0.35f), rng.uniform(0.15f, 0.35f), rng.uniform(0.15f, 0.35f)};
+            m = make_bevelled_box(half, 0.05f, 9, 0.002f, &rng);
+        }
         const Vec3 axis = normalize(Vec3{rng.uniform(-1, 1), rng.uniform(-1, 1) + 0.01f, rng.uniform(-1, 1)});
         const int gx = i % 9, gz = i / 9;
         const Vec3 off{-3.2f + 0.8f * gx + rng.uniform(-0.15f, 0.15f), 0.45f + rng.uniform(0.0f, 0.5f),
@@ -788,9 +795,9 @@ static std::unique_ptr<Scene> scene_multiview(uint64_t seed) {
     }
     const uint32_t ground = add_material(*s, material_lambertian({0.6f, 0.6f, 0.6f}));
     add_quad(*s, {-5, 0, 5}, {5, 0, 5}, {5, 0, -5}, {-5, 0, -5}, ground);
-    add_area_light_quad(*s, {-1.5f, 4.0f, -1.0f}, {-1.5f, 4.0f, 0.0f}, {-0.5f, 4.0f, 0.0f}, {-0.5f, 4.0f, -1.0f},
+    add_area_light_quad(*s, {-1.5f, 4.0f, -1.0f}, {-0.5f, 4.0f, -1.0f}, {-0.5f, 4.0f, 0.0f}, {-1.5f, 4.0f, 0.0f},
                         {20.0f, 19.0f, 17.0f});
-    add_area_light_quad(*s, {1.0f, 3.5f, 1.0f}, {1.0f, 3.5f, 1.8f}, {1.8f, 3.5f, 1.8f}, {1.8f, 3.5f, 1.0f},
+    add_area_light_quad(*s, {1.0f, 3.5f, 1.0f}, {1.8f, 3.5f, 1.0f}, {1.8f, 3.5f, 1.8f}, {1.0f, 3.5f, 1.8f},
                         {12.0f, 14.0f, 18.0f});
     add_environment(*s, {0.45f, 0.55f, 0.8f}, {0.9f, 0.88f, 0.85f}, 0.6f);
     return s;
@@ -842,7 +849,7 @@ static std::unique_ptr<Scene> scene_fullframe(uint64_t seed) {
         m.material_id = add_material(*s, random_material(rng));
         s->meshes.push_back(std::move(m));
     }
-    add_area_light_quad(*s, {-3.0f, 9.0f, -3.0f}, {-3.0f, 9.0f, 3.0f}, {3.0f, 9.0f, 3.0f}, {3.0f, 9.0f, -3.0f},
+    add_area_light_quad(*s, {-3.0f, 9.0f, -3.0f}, {3.0f, 9.0f, -3.0f}, {3.0f, 9.0f, 3.0f}, {-3.0f, 9.0f, 3.0f},
                         {6.0f, 5.6f, 5.0f});
     add_environment(*s, {0.42f, 0.55f, 0.85f}, {0.92f, 0.9f, 0.86f}, 1.0f);
     return s;
@@ -884,8 +891,8 @@ static std::unique_ptr<Scene> scene_training(uint64_t seed) {
         const float L = rng.log_uniform(1.0f, 50.0f);
         const Vec3 c{rng.uniform(-2, 2), rng.uniform(2.5f, 4.0f), rng.uniform(-2, 2)};
         const float h = rng.uniform(0.2f, 0.6f);
-        add_area_light_quad(*s, {c.x - h, c.y, c.z - h}, {c.x - h, c.y, c.z + h}, {c.x + h, c.y, c.z + h},
-                            {c.x + h, c.y, c.z - h}, {L, L * rng.uniform(0.8f, 1.0f), L * rng.uniform(0.7f, 1.0f)});
+        add_area_light_quad(*s, {c.x - h, c.y, c.z - h}, {c.x + h, c.y, c.z - h}, {c.x + h, c.y, c.z + h},
+                            {c.x - h, c.y, c.z + h}, {L, L * rng.uniform(0.8f, 1.0f), L * rng.uniform(0.7f, 1.0f)});
     }
     add_environment(*s, {0.45f, 0.55f, 0.8f}, {0.9f, 0.88f, 0.85f}, rng.uniform(0.0f, 0.5f));
     return s;

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import re
 import sys
 
 import numpy as np
@@ -23,12 +24,31 @@ from . import RenderConfig, Renderer, Scene, _ffi
 from ._ffi import check, lib, ptr
 
 
-def _arg(argv, i, conv, default):
+# The grammars of Rust's str::parse::<u32>() and str::parse::<f32>(): no
+# whitespace, no underscores, no hex; a u32 takes an optional '+' and at most
+# u32::MAX; an f32 takes [+-] then inf / infinity / nan (any case) or a decimal
+# number with an optional exponent.
+_U32 = re.compile(r"\+?[0-9]+")
+_F32 = re.compile(r"[+-]?(?:inf|infinity|nan|(?:[0-9]+(?:\.[0-9]*)?|\.[0-9]+)(?:e[+-]?[0-9]+)?)", re.I)
+
+
+def parse_u32(s: str):
+    if _U32.fullmatch(s) and int(s) <= 0xFFFFFFFF:
+        return int(s)
+    return None
+
+
+def parse_f32(s: str):
+    if _F32.fullmatch(s):
+        return float(np.float32(float(s)))
+    return None
+
+
+def _arg(argv, i, parse, default):
+    """render_headless.rs parses with .parse().ok(): a value that does not parse keeps the default."""
     if len(argv) > i:
-        try:
-            return conv(argv[i])
-        except ValueError:
-            return default  # render_headless.rs parses with .ok(): a bad value keeps the default
+        v = parse(argv[i])
+        return default if v is None else v
     return default
 
 
@@ -58,15 +78,15 @@ def main(argv=None) -> int:
     if len(argv) < 2:
         print(__doc__.strip().splitlines()[2].strip(), file=sys.stderr)
         return 2
-    frames = _arg(argv, 2, int, 128)
+    frames = _arg(argv, 2, parse_u32, 128)
     out = argv[3] if len(argv) > 3 else "render_headless.png"
     flag = lambda i: len(argv) > i and argv[i] == "1"
     denoise, app_like, photons = flag(4), flag(5), flag(6)
-    spp = _arg(argv, 7, int, 4 if app_like else 1)
-    accum = _arg(argv, 8, int, 32 if app_like else frames)
-    w, h = _arg(argv, 9, int, 800), _arg(argv, 10, int, 600)
-    debug_view = _arg(argv, 11, int, 0)
-    zoom, yaw, pitch = _arg(argv, 12, float, 0.0), _arg(argv, 13, float, 0.0), _arg(argv, 14, float, 0.0)
+    spp = _arg(argv, 7, parse_u32, 4 if app_like else 1)
+    accum = _arg(argv, 8, parse_u32, 32 if app_like else frames)
+    w, h = _arg(argv, 9, parse_u32, 800), _arg(argv, 10, parse_u32, 600)
+    debug_view = _arg(argv, 11, parse_u32, 0)
+    zoom, yaw, pitch = _arg(argv, 12, parse_f32, 0.0), _arg(argv, 13, parse_f32, 0.0), _arg(argv, 14, parse_f32, 0.0)
     raw_save = flag(15)
 
     scene = load_scene(argv[1])

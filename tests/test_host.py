@@ -137,3 +137,48 @@ def test_python_cli_loads_gltf_with_sibling_descriptor(rt, tmp_path):
     assert len(plain) == 1 and plain[0]["intensity"] == 1.0
     assert [l["kind"] for l in lights] == [3, 0] and lights[0]["intensity"] == 2.5
     assert load_scene("cornell_box:3").counts()["triangles"] == rt.Scene.builtin("cornell_box", 3).counts()["triangles"]
+
+
+def _headless_args(*args):
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rustracer_amd",
+                       "render_headless")
+    if not os.path.exists(exe):
+        pytest.skip("render_headless not built")
+    env = dict(os.environ, RT_HEADLESS_PRINT_ARGS="1")
+    out = subprocess.run([exe, *args], env=env, capture_output=True, text=True, check=True).stdout
+    return dict(kv.split("=", 1) for kv in out.split())
+
+
+def test_render_headless_unparsable_arguments_keep_defaults():
+    """render_headless.rs:21-66 parses each argument with .parse().ok() and keeps
+    the default when that fails: a bad frame count is 128, not 0."""
+    a = _headless_args("cornell_box", "x12", "o.png", "0", "0", "0", " 4", "-1", "80O", "1e3", "2.5", "1,5",
+                       "0x10", "nan")
+    assert a["frames"] == "128" and a["spp"] == "1" and a["accum"] == "128"
+    assert a["w"] == "800" and a["h"] == "600" and a["debug_view"] == "0"
+    assert a["zoom"] == "0" and a["yaw"] == "0" and a["pitch"] == "nan"
+    a = _headless_args("fullframe:7", "+12", "o.png", "1", "1", "1", "2", "4294967295", "640", "4294967296", "3",
+                       "1.5e1", "-.25", "inf")
+    assert a["scene"] == "fullframe" and a["seed"] == "7" and a["frames"] == "12"
+    assert (a["denoise"], a["app_like"], a["photons"]) == ("1", "1", "1")
+    assert a["spp"] == "2" and a["accum"] == "4294967295" and a["w"] == "640" and a["h"] == "600"
+    assert a["debug_view"] == "3" and a["zoom"] == "15" and a["yaw"] == "-0.25" and a["pitch"] == "inf"
+    a = _headless_args("cornell_box", "5", "o.png", "0", "1")  # app_like: spp 4, accum 32
+    assert a["frames"] == "5" and a["spp"] == "4" and a["accum"] == "32"
+
+
+def test_python_render_headless_parsers_follow_rust_grammar():
+    from rustracer_amd.render_headless import parse_f32, parse_u32
+
+    for s, v in (("12", 12), ("+12", 12), ("4294967295", 4294967295)):
+        assert parse_u32(s) == v
+    for s in ("", "+", "-1", " 1", "1 ", "1_0", "0x10", "4294967296", "1.0"):
+        assert parse_u32(s) is None, s
+    for s, v in (("1", 1.0), ("1.", 1.0), (".5", 0.5), ("-2.5e1", -25.0), ("1E-1", np.float32(0.1)),
+                 ("INF", np.inf), ("-infinity", -np.inf)):
+        assert parse_f32(s) == v, s
+    assert np.isnan(parse_f32("NaN"))
+    for s in ("", ".", "e5", "1e", "0x1p3", " 1", "1_0", "1,5", "--1", "infx"):
+        assert parse_f32(s) is None, s
