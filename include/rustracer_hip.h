@@ -300,6 +300,25 @@ int rt_frame_graph_captures(const rt_renderer *r, uint32_t *captures);
  * scatter: photon_count.wgsl, lib.rs:987-1007, photon_scatter.wgsl) into the
  * renderer's grid, replacing traced photons. n_slots <= photon_count*max_depth. */
 int rt_build_photon_grid_from(rt_renderer *r, const rt_photon *photons, uint32_t n_slots);
+/* Allocation-free variant of rt_build_photon_grid_from. It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It builds the grid of n_slots photon records already in device
+ * memory (d_photons) into caller-owned device outputs, on `stream` (a
+ * hipStream_t of the current device; NULL = the null stream), asynchronously:
+ *   d_grid_meta  [start, count] per cell, meta_cells >= gp->num_cells entries
+ *   d_sorted     the stored photons in cell order, stable (n_slots records fit)
+ *   d_gather_rec 3 float4 per stored photon (position, power, incident), or NULL
+ *   d_n_sorted   one u32: the number of stored photons
+ * The temporaries live in a fixed library-owned device array for
+ * RT_GRID_NOALLOC_MAX_SLOTS slots (about 17 MB), one per GPU, held until the
+ * process ends; larger inputs fail with RT_ERR_CAPACITY. Calls on one GPU share
+ * that array, so calls that overlap on one GPU race. The first call on a device
+ * looks up the array's address (make it before capturing a graph). */
+#define RT_GRID_NOALLOC_MAX_SLOTS (1u << 20)
+int rt_photon_grid_build_noalloc(const rt_photon *d_photons, uint32_t n_slots, const rt_grid_params *gp,
+                                 uint32_t *d_grid_meta, uint32_t meta_cells, rt_photon *d_sorted,
+                                 float *d_gather_rec, uint32_t *d_n_sorted, void *stream);
 
 /* Per-pass timing with HIP events (cfg.flags & RT_FLAG_PROFILE). */
 typedef struct rt_pass_stats {

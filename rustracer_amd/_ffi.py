@@ -210,6 +210,7 @@ SIGNATURES = {
     "rt_scene_add_texture": (_i, [_vp, _u32, _u32, _u32, _vp, _P(_u32)]),
     "rt_scene_texture_array": (_i, [_vp, _u32, _vp, C.c_size_t, _P(_u32), _P(_u32), _P(_u32)]),
     "rt_build_photon_grid_from": (_i, [_vp, _vp, _u32]),
+    "rt_photon_grid_build_noalloc": (_i, [_vp, _u32, _P(GridParams), _vp, _u32, _vp, _vp, _vp, _vp]),
     "rt_get_pass_stats": (_i, [_vp, _P(PassStats)]),
     "rt_reset_pass_stats": (_i, [_vp]),
     "rt_count_gather_traffic": (_i, [_vp, _P(TraceCounters)]),

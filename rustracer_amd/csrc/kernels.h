@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "../../include/rt_types.h"
+#include "../../include/rustracer_hip.h"
 
 namespace rtk {
 
@@ -136,6 +137,12 @@ struct GridBuildLaunch {
 };
 size_t grid_build_temp_bytes(uint32_t n_slots, uint32_t meta_cells);
 hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s);
+// The same build from uploaded photons (keys computed from the records) with
+// the temporaries in the library's fixed per-device workspace instead of
+// g.temp: no allocation. n_slots <= kNoallocMaxSlots; gather_rec may be null.
+constexpr uint32_t kNoallocMaxSlots = RT_GRID_NOALLOC_MAX_SLOTS;
+constexpr int kNoallocMaxDevices = 64;
+hipError_t launch_grid_build_noalloc(GridBuildLaunch g, hipStream_t s);
 
 struct DenoiseLaunch {
     const void *in;

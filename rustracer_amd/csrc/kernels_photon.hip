@@ -15,6 +15,8 @@
 //               equal digits inside a wavefront with ballots
 //   4. meta:    grid_meta [start, count] per cell by binary search of the keys
 //   5. gather:  sorted_photons[i] = photons[slot of i]
+#include <mutex>
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -494,6 +496,7 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const rt_photon *photo
         float4 *dst = reinterpret_cast<float4 *>(sorted + i);
         const float4 v = src[part];
         dst[part] = v;
+        if (!gather_rec) continue;  // uniform: the caller keeps no gather records
         if (part == 0u) {
             gather_rec[3 * i] = make_float4(v.x, v.y, v.z, src[1].x);
         } else if (part == 1u) {
@@ -545,19 +548,23 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-static uint32_t tiles_of(size_t n) { return (uint32_t)((n + rtd::kSortTile - 1) / rtd::kSortTile); }
+static constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+static constexpr uint32_t tiles_of(size_t n) { return (uint32_t)((n + rtd::kSortTile - 1) / rtd::kSortTile); }
+// Device temporaries of a build over n slots (laid out by launch_grid_build).
+static constexpr size_t temp_bytes_for(size_t n) {
+    const uint32_t t = tiles_of(n) > 1u ? tiles_of(n) : 1u;
+    const size_t hist = (size_t)rtd::kRadix * t;
+    const uint32_t th = tiles_of(hist) > 1u ? tiles_of(hist) : 1u;
+    return 4 * align256(n * 4) +                      // keys/vals x 2
+           align256(hist * 4) +                       // histograms
+           align256((size_t)t * 4) +                  // valid slots per tile
+           align256((size_t)(t > th ? t : th) * 4) +  // scan tile sums
+           256;
+}
 
 size_t grid_build_temp_bytes(uint32_t n_slots, uint32_t meta_cells) {
     (void)meta_cells;
-    const size_t n = n_slots;
-    const uint32_t t = std::max(1u, tiles_of(n));
-    const size_t hist = (size_t)rtd::kRadix * t;
-    const uint32_t th = std::max(1u, tiles_of(hist));
-    return 4 * align256(n * 4) +                       // keys/vals x 2
-           align256(hist * 4) +                        // histograms
-           align256((size_t)t * 4) +                   // valid slots per tile
-           align256((size_t)std::max(t, th) * 4) + 256;  // scan tile sums
+    return temp_bytes_for(n_slots);
 }
 
 // Exclusive scan of `in[0..n)` into out.
@@ -636,6 +643,49 @@ hipError_t launch_grid_build(const GridBuildLaunch &g, hipStream_t s) {
     hipLaunchKernelGGL(rtd::grid_gather_kernel, dim3(gather_blocks), dim3(256), 0, s, g.photons,
                        vin, g.n_valid_out, g.sorted, reinterpret_cast<float4 *>(g.gather_rec));
     return hipGetLastError();
+}
+
+// The allocation-free build's workspace: a file-scope device array sized for
+// kNoallocMaxSlots slots. Every GPU the library runs on holds its own copy, at
+// its own address, until the process ends; calls on one GPU share it (the
+// last call's contents stay), so calls that overlap on one GPU race. No part
+// of it needs clearing: every pass overwrites what a later pass reads.
+static constexpr size_t kNoallocScratchBytes = temp_bytes_for(kNoallocMaxSlots);
+static __device__ __align__(256) uint8_t scratch[kNoallocScratchBytes];
+
+// Device address of `scratch` on the current device. In host code the array's
+// name is only a stand-in for the symbol, so kernels get the address that
+// hipGetSymbolAddress returns, looked up once per device with that device
+// current.
+static hipError_t noalloc_scratch(void **out) {
+    static std::mutex mu;
+    static void *ptr[kNoallocMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoallocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        ptr[dev] = p;
+    }
+    *out = ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_grid_build_noalloc(GridBuildLaunch g, hipStream_t s) {
+    if (g.n_slots > kNoallocMaxSlots || temp_bytes_for(g.n_slots) > kNoallocScratchBytes)
+        return hipErrorInvalidValue;  // the caller checks the size first and reports it
+    void *p = nullptr;
+    const hipError_t e = noalloc_scratch(&p);
+    if (e != hipSuccess) return e;
+    g.temp = p;
+    g.temp_bytes = kNoallocScratchBytes;
+    g.keys_in = nullptr;
+    return launch_grid_build(g, s);
 }
 
 }  // namespace rtk

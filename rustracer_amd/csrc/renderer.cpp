@@ -1414,6 +1414,32 @@ int rt_build_photon_grid_from(rt_renderer *r, const rt_photon *photons, uint32_t
     });
 }
 
+int rt_photon_grid_build_noalloc(const rt_photon *d_photons, uint32_t n_slots, const rt_grid_params *gp,
+                                 uint32_t *d_grid_meta, uint32_t meta_cells, rt_photon *d_sorted,
+                                 float *d_gather_rec, uint32_t *d_n_sorted, void *stream) {
+    return api_call([&] {
+        if (!gp || !d_n_sorted || (n_slots && (!d_photons || !d_sorted)) || (meta_cells && !d_grid_meta))
+            throw ApiError(RT_ERR_INVALID, "null argument");
+        if (n_slots > RT_GRID_NOALLOC_MAX_SLOTS)
+            throw ApiError(RT_ERR_CAPACITY, "n_slots " + std::to_string(n_slots) +
+                                                " exceeds the allocation-free workspace of " +
+                                                std::to_string(RT_GRID_NOALLOC_MAX_SLOTS) + " slots");
+        if (meta_cells < gp->num_cells) throw ApiError(RT_ERR_INVALID, "meta_cells below gp->num_cells");
+        rtk::GridBuildLaunch g;
+        g.photons = d_photons;
+        g.n_slots = n_slots;
+        g.gp = *gp;
+        g.meta_cells = meta_cells;
+        g.grid_meta = d_grid_meta;
+        g.sorted = d_sorted;
+        g.gather_rec = d_gather_rec;
+        g.temp = nullptr;  // the library's fixed workspace
+        g.temp_bytes = 0;
+        g.n_valid_out = d_n_sorted;
+        hip_check(rtk::launch_grid_build_noalloc(g, static_cast<hipStream_t>(stream)), "allocation-free grid build");
+    });
+}
+
 int rt_get_pass_stats(rt_renderer *r, rt_pass_stats *out) {
     return api_call([&] {
         if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
