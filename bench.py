@@ -33,21 +33,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Msamples/s @1080p; ms to relMSE≤0.01: batched+learned vs plain à-trous"
 HBM_PEAK_GBS = 8000.0
+CUS, CLOCK_GHZ = 256, 2.4  # MI355X compute units and peak engine clock
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
 # Algorithmic bytes per unit of work of the traversal layouts: a frame's bytes are
 # 32 per child box tested + 48 per triangle tested + 32 per ray query (ray + hit)
-# + 16 per photon candidate + 48 of outputs per pixel.
+# + 36 per photon candidate + 48 of outputs per pixel.
 NODE_BOX_BYTES = 32    # one child box: min/max (binary record 64 B / 2, 4-wide record 128 B / 4)
 TRI_BYTES = 48         # slot triangle: v0, v1-v0, v2-v0 (3 x float4)
 RAY_BYTES = 32         # 24 B ray + 8 B hit
-PHOTON_BYTES = 16       # position read per radius-gather candidate (power/incident only on accept)
+PHOTON_BYTES = 36      # per radius-gather candidate: position + power (16), power + incident (16), incident.z (4)
 PIXEL_OUT_BYTES = 48   # frame + moments + gbuffer float4 writes
 DENOISE_PX_BYTES = 64  # per pass: color in + gbuffer + moments + out, 16 B each
 COMPOSITE_TONEMAP_PX_BYTES = 52  # frame 16 + accumulation 16 read + 16 written + rgba8 4
 GRID_SLOT_BYTES = 32   # count (position + power.r, 16) + scatter key read (16) per slot
 GRID_STORED_BYTES = 128  # stored photon: 64 B record read + 64 B written
 GRID_CELL_BYTES = 8    # [start, count] per cell
+# The reference's own layout, from the oracle's counters of the same frame: a
+# 48-byte BVHNode per node visited, a 128-byte TriangleGpu per triangle tested,
+# a 64-byte PhotonGpu per photon candidate (bvh.rs:79-88, scene.rs:74-94,
+# lib.rs:86-97).
+REF_NODE_BYTES, REF_TRI_BYTES, REF_PHOTON_BYTES = 48, 128, 64
 
 
 TRAVERSALS = {0: "reference BVH2 nodes", 1: "binary child-box records, reference order",
@@ -93,8 +99,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batch measurement")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 720p photon-map workload and the network training throughput")
-    ap.add_argument("--convergence", action="store_true",
-                    help="run the ms-to-relMSE study now (about two minutes) instead of reporting the committed one")
+    ap.add_argument("--no-convergence", action="store_true",
+                    help="skip the ms-to-relMSE study (about two minutes: a 16384-frame FP64 reference, a dataset "
+                         "and a training run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -293,6 +300,28 @@ def main() -> int:
                                           "'traffic' figure); VALU issue bounds this kernel: "
                                           "profiles/gather_fullframe_1080p_valu_counters.md")
 
+    # VALU issue of the gather launch, from rocprofv3 counters of this workload
+    # (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, SQ_THREAD_CYCLES_VALU) kept with the
+    # profiles: the bound the camera pass actually meets
+    valu_file = os.path.join(ROOT, "profiles", "gather_fullframe_1080p_valu.json")
+    if default_run and os.path.exists(valu_file):
+        v = json.load(open(valu_file))
+        form = "path_queues" if args.path_queues else "tile_launch" if args.tile_launch else "persistent"
+        if (v.get("traversal_mode") == tinfo["mode"] and v.get("camera_pass") == form and
+                v.get("frames_per_launch") == frames_per_launch):
+            # one wave64 VALU instruction per cycle per CU (4 SIMDs, 4 cycles each)
+            issue_ms = v["SQ_INSTS_VALU"] / CUS / (CLOCK_GHZ * 1e6)
+            lanes = v["SQ_THREAD_CYCLES_VALU"] / v["SQ_ACTIVE_INST_VALU"]
+            result["valu_roofline"] = {
+                "bound": "valu", "kernel": "gather", "valu_instructions_per_launch": v["SQ_INSTS_VALU"],
+                "issue_ms_at_peak": round(issue_ms, 4), "kernel_avg_ms": round(gather_avg_ms, 4),
+                "issue_frac": round(issue_ms / gather_avg_ms, 4), "active_lanes_per_valu": round(lanes, 2),
+                "lane_frac": round(lanes / 64.0, 4), "source": v["source"],
+                "note": f"issue at one wave64 VALU instruction per cycle per CU, {CUS} CUs at {CLOCK_GHZ} GHz; the "
+                        "HBM roofline above counts algorithmic bytes that L2/MALL serve"}
+            result["roofline"]["note"] = ("bound by VALU issue, not HBM: see valu_roofline; the algorithmic bytes "
+                                          "are served from L2/MALL ('traffic' is the measured HBM traffic)")
+
     if elapsed_other is not None:
         key = "msamples_per_s_render_frames" if args.record_frame else "msamples_per_s_record_frame"
         result[key] = round(w * h * args.steps / elapsed_other / 1e6, 3)
@@ -303,6 +332,17 @@ def main() -> int:
     log(f"HBM copy probe: {copy_gbs:.0f} GB/s (1 GiB copy in {copy_ms:.3f} ms)")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(rt, r, scene, cam, cfg, args)
+        oc = result["cpu_baseline"].pop("oracle_counters_first_frame")
+        # the same frame's algorithmic bytes in the reference's layout, from the
+        # oracle's counters of the reference traversal
+        ref_bytes = (oc["nodes_visited"] * REF_NODE_BYTES + oc["tris_tested"] * REF_TRI_BYTES +
+                     oc["ray_queries"] * RAY_BYTES + oc["photon_reads"] * REF_PHOTON_BYTES + w * h * PIXEL_OUT_BYTES)
+        result["reference_layout_bytes"] = {
+            "frame_bytes_reference_layout": int(ref_bytes), "frame_bytes_this_layout": int(frame_bytes),
+            "oracle_counters": oc,
+            "same_ray_queries": oc["ray_queries"] == cnt["ray_queries"],
+            "note": "the frame the counting launch traced, counted again by the oracle's reference traversal "
+                    "(48 B nodes, 128 B triangles, 64 B photon records)"}
     if not args.no_network and w % 2 == 0 and h % 2 == 0:
         result["network_roofline"] = network_roofline(r, w, h)
     r.close()
@@ -312,7 +352,8 @@ def main() -> int:
         result["photon_plain_720p"] = photon_plain_720p()
         if not args.no_network:
             result["network_training"] = network_training()
-    result["convergence"] = convergence(args)
+    if not args.no_convergence:
+        result["convergence"] = convergence()
     print(json.dumps(result), flush=True)
     return 0
 
@@ -381,36 +422,42 @@ def photon_plain_720p() -> dict:
 
 
 def network_training() -> dict:
-    """Training throughput of the learned denoiser: Adam steps on batches of 32
-    crops of 256x256 (the training workload's batch shape) from a small
-    renderer-made dataset; forward + backward = 3x the forward FLOPs."""
+    """Training throughput of the learned denoiser at the training workload's
+    batch shape: Adam steps on batches of 32 crops of 256x256 from a small
+    renderer-made dataset. FLOPs per step = forward + weight gradients +
+    data gradients (every layer but the first) of the 3x3 layers, plus the
+    output layer's."""
     from rustracer_amd.denoiser import UNet, build_dataset
 
     net = UNet(0, 1)
-    build_dataset(net, n_scenes=2, size=256, target_frames=64, photons=50_000, first_seed=77)
+    build_dataset(net, n_scenes=4, size=256, target_samples=256, photons=50_000, first_seed=77)
     net.train(steps=3, batch=32, crop=256, lr=1e-3, seed=1)
     steps = 20
     t0 = time.perf_counter()
     net.train(steps=steps, batch=32, crop=256, lr=1e-3, seed=2)
     el = time.perf_counter() - t0
     net.close()
-    flops = 3.0 * network_flops(256, 256) * 32
+    flops = network_training_flops(256, 256) * 32
     tf = flops * steps / el / 1e12
     log(f"network training: {el / steps * 1e3:.2f} ms/step (batch 32 x 256^2), {tf:.1f} TFLOP/s")
     return {"bound": "mfma", "ms_per_step": round(el / steps * 1e3, 3), "batch": "32 x 256x256",
-            "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
+            "flops_per_step": int(flops), "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
             "note": "wall time per Adam step incl. the crop gather and the host's per-step crop draw"}
 
 
 def network_flops(w: int, h: int) -> float:
-    """Multiply-add FLOPs of one U-Net forward pass at w x h (3x3 layers + 1x1 output)."""
+    """Multiply-add FLOPs of one U-Net forward pass at w x h (padded to multiples of 16)."""
+    from rustracer_amd.denoiser import network_flops as nf
+    return nf(w, h)
+
+
+def network_training_flops(w: int, h: int) -> float:
+    """One crop's training FLOPs: 3x the forward's, less the first layer's
+    data gradient (the features need none)."""
     from rustracer_amd.denoiser import LAYER_SHAPES
-    full, half = w * h, (w // 2) * (h // 2)
-    f = 0.0
-    for (cin, cout), px in zip(LAYER_SHAPES, (full, full, half, half, full, full)):
-        f += 2.0 * px * 9 * cin * cout
-    return f + 2.0 * full * 32 * 3
+    first = 2.0 * w * h * 9 * 12 * LAYER_SHAPES[0][1]
+    return 3.0 * network_flops(w, h) - first
 
 
 def network_roofline(r, w: int, h: int) -> dict:
@@ -439,34 +486,26 @@ def network_roofline(r, w: int, h: int) -> dict:
     log(f"network pass {avg:.3f} ms at {w}x{h}: {tf:.1f} TFLOP/s")
     return {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4), "ms_per_frame": round(avg, 4),
-            "flops_per_frame": int(flops), "note": "whole pass: features, 6 BF16 MFMA convolutions, pool, "
-                                                   "upsample, output layer, tonemap of its output"}
+            "flops_per_frame": int(flops), "note": "whole pass: features, 18 BF16 MFMA 3x3 convolutions "
+                                                   "(five levels, 32-128 channels), pools, output layer, tonemap"}
 
 
-def convergence(args) -> dict | None:
-    """ms to relMSE <= 0.01 (learned U-Net vs a-trous) on this workload: measured
-    by `python -m rustracer_amd.convergence` (about two minutes: a 16384-frame
-    FP64 reference and a training run); run here with --convergence, otherwise
-    the committed result of that command is reported with its source."""
-    if args.convergence:
-        import subprocess
-        out = subprocess.run([sys.executable, "-m", "rustracer_amd.convergence"], cwd=ROOT, check=True,
-                             capture_output=True, text=True).stdout
-        d = json.loads(out.strip().splitlines()[-1])
-        source = "measured in this run"
-    else:
-        path = os.path.join(ROOT, "profiles", "convergence_fullframe_1080p.md")
-        if not os.path.exists(path):
-            return None
-        text = open(path).read()
-        d = json.loads(text.split("```")[-2].strip())
-        source = "profiles/convergence_fullframe_1080p.md"
+def convergence() -> dict:
+    """ms to relMSE <= 0.01 (learned U-Net vs a-trous) on this workload, measured
+    now by `python -m rustracer_amd.convergence` in a child process: a
+    16384-frame FP64 reference, the network's dataset and training run, then
+    each method's frames until the displayed image reaches relMSE 0.01."""
+    import subprocess
+    out = subprocess.run([sys.executable, "-m", "rustracer_amd.convergence"], cwd=ROOT, check=True,
+                         capture_output=True, text=True).stdout
+    d = json.loads(out.strip().splitlines()[-1])
     methods = {m["method"]: m for m in d["methods"]}
-    out = {"metric": "ms to relMSE<=0.01 (eps 1e-2) vs FP64 16384-frame reference", "source": source,
+    res = {"metric": "ms to relMSE<=0.01 (eps 1e-2) vs FP64 16384-frame reference", "source": "measured in this run",
            "methods": {k: {"frames": v.get("frames"), "ms": v.get("ms")} for k, v in methods.items()}}
-    if "speedup_vs_atrous" in d:
-        out["speedup_learned_vs_atrous"] = d["speedup_vs_atrous"]
-    return out
+    for k in ("speedup_vs_atrous", "network", "training"):
+        if k in d:
+            res["speedup_learned_vs_atrous" if k == "speedup_vs_atrous" else k] = d[k]
+    return res
 
 
 def cpu_baseline(rt, r, scene, cam, cfg, args):
@@ -477,7 +516,10 @@ def cpu_baseline(rt, r, scene, cam, cfg, args):
     from tests.helpers import image_params
     from tests.oracle_ffi import OracleScene
 
-    threads = min(16, os.cpu_count() or 1)
+    # every core this process may run on, unless OMP_NUM_THREADS caps the share
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, max(1, int(os.environ["OMP_NUM_THREADS"])))
     gp, meta, srt = r.grid_params(), r.read_grid_meta(), r.read_sorted_photons()
     # the gather's own outputs (the frame buffer itself holds the a-trous result)
     gpu_mom, gpu_gb = r.read_moments(), r.read_gbuffer()
@@ -485,21 +527,25 @@ def cpu_baseline(rt, r, scene, cam, cfg, args):
     frame_idx = r.sample_count()
     o = OracleScene.from_scene(scene)
     u = rt.camera_to_uniform(cam, c.width, c.height)
-    frames, elapsed, exact = 0, 0.0, None
-    while elapsed < args.cpu_seconds and frames < 16:
+    frames, elapsed, exact, first_stats = 0, 0.0, None, None
+    while (elapsed < args.cpu_seconds and frames < 16) or frames < 2:
         ip = image_params(c, frame_idx + frames, alpha=1.0)
         t0 = time.perf_counter()
-        _, mom, gb, _ = o.gather(u, ip, max_depth=c.max_depth, photon_map=(gp, meta, srt), threads=threads)
+        _, mom, gb, st = o.gather(u, ip, max_depth=c.max_depth, photon_map=(gp, meta, srt), threads=threads)
         elapsed += time.perf_counter() - t0
         if frames == 0:  # the frame the GPU rendered last
             exact = bool(np.array_equal(mom, gpu_mom) and np.array_equal(gb, gpu_gb))
+        if frames == 1:  # the frame rt_count_gather_traffic traced
+            first_stats = {k: int(getattr(st, k)) for k in ("ray_queries", "nodes_visited", "tris_tested",
+                                                             "photon_reads")}
         frames += 1
     msps = frames * c.width * c.height / elapsed / 1e6
     log(f"cpu baseline: {frames} full frames on {threads} threads in {elapsed:.1f}s, "
         f"moments+gbuffer bit-exact vs GPU: {exact}")
     return {"value": round(msps, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"{frames} full {c.width}x{c.height} frames x 1 spp (gather only, same photon map)",
-            "sample_bit_exact_vs_gpu": exact}
+            "sample_bit_exact_vs_gpu": exact, "cores_note": "the cores this process may use (sched_getaffinity), "
+            "capped by OMP_NUM_THREADS", "oracle_counters_first_frame": first_stats}
 
 
 if __name__ == "__main__":

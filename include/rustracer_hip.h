@@ -95,6 +95,12 @@ typedef struct rt_config {
  * a graph is captured again when anything else it was made from changes
  * (config, photon map, textures, buffers). Same results. */
 #define RT_FLAG_GRAPH 128u
+/* Keep a running per-pixel history of the frames' luminance moments (E[l],
+ * E[l^2], weighted as the accumulation is) beside the accumulation. The learned
+ * denoiser's variance features read it; it is kept whenever a denoiser is
+ * attached, and this flag keeps it without one (for rt_unet_dataset_set). No
+ * counterpart in the reference, whose denoiser sees only one frame's moments. */
+#define RT_FLAG_MOMENT_HISTORY 256u
 
 /* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
 int rt_config_default(rt_config *cfg);
@@ -404,30 +410,35 @@ int rt_accum64_relmse(rt_renderer *r, const void *d_image, float eps, double *re
 int rt_accum64_relmse_display(rt_renderer *r, float eps, double *relmse);
 
 /* ───────────── learned denoiser (no counterpart in the reference) ─────────────
- * A small convolutional U-Net on the displayed image: inputs log1p(accumulated
- * radiance), first-hit albedo (pixel-centre ray), G-buffer normal and depth, and
- * the accumulated frame count; output log-radiance residual. BF16 MFMA
- * convolutions with FP32 accumulation; trained on the GPU with Adam on pairs
- * the renderer itself produces (few-frame accumulation -> many-frame target). */
+ * A five-level convolutional U-Net (32-128 channels) on the displayed image:
+ * 12 input channels, log1p(accumulated radiance), first-hit albedo
+ * (pixel-centre ray), G-buffer normal and depth, and two variance channels from
+ * the accumulated luminance moments; output a log-radiance residual. BF16 MFMA
+ * convolutions with FP32 accumulation; trained on the GPU with Adam and an L1
+ * loss on log radiance, on pairs the renderer itself produces (few-frame
+ * accumulation -> many-sample target). Any image size (padded internally to
+ * multiples of 16). Training is deterministic: fixed-order gradient sums. */
 typedef struct rt_unet rt_unet;
 int rt_unet_create(int device, uint64_t seed, rt_unet **out);
 int rt_unet_destroy(rt_unet *u);
 int rt_unet_param_count(const rt_unet *u, size_t *n);
 int rt_unet_get_params(rt_unet *u, float *out, size_t n);
 int rt_unet_set_params(rt_unet *u, const float *in, size_t n);
-/* Training set of `capacity` images of width x height (even, >= 32). */
+/* Training set of `capacity` images of width x height (multiples of 32). */
 int rt_unet_dataset_reset(rt_unet *u, uint32_t width, uint32_t height, uint32_t capacity);
 /* Slot `slot` takes the renderer's current state: which = 0 the network input
- * (accumulation, G-buffer, albedo, frame count), which = 1 the target (the
- * accumulation). The renderer's size must equal the dataset's. */
+ * (accumulation, G-buffer, albedo, moment history: render with
+ * RT_FLAG_MOMENT_HISTORY), which = 1 the target (the accumulation). The
+ * renderer's size must equal the dataset's. */
 int rt_unet_dataset_set(rt_unet *u, uint32_t slot, int which, rt_renderer *r);
 /* `steps` Adam steps on random crops (crop x crop, a multiple of 32) of the
- * slots that hold both; per-step mean squared log error into `losses` (nullable). */
+ * slots that hold both; per-step L1 loss on log radiance into `losses` (nullable). */
 int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, float lr, uint64_t seed,
                   float *losses);
-/* One forward pass (and, with a target, the backward pass) on host tensors:
- * features n*h*w*16 bf16 bits, noisy and target n*h*w*3 log values; pred out
- * n*h*w*3; grads out (param_count floats, same order as the parameters). */
+/* One forward pass (and, with a target, the backward pass) on host tensors
+ * (h, w multiples of 16): features n*h*w*16 bf16 bits, noisy and target n*h*w*3
+ * log values; pred out n*h*w*3; grads out (param_count floats, same order as
+ * the parameters); loss out (mean |pred - target|). */
 int rt_unet_eval(rt_unet *u, uint32_t n, uint32_t h, uint32_t w, const uint16_t *features, const float *noisy,
                  const float *target, float *pred, float *grads, double *loss);
 /* Attach (or detach with NULL) a denoiser: each rt_render_frame then runs it on

@@ -1,14 +1,16 @@
 """CPU reference of the learned denoiser's network (test infrastructure only).
 
-The same U-Net as rustracer_amd/csrc/unet.hip, in numpy: 3x3 convolutions with
-zero padding, ReLU, 2x2 average pool, nearest upsample, a 1x1 output layer with
-a residual, and the mean squared log error. With `quant=True` every tensor the
-GPU stores as bf16 (inputs of each convolution, activations, gradients between
-layers) is rounded to bf16 (round to nearest even) at the same points, and the
-arithmetic is float32; with `quant=False` everything is float64, which is the
-version the tests check against finite differences.
+The same U-Net as rustracer_amd/csrc/unet.hip, in numpy: five levels of 3x3
+convolutions with zero padding and ReLU (32, 64, 128, 128, 128 channels), 2x2
+average pools down, nearest upsamples concatenated after the skip tensor on the
+way up (96, 192, 256, 256 input channels), a 1x1 output layer with a residual on
+the log radiance, and the L1 loss. With `quant=True` every tensor the GPU stores
+as bf16 (inputs of each convolution, activations, gradients between layers) is
+rounded to bf16 (round to nearest even) at the same points, and the arithmetic
+is float32; with `quant=False` everything is float64, which is the version the
+tests check against finite differences.
 
-Parameter vector layout (the GPU's): for each of the six 3x3 layers the weights
+Parameter vector layout (the GPU's): for each of the 18 3x3 layers the weights
 [9][cout][cin] (tap = 3*(dy+1) + (dx+1)) then the bias [cout]; then the output
 layer's weights [3][32] and bias [3].
 """
@@ -16,7 +18,14 @@ from __future__ import annotations
 
 import numpy as np
 
-LAYERS = ((16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (32, 32))
+IN_C = 16  # 12 feature channels, padded with zeros to the MFMA's K step
+ENC = (32, 64, 128, 128, 128)  # channels per level, level 4 is the bottleneck
+LEVELS = len(ENC)
+# (cin, cout) of the 18 layers: two per encoder level, then two per decoder
+# level 3..0 (the first of them reads [skip, upsampled] channels)
+LAYERS = ((IN_C, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (128, 128), (128, 128),
+          (128, 128), (128, 128),
+          (256, 128), (128, 128), (256, 128), (128, 128), (192, 64), (64, 64), (96, 32), (32, 32))
 OUT_C = 32
 
 
@@ -96,67 +105,96 @@ def up(x):
     return x.repeat(2, axis=1).repeat(2, axis=2)
 
 
+def up_back(g):
+    """Adjoint of `up`: the sum of each 2x2 block."""
+    return ((g[:, 0::2, 0::2] + g[:, 0::2, 1::2]) + g[:, 1::2, 0::2]) + g[:, 1::2, 1::2]
+
+
 def forward(p, feats, noisy, target=None, quant=True):
-    """feats (n, h, w, 16) values, noisy/target (n, h, w, 3) log values.
-    Returns pred, loss (or None) and the cache for backward."""
+    """feats (n, h, w, 16) values, noisy/target (n, h, w, 3) log values; h and w
+    multiples of 16. Returns pred, loss (or None) and the cache for backward."""
     dt = np.float32 if quant else np.float64
     q = bf16 if quant else (lambda a: a)
     ws, bs, ow, ob = unpack(np.asarray(p, dt))
     if quant:
         ws = [bf16(w) for w in ws]
     relu = lambda a: np.maximum(a, 0)
-    F = q(np.asarray(feats, dt))
-    A1 = q(relu(conv(F, ws[0]) + bs[0]))
-    A2 = q(relu(conv(A1, ws[1]) + bs[1]))
-    P = q(pool(A2))
-    B1 = q(relu(conv(P, ws[2]) + bs[2]))
-    B2 = q(relu(conv(B1, ws[3]) + bs[3]))
-    CAT = np.concatenate([A2, up(B2)], axis=-1)
-    D1 = q(relu(conv(CAT, ws[4]) + bs[4]))
-    D2 = q(relu(conv(D1, ws[5]) + bs[5]))
-    pred = np.asarray(noisy, dt) + (D2 @ ow.T + ob)
-    cache = dict(F=F, A1=A1, A2=A2, P=P, B1=B1, B2=B2, CAT=CAT, D1=D1, D2=D2, ws=ws, ow=ow, quant=quant)
+    layer = lambda k, x: q(relu(conv(x, ws[k]) + bs[k]))
+    c = dict(ws=ws, ow=ow, quant=quant)
+    c["F"] = x = q(np.asarray(feats, dt))
+    skips = []
+    for lvl in range(LEVELS):
+        if lvl > 0:
+            x = q(pool(skips[-1]))
+            c[f"P{lvl}"] = x
+        c[f"T{lvl}"] = t = layer(2 * lvl, x)
+        x = layer(2 * lvl + 1, t)
+        if lvl < LEVELS - 1:
+            skips.append(x)
+            c[f"S{lvl}"] = x
+    c["B"] = x
+    for lvl in range(LEVELS - 2, -1, -1):
+        k = 2 * LEVELS + 2 * (LEVELS - 2 - lvl)
+        cat = np.concatenate([skips[lvl], up(x)], axis=-1)
+        c[f"CAT{lvl}"] = cat
+        c[f"U{lvl}"] = u = layer(k, cat)
+        c[f"V{lvl}"] = x = layer(k + 1, u)
+    pred = np.asarray(noisy, dt) + (x @ ow.T + ob)
     loss = None
     if target is not None:
         diff = pred - np.asarray(target, dt)
-        loss = float(np.mean(diff.astype(np.float64) ** 2))
-        cache["g"] = diff * dt(2.0 / diff.size)
-    return pred, loss, cache
+        loss = float(np.mean(np.abs(diff.astype(np.float64))))
+        c["g"] = np.sign(diff) * dt(1.0 / diff.size)
+    return pred, loss, c
 
 
 def backward(c):
-    """Gradient of the loss w.r.t. the parameter vector (GPU order)."""
+    """Gradient of the L1 loss w.r.t. the parameter vector (GPU order)."""
     quant = c["quant"]
     q = bf16 if quant else (lambda a: a)
     ws, ow = c["ws"], c["ow"]
     m = lambda a: (a > 0).astype(a.dtype)
     g = c["g"]
-    gw = [None] * 6
-    gb = [None] * 6
-    gow = np.einsum("nhwc,nhwk->ck", g, c["D2"])
-    gob = g.sum(axis=(0, 1, 2))
-    gD2 = q(m(c["D2"]) * (g @ ow))
+    n_layers = len(LAYERS)
+    gw, gb = [None] * n_layers, [None] * n_layers
 
-    def layer(k, G, X):
+    def wgrad(k, G, X):
         gw[k] = conv_back_weights(G, X)
         gb[k] = G.sum(axis=(0, 1, 2))
 
-    layer(5, gD2, c["D1"])
-    gD1 = q(m(c["D1"]) * conv_back_data(gD2, ws[5]))
-    layer(4, gD1, c["CAT"])
-    gCAT = q(m(c["CAT"]) * conv_back_data(gD1, ws[4]))
-    u = gCAT[..., 32:]
-    gB2 = q(((u[:, 0::2, 0::2] + u[:, 0::2, 1::2]) + u[:, 1::2, 0::2]) + u[:, 1::2, 1::2])
-    layer(3, gB2, c["B1"])
-    gB1 = q(m(c["B1"]) * conv_back_data(gB2, ws[3]))
-    layer(2, gB1, c["P"])
-    gP = q(conv_back_data(gB1, ws[2]))
-    gA2 = q(gCAT[..., :32] + up(gP) * gP.dtype.type(0.25) * m(c["A2"]))
-    layer(1, gA2, c["A1"])
-    gA1 = q(m(c["A1"]) * conv_back_data(gA2, ws[1]))
-    layer(0, gA1, c["F"])
+    V0 = c["V0"]
+    gow = np.einsum("nhwc,nhwk->ck", g, V0)
+    gob = g.sum(axis=(0, 1, 2))
+    gx = q(m(V0) * (g @ ow))  # gradient at V_lvl (masked by its ReLU)
+    # decoder, from the output (level 0) to the deepest concatenation; the
+    # gradient of V_lvl (lvl > 0) is the 2x2 sum of the upsampled part of
+    # CAT_{lvl-1}'s gradient, which already carries V_lvl's ReLU mask
+    gcat = {}
+    for lvl in range(0, LEVELS - 1):
+        k = 2 * LEVELS + 2 * (LEVELS - 2 - lvl)
+        if lvl > 0:
+            gx = q(up_back(gcat[lvl - 1][..., ENC[lvl - 1]:]))
+        wgrad(k + 1, gx, c[f"U{lvl}"])
+        gu = q(m(c[f"U{lvl}"]) * conv_back_data(gx, ws[k + 1]))
+        wgrad(k, gu, c[f"CAT{lvl}"])
+        gcat[lvl] = q(m(c[f"CAT{lvl}"]) * conv_back_data(gu, ws[k]))
+    # encoder, from the bottleneck B up: the gradient of a skip tensor S is its
+    # part of the concatenation's gradient plus the pool's share, masked by S
+    gx = q(up_back(gcat[LEVELS - 2][..., ENC[LEVELS - 2]:]))  # gradient at B
+    for lvl in range(LEVELS - 1, -1, -1):
+        t = c[f"T{lvl}"]
+        wgrad(2 * lvl + 1, gx, t)
+        gt = q(m(t) * conv_back_data(gx, ws[2 * lvl + 1]))
+        xin = c["F"] if lvl == 0 else c[f"P{lvl}"]
+        wgrad(2 * lvl, gt, xin)
+        if lvl == 0:
+            break
+        gp = q(conv_back_data(gt, ws[2 * lvl]))
+        s = c[f"S{lvl - 1}"]
+        cs = ENC[lvl - 1]
+        gx = q(gcat[lvl - 1][..., :cs] + up(gp) * gp.dtype.type(0.25) * m(s))
     parts = []
-    for k in range(6):
+    for k in range(n_layers):
         parts += [gw[k].ravel(), gb[k].ravel()]
     parts += [gow.ravel(), gob.ravel()]
     return np.concatenate(parts)

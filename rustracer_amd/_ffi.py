@@ -47,6 +47,7 @@ FLAG_BINARY_RECORDS = 16
 FLAG_PATH_QUEUES = 32
 FLAG_TILE_LAUNCH = 64
 FLAG_GRAPH = 128
+FLAG_MOMENT_HISTORY = 256
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
               "accum64", "batch", "network")
 

@@ -155,6 +155,11 @@ struct DenoiseLaunch {
 hipError_t launch_denoise(const DenoiseLaunch &d, hipStream_t s);
 
 // fstate non-null: alpha comes from fstate->alpha
+// hist = hist * (1 - alpha) + moments.xy * alpha per pixel (the accumulation's
+// weighting, applied to the frame's luminance moments); fstate non-null: alpha
+// comes from fstate->alpha.
+hipError_t launch_moment_history(const void *moments, void *hist, uint32_t n_px, float alpha,
+                                 const FrameState *fstate, hipStream_t s);
 hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rgba8, uint32_t w, uint32_t h,
                                     float alpha, const rt_tonemap_uniform &tm, const void *gbuffer,
                                     const FrameState *fstate, hipStream_t s);

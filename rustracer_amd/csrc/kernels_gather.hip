@@ -1229,18 +1229,19 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
 
 // Workgroups of the persistent gather that are resident at once on the
 // current device (one 64-lane wavefront each).
+// The count is computed once per instance (a function-local static:
+// initialised once even when renderers on several threads launch first).
 template <int M, bool E, bool C, int NS>
 static uint32_t persistent_blocks() {
-    static uint32_t blocks = 0;
-    if (blocks == 0u) {
+    static const uint32_t blocks = [] {
         int per_cu = 0, dev = 0, cus = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtd::gather_persistent_kernel<M, E, C, NS>, 64, 0) !=
                 hipSuccess ||
             hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return 0u;
-        blocks = (uint32_t)std::max(per_cu, 1) * (uint32_t)std::max(cus, 1);
-    }
+        return (uint32_t)std::max(per_cu, 1) * (uint32_t)std::max(cus, 1);
+    }();
     return blocks;
 }
 

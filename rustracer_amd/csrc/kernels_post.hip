@@ -204,6 +204,19 @@ __global__ __launch_bounds__(256) void composite_tonemap_kernel(const float4 *__
     rgba8[i] = tonemap_pixel(a, i, width, tm, gbuf);
 }
 
+// Running luminance moments beside the accumulation, weighted as it is.
+__global__ __launch_bounds__(256) void moment_history_kernel(const float4 *__restrict__ moments,
+                                                             float2 *__restrict__ hist, uint32_t n_px, float alpha,
+                                                             const rtk::FrameState *fstate) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n_px) return;
+    if (fstate) alpha = fstate->alpha;
+    const float4 m = moments[i];
+    const float2 h = hist[i];
+    const float om = 1.0f - alpha;
+    hist[i] = make_float2(h.x * om + m.x * alpha, h.y * om + m.y * alpha);
+}
+
 // One lane stores a frame graph's per-frame values (vector store).
 __global__ void set_frame_state_kernel(rtk::FrameState *dst, rtk::FrameState v) {
     if (threadIdx.x == 0) *dst = v;
@@ -336,6 +349,15 @@ hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rg
     hipLaunchKernelGGL(rtd::composite_tonemap_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                        reinterpret_cast<const float4 *>(frame), reinterpret_cast<float4 *>(accum), rgba8, n, w, alpha,
                        tm, reinterpret_cast<const float4 *>(gbuffer), fstate);
+    return hipGetLastError();
+}
+
+hipError_t launch_moment_history(const void *moments, void *hist, uint32_t n_px, float alpha,
+                                 const FrameState *fstate, hipStream_t s) {
+    if (n_px == 0) return hipSuccess;
+    hipLaunchKernelGGL(rtd::moment_history_kernel, dim3((n_px + 255) / 256), dim3(256), 0, s,
+                       reinterpret_cast<const float4 *>(moments), reinterpret_cast<float2 *>(hist), n_px, alpha,
+                       fstate);
     return hipGetLastError();
 }
 

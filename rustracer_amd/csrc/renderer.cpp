@@ -138,6 +138,10 @@ struct rt_renderer {
     float *d_display = nullptr;
     float *d_albedo = nullptr;
     bool albedo_ok = false;
+    // running luminance moments (float2 per pixel), kept while a denoiser is
+    // attached or RT_FLAG_MOMENT_HISTORY is set; weight of the latest post pass
+    float *d_mom_hist = nullptr;
+    float post_alpha = 1.0f;
     uint64_t acc64_frames = 0;
 
     // many-view batch
@@ -198,7 +202,7 @@ struct rt_renderer {
         void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_gather_rec, d_bframe, d_bmom, d_bgb,
-                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo, d_fstate,
+                        d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo, d_mom_hist, d_fstate,
                         d_tex[0], d_tex[1], d_tex[2], d_tex[3], d_keys};
         for (void *b : bufs)
             if (b) hipFree(b);
@@ -335,8 +339,18 @@ struct rt_renderer {
         return 1.0f / (float)std::min(frames_since_reset, cfg.accumulate_frames);
     }
 
+    bool track_moments() const { return unet != nullptr || (cfg.flags & RT_FLAG_MOMENT_HISTORY) != 0; }
+    // The moment history, allocated zeroed (0 * history is then 0 at a frame
+    // with alpha = 1); before any frame graph that records its update.
+    void ensure_mom_hist() {
+        if (d_mom_hist || !track_moments()) return;
+        hip_check(hipMalloc(&d_mom_hist, n_px() * 8), "hipMalloc moment history");
+        hip_check(hipMemset(d_mom_hist, 0, n_px() * 8), "clear moment history");
+    }
+
     // Renderer::record_frame, lib.rs:666-851 (the present pass is not part of this backend)
     void render_frame() {
+        ensure_mom_hist();
         sample_counter = sample_counter + 1u;
         frames_since_reset = frames_since_reset == 0xFFFFFFFFu ? frames_since_reset : frames_since_reset + 1u;
         const float alpha = current_alpha();
@@ -385,6 +399,7 @@ struct rt_renderer {
         } else {
             post_passes(alpha, nullptr, stream, true);
         }
+        post_alpha = alpha;  // a replayed post graph runs no host code
         if (unet) {
             begin(RT_PASS_NETWORK, ev);
             ensure_albedo();
@@ -412,6 +427,7 @@ struct rt_renderer {
     // another), so the end of one frame's camera pass, where few long paths
     // are left, overlaps the start of the next frame's.
     void render_chunk(uint32_t nf) {
+        ensure_mom_hist();
         if (cfg.enable_photon_map && !photons_built)
             throw ApiError(RT_ERR_INVALID, "photon map enabled but rt_rebuild_photons has not run");
         if (nf < 2u || nf > (uint32_t)kChunk || next_set + nf > (uint32_t)kSets)
@@ -507,7 +523,11 @@ struct rt_renderer {
         hip_check(rtk::launch_composite_tonemap(result, d_accum, d_rgba8, cfg.width, cfg.height, alpha,
                                                 tonemap_uniform(), d_gbuf, fs, s),
                   "composite_tonemap");
+        if (track_moments() && d_mom_hist)
+            hip_check(rtk::launch_moment_history(d_moments, d_mom_hist, (uint32_t)n_px(), alpha, fs, s),
+                      "moment history");
         if (timed) end(RT_PASS_COMPOSITE_TONEMAP, ev, s);
+        post_alpha = alpha;
     }
 
     void sync_streams() {
@@ -598,9 +618,11 @@ struct rt_renderer {
         v.accum = d_accum;
         v.gbuf = d_gbuf;
         v.albedo = d_albedo;
+        v.moments = d_mom_hist;
         v.width = cfg.width;
         v.height = cfg.height;
         v.frames = std::min(frames_since_reset, cfg.accumulate_frames ? cfg.accumulate_frames : 1u);
+        v.ema = post_alpha / std::max(2.0f - post_alpha, 1e-4f);
         return v;
     }
 
@@ -1767,13 +1789,12 @@ int rt_renderer_set_denoiser(rt_renderer *r, rt_unet *u) {
     return api_call([&] {
         if (!r) throw ApiError(RT_ERR_INVALID, "null renderer");
         hip_check(hipStreamSynchronize(r->stream), "sync");
-        if (u && (r->cfg.width % 2 || r->cfg.height % 2))
-            throw ApiError(RT_ERR_INVALID, "the denoiser needs even width and height");
         if (u && !r->d_display) {
             r->d_display = dev_alloc<float>(r->n_px() * 4, "display");
             hip_check(hipMemset(r->d_display, 0, r->n_px() * 16), "memset");
         }
         r->unet = u;
+        r->ensure_mom_hist();
     });
 }
 

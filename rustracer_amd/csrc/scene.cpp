@@ -869,15 +869,19 @@ static std::unique_ptr<Scene> scene_training(uint64_t seed) {
         const float pick = rng.uniform();
         const uint32_t mat = add_material(*s, random_material(rng));
         const Vec3 off{rng.uniform(-2.5f, 2.5f), rng.uniform(0.2f, 1.2f), rng.uniform(-2.5f, 2.5f)};
+        // draws are named one per statement: the order of argument evaluation is unspecified
         if (pick < 0.4f) {
-            Mesh m = make_torus(rng.uniform(0.2f, 0.5f), rng.uniform(0.05f, 0.15f), 48, 24, 0.0f, nullptr);
-            transform_mesh(m, 1.0f, normalize(Vec3{rng.uniform(-1, 1), 1.0f, rng.uniform(-1, 1)}),
-                           rng.uniform(0.0f, 6.2831853f), off);
+            const float R = rng.uniform(0.2f, 0.5f);
+            const float r = rng.uniform(0.05f, 0.15f);
+            Mesh m = make_torus(R, r, 48, 24, 0.0f, nullptr);
+            const Vec3 axis = normalize(Vec3{rng.uniform(-1, 1), 1.0f, rng.uniform(-1, 1)});
+            const float angle = rng.uniform(0.0f, 6.2831853f);
+            transform_mesh(m, 1.0f, axis, angle, off);
             m.material_id = mat;
             s->meshes.push_back(std::move(m));
         } else if (pick < 0.75f) {
-            Mesh m = make_bevelled_box({rng.uniform(0.1f, 0.4f), rng.uniform(0.1f, 0.4f), rng.uniform(0.1f, 0.4f)},
-                                       0.04f, 8, 0.0f, nullptr);
+            const Vec3 half{rng.uniform(0.1f, 0.4f), rng.uniform(0.1f, 0.4f), rng.uniform(0.1f, 0.4f)};
+            Mesh m = make_bevelled_box(half, 0.04f, 8, 0.0f, nullptr);
             transform_mesh(m, 1.0f, normalize(Vec3{0.0f, 1.0f, 0.0f}), rng.uniform(0.0f, 6.2831853f), off);
             m.material_id = mat;
             s->meshes.push_back(std::move(m));
@@ -891,8 +895,9 @@ static std::unique_ptr<Scene> scene_training(uint64_t seed) {
         const float L = rng.log_uniform(1.0f, 50.0f);
         const Vec3 c{rng.uniform(-2, 2), rng.uniform(2.5f, 4.0f), rng.uniform(-2, 2)};
         const float h = rng.uniform(0.2f, 0.6f);
+        const Vec3 radiance{L, L * rng.uniform(0.8f, 1.0f), L * rng.uniform(0.7f, 1.0f)};
         add_area_light_quad(*s, {c.x - h, c.y, c.z - h}, {c.x + h, c.y, c.z - h}, {c.x + h, c.y, c.z + h},
-                            {c.x - h, c.y, c.z + h}, {L, L * rng.uniform(0.8f, 1.0f), L * rng.uniform(0.7f, 1.0f)});
+                            {c.x - h, c.y, c.z + h}, radiance);
     }
     add_environment(*s, {0.45f, 0.55f, 0.8f}, {0.9f, 0.88f, 0.85f}, rng.uniform(0.0f, 0.5f));
     return s;

@@ -8,13 +8,15 @@
 
 namespace rtu {
 
-// The renderer's device buffers a denoising pass reads (float4 per pixel).
+// The renderer's device buffers a denoising pass reads.
 struct FrameViews {
-    const void *accum;   // displayed linear image before denoising
-    const void *gbuf;    // normal.xyz, depth
-    const void *albedo;  // first-hit albedo (pixel-centre ray)
+    const void *accum;    // displayed linear image before denoising (float4 per pixel)
+    const void *gbuf;     // normal.xyz, depth (float4)
+    const void *albedo;   // first-hit albedo of the pixel-centre ray (float4)
+    const void *moments;  // accumulated luminance moments E[l], E[l^2] (float2)
     uint32_t width, height;
-    uint32_t frames;     // frames accumulated so far
+    uint32_t frames;      // frames accumulated so far
+    float ema;            // alpha / (2 - alpha) of the accumulation's current weight
 };
 
 // The renderer's current buffers, with the albedo pass brought up to date and

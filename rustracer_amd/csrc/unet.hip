@@ -1,17 +1,23 @@
-// unet.hip — the learned denoiser: a small convolutional U-Net on the displayed
+// unet.hip — the learned denoiser: a convolutional U-Net on the displayed
 // (accumulated) image, with hand-written BF16 MFMA convolution kernels for the
 // forward pass and both backward passes, FP32 accumulation, and Adam, for gfx950.
 //
 // This part has no counterpart in the reference renderer; its inputs are the
-// renderer's own buffers (accumulation, G-buffer) plus a first-hit albedo pass.
+// renderer's own buffers (accumulation, accumulated luminance moments,
+// G-buffer) plus a first-hit albedo pass.
 //
-// Network (NHWC bf16 activations, channel counts multiples of 16/32):
-//   F  (16: log1p rgb, albedo, normal, depth, sample count, 5 zero)
-//   A1 = relu(conv3x3(F, 16->32))           A2 = relu(conv3x3(A1, 32->32)) -> CAT[0:32]
-//   P  = avgpool2(A2)                       B1 = relu(conv3x3(P, 32->64))  B2 = relu(conv3x3(B1, 64->64))
-//   CAT[32:96] = upsample2(B2)              D1 = relu(conv3x3(CAT, 96->32)) D2 = relu(conv3x3(D1, 32->32))
-//   pred = log1p(noisy) + conv1x1(D2, 32->3) + b   (FP32), image = expm1(pred)
-// Loss: mean over pixels and rgb of (pred - log1p(target))^2.
+// Network (NHWC bf16 activations; image sizes padded to multiples of 16):
+//   F: 12 features (log1p rgb, albedo, normal, depth, 2 variances) + 4 zeros
+//   encoder level l = 0..4 (channels 32, 64, 128, 128, 128):
+//     T_l = relu(conv3x3(X_l)), S_l = relu(conv3x3(T_l)); X_0 = F,
+//     X_l = avgpool2(S_{l-1}); S_4 = B is the bottleneck
+//   decoder level l = 3..0 (channels 128, 128, 64, 32):
+//     CAT_l = [S_l, upsample2(V_{l+1})] (V_4 = B; 256, 256, 192, 96 channels)
+//     U_l = relu(conv3x3(CAT_l)), V_l = relu(conv3x3(U_l))
+//   pred = log1p(noisy) + conv1x1(V_0, 32->3) + b (FP32), image = expm1(pred)
+// Loss: mean over pixels and rgb of |pred - log1p(target)| (L1 on log radiance).
+// Every gradient is a fixed-order sum (per-workgroup partials, then a
+// reduction in index order), so a training run is reproducible bit for bit.
 //
 // Convolution as implicit GEMM on v_mfma_f32_32x32x16_bf16: for the forward
 // and backward-data passes M = 32 output channels, N = 32 consecutive pixels of
@@ -35,6 +41,7 @@
 #include "unet.h"
 
 namespace rtu {
+
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -74,126 +81,159 @@ __device__ __forceinline__ uint4 ld16(const bf16 *p) { return *reinterpret_cast<
 __device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
 
 
+
 // One 3x3 convolution (stride 1, zero padding), optionally with bias and ReLU,
 // or a mask (x > 0) of another tensor at the output position (the ReLU
 // derivative in the backward-data pass). X: [N][H][W][xs] with CIN channels at
-// offset 0; Wt: [9][cout][CIN]; Y: [N][H][W][ys], cout = 32 * NCO channels at
+// offset 0; Wt: [9][cout_total][CIN]; Y: [N][H][W][ys], cout_total channels at
 // offset yoff.
 //
-// A workgroup of four wavefronts takes 32 pixels of kConvRows consecutive rows;
-// the input rows y0-1 .. y0+kConvRows, pixels x0-1 .. x0+32 are staged once in
-// LDS (pixel stride CIN + 8 to spread banks) and serve all 9 taps; the layer's
-// weights are staged in LDS beside them (row stride CIN + 8), so the MFMA loop
-// waits on no global load. Both stagings issue up to eight 16-byte loads per
-// thread before writing any of them. Each wavefront computes
-// kConvRowsPerWave rows for all output channels. The weights are the MFMA's A
+// Work items are (tile, output-channel block): a tile is 32 pixels of
+// kConvRows consecutive rows, a block CB = 32 * NCO output channels. A
+// workgroup of four wavefronts computes one item at a time, each wavefront
+// kConvRowsPerWave rows for all CB channels; the weights are the MFMA's A
 // operand (row = output channel) and the pixels its B operand (column =
-// pixel), so every weight fragment, read once per tap and input-channel step,
-// feeds kConvRowsPerWave MFMAs, and each lane ends with 4
-// consecutive output channels of one pixel per accumulator group: the
-// epilogue stores 8 bytes at a time. The workgroups are persistent (as many as
-// are resident at once): each stages the weights once, then takes the tiles
-// blockIdx.x, blockIdx.x + gridDim.x, ... of the ceil(W/32) x
-// ceil(H/kConvRows) x N tiles.
+// pixel), so every weight fragment feeds kConvRowsPerWave MFMAs and each lane
+// ends with 4 consecutive output channels of one pixel per accumulator group
+// (8-byte epilogue stores).
+//
+// The reduction over 9 taps x CIN runs in CIN / KC slices of KC input
+// channels. Per slice, the input rows y0-1 .. y0+kConvRows, pixels x0-1 ..
+// x0+32 of those channels are staged in LDS (pixel stride KC + 8 to spread
+// banks) and serve all 9 taps, and so are the slice's weights [9][CB][KC + 8].
+// With WONCE (one slice and one block: the whole layer's weights fit) the
+// weights are staged once per workgroup and only the input tiles move. The
+// loads of the next (item, slice) step are issued into registers before the
+// current step's MFMAs and written to LDS after them, so their latency
+// overlaps the compute. The workgroups are persistent (as many as are
+// resident at once) and take items blockIdx.x, blockIdx.x + gridDim.x, ...;
+// the output-channel block varies fastest, so concurrent workgroups share
+// input tiles in L2.
 //
 // The inference pass folds the pool and the upsample into the staging of the
-// layers that read them (the training pass keeps P and CAT for its backward):
-// SRC_POOL reads X at twice the output resolution and stages the 2x2 average
-// of pool_kernel; SRC_CONCAT stages channels [0, c1) from X and [c1, CIN) from
-// X2 (half resolution, stride x2s) at (y/2, x/2), as upsample_kernel writes
-// them into CAT. The staged values are bit-identical to those kernels' outputs.
+// layers that read them (the training pass keeps the pooled and concatenated
+// tensors for its backward): SRC_POOL reads X at twice the output resolution
+// and stages the 2x2 average of pool_kernel; SRC_CONCAT stages channels [0, c1)
+// from X and [c1, CIN) from X2 (half resolution, stride x2s) at (y/2, x/2), as
+// upsample_kernel writes them into the concatenation. The staged values are
+// bit-identical to those kernels' outputs.
 constexpr int kConvRowsPerWave = 2;
 constexpr int kConvRows = 4 * kConvRowsPerWave;
 enum ConvSrc { SRC_PLAIN = 0, SRC_POOL = 1, SRC_CONCAT = 2 };
-template <int CIN, int NCO, int SRC = SRC_PLAIN>
+
+constexpr size_t conv_lds_bytes(int kc, int nco) {
+    return ((size_t)(kConvRows + 2) * 34 * (kc + 8) + (size_t)9 * 32 * nco * (kc + 8)) * 2;
+}
+
+template <int CIN, int KC, int NCO, int SRC, bool WONCE>
 __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X, int xs,
-                                                      const bf16 *__restrict__ Wt, const float *__restrict__ bias,
-                                                      bf16 *__restrict__ Y, int ys, int yoff, int relu,
-                                                      const bf16 *__restrict__ mask, int ms, int H, int W,
-                                                      const bf16 *__restrict__ X2 = nullptr, int x2s = 0,
-                                                      int c1 = 0, int N = 1) {
+                                                      const bf16 *__restrict__ Wt, int cout_total,
+                                                      const float *__restrict__ bias, bf16 *__restrict__ Y, int ys,
+                                                      int yoff, int relu, const bf16 *__restrict__ mask, int ms,
+                                                      int H, int W, const bf16 *__restrict__ X2, int x2s, int c1,
+                                                      int N) {
+    static_assert(CIN % KC == 0 && KC % 16 == 0, "input channels in slices of a multiple of 16");
+    static_assert(!WONCE || CIN == KC, "weights staged once: one slice");
+    static_assert(SRC != SRC_POOL || WONCE, "pooled staging only with the weights staged once");
     constexpr int R = kConvRowsPerWave;
-    constexpr int PS = CIN + 8;  // LDS pixel stride (elements)
-    constexpr int GROUPS = CIN / 8;
-    constexpr int COUT = 32 * NCO;
-    constexpr int WPS = CIN + 8;  // LDS weight row stride (elements)
-    // dynamic LDS: the input tile [kConvRows + 2][34][PS], then the layer's
-    // weights [9][COUT][WPS] (staged once, so the MFMA loop reads only LDS)
+    constexpr int NS = CIN / KC;
+    constexpr int PS = KC + 8;  // LDS pixel and weight-row stride (elements)
+    constexpr int GROUPS = KC / 8;
+    constexpr int CB = 32 * NCO;
+    constexpr int TITEMS = (kConvRows + 2) * 34 * GROUPS;
+    constexpr int TPER = (TITEMS + 255) / 256;
+    constexpr int WITEMS = 9 * CB * GROUPS;
+    constexpr int WPER = WONCE ? 1 : (WITEMS + 255) / 256;
+    constexpr int NL = SRC == SRC_POOL ? 4 : 1;
+    // dynamic LDS: the input slice [kConvRows + 2][34][PS], then the weight
+    // slice [9][CB][PS] (rows tap * CB + output channel)
     extern __shared__ __attribute__((aligned(16))) bf16 tile[];
     bf16 *wl = tile + (kConvRows + 2) * 34 * PS;
     const int l = threadIdx.x % 64, wv = threadIdx.x / 64;
     const int tiles_x = (W + 31) / 32, tiles_y = (H + kConvRows - 1) / kConvRows;
-    const int total = tiles_x * tiles_y * N;
-    {
-        constexpr int WITEMS = 9 * COUT * GROUPS;
-        constexpr int WPER = (WITEMS + 255) / 256;
+    const int ncb = cout_total / CB;
+    const int items = tiles_x * tiles_y * N * ncb;
+    // weight item i: row (tap * CB + j), 8-channel group g of the slice
+    auto w_src = [&](int i, int cb, int s) {
+        const int row = i / GROUPS, g = i % GROUPS;
+        const int tap = row / CB, j = row % CB;
+        return Wt + ((size_t)tap * cout_total + cb * CB + j) * CIN + s * KC + g * 8;
+    };
+    if (WONCE) {
         constexpr int WB = 8;
+        constexpr int WALL = (WITEMS + 255) / 256;
 #pragma unroll
-        for (int k0 = 0; k0 < WPER; k0 += WB) {
+        for (int k0 = 0; k0 < WALL; k0 += WB) {
             uint4 v[WB];
 #pragma unroll
             for (int k = 0; k < WB; k++) {
                 const int i = threadIdx.x + (k0 + k) * 256;
-                if (k0 + k < WPER && i < WITEMS) v[k] = ld16(Wt + (size_t)(i / GROUPS) * CIN + (i % GROUPS) * 8);
+                if (k0 + k < WALL && i < WITEMS) v[k] = ld16(w_src(i, 0, 0));
             }
 #pragma unroll
             for (int k = 0; k < WB; k++) {
                 const int i = threadIdx.x + (k0 + k) * 256;
-                if (k0 + k < WPER && i < WITEMS)
-                    *reinterpret_cast<uint4 *>(wl + (i / GROUPS) * WPS + (i % GROUPS) * 8) = v[k];
+                if (k0 + k < WALL && i < WITEMS)
+                    *reinterpret_cast<uint4 *>(wl + (i / GROUPS) * PS + (i % GROUPS) * 8) = v[k];
             }
         }
     }
-    // Tile staging: thread items threadIdx.x + 256 k. The loads of the next
-    // tile are issued into registers before the current tile's MFMAs and
-    // written to LDS after them, so their latency overlaps the compute.
-    constexpr int ITEMS = (kConvRows + 2) * 34 * GROUPS;
-    constexpr int PER = (ITEMS + 255) / 256;
-    constexpr int NL = SRC == SRC_POOL ? 4 : 1;
-    uint4 raw[PER][NL];
-    auto fetch = [&](int t) {
+    auto item_of = [&](int q) { return (int)blockIdx.x + (q / NS) * (int)gridDim.x; };
+    uint4 rt[TPER][NL];
+    uint4 rw[WPER];
+    auto fetch = [&](int q) {
+        const int item = item_of(q), s = q % NS;
+        const int cb = item % ncb, t = item / ncb;
         const int bx = t % tiles_x, rest = t / tiles_x;
         const int x0 = bx * 32, y0 = (rest % tiles_y) * kConvRows, n = rest / tiles_y;
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
+        for (int k = 0; k < TPER; k++) {
             const int i = threadIdx.x + k * 256;
-            const int g = i % GROUPS, q = i / GROUPS;
-            const int px = q % 34, row = q / 34;
+            const int g = i % GROUPS, qq = i / GROUPS;
+            const int px = qq % 34, row = qq / 34;
             const int yy = y0 - 1 + row, xx = x0 - 1 + px;
+            const int ch = s * KC + g * 8;
 #pragma unroll
-            for (int e = 0; e < NL; e++) raw[k][e] = make_uint4(0u, 0u, 0u, 0u);
-            if (i < ITEMS && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            for (int e = 0; e < NL; e++) rt[k][e] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < TITEMS && yy >= 0 && yy < H && xx >= 0 && xx < W) {
                 if (SRC == SRC_POOL) {
-                    const size_t p00 = ((size_t)(n * 2 * H + 2 * yy) * (2 * W) + 2 * xx) * xs + g * 8;
+                    const size_t p00 = ((size_t)(n * 2 * H + 2 * yy) * (2 * W) + 2 * xx) * xs + ch;
                     const size_t down = (size_t)(2 * W) * xs;
-                    raw[k][0] = ld16(X + p00);
-                    raw[k][NL > 1 ? 1 : 0] = ld16(X + p00 + xs);
-                    raw[k][NL > 2 ? 2 : 0] = ld16(X + p00 + down);
-                    raw[k][NL > 3 ? 3 : 0] = ld16(X + p00 + down + xs);
-                } else if (SRC == SRC_CONCAT && g * 8 >= c1) {
-                    raw[k][0] = ld16(X2 + ((size_t)(n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * x2s + (g * 8 - c1));
+                    rt[k][0] = ld16(X + p00);
+                    rt[k][NL > 1 ? 1 : 0] = ld16(X + p00 + xs);
+                    rt[k][NL > 2 ? 2 : 0] = ld16(X + p00 + down);
+                    rt[k][NL > 3 ? 3 : 0] = ld16(X + p00 + down + xs);
+                } else if (SRC == SRC_CONCAT && ch >= c1) {
+                    rt[k][0] = ld16(X2 + ((size_t)(n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * x2s + (ch - c1));
                 } else {
-                    raw[k][0] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + g * 8);
+                    rt[k][0] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + ch);
                 }
+            }
+        }
+        if (!WONCE) {
+#pragma unroll
+            for (int k = 0; k < WPER; k++) {
+                const int i = threadIdx.x + k * 256;
+                if (i < WITEMS) rw[k] = ld16(w_src(i, cb, s));
             }
         }
     };
     auto commit = [&]() {
 #pragma unroll
-        for (int k = 0; k < PER; k++) {
+        for (int k = 0; k < TPER; k++) {
             const int i = threadIdx.x + k * 256;
-            if (i < ITEMS) {
-                const int g = i % GROUPS, q = i / GROUPS;
-                const int px = q % 34, row = q / 34;
-                uint4 v = raw[k][0];
+            if (i < TITEMS) {
+                const int g = i % GROUPS, qq = i / GROUPS;
+                const int px = qq % 34, row = qq / 34;
+                uint4 v = rt[k][0];
                 if (SRC == SRC_POOL) {
                     // zero loads (outside the image) give the zero that the pool
                     // of a padded position is: 0.25 * (0 + 0 + 0 + 0) = 0
                     float a[8], b[8], d[8], e[8], o[8];
-                    unpack8(raw[k][0], a);
-                    unpack8(raw[k][NL > 1 ? 1 : 0], b);
-                    unpack8(raw[k][NL > 2 ? 2 : 0], d);
-                    unpack8(raw[k][NL > 3 ? 3 : 0], e);
+                    unpack8(rt[k][0], a);
+                    unpack8(rt[k][NL > 1 ? 1 : 0], b);
+                    unpack8(rt[k][NL > 2 ? 2 : 0], d);
+                    unpack8(rt[k][NL > 3 ? 3 : 0], e);
 #pragma unroll
                     for (int c8 = 0; c8 < 8; c8++) o[c8] = 0.25f * (((a[c8] + b[c8]) + d[c8]) + e[c8]);
                     v = pack8(o);
@@ -201,30 +241,40 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
                 *reinterpret_cast<uint4 *>(tile + (row * 34 + px) * PS + g * 8) = v;
             }
         }
+        if (!WONCE) {
+#pragma unroll
+            for (int k = 0; k < WPER; k++) {
+                const int i = threadIdx.x + k * 256;
+                if (i < WITEMS) *reinterpret_cast<uint4 *>(wl + (i / GROUPS) * PS + (i % GROUPS) * 8) = rw[k];
+            }
+        }
     };
-    int t = blockIdx.x;
-    if (t < total) fetch(t);
-    for (; t < total; t += gridDim.x) {
-        __syncthreads();  // every wavefront is done with the previous tile (and the weights are in)
+    floatx16 acc[R][NCO];
+    if (item_of(0) < items) fetch(0);
+    for (int q = 0; item_of(q) < items; q++) {
+        __syncthreads();  // every wavefront is done with the previous step (and the weights are in)
         commit();
         __syncthreads();
-        if (t + (int)gridDim.x < total) fetch(t + (int)gridDim.x);
+        if (item_of(q + 1) < items) fetch(q + 1);
+        const int item = item_of(q), s = q % NS;
+        const int cb = item % ncb, t = item / ncb;
         const int bx = t % tiles_x, rest = t / tiles_x;
         const int x0 = bx * 32, y0 = (rest % tiles_y) * kConvRows, n = rest / tiles_y;
         const int c = l % 32, kg = (l / 32) * 8;
-        floatx16 acc[R][NCO];
+        if (s == 0) {
 #pragma unroll
-        for (int r = 0; r < R; r++)
+            for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int j = 0; j < NCO; j++) acc[r][j] = floatx16{};
+                for (int j = 0; j < NCO; j++) acc[r][j] = floatx16{};
+        }
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
-            const bf16 *wp = wl + (tap * COUT + c) * WPS + kg;
+            const bf16 *wp = wl + (tap * CB + c) * PS + kg;
 #pragma unroll
-            for (int c0 = 0; c0 < CIN; c0 += 16) {
+            for (int c0 = 0; c0 < KC; c0 += 16) {
                 bf16x8 a[NCO], b[R];
 #pragma unroll
-                for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * WPS + c0);
+                for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * PS + c0);
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     b[r] = load8(tile + ((wv * R + r + tap / 3) * 34 + c + tap % 3) * PS + kg + c0);
@@ -235,6 +285,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
                         acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[r], acc[r][j], 0, 0, 0);
             }
         }
+        if (s != NS - 1) continue;
         const int x = x0 + c;
         if (x < W) {
 #pragma unroll
@@ -247,7 +298,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
 #pragma unroll
                     for (int g = 0; g < 4; g++) {
                         // accumulator elements 4g .. 4g+3: output channels co .. co+3 of pixel x
-                        const int co = j * 32 + g * 8 + (l / 32) * 4;
+                        const int co = cb * CB + j * 32 + g * 8 + (l / 32) * 4;
                         float v[4];
 #pragma unroll
                         for (int e = 0; e < 4; e++) v[e] = acc[r][j][4 * g + e];
@@ -278,14 +329,17 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
     }
 }
 
-// Weight gradient of a 3x3 convolution: gW[tap][co][ci] += sum over pixels of
+// Weight gradient of a 3x3 convolution: gW[tap][co][ci] = sum over pixels of
 // G[p][co] * X[p + tap][ci]. Each workgroup takes one (tap, 32 co, 32 ci) block
-// and a strided set of 32-pixel row segments; G and X segments are staged
-// transposed in LDS ([channel][pixel]) so MFMA fragments are 16-byte reads.
+// and the 32-pixel row segments blockIdx.x, blockIdx.x + gridDim.x, ...; G and
+// X segments are staged transposed in LDS ([channel][pixel]) so MFMA fragments
+// are 16-byte reads. The workgroup's sum goes to its own partial array
+// part[blockIdx.x][tap][co][ci] (no atomics): reduce_partials_kernel adds the
+// partials in a fixed order, so the gradient does not depend on scheduling.
 // Grid: (segment groups, 9, cout/32 * ceil(cin/32)).
 __global__ __launch_bounds__(64) void conv3x3_wgrad_kernel(const bf16 *__restrict__ G, int gs, int cout,
                                                            const bf16 *__restrict__ X, int xs, int cin,
-                                                           float *__restrict__ gW, int N, int H, int W) {
+                                                           float *__restrict__ part, int N, int H, int W) {
     __shared__ __attribute__((aligned(16))) bf16 gt[32][32];  // [co][pixel]
     __shared__ __attribute__((aligned(16))) bf16 xt[32][32];  // [ci][pixel]
     const int l = threadIdx.x;
@@ -341,20 +395,21 @@ __global__ __launch_bounds__(64) void conv3x3_wgrad_kernel(const bf16 *__restric
     }
     const int ci = ci0 + l % 32;
     if (ci >= cin) return;
+    float *pg = part + (size_t)blockIdx.x * 9 * cout * cin;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int co = co0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
-        atomicAdd(&gW[((size_t)tap * cout + co) * cin + ci], acc[e]);
+        pg[((size_t)tap * cout + co) * cin + ci] = acc[e];
     }
 }
 
-// Bias gradient: gb[c] += sum over pixels of G[p][c]. Thread t of a block
-// takes channel group t % (c/8) (16-byte loads) for pixels t / (c/8) + k *
-// lanes, grid-strided; the block then reduces its lanes in LDS and adds one
-// value per channel. c is a multiple of 8 and at most 256.
+// Bias gradient partials: part[blockIdx.x][c] = sum over this block's pixels
+// of G[p][c]. Thread t takes channel group t % (c/8) (16-byte loads) for pixels
+// t / (c/8) + k * lanes, grid-strided; the block then adds its lanes in a fixed
+// order. c is a multiple of 8 and at most 256.
 __global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ G, int gs, int c, size_t n_px,
-                                                        float *__restrict__ gb) {
-    __shared__ float part[256 * 8];
+                                                        float *__restrict__ part) {
+    __shared__ float red[256 * 8];
     const int groups = c / 8, lanes = 256 / groups;
     const int g = threadIdx.x % groups, lane = threadIdx.x / groups;
     float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -366,13 +421,24 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__
             for (int i = 0; i < 8; i++) acc[i] += v[i];
         }
 #pragma unroll
-    for (int i = 0; i < 8; i++) part[threadIdx.x * 8 + i] = acc[i];
+    for (int i = 0; i < 8; i++) red[threadIdx.x * 8 + i] = acc[i];
     __syncthreads();
     if ((int)threadIdx.x < c) {
         const int gg = threadIdx.x / 8, i = threadIdx.x % 8;
         float s = 0.0f;
-        for (int l = 0; l < lanes; l++) s += part[(l * groups + gg) * 8 + i];
-        atomicAdd(&gb[threadIdx.x], s);
+        for (int k = 0; k < lanes; k++) s += red[(k * groups + gg) * 8 + i];
+        part[(size_t)blockIdx.x * c + threadIdx.x] = s;
+    }
+}
+
+// out[i] = sum over g = 0 .. n_part-1 (in that order) of part[g * stride + i],
+// for i < n.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float *__restrict__ part, int n_part, size_t stride,
+                                                              size_t n, float *__restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        float s = 0.0f;
+        for (int g = 0; g < n_part; g++) s += part[(size_t)g * stride + i];
+        out[i] = s;
     }
 }
 
@@ -451,31 +517,31 @@ __global__ __launch_bounds__(256) void upsample_back_kernel(const bf16 *__restri
 
 // Output layer (1x1, 32 -> 3, FP32) plus the residual: pred = noisy + b + W·D.
 // With a target: loss partial sums, and the backward pass: gD = (W^T dpred) * (D > 0)
-// as bf16, gW += dpred D^T, gb += dpred (block reductions, then atomics).
+
+// Output layer (1x1, 32 -> 3, FP32) plus the residual: pred = noisy + b + W·D.
+// With a target, the L1 loss and the backward pass: dpred = grad_scale *
+// sign(pred - target), gD = (W^T dpred) * (D > 0) as bf16, and per block the
+// partial sums of gW = dpred D^T, gb = dpred and |pred - target| in
+// part[blockIdx.x][kOutParts] (fixed-order wavefront and block reductions, no
+// atomics; reduce_partials_kernel adds the blocks).
 constexpr int kOutC = 32;
+constexpr int kOutParts = 3 * kOutC + 3 + 1;  // gW, gb, loss
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__ D, const float *__restrict__ noisy,
                                                         const float *__restrict__ w, const float *__restrict__ b,
                                                         float *__restrict__ pred, const float *__restrict__ target,
                                                         size_t n_px, float grad_scale, bf16 *__restrict__ gd,
-                                                        float *__restrict__ gw, float *__restrict__ gbias,
-                                                        double *__restrict__ loss) {
-    __shared__ float red[256];
-    __shared__ float gacc[3 * kOutC + 3];
-    for (int i = threadIdx.x; i < 3 * kOutC + 3; i += 256) gacc[i] = 0.0f;
-    __syncthreads();
-    float lsum = 0.0f;
-    float lw[TRAIN ? 3 * kOutC + 3 : 1];
+                                                        float *__restrict__ part) {
+    __shared__ float red[4][kOutParts];
+    float lw[TRAIN ? kOutParts : 1];
     if (TRAIN)
-        for (int i = 0; i < 3 * kOutC + 3; i++) lw[i] = 0.0f;
+        for (int i = 0; i < kOutParts; i++) lw[i] = 0.0f;
     for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < n_px; p += (size_t)gridDim.x * 256) {
         float d[kOutC];
 #pragma unroll
         for (int i = 0; i < kOutC; i += 8) {
             const uint4 v = *reinterpret_cast<const uint4 *>(D + p * kOutC + i);
-            const bf16 *e = reinterpret_cast<const bf16 *>(&v);
-#pragma unroll
-            for (int j = 0; j < 8; j++) d[i + j] = bf2f(e[j]);
+            unpack8(v, d + i);
         }
         float o[3];
 #pragma unroll
@@ -491,13 +557,13 @@ __global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const float diff = o[c] - target[p * 3 + c];
-                lsum += diff * diff;
-                g[c] = grad_scale * diff;
+                lw[kOutParts - 1] += fabsf(diff);
+                g[c] = diff > 0.0f ? grad_scale : (diff < 0.0f ? -grad_scale : 0.0f);
                 lw[3 * kOutC + c] += g[c];
             }
 #pragma unroll
             for (int i = 0; i < kOutC; i += 8) {
-                uint32_t packed[4];
+                float h[8];
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const int k = i + j;
@@ -507,48 +573,73 @@ __global__ __launch_bounds__(256) void out_layer_kernel(const bf16 *__restrict__
                         s += w[c * kOutC + k] * g[c];
                         lw[c * kOutC + k] += g[c] * d[k];
                     }
-                    const uint32_t h = f2bf(d[k] > 0.0f ? s : 0.0f);
-                    if (j % 2 == 0) packed[j / 2] = h;
-                    else packed[j / 2] |= h << 16;
+                    h[j] = d[k] > 0.0f ? s : 0.0f;
                 }
-                *reinterpret_cast<uint4 *>(gd + p * kOutC + i) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+                *reinterpret_cast<uint4 *>(gd + p * kOutC + i) = pack8(h);
             }
         }
     }
     if (!TRAIN) return;
-    for (int i = 0; i < 3 * kOutC + 3; i++) atomicAdd(&gacc[i], lw[i]);
-    red[threadIdx.x] = lsum;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    for (int i = 0; i < kOutParts; i++) {
+        float v = lw[i];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[wv][i] = v;
     }
-    if (threadIdx.x == 0) atomicAdd(loss, (double)red[0]);
-    for (int i = threadIdx.x; i < 3 * kOutC; i += 256) atomicAdd(&gw[i], gacc[i]);
-    if (threadIdx.x < 3) atomicAdd(&gbias[threadIdx.x], gacc[3 * kOutC + threadIdx.x]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kOutParts; i += 256)
+        part[(size_t)blockIdx.x * kOutParts + i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
 }
 
-// Features from the renderer's buffers: accumulation (float4), G-buffer
-// (normal, depth), albedo (float4), sample count. F: 16 bf16 per pixel; noisy:
-// log1p of the accumulation (FP32 rgb).
+// Features of the network from the renderer's buffers, on a padded image of
+// pw x ph (multiples of 16) whose pixels past the w x h frame repeat its last
+// row and column. F: 16 bf16 per pixel: log1p rgb of the accumulation,
+// first-hit albedo, G-buffer normal and depth, log1p of the luminance variance
+// of the samples (E[l^2] - E[l]^2 of the accumulated moments) and log1p of that
+// variance times the accumulation's attenuation alpha/(2 - alpha) (the
+// residual variance of the mean; it falls as frames add up), then 4 zeros.
+// noisy: log1p of the accumulation (FP32 rgb).
 __global__ __launch_bounds__(256) void features_kernel(const float4 *__restrict__ accum, const float4 *__restrict__ gbuf,
-                                                       const float4 *__restrict__ albedo, float frames_feature,
-                                                       size_t n_px, bf16 *__restrict__ F, float *__restrict__ noisy) {
+                                                       const float4 *__restrict__ albedo, const float2 *__restrict__ mom,
+                                                       float ema, uint32_t w, uint32_t h, uint32_t pw, uint32_t ph,
+                                                       bf16 *__restrict__ F, float *__restrict__ noisy) {
     const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n_px) return;
-    const float4 a = accum[p], g = gbuf[p], al = albedo[p];
+    if (p >= (size_t)pw * ph) return;
+    const uint32_t x = (uint32_t)(p % pw), y = (uint32_t)(p / pw);
+    const size_t src = (size_t)min(y, h - 1u) * w + min(x, w - 1u);
+    const float4 a = accum[src], g = gbuf[src], al = albedo[src];
+    const float2 m = mom[src];
     const float lr = log1pf(fmaxf(a.x, 0.0f)), lg = log1pf(fmaxf(a.y, 0.0f)), lb = log1pf(fmaxf(a.z, 0.0f));
     noisy[p * 3 + 0] = lr;
     noisy[p * 3 + 1] = lg;
     noisy[p * 3 + 2] = lb;
     const float depth = g.w < 1e29f ? log1pf(fminf(fmaxf(g.w, 0.0f), 1e4f)) * 0.25f : -1.0f;
-    const float f[16] = {lr, lg, lb, al.x, al.y, al.z, g.x, g.y, g.z, depth, frames_feature, 0, 0, 0, 0, 0};
-    uint32_t o[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) o[i] = (uint32_t)f2bf(f[2 * i]) | ((uint32_t)f2bf(f[2 * i + 1]) << 16);
+    const float var = fmaxf(m.y - m.x * m.x, 0.0f);
+    const float f[16] = {lr, lg, lb, al.x, al.y, al.z, g.x, g.y, g.z, depth, log1pf(var), log1pf(var * ema),
+                         0.0f, 0.0f, 0.0f, 0.0f};
     uint4 *dst = reinterpret_cast<uint4 *>(F + p * 16);
-    dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-    dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    dst[0] = pack8(f);
+    dst[1] = pack8(f + 8);
+}
+
+// The displayed image: expm1 of the prediction over the w x h frame of the
+// padded pw-wide prediction.
+__global__ __launch_bounds__(256) void image_kernel(const float *__restrict__ pred, uint32_t w, uint32_t h,
+                                                    uint32_t pw, float4 *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (size_t)w * h) return;
+    const size_t p = (i / w) * pw + i % w;
+    out[i] = make_float4(expm1f(pred[p * 3]), expm1f(pred[p * 3 + 1]), expm1f(pred[p * 3 + 2]), 1.0f);
+}
+
+// The training loss of one step: the sum of the out-layer blocks' |diff|
+// partials (in block order, in double) over the element count.
+__global__ void loss_kernel(const float *__restrict__ part, int n_part, double inv_count, double *__restrict__ loss) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int g = 0; g < n_part; g++) s += (double)part[(size_t)g * kOutParts + kOutParts - 1];
+    *loss = s * inv_count;
 }
 
 // log1p of a float4 image's rgb (training targets).
@@ -560,12 +651,6 @@ __global__ __launch_bounds__(256) void log_target_kernel(const float4 *__restric
     out[p * 3 + 0] = log1pf(fmaxf(a.x, 0.0f));
     out[p * 3 + 1] = log1pf(fmaxf(a.y, 0.0f));
     out[p * 3 + 2] = log1pf(fmaxf(a.z, 0.0f));
-}
-
-__global__ __launch_bounds__(256) void image_kernel(const float *__restrict__ pred, size_t n_px, float4 *__restrict__ out) {
-    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= n_px) return;
-    out[p] = make_float4(expm1f(pred[p * 3]), expm1f(pred[p * 3 + 1]), expm1f(pred[p * 3 + 2]), 1.0f);
 }
 
 // Crops of the training set into a batch: sample k of the batch is the
@@ -629,10 +714,6 @@ __global__ __launch_bounds__(256) void weights_bf16_kernel(const float *__restri
     }
 }
 
-__global__ __launch_bounds__(256) void zero_kernel(float *__restrict__ p, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0.0f;
-}
-
 }  // namespace rtu
 
 // ─────────────────────────────── host side ───────────────────────────────
@@ -666,58 +747,114 @@ struct ConvLayer {
     int cin = 0, cout = 0;
     size_t nw() const { return (size_t)9 * cin * cout; }
 };
-// conv1..conv6 of the network described at the top of this file
-const ConvLayer kLayers[6] = {{16, 32}, {32, 32}, {32, 64}, {64, 64}, {96, 32}, {32, 32}};
+// The network described at the top of this file: levels 0..4 of the encoder
+// (layers 2l, 2l+1), then decoder levels 3..0 (layers 10 + 2(3 - l), +1).
+constexpr int kLevels = 5;
+constexpr int kNumLayers = 18;
 constexpr int kFeat = 16;
+constexpr int kEnc[kLevels] = {32, 64, 128, 128, 128};
+constexpr int kDec[kLevels - 1] = {32, 64, 128, 128};
+// channels of the concatenation at decoder level l: skip, then the upsampled
+// output of the level below (the bottleneck for level 3)
+constexpr int cat_up(int l) { return l == kLevels - 2 ? kEnc[kLevels - 1] : kDec[l + 1]; }
+constexpr int cat_ch(int l) { return kEnc[l] + cat_up(l); }
+constexpr int dec_layer(int l) { return 2 * kLevels + 2 * (kLevels - 2 - l); }
+const ConvLayer kLayers[kNumLayers] = {{kFeat, 32}, {32, 32}, {32, 64}, {64, 64}, {64, 128}, {128, 128},
+                                       {128, 128}, {128, 128}, {128, 128}, {128, 128}, {256, 128}, {128, 128},
+                                       {256, 128}, {128, 128}, {192, 64}, {64, 64}, {96, 32}, {32, 32}};
+// image sizes are padded to a multiple of this (four 2x2 pools)
+constexpr uint32_t kPadTo = 1u << (kLevels - 1);
 
-// Activations (and their gradients) of one batch shape.
+// Activations (and their gradients) of one batch shape; H and W multiples of kPadTo.
 struct Acts {
     int N = 0, H = 0, W = 0;
-    size_t px() const { return (size_t)N * H * W; }
-    rtu::bf16 *F = nullptr, *A1 = nullptr, *CAT = nullptr, *P = nullptr, *B1 = nullptr, *B2 = nullptr, *D1 = nullptr,
-              *D2 = nullptr;
+    int h(int l) const { return H >> l; }
+    int w(int l) const { return W >> l; }
+    size_t px(int l = 0) const { return (size_t)N * h(l) * w(l); }
+    rtu::bf16 *F = nullptr, *B = nullptr;
+    rtu::bf16 *T[kLevels] = {}, *P[kLevels] = {}, *CAT[kLevels - 1] = {}, *U[kLevels - 1] = {},
+              *V[kLevels - 1] = {};
     float *noisy = nullptr, *target = nullptr, *pred = nullptr;
     // gradients
-    rtu::bf16 *gD2 = nullptr, *gD1 = nullptr, *gCAT = nullptr, *gB2 = nullptr, *gB1 = nullptr, *gP = nullptr,
-              *gA2 = nullptr, *gA1 = nullptr;
+    rtu::bf16 *gB = nullptr;
+    rtu::bf16 *gT[kLevels] = {}, *gP[kLevels] = {}, *gS[kLevels - 1] = {}, *gCAT[kLevels - 1] = {},
+              *gU[kLevels - 1] = {}, *gV[kLevels - 1] = {};
+    float *part = nullptr;  // partial sums of the weight/bias/out-layer gradients
+    size_t part_floats = 0;
     bool grads = false;
+    std::vector<void *> owned;
+
     void release() {
-        void *all[] = {F, A1, CAT, P, B1, B2, D1, D2, noisy, target, pred, gD2, gD1, gCAT, gB2, gB1, gP, gA2, gA1};
-        for (void *p : all)
-            if (p) (void)hipFree(p);
+        for (void *p : owned) (void)hipFree(p);
         *this = Acts{};
     }
-    void ensure(int n, int h, int w, bool with_grads) {
-        if (n == N && h == H && w == W && (grads || !with_grads)) return;
+    rtu::bf16 *act(size_t n, const char *what) {
+        rtu::bf16 *p = dalloc<rtu::bf16>(n, what);
+        owned.push_back(p);
+        return p;
+    }
+    void ensure(int n, int hh, int ww, bool with_grads, size_t part_need) {
+        if (n == N && hh == H && ww == W && (grads || !with_grads) && part_floats >= part_need) return;
         release();
         N = n;
-        H = h;
-        W = w;
+        H = hh;
+        W = ww;
         grads = with_grads;
-        const size_t P1 = px(), P2 = P1 / 4;
-        F = dalloc<rtu::bf16>(P1 * 16, "unet F");
-        A1 = dalloc<rtu::bf16>(P1 * 32, "unet A1");
-        CAT = dalloc<rtu::bf16>(P1 * 96, "unet CAT");
-        P = dalloc<rtu::bf16>(P2 * 32, "unet P");
-        B1 = dalloc<rtu::bf16>(P2 * 64, "unet B1");
-        B2 = dalloc<rtu::bf16>(P2 * 64, "unet B2");
-        D1 = dalloc<rtu::bf16>(P1 * 32, "unet D1");
-        D2 = dalloc<rtu::bf16>(P1 * 32, "unet D2");
-        noisy = dalloc<float>(P1 * 3, "unet noisy");
-        target = dalloc<float>(P1 * 3, "unet target");
-        pred = dalloc<float>(P1 * 3, "unet pred");
+        F = act(px(0) * kFeat, "unet F");
+        for (int l = 0; l < kLevels; l++) {
+            T[l] = act(px(l) * kEnc[l], "unet T");
+            if (l > 0) P[l] = act(px(l) * kEnc[l - 1], "unet P");
+            if (l < kLevels - 1) {
+                CAT[l] = act(px(l) * cat_ch(l), "unet CAT");
+                U[l] = act(px(l) * kDec[l], "unet U");
+                V[l] = act(px(l) * kDec[l], "unet V");
+            }
+        }
+        B = act(px(kLevels - 1) * kEnc[kLevels - 1], "unet B");
+        for (float **f : {&noisy, &target, &pred}) {
+            *f = dalloc<float>(px(0) * 3, "unet rgb");
+            owned.push_back(*f);
+        }
         if (with_grads) {
-            gD2 = dalloc<rtu::bf16>(P1 * 32, "unet gD2");
-            gD1 = dalloc<rtu::bf16>(P1 * 32, "unet gD1");
-            gCAT = dalloc<rtu::bf16>(P1 * 96, "unet gCAT");
-            gB2 = dalloc<rtu::bf16>(P2 * 64, "unet gB2");
-            gB1 = dalloc<rtu::bf16>(P2 * 64, "unet gB1");
-            gP = dalloc<rtu::bf16>(P2 * 32, "unet gP");
-            gA2 = dalloc<rtu::bf16>(P1 * 32, "unet gA2");
-            gA1 = dalloc<rtu::bf16>(P1 * 32, "unet gA1");
+            gB = act(px(kLevels - 1) * kEnc[kLevels - 1], "unet gB");
+            for (int l = 0; l < kLevels; l++) {
+                gT[l] = act(px(l) * kEnc[l], "unet gT");
+                if (l > 0) gP[l] = act(px(l) * kEnc[l - 1], "unet gP");
+                if (l < kLevels - 1) {
+                    gS[l] = act(px(l) * kEnc[l], "unet gS");
+                    gCAT[l] = act(px(l) * cat_ch(l), "unet gCAT");
+                    gU[l] = act(px(l) * kDec[l], "unet gU");
+                    gV[l] = act(px(l) * kDec[l], "unet gV");
+                }
+            }
+            part = dalloc<float>(part_need, "unet gradient partials");
+            owned.push_back(part);
+            part_floats = part_need;
         }
     }
 };
+
+// Partial arrays of a layer's weight gradient: enough workgroups to fill the
+// GPU (about 2048 of one wavefront), at most one per row segment, at most 256.
+int wgrad_parts(const ConvLayer &L, int N, int H, int W) {
+    const int nblk = (L.cout / 32) * ((L.cin + 31) / 32);
+    const int segs = N * H * ((W + 31) / 32);
+    return std::max(1, std::min({2048 / (9 * nblk), segs, 256}));
+}
+constexpr int kBiasParts = 256;
+constexpr int kOutBlocks = 1024;
+
+// Gradient partial floats a training pass of this shape needs.
+size_t part_floats_for(int N, int H, int W) {
+    size_t need = (size_t)kOutBlocks * rtu::kOutParts;
+    for (int k = 0; k < kNumLayers; k++) {
+        const int l = k < 2 * kLevels ? k / 2 : (kLevels - 2) - (k - 2 * kLevels) / 2;
+        const int parts = wgrad_parts(kLayers[k], N, H >> l, W >> l);
+        need = std::max(need, (size_t)parts * kLayers[k].nw());
+        need = std::max(need, (size_t)kBiasParts * kLayers[k].cout);
+    }
+    return need;
+}
 
 }  // namespace
 
@@ -727,9 +864,9 @@ struct rt_unet {
     // FP32 master parameters: conv weights [9][cout][cin] and biases per layer,
     // then the output layer's w[3][32] and b[3]; one contiguous array.
     size_t n_params = 0;
-    size_t w_off[6] = {}, b_off[6] = {}, ow_off = 0, ob_off = 0;
+    size_t w_off[kNumLayers] = {}, b_off[kNumLayers] = {}, ow_off = 0, ob_off = 0;
     float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
-    rtu::bf16 *d_wf[6] = {}, *d_wt[6] = {};
+    rtu::bf16 *d_wf[kNumLayers] = {}, *d_wt[kNumLayers] = {};
     uint64_t adam_t = 0;
     Acts inf, train;
     // training set
@@ -749,7 +886,7 @@ struct rt_unet {
         void *all[] = {d_p, d_g, d_m, d_v, ds_F, ds_noisy, ds_target, d_sel, d_loss};
         for (void *p : all)
             if (p) (void)hipFree(p);
-        for (int k = 0; k < 6; k++) {
+        for (int k = 0; k < kNumLayers; k++) {
             if (d_wf[k]) (void)hipFree(d_wf[k]);
             if (d_wt[k]) (void)hipFree(d_wt[k]);
         }
@@ -757,7 +894,7 @@ struct rt_unet {
     }
 
     void refresh_bf16() {
-        for (int k = 0; k < 6; k++) {
+        for (int k = 0; k < kNumLayers; k++) {
             const ConvLayer &L = kLayers[k];
             hipLaunchKernelGGL(rtu::weights_bf16_kernel, dim3(blocks_for(L.nw())), dim3(256), 0, stream,
                                d_p + w_off[k], L.nw(), d_wf[k], d_wt[k], L.cout, L.cin);
@@ -765,117 +902,187 @@ struct rt_unet {
         hip_check(hipGetLastError(), "weights to bf16");
     }
 
-    // src: rtu::SRC_POOL (X at 2H x 2W, pooled while staged) or rtu::SRC_CONCAT
-    // (channels from c1 on read from X2, half resolution) for the inference pass.
-    void conv(const rtu::bf16 *X, int xs, int k, bool bwd, rtu::bf16 *Y, int ys, int yoff, bool relu,
-              const rtu::bf16 *mask, int ms, int N, int H, int W, int src = rtu::SRC_PLAIN,
-              const rtu::bf16 *X2 = nullptr, int x2s = 0, int c1 = 0) {
-        const ConvLayer &L = kLayers[k];
-        const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
-        const int tiles = ((W + 31) / 32) * ((H + rtu::kConvRows - 1) / rtu::kConvRows) * N;
-        // the input tile, then the weights [9][cout][cin + 8] (bf16)
-        const size_t lds = (size_t)(rtu::kConvRows + 2) * 34 * (cin + 8) * 2 + (size_t)9 * cout * (cin + 8) * 2;
-        const rtu::bf16 *w = bwd ? d_wt[k] : d_wf[k];
-        const float *b = bwd ? nullptr : d_p + b_off[k];
-        // the channel pairs of the six layers, forward (cin, cout) and backward (cout, cin)
-#define RT_CONV_SRC(CI, NCO, SRC)                                                                                \
-    do {                                                                                                         \
-        /* once per instance: raise the dynamic LDS limit (above 64 KB) and size the persistent grid */          \
-        static int resident = 0;                                                                                 \
-        if (resident == 0) {                                                                                     \
-            if (lds > 65536)                                                                                     \
-                hip_check(hipFuncSetAttribute((const void *)rtu::conv3x3_kernel<CI, NCO, SRC>,                   \
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),             \
-                          "conv LDS size");                                                                      \
-            int per_cu = 0;                                                                                      \
-            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtu::conv3x3_kernel<CI, NCO, SRC>,   \
-                                                                   256, lds),                                    \
-                      "conv occupancy");                                                                         \
-            resident = std::max(per_cu, 1) * cu_count();                                                         \
-        }                                                                                                        \
-        const dim3 grid((unsigned)std::max(1, std::min(tiles, resident)));                                       \
-        hipLaunchKernelGGL((rtu::conv3x3_kernel<CI, NCO, SRC>), grid, dim3(256), lds, stream, X, xs, w, b, Y,   \
-                           ys, yoff, relu ? 1 : 0, mask, ms, H, W, X2, x2s, c1, N);                              \
-    } while (0)
-#define RT_CONV(CI, NCO) RT_CONV_SRC(CI, NCO, rtu::SRC_PLAIN)
-        if (src == rtu::SRC_POOL) {
-            if (cin == 32 && cout == 64) RT_CONV_SRC(32, 2, rtu::SRC_POOL);
-            else throw ApiError(RT_ERR_INVALID, "unsupported pooled convolution");
-        } else if (src == rtu::SRC_CONCAT) {
-            if (cin == 96 && cout == 32) RT_CONV_SRC(96, 1, rtu::SRC_CONCAT);
-            else throw ApiError(RT_ERR_INVALID, "unsupported concatenated convolution");
-        } else if (cin == 16 && cout == 32) RT_CONV(16, 1);
-        else if (cin == 32 && cout == 32) RT_CONV(32, 1);
-        else if (cin == 32 && cout == 64) RT_CONV(32, 2);
-        else if (cin == 64 && cout == 64) RT_CONV(64, 2);
-        else if (cin == 96 && cout == 32) RT_CONV(96, 1);
-        else if (cin == 32 && cout == 96) RT_CONV(32, 3);
-        else if (cin == 64 && cout == 32) RT_CONV(64, 1);
-        else throw ApiError(RT_ERR_INVALID, "unsupported channel count");
-#undef RT_CONV
-#undef RT_CONV_SRC
-    }
-    void wgrad(const rtu::bf16 *G, int k, const rtu::bf16 *X, int xs, int N, int H, int W) {
-        const ConvLayer &L = kLayers[k];
-        const int segs = N * H * ((W + 31) / 32);
-        const dim3 grid((unsigned)std::min(segs, 384), 9, (L.cout / 32) * ((L.cin + 31) / 32));
-        hipLaunchKernelGGL(rtu::conv3x3_wgrad_kernel, grid, dim3(64), 0, stream, G, L.cout, L.cout, X, xs, L.cin,
-                           d_g + w_off[k], N, H, W);
-        hipLaunchKernelGGL(rtu::bias_grad_kernel, dim3(512), dim3(256), 0, stream, G, L.cout, L.cout,
-                           (size_t)N * H * W, d_g + b_off[k]);
+    template <int CIN, int KC, int NCO, int SRC, bool WONCE>
+    void launch_conv(const rtu::bf16 *X, int xs, const rtu::bf16 *w, int cout, const float *b, rtu::bf16 *Y, int ys,
+                     int yoff, bool relu, const rtu::bf16 *mask, int ms, int N, int H, int W, const rtu::bf16 *X2,
+                     int x2s, int c1) {
+        constexpr size_t lds = rtu::conv_lds_bytes(KC, NCO);
+        if (cout % (32 * NCO) || (WONCE && cout != 32 * NCO))
+            throw ApiError(RT_ERR_INVALID, "convolution output channels do not match its blocking");
+        // once per instance (a function-local static, initialised once even
+        // across threads): raise the dynamic LDS limit (above 64 KB) and size
+        // the persistent grid
+        static const int resident = [] {
+            const void *fn = (const void *)rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>;
+            if (lds > 65536)
+                hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                          "conv LDS size");
+            int per_cu = 0;
+            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>,
+                                                                   256, lds),
+                      "conv occupancy");
+            return std::max(per_cu, 1) * cu_count();
+        }();
+        const int items = ((W + 31) / 32) * ((H + rtu::kConvRows - 1) / rtu::kConvRows) * N * (cout / (32 * NCO));
+        const dim3 grid((unsigned)std::max(1, std::min(items, resident)));
+        hipLaunchKernelGGL((rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>), grid, dim3(256), lds, stream, X, xs, w,
+                           cout, b, Y, ys, yoff, relu ? 1 : 0, mask, ms, H, W, X2, x2s, c1, N);
     }
 
-    // Forward pass over a.F / a.noisy; with `target` the loss goes to loss_slot
-    // and the output layer's backward pass fills a.gD2 and the output-layer grads.
+    // Layer k forward (bias, ReLU) or backward-data (bwd: transposed weights,
+    // no bias, mask = the ReLU derivative of the layer's input, or none).
+    // src: rtu::SRC_POOL (X at 2H x 2W, pooled while staged) or rtu::SRC_CONCAT
+    // (channels from c1 on read from X2, half resolution) for the inference pass.
+    void conv(int k, bool bwd, const rtu::bf16 *X, int xs, rtu::bf16 *Y, int ys, int yoff, const rtu::bf16 *mask,
+              int ms, int N, int H, int W, int src = rtu::SRC_PLAIN, const rtu::bf16 *X2 = nullptr, int x2s = 0,
+              int c1 = 0) {
+        const ConvLayer &L = kLayers[k];
+        const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
+        const rtu::bf16 *w = bwd ? d_wt[k] : d_wf[k];
+        const float *b = bwd ? nullptr : d_p + b_off[k];
+        const bool relu = !bwd;
+        using namespace rtu;
+#define RT_CONV(CI, KC, NCO, SRC, ONCE)                                                                          \
+    launch_conv<CI, KC, NCO, SRC, ONCE>(X, xs, w, cout, b, Y, ys, yoff, relu, mask, ms, N, H, W, X2, x2s, c1)
+        if (src == SRC_POOL) {
+            if (cin == 32 && cout == 64) RT_CONV(32, 32, 2, SRC_POOL, true);
+            else throw ApiError(RT_ERR_INVALID, "unsupported pooled convolution");
+        } else if (src == SRC_CONCAT) {
+            if (cin == 96 && cout == 32) RT_CONV(96, 96, 1, SRC_CONCAT, true);
+            else if (cin == 192 && cout == 64) RT_CONV(192, 32, 2, SRC_CONCAT, false);
+            else if (cin == 256 && cout == 128) RT_CONV(256, 32, 4, SRC_CONCAT, false);
+            else throw ApiError(RT_ERR_INVALID, "unsupported concatenated convolution");
+        } else if (cin == 16 && cout == 32) RT_CONV(16, 16, 1, SRC_PLAIN, true);
+        else if (cin == 32 && cout == 32) RT_CONV(32, 32, 1, SRC_PLAIN, true);
+        else if (cin == 32 && cout == 64) RT_CONV(32, 32, 2, SRC_PLAIN, true);
+        else if (cin == 32 && cout == 96) RT_CONV(32, 32, 3, SRC_PLAIN, true);
+        else if (cin == 64 && cout == 32) RT_CONV(64, 64, 1, SRC_PLAIN, true);
+        else if (cin == 64 && cout == 64) RT_CONV(64, 64, 2, SRC_PLAIN, true);
+        else if (cin == 64 && cout == 128) RT_CONV(64, 32, 4, SRC_PLAIN, false);
+        else if (cin == 64 && cout == 192) RT_CONV(64, 32, 2, SRC_PLAIN, false);
+        else if (cin == 96 && cout == 32) RT_CONV(96, 96, 1, SRC_PLAIN, true);
+        else if (cin == 128 && cout == 64) RT_CONV(128, 32, 2, SRC_PLAIN, false);
+        else if (cin == 128 && (cout == 128 || cout == 256)) RT_CONV(128, 32, 4, SRC_PLAIN, false);
+        else if (cin == 192 && cout == 64) RT_CONV(192, 32, 2, SRC_PLAIN, false);
+        else if (cin == 256 && cout == 128) RT_CONV(256, 32, 4, SRC_PLAIN, false);
+        else throw ApiError(RT_ERR_INVALID, "unsupported channel count");
+#undef RT_CONV
+    }
+
+    // Weight and bias gradients of layer k from its output gradient G and its input X.
+    void wgrad(int k, const rtu::bf16 *G, const rtu::bf16 *X, int xs, Acts &a, int N, int H, int W) {
+        const ConvLayer &L = kLayers[k];
+        const int parts = wgrad_parts(L, N, H, W);
+        if ((size_t)parts * L.nw() > a.part_floats || (size_t)kBiasParts * L.cout > a.part_floats)
+            throw ApiError(RT_ERR_INVALID, "gradient partials larger than their buffer");
+        const dim3 grid((unsigned)parts, 9, (L.cout / 32) * ((L.cin + 31) / 32));
+        hipLaunchKernelGGL(rtu::conv3x3_wgrad_kernel, grid, dim3(64), 0, stream, G, L.cout, L.cout, X, xs, L.cin,
+                           a.part, N, H, W);
+        hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(blocks_for(L.nw(), 2048)), dim3(256), 0, stream, a.part,
+                           parts, L.nw(), L.nw(), d_g + w_off[k]);
+        hipLaunchKernelGGL(rtu::bias_grad_kernel, dim3(kBiasParts), dim3(256), 0, stream, G, L.cout, L.cout,
+                           (size_t)N * H * W, a.part);
+        hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(1), dim3(256), 0, stream, a.part, kBiasParts,
+                           (size_t)L.cout, (size_t)L.cout, d_g + b_off[k]);
+    }
+
+    void pool(const rtu::bf16 *X, int xs, int c, rtu::bf16 *P, int N, int H, int W) {  // H, W: input size
+        hipLaunchKernelGGL(rtu::pool_kernel, dim3((unsigned)((W / 2 * (c / 8) + 255) / 256), (unsigned)(N * H / 2)),
+                           dim3(256), 0, stream, X, xs, c, P, H, W);
+    }
+    void upsample(const rtu::bf16 *Bsrc, int c, rtu::bf16 *Y, int ys, int off, int N, int H, int W) {  // H, W: output
+        hipLaunchKernelGGL(rtu::upsample_kernel, dim3((unsigned)((W * (c / 8) + 255) / 256), (unsigned)(N * H)),
+                           dim3(256), 0, stream, Bsrc, c, Y, ys, off, H, W);
+    }
+
+    // Forward pass over a.F / a.noisy. Training (with_target) keeps the pooled
+    // and concatenated tensors for the backward pass, writes the loss partials
+    // and runs the output layer's backward pass into a.gV[0] and the
+    // output-layer gradients; inference folds the first pool and every
+    // upsample + concatenation into the convolutions' staging.
     void forward(Acts &a, bool with_target, double *loss_slot) {
-        const int N = a.N, H = a.H, W = a.W, H2 = H / 2, W2 = W / 2;
-        conv(a.F, 16, 0, false, a.A1, 32, 0, true, nullptr, 0, N, H, W);
-        conv(a.A1, 32, 1, false, a.CAT, 96, 0, true, nullptr, 0, N, H, W);
-        if (with_target) {  // training: P and CAT[32:96] are kept for the backward pass
-            hipLaunchKernelGGL(rtu::pool_kernel, dim3((W2 * 4 + 255) / 256, N * H2), dim3(256), 0, stream, a.CAT,
-                               96, 32, a.P, H, W);
-            conv(a.P, 32, 2, false, a.B1, 64, 0, true, nullptr, 0, N, H2, W2);
-        } else {  // inference: pooled while staged
-            conv(a.CAT, 96, 2, false, a.B1, 64, 0, true, nullptr, 0, N, H2, W2, rtu::SRC_POOL);
+        const int N = a.N;
+        conv(0, false, a.F, kFeat, a.T[0], kEnc[0], 0, nullptr, 0, N, a.h(0), a.w(0));
+        conv(1, false, a.T[0], kEnc[0], a.CAT[0], cat_ch(0), 0, nullptr, 0, N, a.h(0), a.w(0));
+        for (int l = 1; l < kLevels; l++) {
+            const int cs = kEnc[l - 1];
+            if (!with_target && l == 1) {
+                conv(2, false, a.CAT[0], cat_ch(0), a.T[1], kEnc[1], 0, nullptr, 0, N, a.h(1), a.w(1),
+                     rtu::SRC_POOL);
+            } else {
+                pool(a.CAT[l - 1], cat_ch(l - 1), cs, a.P[l], N, a.h(l - 1), a.w(l - 1));
+                conv(2 * l, false, a.P[l], cs, a.T[l], kEnc[l], 0, nullptr, 0, N, a.h(l), a.w(l));
+            }
+            rtu::bf16 *out = l < kLevels - 1 ? a.CAT[l] : a.B;
+            const int os = l < kLevels - 1 ? cat_ch(l) : kEnc[l];
+            conv(2 * l + 1, false, a.T[l], kEnc[l], out, os, 0, nullptr, 0, N, a.h(l), a.w(l));
         }
-        conv(a.B1, 64, 3, false, a.B2, 64, 0, true, nullptr, 0, N, H2, W2);
+        for (int l = kLevels - 2; l >= 0; l--) {
+            const int k = dec_layer(l);
+            const rtu::bf16 *below = l == kLevels - 2 ? a.B : a.V[l + 1];
+            const int bc = cat_up(l);
+            if (with_target) {
+                upsample(below, bc, a.CAT[l], cat_ch(l), kEnc[l], N, a.h(l), a.w(l));
+                conv(k, false, a.CAT[l], cat_ch(l), a.U[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l));
+            } else {
+                conv(k, false, a.CAT[l], cat_ch(l), a.U[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l),
+                     rtu::SRC_CONCAT, below, bc, kEnc[l]);
+            }
+            conv(k + 1, false, a.U[l], kDec[l], a.V[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l));
+        }
+        const float scale = 1.0f / (3.0f * (float)a.px());
         if (with_target) {
-            hipLaunchKernelGGL(rtu::upsample_kernel, dim3((W * 8 + 255) / 256, N * H), dim3(256), 0, stream, a.B2,
-                               64, a.CAT, 96, 32, H, W);
-            conv(a.CAT, 96, 4, false, a.D1, 32, 0, true, nullptr, 0, N, H, W);
-        } else {  // inference: B2 upsampled and concatenated while staged
-            conv(a.CAT, 96, 4, false, a.D1, 32, 0, true, nullptr, 0, N, H, W, rtu::SRC_CONCAT, a.B2, 64, 32);
-        }
-        conv(a.D1, 32, 5, false, a.D2, 32, 0, true, nullptr, 0, N, H, W);
-        const float scale = 2.0f / (3.0f * (float)a.px());
-        if (with_target)
-            hipLaunchKernelGGL(rtu::out_layer_kernel<true>, dim3(blocks_for(a.px(), 1024)), dim3(256), 0, stream, a.D2,
-                               a.noisy, d_p + ow_off, d_p + ob_off, a.pred, a.target, a.px(), scale, a.gD2,
-                               d_g + ow_off, d_g + ob_off, loss_slot);
-        else
-            hipLaunchKernelGGL(rtu::out_layer_kernel<false>, dim3(blocks_for(a.px())), dim3(256), 0, stream, a.D2,
+            hipLaunchKernelGGL(rtu::out_layer_kernel<true>, dim3(kOutBlocks), dim3(256), 0, stream, a.V[0], a.noisy,
+                               d_p + ow_off, d_p + ob_off, a.pred, a.target, a.px(), scale, a.gV[0], a.part);
+            // output-layer gradients (w then b, contiguous) and the loss from the block partials
+            hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(1), dim3(256), 0, stream, a.part, kOutBlocks,
+                               (size_t)rtu::kOutParts, (size_t)(3 * rtu::kOutC + 3), d_g + ow_off);
+            hipLaunchKernelGGL(rtu::loss_kernel, dim3(1), dim3(64), 0, stream, a.part, kOutBlocks,
+                               1.0 / (3.0 * (double)a.px()), loss_slot);
+        } else {
+            hipLaunchKernelGGL(rtu::out_layer_kernel<false>, dim3(blocks_for(a.px())), dim3(256), 0, stream, a.V[0],
                                a.noisy, d_p + ow_off, d_p + ob_off, a.pred, (const float *)nullptr, a.px(), scale,
-                               (rtu::bf16 *)nullptr, (float *)nullptr, (float *)nullptr, (double *)nullptr);
+                               (rtu::bf16 *)nullptr, (float *)nullptr);
+        }
         hip_check(hipGetLastError(), "unet forward");
     }
 
     void backward(Acts &a) {
-        const int N = a.N, H = a.H, W = a.W, H2 = H / 2, W2 = W / 2;
-        wgrad(a.gD2, 5, a.D1, 32, N, H, W);
-        conv(a.gD2, 32, 5, true, a.gD1, 32, 0, false, a.D1, 32, N, H, W);
-        wgrad(a.gD1, 4, a.CAT, 96, N, H, W);
-        conv(a.gD1, 32, 4, true, a.gCAT, 96, 0, false, a.CAT, 96, N, H, W);
-        hipLaunchKernelGGL(rtu::upsample_back_kernel, dim3((W2 * 8 + 255) / 256, N * H2), dim3(256), 0, stream,
-                           a.gCAT, 96, 32, 64, a.gB2, H, W);
-        wgrad(a.gB2, 3, a.B1, 64, N, H2, W2);
-        conv(a.gB2, 64, 3, true, a.gB1, 64, 0, false, a.B1, 64, N, H2, W2);
-        wgrad(a.gB1, 2, a.P, 32, N, H2, W2);
-        conv(a.gB1, 64, 2, true, a.gP, 32, 0, false, nullptr, 0, N, H2, W2);
-        hipLaunchKernelGGL(rtu::pool_back_kernel, dim3((W * 4 + 255) / 256, N * H), dim3(256), 0, stream, a.gCAT, 96,
-                           a.gP, a.CAT, 96, 32, a.gA2, H, W);
-        wgrad(a.gA2, 1, a.A1, 32, N, H, W);
-        conv(a.gA2, 32, 1, true, a.gA1, 32, 0, false, a.A1, 32, N, H, W);
-        wgrad(a.gA1, 0, a.F, 16, N, H, W);
+        const int N = a.N;
+        // decoder, output side first
+        for (int l = 0; l < kLevels - 1; l++) {
+            const int k = dec_layer(l);
+            if (l > 0)  // V_l's gradient: the 2x2 sums of the upsampled part of CAT_{l-1}'s gradient
+                hipLaunchKernelGGL(rtu::upsample_back_kernel,
+                                   dim3((unsigned)((a.w(l) * (kDec[l] / 8) + 255) / 256), (unsigned)(N * a.h(l))),
+                                   dim3(256), 0, stream, a.gCAT[l - 1], cat_ch(l - 1), kEnc[l - 1], kDec[l], a.gV[l],
+                                   a.h(l - 1), a.w(l - 1));
+            wgrad(k + 1, a.gV[l], a.U[l], kDec[l], a, N, a.h(l), a.w(l));
+            conv(k + 1, true, a.gV[l], kDec[l], a.gU[l], kDec[l], 0, a.U[l], kDec[l], N, a.h(l), a.w(l));
+            wgrad(k, a.gU[l], a.CAT[l], cat_ch(l), a, N, a.h(l), a.w(l));
+            conv(k, true, a.gU[l], kDec[l], a.gCAT[l], cat_ch(l), 0, a.CAT[l], cat_ch(l), N, a.h(l), a.w(l));
+        }
+        // the bottleneck's gradient, then the encoder from the bottom up
+        const int lb = kLevels - 1;
+        hipLaunchKernelGGL(rtu::upsample_back_kernel,
+                           dim3((unsigned)((a.w(lb) * (kEnc[lb] / 8) + 255) / 256), (unsigned)(N * a.h(lb))),
+                           dim3(256), 0, stream, a.gCAT[lb - 1], cat_ch(lb - 1), kEnc[lb - 1], kEnc[lb], a.gB,
+                           a.h(lb - 1), a.w(lb - 1));
+        for (int l = kLevels - 1; l >= 0; l--) {
+            const rtu::bf16 *gx = l == kLevels - 1 ? a.gB : a.gS[l];
+            wgrad(2 * l + 1, gx, a.T[l], kEnc[l], a, N, a.h(l), a.w(l));
+            conv(2 * l + 1, true, gx, kEnc[l], a.gT[l], kEnc[l], 0, a.T[l], kEnc[l], N, a.h(l), a.w(l));
+            const rtu::bf16 *xin = l == 0 ? a.F : a.P[l];
+            const int xc = l == 0 ? kFeat : kEnc[l - 1];
+            wgrad(2 * l, a.gT[l], xin, xc, a, N, a.h(l), a.w(l));
+            if (l == 0) break;
+            conv(2 * l, true, a.gT[l], kEnc[l], a.gP[l], xc, 0, nullptr, 0, N, a.h(l), a.w(l));
+            // S_{l-1}'s gradient: its part of CAT_{l-1}'s gradient plus the pool's share, masked by S
+            hipLaunchKernelGGL(rtu::pool_back_kernel,
+                               dim3((unsigned)((a.w(l - 1) * (xc / 8) + 255) / 256), (unsigned)(N * a.h(l - 1))),
+                               dim3(256), 0, stream, a.gCAT[l - 1], cat_ch(l - 1), a.gP[l], a.CAT[l - 1],
+                               cat_ch(l - 1), xc, a.gS[l - 1], a.h(l - 1), a.w(l - 1));
+        }
         hip_check(hipGetLastError(), "unet backward");
     }
 
@@ -883,7 +1090,7 @@ struct rt_unet {
         adam_t++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.0f - std::pow(b1, (float)adam_t), c2 = 1.0f - std::pow(b2, (float)adam_t);
-        for (int k = 0; k < 6; k++) {
+        for (int k = 0; k < kNumLayers; k++) {
             const ConvLayer &L = kLayers[k];
             hipLaunchKernelGGL(rtu::adam_kernel, dim3(blocks_for(L.nw())), dim3(256), 0, stream, d_p + w_off[k],
                                d_g + w_off[k], d_m + w_off[k], d_v + w_off[k], L.nw(), lr, b1, b2, c1, c2, 1e-8f,
@@ -899,12 +1106,14 @@ struct rt_unet {
         hip_check(hipGetLastError(), "adam");
     }
 
-    void features(Acts &a, const rtu::FrameViews &v) {
-        const size_t n = (size_t)v.width * v.height;
-        const float ff = std::log2((float)std::max(v.frames, 1u) + 1.0f) / 8.0f;
+    // Features of a frame into a (padded) activation set whose level-0 size
+    // is the frame's rounded up to kPadTo.
+    void features(const rtu::FrameViews &v, rtu::bf16 *F, float *noisy, uint32_t pw, uint32_t ph) {
+        const size_t n = (size_t)pw * ph;
         hipLaunchKernelGGL(rtu::features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
                            reinterpret_cast<const float4 *>(v.accum), reinterpret_cast<const float4 *>(v.gbuf),
-                           reinterpret_cast<const float4 *>(v.albedo), ff, n, a.F, a.noisy);
+                           reinterpret_cast<const float4 *>(v.albedo), reinterpret_cast<const float2 *>(v.moments),
+                           v.ema, v.width, v.height, pw, ph, F, noisy);
     }
 };
 
@@ -916,7 +1125,7 @@ rt_unet *create_unet(int device, uint64_t seed) {
     u->device = device;
     hip_check(hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking), "hipStreamCreate");
     size_t off = 0;
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < kNumLayers; k++) {
         u->w_off[k] = off;
         off += kLayers[k].nw();
         u->b_off[k] = off;
@@ -929,7 +1138,7 @@ rt_unet *create_unet(int device, uint64_t seed) {
     u->n_params = off;
     std::vector<float> p(off, 0.0f);
     std::mt19937_64 rng(seed);
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < kNumLayers; k++) {
         std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (9.0f * kLayers[k].cin)));
         for (size_t i = 0; i < kLayers[k].nw(); i++) p[u->w_off[k] + i] = nd(rng);
     }
@@ -940,7 +1149,7 @@ rt_unet *create_unet(int device, uint64_t seed) {
     u->d_v = dalloc<float>(off, "unet adam v");
     hip_check(hipMemcpy(u->d_p, p.data(), off * 4, hipMemcpyHostToDevice), "upload params");
     for (float *b : {u->d_g, u->d_m, u->d_v}) hip_check(hipMemset(b, 0, off * 4), "memset");
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < kNumLayers; k++) {
         u->d_wf[k] = dalloc<rtu::bf16>(kLayers[k].nw(), "unet bf16 weights");
         u->d_wt[k] = dalloc<rtu::bf16>(kLayers[k].nw(), "unet bf16 weights (backward)");
     }
@@ -951,9 +1160,11 @@ rt_unet *create_unet(int device, uint64_t seed) {
 }
 
 void check_views(const rtu::FrameViews &v) {
-    if (!v.accum || !v.gbuf || !v.albedo) throw ApiError(RT_ERR_INVALID, "frame views incomplete");
-    if (v.width % 2 || v.height % 2) throw ApiError(RT_ERR_INVALID, "the denoiser needs even width and height");
+    if (!v.accum || !v.gbuf || !v.albedo || !v.moments) throw ApiError(RT_ERR_INVALID, "frame views incomplete");
+    if (v.width == 0 || v.height == 0) throw ApiError(RT_ERR_INVALID, "empty frame");
 }
+
+uint32_t padded(uint32_t n) { return (n + kPadTo - 1) / kPadTo * kPadTo; }
 
 }  // namespace
 
@@ -965,12 +1176,13 @@ void denoise(rt_unet *u, const FrameViews &v, void *out, hipStream_t stream) {
     hipStream_t saved = u->stream;
     u->stream = stream;
     try {
-        u->inf.ensure(1, (int)v.height, (int)v.width, false);
-        u->features(u->inf, v);
+        const uint32_t pw = padded(v.width), ph = padded(v.height);
+        u->inf.ensure(1, (int)ph, (int)pw, false, 0);
+        u->features(v, u->inf.F, u->inf.noisy, pw, ph);
         u->forward(u->inf, false, nullptr);
         const size_t n = (size_t)v.width * v.height;
-        hipLaunchKernelGGL(image_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, u->inf.pred, n,
-                           reinterpret_cast<float4 *>(out));
+        hipLaunchKernelGGL(image_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, u->inf.pred,
+                           v.width, v.height, pw, reinterpret_cast<float4 *>(out));
         hip_check(hipGetLastError(), "unet image");
     } catch (...) {
         u->stream = saved;
@@ -1024,13 +1236,15 @@ int rt_unet_set_params(rt_unet *u, const float *in, size_t n) {
 int rt_unet_dataset_reset(rt_unet *u, uint32_t width, uint32_t height, uint32_t capacity) {
     return rt::api_call([&] {
         if (!u) throw ApiError(RT_ERR_INVALID, "null argument");
-        if (width % 2 || height % 2 || width < 32 || height < 32 || capacity == 0)
-            throw ApiError(RT_ERR_INVALID, "dataset images must be even-sized, >= 32, capacity > 0");
+        if (width % 32 || height % 32 || width < 32 || height < 32 || capacity == 0)
+            throw ApiError(RT_ERR_INVALID, "dataset images must be multiples of 32 in size, capacity > 0");
         hip_check(hipStreamSynchronize(u->stream), "sync");
         for (void *p : {(void *)u->ds_F, (void *)u->ds_noisy, (void *)u->ds_target})
             if (p) (void)hipFree(p);
+        u->ds_F = nullptr;
+        u->ds_noisy = u->ds_target = nullptr;
         const size_t px = (size_t)width * height * capacity;
-        u->ds_F = dalloc<rtu::bf16>(px * 16, "dataset features");
+        u->ds_F = dalloc<rtu::bf16>(px * kFeat, "dataset features");
         u->ds_noisy = dalloc<float>(px * 3, "dataset inputs");
         u->ds_target = dalloc<float>(px * 3, "dataset targets");
         u->ds_w = width;
@@ -1047,15 +1261,12 @@ int rt_unet_dataset_set(rt_unet *u, uint32_t slot, int which, rt_renderer *r) {
         if (slot >= u->ds_cap) throw ApiError(RT_ERR_INVALID, "slot out of range");
         hip_check(hipStreamSynchronize(u->stream), "sync");
         const rtu::FrameViews v = rtu::frame_views(r);
-        if (v.width != u->ds_w || v.height != u->ds_h) throw ApiError(RT_ERR_INVALID, "image size differs from the dataset");
+        if (v.width != u->ds_w || v.height != u->ds_h)
+            throw ApiError(RT_ERR_INVALID, "image size differs from the dataset");
         const size_t n = (size_t)v.width * v.height, base = (size_t)slot * n;
         if (which == 0) {
             check_views(v);
-            const float ff = std::log2((float)std::max(v.frames, 1u) + 1.0f) / 8.0f;
-            hipLaunchKernelGGL(rtu::features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, u->stream,
-                               reinterpret_cast<const float4 *>(v.accum), reinterpret_cast<const float4 *>(v.gbuf),
-                               reinterpret_cast<const float4 *>(v.albedo), ff, n, u->ds_F + base * 16,
-                               u->ds_noisy + base * 3);
+            u->features(v, u->ds_F + base * kFeat, u->ds_noisy + base * 3, v.width, v.height);
             u->ds_has_in[slot] = 1;
         } else {
             hipLaunchKernelGGL(rtu::log_target_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, u->stream,
@@ -1077,19 +1288,20 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
         if (ready.empty()) throw ApiError(RT_ERR_INVALID, "no dataset slot holds both an input and a target");
         if (crop % 32 || crop < 32 || crop > u->ds_w || crop > u->ds_h || batch == 0 || batch > 256)
             throw ApiError(RT_ERR_INVALID, "crop must be a multiple of 32 within the images; 1 <= batch <= 256");
-        u->train.ensure((int)batch, (int)crop, (int)crop, true);
+        u->train.ensure((int)batch, (int)crop, (int)crop, true, part_floats_for((int)batch, (int)crop, (int)crop));
         const size_t sel_n = (size_t)3 * batch * std::max(steps, 1u);
         if (u->sel_cap < sel_n) {
             if (u->d_sel) (void)hipFree(u->d_sel);
+            u->d_sel = nullptr;
             u->d_sel = dalloc<int>(sel_n, "crop selection");
             u->sel_cap = sel_n;
         }
         if (u->loss_cap < steps) {
             if (u->d_loss) (void)hipFree(u->d_loss);
+            u->d_loss = nullptr;
             u->d_loss = dalloc<double>(steps, "losses");
             u->loss_cap = steps;
         }
-        hip_check(hipMemsetAsync(u->d_loss, 0, 8ull * std::max(steps, 1u), u->stream), "clear losses");
         // every step's crops drawn up front and uploaded once
         std::mt19937_64 rng(seed);
         std::vector<int> sel((size_t)3 * batch * std::max(steps, 1u));
@@ -1105,8 +1317,7 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
                                u->stream, u->ds_F, u->ds_noisy, u->ds_target, u->d_sel + (size_t)3 * batch * s,
                                (int)u->ds_w, (int)u->ds_h, (int)crop, (int)crop, u->train.F, u->train.noisy,
                                u->train.target);
-            hipLaunchKernelGGL(rtu::zero_kernel, dim3(blocks_for(u->n_params)), dim3(256), 0, u->stream, u->d_g,
-                               u->n_params);
+            // every gradient is overwritten (reduced from its partials), so none is cleared
             u->forward(u->train, true, u->d_loss + s);
             u->backward(u->train);
             u->adam(lr);
@@ -1115,42 +1326,36 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
         if (losses && steps) {
             std::vector<double> l(steps);
             hip_check(hipMemcpy(l.data(), u->d_loss, 8ull * steps, hipMemcpyDeviceToHost), "read losses");
-            const double norm = 1.0 / (3.0 * (double)u->train.px());
-            for (uint32_t s = 0; s < steps; s++) losses[s] = (float)(l[s] * norm);
+            for (uint32_t s = 0; s < steps; s++) losses[s] = (float)l[s];
         }
     });
 }
 
-// Test hooks: one forward (and backward) pass on caller-supplied host tensors.
+// Test hook: one forward (and backward) pass on caller-supplied host tensors
+// (h and w multiples of 16, features padded to 16 channels).
 int rt_unet_eval(rt_unet *u, uint32_t n, uint32_t h, uint32_t w, const uint16_t *features, const float *noisy,
                  const float *target, float *pred, float *grads, double *loss) {
     return rt::api_call([&] {
         if (!u || !features || !noisy || !pred) throw ApiError(RT_ERR_INVALID, "null argument");
-        if (h % 2 || w % 2 || n == 0) throw ApiError(RT_ERR_INVALID, "even sizes required");
+        if (h % kPadTo || w % kPadTo || n == 0 || h == 0 || w == 0)
+            throw ApiError(RT_ERR_INVALID, "image sizes must be multiples of 16");
         Acts &a = u->train;
-        a.ensure((int)n, (int)h, (int)w, true);
+        a.ensure((int)n, (int)h, (int)w, true, part_floats_for((int)n, (int)h, (int)w));
         const size_t px = (size_t)n * h * w;
         hip_check(hipStreamSynchronize(u->stream), "sync");
-        hip_check(hipMemcpy(a.F, features, px * 16 * 2, hipMemcpyHostToDevice), "upload features");
+        hip_check(hipMemcpy(a.F, features, px * kFeat * 2, hipMemcpyHostToDevice), "upload features");
         hip_check(hipMemcpy(a.noisy, noisy, px * 12, hipMemcpyHostToDevice), "upload inputs");
         if (target) hip_check(hipMemcpy(a.target, target, px * 12, hipMemcpyHostToDevice), "upload targets");
         if (u->loss_cap < 1) {
             u->d_loss = dalloc<double>(1, "loss");
             u->loss_cap = 1;
         }
-        hip_check(hipMemsetAsync(u->d_loss, 0, 8, u->stream), "clear loss");
-        hipLaunchKernelGGL(rtu::zero_kernel, dim3(blocks_for(u->n_params)), dim3(256), 0, u->stream, u->d_g,
-                           u->n_params);
         u->forward(a, target != nullptr, u->d_loss);
         if (target) u->backward(a);
         hip_check(hipStreamSynchronize(u->stream), "sync");
         hip_check(hipMemcpy(pred, a.pred, px * 12, hipMemcpyDeviceToHost), "read pred");
         if (target && grads) hip_check(hipMemcpy(grads, u->d_g, u->n_params * 4, hipMemcpyDeviceToHost), "read grads");
-        if (target && loss) {
-            double l = 0;
-            hip_check(hipMemcpy(&l, u->d_loss, 8, hipMemcpyDeviceToHost), "read loss");
-            *loss = l / (3.0 * (double)px);
-        }
+        if (target && loss) hip_check(hipMemcpy(loss, u->d_loss, 8, hipMemcpyDeviceToHost), "read loss");
     });
 }
 

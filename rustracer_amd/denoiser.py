@@ -1,26 +1,37 @@
-"""The learned denoiser: a small U-Net trained on the GPU on pairs the renderer
-itself produces (few-frame accumulations -> many-frame targets of seeded
-procedural scenes), then attached to a Renderer so each frame's displayed image
-is the network's output.
+"""The learned denoiser: a five-level U-Net trained on the GPU on pairs the
+renderer itself produces (1/2/4/8-frame accumulations -> 4096-sample targets of
+seeded procedural scenes), then attached to a Renderer so each frame's
+displayed image is the network's output.
 
-    python -m rustracer_amd.denoiser --scenes 24 --steps 600 --out weights.f32
+    python -m rustracer_amd.denoiser --scenes 2000 --steps 2000 --out weights.f32
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import sys
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import RenderConfig, Renderer, Scene
+from . import RenderConfig, Renderer, Scene, _ffi
 from ._ffi import check, lib, ptr
 
-INPUT_FRAMES = (1, 2, 4, 8, 16, 32)
-# (cin, cout) of the six 3x3 convolutions: two at full resolution, two at half,
-# two at full after the skip concatenation (the layout of csrc/unet.hip)
-LAYER_SHAPES = ((16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (32, 32))
+INPUT_FRAMES = (1, 2, 4, 8)
+TARGET_SAMPLES = 4096
+TARGET_SPP = 16  # the target is rendered as TARGET_SAMPLES / TARGET_SPP frames of this many samples
+# (cin, cout) of the 18 3x3 convolutions (the layout of csrc/unet.hip): two per
+# encoder level (32, 64, 128, 128, 128 channels), then two per decoder level
+# 3..0 reading [skip, upsampled] concatenations of 256, 256, 192, 96 channels
+LAYER_SHAPES = ((16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (128, 128), (128, 128),
+                (128, 128), (128, 128), (256, 128), (128, 128), (256, 128), (128, 128), (192, 64), (64, 64),
+                (96, 32), (32, 32))
+# resolution level (0 = full, l = 1/2^l per axis) of each layer
+LAYER_LEVELS = (0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 3, 3, 2, 2, 1, 1, 0, 0)
+OUT_C = 32
+PAD_TO = 16  # the network pads image sizes to multiples of this
 
 
 class UNet:
@@ -28,6 +39,7 @@ class UNet:
         h = C.c_void_p()
         check(lib().rt_unet_create(device, seed, C.byref(h)), "rt_unet_create")
         self._h = h
+        self._ds_lock = threading.Lock()
 
     def close(self) -> None:
         if self._h:
@@ -62,16 +74,17 @@ class UNet:
         check(lib().rt_unet_dataset_reset(self._h, width, height, capacity), "rt_unet_dataset_reset")
 
     def dataset_set(self, slot: int, which: int, renderer: Renderer) -> None:
-        check(lib().rt_unet_dataset_set(self._h, slot, which, renderer._h), "rt_unet_dataset_set")
+        with self._ds_lock:  # one network, filled from several rendering threads
+            check(lib().rt_unet_dataset_set(self._h, slot, which, renderer._h), "rt_unet_dataset_set")
 
-    def train(self, steps: int, batch: int = 16, crop: int = 128, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+    def train(self, steps: int, batch: int = 32, crop: int = 256, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
         losses = np.zeros(max(steps, 1), np.float32)
         check(lib().rt_unet_train(self._h, steps, batch, crop, lr, seed, ptr(losses)), "rt_unet_train")
         return losses[:steps]
 
     def eval(self, features: np.ndarray, noisy: np.ndarray, target: np.ndarray | None = None):
-        """One pass on host tensors (n, h, w, 16) bf16 bits / (n, h, w, 3) float32.
-        Returns pred, and with a target also (grads, loss)."""
+        """One pass on host tensors (n, h, w, 16) bf16 bits / (n, h, w, 3) float32,
+        h and w multiples of 16. Returns pred, and with a target also (grads, loss)."""
         n, h, w, _ = features.shape
         f = np.ascontiguousarray(features, np.uint16)
         x = np.ascontiguousarray(noisy, np.float32)
@@ -84,48 +97,73 @@ class UNet:
         return pred if t is None else (pred, grads, loss.value)
 
 
-def _set(r, **fields) -> None:
-    c = r.config
-    for k, v in fields.items():
-        setattr(c, k, v)
-    r.config = c
+def network_flops(w: int, h: int) -> float:
+    """Multiply-add FLOPs of one forward pass at w x h (padded to multiples of
+    16): the 3x3 layers (12 of the first layer's 16 input channels carry data)
+    and the 1x1 output layer."""
+    pw, ph = -(-w // PAD_TO) * PAD_TO, -(-h // PAD_TO) * PAD_TO
+    f = 0.0
+    for k, ((cin, cout), lvl) in enumerate(zip(LAYER_SHAPES, LAYER_LEVELS)):
+        cin_used = 12 if k == 0 else cin
+        f += 2.0 * (pw >> lvl) * (ph >> lvl) * 9 * cin_used * cout
+    return f + 2.0 * pw * ph * OUT_C * 3
 
 
-def build_dataset(unet: UNet, n_scenes: int, size: int = 256, target_frames: int = 512, photons: int = 200_000,
-                  first_seed: int = 1000, log=None) -> int:
-    """Renders the training pairs into the network's dataset; returns the slot count."""
+def _render_scene_pair(unet: UNet, k: int, seed: int, size: int, target_samples: int, photons: int) -> None:
+    """Scene `seed` into dataset slots k*4 .. k*4+3: the target (target_samples
+    samples) and the 1/2/4/8-frame inputs (1 spp per frame)."""
+    scene = Scene.builtin("training", seed)
+    cam = Scene.builtin_camera("training", seed, 0)
+    cfg = RenderConfig(width=size, height=size, spp=TARGET_SPP, max_depth=8, enable_denoise=False,
+                       enable_photon_map=photons > 0, photon_count=max(photons, 1), accumulate_frames=1 << 20,
+                       flags=_ffi.FLAG_MOMENT_HISTORY)
+    first = k * len(INPUT_FRAMES)
+    with Renderer(cfg, scene, 0) as r:
+        r.update_camera(cam)
+        if photons > 0:
+            r.rebuild_photons()
+        r.render_frames(max(1, target_samples // TARGET_SPP))
+        for j in range(len(INPUT_FRAMES)):
+            unet.dataset_set(first + j, 1, r)  # the same target for every input of this scene
+        c = r.config
+        c.spp = 1
+        r.config = c
+        for j, frames in enumerate(INPUT_FRAMES):
+            r.reset_accumulation()
+            r.render_frames(frames)
+            unet.dataset_set(first + j, 0, r)
+
+
+def build_dataset(unet: UNet, n_scenes: int, size: int = 256, target_samples: int = TARGET_SAMPLES,
+                  photons: int = 200_000, first_seed: int = 1000, workers: int = 8, log=None) -> int:
+    """Renders the training pairs of scenes first_seed .. first_seed + n_scenes - 1
+    into the network's dataset (slot 4k + j holds scene k's input j); returns
+    the slot count. Scenes are built and rendered by `workers` threads (the BVH
+    build runs on the host); every slot's contents depend only on its scene."""
     unet.dataset_reset(size, size, n_scenes * len(INPUT_FRAMES))
-    slot = 0
-    for k in range(n_scenes):
-        seed = first_seed + k
-        scene = Scene.builtin("training", seed)
-        cam = Scene.builtin_camera("training", seed, 0)
-        cfg = RenderConfig(width=size, height=size, spp=1, max_depth=8, enable_denoise=False,
-                           enable_photon_map=photons > 0, photon_count=max(photons, 1), accumulate_frames=1 << 20)
-        with Renderer(cfg, scene, 0) as r:
-            r.update_camera(cam)
-            if photons > 0:
-                r.rebuild_photons()
-            r.render_frames(target_frames)
-            first = slot
-            for _ in INPUT_FRAMES:
-                unet.dataset_set(slot, 1, r)  # the same target for every input of this scene
-                slot += 1
-            for j, frames in enumerate(INPUT_FRAMES):
-                r.reset_accumulation()
-                r.render_frames(frames)
-                unet.dataset_set(first + j, 0, r)
+    done = [0]
+    lock = threading.Lock()
+
+    def one(k):
+        _render_scene_pair(unet, k, first_seed + k, size, target_samples, photons)
         if log:
-            log(f"training scene {k + 1}/{n_scenes} (seed {seed})")
-    return slot
+            with lock:
+                done[0] += 1
+                if done[0] % max(1, n_scenes // 20) == 0 or done[0] == n_scenes:
+                    log(f"training scenes {done[0]}/{n_scenes}")
+
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        for f in [ex.submit(one, k) for k in range(n_scenes)]:
+            f.result()
+    return n_scenes * len(INPUT_FRAMES)
 
 
-def train_denoiser(n_scenes: int = 24, steps: int = 600, batch: int = 16, crop: int = 128, lr: float = 1e-3,
-                   seed: int = 1, log=None):
+def train_denoiser(n_scenes: int = 64, steps: int = 800, batch: int = 32, crop: int = 256, lr: float = 1e-3,
+                   seed: int = 1, target_samples: int = TARGET_SAMPLES, log=None):
     """A trained UNet, the loss curve and the seconds spent (data, training)."""
     unet = UNet(0, seed)
     t0 = time.perf_counter()
-    build_dataset(unet, n_scenes, log=log)
+    build_dataset(unet, n_scenes, target_samples=target_samples, log=log)
     t1 = time.perf_counter()
     losses = unet.train(steps, batch, crop, lr, seed)
     t2 = time.perf_counter()
@@ -137,18 +175,23 @@ def train_denoiser(n_scenes: int = 24, steps: int = 600, batch: int = 16, crop: 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scenes", type=int, default=24)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--crop", type=int, default=128)
+    ap.add_argument("--scenes", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--crop", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--target-samples", type=int, default=TARGET_SAMPLES)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
 
     def log(m):
         print(f"[denoiser] {m}", file=sys.stderr, flush=True)
 
-    unet, losses, _ = train_denoiser(args.scenes, args.steps, args.batch, args.crop, args.lr, log=log)
+    unet, losses, (data_s, train_s) = train_denoiser(args.scenes, args.steps, args.batch, args.crop, args.lr,
+                                                     target_samples=args.target_samples, log=log)
+    print({"scenes": args.scenes, "steps": args.steps, "data_seconds": round(data_s, 2),
+           "train_seconds": round(train_s, 2), "loss_first10": float(losses[:10].mean()),
+           "loss_last10": float(losses[-10:].mean())}, flush=True)
     if args.out:
         unet.get_params().tofile(args.out)
     return 0

@@ -27,8 +27,8 @@ def _params(seed):
 
 def _inputs(seed, n, h, w):
     rng = np.random.default_rng(seed)
-    f = rng.normal(0, 1, (n, h, w, 16)).astype(np.float32)
-    f[..., 11:] = 0.0
+    f = rng.normal(0, 1, (n, h, w, U.IN_C)).astype(np.float32)
+    f[..., 12:] = 0.0  # 12 feature channels, zero padding to 16
     x = rng.normal(0, 1, (n, h, w, 3)).astype(np.float32)
     t = (x + rng.normal(0, 0.3, x.shape)).astype(np.float32)
     return U.bf16_bits(f), x, t
@@ -38,7 +38,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("n,h,w", [(2, 32, 64), (1, 40, 72)])
+@pytest.mark.parametrize("n,h,w", [(2, 32, 64), (1, 48, 80)])
 def test_forward_backward_match_reference(n, h, w):
     print(f"unet eval n={n} h={h} w={w}", flush=True)
     p = _params(1)
@@ -80,7 +80,7 @@ def test_training_lowers_the_loss_and_the_held_out_error():
     a held-out scene the network's 2-frame image is closer to a 256-frame
     reference than the 2-frame accumulation itself."""
     net = UNet(0, 5)
-    build_dataset(net, n_scenes=6, size=128, target_frames=256, photons=50_000, first_seed=300)
+    build_dataset(net, n_scenes=6, size=128, target_samples=256, photons=50_000, first_seed=300)
     losses = net.train(steps=200, batch=8, crop=96, lr=2e-3, seed=9)
     print(f"loss {losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}", flush=True)
     assert losses[-20:].mean() < 0.8 * losses[:20].mean()
@@ -145,7 +145,7 @@ def test_chunked_frames_with_denoiser_match_frame_by_frame():
     assert res[0][3] == res[1][3] == 9
 
 
-@pytest.mark.parametrize("n,h,w", [(1, 40, 72), (2, 34, 66)])
+@pytest.mark.parametrize("n,h,w", [(1, 48, 80), (2, 32, 96)])
 def test_inference_fused_staging_equals_training_forward(n, h, w):
     """The inference pass stages the pool and the upsample inside the
     convolutions; its prediction equals the training forward's (separate pool

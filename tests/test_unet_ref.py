@@ -10,7 +10,7 @@ sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
 import unet_ref as U  # noqa: E402
 
 
-def _random_problem(seed, n=1, h=8, w=8):
+def _random_problem(seed, n=1, h=16, w=16):
     rng = np.random.default_rng(seed)
     p = np.zeros(U.param_count())
     ws_off = 0
@@ -20,7 +20,7 @@ def _random_problem(seed, n=1, h=8, w=8):
         p[ws_off:ws_off + co] = rng.normal(0, 0.1, co)
         ws_off += co
     p[ws_off:] = rng.normal(0, 0.2, p.size - ws_off)
-    feats = rng.normal(0, 1, (n, h, w, 16))
+    feats = rng.normal(0, 1, (n, h, w, U.IN_C))
     noisy = rng.normal(0, 1, (n, h, w, 3))
     target = rng.normal(0, 1, (n, h, w, 3))
     return p, feats, noisy, target
