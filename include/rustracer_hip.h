@@ -441,6 +441,16 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
  * the parameters); loss out (mean |pred - target|). */
 int rt_unet_eval(rt_unet *u, uint32_t n, uint32_t h, uint32_t w, const uint16_t *features, const float *noisy,
                  const float *target, float *pred, float *grads, double *loss);
+/* The convolution variant (input-channel slice width, output-channel blocks of
+ * 32, weights staged once) picked for each layer shape so far: the first use of
+ * a shape times every compiled variant that fits and keeps the fastest (all of
+ * them give identical results). One text line per shape, "cin cout src N H W
+ * mask bias -> kc nco once"; *needed = bytes including the terminator. */
+int rt_unet_conv_choices(const rt_unet *u, char *buf, size_t cap, size_t *needed);
+/* How a convolution variant is picked: 1 (default) the autotuned fastest, 0 the
+ * first compiled for the shape, k >= 2 the (k-2)-th that fits (the last if
+ * fewer): a test hook that runs every variant. */
+int rt_unet_set_variant_mode(rt_unet *u, int mode);
 /* Attach (or detach with NULL) a denoiser: each rt_render_frame then runs it on
  * the accumulation and tonemaps its output into the rgba8 image. */
 int rt_renderer_set_denoiser(rt_renderer *r, rt_unet *u);

@@ -236,6 +236,8 @@ SIGNATURES = {
     "rt_unet_dataset_set": (_i, [_vp, C.c_uint32, C.c_int, _vp]),
     "rt_unet_train": (_i, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _vp]),
     "rt_unet_eval": (_i, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp, _vp, _P(C.c_double)]),
+    "rt_unet_conv_choices": (_i, [_vp, _vp, C.c_size_t, _P(C.c_size_t)]),
+    "rt_unet_set_variant_mode": (_i, [_vp, C.c_int]),
     "rt_renderer_set_denoiser": (_i, [_vp, _vp]),
     "rt_read_display": (_i, [_vp, _vp, C.c_size_t]),
     "rt_read_albedo": (_i, [_vp, _vp, C.c_size_t]),

@@ -29,7 +29,10 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -98,7 +101,8 @@ __device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4
 // (8-byte epilogue stores).
 //
 // The reduction over 9 taps x CIN runs in CIN / KC slices of KC input
-// channels. Per slice, the input rows y0-1 .. y0+kConvRows, pixels x0-1 ..
+// channels, always in the same order (16-channel groups in increasing order,
+// the 9 taps inside each), so every variant of a layer gives the same bits. Per slice, the input rows y0-1 .. y0+kConvRows, pixels x0-1 ..
 // x0+32 of those channels are staged in LDS (pixel stride KC + 8 to spread
 // banks) and serve all 9 taps, and so are the slice's weights [9][CB][KC + 8].
 // With WONCE (one slice and one block: the whole layer's weights fit) the
@@ -134,7 +138,6 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
                                                       int N) {
     static_assert(CIN % KC == 0 && KC % 16 == 0, "input channels in slices of a multiple of 16");
     static_assert(!WONCE || CIN == KC, "weights staged once: one slice");
-    static_assert(SRC != SRC_POOL || WONCE, "pooled staging only with the weights staged once");
     constexpr int R = kConvRowsPerWave;
     constexpr int NS = CIN / KC;
     constexpr int PS = KC + 8;  // LDS pixel and weight-row stride (elements)
@@ -267,14 +270,17 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
 #pragma unroll
                 for (int j = 0; j < NCO; j++) acc[r][j] = floatx16{};
         }
+        // canonical order: 16-channel groups in increasing order, the 9 taps
+        // inside each; every slicing and blocking of a layer therefore sums in
+        // the same order and gives the same bits
 #pragma unroll
-        for (int tap = 0; tap < 9; tap++) {
-            const bf16 *wp = wl + (tap * CB + c) * PS + kg;
+        for (int c0 = 0; c0 < KC; c0 += 16) {
 #pragma unroll
-            for (int c0 = 0; c0 < KC; c0 += 16) {
+            for (int tap = 0; tap < 9; tap++) {
+                const bf16 *wp = wl + (tap * CB + c) * PS + kg + c0;
                 bf16x8 a[NCO], b[R];
 #pragma unroll
-                for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * PS + c0);
+                for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * PS);
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     b[r] = load8(tile + ((wv * R + r + tap / 3) * 34 + c + tap % 3) * PS + kg + c0);
@@ -436,8 +442,17 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const float *__restrict__ part, int n_part, size_t stride,
                                                               size_t n, float *__restrict__ out) {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        // eight loads in flight, added in index order
         float s = 0.0f;
-        for (int g = 0; g < n_part; g++) s += part[(size_t)g * stride + i];
+        int g = 0;
+        for (; g + 8 <= n_part; g += 8) {
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = part[(size_t)(g + k) * stride + i];
+#pragma unroll
+            for (int k = 0; k < 8; k++) s += v[k];
+        }
+        for (; g < n_part; g++) s += part[(size_t)g * stride + i];
         out[i] = s;
     }
 }
@@ -638,7 +653,15 @@ __global__ __launch_bounds__(256) void image_kernel(const float *__restrict__ pr
 __global__ void loss_kernel(const float *__restrict__ part, int n_part, double inv_count, double *__restrict__ loss) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double s = 0.0;
-    for (int g = 0; g < n_part; g++) s += (double)part[(size_t)g * kOutParts + kOutParts - 1];
+    int g = 0;
+    for (; g + 8 <= n_part; g += 8) {  // eight loads in flight, added in block order
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = part[(size_t)(g + k) * kOutParts + kOutParts - 1];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += (double)v[k];
+    }
+    for (; g < n_part; g++) s += (double)part[(size_t)g * kOutParts + kOutParts - 1];
     *loss = s * inv_count;
 }
 
@@ -856,6 +879,76 @@ size_t part_floats_for(int N, int H, int W) {
     return need;
 }
 
+// One convolution launch's arguments (see conv3x3_kernel).
+struct ConvArgs {
+    const rtu::bf16 *X = nullptr;
+    int xs = 0;
+    const rtu::bf16 *w = nullptr;
+    int cout = 0;
+    const float *b = nullptr;
+    rtu::bf16 *Y = nullptr;
+    int ys = 0, yoff = 0, relu = 0;
+    const rtu::bf16 *mask = nullptr;
+    int ms = 0, N = 0, H = 0, W = 0;
+    const rtu::bf16 *X2 = nullptr;
+    int x2s = 0, c1 = 0;
+};
+typedef void (*ConvLaunch)(const ConvArgs &, hipStream_t);
+// cin, cout, src, N, H, W, mask, bias
+typedef std::array<int, 8> ConvKey;
+
+template <int CIN, int KC, int NCO, int SRC, bool WONCE>
+void conv_launch(const ConvArgs &a, hipStream_t s) {
+    constexpr size_t lds = rtu::conv_lds_bytes(KC, NCO);
+    static_assert(lds <= 160 * 1024, "LDS of one workgroup");
+    // once per instance (a function-local static, initialised once even
+    // across threads): raise the dynamic LDS limit (above 64 KB) and size
+    // the persistent grid
+    static const int resident = [] {
+        const void *fn = (const void *)rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>;
+        if (lds > 65536)
+            hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "conv LDS size");
+        int per_cu = 0;
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>,
+                                                               256, lds),
+                  "conv occupancy");
+        return std::max(per_cu, 1) * cu_count();
+    }();
+    const int items = ((a.W + 31) / 32) * ((a.H + rtu::kConvRows - 1) / rtu::kConvRows) * a.N * (a.cout / (32 * NCO));
+    const dim3 grid((unsigned)std::max(1, std::min(items, resident)));
+    hipLaunchKernelGGL((rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>), grid, dim3(256), lds, s, a.X, a.xs, a.w, a.cout,
+                       a.b, a.Y, a.ys, a.yoff, a.relu, a.mask, a.ms, a.H, a.W, a.X2, a.x2s, a.c1, a.N);
+}
+
+// The compiled variants: input channels, slice width KC, output-channel block
+// 32 * NCO, staging source, weights staged once (one slice, one block). The
+// first listed for a shape is the choice without tuning.
+struct ConvVariant {
+    int cin, kc, nco, src;
+    bool wonce;
+    ConvLaunch fn;
+};
+#define RT_CV(CI, KC, NCO, SRC, ONCE) {CI, KC, NCO, rtu::SRC, ONCE, conv_launch<CI, KC, NCO, rtu::SRC, ONCE>}
+const ConvVariant kConvVariants[] = {
+    RT_CV(16, 16, 1, SRC_PLAIN, true),
+    RT_CV(32, 32, 1, SRC_PLAIN, true),    RT_CV(32, 16, 1, SRC_PLAIN, false),
+    RT_CV(32, 32, 2, SRC_PLAIN, true),    RT_CV(32, 16, 2, SRC_PLAIN, false),
+    RT_CV(32, 32, 3, SRC_PLAIN, true),    RT_CV(32, 16, 3, SRC_PLAIN, false),
+    RT_CV(32, 32, 2, SRC_POOL, true),     RT_CV(32, 16, 2, SRC_POOL, false),   RT_CV(32, 16, 1, SRC_POOL, false),
+    RT_CV(64, 64, 1, SRC_PLAIN, true),    RT_CV(64, 32, 1, SRC_PLAIN, false),  RT_CV(64, 16, 1, SRC_PLAIN, false),
+    RT_CV(64, 64, 2, SRC_PLAIN, true),    RT_CV(64, 32, 2, SRC_PLAIN, false),  RT_CV(64, 16, 2, SRC_PLAIN, false),
+    RT_CV(64, 32, 4, SRC_PLAIN, false),   RT_CV(64, 16, 4, SRC_PLAIN, false),
+    RT_CV(96, 96, 1, SRC_PLAIN, true),    RT_CV(96, 32, 1, SRC_PLAIN, false),  RT_CV(96, 16, 1, SRC_PLAIN, false),
+    RT_CV(96, 96, 1, SRC_CONCAT, true),   RT_CV(96, 32, 1, SRC_CONCAT, false), RT_CV(96, 16, 1, SRC_CONCAT, false),
+    RT_CV(128, 32, 4, SRC_PLAIN, false),  RT_CV(128, 16, 4, SRC_PLAIN, false), RT_CV(128, 32, 2, SRC_PLAIN, false),
+    RT_CV(128, 16, 2, SRC_PLAIN, false),  RT_CV(128, 32, 1, SRC_PLAIN, false),
+    RT_CV(192, 32, 2, SRC_PLAIN, false),  RT_CV(192, 16, 2, SRC_PLAIN, false), RT_CV(192, 32, 1, SRC_PLAIN, false),
+    RT_CV(192, 32, 2, SRC_CONCAT, false), RT_CV(192, 16, 2, SRC_CONCAT, false), RT_CV(192, 32, 1, SRC_CONCAT, false),
+    RT_CV(256, 32, 4, SRC_PLAIN, false),  RT_CV(256, 16, 4, SRC_PLAIN, false), RT_CV(256, 32, 2, SRC_PLAIN, false),
+    RT_CV(256, 32, 4, SRC_CONCAT, false), RT_CV(256, 16, 4, SRC_CONCAT, false), RT_CV(256, 32, 2, SRC_CONCAT, false),
+};
+#undef RT_CV
+
 }  // namespace
 
 struct rt_unet {
@@ -878,6 +971,9 @@ struct rt_unet {
     size_t sel_cap = 0;
     double *d_loss = nullptr;
     size_t loss_cap = 0;
+    // convolution variant chosen per shape (autotuned at first use)
+    std::map<ConvKey, int> conv_choice;
+    int variant_mode = 1;  // 0 first listed, 1 autotuned, k >= 2 the (k-2)-th that fits
 
     ~rt_unet() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -902,33 +998,6 @@ struct rt_unet {
         hip_check(hipGetLastError(), "weights to bf16");
     }
 
-    template <int CIN, int KC, int NCO, int SRC, bool WONCE>
-    void launch_conv(const rtu::bf16 *X, int xs, const rtu::bf16 *w, int cout, const float *b, rtu::bf16 *Y, int ys,
-                     int yoff, bool relu, const rtu::bf16 *mask, int ms, int N, int H, int W, const rtu::bf16 *X2,
-                     int x2s, int c1) {
-        constexpr size_t lds = rtu::conv_lds_bytes(KC, NCO);
-        if (cout % (32 * NCO) || (WONCE && cout != 32 * NCO))
-            throw ApiError(RT_ERR_INVALID, "convolution output channels do not match its blocking");
-        // once per instance (a function-local static, initialised once even
-        // across threads): raise the dynamic LDS limit (above 64 KB) and size
-        // the persistent grid
-        static const int resident = [] {
-            const void *fn = (const void *)rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>;
-            if (lds > 65536)
-                hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                          "conv LDS size");
-            int per_cu = 0;
-            hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>,
-                                                                   256, lds),
-                      "conv occupancy");
-            return std::max(per_cu, 1) * cu_count();
-        }();
-        const int items = ((W + 31) / 32) * ((H + rtu::kConvRows - 1) / rtu::kConvRows) * N * (cout / (32 * NCO));
-        const dim3 grid((unsigned)std::max(1, std::min(items, resident)));
-        hipLaunchKernelGGL((rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>), grid, dim3(256), lds, stream, X, xs, w,
-                           cout, b, Y, ys, yoff, relu ? 1 : 0, mask, ms, H, W, X2, x2s, c1, N);
-    }
-
     // Layer k forward (bias, ReLU) or backward-data (bwd: transposed weights,
     // no bias, mask = the ReLU derivative of the layer's input, or none).
     // src: rtu::SRC_POOL (X at 2H x 2W, pooled while staged) or rtu::SRC_CONCAT
@@ -937,36 +1006,83 @@ struct rt_unet {
               int ms, int N, int H, int W, int src = rtu::SRC_PLAIN, const rtu::bf16 *X2 = nullptr, int x2s = 0,
               int c1 = 0) {
         const ConvLayer &L = kLayers[k];
-        const int cin = bwd ? L.cout : L.cin, cout = bwd ? L.cin : L.cout;
-        const rtu::bf16 *w = bwd ? d_wt[k] : d_wf[k];
-        const float *b = bwd ? nullptr : d_p + b_off[k];
-        const bool relu = !bwd;
-        using namespace rtu;
-#define RT_CONV(CI, KC, NCO, SRC, ONCE)                                                                          \
-    launch_conv<CI, KC, NCO, SRC, ONCE>(X, xs, w, cout, b, Y, ys, yoff, relu, mask, ms, N, H, W, X2, x2s, c1)
-        if (src == SRC_POOL) {
-            if (cin == 32 && cout == 64) RT_CONV(32, 32, 2, SRC_POOL, true);
-            else throw ApiError(RT_ERR_INVALID, "unsupported pooled convolution");
-        } else if (src == SRC_CONCAT) {
-            if (cin == 96 && cout == 32) RT_CONV(96, 96, 1, SRC_CONCAT, true);
-            else if (cin == 192 && cout == 64) RT_CONV(192, 32, 2, SRC_CONCAT, false);
-            else if (cin == 256 && cout == 128) RT_CONV(256, 32, 4, SRC_CONCAT, false);
-            else throw ApiError(RT_ERR_INVALID, "unsupported concatenated convolution");
-        } else if (cin == 16 && cout == 32) RT_CONV(16, 16, 1, SRC_PLAIN, true);
-        else if (cin == 32 && cout == 32) RT_CONV(32, 32, 1, SRC_PLAIN, true);
-        else if (cin == 32 && cout == 64) RT_CONV(32, 32, 2, SRC_PLAIN, true);
-        else if (cin == 32 && cout == 96) RT_CONV(32, 32, 3, SRC_PLAIN, true);
-        else if (cin == 64 && cout == 32) RT_CONV(64, 64, 1, SRC_PLAIN, true);
-        else if (cin == 64 && cout == 64) RT_CONV(64, 64, 2, SRC_PLAIN, true);
-        else if (cin == 64 && cout == 128) RT_CONV(64, 32, 4, SRC_PLAIN, false);
-        else if (cin == 64 && cout == 192) RT_CONV(64, 32, 2, SRC_PLAIN, false);
-        else if (cin == 96 && cout == 32) RT_CONV(96, 96, 1, SRC_PLAIN, true);
-        else if (cin == 128 && cout == 64) RT_CONV(128, 32, 2, SRC_PLAIN, false);
-        else if (cin == 128 && (cout == 128 || cout == 256)) RT_CONV(128, 32, 4, SRC_PLAIN, false);
-        else if (cin == 192 && cout == 64) RT_CONV(192, 32, 2, SRC_PLAIN, false);
-        else if (cin == 256 && cout == 128) RT_CONV(256, 32, 4, SRC_PLAIN, false);
-        else throw ApiError(RT_ERR_INVALID, "unsupported channel count");
-#undef RT_CONV
+        ConvArgs a;
+        a.X = X;
+        a.xs = xs;
+        a.w = bwd ? d_wt[k] : d_wf[k];
+        a.cout = bwd ? L.cin : L.cout;
+        a.b = bwd ? nullptr : d_p + b_off[k];
+        a.Y = Y;
+        a.ys = ys;
+        a.yoff = yoff;
+        a.relu = bwd ? 0 : 1;
+        a.mask = mask;
+        a.ms = ms;
+        a.N = N;
+        a.H = H;
+        a.W = W;
+        a.X2 = X2;
+        a.x2s = x2s;
+        a.c1 = c1;
+        run_conv(bwd ? L.cout : L.cin, src, a);
+    }
+
+    // The variants compiled for (cin, src) that can produce a.cout channels;
+    // with more than one, the fastest on this shape is timed once and kept.
+    // All variants sum in the same order, so the choice never changes a bit.
+    void run_conv(int cin, int src, const ConvArgs &a) {
+        std::vector<int> cands;
+        for (int i = 0; i < (int)(sizeof(kConvVariants) / sizeof(kConvVariants[0])); i++) {
+            const ConvVariant &v = kConvVariants[i];
+            if (v.cin == cin && v.src == src && a.cout % (32 * v.nco) == 0 && (!v.wonce || a.cout == 32 * v.nco))
+                cands.push_back(i);
+        }
+        if (cands.empty()) throw ApiError(RT_ERR_INVALID, "no convolution variant for this channel count");
+        int pick = cands[0];
+        if (variant_mode >= 2) {
+            pick = cands[std::min((size_t)(variant_mode - 2), cands.size() - 1)];
+        } else if (cands.size() > 1 && variant_mode == 1) {
+            const ConvKey key{cin, a.cout, src, a.N, a.H, a.W, a.mask ? 1 : 0, a.b ? 1 : 0};
+            auto it = conv_choice.find(key);
+            if (it != conv_choice.end()) {
+                pick = it->second;
+            } else {
+                pick = tune_conv(cands, a);
+                conv_choice[key] = pick;
+            }
+        }
+        kConvVariants[pick].fn(a, stream);
+    }
+
+    int tune_conv(const std::vector<int> &cands, const ConvArgs &a) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hip_check(hipEventCreate(&e0), "tuning event");
+        hip_check(hipEventCreate(&e1), "tuning event");
+        int best = cands[0];
+        float best_ms = 1e30f;
+        try {
+            for (int c : cands) {
+                kConvVariants[c].fn(a, stream);  // first launch: one-time set-up, caches
+                hip_check(hipEventRecord(e0, stream), "tuning event");
+                kConvVariants[c].fn(a, stream);
+                kConvVariants[c].fn(a, stream);
+                hip_check(hipEventRecord(e1, stream), "tuning event");
+                hip_check(hipEventSynchronize(e1), "tuning sync");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "tuning time");
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return best;
     }
 
     // Weight and bias gradients of layer k from its output gradient G and its input X.
@@ -1327,6 +1443,37 @@ int rt_unet_train(rt_unet *u, uint32_t steps, uint32_t batch, uint32_t crop, flo
             std::vector<double> l(steps);
             hip_check(hipMemcpy(l.data(), u->d_loss, 8ull * steps, hipMemcpyDeviceToHost), "read losses");
             for (uint32_t s = 0; s < steps; s++) losses[s] = (float)l[s];
+        }
+    });
+}
+
+int rt_unet_set_variant_mode(rt_unet *u, int mode) {
+    return rt::api_call([&] {
+        if (!u || mode < 0) throw ApiError(RT_ERR_INVALID, "null argument or negative mode");
+        hip_check(hipStreamSynchronize(u->stream), "sync");
+        u->variant_mode = mode;
+    });
+}
+
+// The convolution variant chosen for each shape so far, one line per shape:
+// "cin cout src N H W mask bias -> kc nco wonce".
+int rt_unet_conv_choices(const rt_unet *u, char *buf, size_t cap, size_t *needed) {
+    return rt::api_call([&] {
+        if (!u) throw ApiError(RT_ERR_INVALID, "null argument");
+        std::string out;
+        for (const auto &kv : u->conv_choice) {
+            const ConvKey &k = kv.first;
+            const ConvVariant &v = kConvVariants[kv.second];
+            char line[160];
+            std::snprintf(line, sizeof(line), "%d %d %d %d %d %d %d %d -> %d %d %d\n", k[0], k[1], k[2], k[3], k[4],
+                          k[5], k[6], k[7], v.kc, v.nco, v.wonce ? 1 : 0);
+            out += line;
+        }
+        if (needed) *needed = out.size() + 1;
+        if (buf && cap) {
+            const size_t n = std::min(cap - 1, out.size());
+            std::memcpy(buf, out.data(), n);
+            buf[n] = '\0';
         }
     });
 }

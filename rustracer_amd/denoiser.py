@@ -82,6 +82,18 @@ class UNet:
         check(lib().rt_unet_train(self._h, steps, batch, crop, lr, seed, ptr(losses)), "rt_unet_train")
         return losses[:steps]
 
+    def set_variant_mode(self, mode: int) -> None:
+        """1 autotuned (default), 0 the first compiled variant per shape, k >= 2 the (k-2)-th."""
+        check(lib().rt_unet_set_variant_mode(self._h, mode), "rt_unet_set_variant_mode")
+
+    def conv_choices(self) -> str:
+        """The convolution variant chosen per layer shape (autotuned at first use)."""
+        n = C.c_size_t()
+        check(lib().rt_unet_conv_choices(self._h, None, 0, C.byref(n)), "rt_unet_conv_choices")
+        buf = C.create_string_buffer(n.value)
+        check(lib().rt_unet_conv_choices(self._h, buf, n.value, None), "rt_unet_conv_choices")
+        return buf.value.decode()
+
     def eval(self, features: np.ndarray, noisy: np.ndarray, target: np.ndarray | None = None):
         """One pass on host tensors (n, h, w, 16) bf16 bits / (n, h, w, 3) float32,
         h and w multiples of 16. Returns pred, and with a target also (grads, loss)."""

@@ -158,3 +158,28 @@ def test_inference_fused_staging_equals_training_forward(n, h, w):
     pred_train, _, _ = net.eval(fb, x, t)
     pred_infer = net.eval(fb, x, None)
     assert np.array_equal(pred_train.view(np.uint32), pred_infer.view(np.uint32))
+
+
+@pytest.mark.parametrize("n,h,w", [(1, 48, 80)])
+def test_every_convolution_variant_gives_the_same_bits(n, h, w):
+    """Each layer shape has several compiled variants (input-channel slice
+    width, output-channel blocking, weights staged once); all sum in one
+    order, so prediction, loss and every gradient are bit-identical whichever
+    runs, and the autotuner's timing-based choice never changes a result."""
+    print(f"unet variants n={n} h={h} w={w}", flush=True)
+    p = _params(8)
+    fb, x, t = _inputs(9, n, h, w)
+    net = UNet(0, 1)
+    net.set_params(p)
+    net.set_variant_mode(0)
+    base = net.eval(fb, x, t)
+    base_inf = net.eval(fb, x)
+    for mode in (1, 2, 3, 4, 5, 6):
+        net.set_variant_mode(mode)
+        pred, grads, loss = net.eval(fb, x, t)
+        assert np.array_equal(pred.view(np.uint32), base[0].view(np.uint32)), mode
+        assert np.array_equal(grads.view(np.uint32), base[1].view(np.uint32)), mode
+        assert loss == base[2], mode
+        assert np.array_equal(net.eval(fb, x).view(np.uint32), base_inf.view(np.uint32)), mode
+    assert "->" in net.conv_choices()  # mode 1 tuned and recorded its picks
+    net.close()
