@@ -754,18 +754,20 @@ inline V3 bounce_dir(V3 rd, V3 n, float rough, Rng *rng) {
     return normalize(mix(mirror, to_world(diff, n), blur));
 }
 
-// photon_trace.wgsl:101-260, one invocation.
-void photon_one(const or_scene *s, const rt_grid_params *gp, uint32_t photon_index, uint32_t max_depth,
-                rt_photon *photons) {
+// photon_trace.wgsl:101-175: light pick and emission of photon `photon_index`
+// (false: no light, or a light kind that emits nothing). Leaves the RNG where
+// the trace continues.
+static bool photon_emit(const or_scene *s, const rt_grid_params *gp, uint32_t photon_index, Rng &rng, V3 &origin,
+                        V3 &dir, V3 &power, uint32_t *light_index) {
     const uint32_t num_lights = s->n_lights;
-    if (num_lights == 0u) return;
-    Rng rng = rng_init(photon_index * 1103515245u + 12345u);
+    if (num_lights == 0u) return false;
+    rng = rng_init(photon_index * 1103515245u + 12345u);
     const uint32_t li = clamp_index(wgsl_u32(rng_f32(&rng) * (float)num_lights), num_lights, nullptr);
+    if (light_index) *light_index = li;
     const rt_light &light = s->lights[li];
     const V3 ldata = L3(light.data);
     const V3 lcolor = L3(light.color);
-    V3 origin = ldata, dir;
-    V3 power;
+    origin = ldata;
     if (light.kind == 0u) {
         float u1, u2;
         for (;;) {
@@ -812,8 +814,17 @@ void photon_one(const or_scene *s, const rt_grid_params *gp, uint32_t photon_ind
         const float t = wclamp(d.y * 0.5f + 0.5f, 0.0f, 1.0f);
         power = mix(lcolor, ldata, wgsl_pow(t, 0.5f)) * light.intensity;
     } else {
-        return;
+        return false;
     }
+    return true;
+}
+
+// photon_trace.wgsl:101-260, one invocation.
+void photon_one(const or_scene *s, const rt_grid_params *gp, uint32_t photon_index, uint32_t max_depth,
+                rt_photon *photons) {
+    Rng rng;
+    V3 origin, dir, power;
+    if (!photon_emit(s, gp, photon_index, rng, origin, dir, power, nullptr)) return;
 
     for (uint32_t bounce = 0u; bounce < max_depth; bounce++) {
         const Hit hit = trace_bvh(s, origin, dir, nullptr);
@@ -995,6 +1006,26 @@ int or_gather(const or_scene *s, const rt_camera_uniform *cam, const rt_image_pa
         }
     }
     if (st) *st = total;
+    return 0;
+}
+
+int or_photon_emit(const or_scene *s, const rt_grid_params *gp, uint32_t photon_index, float origin[3],
+                   float dir[3], float power[3], uint32_t *light_index) {
+    Rng rng;
+    V3 o, d, pw;
+    if (!photon_emit(s, gp, photon_index, rng, o, d, pw, light_index)) return 1;
+    origin[0] = o.x, origin[1] = o.y, origin[2] = o.z;
+    dir[0] = d.x, dir[1] = d.y, dir[2] = d.z;
+    power[0] = pw.x, power[1] = pw.y, power[2] = pw.z;
+    return 0;
+}
+
+int or_sample_spec_dirs(const float rd[3], const float n[3], float a, uint32_t seed, uint32_t count, float *out) {
+    Rng rng = rng_init(seed);
+    for (uint32_t i = 0; i < count; i++) {
+        const V3 o = sample_spec_dir(v3(rd[0], rd[1], rd[2]), v3(n[0], n[1], n[2]), a, &rng);
+        out[3 * i] = o.x, out[3 * i + 1] = o.y, out[3 * i + 2] = o.z;
+    }
     return 0;
 }
 

@@ -92,6 +92,14 @@ int or_project_marker(const float cam_position[3], const float cam_look_at[3], c
                       float fov_degrees, float near_plane, uint32_t width, uint32_t height, const float pos[3],
                       float radius, float out[4]);
 
+/* photon_trace.wgsl:101-175 for one photon index: the light picked, the
+ * emitted ray and its power, as or_photon_trace emits them (1 = no emission). */
+int or_photon_emit(const or_scene *s, const rt_grid_params *gp, uint32_t photon_index, float origin[3],
+                   float dir[3], float power[3], uint32_t *light_index);
+/* `count` successive sample_spec_dir draws (gather.wgsl:230-256) from one PCG
+ * stream rng_init(seed), as out[3 * i .. 3 * i + 2]. */
+int or_sample_spec_dirs(const float rd[3], const float n[3], float a, uint32_t seed, uint32_t count, float *out);
+
 /* PCG primitives, exposed for the known-answer tests. */
 uint32_t or_rng_init(uint32_t seed);
 uint32_t or_rng_next(uint32_t *state);

@@ -61,6 +61,10 @@ def lib():
         L.or_project_marker.argtypes = [vp, vp, vp, C.c_float, C.c_float, u32, u32, vp, C.c_float, vp]
         L.or_photon_estimate.argtypes = [C.POINTER(OrPhotonMap), vp, vp, C.c_float, vp, C.POINTER(u32),
                                          C.POINTER(C.c_uint64)]
+        L.or_photon_emit.argtypes = [C.POINTER(OrScene), C.POINTER(GridParams), u32, vp, vp, vp, C.POINTER(u32)]
+        L.or_photon_emit.restype = C.c_int
+        L.or_sample_spec_dirs.argtypes = [vp, vp, C.c_float, u32, u32, vp]
+        L.or_sample_spec_dirs.restype = C.c_int
         L.or_rng_init.argtypes = [u32]
         L.or_rng_init.restype = u32
         L.or_rng_next.argtypes = [C.POINTER(u32)]
@@ -156,6 +160,14 @@ class OracleScene:
         assert rc == 0, rc
         return frame, mom, gb, st
 
+    def photon_emit(self, gp, photon_index):
+        """(origin, dir, power, light index) of photon `photon_index`, or None."""
+        o, d, p = (np.zeros(3, np.float32) for _ in range(3))
+        li = C.c_uint32()
+        if lib().or_photon_emit(C.byref(self.s), C.byref(gp), photon_index, _p(o), _p(d), _p(p), C.byref(li)):
+            return None
+        return o, d, p, li.value
+
     def photon_trace(self, gp, n_photons, max_depth=6, n_slots=None, threads=8):
         n_slots = n_slots or gp.photon_slots
         photons = np.zeros(n_slots, PHOTON)
@@ -238,3 +250,12 @@ def math_eval(name, x, y=None):
 def rng_stream(seed, n):
     state = C.c_uint32(lib().or_rng_init(seed))
     return np.array([lib().or_rng_next(C.byref(state)) for _ in range(n)], np.uint32)
+
+
+def sample_spec_dirs(rd, n, a, count, seed):
+    """`count` successive sample_spec_dir draws from one PCG stream (rng_init(seed))."""
+    rd = np.ascontiguousarray(rd, np.float32)
+    n = np.ascontiguousarray(n, np.float32)
+    out = np.zeros((count, 3), np.float32)
+    assert lib().or_sample_spec_dirs(_p(rd), _p(n), a, seed, count, _p(out)) == 0
+    return out
