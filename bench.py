@@ -57,7 +57,8 @@ REF_NODE_BYTES, REF_TRI_BYTES, REF_PHOTON_BYTES = 48, 128, 64
 
 
 TRAVERSALS = {0: "reference BVH2 nodes", 1: "binary child-box records, reference order",
-              2: "binary child-box records, nearest first", 3: "4-wide records, reference order"}
+              2: "binary child-box records, nearest first", 3: "4-wide records, reference order",
+              4: "4-wide records, reference order, watertight triangle test"}
 
 
 PROFILE_FRAMES = 8     # frames of the per-pass kernel times (one rt_render_frames launch)
@@ -82,6 +83,8 @@ def main() -> int:
     ap.add_argument("--reference-traversal", action="store_true")
     ap.add_argument("--nearest-first", action="store_true",
                     help="child-box traversal nearest child first (not bit-exact by construction)")
+    ap.add_argument("--watertight", action="store_true",
+                    help="watertight ray/triangle test instead of Moller-Trumbore (not bit-exact by construction)")
     ap.add_argument("--binary-records", action="store_true",
                     help="binary child-box records instead of the 4-wide records (both exact)")
     ap.add_argument("--path-queues", action="store_true",
@@ -122,6 +125,7 @@ def main() -> int:
     flags = ((_ffi.FLAG_REFERENCE_TRAVERSAL if args.reference_traversal else 0) |
              (_ffi.FLAG_NEAREST_FIRST if args.nearest_first else 0) |
              (_ffi.FLAG_BINARY_RECORDS if args.binary_records else 0) |
+             (_ffi.FLAG_WATERTIGHT if args.watertight else 0) |
              (_ffi.FLAG_PATH_QUEUES if args.path_queues else 0) |
              (_ffi.FLAG_TILE_LAUNCH if args.tile_launch else 0))
     photons_on = not args.no_photons

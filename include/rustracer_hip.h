@@ -101,6 +101,13 @@ typedef struct rt_config {
  * attached, and this flag keeps it without one (for rt_unet_dataset_set). No
  * counterpart in the reference, whose denoiser sees only one frame's moments. */
 #define RT_FLAG_MOMENT_HISTORY 256u
+/* Test triangles with the watertight ray/triangle test (Woop, Benthin, Wald
+ * 2013) instead of the reference's Moller-Trumbore, on the 4-wide records
+ * (traversal mode 4): a ray through an edge or vertex shared by two triangles
+ * hits one of them. Not bit-exact with the reference: a hit's t differs in the
+ * last bits and edge rays may pick the other triangle of the edge (counted by
+ * the tests). No counterpart in the reference. */
+#define RT_FLAG_WATERTIGHT 512u
 
 /* Fill `cfg` with RenderConfig::default() (lib.rs:43-64) plus extension defaults. */
 int rt_config_default(rt_config *cfg);
@@ -349,6 +356,11 @@ typedef struct rt_trace_counters {
     uint64_t nodes_visited;
     uint64_t tris_tested;
     uint64_t photon_reads;
+    /* loop steps per phase (0 traversal, 1 shading: one bounce, 2 photon
+     * search: one batch of candidates), summed over lanes and over wavefronts:
+     * lane_steps / wave_steps = mean active lanes of 64 in that phase */
+    uint64_t lane_steps[3];
+    uint64_t wave_steps[3];
 } rt_trace_counters;
 int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out);
 

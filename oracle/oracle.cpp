@@ -5,13 +5,16 @@
 //
 // Restates, line by line (reference paths relative to the repository root):
 //   rustracer-renderer/shaders/common.wgsl      (all)
-//   rustracer-renderer/shaders/gather.wgsl      (all; texture lookups return the
-//                                                 reference's empty placeholder texel)
+//   rustracer-renderer/shaders/gather.wgsl      (all; textures sampled as
+//                                                 include/rt_texture.h fixes it)
 //   rustracer-renderer/shaders/photon_trace.wgsl
 //   rustracer-renderer/shaders/photon_count.wgsl + CPU scan lib.rs:987-1007
 //   rustracer-renderer/shaders/photon_scatter.wgsl (made stable: ascending slot)
 //   rustracer-renderer/shaders/denoise.wgsl, composite.wgsl, tonemap.wgsl
 //   rustracer-app/examples/common/mod.rs:37-47 (CPU tonemap of save_png)
+// Beyond the reference: the watertight ray/triangle test of the opt-in
+// watertight traversal (Woop, Benthin, Wald, JCGT 2013), as a brute-force
+// closest hit, so the GPU mode can be checked bit for bit.
 // Conscious divergences (identical in the HIP backend):
 //   * an out-of-range light index (rng_f32 == 1.0, gather.wgsl:155,
 //     photon_trace.wgsl:113) is clamped to n-1, i.e. wgpu's "Restrict" policy;
@@ -20,6 +23,9 @@
 //   * within a grid cell photons are ordered by ascending slot index, where the
 //     reference keeps atomic arrival order (photon_scatter.wgsl:35).
 // Parity pins: see tests/test_oracle_*.py (closed forms, brute force, libm).
+// Parity unpinned (no check independent of this file): the normal-map tangent
+// frame (common.wgsl tangent_frame, gather.wgsl:330-336); the GPU matches this
+// restatement bit for bit on textured scenes.
 
 #include "oracle.h"
 
@@ -137,6 +143,50 @@ inline float intersect_triangle(V3 ro, V3 rd, V3 v0, V3 v1, V3 v2) {
     const float t = f * dot(edge2, q);
     if (t > EPS) return t;
     return INF;
+}
+
+// ── Watertight ray/triangle test (opt-in mode; not in the reference) ──
+// Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013,
+// section 3: permute so the largest |direction| axis is z (swapping x and y
+// when that component is negative, which keeps the winding), shear the
+// vertices into ray space, take the three 2-D edge functions, and recompute
+// them in double when one is exactly zero. Float32, no contraction; two-sided,
+// with the reference's t > 1e-6.
+inline float axis(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+inline float intersect_watertight(V3 ro, V3 rd, V3 v0, V3 v1, V3 v2) {
+    const float mx = std::fabs(rd.x), my = std::fabs(rd.y), mz = std::fabs(rd.z);
+    const int kz = (mx >= my && mx >= mz) ? 0 : (my >= mz ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kz + 2) % 3;
+    const float dz = axis(rd, kz);
+    if (dz < 0.0f) std::swap(kx, ky);
+    const float shx = axis(rd, kx) / dz, shy = axis(rd, ky) / dz, shz = 1.0f / dz;
+    const V3 p[3] = {v0 - ro, v1 - ro, v2 - ro};
+    float x[3], y[3], z[3];
+    for (int i = 0; i < 3; i++) {
+        z[i] = axis(p[i], kz);
+        x[i] = axis(p[i], kx) - shx * z[i];
+        y[i] = axis(p[i], ky) - shy * z[i];
+    }
+    // edge function opposite each vertex: (1,2) for vertex 0, (2,0), (0,1)
+    float e[3];
+    for (int i = 0; i < 3; i++) {
+        const int a = (i + 2) % 3, b = (i + 1) % 3;
+        e[i] = x[a] * y[b] - y[a] * x[b];
+    }
+    if (e[0] == 0.0f || e[1] == 0.0f || e[2] == 0.0f) {
+        for (int i = 0; i < 3; i++) {
+            const int a = (i + 2) % 3, b = (i + 1) % 3;
+            e[i] = (float)((double)x[a] * (double)y[b] - (double)y[a] * (double)x[b]);
+        }
+    }
+    const bool any_neg = e[0] < 0.0f || e[1] < 0.0f || e[2] < 0.0f;
+    const bool any_pos = e[0] > 0.0f || e[1] > 0.0f || e[2] > 0.0f;
+    if (any_neg && any_pos) return INF;
+    const float det = (e[0] + e[1]) + e[2];
+    if (det == 0.0f) return INF;
+    const float tsc = (e[0] * (shz * z[0]) + e[1] * (shz * z[1])) + e[2] * (shz * z[2]);
+    const float t = tsc / det;
+    return t > EPS ? t : INF;
 }
 
 // ── common.wgsl:171-251 ──
@@ -978,6 +1028,35 @@ int or_trace_brute(const or_scene *s, const rt_ray *rays, uint32_t n, rt_hit *ou
         }
         out[i].t = best;
         out[i].prim_id = best_id;
+    }
+    return 0;
+}
+
+int or_trace_brute_watertight(const or_scene *s, const rt_ray *rays, uint32_t n, rt_hit *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        const V3 ro = L3(rays[i].origin), rd = L3(rays[i].dir);
+        float best = INF;
+        uint32_t best_id = 0u;
+        for (uint32_t k = 0; k < s->n_tris; k++) {
+            const rt_triangle &tr = s->tris[k];
+            const float t = intersect_watertight(ro, rd, L3(tr.v0), L3(tr.v1), L3(tr.v2));
+            if (t < best) {
+                best = t;
+                best_id = k;
+            }
+        }
+        out[i].t = best;
+        out[i].prim_id = best_id;
+    }
+    return 0;
+}
+
+int or_watertight_t(const or_scene *s, const rt_ray *rays, const uint32_t *prim_ids, uint32_t n, float *t_out) {
+    for (uint32_t i = 0; i < n; i++) {
+        if (prim_ids[i] >= s->n_tris) return -1;
+        const rt_triangle &tr = s->tris[prim_ids[i]];
+        t_out[i] = intersect_watertight(L3(rays[i].origin), L3(rays[i].dir), L3(tr.v0), L3(tr.v1), L3(tr.v2));
     }
     return 0;
 }

@@ -62,6 +62,11 @@ typedef struct or_stats {
 
 int or_trace_closest(const or_scene *s, const rt_ray *rays, uint32_t n, rt_hit *out, or_stats *st);
 int or_trace_brute(const or_scene *s, const rt_ray *rays, uint32_t n, rt_hit *out);
+// Watertight test (Woop, Benthin, Wald 2013) of the opt-in mode, closest hit over
+// every triangle in index order (strict <, so the lowest index wins a tie).
+int or_trace_brute_watertight(const or_scene *s, const rt_ray *rays, uint32_t n, rt_hit *out);
+// The watertight t of ray i against triangle prim_ids[i] (1e38 for a miss).
+int or_watertight_t(const or_scene *s, const rt_ray *rays, const uint32_t *prim_ids, uint32_t n, float *t_out);
 
 int or_gather(const or_scene *s, const rt_camera_uniform *cam, const rt_image_params *ip,
               const or_photon_map *pm, uint32_t max_depth, uint32_t row_begin, uint32_t row_end,

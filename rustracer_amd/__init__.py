@@ -397,7 +397,7 @@ class Renderer:
         check(lib().rt_trace_closest_device(self._h, d_rays, n, d_out), "rt_trace_closest_device")
 
     def traversal_info(self) -> dict:
-        """Active traversal (0 BVH2 nodes, 1 binary records, 2 nearest-first, 3 4-wide)."""
+        """Active traversal (0 BVH2 nodes, 1 binary records, 2 nearest-first, 3 4-wide, 4 4-wide watertight)."""
         mode, depth, n4 = C.c_uint32(), C.c_uint32(), C.c_uint32()
         check(lib().rt_traversal_info(self._h, C.byref(mode), C.byref(depth), C.byref(n4)),
               "rt_traversal_info")
@@ -434,8 +434,12 @@ class Renderer:
     def count_gather_traffic(self) -> dict:
         t = _ffi.TraceCounters()
         check(lib().rt_count_gather_traffic(self._h, C.byref(t)), "rt_count_gather_traffic")
+        phases = ("traversal", "shading", "photon_search")
         return dict(ray_queries=t.ray_queries, nodes_visited=t.nodes_visited, tris_tested=t.tris_tested,
-                    photon_reads=t.photon_reads)
+                    photon_reads=t.photon_reads,
+                    active_lanes={p: round(t.lane_steps[k] / max(t.wave_steps[k], 1), 2) for k, p in enumerate(phases)},
+                    lane_steps={p: t.lane_steps[k] for k, p in enumerate(phases)},
+                    wave_steps={p: t.wave_steps[k] for k, p in enumerate(phases)})
 
     # -- many views in one persistent launch --
     def set_views(self, views) -> None:

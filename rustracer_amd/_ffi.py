@@ -48,6 +48,7 @@ FLAG_PATH_QUEUES = 32
 FLAG_TILE_LAUNCH = 64
 FLAG_GRAPH = 128
 FLAG_MOMENT_HISTORY = 256
+FLAG_WATERTIGHT = 512
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
               "accum64", "batch", "network")
 
@@ -128,7 +129,8 @@ class BatchTuning(C.Structure):
 
 class TraceCounters(C.Structure):
     _fields_ = [("ray_queries", C.c_uint64), ("nodes_visited", C.c_uint64),
-                ("tris_tested", C.c_uint64), ("photon_reads", C.c_uint64)]
+                ("tris_tested", C.c_uint64), ("photon_reads", C.c_uint64),
+                ("lane_steps", C.c_uint64 * 3), ("wave_steps", C.c_uint64 * 3)]
 
 
 for _s, _size in ((Config, 96), (CameraUniform, 96), (ImageParams, 80), (GridParams, 48),

@@ -100,9 +100,19 @@ def main(argv=None) -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-scenes", type=int, default=64)
     ap.add_argument("--train-steps", type=int, default=800)
+    ap.add_argument("--weights", default=None,
+                    help="use these trained parameters (float32, from `python -m rustracer_amd.denoiser --out`) "
+                         "instead of training now")
     args = ap.parse_args(argv)
     unet, train_info = None, None
-    if not args.no_learned:
+    if args.weights and not args.no_learned:
+        import numpy as np
+        from .denoiser import UNet
+        unet = UNet(0, 1)
+        unet.set_params(np.fromfile(args.weights, np.float32))
+        train_info = {"network": "5-level U-Net, 32-128 channels, 12 input channels, L1 on log radiance",
+                      "weights": args.weights}
+    elif not args.no_learned:
         from .denoiser import train_denoiser
         unet, losses, (data_s, train_s) = train_denoiser(args.train_scenes, args.train_steps, log=_log)
         train_info = {"network": "5-level U-Net, 32-128 channels, 12 input channels, L1 on log radiance",

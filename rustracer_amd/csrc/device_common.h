@@ -191,9 +191,18 @@ struct Hit {
     uint32_t prim;  // primitive index (closest hit)
 };
 
+// Counters of the counting build. step_lane[p] / step_wave[p]: loop steps of
+// phase p (0 traversal, 1 shading (one bounce), 2 photon search) summed over
+// lanes and over wavefronts; their ratio is the mean number of active lanes.
 struct TraceCount {
     uint32_t nodes, tris;
+    uint32_t step_lane[3], step_wave[3];
 };
+__device__ __forceinline__ void count_step(TraceCount &c, int phase) {
+    c.step_lane[phase]++;
+    const unsigned long long active = __ballot(1);
+    if (__lane_id() == (uint32_t)(__ffsll((long long)active) - 1)) c.step_wave[phase]++;
+}
 
 // Möller–Trumbore on a slot record, common.wgsl:78-100 (edges precomputed).
 __device__ __forceinline__ float mt_slot(F3 ro, F3 rd, F3 v0, F3 edge1, F3 edge2) {
@@ -210,6 +219,61 @@ __device__ __forceinline__ float mt_slot(F3 ro, F3 rd, F3 v0, F3 edge1, F3 edge2
     const float t = f * dot(edge2, q);
     if (t > kEps) return t;
     return kInf;
+}
+
+// Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013), the opt-in alternative to
+// Moller-Trumbore. The ray is set up once: the axis kz of its largest
+// direction component, the other two axes (swapped when dir[kz] < 0, which
+// keeps the winding) and the shear constants. Per triangle the vertices are
+// taken relative to the origin, sheared so the ray runs along +z, and the
+// three 2-D edge functions are evaluated; an edge shared by two triangles gives
+// the same two products in both (the vertices are stored as they are, not as
+// edges), so its functions are exact negatives and no ray passes between the
+// triangles. An edge function that rounds to exactly zero is recomputed in
+// double precision. Two-sided, with the reference's t > 1e-6.
+struct WtRay {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+__device__ __forceinline__ float comp(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ WtRay wt_ray(F3 rd) {
+    const float ax = __builtin_fabsf(rd.x), ay = __builtin_fabsf(rd.y), az = __builtin_fabsf(rd.z);
+    WtRay w;
+    w.kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    w.kx = w.kz == 2 ? 0 : w.kz + 1;
+    w.ky = w.kx == 2 ? 0 : w.kx + 1;
+    const float dz = comp(rd, w.kz);
+    if (dz < 0.0f) {
+        const int t = w.kx;
+        w.kx = w.ky;
+        w.ky = t;
+    }
+    w.sx = comp(rd, w.kx) / dz;
+    w.sy = comp(rd, w.ky) / dz;
+    w.sz = 1.0f / dz;
+    return w;
+}
+__device__ __forceinline__ float wt_slot(const WtRay &w, F3 ro, F3 v0, F3 v1, F3 v2) {
+    const F3 A = v0 - ro, B = v1 - ro, C = v2 - ro;
+    const float az = comp(A, w.kz), bz = comp(B, w.kz), cz = comp(C, w.kz);
+    const float ax = comp(A, w.kx) - w.sx * az, ay = comp(A, w.ky) - w.sy * az;
+    const float bx = comp(B, w.kx) - w.sx * bz, by = comp(B, w.ky) - w.sy * bz;
+    const float cx = comp(C, w.kx) - w.sx * cz, cy = comp(C, w.ky) - w.sy * cz;
+    float u = cx * by - cy * bx;
+    float v = ax * cy - ay * cx;
+    float e = bx * ay - by * ax;
+    if (u == 0.0f || v == 0.0f || e == 0.0f) {
+        u = (float)((double)cx * (double)by - (double)cy * (double)bx);
+        v = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+        e = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    }
+    if ((u < 0.0f || v < 0.0f || e < 0.0f) && (u > 0.0f || v > 0.0f || e > 0.0f)) return kInf;
+    const float det = u + v + e;
+    if (det == 0.0f) return kInf;
+    const float T = (u * (w.sz * az) + v * (w.sz * bz)) + e * (w.sz * cz);
+    const float t = T / det;
+    return t > kEps ? t : kInf;
 }
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS
@@ -249,6 +313,45 @@ __device__ __forceinline__ void slab(F3 bmin, F3 bmax, F3 ro, F3 inv, float &te,
     ta = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
 }
 
+// Conservative slab test of the watertight mode (Ize, "Robust BVH Ray
+// Traversal", JCGT 2(2), 2013): an axis whose product is 0 * inf (a zero
+// direction component with the origin in a face plane) is inside the slab for
+// every t, and the interval is widened at both ends by kWtBoxMargin of its
+// magnitude, far beyond the 2*gamma_3 that the rounding of (b - o) * (1 / d)
+// can take away, plus `slack`. The slack covers the watertight test's own t:
+// it interpolates the vertices' distances along the ray, so its error is
+// absolute, a few ulps of the distance to the farthest vertex, not of t (a hit
+// 1e-4 from the origin of a triangle 1 away is good to ~1e-7, not 1e-11).
+// With slack = kWtBoxMargin * 2 * (L-inf distance from the origin to the far
+// corner of the scene box), a box holding a hit nearer than the best so far is
+// never culled, so traversal finds the hit a test of every triangle finds.
+constexpr float kWtBoxMargin = 1.0f / 65536.0f;
+__device__ __forceinline__ float wt_slack(F3 ro, F3 root_min, F3 root_max) {
+    const float dx = fmaxf(__builtin_fabsf(root_min.x - ro.x), __builtin_fabsf(root_max.x - ro.x));
+    const float dy = fmaxf(__builtin_fabsf(root_min.y - ro.y), __builtin_fabsf(root_max.y - ro.y));
+    const float dz = fmaxf(__builtin_fabsf(root_min.z - ro.z), __builtin_fabsf(root_max.z - ro.z));
+    return kWtBoxMargin * 2.0f * fmaxf(fmaxf(dx, dy), dz);
+}
+__device__ __forceinline__ void slab_axis_wt(float b0, float b1, float o, float inv, float &n, float &f) {
+    const float t0 = (b0 - o) * inv, t1 = (b1 - o) * inv;
+    n = fminf(t0, t1);
+    f = fmaxf(t0, t1);
+    if (t0 != t0 || t1 != t1) {
+        n = -kInf;
+        f = kInf;
+    }
+}
+__device__ __forceinline__ void slab_wt(F3 bmin, F3 bmax, F3 ro, F3 inv, float slack, float &te, float &ta) {
+    float nx, fx, ny, fy, nz, fz;
+    slab_axis_wt(bmin.x, bmax.x, ro.x, inv.x, nx, fx);
+    slab_axis_wt(bmin.y, bmax.y, ro.y, inv.y, ny, fy);
+    slab_axis_wt(bmin.z, bmax.z, ro.z, inv.z, nz, fz);
+    te = fmaxf(fmaxf(nx, ny), nz);
+    ta = fminf(fminf(fx, fy), fz);
+    te -= __builtin_fabsf(te) * kWtBoxMargin + slack;
+    ta += __builtin_fabsf(ta) * kWtBoxMargin + slack;
+}
+
 // Reference-order traversal, gather.wgsl:66-88: left child first, strict
 // `te < hit.t` and `t < hit.t`, right child dropped when 63 entries are used.
 // EARLY_EXIT (shadow queries): the same traversal, stopped as soon as the best
@@ -261,6 +364,7 @@ __device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, L
     uint32_t idx = 0u, sp = 0u;
     if (S.n_nodes == 0u) return hit;
     for (;;) {
+        if (COUNT) count_step(cnt, 0);
         const rt_bvh_node &n = S.nodes[idx];
         const float4 a = *reinterpret_cast<const float4 *>(&n.bbox_min[0]);
         const float4 b = *reinterpret_cast<const float4 *>(&n.bbox_max[0]);
@@ -337,6 +441,7 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
     uint32_t sp = 0u;
     uint32_t cur = __float_as_uint(S.root_min.w);  // root ref (internal 0, or a leaf ref)
     for (;;) {
+        if (COUNT) count_step(cnt, 0);
         if (ref_is_leaf(cur)) {
             const uint32_t first = cur & 0x07FFFFFFu;
             const uint32_t count = ((cur >> 27) & 15u) + 1u;
@@ -417,21 +522,32 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
 // Record (128 bytes): min.x[4], min.y[4], min.z[4], max.x[4], max.y[4],
 // max.z[4] (SoA over the children), refs[4] (absent child = 0xFFFFFFFF),
 // kill bits (3 per child: x, y, z), 12 bytes of padding.
-template <bool EARLY_EXIT, bool COUNT>
+//
+// WT: the slot triangles hold the vertices (v0, v1, v2) and are tested with
+// the watertight test instead of Moller-Trumbore, and boxes with the
+// conservative slab_wt (traversal mode 4): the closest hit is the one a
+// watertight test of every triangle finds.
+template <bool EARLY_EXIT, bool COUNT, bool WT = false>
 __device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
     Hit hit{kInf, 0u};
     const F3 inv = fs(1.0f) / rd;
+    const WtRay wr = WT ? wt_ray(rd) : WtRay{0, 0, 0, 0.0f, 0.0f, 0.0f};
     if (S.n_nodes == 0u) return hit;
     const uint32_t infmask = (__builtin_isinf(inv.x) ? 1u : 0u) | (__builtin_isinf(inv.y) ? 2u : 0u) |
                              (__builtin_isinf(inv.z) ? 4u : 0u);
-    const uint32_t inf4 = infmask * 0x249u;  // replicated for the four children
+    // the kill bits reproduce the reference's culling of rays in a box's min
+    // face plane; the watertight mode's conservative slab keeps those boxes
+    const uint32_t inf4 = WT ? 0u : infmask * 0x249u;  // replicated for the four children
     float te, ta;
-    slab(xyz(S.root_min), xyz(S.root_max), ro, inv, te, ta);
+    const float slack = WT ? wt_slack(ro, xyz(S.root_min), xyz(S.root_max)) : 0.0f;
+    if (WT) slab_wt(xyz(S.root_min), xyz(S.root_max), ro, inv, slack, te, ta);
+    else slab(xyz(S.root_min), xyz(S.root_max), ro, inv, te, ta);
     if (COUNT) cnt.nodes++;
     if (!(te <= ta && ta > 0.0f)) return hit;
     uint32_t sp = 0u;
     uint32_t cur = __float_as_uint(S.root_min.w);
     for (;;) {
+        if (COUNT) count_step(cnt, 0);
         if (ref_is_leaf(cur)) {
             const uint32_t first = cur & 0x07FFFFFFu;
             const uint32_t count = ((cur >> 27) & 15u) + 1u;
@@ -449,13 +565,14 @@ __device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneS
                     b2 = q[5];
                 }
                 if (COUNT) cnt.tris += two ? 2u : 1u;
-                const float t = mt_slot(ro, rd, xyz(a0), xyz(a1), xyz(a2));
+                const float t = WT ? wt_slot(wr, ro, xyz(a0), xyz(a1), xyz(a2)) : mt_slot(ro, rd, xyz(a0), xyz(a1), xyz(a2));
                 if (t < hit.t) {
                     hit.t = t;
                     hit.prim = __float_as_uint(a0.w);
                 }
                 if (two) {
-                    const float u = mt_slot(ro, rd, xyz(b0), xyz(b1), xyz(b2));
+                    const float u =
+                        WT ? wt_slot(wr, ro, xyz(b0), xyz(b1), xyz(b2)) : mt_slot(ro, rd, xyz(b0), xyz(b1), xyz(b2));
                     if (u < hit.t) {
                         hit.t = u;
                         hit.prim = __float_as_uint(b0.w);
@@ -477,7 +594,8 @@ __device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneS
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 float e, a;
-                slab(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), ro, inv, e, a);
+                if (WT) slab_wt(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), ro, inv, slack, e, a);
+                else slab(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), ro, inv, e, a);
                 tes[k] = e;
                 if (r[k] != 0xFFFFFFFFu && e <= a && a > 0.0f && e < hit.t) pass |= 1u << k;
             }
@@ -525,12 +643,14 @@ __device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneS
 
 // Traversal modes: 0 = the reference's BVH2 nodes, 1 = child-box records in the
 // reference's order, 2 = child-box records nearest child first, 3 = 4-wide
-// records in the reference's order (default).
+// records in the reference's order (default), 4 = 4-wide records with the
+// watertight triangle test (slot triangles as vertices).
 template <int MODE, bool EARLY_EXIT, bool COUNT>
 __device__ __forceinline__ Hit trace_mode(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk,
                                           TraceCount &cnt) {
     if (MODE == 0) return trace_reference<EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
     if (MODE == 3) return trace_wide4<EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
+    if (MODE == 4) return trace_wide4<EARLY_EXIT, COUNT, true>(S, ro, rd, t_max, stk, cnt);
     return trace_wide<MODE == 2, EARLY_EXIT, COUNT>(S, ro, rd, t_max, stk, cnt);
 }
 

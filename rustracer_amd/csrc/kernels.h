@@ -100,6 +100,9 @@ struct BatchLaunch {
 };
 // g.frame / g.moments / g.gbuffer point at the batch outputs (sum of view pixels).
 hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipStream_t s);
+// Slot triangles of the watertight traversal (mode 4): (v0, prim bits), v1, v2
+// of triangle prims[k] for slot k, copied from the uploaded triangles.
+hipError_t launch_wt_slots(const rt_triangle *tris, const uint32_t *prims, uint32_t n_slots, void *out, hipStream_t s);
 hipError_t launch_trace_rays(const TraceInputs &t, uint32_t mode, const rt_ray *rays, uint32_t n, rt_hit *out,
                              hipStream_t s);
 

@@ -305,6 +305,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                        F3 &rd, F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
                        uint32_t py, bool *smooth = nullptr) {
     if (smooth) *smooth = false;
+    if (COUNT) count_step(tr.cnt, 1);
     const rt_image_params &ip = A.ip;
     const uint32_t debug_view = ip._pad[0];
     const Hit hit = tr.closest(ro, rd);
@@ -451,6 +452,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                     if (k >= n_ranges) break;
                     const uint2 meta = metas[k];
                     for (uint32_t i = 0u; i < meta.y && !done; i += kPhotonBatch) {
+                        if (COUNT) count_step(tr.cnt, 2);
                         // one round trip per batch: position, power and incident
                         // direction of each candidate come in one 48-byte record
                         const uint32_t nb = min(meta.y - i, kPhotonBatch);
@@ -674,6 +676,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
         atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
         atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+        for (int k = 0; k < 3; k++) {
+            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
+            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
+        }
     }
 }
 
@@ -785,6 +791,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
         atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
         atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+        for (int k = 0; k < 3; k++) {
+            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
+            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
+        }
     }
 }
 
@@ -928,6 +938,10 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
         atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
         atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+        for (int k = 0; k < 3; k++) {
+            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
+            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
+        }
     }
 }
 
@@ -982,6 +996,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
         atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
         atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
         atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
+        for (int k = 0; k < 3; k++) {
+            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
+            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
+        }
     }
 }
 
@@ -1188,6 +1206,24 @@ TraceScene make_trace_scene(const rtk::TraceInputs &t) {
 
 }  // namespace rtd
 
+namespace rtd {
+
+// Slot k of the watertight traversal: the vertices of triangle prims[k] as
+// uploaded (no edge subtraction, so shared vertices stay identical).
+__global__ __launch_bounds__(256) void wt_slots_kernel(const rt_triangle *__restrict__ tris,
+                                                       const uint32_t *__restrict__ prims, uint32_t n,
+                                                       float4 *__restrict__ out) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t p = prims[k];
+    const rt_triangle &t = tris[p];
+    out[3 * k] = make_float4(t.v0[0], t.v0[1], t.v0[2], __uint_as_float(p));
+    out[3 * k + 1] = make_float4(t.v1[0], t.v1[1], t.v1[2], 0.0f);
+    out[3 * k + 2] = make_float4(t.v2[0], t.v2[1], t.v2[2], 0.0f);
+}
+
+}  // namespace rtd
+
 namespace rtk {
 
 static rtd::GatherArgs gather_args(const GatherLaunch &g) {
@@ -1308,6 +1344,7 @@ hipError_t launch_gather(const GatherLaunch &g, hipStream_t s) {
     if (g.mode == 0) { RT_GATHER_M(0) }
     else if (g.mode == 1) { RT_GATHER_M(1) }
     else if (g.mode == 2) { RT_GATHER_M(2) }
+    else if (g.mode == 4) { RT_GATHER_M(4) }
     else { RT_GATHER_M(3) }
 #undef RT_GATHER_M
 #undef RT_GATHER
@@ -1337,6 +1374,7 @@ hipError_t launch_gather_batch(const GatherLaunch &g, const BatchLaunch &b, hipS
     if (g.mode == 0) { if (g.early_shadow) RT_BATCH(0, true); else RT_BATCH(0, false); }
     else if (g.mode == 1) { if (g.early_shadow) RT_BATCH(1, true); else RT_BATCH(1, false); }
     else if (g.mode == 2) { if (g.early_shadow) RT_BATCH(2, true); else RT_BATCH(2, false); }
+    else if (g.mode == 4) { if (g.early_shadow) RT_BATCH(4, true); else RT_BATCH(4, false); }
     else { if (g.early_shadow) RT_BATCH(3, true); else RT_BATCH(3, false); }
 #undef RT_BATCH
     return hipGetLastError();
@@ -1351,6 +1389,7 @@ hipError_t launch_albedo(const TraceInputs &t, uint32_t mode, const rt_triangle 
     const dim3 grid((n + 63u) / 64u), block(64);
     if (mode == 0) hipLaunchKernelGGL((rtd::albedo_kernel<0>), grid, block, 0, s, ts, tris, mats, cam, w, h, o);
     else if (mode == 3) hipLaunchKernelGGL((rtd::albedo_kernel<3>), grid, block, 0, s, ts, tris, mats, cam, w, h, o);
+    else if (mode == 4) hipLaunchKernelGGL((rtd::albedo_kernel<4>), grid, block, 0, s, ts, tris, mats, cam, w, h, o);
     else hipLaunchKernelGGL((rtd::albedo_kernel<1>), grid, block, 0, s, ts, tris, mats, cam, w, h, o);
     return hipGetLastError();
 }
@@ -1363,7 +1402,16 @@ hipError_t launch_trace_rays(const TraceInputs &t, uint32_t mode, const rt_ray *
     if (mode == 0) hipLaunchKernelGGL((rtd::trace_rays_kernel<0>), grid, block, 0, s, ts, rays, n, out);
     else if (mode == 1) hipLaunchKernelGGL((rtd::trace_rays_kernel<1>), grid, block, 0, s, ts, rays, n, out);
     else if (mode == 2) hipLaunchKernelGGL((rtd::trace_rays_kernel<2>), grid, block, 0, s, ts, rays, n, out);
+    else if (mode == 4) hipLaunchKernelGGL((rtd::trace_rays_kernel<4>), grid, block, 0, s, ts, rays, n, out);
     else hipLaunchKernelGGL((rtd::trace_rays_kernel<3>), grid, block, 0, s, ts, rays, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_wt_slots(const rt_triangle *tris, const uint32_t *prims, uint32_t n_slots, void *out,
+                           hipStream_t s) {
+    if (n_slots == 0) return hipSuccess;
+    hipLaunchKernelGGL(rtd::wt_slots_kernel, dim3((n_slots + 255u) / 256u), dim3(256), 0, s, tris, prims, n_slots,
+                       reinterpret_cast<float4 *>(out));
     return hipGetLastError();
 }
 

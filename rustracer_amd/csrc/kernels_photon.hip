@@ -538,6 +538,7 @@ hipError_t launch_photon_trace(const PhotonLaunch &p, hipStream_t s) {
     if (p.mode == 0) hipLaunchKernelGGL((rtd::photon_trace_kernel<0>), grid, block, 0, s, A);
     else if (p.mode == 1) hipLaunchKernelGGL((rtd::photon_trace_kernel<1>), grid, block, 0, s, A);
     else if (p.mode == 2) hipLaunchKernelGGL((rtd::photon_trace_kernel<2>), grid, block, 0, s, A);
+    else if (p.mode == 4) hipLaunchKernelGGL((rtd::photon_trace_kernel<4>), grid, block, 0, s, A);
     else hipLaunchKernelGGL((rtd::photon_trace_kernel<3>), grid, block, 0, s, A);
     return hipGetLastError();
 }

@@ -61,7 +61,8 @@ struct rt_renderer {
     uint32_t *d_prims = nullptr;
     rt_material *d_mats = nullptr;
     rt_light *d_lights = nullptr;
-    float *d_slot_tris = nullptr;  // float4[3 * n_prims]
+    float *d_slot_tris = nullptr;     // float4[3 * n_prims]: v0, v1 - v0, v2 - v0
+    float *d_slot_tris_wt = nullptr;  // float4[3 * n_prims]: v0, v1, v2 (watertight traversal)
     float *d_wnodes = nullptr;     // float4[4 * n_internal]
     float *d_w4nodes = nullptr;    // float4[8 * n_rec4]
     uint32_t n_rec4 = 0;
@@ -199,7 +200,7 @@ struct rt_renderer {
             hipEventDestroy(p.start);
             hipEventDestroy(p.stop);
         }
-        void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_wnodes, d_accum,
+        void *bufs[] = {d_tris,  d_nodes, d_prims,   d_mats,    d_lights, d_slot_tris, d_slot_tris_wt, d_wnodes, d_accum,
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_gather_rec, d_bframe, d_bmom, d_bgb,
                         d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo, d_mom_hist, d_fstate,
@@ -240,7 +241,7 @@ struct rt_renderer {
         rtk::TraceInputs t;
         t.nodes = d_nodes;
         t.prims = d_prims;
-        t.slot_tris = d_slot_tris;
+        t.slot_tris = traversal_mode() == 4u ? d_slot_tris_wt : d_slot_tris;
         t.wnodes = d_wnodes;
         t.w4nodes = d_w4nodes;
         t.n_nodes = n_nodes;
@@ -248,12 +249,22 @@ struct rt_renderer {
         std::memcpy(t.root_max, root_max, sizeof(root_max));
         return t;
     }
-    // 0 reference BVH2 nodes, 1 binary child-box records, 2 binary nearest first, 3 4-wide records
+    // 0 reference BVH2 nodes, 1 binary child-box records, 2 binary nearest first,
+    // 3 4-wide records, 4 4-wide records with the watertight triangle test
     uint32_t traversal_mode() const {
         if (!wide_ok || (cfg.flags & RT_FLAG_REFERENCE_TRAVERSAL)) return 0u;
         if (cfg.flags & RT_FLAG_NEAREST_FIRST) return 2u;
         if ((cfg.flags & RT_FLAG_BINARY_RECORDS) || !wide4_ok) return 1u;
+        if ((cfg.flags & RT_FLAG_WATERTIGHT) && d_slot_tris_wt) return 4u;
         return 3u;
+    }
+    // The watertight traversal's slot triangles, built at the first config
+    // that asks for it.
+    void ensure_wt_slots() {
+        if (d_slot_tris_wt || !(cfg.flags & RT_FLAG_WATERTIGHT) || !wide4_ok) return;
+        hip_check(hipMalloc(&d_slot_tris_wt, (size_t)std::max(n_prims, 1u) * 48), "hipMalloc watertight slots");
+        hip_check(rtk::launch_wt_slots(d_tris, d_prims, n_prims, d_slot_tris_wt, stream), "watertight slots");
+        hip_check(hipStreamSynchronize(stream), "sync");
     }
 
     rt_image_params image_params(float alpha) const {
@@ -1043,7 +1054,7 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->temp_bytes = rtk::grid_build_temp_bytes(r->photon_slots, r->meta_cells);
     r->d_temp = dev_alloc<char>(r->temp_bytes, "grid temp");
     r->d_nvalid = dev_alloc<uint32_t>(1, "n_valid");
-    r->d_counters = dev_alloc<unsigned long long>(4, "counters");
+    r->d_counters = dev_alloc<unsigned long long>(10, "counters");
     r->d_regen = dev_alloc<uint32_t>(8u * (rt_renderer::kSets + 1u), "pixel counters");
 
     // texture arrays: the scene's, else the reference's 1x1 placeholder per role
@@ -1052,6 +1063,7 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
 
     // default camera (lib.rs:266-270)
     r->cam = rt::camera_to_uniform(rt::camera_default(), r->cfg.width, r->cfg.height);
+    r->ensure_wt_slots();
     hip_check(hipDeviceSynchronize(), "upload sync");
     return r.release();
 }
@@ -1256,6 +1268,7 @@ int rt_set_config(rt_renderer *r, const rt_config *cfg) {
             throw ApiError(RT_ERR_INVALID, "size, photon and grid fields are fixed at creation");
         validate_config(*cfg);
         r->cfg = *cfg;
+        r->ensure_wt_slots();
     });
 }
 
@@ -1485,7 +1498,7 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
         if (r->cfg.enable_photon_map && !r->photons_built)
             throw ApiError(RT_ERR_INVALID, "photon map enabled but not built");
-        hip_check(hipMemsetAsync(r->d_counters, 0, 32, r->stream), "clear counters");
+        hip_check(hipMemsetAsync(r->d_counters, 0, 80, r->stream), "clear counters");
         r->ensure_path_queues();
         const uint32_t saved = r->sample_counter;
         r->sample_counter = saved + 1u;  // the frame the next rt_render_frame would trace
@@ -1493,12 +1506,16 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         r->sample_counter = saved;
         g.counters = r->d_counters;
         hip_check(rtk::launch_gather(g, r->stream), "gather (counting)");
-        unsigned long long c[4];
+        unsigned long long c[10];
         read_dev(r, c, r->d_counters, sizeof(c));
         out->ray_queries = c[0];
         out->nodes_visited = c[1];
         out->tris_tested = c[2];
         out->photon_reads = c[3];
+        for (int k = 0; k < 3; k++) {
+            out->lane_steps[k] = c[4 + k];
+            out->wave_steps[k] = c[7 + k];
+        }
     });
 }
 

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import json
 import sys
 import threading
 import time
@@ -185,6 +186,71 @@ def train_denoiser(n_scenes: int = 64, steps: int = 800, batch: int = 32, crop: 
     return unet, losses, (t1 - t0, t2 - t1)
 
 
+def evaluate_held_out(unet: UNet, seeds, size: int = 256, ref_frames: int = 4096, photons: int = 200_000,
+                      eps: float = 1e-2) -> dict:
+    """relMSE (eps as in the convergence study) of the 1/2/4/8-frame
+    accumulation and of the network's image against a ref_frames FP64
+    reference, averaged over held-out training scenes `seeds`."""
+    plain = {k: [] for k in INPUT_FRAMES}
+    learned = {k: [] for k in INPUT_FRAMES}
+    for seed in seeds:
+        scene = Scene.builtin("training", seed)
+        cam = Scene.builtin_camera("training", seed, 0)
+        cfg = RenderConfig(width=size, height=size, spp=1, max_depth=8, enable_denoise=False,
+                           enable_photon_map=photons > 0, photon_count=max(photons, 1), accumulate_frames=1)
+        with Renderer(cfg, scene, 0) as r:
+            r.update_camera(cam)
+            if photons > 0:
+                r.rebuild_photons()
+            r.accum64_reset()
+            for _ in range(ref_frames):
+                r.record_frame()
+                r.accum64_add()
+            c = r.config
+            c.accumulate_frames = 1 << 20
+            r.config = c
+            start = r.sample_count() + 1000
+            for k in INPUT_FRAMES:
+                for net, out in ((None, plain), (unet, learned)):
+                    r.set_denoiser(net)
+                    r.reset_accumulation()
+                    r.set_sample_count(start)
+                    r.render_frames(k)
+                    out[k].append(r.accum64_relmse_display(eps))
+            r.set_denoiser(None)
+    return {"scenes": list(seeds), "reference_frames": ref_frames, "eps": eps,
+            "relmse_accumulation": {k: float(np.mean(v)) for k, v in plain.items()},
+            "relmse_network": {k: float(np.mean(v)) for k, v in learned.items()}}
+
+
+def profile_passes(log, seed: int = 1) -> int:
+    """Network passes on random inputs in a fixed order (for a kernel trace):
+    autotune + 5 inference passes at 1920x1088, autotune + 3 training passes
+    (forward + backward) at 32 x 256x256."""
+    rng = np.random.default_rng(seed)
+    unet = UNet(0, 1)
+
+    def inputs(n, h, w, target):
+        f = rng.normal(size=(n, h, w, 16)).astype(np.float32)
+        f[..., 12:] = 0
+        fb = (f.view(np.uint32) >> 16).astype(np.uint16)  # bf16 by truncation: any bits serve here
+        x = rng.normal(size=(n, h, w, 3)).astype(np.float32)
+        t = (x + 0.3 * rng.normal(size=x.shape)).astype(np.float32) if target else None
+        return fb, x, t
+
+    fb, x, _ = inputs(1, 1088, 1920, False)
+    log("inference 1920x1088: autotune + 5 passes")
+    for _ in range(6):
+        unet.eval(fb, x)
+    fb, x, t = inputs(32, 256, 256, True)
+    log("training 32 x 256x256: autotune + 3 passes")
+    for _ in range(4):
+        unet.eval(fb, x, t)
+    print(unet.conv_choices(), flush=True)
+    unet.close()
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--scenes", type=int, default=2000)
@@ -193,19 +259,37 @@ def main(argv=None) -> int:
     ap.add_argument("--crop", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--target-samples", type=int, default=TARGET_SAMPLES)
-    ap.add_argument("--out", default=None)
+    ap.add_argument("--out", default=None, help="write the trained parameters (float32) here")
+    ap.add_argument("--held-out", type=int, default=8, help="held-out scenes evaluated after training")
+    ap.add_argument("--profile-passes", action="store_true",
+                    help="only run network passes on random inputs, for a kernel trace: one autotuning pass and "
+                         "five inference passes at 1920x1088, one autotuning and three training passes at "
+                         "32 x 256x256; prints the chosen convolution variants")
     args = ap.parse_args(argv)
 
     def log(m):
         print(f"[denoiser] {m}", file=sys.stderr, flush=True)
 
+    if args.profile_passes:
+        return profile_passes(log)
+
     unet, losses, (data_s, train_s) = train_denoiser(args.scenes, args.steps, args.batch, args.crop, args.lr,
                                                      target_samples=args.target_samples, log=log)
-    print({"scenes": args.scenes, "steps": args.steps, "data_seconds": round(data_s, 2),
-           "train_seconds": round(train_s, 2), "loss_first10": float(losses[:10].mean()),
-           "loss_last10": float(losses[-10:].mean())}, flush=True)
+    samples = args.scenes * 256 * 256 * (args.target_samples + sum(INPUT_FRAMES))
+    res = {"scenes": args.scenes, "image": "256x256", "inputs": list(INPUT_FRAMES),
+           "target_samples": args.target_samples, "dataset_seconds": round(data_s, 2),
+           "dataset_gsamples_per_s": round(samples / data_s / 1e9, 3), "steps": args.steps,
+           "batch": f"{args.batch} x {args.crop}x{args.crop}", "train_seconds": round(train_s, 2),
+           "ms_per_step": round(train_s / max(args.steps, 1) * 1e3, 3),
+           "loss_first10": float(losses[:10].mean()), "loss_last10": float(losses[-10:].mean()),
+           "loss_curve": [float(losses[i:i + 100].mean()) for i in range(0, len(losses), max(1, len(losses) // 20))]}
     if args.out:
         unet.get_params().tofile(args.out)
+    if args.held_out > 0:
+        log(f"held-out evaluation on {args.held_out} scenes")
+        res["held_out"] = evaluate_held_out(unet, range(900_000, 900_000 + args.held_out),
+                                            ref_frames=args.target_samples)
+    print(json.dumps(res), flush=True)
     return 0
 
 

@@ -128,3 +128,34 @@ def first_diff(a: np.ndarray, b: np.ndarray) -> str:
         return "identical"
     i = tuple(bad[0])
     return f"{len(bad)} differing values, first at {i}: {a[i]!r} vs {b[i]!r}"
+
+
+def icosphere(levels, seed):
+    """A closed triangle mesh: an icosahedron subdivided `levels` times, its
+    vertices on the unit sphere with a 2% seeded radial jitter (so no two faces
+    are coplanar). Shared edges are stored once per triangle with the same
+    vertex indices. Returns (float32 vertices, uint32 faces)."""
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = [np.array(x, float) / np.linalg.norm(x) for x in v]
+    for _ in range(levels):
+        mid = {}
+        nf = []
+        for a, b, c in f:
+            ids = []
+            for i, j in ((a, b), (b, c), (c, a)):
+                key = (min(i, j), max(i, j))
+                if key not in mid:
+                    m = v[i] + v[j]
+                    v.append(m / np.linalg.norm(m))
+                    mid[key] = len(v) - 1
+                ids.append(mid[key])
+            nf += [(a, ids[0], ids[2]), (b, ids[1], ids[0]), (c, ids[2], ids[1]), (ids[0], ids[1], ids[2])]
+        f = nf
+    rng = np.random.default_rng(seed)
+    V = np.array(v) * (1 + 0.02 * rng.standard_normal((len(v), 1)))
+    return V.astype(np.float32), np.array(f, np.uint32)

@@ -50,6 +50,8 @@ def lib():
         vp, u32 = C.c_void_p, C.c_uint32
         L.or_trace_closest.argtypes = [C.POINTER(OrScene), vp, u32, vp, C.POINTER(OrStats)]
         L.or_trace_brute.argtypes = [C.POINTER(OrScene), vp, u32, vp]
+        L.or_trace_brute_watertight.argtypes = [C.POINTER(OrScene), vp, u32, vp]
+        L.or_watertight_t.argtypes = [C.POINTER(OrScene), vp, vp, u32, vp]
         L.or_gather.argtypes = [C.POINTER(OrScene), C.POINTER(CameraUniform), C.POINTER(ImageParams),
                                 C.POINTER(OrPhotonMap), u32, u32, u32, vp, vp, vp, C.c_int, C.POINTER(OrStats)]
         L.or_photon_trace.argtypes = [C.POINTER(OrScene), C.POINTER(GridParams), u32, u32, vp, C.c_int]
@@ -74,7 +76,7 @@ def lib():
         L.or_intersect_triangle.argtypes = [vp, vp, vp, vp, vp]
         L.or_intersect_triangle.restype = C.c_float
         L.or_math_eval.argtypes = [C.c_int, vp, vp, vp, u32]
-        for fn in (L.or_trace_closest, L.or_trace_brute, L.or_gather, L.or_photon_trace, L.or_photon_grid,
+        for fn in (L.or_trace_closest, L.or_trace_brute, L.or_trace_brute_watertight, L.or_watertight_t, L.or_gather, L.or_photon_trace, L.or_photon_grid,
                    L.or_denoise_pass, L.or_composite, L.or_tonemap, L.or_tonemap_cpu, L.or_project_marker,
                    L.or_photon_estimate):
             fn.restype = C.c_int
@@ -140,6 +142,21 @@ class OracleScene:
         r = np.ascontiguousarray(rays, RAY)
         out = np.zeros(len(r), HIT)
         assert lib().or_trace_brute(C.byref(self.s), _p(r), len(r), _p(out)) == 0
+        return out
+
+    def brute_watertight(self, rays):
+        """Closest watertight hit over every triangle (lowest index wins a tie)."""
+        r = np.ascontiguousarray(rays, RAY)
+        out = np.zeros(len(r), HIT)
+        assert lib().or_trace_brute_watertight(C.byref(self.s), _p(r), len(r), _p(out)) == 0
+        return out
+
+    def watertight_t(self, rays, prim_ids):
+        """The watertight t of each ray against its own triangle."""
+        r = np.ascontiguousarray(rays, RAY)
+        p = np.ascontiguousarray(prim_ids, np.uint32)
+        out = np.zeros(len(r), np.float32)
+        assert lib().or_watertight_t(C.byref(self.s), _p(r), _p(p), len(r), _p(out)) == 0
         return out
 
     def gather(self, cam, ip, max_depth=6, rows=None, photon_map=None, threads=8):

@@ -346,7 +346,10 @@ def test_camera_pass_forms_same_work(scene_name, spp, photons, other):
     a, b = out[0], out[other]
     for name, x, y in zip(("frame", "moments", "gbuffer", "accumulation", "rgba8"), a[:5], b[:5]):
         _check_equal(name, x, y)
-    assert a[5] == b[5], (a[5], b[5])
+    # per-lane work is the same; wave steps and active lanes per wave step depend
+    # on which rays share a wave, which is what the forms change
+    per_lane = {k: v for k, v in a[5].items() if k not in ("wave_steps", "active_lanes")}
+    assert per_lane == {k: v for k, v in b[5].items() if k not in ("wave_steps", "active_lanes")}, (a[5], b[5])
     assert a[5]["ray_queries"] > 97 * 61 * spp
 
 

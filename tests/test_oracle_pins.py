@@ -353,3 +353,50 @@ def test_photon_emission_per_light_kind():
     zen, hor = np.array([0.2, 0.4, 0.9]), np.array([0.9, 0.8, 0.6])
     tw = np.sqrt(np.clip(e[:, 1] * 0.5 + 0.5, 0, 1))[:, None]
     assert np.allclose(P, (zen * (1 - tw) + hor * tw) * 0.7, rtol=2e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+def test_alpha_cutout_pass_through(depth):
+    """An emissive cutout quad (2x2 alpha checker, bilinear/repeat) in front of
+    a second emitter: a camera ray meeting an opaque texel sees the front
+    emission E2; at a transparent texel (alpha < 0.5) the path continues 0.002
+    behind it and the bounce count advances (gather.wgsl:315-323), so it sees
+    the back emitter E1 with depth 2 and nothing with depth 1."""
+    E1, E2 = np.array([1.0, 2.0, 3.0], F), np.array([4.0, 0.5, 0.25], F)
+    s = rt.Scene()
+    back = s.add_material(rt.make_material(3, albedo=(0, 0, 0), emissive=tuple(E1)))
+    s.add_mesh([[-4, -4, -5], [4, -4, -5], [4, 4, -5], [-4, 4, -5]], [0, 1, 2, 0, 2, 3], back,
+               normals=[[0, 0, 1]] * 4)
+    tex = np.ones((2, 2, 4), np.float32)
+    tex[0, 1, 3] = tex[1, 0, 3] = 0.0  # alpha checker: texel (x, y) opaque when x == y
+    layer = s.add_texture(0, tex)
+    front = rt.make_material(3, albedo=(0, 0, 0), emissive=tuple(E2))
+    front["tex_albedo"] = layer
+    front["flags"] = 1  # RT_MAT_FLAG_ALPHA_CUTOUT
+    m = s.add_material(front)
+    s.add_mesh([[-1, -1, -3], [1, -1, -3], [1, 1, -3], [-1, 1, -3]], [0, 1, 2, 0, 2, 3], m,
+               normals=[[0, 0, 1]] * 4, uvs=[[0, 0], [1, 0], [1, 1], [0, 1]])
+    w = h = 64
+    cam = rt.camera((0.0, 0.0, 0.0), (0.0, 0.0, -1.0), fov_degrees=40)
+    cfg = rt.RenderConfig(width=w, height=h, spp=4, ambient=0.0, enable_denoise=False, max_depth=depth)
+    frame, _, _, _ = orc.OracleScene.from_scene(s).gather(rt.camera_to_uniform(cam, w, h), image_params(cfg, 1),
+                                                          max_depth=depth)
+    # pixel footprint -> quad uv (u = (x + 1) / 2, v = (y + 1) / 2 at distance 3)
+    fy = np.tan(np.radians(20))
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+    opaque = np.ones((h, w), bool)
+    clear = np.ones((h, w), bool)
+    for jx in (-0.5, 0.0, 0.5):
+        for jy in (-0.5, 0.0, 0.5):
+            u = (3 * ((xs + jx) * 2 / w - 1) * fy + 1) / 2
+            v = (3 * (1 - (ys + jy) * 2 / h) * fy + 1) / 2
+            inside = (u > 0.02) & (u < 0.98) & (v > 0.02) & (v < 0.98)
+            # near a texel centre (0.25 / 0.75): the bilinear alpha is that texel's
+            near = (np.abs((u % 0.5) - 0.25) < 0.1) & (np.abs((v % 0.5) - 0.25) < 0.1)
+            tx, ty = (u > 0.5).astype(int), (v > 0.5).astype(int)
+            opaque &= inside & near & (tx == ty)
+            clear &= inside & near & (tx != ty)
+    assert opaque.sum() > 50 and clear.sum() > 50
+    assert np.all(frame[opaque][:, :3] == E2)
+    want = E1 if depth == 2 else np.zeros(3, F)
+    assert np.all(frame[clear][:, :3] == want)
