@@ -351,16 +351,19 @@ int rt_reset_pass_stats(rt_renderer *r);
 /* Traversal counters of one extra, untimed gather launch of the current frame
  * state (rays, BVH nodes loaded, triangle records loaded, photon records read),
  * for the algorithmic-bytes figure of the roofline. */
+#define RT_COUNT_PHASES 5
 typedef struct rt_trace_counters {
     uint64_t ray_queries;
     uint64_t nodes_visited;
     uint64_t tris_tested;
     uint64_t photon_reads;
-    /* loop steps per phase (0 traversal, 1 shading: one bounce, 2 photon
-     * search: one batch of candidates), summed over lanes and over wavefronts:
-     * lane_steps / wave_steps = mean active lanes of 64 in that phase */
-    uint64_t lane_steps[3];
-    uint64_t wave_steps[3];
+    /* loop steps per phase, summed over lanes and over wavefronts:
+     * lane_steps / wave_steps = mean active lanes of 64 in that phase.
+     * Phases: 0 traversal steps of closest-hit queries, 1 one bounce body,
+     * 2 one batch of photon candidates, 3 traversal steps of shadow queries,
+     * 4 one iteration of the persistent camera pass's loop (live lanes) */
+    uint64_t lane_steps[RT_COUNT_PHASES];
+    uint64_t wave_steps[RT_COUNT_PHASES];
 } rt_trace_counters;
 int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out);
 

@@ -434,7 +434,7 @@ class Renderer:
     def count_gather_traffic(self) -> dict:
         t = _ffi.TraceCounters()
         check(lib().rt_count_gather_traffic(self._h, C.byref(t)), "rt_count_gather_traffic")
-        phases = ("traversal", "shading", "photon_search")
+        phases = ("traversal", "shading", "photon_search", "shadow_traversal", "camera_loop")
         return dict(ray_queries=t.ray_queries, nodes_visited=t.nodes_visited, tris_tested=t.tris_tested,
                     photon_reads=t.photon_reads,
                     active_lanes={p: round(t.lane_steps[k] / max(t.wave_steps[k], 1), 2) for k, p in enumerate(phases)},

@@ -14,6 +14,7 @@
 #include "../../include/rt_texture.h"
 #include "../../include/rt_types.h"
 #include "../../include/rt_wgsl_math.h"
+#include "../../include/rustracer_hip.h"
 
 namespace rtd {
 
@@ -192,11 +193,16 @@ struct Hit {
 };
 
 // Counters of the counting build. step_lane[p] / step_wave[p]: loop steps of
-// phase p (0 traversal, 1 shading (one bounce), 2 photon search) summed over
-// lanes and over wavefronts; their ratio is the mean number of active lanes.
+// phase p summed over lanes and over wavefronts; their ratio is the mean number
+// of active lanes. Phases: traversal steps of closest-hit queries, one bounce
+// body, one batch of photon candidates, traversal steps of shadow queries, one
+// iteration of the persistent camera pass's loop (its live lanes).
+enum CountPhase { kPhaseTraversal = 0, kPhaseBounce, kPhasePhotons, kPhaseShadow, kPhaseLoop, kCountPhases };
+static_assert(kCountPhases == RT_COUNT_PHASES, "counter layout of rt_trace_counters");
 struct TraceCount {
     uint32_t nodes, tris;
-    uint32_t step_lane[3], step_wave[3];
+    uint32_t step_lane[kCountPhases], step_wave[kCountPhases];
+    uint32_t trav_phase;  // the phase the traversal loop counts into
 };
 __device__ __forceinline__ void count_step(TraceCount &c, int phase) {
     c.step_lane[phase]++;
@@ -364,7 +370,7 @@ __device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, L
     uint32_t idx = 0u, sp = 0u;
     if (S.n_nodes == 0u) return hit;
     for (;;) {
-        if (COUNT) count_step(cnt, 0);
+        if (COUNT) count_step(cnt, cnt.trav_phase);
         const rt_bvh_node &n = S.nodes[idx];
         const float4 a = *reinterpret_cast<const float4 *>(&n.bbox_min[0]);
         const float4 b = *reinterpret_cast<const float4 *>(&n.bbox_max[0]);
@@ -441,7 +447,7 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
     uint32_t sp = 0u;
     uint32_t cur = __float_as_uint(S.root_min.w);  // root ref (internal 0, or a leaf ref)
     for (;;) {
-        if (COUNT) count_step(cnt, 0);
+        if (COUNT) count_step(cnt, cnt.trav_phase);
         if (ref_is_leaf(cur)) {
             const uint32_t first = cur & 0x07FFFFFFu;
             const uint32_t count = ((cur >> 27) & 15u) + 1u;
@@ -547,7 +553,7 @@ __device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneS
     uint32_t sp = 0u;
     uint32_t cur = __float_as_uint(S.root_min.w);
     for (;;) {
-        if (COUNT) count_step(cnt, 0);
+        if (COUNT) count_step(cnt, cnt.trav_phase);
         if (ref_is_leaf(cur)) {
             const uint32_t first = cur & 0x07FFFFFFu;
             const uint32_t count = ((cur >> 27) & 15u) + 1u;

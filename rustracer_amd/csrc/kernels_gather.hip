@@ -60,19 +60,39 @@ struct Tracer {
     TraceCount cnt;
     uint32_t queries;
     __device__ Hit closest(F3 ro, F3 rd) {
-        if (COUNT) queries++;
+        if (COUNT) {
+            queries++;
+            cnt.trav_phase = kPhaseTraversal;
+        }
         return trace_mode<MODE, false, COUNT>(ts, ro, rd, kInf, stk, cnt);
     }
     // gather.wgsl:92-93: is the closest hit nearer than max_dist - 0.001? EARLY
     // stops the same traversal as soon as a hit that near is found (then `h`
     // is that hit, not necessarily the closest).
     __device__ bool shadowed(F3 origin, F3 dir, float max_dist, Hit &h) {
-        if (COUNT) queries++;
+        if (COUNT) {
+            queries++;
+            cnt.trav_phase = kPhaseShadow;
+        }
         const float lim = max_dist - 0.001f;
         h = trace_mode<MODE, EARLY, COUNT>(ts, origin, dir, lim, stk, cnt);
         return !(h.t >= lim);
     }
 };
+
+// The counting build's totals of one lane: counters[0..3] queries, child boxes,
+// triangles, photon candidates; then the lane and wave steps of each phase.
+template <class T>
+__device__ void flush_counts(unsigned long long *counters, const T &tr, uint32_t photon_reads) {
+    atomicAdd(&counters[0], (unsigned long long)tr.queries);
+    atomicAdd(&counters[1], (unsigned long long)tr.cnt.nodes);
+    atomicAdd(&counters[2], (unsigned long long)tr.cnt.tris);
+    atomicAdd(&counters[3], (unsigned long long)photon_reads);
+    for (int k = 0; k < kCountPhases; k++) {
+        atomicAdd(&counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
+        atomicAdd(&counters[4 + kCountPhases + k], (unsigned long long)tr.cnt.step_wave[k]);
+    }
+}
 
 // gather.wgsl:91-103 with alpha-cutout: the closest hit decides, and a
 // transparent texel of a cutout material lets the shadow ray through. Scenes
@@ -117,7 +137,11 @@ __device__ __forceinline__ F3 sky_color(F3 dir, F3 zenith, F3 horizon, float int
     return mix(horizon, zenith, wgsl_pow(t, 0.5f)) * intensity;
 }
 
-// gather.wgsl:152-206 (out-of-range light index clamped to n-1)
+// gather.wgsl:152-206 (out-of-range light index clamped to n-1). Each light
+// kind sets the direction, the shadow ray's reach and the unoccluded
+// contribution; one shadow query after the branches serves every kind, so the
+// lanes of a wavefront that picked different kinds traverse together. The
+// arithmetic of each kind is the shader's, in its order.
 template <class T>
 __device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rng &rng) {
     const uint32_t n = A.n_lights;
@@ -125,21 +149,28 @@ __device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rn
     li = li < n ? li : n - 1u;
     const rt_light &l = A.lights[li];
     const F3 ldata = ld3(l.data), lcolor = ld3(l.color);
+    F3 ld = sn, e = fs(0.0f);
+    float reach = 0.0f;
+    bool cast = false;
     if (l.kind == 0u) {
         const F3 to = ldata - hp;
         const float r2 = dot(to, to);
         const float r = wgsl_sqrt(r2);
-        const F3 ld = to / r;
-        if (dot(sn, ld) <= 0.0f) return LightSample{fs(0.0f), ld};
-        if (occluded(A, tr, hp + sn * 0.01f, ld, r - 0.001f)) return LightSample{fs(0.0f), ld};
-        const F3 ple = ld3(A.ip.point_light_emission);
-        return LightSample{lcolor * l.intensity * ple * wmax(dot(sn, ld), 0.0f) / (r2 + 0.01f) * (float)n, ld};
+        ld = to / r;
+        if (!(dot(sn, ld) <= 0.0f)) {  // the shader's `<= 0 -> return`, NaN included
+            cast = true;
+            reach = r - 0.001f;
+            const F3 ple = ld3(A.ip.point_light_emission);
+            e = lcolor * l.intensity * ple * wmax(dot(sn, ld), 0.0f) / (r2 + 0.01f) * (float)n;
+        }
     } else if (l.kind == 1u) {
-        const F3 ld = normalize(ldata);
+        ld = normalize(ldata);
         const float ndl = dot(sn, ld);
-        if (ndl <= 0.0f) return LightSample{fs(0.0f), ld};
-        if (occluded(A, tr, hp + sn * 0.01f, ld, 1e30f)) return LightSample{fs(0.0f), ld};
-        return LightSample{lcolor * l.intensity * ndl * (float)n, ld};
+        if (!(ndl <= 0.0f)) {
+            cast = true;
+            reach = 1e30f;
+            e = lcolor * l.intensity * ndl * (float)n;
+        }
     } else if (l.kind == 2u) {
         uint32_t ti = wgsl_u32(l.data[0]);
         ti = ti < A.n_tris ? ti : A.n_tris - 1u;
@@ -162,17 +193,21 @@ __device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rn
         const F3 to = lp - hp;
         const float r2 = dot(to, to);
         const float r = wgsl_sqrt(r2);
-        const F3 ld = to / r;
+        ld = to / r;
         const float ndl = dot(sn, ld);
-        if (ndl <= 0.0f) return LightSample{fs(0.0f), ld};
-        const F3 ln = normalize(cross(v1 - v0, v2 - v0));
-        const float ndl_l = dot(ln, -ld);
-        if (ndl_l <= 0.0f) return LightSample{fs(0.0f), ld};
-        if (occluded(A, tr, hp + sn * 0.01f, ld, r - 0.001f)) return LightSample{fs(0.0f), ld};
-        const float area = 0.5f * length(cross(v1 - v0, v2 - v0));
-        return LightSample{le * l.intensity * ndl * ndl_l * area * (float)n / (r2 + 0.01f), ld};
+        if (!(ndl <= 0.0f)) {
+            const F3 ln = normalize(cross(v1 - v0, v2 - v0));
+            const float ndl_l = dot(ln, -ld);
+            if (!(ndl_l <= 0.0f)) {
+                cast = true;
+                reach = r - 0.001f;
+                const float area = 0.5f * length(cross(v1 - v0, v2 - v0));
+                e = le * l.intensity * ndl * ndl_l * area * (float)n / (r2 + 0.01f);
+            }
+        }
     }
-    return LightSample{fs(0.0f), sn};
+    if (cast && occluded(A, tr, hp + sn * 0.01f, ld, reach)) e = fs(0.0f);
+    return LightSample{e, ld};
 }
 
 // gather.wgsl:210-227
@@ -305,7 +340,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                        F3 &rd, F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
                        uint32_t py, bool *smooth = nullptr) {
     if (smooth) *smooth = false;
-    if (COUNT) count_step(tr.cnt, 1);
+    if (COUNT) count_step(tr.cnt, kPhaseBounce);
     const rt_image_params &ip = A.ip;
     const uint32_t debug_view = ip._pad[0];
     const Hit hit = tr.closest(ro, rd);
@@ -452,7 +487,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                     if (k >= n_ranges) break;
                     const uint2 meta = metas[k];
                     for (uint32_t i = 0u; i < meta.y && !done; i += kPhotonBatch) {
-                        if (COUNT) count_step(tr.cnt, 2);
+                        if (COUNT) count_step(tr.cnt, kPhasePhotons);
                         // one round trip per batch: position, power and incident
                         // direction of each candidate come in one 48-byte record
                         const uint32_t nb = min(meta.y - i, kPhotonBatch);
@@ -672,14 +707,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     if (px < A.ip.width && py < A.ip.height)
         gather_pixel<MODE, EARLY, COUNT>(A, V, tr, px, py, (size_t)py * A.ip.width + px, photon_reads);
     if (COUNT) {
-        atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
-        atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
-        atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
-        atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
-        for (int k = 0; k < 3; k++) {
-            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
-            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
-        }
+        flush_counts(A.counters, tr, photon_reads);
     }
 }
 
@@ -770,6 +798,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
         }
         if (__ballot(live) == 0ull) break;
         if (live) {
+            if (COUNT) count_step(tr.cnt, kPhaseLoop);
             const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads, px, py);
             b++;
             if (!cont || b >= A.max_depth) {
@@ -787,14 +816,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
         }
     }
     if (COUNT) {
-        atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
-        atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
-        atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
-        atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
-        for (int k = 0; k < 3; k++) {
-            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
-            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
-        }
+        flush_counts(A.counters, tr, photon_reads);
     }
 }
 
@@ -934,14 +956,7 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     const uint32_t back = wave_append(push && smooth, &A.path_count[2]);
     if (push) queue_put(A.queue[1], A.n_px, smooth ? A.n_px - 1u - back : front, ro, rd, thr, idx, rng.state);
     if (COUNT) {
-        atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
-        atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
-        atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
-        atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
-        for (int k = 0; k < 3; k++) {
-            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
-            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
-        }
+        flush_counts(A.counters, tr, photon_reads);
     }
 }
 
@@ -992,14 +1007,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
         }
     }
     if (COUNT) {
-        atomicAdd(&A.counters[0], (unsigned long long)tr.queries);
-        atomicAdd(&A.counters[1], (unsigned long long)tr.cnt.nodes);
-        atomicAdd(&A.counters[2], (unsigned long long)tr.cnt.tris);
-        atomicAdd(&A.counters[3], (unsigned long long)photon_reads);
-        for (int k = 0; k < 3; k++) {
-            atomicAdd(&A.counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
-            atomicAdd(&A.counters[7 + k], (unsigned long long)tr.cnt.step_wave[k]);
-        }
+        flush_counts(A.counters, tr, photon_reads);
     }
 }
 

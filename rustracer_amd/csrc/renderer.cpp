@@ -34,6 +34,9 @@ T *dev_alloc(size_t count, const char *what) {
 }
 
 constexpr uint32_t kTraversalStackWords = rtk::kTraversalStackWords;
+// counting launch: queries, child boxes, triangles, photon candidates, then
+// lane steps and wave steps per phase
+constexpr int kCounterWords = 4 + 2 * RT_COUNT_PHASES;
 
 struct PendingTiming {
     int pass;
@@ -1054,7 +1057,7 @@ static rt_renderer *create_renderer(const rt_config &cfg_in, const rt_triangle *
     r->temp_bytes = rtk::grid_build_temp_bytes(r->photon_slots, r->meta_cells);
     r->d_temp = dev_alloc<char>(r->temp_bytes, "grid temp");
     r->d_nvalid = dev_alloc<uint32_t>(1, "n_valid");
-    r->d_counters = dev_alloc<unsigned long long>(10, "counters");
+    r->d_counters = dev_alloc<unsigned long long>(kCounterWords, "counters");
     r->d_regen = dev_alloc<uint32_t>(8u * (rt_renderer::kSets + 1u), "pixel counters");
 
     // texture arrays: the scene's, else the reference's 1x1 placeholder per role
@@ -1498,7 +1501,8 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         if (!r || !out) throw ApiError(RT_ERR_INVALID, "null argument");
         if (r->cfg.enable_photon_map && !r->photons_built)
             throw ApiError(RT_ERR_INVALID, "photon map enabled but not built");
-        hip_check(hipMemsetAsync(r->d_counters, 0, 80, r->stream), "clear counters");
+        hip_check(hipMemsetAsync(r->d_counters, 0, sizeof(unsigned long long) * kCounterWords, r->stream),
+                  "clear counters");
         r->ensure_path_queues();
         const uint32_t saved = r->sample_counter;
         r->sample_counter = saved + 1u;  // the frame the next rt_render_frame would trace
@@ -1506,15 +1510,15 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         r->sample_counter = saved;
         g.counters = r->d_counters;
         hip_check(rtk::launch_gather(g, r->stream), "gather (counting)");
-        unsigned long long c[10];
+        unsigned long long c[kCounterWords];
         read_dev(r, c, r->d_counters, sizeof(c));
         out->ray_queries = c[0];
         out->nodes_visited = c[1];
         out->tris_tested = c[2];
         out->photon_reads = c[3];
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < RT_COUNT_PHASES; k++) {
             out->lane_steps[k] = c[4 + k];
-            out->wave_steps[k] = c[7 + k];
+            out->wave_steps[k] = c[4 + RT_COUNT_PHASES + k];
         }
     });
 }
