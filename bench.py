@@ -326,6 +326,21 @@ def main() -> int:
             result["roofline"]["note"] = ("bound by VALU issue, not HBM: see valu_roofline; the algorithmic bytes "
                                           "are served from L2/MALL ('traffic' is the measured HBM traffic)")
 
+    # where the camera pass's time goes: shares of the wavefronts' cycles per
+    # phase, from the shader clock read at the phase boundaries in the counting
+    # launch (bounce = closest-hit traversal + shading + shadow traversal +
+    # photon search; the loop adds the pixel hand-out and sample ends)
+    cyc = cnt.get("wave_cycles", {})
+    if cyc.get("camera_loop"):
+        tot = cyc["camera_loop"]
+        inner = cyc["traversal"] + cyc["shadow_traversal"] + cyc["photon_search"]
+        result["gather_cycle_share"] = {
+            "closest_traversal": round(cyc["traversal"] / tot, 4),
+            "shadow_traversal": round(cyc["shadow_traversal"] / tot, 4),
+            "photon_search": round(cyc["photon_search"] / tot, 4),
+            "rest_of_bounce": round((cyc["shading"] - inner) / tot, 4),
+            "refill_and_sample_end": round((tot - cyc["shading"]) / tot, 4)}
+
     if elapsed_other is not None:
         key = "msamples_per_s_render_frames" if args.record_frame else "msamples_per_s_record_frame"
         result[key] = round(w * h * args.steps / elapsed_other / 1e6, 3)

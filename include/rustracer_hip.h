@@ -364,6 +364,10 @@ typedef struct rt_trace_counters {
      * 4 one iteration of the persistent camera pass's loop (live lanes) */
     uint64_t lane_steps[RT_COUNT_PHASES];
     uint64_t wave_steps[RT_COUNT_PHASES];
+    /* shader-clock cycles wavefronts spent in each phase's code regions, summed
+     * over wavefronts (phase 1: whole bounces, their traversals and photon
+     * search included; phase 4: whole iterations of the persistent loop) */
+    uint64_t wave_cycles[RT_COUNT_PHASES];
 } rt_trace_counters;
 int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out);
 

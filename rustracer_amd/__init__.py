@@ -439,7 +439,8 @@ class Renderer:
                     photon_reads=t.photon_reads,
                     active_lanes={p: round(t.lane_steps[k] / max(t.wave_steps[k], 1), 2) for k, p in enumerate(phases)},
                     lane_steps={p: t.lane_steps[k] for k, p in enumerate(phases)},
-                    wave_steps={p: t.wave_steps[k] for k, p in enumerate(phases)})
+                    wave_steps={p: t.wave_steps[k] for k, p in enumerate(phases)},
+                    wave_cycles={p: t.wave_cycles[k] for k, p in enumerate(phases)})
 
     # -- many views in one persistent launch --
     def set_views(self, views) -> None:

@@ -133,7 +133,8 @@ COUNT_PHASES = 5  # RT_COUNT_PHASES
 class TraceCounters(C.Structure):
     _fields_ = [("ray_queries", C.c_uint64), ("nodes_visited", C.c_uint64),
                 ("tris_tested", C.c_uint64), ("photon_reads", C.c_uint64),
-                ("lane_steps", C.c_uint64 * COUNT_PHASES), ("wave_steps", C.c_uint64 * COUNT_PHASES)]
+                ("lane_steps", C.c_uint64 * COUNT_PHASES), ("wave_steps", C.c_uint64 * COUNT_PHASES),
+                ("wave_cycles", C.c_uint64 * COUNT_PHASES)]
 
 
 for _s, _size in ((Config, 96), (CameraUniform, 96), (ImageParams, 80), (GridParams, 48),

@@ -199,32 +199,45 @@ struct Hit {
 // iteration of the persistent camera pass's loop (its live lanes).
 enum CountPhase { kPhaseTraversal = 0, kPhaseBounce, kPhasePhotons, kPhaseShadow, kPhaseLoop, kCountPhases };
 static_assert(kCountPhases == RT_COUNT_PHASES, "counter layout of rt_trace_counters");
+// wave_cycles[p]: shader-clock cycles wavefronts spent in regions of phase p
+// (each execution of a region by a wavefront counted once, by its first active
+// lane); for the loop phase, whole iterations of the camera pass's loop.
 struct TraceCount {
     uint32_t nodes, tris;
     uint32_t step_lane[kCountPhases], step_wave[kCountPhases];
     uint32_t trav_phase;  // the phase the traversal loop counts into
+    uint64_t wave_cycles[kCountPhases];
 };
+__device__ __forceinline__ bool first_active_lane() {
+    const unsigned long long active = __ballot(1);
+    return __lane_id() == (uint32_t)(__ffsll((long long)active) - 1);
+}
 __device__ __forceinline__ void count_step(TraceCount &c, int phase) {
     c.step_lane[phase]++;
-    const unsigned long long active = __ballot(1);
-    if (__lane_id() == (uint32_t)(__ffsll((long long)active) - 1)) c.step_wave[phase]++;
+    if (first_active_lane()) c.step_wave[phase]++;
+}
+// Time of a region of phase `phase` that started at t0 = clock64().
+__device__ __forceinline__ void count_time(TraceCount &c, int phase, uint64_t t0) {
+    const uint64_t dt = clock64() - t0;
+    if (first_active_lane()) c.wave_cycles[phase] += dt;
 }
 
 // Möller–Trumbore on a slot record, common.wgsl:78-100 (edges precomputed).
+// Every quantity is computed and the shader's early returns become one
+// select: the same value for every input (NaNs included), without branches
+// (in a wavefront some lane nearly always gets past each test anyway).
 __device__ __forceinline__ float mt_slot(F3 ro, F3 rd, F3 v0, F3 edge1, F3 edge2) {
     const F3 h = cross(rd, edge2);
     const float a = dot(edge1, h);
-    if (__builtin_fabsf(a) < 1e-7f) return kInf;
     const float f = 1.0f / a;
     const F3 s = ro - v0;
     const float u = f * dot(s, h);
-    if (u < 0.0f || u > 1.0f) return kInf;
     const F3 q = cross(s, edge1);
     const float v = f * dot(rd, q);
-    if (v < 0.0f || u + v > 1.0f) return kInf;
     const float t = f * dot(edge2, q);
-    if (t > kEps) return t;
-    return kInf;
+    const bool hit = !(__builtin_fabsf(a) < 1e-7f) & !(u < 0.0f) & !(u > 1.0f) & !(v < 0.0f) & !(u + v > 1.0f) &
+                     (t > kEps);
+    return hit ? t : kInf;
 }
 
 // Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight
@@ -282,30 +295,64 @@ __device__ __forceinline__ float wt_slot(const WtRay &w, F3 ro, F3 v0, F3 v1, F3
     return t > kEps ? t : kInf;
 }
 
-// Per-lane traversal stack: the first kLdsStack entries live in LDS
-// (lane-interleaved rows, one bank per lane), deeper entries in a private
-// array owned by the kernel (scratch memory). The stack holder itself is a
-// pair of registers; the spill array is only touched past kLdsStack entries.
+// Per-lane traversal stack of words: the first kLdsStack words live in LDS,
+// deeper words in a private array owned by the kernel (scratch memory). The
+// LDS part is laid out in pairs: words 2j and 2j+1 of a lane are adjacent,
+// and pair row j holds the pairs of the 64 lanes of the wavefront side by
+// side, so a two-word entry (record, entry distance) is one 64-bit access and
+// a row access by the wavefront covers 512 contiguous bytes. The spill array
+// is only touched past kLdsStack words.
 constexpr int kLdsStack = 32;
 constexpr int kStackWords = 96;
 constexpr int kSpillWords = kStackWords - kLdsStack;
 constexpr int kSpillAlloc = kSpillWords > 0 ? kSpillWords : 1;
+constexpr uint32_t kStackLanes = 64;  // lanes sharing the LDS rows (one wavefront per workgroup)
 
 // Address-space-qualified pointers, so stack accesses compile to ds_* (LDS)
 // and scratch_* (private) instructions instead of flat ones.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u32x2 lds_u2;
 typedef __attribute__((address_space(5))) uint32_t priv_u32;
 
 struct LaneStack {
-    lds_u32 *lds;     // &shared[row 0][lane]; row stride = blockDim
-    uint32_t stride;
+    lds_u32 *lds;     // this lane's pair 0 in the workgroup's kLdsStack * 64 LDS words
     priv_u32 *spill;  // kSpillAlloc private words
+    __device__ __forceinline__ void init(uint32_t *rows, uint32_t *spill_words) {
+        lds = (lds_u32 *)&rows[2u * (threadIdx.x % kStackLanes)];
+        spill = (priv_u32 *)spill_words;
+    }
+    __device__ __forceinline__ static uint32_t at(uint32_t w) { return (w >> 1) * (2u * kStackLanes) + (w & 1u); }
     __device__ __forceinline__ void put(uint32_t sp, uint32_t v) {
-        if (kSpillWords == 0 || sp < (uint32_t)kLdsStack) lds[sp * stride] = v;
+        if (kSpillWords == 0 || sp < (uint32_t)kLdsStack) lds[at(sp)] = v;
         else spill[sp - kLdsStack] = v;
     }
     __device__ __forceinline__ uint32_t get(uint32_t sp) const {
-        return (kSpillWords == 0 || sp < (uint32_t)kLdsStack) ? lds[sp * stride] : spill[sp - kLdsStack];
+        return (kSpillWords == 0 || sp < (uint32_t)kLdsStack) ? lds[at(sp)] : spill[sp - kLdsStack];
+    }
+    // Two-word entries at an even word index (pair row sp / 2).
+    __device__ __forceinline__ static uint32_t at2(uint32_t sp) { return sp * kStackLanes; }
+    __device__ __forceinline__ void put2_lds(uint32_t sp, uint32_t a, uint32_t b) {
+        u32x2 v;
+        v.x = a;
+        v.y = b;
+        *(lds_u2 *)&lds[at2(sp)] = v;
+    }
+    __device__ __forceinline__ uint2 get2_lds(uint32_t sp) const {
+        const u32x2 v = *(const lds_u2 *)&lds[at2(sp)];
+        return make_uint2(v.x, v.y);
+    }
+    __device__ __forceinline__ void put2(uint32_t sp, uint32_t a, uint32_t b) {
+        if (kSpillWords == 0 || sp < (uint32_t)kLdsStack) {
+            put2_lds(sp, a, b);
+        } else {
+            spill[sp - kLdsStack] = a;
+            spill[sp + 1u - kLdsStack] = b;
+        }
+    }
+    __device__ __forceinline__ uint2 get2(uint32_t sp) const {
+        if (kSpillWords == 0 || sp < (uint32_t)kLdsStack) return get2_lds(sp);
+        return make_uint2(spill[sp - kLdsStack], spill[sp + 1u - kLdsStack]);
     }
 };
 
@@ -480,8 +527,7 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
             const uint32_t lref = __float_as_uint(a.w), rref = __float_as_uint(b.w);
             if (hl && hr) {
                 const bool lfirst = NEAREST ? (tel <= ter) : true;
-                stk.put(sp, lfirst ? rref : lref);
-                stk.put(sp + 1u, __float_as_uint(lfirst ? ter : tel));
+                stk.put2(sp, lfirst ? rref : lref, __float_as_uint(lfirst ? ter : tel));
                 sp += 2u;
                 cur = lfirst ? lref : rref;
                 continue;
@@ -495,9 +541,10 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
         bool found = false;
         while (sp > 0u) {
             sp -= 2u;
-            const float et = __uint_as_float(stk.get(sp + 1u));
+            const uint2 e = stk.get2(sp);
+            const float et = __uint_as_float(e.y);
             if (NEAREST ? et <= hit.t : et < hit.t) {
-                cur = stk.get(sp);
+                cur = e.x;
                 found = true;
                 break;
             }
@@ -533,118 +580,202 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
 // the watertight test instead of Moller-Trumbore, and boxes with the
 // conservative slab_wt (traversal mode 4): the closest hit is the one a
 // watertight test of every triangle finds.
-template <bool EARLY_EXIT, bool COUNT, bool WT = false>
-__device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
-    Hit hit{kInf, 0u};
-    const F3 inv = fs(1.0f) / rd;
-    const WtRay wr = WT ? wt_ray(rd) : WtRay{0, 0, 0, 0.0f, 0.0f, 0.0f};
-    if (S.n_nodes == 0u) return hit;
-    const uint32_t infmask = (__builtin_isinf(inv.x) ? 1u : 0u) | (__builtin_isinf(inv.y) ? 2u : 0u) |
-                             (__builtin_isinf(inv.z) ? 4u : 0u);
+//
+// The traversal is written as a resumable query: Trav4 holds one query's
+// state (ray, best hit, current record, stack depth; the stack itself lives in
+// the lane's LaneStack), trav4_begin tests the root box, and a step is a
+// leaf's triangles or a record's four children, then the pops up to the next
+// entry to visit. trace_wide4 runs the steps to the end.
+template <bool WT>
+struct Trav4 {
+    F3 ro, rd, inv;
+    float lim;      // stop once the best hit is nearer than this (0: closest-hit query)
+    Hit hit;
+    uint32_t cur, sp;
+    uint32_t inf4;  // kill-bit mask of the axes whose 1/rd is infinite, for the four children
+    WtRay wr;       // WT only
+    float slack;    // WT only
+};
+
+// Starts a query; returns true when it is already answered (the root box is
+// missed, or the scene is empty). early_lim > 0 makes it an early-exit query.
+template <bool WT, bool COUNT>
+__device__ __forceinline__ bool trav4_begin(const TraceScene &S, Trav4<WT> &q, F3 ro, F3 rd, float early_lim,
+                                            TraceCount &cnt) {
+    q.ro = ro;
+    q.rd = rd;
+    q.inv = fs(1.0f) / rd;
+    q.lim = early_lim;
+    q.hit = Hit{kInf, 0u};
+    q.sp = 0u;
+    if (WT) q.wr = wt_ray(rd);
+    if (S.n_nodes == 0u) return true;
+    const uint32_t infmask = (__builtin_isinf(q.inv.x) ? 1u : 0u) | (__builtin_isinf(q.inv.y) ? 2u : 0u) |
+                             (__builtin_isinf(q.inv.z) ? 4u : 0u);
     // the kill bits reproduce the reference's culling of rays in a box's min
     // face plane; the watertight mode's conservative slab keeps those boxes
-    const uint32_t inf4 = WT ? 0u : infmask * 0x249u;  // replicated for the four children
+    q.inf4 = WT ? 0u : infmask * 0x249u;  // replicated for the four children
     float te, ta;
-    const float slack = WT ? wt_slack(ro, xyz(S.root_min), xyz(S.root_max)) : 0.0f;
-    if (WT) slab_wt(xyz(S.root_min), xyz(S.root_max), ro, inv, slack, te, ta);
-    else slab(xyz(S.root_min), xyz(S.root_max), ro, inv, te, ta);
-    if (COUNT) cnt.nodes++;
-    if (!(te <= ta && ta > 0.0f)) return hit;
-    uint32_t sp = 0u;
-    uint32_t cur = __float_as_uint(S.root_min.w);
-    for (;;) {
-        if (COUNT) count_step(cnt, cnt.trav_phase);
-        if (ref_is_leaf(cur)) {
-            const uint32_t first = cur & 0x07FFFFFFu;
-            const uint32_t count = ((cur >> 27) & 15u) + 1u;
-            // two triangles per step: both loads in flight before either test;
-            // the tests still run in slot order
-            for (uint32_t i = 0u; i < count; i += 2u) {
-                const uint32_t slot = first + i;
-                const bool two = i + 1u < count;
-                const float4 *q = S.slot_tris + 3u * slot;
-                const float4 a0 = q[0], a1 = q[1], a2 = q[2];
-                float4 b0 = a0, b1 = a1, b2 = a2;
-                if (two) {
-                    b0 = q[3];
-                    b1 = q[4];
-                    b2 = q[5];
-                }
-                if (COUNT) cnt.tris += two ? 2u : 1u;
-                const float t = WT ? wt_slot(wr, ro, xyz(a0), xyz(a1), xyz(a2)) : mt_slot(ro, rd, xyz(a0), xyz(a1), xyz(a2));
-                if (t < hit.t) {
-                    hit.t = t;
-                    hit.prim = __float_as_uint(a0.w);
-                }
-                if (two) {
-                    const float u =
-                        WT ? wt_slot(wr, ro, xyz(b0), xyz(b1), xyz(b2)) : mt_slot(ro, rd, xyz(b0), xyz(b1), xyz(b2));
-                    if (u < hit.t) {
-                        hit.t = u;
-                        hit.prim = __float_as_uint(b0.w);
-                    }
-                }
-            }
-            if (EARLY_EXIT && hit.t < t_max) return hit;
-        } else {
-            const float4 *w = S.w4nodes + 8u * cur;
-            const float4 mnx = w[0], mny = w[1], mnz = w[2], mxx = w[3], mxy = w[4], mxz = w[5];
-            const uint4 refs = *reinterpret_cast<const uint4 *>(&w[6]);
-            const uint32_t kill = inf4 != 0u ? __float_as_uint(w[7].x) & inf4 : 0u;
-            const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmny[4] = {mny.x, mny.y, mny.z, mny.w};
-            const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
-            const float bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
-            const uint32_t r[4] = {refs.x, refs.y, refs.z, refs.w};
-            float tes[4];
-            uint32_t pass = 0u;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                float e, a;
-                if (WT) slab_wt(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), ro, inv, slack, e, a);
-                else slab(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), ro, inv, e, a);
-                tes[k] = e;
-                if (r[k] != 0xFFFFFFFFu && e <= a && a > 0.0f && e < hit.t) pass |= 1u << k;
-            }
-            // kill bits only matter for rays with a zero direction component
-            // (rare): one branch per record instead of one per child
-            if (kill != 0u) {
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint32_t kb = (kill >> (3 * k)) & 7u;
-                    if (((kb & 1u) && ro.x == bmnx[k]) || ((kb & 2u) && ro.y == bmny[k]) ||
-                        ((kb & 4u) && ro.z == bmnz[k]))
-                        pass &= ~(1u << k);
-                }
-            }
-            if (COUNT) cnt.nodes += 4;
-            if (pass != 0u) {
-                // push passing children after the first, rightmost first
-#pragma unroll
-                for (int k = 3; k >= 1; k--) {
-                    if (((pass >> k) & 1u) && (pass & ((1u << k) - 1u))) {
-                        stk.put(sp, r[k]);
-                        stk.put(sp + 1u, __float_as_uint(tes[k]));
-                        sp += 2u;
-                    }
-                }
-                const int f = __builtin_ctz(pass);
-                cur = f == 0 ? r[0] : f == 1 ? r[1] : f == 2 ? r[2] : r[3];
-                continue;
-            }
-        }
-        bool found = false;
-        while (sp > 0u) {
-            sp -= 2u;
-            const float et = __uint_as_float(stk.get(sp + 1u));
-            if (et < hit.t) {
-                cur = stk.get(sp);
-                found = true;
-                break;
-            }
-        }
-        if (!found) break;
+    if (WT) {
+        q.slack = wt_slack(ro, xyz(S.root_min), xyz(S.root_max));
+        slab_wt(xyz(S.root_min), xyz(S.root_max), ro, q.inv, q.slack, te, ta);
+    } else {
+        slab(xyz(S.root_min), xyz(S.root_max), ro, q.inv, te, ta);
     }
-    return hit;
+    if (COUNT) cnt.nodes++;
+    q.cur = __float_as_uint(S.root_min.w);
+    return !(te <= ta && ta > 0.0f);
+}
+
+// The parts of a traversal step of a started query. trav4_leaf tests the
+// triangles of the leaf q.cur; it returns true when an early-exit query is answered.
+// trav4_node tests the four children of record q.cur and returns true when it
+// descended (q.cur set, the other passing children pushed). trav4_pop pops up
+// to the next entry still nearer than the best hit; it returns true when the
+// stack is empty (the query is answered).
+template <bool WT, bool COUNT>
+__device__ __forceinline__ bool trav4_leaf(const TraceScene &S, Trav4<WT> &q, TraceCount &cnt) {
+    const uint32_t first = q.cur & 0x07FFFFFFu;
+    const uint32_t count = ((q.cur >> 27) & 15u) + 1u;
+    // two triangles per step: both loads in flight before either test;
+    // the tests still run in slot order
+    for (uint32_t i = 0u; i < count; i += 2u) {
+        const uint32_t slot = first + i;
+        const bool two = i + 1u < count;
+        const float4 *p = S.slot_tris + 3u * slot;
+        // the second triangle's loads are issued with the first's (from the
+        // first slot again when there is no second: same lines, result unused)
+        const float4 *pb = p + (two ? 3u : 0u);
+        const float4 a0 = p[0], a1 = p[1], a2 = p[2];
+        const float4 b0 = pb[0], b1 = pb[1], b2 = pb[2];
+        if (COUNT) cnt.tris += two ? 2u : 1u;
+        const float t = WT ? wt_slot(q.wr, q.ro, xyz(a0), xyz(a1), xyz(a2))
+                           : mt_slot(q.ro, q.rd, xyz(a0), xyz(a1), xyz(a2));
+        if (t < q.hit.t) {
+            q.hit.t = t;
+            q.hit.prim = __float_as_uint(a0.w);
+        }
+        if (two) {
+            const float u = WT ? wt_slot(q.wr, q.ro, xyz(b0), xyz(b1), xyz(b2))
+                               : mt_slot(q.ro, q.rd, xyz(b0), xyz(b1), xyz(b2));
+            if (u < q.hit.t) {
+                q.hit.t = u;
+                q.hit.prim = __float_as_uint(b0.w);
+            }
+        }
+    }
+    return q.hit.t < q.lim;
+}
+
+template <bool WT, bool COUNT>
+__device__ __forceinline__ bool trav4_node(const TraceScene &S, Trav4<WT> &q, LaneStack &stk, TraceCount &cnt) {
+    const float4 *w = S.w4nodes + 8u * q.cur;
+    const float4 mnx = w[0], mny = w[1], mnz = w[2], mxx = w[3], mxy = w[4], mxz = w[5];
+    const uint4 refs = *reinterpret_cast<const uint4 *>(&w[6]);
+    const uint32_t kill = q.inf4 != 0u ? __float_as_uint(w[7].x) & q.inf4 : 0u;
+    const float bmnx[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bmny[4] = {mny.x, mny.y, mny.z, mny.w};
+    const float bmnz[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bmxx[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
+    const float bmxy[4] = {mxy.x, mxy.y, mxy.z, mxy.w}, bmxz[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+    const uint32_t r[4] = {refs.x, refs.y, refs.z, refs.w};
+    float tes[4];
+    uint32_t pass = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float e, a;
+        if (WT) slab_wt(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), q.ro, q.inv, q.slack, e, a);
+        else slab(f3(bmnx[k], bmny[k], bmnz[k]), f3(bmxx[k], bmxy[k], bmxz[k]), q.ro, q.inv, e, a);
+        tes[k] = e;
+        // every condition evaluated (no short-circuit branches)
+        const bool ok = (r[k] != 0xFFFFFFFFu) & (e <= a) & (a > 0.0f) & (e < q.hit.t);
+        pass |= (uint32_t)ok << k;
+    }
+    // kill bits only matter for rays with a zero direction component
+    // (rare): one branch per record instead of one per child
+    if (kill != 0u) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t kb = (kill >> (3 * k)) & 7u;
+            if (((kb & 1u) && q.ro.x == bmnx[k]) || ((kb & 2u) && q.ro.y == bmny[k]) ||
+                ((kb & 4u) && q.ro.z == bmnz[k]))
+                pass &= ~(1u << k);
+        }
+    }
+    if (COUNT) cnt.nodes += 4;
+    if (pass != 0u) {
+        // push passing children after the first, rightmost first; when no lane
+        // can reach the spill words (at most three entries are pushed) the
+        // pushes go straight to LDS
+        const bool shallow = q.sp + 6u <= (uint32_t)kLdsStack;
+        if (__ballot(!shallow) == 0ull) {
+            // every candidate is written at the top, which only advances
+            // past the pushed ones (the slots written stay below kLdsStack)
+#pragma unroll
+            for (int k = 3; k >= 1; k--) {
+                stk.put2_lds(q.sp, r[k], __float_as_uint(tes[k]));
+                q.sp += (((pass >> k) & 1u) && (pass & ((1u << k) - 1u))) ? 2u : 0u;
+            }
+        } else {
+#pragma unroll
+            for (int k = 3; k >= 1; k--) {
+                if (((pass >> k) & 1u) && (pass & ((1u << k) - 1u))) {
+                    stk.put2(q.sp, r[k], __float_as_uint(tes[k]));
+                    q.sp += 2u;
+                }
+            }
+        }
+        const int f = __builtin_ctz(pass);
+        q.cur = f == 0 ? r[0] : f == 1 ? r[1] : f == 2 ? r[2] : r[3];
+        return true;
+    }
+    return false;
+}
+
+template <bool WT>
+__device__ __forceinline__ bool trav4_pop(Trav4<WT> &q, LaneStack &stk) {
+    if (__ballot(q.sp > (uint32_t)kLdsStack) == 0ull) {
+        // no lane's entries reach the spill words
+        while (q.sp > 0u) {
+            q.sp -= 2u;
+            const uint2 e = stk.get2_lds(q.sp);
+            if (__uint_as_float(e.y) < q.hit.t) {
+                q.cur = e.x;
+                return false;
+            }
+        }
+        return true;
+    }
+    while (q.sp > 0u) {
+        q.sp -= 2u;
+        const uint2 e = stk.get2(q.sp);
+        if (__uint_as_float(e.y) < q.hit.t) {
+            q.cur = e.x;
+            return false;
+        }
+    }
+    return true;
+}
+
+// One step of a started query (a leaf or a record, then the pops); returns
+// true when the query is answered. Lanes of a wavefront at a leaf and lanes at
+// a record run their steps in the same loop iteration.
+template <bool WT, bool COUNT>
+__device__ __forceinline__ bool trav4_step(const TraceScene &S, Trav4<WT> &q, LaneStack &stk, TraceCount &cnt) {
+    if (COUNT) count_step(cnt, cnt.trav_phase);
+    if (ref_is_leaf(q.cur)) {
+        if (trav4_leaf<WT, COUNT>(S, q, cnt)) return true;
+    } else if (trav4_node<WT, COUNT>(S, q, stk, cnt)) {
+        return false;
+    }
+    return trav4_pop(q, stk);
+}
+
+template <bool EARLY_EXIT, bool COUNT, bool WT = false>
+__device__ Hit trace_wide4(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneStack &stk, TraceCount &cnt) {
+    Trav4<WT> q;
+    if (!trav4_begin<WT, COUNT>(S, q, ro, rd, EARLY_EXIT ? t_max : 0.0f, cnt))
+        while (!trav4_step<WT, COUNT>(S, q, stk, cnt)) {
+        }
+    return q.hit;
 }
 
 // Traversal modes: 0 = the reference's BVH2 nodes, 1 = child-box records in the

@@ -48,8 +48,12 @@ struct GatherArgs {
     uint32_t frame_stride;  // float4 elements between the outputs of consecutive frames
 };
 
+// The direct light's sample at a shading point: the unoccluded contribution,
+// the direction, and whether the shader casts a shadow ray (and its reach).
 struct LightSample {
     F3 e, wi;
+    float reach;
+    bool cast;
 };
 
 // The tracing context of one lane: closest-hit query and shadow query.
@@ -60,11 +64,14 @@ struct Tracer {
     TraceCount cnt;
     uint32_t queries;
     __device__ Hit closest(F3 ro, F3 rd) {
+        const uint64_t t0 = COUNT ? clock64() : 0u;
         if (COUNT) {
             queries++;
             cnt.trav_phase = kPhaseTraversal;
         }
-        return trace_mode<MODE, false, COUNT>(ts, ro, rd, kInf, stk, cnt);
+        const Hit h = trace_mode<MODE, false, COUNT>(ts, ro, rd, kInf, stk, cnt);
+        if (COUNT) count_time(cnt, kPhaseTraversal, t0);
+        return h;
     }
     // gather.wgsl:92-93: is the closest hit nearer than max_dist - 0.001? EARLY
     // stops the same traversal as soon as a hit that near is found (then `h`
@@ -74,33 +81,40 @@ struct Tracer {
             queries++;
             cnt.trav_phase = kPhaseShadow;
         }
+        const uint64_t t0 = COUNT ? clock64() : 0u;
         const float lim = max_dist - 0.001f;
         h = trace_mode<MODE, EARLY, COUNT>(ts, origin, dir, lim, stk, cnt);
+        if (COUNT) count_time(cnt, kPhaseShadow, t0);
         return !(h.t >= lim);
     }
 };
 
 // The counting build's totals of one lane: counters[0..3] queries, child boxes,
 // triangles, photon candidates; then the lane and wave steps of each phase.
-template <class T>
-__device__ void flush_counts(unsigned long long *counters, const T &tr, uint32_t photon_reads) {
-    atomicAdd(&counters[0], (unsigned long long)tr.queries);
-    atomicAdd(&counters[1], (unsigned long long)tr.cnt.nodes);
-    atomicAdd(&counters[2], (unsigned long long)tr.cnt.tris);
+__device__ void flush_counts(unsigned long long *counters, uint32_t queries, const TraceCount &cnt,
+                             uint32_t photon_reads) {
+    atomicAdd(&counters[0], (unsigned long long)queries);
+    atomicAdd(&counters[1], (unsigned long long)cnt.nodes);
+    atomicAdd(&counters[2], (unsigned long long)cnt.tris);
     atomicAdd(&counters[3], (unsigned long long)photon_reads);
     for (int k = 0; k < kCountPhases; k++) {
-        atomicAdd(&counters[4 + k], (unsigned long long)tr.cnt.step_lane[k]);
-        atomicAdd(&counters[4 + kCountPhases + k], (unsigned long long)tr.cnt.step_wave[k]);
+        atomicAdd(&counters[4 + k], (unsigned long long)cnt.step_lane[k]);
+        atomicAdd(&counters[4 + kCountPhases + k], (unsigned long long)cnt.step_wave[k]);
+        atomicAdd(&counters[4 + 2 * kCountPhases + k], (unsigned long long)cnt.wave_cycles[k]);
     }
+}
+template <class T>
+__device__ void flush_counts(unsigned long long *counters, const T &tr, uint32_t photon_reads) {
+    flush_counts(counters, tr.queries, tr.cnt, photon_reads);
 }
 
 // gather.wgsl:91-103 with alpha-cutout: the closest hit decides, and a
 // transparent texel of a cutout material lets the shadow ray through. Scenes
 // with cutout materials run with EARLY = false (the closest hit is needed).
-template <class T>
-__device__ bool occluded(const GatherArgs &A, T &tr, F3 origin, F3 dir, float max_dist) {
-    Hit h;
-    if (!tr.shadowed(origin, dir, max_dist, h)) return false;
+// `h` is the answer of the query (the closest hit, or with EARLY the first hit
+// found nearer than lim = max_dist - 0.001).
+__device__ bool shadow_blocked(const GatherArgs &A, const Hit &h, F3 origin, F3 dir, float lim) {
+    if (h.t >= lim) return false;
     const rt_triangle &bt = A.tris[h.prim];
     const rt_material &bm = A.mats[bt.material_id];
     if ((bm.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && bm.tex_albedo != RT_TEX_NONE) {
@@ -108,6 +122,12 @@ __device__ bool occluded(const GatherArgs &A, T &tr, F3 origin, F3 dir, float ma
         if (tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], uv, bm.tex_albedo).w < 0.5f) return false;
     }
     return true;
+}
+template <class T>
+__device__ bool occluded(const GatherArgs &A, T &tr, F3 origin, F3 dir, float max_dist) {
+    Hit h;
+    tr.shadowed(origin, dir, max_dist, h);
+    return shadow_blocked(A, h, origin, dir, max_dist - 0.001f);
 }
 
 // gather.wgsl:119-141
@@ -137,13 +157,12 @@ __device__ __forceinline__ F3 sky_color(F3 dir, F3 zenith, F3 horizon, float int
     return mix(horizon, zenith, wgsl_pow(t, 0.5f)) * intensity;
 }
 
-// gather.wgsl:152-206 (out-of-range light index clamped to n-1). Each light
-// kind sets the direction, the shadow ray's reach and the unoccluded
-// contribution; one shadow query after the branches serves every kind, so the
-// lanes of a wavefront that picked different kinds traverse together. The
-// arithmetic of each kind is the shader's, in its order.
-template <class T>
-__device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rng &rng) {
+// gather.wgsl:152-206 up to the shadow query (out-of-range light index
+// clamped to n-1). Each light kind sets the direction, the shadow ray's reach
+// and the unoccluded contribution; one shadow query after the branches serves
+// every kind, so the lanes of a wavefront that picked different kinds
+// traverse together. The arithmetic of each kind is the shader's, in its order.
+__device__ LightSample sample_light(const GatherArgs &A, F3 hp, F3 sn, Rng &rng) {
     const uint32_t n = A.n_lights;
     uint32_t li = wgsl_u32(rng_f32(rng) * (float)n);
     li = li < n ? li : n - 1u;
@@ -206,8 +225,7 @@ __device__ LightSample direct_light(const GatherArgs &A, T &tr, F3 hp, F3 sn, Rn
             }
         }
     }
-    if (cast && occluded(A, tr, hp + sn * 0.01f, ld, reach)) e = fs(0.0f);
-    return LightSample{e, ld};
+    return LightSample{e, ld, reach, cast};
 }
 
 // gather.wgsl:210-227
@@ -330,23 +348,41 @@ struct FirstHit {
 // photon candidates whose gather records are loaded together in the radius gather
 constexpr uint32_t kPhotonBatch = RT_PHOTON_BATCH;
 
-// One iteration of the bounce loop of gather.wgsl:298-537 at depth b: the
-// closest hit of (ro, rd) and its shading. Returns true when the path goes on
-// (ro, rd and thr then hold the next ray), false when it ends. When `smooth` is
-// given it is set to whether the next ray was scattered by a specular lobe or
-// a dielectric (rather than diffusely, or passed through a cutout texel).
-template <int MODE, bool EARLY, bool COUNT, class AccType>
-__device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, COUNT> &tr, uint32_t b, F3 &ro,
-                       F3 &rd, F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
-                       uint32_t py, bool *smooth = nullptr) {
+// One iteration of the bounce loop of gather.wgsl:298-537 at depth b, in two
+// parts around the direct light's shadow query.
+//
+// bounce_hit: the shading of the closest hit `hit` of the ray (ro, rd) up to
+// the light sample. kPathEnds: the path ends (sky, debug view); kPathGoesOn:
+// ro, rd and thr hold the next ray (dielectric, or a cutout texel passed);
+// kSurfaceLit: `sf` holds the surface and its light sample, and the shadow
+// query of sf.ls (when sf.ls.cast) must be answered before bounce_light.
+//
+// bounce_light: the rest of the bounce once the shadow query is answered:
+// direct light, photon radius gather, next direction and Russian roulette.
+// Returns true when the path goes on (ro, rd and thr then hold the next ray).
+//
+// When `smooth` is given it is set to whether the next ray was scattered by a
+// specular lobe or a dielectric (rather than diffusely, or passed through a
+// cutout texel).
+enum BounceStep : uint32_t { kPathEnds = 0u, kPathGoesOn = 1u, kSurfaceLit = 2u };
+struct SurfaceState {
+    F3 hp, shn;                 // hit point, shading normal
+    F3 f0, dscale, albedo;
+    float a, spec_prob;
+    bool do_spec;
+    LightSample ls;
+};
+
+template <class AccType>
+__device__ uint32_t bounce_hit(const GatherArgs &A, const Env &env, const Hit &hit, uint32_t b, F3 &ro, F3 &rd,
+                               F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t px, uint32_t py, bool *smooth,
+                               SurfaceState &sf) {
     if (smooth) *smooth = false;
-    if (COUNT) count_step(tr.cnt, kPhaseBounce);
     const rt_image_params &ip = A.ip;
     const uint32_t debug_view = ip._pad[0];
-    const Hit hit = tr.closest(ro, rd);
     if (hit.t >= kInf) {
         if (b > 0u) acc.add(thr * sky_color(rd, env.color, env.horizon, env.intensity));
-        return false;
+        return kPathEnds;
     }
     const rt_triangle &trg = A.tris[hit.prim];
     const rt_material &mat = A.mats[trg.material_id];
@@ -357,7 +393,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
     if ((mat.flags & RT_MAT_FLAG_ALPHA_CUTOUT) != 0u && mat.tex_albedo != RT_TEX_NONE) {
         if (tex_sample(A.tex[RT_TEX_ROLE_ALBEDO], bb.uv, mat.tex_albedo).w < 0.5f) {
             ro = hp + rd * 0.002f;
-            return true;
+            return kPathGoesOn;
         }
     }
     const F3 gn = get_triangle_normal(trg);
@@ -405,7 +441,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
             case 21u: acc.col = hp; break;
             default: stop = false; break;
         }
-        if (stop) return false;
+        if (stop) return kPathEnds;
     }
 
     if (mat.kind == 2u) {
@@ -422,11 +458,10 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
             ro = hp - shn * 0.001f;
             thr = thr * albedo;
         }
-        return true;
+        return kPathGoesOn;
     }
 
     const F3 f0 = mix(fs(0.04f), albedo, fs(metal));
-    const F3 wo = -rd;
     const float a = wmax(rough * rough, 1e-3f);
     const F3 fr = fresnel_schlick(__builtin_fabsf(dot(rd, shn)), f0);
     const F3 diffuse_weight = (1.0f - metal) * albedo;
@@ -436,8 +471,28 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
     const float spec_prob = wclamp(sel, 0.001f, 0.999f);
     const bool do_spec = rng_f32(rng) < spec_prob;
     const F3 dscale = diffuse_weight / (1.0f - spec_prob);
+    sf.hp = hp;
+    sf.shn = shn;
+    sf.f0 = f0;
+    sf.dscale = dscale;
+    sf.albedo = albedo;
+    sf.a = a;
+    sf.spec_prob = spec_prob;
+    sf.do_spec = do_spec;
+    sf.ls = sample_light(A, hp, shn, rng);
+    return kSurfaceLit;
+}
 
-    const LightSample ls = direct_light(A, tr, hp, shn, rng);
+template <bool COUNT, class AccType>
+__device__ bool bounce_light(const GatherArgs &A, const Env &env, const SurfaceState &sf, bool occl, F3 &ro, F3 &rd,
+                             F3 &thr, Rng &rng, AccType &acc, TraceCount &cnt, uint32_t &photon_reads, bool *smooth) {
+    const rt_image_params &ip = A.ip;
+    const F3 hp = sf.hp, shn = sf.shn, f0 = sf.f0, dscale = sf.dscale, albedo = sf.albedo;
+    const float a = sf.a, spec_prob = sf.spec_prob;
+    const bool do_spec = sf.do_spec;
+    const F3 wo = -rd;
+    LightSample ls = sf.ls;
+    if (occl) ls.e = fs(0.0f);
     if (!do_spec) {
         if (env.intensity > 0.0f)
             acc.add(thr * dscale * sky_color(shn, env.color, env.horizon, env.intensity));
@@ -447,6 +502,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
         acc.add(thr * spec_ggx(wo, shn, ls.wi, a, f0) * ls.e / spec_prob);
     }
 
+    const uint64_t t_ph = COUNT ? clock64() : 0u;
     if (ip.use_photons == 1u && !do_spec) {
         // gather.wgsl:473-514
         const rt_grid_params &gp = A.gp;
@@ -487,7 +543,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
                     if (k >= n_ranges) break;
                     const uint2 meta = metas[k];
                     for (uint32_t i = 0u; i < meta.y && !done; i += kPhotonBatch) {
-                        if (COUNT) count_step(tr.cnt, kPhasePhotons);
+                        if (COUNT) count_step(cnt, kPhasePhotons);
                         // one round trip per batch: position, power and incident
                         // direction of each candidate come in one 48-byte record
                         const uint32_t nb = min(meta.y - i, kPhotonBatch);
@@ -537,6 +593,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
             acc.col = f3(1.0f - density, density, 0.0f) * 2.0f;
         }
     }
+    if (COUNT && ip.use_photons == 1u) count_time(cnt, kPhasePhotons, t_ph);
 
     if (do_spec) {
         if (smooth) *smooth = true;
@@ -556,6 +613,26 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
     if (p <= 0.0f || rng_f32(rng) > p) return false;
     thr = thr / p;
     return true;
+}
+
+// One whole bounce for callers that trace in place: closest hit, shading, the
+// shadow query, the rest of the shading.
+template <int MODE, bool EARLY, bool COUNT, class AccType>
+__device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, COUNT> &tr, uint32_t b, F3 &ro,
+                       F3 &rd, F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
+                       uint32_t py, bool *smooth = nullptr) {
+    const uint64_t t0 = COUNT ? clock64() : 0u;
+    if (COUNT) count_step(tr.cnt, kPhaseBounce);
+    const Hit hit = tr.closest(ro, rd);
+    SurfaceState sf;
+    const uint32_t st = bounce_hit(A, env, hit, b, ro, rd, thr, rng, acc, fh, px, py, smooth, sf);
+    bool cont = st == kPathGoesOn;
+    if (st == kSurfaceLit) {
+        const bool occl = sf.ls.cast && occluded(A, tr, sf.hp + sf.shn * 0.01f, sf.ls.wi, sf.ls.reach);
+        cont = bounce_light<COUNT>(A, env, sf, occl, ro, rd, thr, rng, acc, tr.cnt, photon_reads, smooth);
+    }
+    if (COUNT) count_time(tr.cnt, kPhaseBounce, t0);
+    return cont;
 }
 
 // gather.wgsl:262-263: the pixel's seed; gather.wgsl:291-296: the jittered camera ray of one sample.
@@ -698,10 +775,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     const uint32_t py = (tile / A.tiles_x) * 8u + (threadIdx.x >> 3);
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
+    stk.init(stack_rows, spill_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
     uint32_t photon_reads = 0u;
     const ViewParams V = frame_view(A);
     if (px < A.ip.width && py < A.ip.height)
@@ -728,36 +803,24 @@ constexpr uint32_t kBands = 8;
 #define RT_PERSISTENT_WAVES 4  // wavefronts per SIMD the register budget is cut for
 #endif
 
-template <int MODE, bool EARLY, bool COUNT, int NS = 8>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTENT_WAVES))) void gather_persistent_kernel(
-    GatherArgs A) {
-    __shared__ uint32_t stack_rows[kLdsStack * 64];
-    uint32_t spill_words[kSpillAlloc];
-    LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
-    uint32_t photon_reads = 0u;
-    const ViewParams V = frame_view(A);
-    const uint32_t spp = max(1u, V.spp);
-    const Env env = find_env(A);
-    const uint32_t lane = __lane_id();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const uint32_t n_slots = A.n_tiles * 64u;
-    const uint32_t band_len = (A.n_tiles + kBands - 1u) / kBands * 64u;
-    const uint32_t n_frames = A.n_frames;
-    uint32_t band = blockIdx.x % kBands, bands_left = kBands;
-    bool live = false;
-    uint32_t px = 0u, py = 0u, s = 0u, b = 0u, fi = 0u;
-    Rng rng{0u};
-    AccT<NS> acc;
-    FirstHit fh;
-    acc_clear(acc, fh);
-    F3 ro = fs(0.0f), rd = fs(0.0f), thr = fs(0.0f);
-    for (;;) {
-        // lanes without a pixel take the next slots of the current band; a
-        // slot past the image (edge tiles) is skipped by taking another
+// The pixel hand-out of the persistent forms: a wavefront's lanes without a
+// pixel take the next slots of its current band; a slot past the image (edge
+// tiles) is skipped by taking another. A band's counter runs over the band's
+// slots of frame 0, then of frame 1, ... of the frames of the launch.
+struct PixelFeed {
+    uint32_t band, bands_left, n_slots, band_len;
+    __device__ explicit PixelFeed(const GatherArgs &A)
+        : band(blockIdx.x % kBands),
+          bands_left(kBands),
+          n_slots(A.n_tiles * 64u),
+          band_len((A.n_tiles + kBands - 1u) / kBands * 64u) {}
+    // Lanes with live == false take a pixel (px, py of frame fi) when there
+    // is one; returns true on the lanes that took one. A wavefront takes
+    // pixels once A.refill_min of its lanes have none (or none has one).
+    __device__ bool take(const GatherArgs &A, bool &live, uint32_t &px, uint32_t &py, uint32_t &fi) {
+        const uint32_t lane = __lane_id();
+        const unsigned long long below = (1ull << lane) - 1ull;
+        bool took = false;
         while (bands_left > 0u) {
             const unsigned long long need = __ballot(!live);
             if (need == 0ull) break;
@@ -767,11 +830,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
             uint32_t base = 0u;
             if (lane == (uint32_t)leader) base = atomicAdd(&A.regen[band], n_need);
             base = __builtin_amdgcn_readfirstlane(__shfl(base, leader, 64));
-            // a band's counter runs over the band's slots of frame 0, then
-            // of frame 1, ... of the frames of this launch
             const uint32_t band0 = band * band_len;
             const uint32_t len = band0 < n_slots ? min(band_len, n_slots - band0) : 0u;
-            const uint32_t total = len * n_frames;
+            const uint32_t total = len * A.n_frames;
             if (!live) {
                 const uint32_t k = base + (uint32_t)__popcll(need & below);
                 if (k < total) {
@@ -781,13 +842,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
                     py = (tile / A.tiles_x) * 8u + (l >> 3);
                     if (px < A.ip.width && py < A.ip.height) {
                         live = true;
+                        took = true;
                         fi = f;
-                        rng = pixel_rng(V, px, py, V.frame + f);
-                        acc_clear(acc, fh);
-                        s = 0u;
-                        b = 0u;
-                        camera_ray(V, px, py, rng, ro, rd);
-                        thr = fs(1.0f);
                     }
                 }
             }
@@ -795,6 +851,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
                 band = (band + 1u) % kBands;
                 bands_left--;
             }
+        }
+        return took;
+    }
+};
+
+template <int MODE, bool EARLY, bool COUNT, int NS = 8>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTENT_WAVES))) void gather_persistent_kernel(
+    GatherArgs A) {
+    __shared__ uint32_t stack_rows[kLdsStack * 64];
+    uint32_t spill_words[kSpillAlloc];
+    LaneStack stk;
+    stk.init(stack_rows, spill_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
+    uint32_t photon_reads = 0u;
+    const ViewParams V = frame_view(A);
+    const uint32_t spp = max(1u, V.spp);
+    const Env env = find_env(A);
+    PixelFeed feed(A);
+    bool live = false;
+    uint32_t px = 0u, py = 0u, s = 0u, b = 0u, fi = 0u;
+    Rng rng{0u};
+    AccT<NS> acc;
+    FirstHit fh;
+    acc_clear(acc, fh);
+    F3 ro = fs(0.0f), rd = fs(0.0f), thr = fs(0.0f);
+    for (;;) {
+        const uint64_t t_it = COUNT ? clock64() : 0u;
+        if (feed.take(A, live, px, py, fi)) {
+            rng = pixel_rng(V, px, py, V.frame + fi);
+            acc_clear(acc, fh);
+            s = 0u;
+            b = 0u;
+            camera_ray(V, px, py, rng, ro, rd);
+            thr = fs(1.0f);
         }
         if (__ballot(live) == 0ull) break;
         if (live) {
@@ -814,6 +904,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
                 }
             }
         }
+        if (COUNT) count_time(tr.cnt, kPhaseLoop, t_it);
     }
     if (COUNT) {
         flush_counts(A.counters, tr, photon_reads);
@@ -918,10 +1009,8 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     const uint32_t py = (tile / A.tiles_x) * 8u + (threadIdx.x >> 3);
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
+    stk.init(stack_rows, spill_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
     uint32_t photon_reads = 0u;
     const ViewParams V = frame_view(A);
     const uint32_t spp = max(1u, V.spp);
@@ -970,10 +1059,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{0u, 0u}, 0u};
+    stk.init(stack_rows, spill_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
     uint32_t photon_reads = 0u;
     const uint32_t n_front = __builtin_amdgcn_readfirstlane(A.path_count[1]);
     const uint32_t count = n_front + __builtin_amdgcn_readfirstlane(A.path_count[2]);
@@ -1039,10 +1126,8 @@ __global__ __launch_bounds__(64) void gather_batch_kernel(BatchArgs B) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{0u, 0u}, 0u};
+    stk.init(stack_rows, spill_words);
+    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{}, 0u};
     uint32_t photon_reads = 0u;
     const uint32_t rounds = (B.tile_w * B.tile_h) / 64u;
     for (;;) {
@@ -1078,10 +1163,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{0u, 0u}, 0u};
+    stk.init(stack_rows, spill_words);
+    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{}, 0u};
     uint32_t photon_reads = 0u;
     const Env env = find_env(B.g);
     const uint32_t lane = __lane_id();
@@ -1159,10 +1242,8 @@ __global__ __launch_bounds__(64) void trace_rays_kernel(TraceScene ts, const rt_
     if (i >= n) return;
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    TraceCount cnt{0u, 0u};
+    stk.init(stack_rows, spill_words);
+    TraceCount cnt{};
     const rt_ray r = rays[i];
     const F3 ro = ld3(r.origin), rd = ld3(r.dir);
     const Hit h = trace_mode<MODE, false, false>(ts, ro, rd, kInf, stk, cnt);
@@ -1180,10 +1261,8 @@ __global__ __launch_bounds__(64) void albedo_kernel(TraceScene ts, const rt_tria
     if (i >= w * h) return;
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
-    TraceCount cnt{0u, 0u};
+    stk.init(stack_rows, spill_words);
+    TraceCount cnt{};
     const uint32_t px = i % w, py = i / w;
     const float ux = (float)px * cam.inv_half_res[0] - 1.0f;
     const float uy = 1.0f - (float)py * cam.inv_half_res[1];
@@ -1275,17 +1354,19 @@ static rtd::GatherArgs gather_args(const GatherLaunch &g) {
 // current device (one 64-lane wavefront each).
 // The count is computed once per instance (a function-local static:
 // initialised once even when renderers on several threads launch first).
+template <class K>
+static uint32_t resident_blocks(K kernel) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0u;
+    return (uint32_t)std::max(per_cu, 1) * (uint32_t)std::max(cus, 1);
+}
+
 template <int M, bool E, bool C, int NS>
 static uint32_t persistent_blocks() {
-    static const uint32_t blocks = [] {
-        int per_cu = 0, dev = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtd::gather_persistent_kernel<M, E, C, NS>, 64, 0) !=
-                hipSuccess ||
-            hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            return 0u;
-        return (uint32_t)std::max(per_cu, 1) * (uint32_t)std::max(cus, 1);
-    }();
+    static const uint32_t blocks = resident_blocks(rtd::gather_persistent_kernel<M, E, C, NS>);
     return blocks;
 }
 

@@ -68,9 +68,7 @@ __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
     if (photon_index >= A.n_photons) return;
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
-    stk.spill = (priv_u32 *)spill_words;
-    stk.lds = (lds_u32 *)&stack_rows[threadIdx.x];
-    stk.stride = 64u;
+    stk.init(stack_rows, spill_words);
     TraceCount cnt{0u, 0u};
     const uint32_t num_lights = A.n_lights;
     Rng rng = rng_init(photon_index * 1103515245u + 12345u);

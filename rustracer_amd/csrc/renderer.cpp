@@ -36,7 +36,7 @@ T *dev_alloc(size_t count, const char *what) {
 constexpr uint32_t kTraversalStackWords = rtk::kTraversalStackWords;
 // counting launch: queries, child boxes, triangles, photon candidates, then
 // lane steps and wave steps per phase
-constexpr int kCounterWords = 4 + 2 * RT_COUNT_PHASES;
+constexpr int kCounterWords = 4 + 3 * RT_COUNT_PHASES;
 
 struct PendingTiming {
     int pass;
@@ -1519,6 +1519,7 @@ int rt_count_gather_traffic(rt_renderer *r, rt_trace_counters *out) {
         for (int k = 0; k < RT_COUNT_PHASES; k++) {
             out->lane_steps[k] = c[4 + k];
             out->wave_steps[k] = c[4 + RT_COUNT_PHASES + k];
+            out->wave_cycles[k] = c[4 + 2 * RT_COUNT_PHASES + k];
         }
     });
 }

@@ -346,11 +346,11 @@ def test_camera_pass_forms_same_work(scene_name, spp, photons, other):
     a, b = out[0], out[other]
     for name, x, y in zip(("frame", "moments", "gbuffer", "accumulation", "rgba8"), a[:5], b[:5]):
         _check_equal(name, x, y)
-    # per-lane work is the same; wave steps and active lanes per wave step depend
-    # on which rays share a wave, which is what the forms change, and only the
-    # persistent kernel has a camera loop to count
+    # per-lane work is the same; wave steps, active lanes per wave step and wave
+    # cycles depend on which rays share a wave, which is what the forms change,
+    # and only the persistent kernel has a camera loop to count
     def per_lane(c):
-        out = {k: v for k, v in c.items() if k not in ("wave_steps", "active_lanes", "lane_steps")}
+        out = {k: v for k, v in c.items() if k not in ("wave_steps", "active_lanes", "lane_steps", "wave_cycles")}
         out["lane_steps"] = {k: v for k, v in c["lane_steps"].items() if k != "camera_loop"}
         return out
     assert per_lane(a[5]) == per_lane(b[5]), (a[5], b[5])
