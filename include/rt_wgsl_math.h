@@ -10,9 +10,13 @@
  * (absolute 2^-11 for sin/cos on [-pi, pi]; exp/exp2 3+2|x| ULP; pow inherited
  * from exp2(y*log2(x))), so every GPU driver answers differently in the last
  * bits. This backend picks one answer and writes it down here, built only from
- * IEEE-754 +, -, *, /, sqrt, rint/floor and bit operations, so that the HIP
+ * IEEE-754 +, -, *, /, fma, sqrt, rint/floor and bit operations, so that the HIP
  * kernels (compiled with -ffp-contract=off) and a CPU build of the same
  * functions produce identical bits.
+ *
+ * Polynomials are evaluated by Horner's rule with fused multiply-adds
+ * (fmaf: one rounding per step on the host, where C's fmaf is the IEEE
+ * operation, and on gfx950, v_fma_f32), the rest with single operations.
  *
  * Methods (own implementations of textbook algorithms):
  *   sin/cos : Cody-Waite reduction by pi/2 with a 3-part split of pi/2
@@ -38,6 +42,7 @@
 #endif
 
 RT_WM_FN uint32_t wm_f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+RT_WM_FN float wm_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 RT_WM_FN float wm_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 /* min/max with one NaN rule: a NaN operand yields the other operand; for equal
@@ -65,19 +70,19 @@ RT_WM_FN float wgsl_inverse_sqrt(float x) { return 1.0f / __builtin_sqrtf(x); }
 
 RT_WM_FN float wm_sin_poly(float r) {
     const float r2 = r * r;
-    float p = 2.75573192e-06f;           /*  1/9! */
-    p = p * r2 + -1.98412698e-04f;       /* -1/7! */
-    p = p * r2 + 8.33333377e-03f;        /*  1/5! */
-    p = p * r2 + -1.66666672e-01f;       /* -1/3! */
-    return r + (r * r2) * p;
+    float p = 2.75573192e-06f;                   /*  1/9! */
+    p = wm_fma(p, r2, -1.98412698e-04f);         /* -1/7! */
+    p = wm_fma(p, r2, 8.33333377e-03f);          /*  1/5! */
+    p = wm_fma(p, r2, -1.66666672e-01f);         /* -1/3! */
+    return wm_fma(r * r2, p, r);
 }
 RT_WM_FN float wm_cos_poly(float r) {
     const float r2 = r * r;
-    float p = -2.75573188e-07f;          /* -1/10! */
-    p = p * r2 + 2.48015876e-05f;        /*  1/8!  */
-    p = p * r2 + -1.38888892e-03f;       /* -1/6!  */
-    p = p * r2 + 4.16666679e-02f;        /*  1/4!  */
-    return (1.0f - 0.5f * r2) + (r2 * r2) * p;
+    float p = -2.75573188e-07f;                  /* -1/10! */
+    p = wm_fma(p, r2, 2.48015876e-05f);          /*  1/8!  */
+    p = wm_fma(p, r2, -1.38888892e-03f);         /* -1/6!  */
+    p = wm_fma(p, r2, 4.16666679e-02f);          /*  1/4!  */
+    return wm_fma(r2 * r2, p, 1.0f - 0.5f * r2);
 }
 
 /* Reduce x to r in [-pi/4, pi/4] and the quadrant k mod 4. Valid for |x| <= 8192. */
@@ -111,14 +116,14 @@ RT_WM_FN float wgsl_exp2(float x) {
     if (x < -151.0f) return 0.0f;
     const float n = __builtin_rintf(x);
     const float f = x - n; /* exact, |f| <= 1/2 */
-    float p = 1.52527336e-05f;            /* ln2^7/7! */
-    p = p * f + 1.54035297e-04f;          /* ln2^6/6! */
-    p = p * f + 1.33335579e-03f;          /* ln2^5/5! */
-    p = p * f + 9.61812865e-03f;          /* ln2^4/4! */
-    p = p * f + 5.55041097e-02f;          /* ln2^3/3! */
-    p = p * f + 2.40226507e-01f;          /* ln2^2/2! */
-    p = p * f + 6.93147182e-01f;          /* ln2      */
-    p = p * f + 1.0f;
+    float p = 1.52527336e-05f;                   /* ln2^7/7! */
+    p = wm_fma(p, f, 1.54035297e-04f);           /* ln2^6/6! */
+    p = wm_fma(p, f, 1.33335579e-03f);           /* ln2^5/5! */
+    p = wm_fma(p, f, 9.61812865e-03f);           /* ln2^4/4! */
+    p = wm_fma(p, f, 5.55041097e-02f);           /* ln2^3/3! */
+    p = wm_fma(p, f, 2.40226507e-01f);           /* ln2^2/2! */
+    p = wm_fma(p, f, 6.93147182e-01f);           /* ln2      */
+    p = wm_fma(p, f, 1.0f);
     const int32_t e = (int32_t)n;
     if (e >= -126) {
         if (e > 127) return (p * 1.70141183e+38f) * 2.0f; /* 2^127 * 2 */
@@ -141,13 +146,13 @@ RT_WM_FN float wgsl_log2(float x) {
     if (m > 1.41421354f) { m = m * 0.5f; e = e + 1; }
     const float s = (m - 1.0f) / (m + 1.0f);
     const float s2 = s * s;
-    float p = 9.09090936e-02f;   /* 1/11 */
-    p = p * s2 + 1.11111112e-01f; /* 1/9 */
-    p = p * s2 + 1.42857149e-01f; /* 1/7 */
-    p = p * s2 + 2.00000003e-01f; /* 1/5 */
-    p = p * s2 + 3.33333343e-01f; /* 1/3 */
-    const float ln_m = (2.0f * s) + (2.0f * s) * (s2 * p);
-    return (float)e + ln_m * 1.44269502f; /* log2(e) */
+    float p = 9.09090936e-02f;             /* 1/11 */
+    p = wm_fma(p, s2, 1.11111112e-01f);    /* 1/9 */
+    p = wm_fma(p, s2, 1.42857149e-01f);    /* 1/7 */
+    p = wm_fma(p, s2, 2.00000003e-01f);    /* 1/5 */
+    p = wm_fma(p, s2, 3.33333343e-01f);    /* 1/3 */
+    const float ln_m = wm_fma(2.0f * s, s2 * p, 2.0f * s);
+    return wm_fma(ln_m, 1.44269502f, (float)e); /* log2(e) */
 }
 
 RT_WM_FN float wgsl_exp(float x) { return wgsl_exp2(x * 1.44269502f); }

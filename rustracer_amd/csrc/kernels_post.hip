@@ -286,10 +286,21 @@ __global__ __launch_bounds__(256) void relmse_final_kernel(const double *__restr
 
 // Plain streaming copy (16 B per lane per iteration): the measured-HBM
 // denominator of the rooflines and the calibration kernel of the traffic counters.
+// Grid-stride copy with four independent 16-byte loads in flight per lane
+// before their stores (one load per lane per iteration leaves HBM idle
+// between a wavefront's round trips).
 __global__ __launch_bounds__(256) void hbm_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
                                                        size_t n) {
     const size_t stride = (size_t)gridDim.x * 256u;
-    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    for (; i + 3u * stride < n; i += 4u * stride) {
+        const float4 a = src[i], b = src[i + stride], c = src[i + 2u * stride], d = src[i + 3u * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2u * stride] = c;
+        dst[i + 3u * stride] = d;
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
 }
 
 }  // namespace rtd
@@ -320,7 +331,8 @@ hipError_t launch_relmse(const void *img, const double *sum, uint64_t n_frames, 
 hipError_t launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 16;
     if (n == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256u * 8u * 8u);
+    // 8 workgroups of 256 lanes per CU, each lane copying 4 x 16 B per iteration
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 1023) / 1024, 256u * 8u);
     hipLaunchKernelGGL(rtd::hbm_copy_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4 *>(src),
                        reinterpret_cast<float4 *>(dst), n);
     return hipGetLastError();
