@@ -192,34 +192,79 @@ struct Hit {
     uint32_t prim;  // primitive index (closest hit)
 };
 
-// Counters of the counting build. step_lane[p] / step_wave[p]: loop steps of
-// phase p summed over lanes and over wavefronts; their ratio is the mean number
-// of active lanes. Phases: traversal steps of closest-hit queries, one bounce
-// body, one batch of photon candidates, traversal steps of shadow queries, one
-// iteration of the persistent camera pass's loop (its live lanes).
+// Counters of the counting build. Phases: traversal steps of closest-hit
+// queries, one bounce body, one batch of photon candidates, traversal steps of
+// shadow queries, one iteration of the persistent camera pass's loop (its live
+// lanes). Per phase: lane steps and wave steps (their ratio is the mean number
+// of active lanes) and the shader-clock cycles wavefronts spent in the
+// phase's code regions (each execution of a region by a wavefront counted
+// once; for the loop phase, whole iterations).
 enum CountPhase { kPhaseTraversal = 0, kPhaseBounce, kPhasePhotons, kPhaseShadow, kPhaseLoop, kCountPhases };
 static_assert(kCountPhases == RT_COUNT_PHASES, "counter layout of rt_trace_counters");
-// wave_cycles[p]: shader-clock cycles wavefronts spent in regions of phase p
-// (each execution of a region by a wavefront counted once, by its first active
-// lane); for the loop phase, whole iterations of the camera pass's loop.
-struct TraceCount {
-    uint32_t nodes, tris;
-    uint32_t step_lane[kCountPhases], step_wave[kCountPhases];
-    uint32_t trav_phase;  // the phase the traversal loop counts into
-    uint64_t wave_cycles[kCountPhases];
+// Counter words, in the order of rt_trace_counters.
+enum CountWord {
+    kCntQueries = 0,
+    kCntNodes,
+    kCntTris,
+    kCntPhotons,
+    kCntLaneSteps,
+    kCntWaveSteps = kCntLaneSteps + kCountPhases,
+    kCntCycles = kCntWaveSteps + kCountPhases,
+    kCntWords = kCntCycles + kCountPhases
 };
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
 __device__ __forceinline__ bool first_active_lane() {
     const unsigned long long active = __ballot(1);
     return __lane_id() == (uint32_t)(__ffsll((long long)active) - 1);
 }
-__device__ __forceinline__ void count_step(TraceCount &c, int phase) {
-    c.step_lane[phase]++;
-    if (first_active_lane()) c.step_wave[phase]++;
+// The counters live in LDS, one set per wavefront (the counting kernels run
+// one-wavefront workgroups): each update adds the wavefront's total, with an
+// LDS atomic issued by its first active lane (an atomic, so the compiler keeps
+// no per-lane copy of a counter in a register across a loop), so counting
+// keeps nothing in registers.
+__device__ __forceinline__ void lds_add(lds_u64 *p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-// Time of a region of phase `phase` that started at t0 = clock64().
-__device__ __forceinline__ void count_time(TraceCount &c, int phase, uint64_t t0) {
-    const uint64_t dt = clock64() - t0;
-    if (first_active_lane()) c.wave_cycles[phase] += dt;
+struct TraceCount {
+    lds_u64 *w;           // kCntWords words (null outside the counting build)
+    uint32_t trav_phase;  // the phase the traversal loop counts into
+    // every active lane adds k to `word`
+    __device__ __forceinline__ void add(int word, uint32_t k) {
+        const unsigned long long active = __ballot(1);
+        if (__lane_id() == (uint32_t)(__ffsll((long long)active) - 1))
+            lds_add(&w[word], (unsigned long long)k * (unsigned long long)__popcll(active));
+    }
+};
+__device__ __forceinline__ void count_step(TraceCount &c, int phase) {
+    const unsigned long long active = __ballot(1);
+    if (__lane_id() == (uint32_t)(__ffsll((long long)active) - 1)) {
+        lds_add(&c.w[kCntLaneSteps + phase], (unsigned long long)__popcll(active));
+        lds_add(&c.w[kCntWaveSteps + phase], 1ull);
+    }
+}
+// A traversal loop step, counted into the phase of the query being traced.
+__device__ __forceinline__ void count_traversal_step(TraceCount &c) {
+    if (c.trav_phase == kPhaseShadow) count_step(c, kPhaseShadow);
+    else count_step(c, kPhaseTraversal);
+}
+// The shader clock's low 32 bits (differences are taken modulo 2^32, exact
+// for regions shorter than ~1.7 s).
+__device__ __forceinline__ uint32_t phase_clock() { return (uint32_t)clock64(); }
+// Time of a region of phase `phase` that started at t0 = phase_clock().
+__device__ __forceinline__ void count_time(TraceCount &c, int phase, uint32_t t0) {
+    const uint32_t dt = phase_clock() - t0;
+    if (first_active_lane()) lds_add(&c.w[kCntCycles + phase], dt);
+}
+// Counting kernels: the LDS words zeroed at the start, added to the launch's
+// counters at the end (by the first active lane; lanes may have left).
+__device__ __forceinline__ void count_begin(unsigned long long *words) {
+    if (first_active_lane())
+        for (int k = 0; k < kCntWords; k++) ((lds_u64 *)words)[k] = 0ull;
+}
+__device__ __forceinline__ void count_flush(unsigned long long *counters, unsigned long long *words) {
+    if (first_active_lane())
+        for (int k = 0; k < kCntWords; k++) atomicAdd(&counters[k], ((lds_u64 *)words)[k]);
 }
 
 // Möller–Trumbore on a slot record, common.wgsl:78-100 (edges precomputed).
@@ -417,12 +462,12 @@ __device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, L
     uint32_t idx = 0u, sp = 0u;
     if (S.n_nodes == 0u) return hit;
     for (;;) {
-        if (COUNT) count_step(cnt, cnt.trav_phase);
+        if (COUNT) count_traversal_step(cnt);
         const rt_bvh_node &n = S.nodes[idx];
         const float4 a = *reinterpret_cast<const float4 *>(&n.bbox_min[0]);
         const float4 b = *reinterpret_cast<const float4 *>(&n.bbox_max[0]);
         const uint32_t prim_count = n.prim_count;
-        if (COUNT) cnt.nodes++;
+        if (COUNT) cnt.add(kCntNodes, 1);
         float te, ta;
         slab(xyz(a), xyz(b), ro, inv, te, ta);
         bool pop = true;
@@ -432,7 +477,7 @@ __device__ Hit trace_reference(const TraceScene &S, F3 ro, F3 rd, float t_max, L
                 for (uint32_t i = 0u; i < prim_count; i++) {
                     const uint32_t slot = left_first + i;
                     const float4 q0 = S.slot_tris[3 * slot], q1 = S.slot_tris[3 * slot + 1], q2 = S.slot_tris[3 * slot + 2];
-                    if (COUNT) cnt.tris++;
+                    if (COUNT) cnt.add(kCntTris, 1);
                     const float t = mt_slot(ro, rd, xyz(q0), xyz(q1), xyz(q2));
                     if (t < hit.t) {
                         hit.t = t;
@@ -489,19 +534,19 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
     if (S.n_nodes == 0u) return hit;
     float te, ta;
     slab(xyz(S.root_min), xyz(S.root_max), ro, inv, te, ta);
-    if (COUNT) cnt.nodes++;
+    if (COUNT) cnt.add(kCntNodes, 1);
     if (!(te <= ta && ta > 0.0f)) return hit;
     uint32_t sp = 0u;
     uint32_t cur = __float_as_uint(S.root_min.w);  // root ref (internal 0, or a leaf ref)
     for (;;) {
-        if (COUNT) count_step(cnt, cnt.trav_phase);
+        if (COUNT) count_traversal_step(cnt);
         if (ref_is_leaf(cur)) {
             const uint32_t first = cur & 0x07FFFFFFu;
             const uint32_t count = ((cur >> 27) & 15u) + 1u;
             for (uint32_t i = 0u; i < count; i++) {
                 const uint32_t slot = first + i;
                 const float4 q0 = S.slot_tris[3 * slot], q1 = S.slot_tris[3 * slot + 1], q2 = S.slot_tris[3 * slot + 2];
-                if (COUNT) cnt.tris++;
+                if (COUNT) cnt.add(kCntTris, 1);
                 const float t = mt_slot(ro, rd, xyz(q0), xyz(q1), xyz(q2));
                 if (!NEAREST) {
                     if (t < hit.t) {
@@ -518,7 +563,7 @@ __device__ Hit trace_wide(const TraceScene &S, F3 ro, F3 rd, float t_max, LaneSt
         } else {
             const float4 *w = S.wnodes + 4u * cur;
             const float4 a = w[0], b = w[1], c = w[2], d = w[3];
-            if (COUNT) cnt.nodes += 2;
+            if (COUNT) cnt.add(kCntNodes, 2);
             float tel, tal, ter, tar;
             slab(xyz(a), xyz(b), ro, inv, tel, tal);
             slab(xyz(c), xyz(d), ro, inv, ter, tar);
@@ -622,7 +667,7 @@ __device__ __forceinline__ bool trav4_begin(const TraceScene &S, Trav4<WT> &q, F
     } else {
         slab(xyz(S.root_min), xyz(S.root_max), ro, q.inv, te, ta);
     }
-    if (COUNT) cnt.nodes++;
+    if (COUNT) cnt.add(kCntNodes, 1);
     q.cur = __float_as_uint(S.root_min.w);
     return !(te <= ta && ta > 0.0f);
 }
@@ -648,7 +693,10 @@ __device__ __forceinline__ bool trav4_leaf(const TraceScene &S, Trav4<WT> &q, Tr
         const float4 *pb = p + (two ? 3u : 0u);
         const float4 a0 = p[0], a1 = p[1], a2 = p[2];
         const float4 b0 = pb[0], b1 = pb[1], b2 = pb[2];
-        if (COUNT) cnt.tris += two ? 2u : 1u;
+        if (COUNT) {
+            cnt.add(kCntTris, 1);
+            if (two) cnt.add(kCntTris, 1);
+        }
         const float t = WT ? wt_slot(q.wr, q.ro, xyz(a0), xyz(a1), xyz(a2))
                            : mt_slot(q.ro, q.rd, xyz(a0), xyz(a1), xyz(a2));
         if (t < q.hit.t) {
@@ -700,7 +748,7 @@ __device__ __forceinline__ bool trav4_node(const TraceScene &S, Trav4<WT> &q, La
                 pass &= ~(1u << k);
         }
     }
-    if (COUNT) cnt.nodes += 4;
+    if (COUNT) cnt.add(kCntNodes, 4);
     if (pass != 0u) {
         // push passing children after the first, rightmost first; when no lane
         // can reach the spill words (at most three entries are pushed) the
@@ -760,7 +808,7 @@ __device__ __forceinline__ bool trav4_pop(Trav4<WT> &q, LaneStack &stk) {
 // a record run their steps in the same loop iteration.
 template <bool WT, bool COUNT>
 __device__ __forceinline__ bool trav4_step(const TraceScene &S, Trav4<WT> &q, LaneStack &stk, TraceCount &cnt) {
-    if (COUNT) count_step(cnt, cnt.trav_phase);
+    if (COUNT) count_traversal_step(cnt);
     if (ref_is_leaf(q.cur)) {
         if (trav4_leaf<WT, COUNT>(S, q, cnt)) return true;
     } else if (trav4_node<WT, COUNT>(S, q, stk, cnt)) {

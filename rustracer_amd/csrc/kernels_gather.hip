@@ -62,11 +62,10 @@ struct Tracer {
     const TraceScene &ts;
     LaneStack &stk;
     TraceCount cnt;
-    uint32_t queries;
     __device__ Hit closest(F3 ro, F3 rd) {
-        const uint64_t t0 = COUNT ? clock64() : 0u;
+        const uint32_t t0 = COUNT ? phase_clock() : 0u;
         if (COUNT) {
-            queries++;
+            cnt.add(kCntQueries, 1);
             cnt.trav_phase = kPhaseTraversal;
         }
         const Hit h = trace_mode<MODE, false, COUNT>(ts, ro, rd, kInf, stk, cnt);
@@ -78,35 +77,16 @@ struct Tracer {
     // is that hit, not necessarily the closest).
     __device__ bool shadowed(F3 origin, F3 dir, float max_dist, Hit &h) {
         if (COUNT) {
-            queries++;
+            cnt.add(kCntQueries, 1);
             cnt.trav_phase = kPhaseShadow;
         }
-        const uint64_t t0 = COUNT ? clock64() : 0u;
+        const uint32_t t0 = COUNT ? phase_clock() : 0u;
         const float lim = max_dist - 0.001f;
         h = trace_mode<MODE, EARLY, COUNT>(ts, origin, dir, lim, stk, cnt);
         if (COUNT) count_time(cnt, kPhaseShadow, t0);
         return !(h.t >= lim);
     }
 };
-
-// The counting build's totals of one lane: counters[0..3] queries, child boxes,
-// triangles, photon candidates; then the lane and wave steps of each phase.
-__device__ void flush_counts(unsigned long long *counters, uint32_t queries, const TraceCount &cnt,
-                             uint32_t photon_reads) {
-    atomicAdd(&counters[0], (unsigned long long)queries);
-    atomicAdd(&counters[1], (unsigned long long)cnt.nodes);
-    atomicAdd(&counters[2], (unsigned long long)cnt.tris);
-    atomicAdd(&counters[3], (unsigned long long)photon_reads);
-    for (int k = 0; k < kCountPhases; k++) {
-        atomicAdd(&counters[4 + k], (unsigned long long)cnt.step_lane[k]);
-        atomicAdd(&counters[4 + kCountPhases + k], (unsigned long long)cnt.step_wave[k]);
-        atomicAdd(&counters[4 + 2 * kCountPhases + k], (unsigned long long)cnt.wave_cycles[k]);
-    }
-}
-template <class T>
-__device__ void flush_counts(unsigned long long *counters, const T &tr, uint32_t photon_reads) {
-    flush_counts(counters, tr.queries, tr.cnt, photon_reads);
-}
 
 // gather.wgsl:91-103 with alpha-cutout: the closest hit decides, and a
 // transparent texel of a cutout material lets the shadow ray through. Scenes
@@ -485,7 +465,7 @@ __device__ uint32_t bounce_hit(const GatherArgs &A, const Env &env, const Hit &h
 
 template <bool COUNT, class AccType>
 __device__ bool bounce_light(const GatherArgs &A, const Env &env, const SurfaceState &sf, bool occl, F3 &ro, F3 &rd,
-                             F3 &thr, Rng &rng, AccType &acc, TraceCount &cnt, uint32_t &photon_reads, bool *smooth) {
+                             F3 &thr, Rng &rng, AccType &acc, TraceCount &cnt, bool *smooth) {
     const rt_image_params &ip = A.ip;
     const F3 hp = sf.hp, shn = sf.shn, f0 = sf.f0, dscale = sf.dscale, albedo = sf.albedo;
     const float a = sf.a, spec_prob = sf.spec_prob;
@@ -502,7 +482,7 @@ __device__ bool bounce_light(const GatherArgs &A, const Env &env, const SurfaceS
         acc.add(thr * spec_ggx(wo, shn, ls.wi, a, f0) * ls.e / spec_prob);
     }
 
-    const uint64_t t_ph = COUNT ? clock64() : 0u;
+    const uint32_t t_ph = COUNT ? phase_clock() : 0u;
     if (ip.use_photons == 1u && !do_spec) {
         // gather.wgsl:473-514
         const rt_grid_params &gp = A.gp;
@@ -568,7 +548,7 @@ __device__ bool bounce_light(const GatherArgs &A, const Env &env, const SurfaceS
                                     done = true;
                                 } else {
                                     budget -= 1u;
-                                    if (COUNT) photon_reads++;
+                                    if (COUNT) cnt.add(kCntPhotons, 1);
                                     const F3 d = xyz(g0[j]) - hp;
                                     const float d2 = dot(d, d);
                                     if (d2 < r2) {
@@ -619,9 +599,9 @@ __device__ bool bounce_light(const GatherArgs &A, const Env &env, const SurfaceS
 // shadow query, the rest of the shading.
 template <int MODE, bool EARLY, bool COUNT, class AccType>
 __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, COUNT> &tr, uint32_t b, F3 &ro,
-                       F3 &rd, F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t &photon_reads, uint32_t px,
+                       F3 &rd, F3 &thr, Rng &rng, AccType &acc, FirstHit &fh, uint32_t px,
                        uint32_t py, bool *smooth = nullptr) {
-    const uint64_t t0 = COUNT ? clock64() : 0u;
+    const uint32_t t0 = COUNT ? phase_clock() : 0u;
     if (COUNT) count_step(tr.cnt, kPhaseBounce);
     const Hit hit = tr.closest(ro, rd);
     SurfaceState sf;
@@ -629,7 +609,7 @@ __device__ bool bounce(const GatherArgs &A, const Env &env, Tracer<MODE, EARLY, 
     bool cont = st == kPathGoesOn;
     if (st == kSurfaceLit) {
         const bool occl = sf.ls.cast && occluded(A, tr, sf.hp + sf.shn * 0.01f, sf.ls.wi, sf.ls.reach);
-        cont = bounce_light<COUNT>(A, env, sf, occl, ro, rd, thr, rng, acc, tr.cnt, photon_reads, smooth);
+        cont = bounce_light<COUNT>(A, env, sf, occl, ro, rd, thr, rng, acc, tr.cnt, smooth);
     }
     if (COUNT) count_time(tr.cnt, kPhaseBounce, t0);
     return cont;
@@ -742,7 +722,7 @@ __device__ void finalize(const GatherArgs &A, AccT<NS> &acc, uint32_t spp, const
 // moments / G-buffer arrays.
 template <int MODE, bool EARLY, bool COUNT>
 __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MODE, EARLY, COUNT> &tr, uint32_t px,
-                             uint32_t py, size_t out_idx, uint32_t &photon_reads) {
+                             uint32_t py, size_t out_idx) {
     Rng rng = pixel_rng(V, px, py);
     Acc acc;
     FirstHit fh;
@@ -754,7 +734,7 @@ __device__ void gather_pixel(const GatherArgs &A, const ViewParams &V, Tracer<MO
         camera_ray(V, px, py, rng, ro, rd);
         F3 thr = fs(1.0f);
         for (uint32_t b = 0u; b < A.max_depth; b++)
-            if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads, px, py)) break;
+            if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, px, py)) break;
         acc.end_sample();
     }
     finalize(A, acc, spp, fh, out_idx);
@@ -776,13 +756,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
-    uint32_t photon_reads = 0u;
+    __shared__ unsigned long long cnt_words[kCntWords];
+    if (COUNT) count_begin(cnt_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{(lds_u64 *)cnt_words, 0u}};
     const ViewParams V = frame_view(A);
     if (px < A.ip.width && py < A.ip.height)
-        gather_pixel<MODE, EARLY, COUNT>(A, V, tr, px, py, (size_t)py * A.ip.width + px, photon_reads);
+        gather_pixel<MODE, EARLY, COUNT>(A, V, tr, px, py, (size_t)py * A.ip.width + px);
     if (COUNT) {
-        flush_counts(A.counters, tr, photon_reads);
+        count_flush(A.counters, cnt_words);
     }
 }
 
@@ -863,8 +844,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
-    uint32_t photon_reads = 0u;
+    __shared__ unsigned long long cnt_words[kCntWords];
+    if (COUNT) count_begin(cnt_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{(lds_u64 *)cnt_words, 0u}};
     const ViewParams V = frame_view(A);
     const uint32_t spp = max(1u, V.spp);
     const Env env = find_env(A);
@@ -877,7 +859,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
     acc_clear(acc, fh);
     F3 ro = fs(0.0f), rd = fs(0.0f), thr = fs(0.0f);
     for (;;) {
-        const uint64_t t_it = COUNT ? clock64() : 0u;
+        const uint32_t t_it = COUNT ? phase_clock() : 0u;
         if (feed.take(A, live, px, py, fi)) {
             rng = pixel_rng(V, px, py, V.frame + fi);
             acc_clear(acc, fh);
@@ -889,7 +871,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
         if (__ballot(live) == 0ull) break;
         if (live) {
             if (COUNT) count_step(tr.cnt, kPhaseLoop);
-            const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads, px, py);
+            const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, px, py);
             b++;
             if (!cont || b >= A.max_depth) {
                 acc.end_sample();
@@ -907,7 +889,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
         if (COUNT) count_time(tr.cnt, kPhaseLoop, t_it);
     }
     if (COUNT) {
-        flush_counts(A.counters, tr, photon_reads);
+        count_flush(A.counters, cnt_words);
     }
 }
 
@@ -1010,8 +992,9 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
-    uint32_t photon_reads = 0u;
+    __shared__ unsigned long long cnt_words[kCntWords];
+    if (COUNT) count_begin(cnt_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{(lds_u64 *)cnt_words, 0u}};
     const ViewParams V = frame_view(A);
     const uint32_t spp = max(1u, V.spp);
     const uint32_t idx = py * A.ip.width + px;
@@ -1032,7 +1015,7 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
         thr = fs(1.0f);
         const Env env = find_env(A);
         const bool cont =
-            bounce<MODE, EARLY, COUNT>(A, env, tr, 0u, ro, rd, thr, rng, acc, fh, photon_reads, px, py, &smooth);
+            bounce<MODE, EARLY, COUNT>(A, env, tr, 0u, ro, rd, thr, rng, acc, fh, px, py, &smooth);
         if (cont && A.max_depth > 1u) {
             push = true;
             store_running(A, idx, acc, 0u);
@@ -1045,7 +1028,7 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     const uint32_t back = wave_append(push && smooth, &A.path_count[2]);
     if (push) queue_put(A.queue[1], A.n_px, smooth ? A.n_px - 1u - back : front, ro, rd, thr, idx, rng.state);
     if (COUNT) {
-        flush_counts(A.counters, tr, photon_reads);
+        count_flush(A.counters, cnt_words);
     }
 }
 
@@ -1060,8 +1043,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{}, 0u};
-    uint32_t photon_reads = 0u;
+    __shared__ unsigned long long cnt_words[kCntWords];
+    if (COUNT) count_begin(cnt_words);
+    Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{(lds_u64 *)cnt_words, 0u}};
     const uint32_t n_front = __builtin_amdgcn_readfirstlane(A.path_count[1]);
     const uint32_t count = n_front + __builtin_amdgcn_readfirstlane(A.path_count[2]);
     const float4 *qin = A.queue[1];
@@ -1086,7 +1070,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
             uint32_t unused;
             load_running(A, idx, acc, unused);
             for (uint32_t b = 1u; b < A.max_depth; b++)
-                if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads,
+                if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh,
                                                 idx % A.ip.width, idx / A.ip.width))
                     break;
             load_samples(A, idx, acc, fh);
@@ -1094,7 +1078,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
         }
     }
     if (COUNT) {
-        flush_counts(A.counters, tr, photon_reads);
+        count_flush(A.counters, cnt_words);
     }
 }
 
@@ -1127,8 +1111,7 @@ __global__ __launch_bounds__(64) void gather_batch_kernel(BatchArgs B) {
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{}, 0u};
-    uint32_t photon_reads = 0u;
+    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{nullptr, 0u}};
     const uint32_t rounds = (B.tile_w * B.tile_h) / 64u;
     for (;;) {
         uint32_t first = 0u;
@@ -1147,8 +1130,7 @@ __global__ __launch_bounds__(64) void gather_batch_kernel(BatchArgs B) {
                 const uint32_t k = r * 64u + threadIdx.x;
                 const uint32_t px = tx * B.tile_w + k % B.tile_w, py = ty * B.tile_h + k / B.tile_w;
                 if (px < V.width && py < V.height)
-                    gather_pixel<MODE, EARLY, false>(B.g, V, tr, px, py, bv.pixel_offset + (size_t)py * V.width + px,
-                                                     photon_reads);
+                    gather_pixel<MODE, EARLY, false>(B.g, V, tr, px, py, bv.pixel_offset + (size_t)py * V.width + px);
             }
         }
     }
@@ -1164,8 +1146,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{}, 0u};
-    uint32_t photon_reads = 0u;
+    Tracer<MODE, EARLY, false> tr{B.g.ts, stk, TraceCount{nullptr, 0u}};
     const Env env = find_env(B.g);
     const uint32_t lane = __lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -1214,7 +1195,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
         if (__ballot(live) == 0ull) break;
         if (live) {
             const bool cont =
-                bounce<MODE, EARLY, false>(B.g, env, tr, b, ro, rd, thr, rng, acc, fh, photon_reads, px, py);
+                bounce<MODE, EARLY, false>(B.g, env, tr, b, ro, rd, thr, rng, acc, fh, px, py);
             b++;
             if (!cont || b >= B.g.max_depth) {
                 acc.end_sample();
@@ -1243,7 +1224,7 @@ __global__ __launch_bounds__(64) void trace_rays_kernel(TraceScene ts, const rt_
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    TraceCount cnt{};
+    TraceCount cnt{nullptr, 0u};
     const rt_ray r = rays[i];
     const F3 ro = ld3(r.origin), rd = ld3(r.dir);
     const Hit h = trace_mode<MODE, false, false>(ts, ro, rd, kInf, stk, cnt);
@@ -1262,7 +1243,7 @@ __global__ __launch_bounds__(64) void albedo_kernel(TraceScene ts, const rt_tria
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    TraceCount cnt{};
+    TraceCount cnt{nullptr, 0u};
     const uint32_t px = i % w, py = i / w;
     const float ux = (float)px * cam.inv_half_res[0] - 1.0f;
     const float uy = 1.0f - (float)py * cam.inv_half_res[1];

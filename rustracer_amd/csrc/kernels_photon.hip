@@ -69,7 +69,7 @@ __global__ __launch_bounds__(64) void photon_trace_kernel(PhotonArgs A) {
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
     stk.init(stack_rows, spill_words);
-    TraceCount cnt{0u, 0u};
+    TraceCount cnt{nullptr, 0u};
     const uint32_t num_lights = A.n_lights;
     Rng rng = rng_init(photon_index * 1103515245u + 12345u);
     uint32_t li = wgsl_u32(rng_f32(rng) * (float)num_lights);
