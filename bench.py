@@ -44,6 +44,7 @@ TRI_BYTES = 48         # slot triangle: v0, v1-v0, v2-v0 (3 x float4)
 RAY_BYTES = 32         # 24 B ray + 8 B hit
 PHOTON_BYTES = 36      # per radius-gather candidate: position + power (16), power + incident (16), incident.z (4)
 PIXEL_OUT_BYTES = 48   # frame + moments + gbuffer float4 writes
+PRESENT_REPEATS = 5
 DENOISE_PX_BYTES = 64  # per pass: color in + gbuffer + moments + out, 16 B each
 COMPOSITE_TONEMAP_PX_BYTES = 52  # frame 16 + accumulation 16 read + 16 written + rgba8 4
 GRID_SLOT_BYTES = 32   # count (position + power.r, 16) + scatter key read (16) per slot
@@ -205,8 +206,17 @@ def main() -> int:
         r.synchronize()
     issued += PROFILE_FRAMES
     stats = r.pass_stats()
-    set_profile(False)
     w, h = args.width, args.height
+    # the present pass (fullscreen.wgsl) of the last output into a 4K surface and
+    # into one of the render size: mean kernel time over a few launches
+    present_ms = {}
+    for sw, sh in ((3840, 2160), (w, h)):
+        r.reset_pass_stats()
+        for _ in range(PRESENT_REPEATS):
+            r.read_present(sw, sh)
+        ms_, n_ = r.pass_stats()["present"]
+        present_ms[(sw, sh)] = ms_ / max(n_, 1)
+    set_profile(False)
     samples = w * h * args.steps
     msps = samples / elapsed / 1e6
     ms_per_step = elapsed / args.steps * 1e3
@@ -278,6 +288,12 @@ def main() -> int:
             result["pass_rooflines"][name] = {"bound": "hbm", "achieved": round(gbs, 1), "unit": "GB/s",
                                               "frac": round(gbs / HBM_PEAK_GBS, 4),
                                               "bytes_per_launch": int(nbytes)}
+    for (sw, sh), ms_ in present_ms.items():
+        nbytes = w * h * 4 + sw * sh * 4  # rgba8 output read once, bgra8 surface written once
+        gbs = nbytes / (ms_ * 1e-3) / 1e9
+        result["pass_rooflines"][f"present_{sw}x{sh}"] = {"bound": "hbm", "achieved": round(gbs, 1), "unit": "GB/s",
+                                                           "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                                           "bytes_per_launch": int(nbytes), "ms": round(ms_, 4)}
     if photons_on and photon_stats.get("grid_build", (0, 0))[1]:
         srt = r.read_sorted_photons()
         n_slots = min(max(args.photons * 6 // 256, 1) * 256, gp.photon_slots)

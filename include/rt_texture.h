@@ -54,4 +54,60 @@ RT_WM_FN void rt_tex_sample(const rt_tex_array *t, float u, float v, uint32_t la
     }
 }
 
+/*
+ * The present pass (fullscreen.wgsl, drawn by Renderer::record_frame,
+ * lib.rs:834-850): the tonemapped rgba8 output texture sampled through the
+ * quad sampler (ClampToEdge, linear min/mag filter, no mip levels;
+ * lib.rs:555-562) at the fullscreen triangle's interpolated uv, written to a
+ * Bgra8Unorm surface (rustracer-app/src/main.rs:320-324). The surface keeps
+ * its own size when the window is resized (main.rs:156-162), so the pass
+ * resamples the render-size output to the surface size.
+ *
+ * Surface pixel p of n_s along an axis has its fragment centre at p + 0.5 and
+ * interpolated coordinate u = (p + 0.5) / n_s (fullscreen.wgsl:16-19: uv =
+ * ((x + 1) / 2, (1 - y) / 2) over clip space, rows top to bottom); the Vulkan
+ * bilinear rule samples texel position x = u*n_t - 0.5 between texels floor(x)
+ * and floor(x) + 1, both clamped into [0, n_t). The interpolation precision,
+ * the sub-texel precision of the weights and the float -> unorm8 rounding are
+ * implementation-defined on the GPU; this backend fixes them as:
+ *   x = (p + 0.5) * s - 0.5 in float32, with s = n_t / n_s rounded once;
+ *   the fraction x - floor(x) rounded to 8 bits (round to nearest even):
+ *   a = rint((x - floor(x)) * 256) in 0..256, weights 256 - a and a (the
+ *   sub-texel precision Vulkan drivers report as subTexelPrecisionBits = 8);
+ *   per channel the exact integer blend S = wy0*(wx0*t00 + wx1*t10) +
+ *   wy1*(wx0*t01 + wx1*t11) of the bytes, and the unorm8 result
+ *   (S + 32768) >> 16, the bilinear value S / 65536 rounded half up.
+ */
+RT_WM_FN float rt_present_scale(uint32_t n_t, uint32_t n_s) { return (float)n_t / (float)n_s; }
+
+RT_WM_FN void rt_present_axis(uint32_t p, float scale, uint32_t n_t, uint32_t *i0, uint32_t *i1, uint32_t *a) {
+    const float x = ((float)p + 0.5f) * scale - 0.5f;
+    const float f = __builtin_floorf(x);
+    /* 0..256; the mask changes no value and lets the GPU compiler use 24-bit multiplies */
+    *a = (uint32_t)__builtin_rintf((x - f) * 256.0f) & 0x1FFu;
+    const int32_t last = (int32_t)n_t - 1;
+    const int32_t k = (int32_t)f; /* x >= -0.5, so k >= -1 */
+    *i0 = (uint32_t)(k < 0 ? 0 : (k > last ? last : k));
+    *i1 = (uint32_t)(k + 1 > last ? last : (k + 1 < 0 ? 0 : k + 1));
+}
+
+/* Four rgba8 texels (byte 0 = r) filtered with 8-bit weights (ax, ay), as one
+ * bgra8 surface pixel (byte 0 = b, byte 3 = a). */
+RT_WM_FN uint32_t rt_present_blend(uint32_t t00, uint32_t t10, uint32_t t01, uint32_t t11, uint32_t ax,
+                                   uint32_t ay) {
+    ax &= 0x1FFu;
+    ay &= 0x1FFu;
+    const uint32_t bx = (256u - ax) & 0x1FFu, by = (256u - ay) & 0x1FFu;
+    uint32_t out = 0;
+    for (int c = 0; c < 4; c++) {
+        const int sh = 8 * c;
+        const uint32_t r0 = bx * ((t00 >> sh) & 0xFFu) + ax * ((t10 >> sh) & 0xFFu);
+        const uint32_t r1 = bx * ((t01 >> sh) & 0xFFu) + ax * ((t11 >> sh) & 0xFFu);
+        const uint32_t q = (by * r0 + ay * r1 + 32768u) >> 16;
+        const int dst = c == 3 ? 24 : 8 * (2 - c); /* r -> byte 2, g -> 1, b -> 0, a -> 3 */
+        out |= q << dst;
+    }
+    return out;
+}
+
 #endif /* RT_TEXTURE_H */

@@ -275,6 +275,20 @@ int rt_read_moments(rt_renderer *r, float *rgba, size_t n_floats);
 int rt_read_gbuffer(rt_renderer *r, float *rgba, size_t n_floats);
 int rt_read_output_rgba8(rt_renderer *r, uint8_t *rgba, size_t n_bytes);
 int rt_write_accumulation(rt_renderer *r, const float *rgba, size_t n_floats);
+
+/* The present pass of Renderer::record_frame (lib.rs:834-850, fullscreen.wgsl:
+ * the fullscreen-triangle render pass into the surface_view argument of
+ * record_frame, lib.rs:667-670): the tonemapped rgba8 output, sampled with
+ * ClampToEdge bilinear filtering (lib.rs:555-562), written as Bgra8Unorm pixels
+ * (the app's surface format, rustracer-app/src/main.rs:320-324) into a surface
+ * of surface_w x surface_h that need not have the render size. d_surface is
+ * device memory with row_pitch bytes between rows (a multiple of 4, at least
+ * 4*surface_w); the pass is queued on the renderer's stream after the frames
+ * already issued and returns without waiting. Sides are 1..32768. */
+int rt_present(rt_renderer *r, void *d_surface, uint32_t surface_w, uint32_t surface_h, size_t row_pitch);
+/* rt_present into a surface buffer the renderer owns (allocated on the first
+ * call of a larger size), read back to the host with tight rows. */
+int rt_read_present(rt_renderer *r, uint32_t surface_w, uint32_t surface_h, uint8_t *bgra, size_t n_bytes);
 int rt_read_photons(rt_renderer *r, rt_photon *out, size_t capacity, uint32_t *n_slots);
 int rt_read_sorted_photons(rt_renderer *r, rt_photon *out, size_t capacity, uint32_t *n_stored);
 int rt_read_grid_meta(rt_renderer *r, uint32_t *meta, size_t n_uint32, uint32_t *n_cells);
@@ -409,6 +423,7 @@ int rt_read_batch_view(rt_renderer *r, uint32_t view, float *frame, float *momen
                        size_t n_floats);
 #define RT_PASS_BATCH 7
 #define RT_PASS_NETWORK 8   /* learned denoiser: albedo pass, network, tonemap of its output */
+#define RT_PASS_PRESENT 9   /* rt_present / rt_read_present */
 
 /* Measured HBM bandwidth: a plain device-to-device copy kernel of `bytes`,
  * repeated `iters` times; *gbps counts bytes read + written. No counterpart in

@@ -1303,6 +1303,34 @@ int or_tonemap_cpu(const float *accum, uint64_t n_px, uint8_t *rgba) {
 }
 
 
+// fullscreen.wgsl:13-25: vs_main's fullscreen triangle gives each surface pixel
+// uv = ((x + 0.5) / sw, (y + 0.5) / sh); fs_main samples the output texture
+// there with the quad sampler (lib.rs:555-562: ClampToEdge, linear). The
+// sampler's arithmetic is the shared rt_present_axis / rt_present_blend.
+int or_present(const uint8_t *rgba, uint32_t tw, uint32_t th, uint8_t *bgra, uint32_t sw, uint32_t sh) {
+    if (!rgba || !bgra || tw == 0 || th == 0) return -1;
+    const float scale_x = rt_present_scale(tw, sw), scale_y = rt_present_scale(th, sh);
+    for (uint32_t y = 0; y < sh; y++) {
+        uint32_t y0, y1, ay;
+        rt_present_axis(y, scale_y, th, &y0, &y1, &ay);
+        for (uint32_t x = 0; x < sw; x++) {
+            uint32_t x0, x1, ax;
+            rt_present_axis(x, scale_x, tw, &x0, &x1, &ax);
+            uint32_t t[4];
+            const uint64_t at[4] = {(uint64_t)y0 * tw + x0, (uint64_t)y0 * tw + x1, (uint64_t)y1 * tw + x0,
+                                    (uint64_t)y1 * tw + x1};
+            for (int k = 0; k < 4; k++) {
+                const uint8_t *p = rgba + 4 * at[k];
+                t[k] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+            }
+            const uint32_t o = rt_present_blend(t[0], t[1], t[2], t[3], ax, ay);
+            uint8_t *q = bgra + 4 * ((uint64_t)y * sw + x);
+            for (int k = 0; k < 4; k++) q[k] = (uint8_t)(o >> (8 * k));
+        }
+    }
+    return 0;
+}
+
 int or_photon_estimate(const or_photon_map *pm, const float hp[3], const float n[3], float radius, float sum[3],
                        uint32_t *hits, uint64_t *reads) {
     uint64_t r = 0;

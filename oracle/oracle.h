@@ -87,6 +87,9 @@ int or_composite(const float *frame, float *accum, uint32_t w, uint32_t h, float
 int or_tonemap(const float *accum, const float *gbuffer, const rt_tonemap_uniform *p,
                uint32_t w, uint32_t h, uint8_t *rgba);
 int or_tonemap_cpu(const float *accum, uint64_t n_px, uint8_t *rgba);
+/* fullscreen.wgsl:11-25 through the quad sampler (lib.rs:555-562): the rgba8
+ * output (tw x th) resampled into a bgra8 surface of sw x sh, tight rows. */
+int or_present(const uint8_t *rgba, uint32_t tw, uint32_t th, uint8_t *bgra, uint32_t sw, uint32_t sh);
 /* The photon radius gather of gather.wgsl:473-514 at one point: sum of
  * power * max(n . -incident, 0) and the photon count, as or_gather uses it. */
 int or_photon_estimate(const or_photon_map *pm, const float hp[3], const float n[3], float radius, float sum[3],

@@ -361,6 +361,17 @@ class Renderer:
         check(lib().rt_read_output_rgba8(self._h, ptr(out), out.size), "rt_read_output_rgba8")
         return out
 
+    def read_present(self, surface_w: int, surface_h: int) -> np.ndarray:
+        """The present pass (fullscreen.wgsl) into a surface_w x surface_h
+        bgra8 surface, read back as (surface_h, surface_w, 4) bytes b, g, r, a."""
+        out = np.zeros((surface_h, surface_w, 4), np.uint8)
+        check(lib().rt_read_present(self._h, surface_w, surface_h, ptr(out), out.size), "rt_read_present")
+        return out
+
+    def present(self, d_surface: int, surface_w: int, surface_h: int, row_pitch: int) -> None:
+        """The present pass into device memory at d_surface (row_pitch bytes per row)."""
+        check(lib().rt_present(self._h, C.c_void_p(d_surface), surface_w, surface_h, row_pitch), "rt_present")
+
     def write_accumulation(self, rgba: np.ndarray) -> None:
         a = np.ascontiguousarray(rgba, np.float32)
         check(lib().rt_write_accumulation(self._h, ptr(a), a.size), "rt_write_accumulation")

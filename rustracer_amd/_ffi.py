@@ -50,7 +50,7 @@ FLAG_GRAPH = 128
 FLAG_MOMENT_HISTORY = 256
 FLAG_WATERTIGHT = 512
 PASS_NAMES = ("gather", "denoise", "composite_tonemap", "photon_trace", "grid_build", "trace_rays",
-              "accum64", "batch", "network")
+              "accum64", "batch", "network", "present")
 
 
 class Config(C.Structure):
@@ -202,6 +202,8 @@ SIGNATURES = {
     "rt_read_gbuffer": (_i, [_vp, _vp, C.c_size_t]),
     "rt_read_output_rgba8": (_i, [_vp, _vp, C.c_size_t]),
     "rt_write_accumulation": (_i, [_vp, _vp, C.c_size_t]),
+    "rt_present": (_i, [_vp, _vp, _u32, _u32, C.c_size_t]),
+    "rt_read_present": (_i, [_vp, _u32, _u32, _vp, C.c_size_t]),
     "rt_read_photons": (_i, [_vp, _vp, C.c_size_t, _P(_u32)]),
     "rt_read_sorted_photons": (_i, [_vp, _vp, C.c_size_t, _P(_u32)]),
     "rt_read_grid_meta": (_i, [_vp, _vp, C.c_size_t, _P(_u32)]),

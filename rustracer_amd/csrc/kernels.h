@@ -168,6 +168,10 @@ hipError_t launch_composite_tonemap(const void *frame, void *accum, uint32_t *rg
                                     const FrameState *fstate, hipStream_t s);
 hipError_t launch_tonemap(const void *img, uint32_t *rgba8, uint32_t w, uint32_t h, const rt_tonemap_uniform &tm,
                           const void *gbuffer, hipStream_t s);
+// The present pass: rgba8 output (tw x th, tight rows) resampled into a bgra8
+// surface of sw x sh with pitch_words 32-bit words between rows.
+hipError_t launch_present(const uint32_t *rgba8, uint32_t tw, uint32_t th, uint32_t *surface, uint32_t sw,
+                          uint32_t sh, uint32_t pitch_words, hipStream_t s);
 // First-hit albedo of the pixel-centre camera ray (rgb; w = 1 on a hit, 0 on a miss).
 hipError_t launch_albedo(const TraceInputs &t, uint32_t mode, const rt_triangle *tris, const rt_material *mats,
                          const rt_camera_uniform &cam, uint32_t w, uint32_t h, void *out, hipStream_t s);

@@ -284,23 +284,46 @@ __global__ __launch_bounds__(256) void relmse_final_kernel(const double *__restr
     if (threadIdx.x == 0) out[0] = red[0] * scale;
 }
 
-// Plain streaming copy (16 B per lane per iteration): the measured-HBM
-// denominator of the rooflines and the calibration kernel of the traffic counters.
-// Grid-stride copy with four independent 16-byte loads in flight per lane
-// before their stores (one load per lane per iteration leaves HBM idle
-// between a wavefront's round trips).
+// Plain streaming copy, one 16-byte load and store per lane and no loop: the
+// measured-HBM denominator of the rooflines and the calibration kernel of the
+// traffic counters. Measured on a 1 GiB copy: 6.2 TB/s (read + write) for this
+// form; grid-stride loops over 2k-32k workgroups with one or four loads in
+// flight per lane reached 4.2-5.1 TB/s, hipMemcpy device-to-device 5.0 TB/s.
 __global__ __launch_bounds__(256) void hbm_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
                                                        size_t n) {
-    const size_t stride = (size_t)gridDim.x * 256u;
-    size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
-    for (; i + 3u * stride < n; i += 4u * stride) {
-        const float4 a = src[i], b = src[i + stride], c = src[i + 2u * stride], d = src[i + 3u * stride];
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2u * stride] = c;
-        dst[i + 3u * stride] = d;
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// The present pass (fullscreen.wgsl): a wavefront per 256 pixels of a surface
+// row, four per lane at a stride of 64, so every store instruction writes 256
+// contiguous bytes and the row's vertical taps and weights are worked out once
+// per lane. The four texels of each pixel come from two output rows that
+// neighbouring lanes share (L2 after the first touch). The blend is integer
+// (24-bit multiplies on bytes picked out by SDWA operands); with the float
+// sample positions and 64-bit addresses it is about 77 VALU instructions per
+// pixel, which bounds the pass before HBM does (1080p -> 4K: 0.022 ms).
+constexpr uint32_t kPresentPx = 4;  // pixels per lane
+
+__global__ __launch_bounds__(256) void present_kernel(const uint32_t *__restrict__ rgba8, uint32_t tw, uint32_t th,
+                                                      uint32_t *__restrict__ surface, uint32_t sw, uint32_t sh,
+                                                      uint32_t pitch_words, float scale_x, float scale_y) {
+    const uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6);
+    if (y >= sh) return;
+    uint32_t y0, y1, ay;
+    rt_present_axis(y, scale_y, th, &y0, &y1, &ay);
+    const uint32_t *r0 = rgba8 + (size_t)y0 * tw, *r1 = rgba8 + (size_t)y1 * tw;
+    uint32_t *out = surface + (size_t)y * pitch_words;
+    const uint32_t xb = blockIdx.x * (64u * kPresentPx) + (threadIdx.x & 63u);
+#pragma unroll
+    for (uint32_t k = 0; k < kPresentPx; k++) {
+        const uint32_t x = xb + 64u * k;
+        if (x < sw) {
+            uint32_t x0, x1, ax;
+            rt_present_axis(x, scale_x, tw, &x0, &x1, &ax);
+            out[x] = rt_present_blend(r0[x0], r0[x1], r1[x0], r1[x1], ax, ay);
+        }
     }
-    for (; i < n; i += stride) dst[i] = src[i];
 }
 
 }  // namespace rtd
@@ -331,10 +354,10 @@ hipError_t launch_relmse(const void *img, const double *sum, uint64_t n_frames, 
 hipError_t launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 16;
     if (n == 0) return hipSuccess;
-    // 8 workgroups of 256 lanes per CU, each lane copying 4 x 16 B per iteration
-    const unsigned blocks = (unsigned)std::min<size_t>((n + 1023) / 1024, 256u * 8u);
-    hipLaunchKernelGGL(rtd::hbm_copy_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const float4 *>(src),
-                       reinterpret_cast<float4 *>(dst), n);
+    const size_t blocks = (n + 255) / 256;
+    if (blocks > 0x7FFFFFFFu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(rtd::hbm_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                       reinterpret_cast<const float4 *>(src), reinterpret_cast<float4 *>(dst), n);
     return hipGetLastError();
 }
 
@@ -375,6 +398,15 @@ hipError_t launch_moment_history(const void *moments, void *hist, uint32_t n_px,
 
 hipError_t launch_set_frame_state(FrameState *dst, const FrameState &v, hipStream_t s) {
     hipLaunchKernelGGL(rtd::set_frame_state_kernel, dim3(1), dim3(64), 0, s, dst, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_present(const uint32_t *rgba8, uint32_t tw, uint32_t th, uint32_t *surface, uint32_t sw,
+                          uint32_t sh, uint32_t pitch_words, hipStream_t s) {
+    if (sw == 0 || sh == 0) return hipSuccess;
+    const uint32_t per_block = 64u * rtd::kPresentPx;
+    hipLaunchKernelGGL(rtd::present_kernel, dim3((sw + per_block - 1u) / per_block, (sh + 3u) / 4u), dim3(256), 0, s,
+                       rgba8, tw, th, surface, sw, sh, pitch_words, rt_present_scale(tw, sw), rt_present_scale(th, sh));
     return hipGetLastError();
 }
 

@@ -105,6 +105,8 @@ struct rt_renderer {
     uint32_t photon_slots = 0;
     rt_photon *d_photons = nullptr, *d_sorted = nullptr;
     uint32_t *d_keys = nullptr;     // grid cell key per photon slot, written by the photon trace
+    uint32_t *d_surface = nullptr;  // bgra8 surface of rt_read_present (allocated at the first use of a larger size)
+    size_t surface_words = 0;
     float *d_gather_rec = nullptr;  // 3 float4 per sorted photon (radius-gather record)
     uint32_t *d_meta = nullptr;
     void *d_temp = nullptr;
@@ -207,7 +209,7 @@ struct rt_renderer {
                         d_frame, d_ping,  d_moments, d_gbuf,    d_rgba8,  d_photons,   d_sorted, d_meta,
                         d_temp,  d_nvalid, d_counters, d_regen, d_acc64, d_gather_rec, d_bframe, d_bmom, d_bgb,
                         d_views, d_tiles, d_counter, d_w4nodes, d_wave, d_relmse, d_display, d_albedo, d_mom_hist, d_fstate,
-                        d_tex[0], d_tex[1], d_tex[2], d_tex[3], d_keys};
+                        d_tex[0], d_tex[1], d_tex[2], d_tex[3], d_keys, d_surface};
         for (void *b : bufs)
             if (b) hipFree(b);
         if (stream) hipStreamDestroy(stream);
@@ -1305,6 +1307,52 @@ int rt_read_output_rgba8(rt_renderer *r, uint8_t *rgba, size_t n) {
         if (!r || !rgba) throw ApiError(RT_ERR_INVALID, "null argument");
         if (n < r->n_px() * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than width*height*4 bytes");
         read_dev(r, rgba, r->d_rgba8, r->n_px() * 4);
+    });
+}
+
+// Largest surface side the present pass accepts (pixel centres stay exact in float32).
+static constexpr uint32_t kMaxSurfaceSide = 32768;
+
+static void present_into(rt_renderer *r, uint32_t *d_surface, uint32_t sw, uint32_t sh, size_t pitch_words) {
+    hipEvent_t ev;
+    r->begin(RT_PASS_PRESENT, ev);
+    hip_check(rtk::launch_present(r->d_rgba8, r->cfg.width, r->cfg.height, d_surface, sw, sh, (uint32_t)pitch_words,
+                                  r->stream),
+              "present");
+    r->end(RT_PASS_PRESENT, ev);
+}
+
+static void check_surface(uint32_t sw, uint32_t sh) {
+    if (sw == 0 || sh == 0 || sw > kMaxSurfaceSide || sh > kMaxSurfaceSide)
+        throw ApiError(RT_ERR_INVALID, "surface size must be 1..32768 in each dimension");
+}
+
+int rt_present(rt_renderer *r, void *d_surface, uint32_t surface_w, uint32_t surface_h, size_t row_pitch) {
+    return api_call([&] {
+        if (!r || !d_surface) throw ApiError(RT_ERR_INVALID, "null argument");
+        check_surface(surface_w, surface_h);
+        if (row_pitch % 4 != 0 || row_pitch < (size_t)surface_w * 4 || row_pitch / 4 > 0xFFFFFFFFu)
+            throw ApiError(RT_ERR_INVALID, "row_pitch must be a multiple of 4 and at least surface_w*4 bytes");
+        present_into(r, static_cast<uint32_t *>(d_surface), surface_w, surface_h, row_pitch / 4);
+    });
+}
+
+int rt_read_present(rt_renderer *r, uint32_t surface_w, uint32_t surface_h, uint8_t *bgra, size_t n_bytes) {
+    return api_call([&] {
+        if (!r || !bgra) throw ApiError(RT_ERR_INVALID, "null argument");
+        check_surface(surface_w, surface_h);
+        const size_t words = (size_t)surface_w * surface_h;
+        if (n_bytes < words * 4) throw ApiError(RT_ERR_INVALID, "buffer smaller than surface_w*surface_h*4 bytes");
+        if (words > r->surface_words) {
+            hip_check(hipStreamSynchronize(r->stream), "sync");
+            if (r->d_surface) hip_check(hipFree(r->d_surface), "free surface");
+            r->d_surface = nullptr;
+            r->surface_words = 0;
+            r->d_surface = dev_alloc<uint32_t>(words, "surface");
+            r->surface_words = words;
+        }
+        present_into(r, r->d_surface, surface_w, surface_h, surface_w);
+        read_dev(r, bgra, r->d_surface, words * 4);
     });
 }
 

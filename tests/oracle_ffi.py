@@ -60,6 +60,7 @@ def lib():
         L.or_composite.argtypes = [vp, vp, u32, u32, C.c_float]
         L.or_tonemap.argtypes = [vp, vp, C.POINTER(ToneMapUniform), u32, u32, vp]
         L.or_tonemap_cpu.argtypes = [vp, C.c_uint64, vp]
+        L.or_present.argtypes = [vp, u32, u32, vp, u32, u32]
         L.or_project_marker.argtypes = [vp, vp, vp, C.c_float, C.c_float, u32, u32, vp, C.c_float, vp]
         L.or_photon_estimate.argtypes = [C.POINTER(OrPhotonMap), vp, vp, C.c_float, vp, C.POINTER(u32),
                                          C.POINTER(C.c_uint64)]
@@ -78,7 +79,7 @@ def lib():
         L.or_math_eval.argtypes = [C.c_int, vp, vp, vp, u32]
         for fn in (L.or_trace_closest, L.or_trace_brute, L.or_trace_brute_watertight, L.or_watertight_t, L.or_gather, L.or_photon_trace, L.or_photon_grid,
                    L.or_denoise_pass, L.or_composite, L.or_tonemap, L.or_tonemap_cpu, L.or_project_marker,
-                   L.or_photon_estimate):
+                   L.or_photon_estimate, L.or_present):
             fn.restype = C.c_int
         _lib = L
     return _lib
@@ -249,6 +250,14 @@ def tonemap_cpu(accum):
     a = np.ascontiguousarray(accum, np.float32)
     out = np.zeros(a.shape[:-1] + (4,), np.uint8)
     assert lib().or_tonemap_cpu(_p(a), a.size // 4, _p(out)) == 0
+    return out
+
+
+def present(rgba8, sw: int, sh: int):
+    """The oracle's present pass: (th, tw, 4) rgba8 -> (sh, sw, 4) bgra8."""
+    t = np.ascontiguousarray(rgba8, np.uint8)
+    out = np.zeros((sh, sw, 4), np.uint8)
+    assert lib().or_present(_p(t), t.shape[1], t.shape[0], _p(out), sw, sh) == 0
     return out
 
 

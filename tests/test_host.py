@@ -182,3 +182,12 @@ def test_python_render_headless_parsers_follow_rust_grammar():
     assert np.isnan(parse_f32("NaN"))
     for s in ("", ".", "e5", "1e", "0x1p3", " 1", "1_0", "1,5", "--1", "infx"):
         assert parse_f32(s) is None, s
+
+
+def test_present_rejects_missing_renderer_without_a_device():
+    """The present entry points check their arguments before any GPU call."""
+    lib = _ffi.lib()
+    buf = np.zeros(64, np.uint8)
+    assert lib.rt_present(None, None, 4, 4, 16) != 0
+    assert lib.rt_read_present(None, 4, 4, buf.ctypes.data, buf.size) != 0
+    assert b"null" in lib.rt_last_error()

@@ -130,3 +130,91 @@ def test_cpu_tonemap_truncates():
     v = np.where(acc[0, :, 0] == -1.0, 0, v)  # -1 / 0 = -inf, clamped to 0
     assert list(out[0, :, 0]) == [int(x) for x in v]
     assert (out[..., 3] == 255).all()
+
+
+# --- present pass (fullscreen.wgsl through the ClampToEdge linear quad sampler) ---
+
+def _rgba(th, tw, seed):
+    rng = np.random.default_rng(seed)
+    img = rng.integers(0, 256, (th, tw, 4), dtype=np.uint8)
+    img[..., 3] = 255
+    return img
+
+
+def _bgra_of(rgba):
+    return rgba[..., [2, 1, 0, 3]]
+
+
+@pytest.mark.parametrize("th,tw", [(1, 1), (23, 37), (48, 64)])
+def test_present_same_size_is_channel_swap(th, tw):
+    """At the render size every fragment centre is a texel centre: the pass
+    copies the output with r and b exchanged (Bgra8Unorm surface)."""
+    img = _rgba(th, tw, 3)
+    assert np.array_equal(orc.present(img, tw, th), _bgra_of(img))
+
+
+def test_present_doubling_weights():
+    """Twice the size: surface pixel 2k+1 lies a quarter texel right of texel
+    k, 2k a quarter texel left, so the weights are 3/4 and 1/4 per axis (9, 3,
+    3, 1 sixteenths, exact in 8-bit weights), with the edge texel repeated past
+    the border; the unorm8 result is the sum rounded half up."""
+    th, tw = 9, 13
+    img = _rgba(th, tw, 5).astype(np.int64)
+    out = orc.present(img.astype(np.uint8), 2 * tw, 2 * th).astype(np.int64)
+
+    def taps(n, k):  # surface index -> (i0, i1, w0, w1) in quarters
+        t = k // 2
+        return (max(t - 1, 0), t, 1, 3) if k % 2 == 0 else (t, min(t + 1, n - 1), 3, 1)
+
+    for y in range(2 * th):
+        y0, y1, wy0, wy1 = taps(th, y)
+        for x in range(2 * tw):
+            x0, x1, wx0, wx1 = taps(tw, x)
+            s = (wy0 * (wx0 * img[y0, x0] + wx1 * img[y0, x1]) + wy1 * (wx0 * img[y1, x0] + wx1 * img[y1, x1]))
+            assert np.array_equal(out[y, x][[2, 1, 0, 3]], (s + 8) // 16), (y, x)
+
+
+def _sample_points(n_s, n_t):
+    """Texel positions of the surface pixels' centres, (p + 0.5) * n_t / n_s - 0.5,
+    in float32 as the pass computes them, and the same with the fraction
+    rounded to 1/256 (8-bit sub-texel weights)."""
+    scale = np.float32(n_t) / np.float32(n_s)
+    x = (np.arange(n_s, dtype=np.float32) + np.float32(0.5)) * scale - np.float32(0.5)
+    f = np.floor(x)
+    return x.astype(np.float64), f.astype(np.float64) + np.rint((x - f) * np.float32(256)).astype(np.float64) / 256
+
+
+@pytest.mark.parametrize("tw,th,sw,sh", [(64, 48, 37, 29), (40, 30, 100, 31), (17, 9, 17, 40), (64, 48, 1, 1),
+                                         (33, 21, 333, 7)])
+def test_present_matches_textbook_bilinear(tw, th, sw, sh):
+    """Against scipy's linear interpolation (order 1, edge samples repeated:
+    mode 'nearest'): at the 8-bit sample points every channel is the float64
+    value rounded half up (except within 1e-6 of a tie, where float64 itself
+    may round either way); at the exact sample points it is within one unit."""
+    from scipy.ndimage import map_coordinates
+
+    img = _rgba(th, tw, 11)
+    out = orc.present(img, sw, sh)
+    (xs, xq), (ys, yq) = _sample_points(sw, tw), _sample_points(sh, th)
+    for c, dst in ((0, 2), (1, 1), (2, 0), (3, 3)):
+        chan = img[..., c].astype(np.float64)
+        got = out[..., dst].astype(np.float64)
+        ref = map_coordinates(chan, np.meshgrid(yq, xq, indexing="ij"), order=1, mode="nearest")
+        want = np.floor(ref + 0.5)
+        tie = np.abs(ref - np.floor(ref) - 0.5) < 1e-6
+        assert np.all((got == want) | (tie & (np.abs(got - want) <= 1))), c
+        exact = map_coordinates(chan, np.meshgrid(ys, xs, indexing="ij"), order=1, mode="nearest")
+        assert np.all(np.abs(got - exact) <= 1.0), c
+
+
+def test_present_constant_and_ramp():
+    """A constant output presents as the same constant at any size; a
+    horizontal ramp stays monotone when enlarged."""
+    img = np.full((7, 5, 4), (200, 100, 50, 255), np.uint8)
+    out = orc.present(img, 23, 3)
+    assert np.all(out == np.array([50, 100, 200, 255], np.uint8))
+    ramp = np.zeros((2, 16, 4), np.uint8)
+    ramp[..., 0] = (np.arange(16) * 17)[None, :]
+    ramp[..., 3] = 255
+    big = orc.present(ramp, 61, 5)[..., 2].astype(int)
+    assert np.all(np.diff(big, axis=1) >= 0) and big[0, 0] == 0 and big[0, -1] == 255
