@@ -92,13 +92,15 @@ __device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4
 // offset yoff.
 //
 // Work items are (tile, output-channel block): a tile is 32 pixels of
-// kConvRows consecutive rows, a block CB = 32 * NCO output channels. A
-// workgroup of four wavefronts computes one item at a time, each wavefront
-// kConvRowsPerWave rows for all CB channels; the weights are the MFMA's A
-// operand (row = output channel) and the pixels its B operand (column =
-// pixel), so every weight fragment feeds kConvRowsPerWave MFMAs and each lane
-// ends with 4 consecutive output channels of one pixel per accumulator group
-// (8-byte epilogue stores).
+// kConvRows = 4 * RPW consecutive rows, a block CB = 32 * NCO output channels.
+// A workgroup of four wavefronts computes one item at a time, each wavefront
+// RPW rows (2 or 4) for all CB channels; the weights are the MFMA's A operand
+// (row = output channel) and the pixels its B operand (column = pixel), so
+// every weight fragment feeds RPW MFMAs and each lane ends with 4 consecutive
+// output channels of one pixel per accumulator group (8-byte epilogue
+// stores). Per 16-channel group a wavefront reads its RPW + 2 input rows at
+// the three column shifts once (3 * (RPW + 2) fragments serve the 9 * RPW
+// products).
 //
 // The reduction over 9 taps x CIN runs in CIN / KC slices of KC input
 // channels, always in the same order (16-channel groups in increasing order,
@@ -121,15 +123,14 @@ __device__ __forceinline__ void st16(bf16 *p, uint4 v) { *reinterpret_cast<uint4
 // from X and [c1, CIN) from X2 (half resolution, stride x2s) at (y/2, x/2), as
 // upsample_kernel writes them into the concatenation. The staged values are
 // bit-identical to those kernels' outputs.
-constexpr int kConvRowsPerWave = 2;
-constexpr int kConvRows = 4 * kConvRowsPerWave;
 enum ConvSrc { SRC_PLAIN = 0, SRC_POOL = 1, SRC_CONCAT = 2 };
 
-constexpr size_t conv_lds_bytes(int kc, int nco) {
-    return ((size_t)(kConvRows + 2) * 34 * (kc + 8) + (size_t)9 * 32 * nco * (kc + 8)) * 2;
+// rpw rows per wavefront, 4 * rpw rows per tile
+constexpr size_t conv_lds_bytes(int kc, int nco, int rpw) {
+    return ((size_t)(4 * rpw + 2) * 34 * (kc + 8) + (size_t)9 * 32 * nco * (kc + 8)) * 2;
 }
 
-template <int CIN, int KC, int NCO, int SRC, bool WONCE>
+template <int CIN, int KC, int NCO, int SRC, bool WONCE, int RPW>
 __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X, int xs,
                                                       const bf16 *__restrict__ Wt, int cout_total,
                                                       const float *__restrict__ bias, bf16 *__restrict__ Y, int ys,
@@ -138,7 +139,8 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
                                                       int N) {
     static_assert(CIN % KC == 0 && KC % 16 == 0, "input channels in slices of a multiple of 16");
     static_assert(!WONCE || CIN == KC, "weights staged once: one slice");
-    constexpr int R = kConvRowsPerWave;
+    constexpr int R = RPW;
+    constexpr int kConvRows = 4 * RPW;
     constexpr int NS = CIN / KC;
     constexpr int PS = KC + 8;  // LDS pixel and weight-row stride (elements)
     constexpr int GROUPS = KC / 8;
@@ -264,6 +266,7 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
         const int bx = t % tiles_x, rest = t / tiles_x;
         const int x0 = bx * 32, y0 = (rest % tiles_y) * kConvRows, n = rest / tiles_y;
         const int c = l % 32, kg = (l / 32) * 8;
+        const bf16 *trow = tile + (wv * R * 34 + c) * PS + kg;
         if (s == 0) {
 #pragma unroll
             for (int r = 0; r < R; r++)
@@ -275,20 +278,25 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
         // the same order and gives the same bits
 #pragma unroll
         for (int c0 = 0; c0 < KC; c0 += 16) {
+            // the wavefront's R + 2 input rows at the three column shifts, read
+            // once for all nine taps (tap (dy, dx) of row r is row r + dy, shift dx)
+            bf16x8 b[R + 2][3];
+#pragma unroll
+            for (int rr = 0; rr < R + 2; rr++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) b[rr][dx] = load8(trow + (rr * 34 + dx) * PS + c0);
 #pragma unroll
             for (int tap = 0; tap < 9; tap++) {
                 const bf16 *wp = wl + (tap * CB + c) * PS + kg + c0;
-                bf16x8 a[NCO], b[R];
+                bf16x8 a[NCO];
 #pragma unroll
                 for (int j = 0; j < NCO; j++) a[j] = load8(wp + j * 32 * PS);
 #pragma unroll
                 for (int r = 0; r < R; r++)
-                    b[r] = load8(tile + ((wv * R + r + tap / 3) * 34 + c + tap % 3) * PS + kg + c0);
-#pragma unroll
-                for (int r = 0; r < R; r++)
 #pragma unroll
                     for (int j = 0; j < NCO; j++)
-                        acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[r], acc[r][j], 0, 0, 0);
+                        acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], b[r + tap / 3][tap % 3],
+                                                                            acc[r][j], 0, 0, 0);
             }
         }
         if (s != NS - 1) continue;
@@ -336,76 +344,123 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
 }
 
 // Weight gradient of a 3x3 convolution: gW[tap][co][ci] = sum over pixels of
-// G[p][co] * X[p + tap][ci]. Each workgroup takes one (tap, 32 co, 32 ci) block
-// and the 32-pixel row segments blockIdx.x, blockIdx.x + gridDim.x, ...; G and
-// X segments are staged transposed in LDS ([channel][pixel]) so MFMA fragments
-// are 16-byte reads. The workgroup's sum goes to its own partial array
-// part[blockIdx.x][tap][co][ci] (no atomics): reduce_partials_kernel adds the
-// partials in a fixed order, so the gradient does not depend on scheduling.
-// Grid: (segment groups, 9, cout/32 * ceil(cin/32)).
-__global__ __launch_bounds__(64) void conv3x3_wgrad_kernel(const bf16 *__restrict__ G, int gs, int cout,
-                                                           const bf16 *__restrict__ X, int xs, int cin,
-                                                           float *__restrict__ part, int N, int H, int W) {
-    __shared__ __attribute__((aligned(16))) bf16 gt[32][32];  // [co][pixel]
-    __shared__ __attribute__((aligned(16))) bf16 xt[32][32];  // [ci][pixel]
-    const int l = threadIdx.x;
-    const int tap = blockIdx.y;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+// G[p][co] * X[p + tap][ci], as an MFMA GEMM with M = 32 output channels, N =
+// 32 input channels and K = pixels. A workgroup of three wavefronts takes one
+// (32 co, 32 ci) block (blockIdx.x, which varies fastest, so the workgroups of
+// one tile range run together and share its loads in L2) and all nine taps,
+// over the tiles blockIdx.y, blockIdx.y + gridDim.y, ... of 32 pixels x
+// kWgRows rows. Per
+// tile, G and the (kWgRows + 2) x 34 input window are staged transposed in LDS
+// ([channel][pixel]); the window is staged once per column shift dx, so every
+// MFMA fragment is an aligned 16-byte read. A tile's global loads are all
+// issued before any of them is written to LDS, and the next tile's loads are
+// issued before this tile's MFMAs. Wavefront w accumulates the taps
+// of row dy = w (3w .. 3w + 2), each G fragment feeding its three MFMAs. The
+// workgroup's sum goes to its own partial array part[blockIdx.y][tap][co][ci]
+// (no atomics): reduce_partials_kernel adds the partials in a fixed order, so
+// the gradient does not depend on scheduling.
+constexpr int kWgRows = 4;                      // rows per tile
+constexpr int kWgK = 32 * kWgRows;              // pixels per tile
+constexpr int kWgGS = kWgK + 8;                 // LDS stride of one G channel (elements)
+constexpr int kWgXS = (kWgRows + 2) * 32 + 8;   // LDS stride of one input channel in one dx copy
+
+__global__ __launch_bounds__(192) void conv3x3_wgrad_kernel(const bf16 *__restrict__ G, int gs, int cout,
+                                                            const bf16 *__restrict__ X, int xs, int cin,
+                                                            float *__restrict__ part, int N, int H, int W) {
+    __shared__ __attribute__((aligned(16))) bf16 gt[32 * kWgGS];      // [co][row * 32 + x]
+    __shared__ __attribute__((aligned(16))) bf16 xt[3][32 * kWgXS];   // [dx][ci][row * 32 + x], row 0 = y0 - 1
+    const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nci = (cin + 31) / 32;
-    const int co0 = (blockIdx.z / nci) * 32, ci0 = (blockIdx.z % nci) * 32;
-    const int segs_x = (W + 31) / 32;
-    const int total = N * H * segs_x;
-    const int lp = l / 2, lh = (l % 2) * 16;  // staging: pixel, 16-channel half
-    floatx16 acc = {};
-    for (int s = blockIdx.x; s < total; s += gridDim.x) {
-        const int sx = s % segs_x, ny = s / segs_x;
-        const int y = ny % H, n = ny / H;
-        const int x = sx * 32 + lp;
-        {
-            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
-            if (x < W) {
-                const bf16 *gp = G + ((size_t)(n * H + y) * W + x) * gs + co0 + lh;
-                v0 = *reinterpret_cast<const uint4 *>(gp);
-                v1 = *reinterpret_cast<const uint4 *>(gp + 8);
-            }
-            const bf16 *e0 = reinterpret_cast<const bf16 *>(&v0), *e1 = reinterpret_cast<const bf16 *>(&v1);
+    const int co0 = (blockIdx.x / nci) * 32, ci0 = (blockIdx.x % nci) * 32;
+    const int tiles_x = (W + 31) / 32, tiles_y = (H + kWgRows - 1) / kWgRows;
+    const int total = N * tiles_y * tiles_x;
+    // a tile's loads go to registers first (all in flight together), then to
+    // LDS; the next tile's loads are issued before this tile's MFMAs
+    constexpr int GITEMS = kWgK * 4, XITEMS = (kWgRows + 2) * 34 * 4;
+    constexpr int GPER = (GITEMS + 191) / 192, XPER = (XITEMS + 191) / 192;
+    uint4 rg[GPER], rx[XPER];
+    auto fetch = [&](int t) {
+        const int tx = t % tiles_x, rest = t / tiles_x;
+        const int x0 = tx * 32, y0 = (rest % tiles_y) * kWgRows, n = rest / tiles_y;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                gt[lh + j][lp] = e0[j];
-                gt[lh + 8 + j][lp] = e1[j];
+        for (int k = 0; k < GPER; k++) {
+            const int i = threadIdx.x + k * 192;
+            const int px = i >> 2, g = i & 3;
+            const int yy = y0 + (px >> 5), xx = x0 + (px & 31);
+            rg[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < GITEMS && yy < H && xx < W) rg[k] = ld16(G + ((size_t)(n * H + yy) * W + xx) * gs + co0 + g * 8);
+        }
+        // the input window: rows y0 - 1 .. y0 + kWgRows, columns x0 - 1 .. x0 + 32
+#pragma unroll
+        for (int k = 0; k < XPER; k++) {
+            const int i = threadIdx.x + k * 192;
+            const int g = i & 3, q = i >> 2;
+            const int c = q % 34, r = q / 34;
+            const int yy = y0 - 1 + r, xx = x0 - 1 + c, ch = ci0 + g * 8;
+            rx[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < XITEMS && yy >= 0 && yy < H && xx >= 0 && xx < W && ch < cin)
+                rx[k] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + ch);
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int k = 0; k < GPER; k++) {
+            const int i = threadIdx.x + k * 192;
+            if (i < GITEMS) {
+                const int px = i >> 2, g = i & 3;
+                const bf16 *e = reinterpret_cast<const bf16 *>(&rg[k]);
+#pragma unroll
+                for (int j = 0; j < 8; j++) gt[(g * 8 + j) * kWgGS + px] = e[j];
             }
         }
-        {
-            const int yy = y + dy, xx = x + dx;
-            uint4 v0 = make_uint4(0u, 0u, 0u, 0u), v1 = v0;
-            if (x < W && yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                const bf16 *xp = X + ((size_t)(n * H + yy) * W + xx) * xs;
-                if (ci0 + lh < cin) v0 = *reinterpret_cast<const uint4 *>(xp + ci0 + lh);
-                if (ci0 + lh + 8 < cin) v1 = *reinterpret_cast<const uint4 *>(xp + ci0 + lh + 8);
-            }
-            const bf16 *e0 = reinterpret_cast<const bf16 *>(&v0), *e1 = reinterpret_cast<const bf16 *>(&v1);
+        // copy dx holds window column xi + dx at position xi = 0 .. 31
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                xt[lh + j][lp] = e0[j];
-                xt[lh + 8 + j][lp] = e1[j];
+        for (int k = 0; k < XPER; k++) {
+            const int i = threadIdx.x + k * 192;
+            if (i < XITEMS) {
+                const int g = i & 3, q = i >> 2;
+                const int c = q % 34, r = q / 34;
+                const bf16 *e = reinterpret_cast<const bf16 *>(&rx[k]);
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) {
+                    const int xi = c - dx;
+                    if (xi >= 0 && xi < 32) {
+#pragma unroll
+                        for (int j = 0; j < 8; j++) xt[dx][(g * 8 + j) * kWgXS + r * 32 + xi] = e[j];
+                    }
+                }
             }
         }
+    };
+    floatx16 acc[3] = {};
+    if ((int)blockIdx.y < total) fetch(blockIdx.y);
+    for (int t = blockIdx.y; t < total; t += gridDim.y) {
+        __syncthreads();  // every wavefront is done with the previous tile
+        commit();
         __syncthreads();
+        if (t + (int)gridDim.y < total) fetch(t + gridDim.y);
+        // K steps of 16 pixels in pixel order; a lane's 8 pixels lie in one row
+#pragma unroll 2
+        for (int ks = 0; ks < kWgK / 16; ks++) {
+            const int k = ks * 16 + (l >> 5) * 8;
+            const bf16x8 a = load8(&gt[(l & 31) * kWgGS + k]);
+            const bf16 *xb = &xt[0][(l & 31) * kWgXS + ((k >> 5) + wv) * 32 + (k & 31)];
 #pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-            const bf16x8 a = load8(&gt[l % 32][ks * 16 + (l / 32) * 8]);
-            const bf16x8 b = load8(&xt[l % 32][ks * 16 + (l / 32) * 8]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            for (int dx = 0; dx < 3; dx++)
+                acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, load8(xb + dx * 32 * kWgXS), acc[dx], 0, 0, 0);
         }
-        __syncthreads();
     }
-    const int ci = ci0 + l % 32;
+    const int ci = ci0 + (l & 31);
     if (ci >= cin) return;
-    float *pg = part + (size_t)blockIdx.x * 9 * cout * cin;
+    float *pg = part + (size_t)blockIdx.y * 9 * cout * cin;
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        const int co = co0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
-        pg[((size_t)tap * cout + co) * cin + ci] = acc[e];
+    for (int dx = 0; dx < 3; dx++) {
+        const int tap = wv * 3 + dx;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int co = co0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
+            pg[((size_t)tap * cout + co) * cin + ci] = acc[dx][e];
+        }
     }
 }
 
@@ -858,11 +913,11 @@ struct Acts {
 };
 
 // Partial arrays of a layer's weight gradient: enough workgroups to fill the
-// GPU (about 2048 of one wavefront), at most one per row segment, at most 256.
+// GPU (about 1024 of three wavefronts, four per CU), at most one per tile.
 int wgrad_parts(const ConvLayer &L, int N, int H, int W) {
     const int nblk = (L.cout / 32) * ((L.cin + 31) / 32);
-    const int segs = N * H * ((W + 31) / 32);
-    return std::max(1, std::min({2048 / (9 * nblk), segs, 256}));
+    const int tiles = N * ((H + rtu::kWgRows - 1) / rtu::kWgRows) * ((W + 31) / 32);
+    return std::max(1, std::min({1024 / nblk, tiles, 1024}));
 }
 constexpr int kBiasParts = 256;
 constexpr int kOutBlocks = 1024;
@@ -897,26 +952,27 @@ typedef void (*ConvLaunch)(const ConvArgs &, hipStream_t);
 // cin, cout, src, N, H, W, mask, bias
 typedef std::array<int, 8> ConvKey;
 
-template <int CIN, int KC, int NCO, int SRC, bool WONCE>
+template <int CIN, int KC, int NCO, int SRC, bool WONCE, int RPW>
 void conv_launch(const ConvArgs &a, hipStream_t s) {
-    constexpr size_t lds = rtu::conv_lds_bytes(KC, NCO);
+    constexpr size_t lds = rtu::conv_lds_bytes(KC, NCO, RPW);
     static_assert(lds <= 160 * 1024, "LDS of one workgroup");
     // once per instance (a function-local static, initialised once even
     // across threads): raise the dynamic LDS limit (above 64 KB) and size
     // the persistent grid
     static const int resident = [] {
-        const void *fn = (const void *)rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>;
+        const void *fn = (const void *)rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE, RPW>;
         if (lds > 65536)
             hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "conv LDS size");
         int per_cu = 0;
-        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>,
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE, RPW>,
                                                                256, lds),
                   "conv occupancy");
         return std::max(per_cu, 1) * cu_count();
     }();
-    const int items = ((a.W + 31) / 32) * ((a.H + rtu::kConvRows - 1) / rtu::kConvRows) * a.N * (a.cout / (32 * NCO));
+    constexpr int rows = 4 * RPW;
+    const int items = ((a.W + 31) / 32) * ((a.H + rows - 1) / rows) * a.N * (a.cout / (32 * NCO));
     const dim3 grid((unsigned)std::max(1, std::min(items, resident)));
-    hipLaunchKernelGGL((rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE>), grid, dim3(256), lds, s, a.X, a.xs, a.w, a.cout,
+    hipLaunchKernelGGL((rtu::conv3x3_kernel<CIN, KC, NCO, SRC, WONCE, RPW>), grid, dim3(256), lds, s, a.X, a.xs, a.w, a.cout,
                        a.b, a.Y, a.ys, a.yoff, a.relu, a.mask, a.ms, a.H, a.W, a.X2, a.x2s, a.c1, a.N);
 }
 
@@ -926,9 +982,11 @@ void conv_launch(const ConvArgs &a, hipStream_t s) {
 struct ConvVariant {
     int cin, kc, nco, src;
     bool wonce;
+    int rpw;
     ConvLaunch fn;
 };
-#define RT_CV(CI, KC, NCO, SRC, ONCE) {CI, KC, NCO, rtu::SRC, ONCE, conv_launch<CI, KC, NCO, rtu::SRC, ONCE>}
+#define RT_CV(CI, KC, NCO, SRC, ONCE) {CI, KC, NCO, rtu::SRC, ONCE, 2, conv_launch<CI, KC, NCO, rtu::SRC, ONCE, 2>}
+#define RT_CV4(CI, KC, NCO, SRC, ONCE) {CI, KC, NCO, rtu::SRC, ONCE, 4, conv_launch<CI, KC, NCO, rtu::SRC, ONCE, 4>}
 const ConvVariant kConvVariants[] = {
     RT_CV(16, 16, 1, SRC_PLAIN, true),
     RT_CV(32, 32, 1, SRC_PLAIN, true),    RT_CV(32, 16, 1, SRC_PLAIN, false),
@@ -946,8 +1004,15 @@ const ConvVariant kConvVariants[] = {
     RT_CV(192, 32, 2, SRC_CONCAT, false), RT_CV(192, 16, 2, SRC_CONCAT, false), RT_CV(192, 32, 1, SRC_CONCAT, false),
     RT_CV(256, 32, 4, SRC_PLAIN, false),  RT_CV(256, 16, 4, SRC_PLAIN, false), RT_CV(256, 32, 2, SRC_PLAIN, false),
     RT_CV(256, 32, 4, SRC_CONCAT, false), RT_CV(256, 16, 4, SRC_CONCAT, false), RT_CV(256, 32, 2, SRC_CONCAT, false),
+    // sixteen-row tiles (four rows per wavefront)
+    RT_CV4(16, 16, 1, SRC_PLAIN, true),   RT_CV4(32, 32, 1, SRC_PLAIN, true),   RT_CV4(32, 16, 2, SRC_PLAIN, false),
+    RT_CV4(32, 16, 1, SRC_POOL, false),   RT_CV4(64, 16, 2, SRC_PLAIN, false),  RT_CV4(64, 32, 1, SRC_PLAIN, false),
+    RT_CV4(96, 16, 1, SRC_CONCAT, false), RT_CV4(96, 32, 1, SRC_CONCAT, false), RT_CV4(96, 16, 1, SRC_PLAIN, false),
+    RT_CV4(128, 16, 2, SRC_PLAIN, false), RT_CV4(128, 32, 1, SRC_PLAIN, false), RT_CV4(192, 16, 2, SRC_CONCAT, false),
+    RT_CV4(192, 16, 2, SRC_PLAIN, false), RT_CV4(256, 16, 2, SRC_CONCAT, false), RT_CV4(256, 16, 2, SRC_PLAIN, false),
 };
 #undef RT_CV
+#undef RT_CV4
 
 }  // namespace
 
@@ -1091,8 +1156,8 @@ struct rt_unet {
         const int parts = wgrad_parts(L, N, H, W);
         if ((size_t)parts * L.nw() > a.part_floats || (size_t)kBiasParts * L.cout > a.part_floats)
             throw ApiError(RT_ERR_INVALID, "gradient partials larger than their buffer");
-        const dim3 grid((unsigned)parts, 9, (L.cout / 32) * ((L.cin + 31) / 32));
-        hipLaunchKernelGGL(rtu::conv3x3_wgrad_kernel, grid, dim3(64), 0, stream, G, L.cout, L.cout, X, xs, L.cin,
+        const dim3 grid((L.cout / 32) * ((L.cin + 31) / 32), (unsigned)parts);
+        hipLaunchKernelGGL(rtu::conv3x3_wgrad_kernel, grid, dim3(192), 0, stream, G, L.cout, L.cout, X, xs, L.cin,
                            a.part, N, H, W);
         hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(blocks_for(L.nw(), 2048)), dim3(256), 0, stream, a.part,
                            parts, L.nw(), L.nw(), d_g + w_off[k]);
@@ -1456,7 +1521,7 @@ int rt_unet_set_variant_mode(rt_unet *u, int mode) {
 }
 
 // The convolution variant chosen for each shape so far, one line per shape:
-// "cin cout src N H W mask bias -> kc nco wonce".
+// "cin cout src N H W mask bias -> kc nco wonce rows_per_wavefront".
 int rt_unet_conv_choices(const rt_unet *u, char *buf, size_t cap, size_t *needed) {
     return rt::api_call([&] {
         if (!u) throw ApiError(RT_ERR_INVALID, "null argument");
@@ -1465,8 +1530,8 @@ int rt_unet_conv_choices(const rt_unet *u, char *buf, size_t cap, size_t *needed
             const ConvKey &k = kv.first;
             const ConvVariant &v = kConvVariants[kv.second];
             char line[160];
-            std::snprintf(line, sizeof(line), "%d %d %d %d %d %d %d %d -> %d %d %d\n", k[0], k[1], k[2], k[3], k[4],
-                          k[5], k[6], k[7], v.kc, v.nco, v.wonce ? 1 : 0);
+            std::snprintf(line, sizeof(line), "%d %d %d %d %d %d %d %d -> %d %d %d %d\n", k[0], k[1], k[2], k[3],
+                          k[4], k[5], k[6], k[7], v.kc, v.nco, v.wonce ? 1 : 0, v.rpw);
             out += line;
         }
         if (needed) *needed = out.size() + 1;
