@@ -358,15 +358,26 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(const bf16 *__restrict__ X
 // of row dy = w (3w .. 3w + 2), each G fragment feeding its three MFMAs. The
 // workgroup's sum goes to its own partial array part[blockIdx.y][tap][co][ci]
 // (no atomics): reduce_partials_kernel adds the partials in a fixed order, so
-// the gradient does not depend on scheduling.
+// the gradient does not depend on scheduling. The workgroups of input-channel
+// block 0 also write the bias gradient's partials bpart[blockIdx.y][wavefront][co]:
+// G summed over their pixels by one more MFMA per K step, against ones.
 constexpr int kWgRows = 4;                      // rows per tile
 constexpr int kWgK = 32 * kWgRows;              // pixels per tile
 constexpr int kWgGS = kWgK + 8;                 // LDS stride of one G channel (elements)
 constexpr int kWgXS = (kWgRows + 2) * 32 + 8;   // LDS stride of one input channel in one dx copy
 
+// Four bf16 of one channel (16-bit word j of the four 8-channel vectors a..d)
+// as one 8-byte LDS word.
+__device__ __forceinline__ uint2 pick4(const uint4 &a, const uint4 &b, const uint4 &c, const uint4 &d, int j) {
+    const uint32_t wa = (&a.x)[j >> 1], wb = (&b.x)[j >> 1], wc = (&c.x)[j >> 1], wd = (&d.x)[j >> 1];
+    if (j & 1) return make_uint2((wa >> 16) | (wb & 0xFFFF0000u), (wc >> 16) | (wd & 0xFFFF0000u));
+    return make_uint2((wa & 0xFFFFu) | (wb << 16), (wc & 0xFFFFu) | (wd << 16));
+}
+
 __global__ __launch_bounds__(192) void conv3x3_wgrad_kernel(const bf16 *__restrict__ G, int gs, int cout,
                                                             const bf16 *__restrict__ X, int xs, int cin,
-                                                            float *__restrict__ part, int N, int H, int W) {
+                                                            float *__restrict__ part, float *__restrict__ bpart,
+                                                            int N, int H, int W) {
     __shared__ __attribute__((aligned(16))) bf16 gt[32 * kWgGS];      // [co][row * 32 + x]
     __shared__ __attribute__((aligned(16))) bf16 xt[3][32 * kWgXS];   // [dx][ci][row * 32 + x], row 0 = y0 - 1
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -374,65 +385,60 @@ __global__ __launch_bounds__(192) void conv3x3_wgrad_kernel(const bf16 *__restri
     const int co0 = (blockIdx.x / nci) * 32, ci0 = (blockIdx.x % nci) * 32;
     const int tiles_x = (W + 31) / 32, tiles_y = (H + kWgRows - 1) / kWgRows;
     const int total = N * tiles_y * tiles_x;
-    // a tile's loads go to registers first (all in flight together), then to
-    // LDS; the next tile's loads are issued before this tile's MFMAs
-    constexpr int GITEMS = kWgK * 4, XITEMS = (kWgRows + 2) * 34 * 4;
-    constexpr int GPER = (GITEMS + 191) / 192, XPER = (XITEMS + 191) / 192;
-    uint4 rg[GPER], rx[XPER];
+    // Staging items (one per thread): 8 channels x 4 consecutive pixels of one
+    // row. G: rows 0 .. kWgRows - 1 (threads below 32 * kWgRows); the window:
+    // rows 0 .. kWgRows + 1, where a thread loads window columns 4m .. 4m + 5
+    // and writes columns 4m + dx .. 4m + dx + 3 to copy dx. A tile's loads are
+    // all issued before any LDS write, and the next tile's before the MFMAs.
+    static_assert((kWgRows + 2) * 32 == 192, "one window item per thread");
+    const int sg = threadIdx.x & 3, sm = (threadIdx.x >> 2) & 7, sr = threadIdx.x >> 5;
+    const bool g_item = sr < kWgRows;
+    uint4 rg[4], rx[6];
     auto fetch = [&](int t) {
         const int tx = t % tiles_x, rest = t / tiles_x;
         const int x0 = tx * 32, y0 = (rest % tiles_y) * kWgRows, n = rest / tiles_y;
+        {
+            const int yy = y0 + sr;
 #pragma unroll
-        for (int k = 0; k < GPER; k++) {
-            const int i = threadIdx.x + k * 192;
-            const int px = i >> 2, g = i & 3;
-            const int yy = y0 + (px >> 5), xx = x0 + (px & 31);
-            rg[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (i < GITEMS && yy < H && xx < W) rg[k] = ld16(G + ((size_t)(n * H + yy) * W + xx) * gs + co0 + g * 8);
+            for (int p = 0; p < 4; p++) {
+                const int xx = x0 + 4 * sm + p;
+                rg[p] = make_uint4(0u, 0u, 0u, 0u);
+                if (g_item && yy < H && xx < W)
+                    rg[p] = ld16(G + ((size_t)(n * H + yy) * W + xx) * gs + co0 + sg * 8);
+            }
         }
-        // the input window: rows y0 - 1 .. y0 + kWgRows, columns x0 - 1 .. x0 + 32
+        {
+            const int yy = y0 - 1 + sr, ch = ci0 + sg * 8;
 #pragma unroll
-        for (int k = 0; k < XPER; k++) {
-            const int i = threadIdx.x + k * 192;
-            const int g = i & 3, q = i >> 2;
-            const int c = q % 34, r = q / 34;
-            const int yy = y0 - 1 + r, xx = x0 - 1 + c, ch = ci0 + g * 8;
-            rx[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (i < XITEMS && yy >= 0 && yy < H && xx >= 0 && xx < W && ch < cin)
-                rx[k] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + ch);
+            for (int p = 0; p < 6; p++) {
+                const int xx = x0 - 1 + 4 * sm + p;
+                rx[p] = make_uint4(0u, 0u, 0u, 0u);
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W && ch < cin)
+                    rx[p] = ld16(X + ((size_t)(n * H + yy) * W + xx) * xs + ch);
+            }
         }
     };
     auto commit = [&]() {
+        if (g_item) {
 #pragma unroll
-        for (int k = 0; k < GPER; k++) {
-            const int i = threadIdx.x + k * 192;
-            if (i < GITEMS) {
-                const int px = i >> 2, g = i & 3;
-                const bf16 *e = reinterpret_cast<const bf16 *>(&rg[k]);
-#pragma unroll
-                for (int j = 0; j < 8; j++) gt[(g * 8 + j) * kWgGS + px] = e[j];
-            }
+            for (int j = 0; j < 8; j++)
+                *reinterpret_cast<uint2 *>(&gt[(sg * 8 + j) * kWgGS + sr * 32 + 4 * sm]) =
+                    pick4(rg[0], rg[1], rg[2], rg[3], j);
         }
-        // copy dx holds window column xi + dx at position xi = 0 .. 31
 #pragma unroll
-        for (int k = 0; k < XPER; k++) {
-            const int i = threadIdx.x + k * 192;
-            if (i < XITEMS) {
-                const int g = i & 3, q = i >> 2;
-                const int c = q % 34, r = q / 34;
-                const bf16 *e = reinterpret_cast<const bf16 *>(&rx[k]);
+        for (int dx = 0; dx < 3; dx++)
 #pragma unroll
-                for (int dx = 0; dx < 3; dx++) {
-                    const int xi = c - dx;
-                    if (xi >= 0 && xi < 32) {
-#pragma unroll
-                        for (int j = 0; j < 8; j++) xt[dx][(g * 8 + j) * kWgXS + r * 32 + xi] = e[j];
-                    }
-                }
-            }
-        }
+            for (int j = 0; j < 8; j++)
+                *reinterpret_cast<uint2 *>(&xt[dx][(sg * 8 + j) * kWgXS + sr * 32 + 4 * sm]) =
+                    pick4(rx[dx], rx[dx + 1], rx[dx + 2], rx[dx + 3], j);
     };
     floatx16 acc[3] = {};
+    // bias gradient: the workgroups of input-channel block 0 also sum G over
+    // their pixels, one more MFMA against ones per K step, taken by wavefront
+    // ks % 3 (each wavefront keeps its own partial)
+    const bool bias_wg = blockIdx.x % nci == 0;
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, make_uint4(0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u));
+    floatx16 accb = {};
     if ((int)blockIdx.y < total) fetch(blockIdx.y);
     for (int t = blockIdx.y; t < total; t += gridDim.y) {
         __syncthreads();  // every wavefront is done with the previous tile
@@ -448,7 +454,15 @@ __global__ __launch_bounds__(192) void conv3x3_wgrad_kernel(const bf16 *__restri
 #pragma unroll
             for (int dx = 0; dx < 3; dx++)
                 acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, load8(xb + dx * 32 * kWgXS), acc[dx], 0, 0, 0);
+            if (bias_wg && ks % 3 == wv) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, ones, accb, 0, 0, 0);
         }
+    }
+    // every column of accb holds the sum over the pixels for output channels
+    // (e / 4) * 8 + (l / 32) * 4 + e % 4; column 0 (lanes 0 and 32) stores it
+    if (bias_wg && (l & 31) == 0) {
+        float *pb = bpart + ((size_t)blockIdx.y * 3 + wv) * cout + co0;
+#pragma unroll
+        for (int e = 0; e < 16; e++) pb[(e / 4) * 8 + (l / 32) * 4 + e % 4] = accb[e];
     }
     const int ci = ci0 + (l & 31);
     if (ci >= cin) return;
@@ -461,34 +475,6 @@ __global__ __launch_bounds__(192) void conv3x3_wgrad_kernel(const bf16 *__restri
             const int co = co0 + (e / 4) * 8 + (l / 32) * 4 + e % 4;
             pg[((size_t)tap * cout + co) * cin + ci] = acc[dx][e];
         }
-    }
-}
-
-// Bias gradient partials: part[blockIdx.x][c] = sum over this block's pixels
-// of G[p][c]. Thread t takes channel group t % (c/8) (16-byte loads) for pixels
-// t / (c/8) + k * lanes, grid-strided; the block then adds its lanes in a fixed
-// order. c is a multiple of 8 and at most 256.
-__global__ __launch_bounds__(256) void bias_grad_kernel(const bf16 *__restrict__ G, int gs, int c, size_t n_px,
-                                                        float *__restrict__ part) {
-    __shared__ float red[256 * 8];
-    const int groups = c / 8, lanes = 256 / groups;
-    const int g = threadIdx.x % groups, lane = threadIdx.x / groups;
-    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (lane < lanes)
-        for (size_t p = (size_t)blockIdx.x * lanes + lane; p < n_px; p += (size_t)gridDim.x * lanes) {
-            float v[8];
-            unpack8(*reinterpret_cast<const uint4 *>(G + p * gs + g * 8), v);
-#pragma unroll
-            for (int i = 0; i < 8; i++) acc[i] += v[i];
-        }
-#pragma unroll
-    for (int i = 0; i < 8; i++) red[threadIdx.x * 8 + i] = acc[i];
-    __syncthreads();
-    if ((int)threadIdx.x < c) {
-        const int gg = threadIdx.x / 8, i = threadIdx.x % 8;
-        float s = 0.0f;
-        for (int k = 0; k < lanes; k++) s += red[(k * groups + gg) * 8 + i];
-        part[(size_t)blockIdx.x * c + threadIdx.x] = s;
     }
 }
 
@@ -509,6 +495,27 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float *__res
         }
         for (; g < n_part; g++) s += part[(size_t)g * stride + i];
         out[i] = s;
+    }
+}
+
+// First level of a two-level fixed-order sum: out2[c * n + i] = sum over g in
+// [c * chunk, min((c + 1) * chunk, n_part)), in that order, of part[g * stride
+// + i]; grid.y = c. reduce_partials_kernel then adds the chunks in order.
+__global__ __launch_bounds__(256) void reduce_chunks_kernel(const float *__restrict__ part, int n_part, size_t stride,
+                                                            size_t n, int chunk, float *__restrict__ out2) {
+    const int g0 = (int)blockIdx.y * chunk, g1 = min(g0 + chunk, n_part);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        float s = 0.0f;
+        int g = g0;
+        for (; g + 8 <= g1; g += 8) {
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = part[(size_t)(g + k) * stride + i];
+#pragma unroll
+            for (int k = 0; k < 8; k++) s += v[k];
+        }
+        for (; g < g1; g++) s += part[(size_t)g * stride + i];
+        out2[(size_t)blockIdx.y * n + i] = s;
     }
 }
 
@@ -813,6 +820,26 @@ T *dalloc(size_t count, const char *what) {
 
 unsigned blocks_for(size_t n, unsigned cap = 8192) { return (unsigned)std::min<size_t>((n + 255) / 256, cap); }
 
+// Fixed-order sum of n_part partial arrays (part[g * stride + i], i < n) into
+// out: one pass up to kRedChunk * 2 partials, else chunks of kRedChunk summed
+// into scratch (reduce_scratch_floats(n_part, n) floats), then the chunks.
+constexpr int kRedChunk = 32;
+size_t reduce_scratch_floats(int n_part, size_t n) {
+    return n_part <= 2 * kRedChunk ? 0 : (size_t)((n_part + kRedChunk - 1) / kRedChunk) * n;
+}
+void reduce_fixed(const float *part, int n_part, size_t stride, size_t n, float *out, float *scratch, hipStream_t s) {
+    if (n_part <= 2 * kRedChunk) {
+        hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(blocks_for(n, 2048)), dim3(256), 0, s, part, n_part,
+                           stride, n, out);
+        return;
+    }
+    const int chunks = (n_part + kRedChunk - 1) / kRedChunk;
+    hipLaunchKernelGGL(rtu::reduce_chunks_kernel, dim3(blocks_for(n, 256), chunks), dim3(256), 0, s, part, n_part,
+                       stride, n, kRedChunk, scratch);
+    hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(blocks_for(n, 2048)), dim3(256), 0, s, scratch, chunks, n, n,
+                       out);
+}
+
 // compute units of the current device
 int cu_count() {
     int dev = 0, cus = 0;
@@ -919,17 +946,17 @@ int wgrad_parts(const ConvLayer &L, int N, int H, int W) {
     const int tiles = N * ((H + rtu::kWgRows - 1) / rtu::kWgRows) * ((W + 31) / 32);
     return std::max(1, std::min({1024 / nblk, tiles, 1024}));
 }
-constexpr int kBiasParts = 256;
 constexpr int kOutBlocks = 1024;
 
 // Gradient partial floats a training pass of this shape needs.
 size_t part_floats_for(int N, int H, int W) {
-    size_t need = (size_t)kOutBlocks * rtu::kOutParts;
+    size_t need = (size_t)kOutBlocks * rtu::kOutParts + reduce_scratch_floats(kOutBlocks, 3 * rtu::kOutC + 3);
     for (int k = 0; k < kNumLayers; k++) {
         const int l = k < 2 * kLevels ? k / 2 : (kLevels - 2) - (k - 2 * kLevels) / 2;
         const int parts = wgrad_parts(kLayers[k], N, H >> l, W >> l);
-        need = std::max(need, (size_t)parts * kLayers[k].nw());
-        need = std::max(need, (size_t)kBiasParts * kLayers[k].cout);
+        const ConvLayer &L = kLayers[k];
+        need = std::max(need, (size_t)parts * (L.nw() + 3 * L.cout) +
+                                  std::max(reduce_scratch_floats(parts, L.nw()), reduce_scratch_floats(3 * parts, L.cout)));
     }
     return need;
 }
@@ -1154,17 +1181,19 @@ struct rt_unet {
     void wgrad(int k, const rtu::bf16 *G, const rtu::bf16 *X, int xs, Acts &a, int N, int H, int W) {
         const ConvLayer &L = kLayers[k];
         const int parts = wgrad_parts(L, N, H, W);
-        if ((size_t)parts * L.nw() > a.part_floats || (size_t)kBiasParts * L.cout > a.part_floats)
+        // weight partials [parts][nw], then bias partials [parts][3 wavefronts][cout],
+        // then the reductions' scratch
+        float *bpart = a.part + (size_t)parts * L.nw();
+        float *scratch = bpart + (size_t)parts * 3 * L.cout;
+        if ((size_t)parts * (L.nw() + 3 * L.cout) +
+                std::max(reduce_scratch_floats(parts, L.nw()), reduce_scratch_floats(3 * parts, L.cout)) >
+            a.part_floats)
             throw ApiError(RT_ERR_INVALID, "gradient partials larger than their buffer");
         const dim3 grid((L.cout / 32) * ((L.cin + 31) / 32), (unsigned)parts);
         hipLaunchKernelGGL(rtu::conv3x3_wgrad_kernel, grid, dim3(192), 0, stream, G, L.cout, L.cout, X, xs, L.cin,
-                           a.part, N, H, W);
-        hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(blocks_for(L.nw(), 2048)), dim3(256), 0, stream, a.part,
-                           parts, L.nw(), L.nw(), d_g + w_off[k]);
-        hipLaunchKernelGGL(rtu::bias_grad_kernel, dim3(kBiasParts), dim3(256), 0, stream, G, L.cout, L.cout,
-                           (size_t)N * H * W, a.part);
-        hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(1), dim3(256), 0, stream, a.part, kBiasParts,
-                           (size_t)L.cout, (size_t)L.cout, d_g + b_off[k]);
+                           a.part, bpart, N, H, W);
+        reduce_fixed(a.part, parts, L.nw(), L.nw(), d_g + w_off[k], scratch, stream);
+        reduce_fixed(bpart, 3 * parts, (size_t)L.cout, (size_t)L.cout, d_g + b_off[k], scratch, stream);
     }
 
     void pool(const rtu::bf16 *X, int xs, int c, rtu::bf16 *P, int N, int H, int W) {  // H, W: input size
@@ -1216,8 +1245,8 @@ struct rt_unet {
             hipLaunchKernelGGL(rtu::out_layer_kernel<true>, dim3(kOutBlocks), dim3(256), 0, stream, a.V[0], a.noisy,
                                d_p + ow_off, d_p + ob_off, a.pred, a.target, a.px(), scale, a.gV[0], a.part);
             // output-layer gradients (w then b, contiguous) and the loss from the block partials
-            hipLaunchKernelGGL(rtu::reduce_partials_kernel, dim3(1), dim3(256), 0, stream, a.part, kOutBlocks,
-                               (size_t)rtu::kOutParts, (size_t)(3 * rtu::kOutC + 3), d_g + ow_off);
+            reduce_fixed(a.part, kOutBlocks, (size_t)rtu::kOutParts, (size_t)(3 * rtu::kOutC + 3), d_g + ow_off,
+                         a.part + (size_t)kOutBlocks * rtu::kOutParts, stream);
             hipLaunchKernelGGL(rtu::loss_kernel, dim3(1), dim3(64), 0, stream, a.part, kOutBlocks,
                                1.0 / (3.0 * (double)a.px()), loss_slot);
         } else {
