@@ -495,75 +495,91 @@ __device__ bool bounce_light(const GatherArgs &A, const Env &env, const SurfaceS
         const int32_t bz = wgsl_i32((hp.z - gp.room_min_z) / gp.cell_size);
         const int32_t gxmax = (int32_t)gp.grid_x - 1, gymax = (int32_t)gp.grid_y - 1,
                       gzmax = (int32_t)gp.grid_z - 1;
-        // Same visiting order, budget and stop rules as the shader's loops; the
-        // loads are grouped to shorten the chain of dependent memory reads: the
-        // three cell records of a row together, then the gather records of up
-        // to four candidates together. Sums are added in candidate order.
-        for (int32_t dz = -1; dz <= 1 && !done; dz++) {
-            for (int32_t dy = -1; dy <= 1 && !done; dy++) {
-                const int32_t cy = iclamp((int32_t)((uint32_t)by + (uint32_t)dy), 0, gymax);
-                const int32_t cz = iclamp((int32_t)((uint32_t)bz + (uint32_t)dz), 0, gzmax);
-                const uint32_t row = (uint32_t)cy * gp.grid_x + (uint32_t)cz * gp.grid_x * gp.grid_y;
-                uint2 metas[3];
+        // Same visiting order, budget and stop rules as the shader's loops: rows
+        // (dz, dy) in order, each row's three cells (one contiguous run away
+        // from the x borders: grid_meta is an exclusive scan), candidates in
+        // order. Each lane walks its own cursor over that sequence and takes its
+        // next batch of up to kPhotonBatch candidates every step, so the
+        // wavefront runs as many steps as its longest walk rather than, row by
+        // row, as many as that row's longest. The next row's cell records are
+        // loaded while the current row is walked; a batch's gather records are
+        // loaded together. Sums are added in candidate order.
+        const bool interior = bx >= 1 && bx <= gxmax - 1;
+        auto load_row = [&](uint32_t r, uint2 m[3]) {
+            const int32_t dz = (int32_t)(r / 3u) - 1, dy = (int32_t)(r % 3u) - 1;
+            const int32_t cy = iclamp((int32_t)((uint32_t)by + (uint32_t)dy), 0, gymax);
+            const int32_t cz = iclamp((int32_t)((uint32_t)bz + (uint32_t)dz), 0, gzmax);
+            const uint32_t row = (uint32_t)cy * gp.grid_x + (uint32_t)cz * gp.grid_x * gp.grid_y;
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const int32_t cx = iclamp((int32_t)((uint32_t)bx + (uint32_t)(k - 1)), 0, gxmax);
-                    metas[k] = A.grid_meta[(uint32_t)cx + row];
+            for (int k = 0; k < 3; k++) {
+                const int32_t cx = iclamp((int32_t)((uint32_t)bx + (uint32_t)(k - 1)), 0, gxmax);
+                m[k] = A.grid_meta[(uint32_t)cx + row];
+            }
+        };
+        uint2 nxt[3], rows[3];
+        load_row(0u, nxt);
+        uint32_t next_row = 0u;  // the row whose records nxt holds
+        int n_ranges = 0, k_range = 0;
+        uint32_t cur = 0u, left = 0u;
+        while (!done) {
+            // the next non-empty range of the walk, opening rows as needed
+            while (left == 0u) {
+                if (k_range < n_ranges) {
+                    cur = rows[k_range].x;
+                    left = rows[k_range].y;
+                    k_range++;
+                    continue;
                 }
-                // Away from the x borders the three cells are consecutive and
-                // grid_meta is an exclusive scan, so their photons form one
-                // contiguous run in the same order: walk it as one range.
-                int n_ranges = 3;
-                if (bx >= 1 && bx <= gxmax - 1) {
-                    metas[0].y = metas[0].y + metas[1].y + metas[2].y;
-                    n_ranges = 1;
+                if (next_row == 9u) break;
+#pragma unroll
+                for (int k = 0; k < 3; k++) rows[k] = nxt[k];
+                if (interior) rows[0].y = rows[0].y + rows[1].y + rows[2].y;
+                n_ranges = interior ? 1 : 3;
+                k_range = 0;
+                next_row++;
+                if (next_row < 9u) load_row(next_row, nxt);
+            }
+            if (left == 0u) break;  // all 27 cells walked
+            if (COUNT) count_step(cnt, kPhasePhotons);
+            // one round trip per batch: position, power and incident direction
+            // of each candidate come in one 48-byte record
+            const uint32_t nb = min(left, kPhotonBatch);
+            float4 g0[kPhotonBatch], g1[kPhotonBatch];
+            float g2[kPhotonBatch];
+#pragma unroll
+            for (uint32_t j = 0; j < kPhotonBatch; j++) {
+                if (j < nb) {
+                    const float4 *gr = A.gather_rec + 3u * (cur + j);
+                    g0[j] = gr[0];
+                    g1[j] = gr[1];
+                    g2[j] = gr[2].x;
+                } else {
+                    g0[j] = g1[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    g2[j] = 0.0f;
                 }
+            }
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    if (k >= n_ranges) break;
-                    const uint2 meta = metas[k];
-                    for (uint32_t i = 0u; i < meta.y && !done; i += kPhotonBatch) {
-                        if (COUNT) count_step(cnt, kPhasePhotons);
-                        // one round trip per batch: position, power and incident
-                        // direction of each candidate come in one 48-byte record
-                        const uint32_t nb = min(meta.y - i, kPhotonBatch);
-                        float4 g0[kPhotonBatch], g1[kPhotonBatch];
-                        float g2[kPhotonBatch];
-#pragma unroll
-                        for (uint32_t j = 0; j < kPhotonBatch; j++) {
-                            if (j < nb) {
-                                const float4 *gr = A.gather_rec + 3u * (meta.x + i + j);
-                                g0[j] = gr[0];
-                                g1[j] = gr[1];
-                                g2[j] = gr[2].x;
-                            } else {
-                                g0[j] = g1[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                                g2[j] = 0.0f;
-                            }
-                        }
-#pragma unroll
-                        for (uint32_t j = 0; j < kPhotonBatch; j++) {
-                            if (j < nb && !done) {
-                                if (budget == 0u) {
-                                    done = true;
-                                } else {
-                                    budget -= 1u;
-                                    if (COUNT) cnt.add(kCntPhotons, 1);
-                                    const F3 d = xyz(g0[j]) - hp;
-                                    const float d2 = dot(d, d);
-                                    if (d2 < r2) {
-                                        const F3 pw = f3(g0[j].w, g1[j].x, g1[j].y);
-                                        const F3 inc = f3(g1[j].z, g1[j].w, g2[j]);
-                                        ph_sum = ph_sum + pw * wmax(dot(shn, -inc), 0.0f);
-                                        ph_hits += 1u;
-                                        if (ph_hits >= 64u) done = true;
-                                    }
-                                }
-                            }
+            for (uint32_t j = 0; j < kPhotonBatch; j++) {
+                if (j < nb && !done) {
+                    if (budget == 0u) {
+                        done = true;
+                    } else {
+                        budget -= 1u;
+                        if (COUNT) cnt.add(kCntPhotons, 1);
+                        const F3 d = xyz(g0[j]) - hp;
+                        const float d2 = dot(d, d);
+                        if (d2 < r2) {
+                            const F3 pw = f3(g0[j].w, g1[j].x, g1[j].y);
+                            const F3 inc = f3(g1[j].z, g1[j].w, g2[j]);
+                            ph_sum = ph_sum + pw * wmax(dot(shn, -inc), 0.0f);
+                            ph_hits += 1u;
+                            if (ph_hits >= 64u) done = true;
                         }
                     }
                 }
             }
+            cur += nb;
+            left -= nb;
         }
         if (ph_hits > 0u)
             acc.add(thr * dscale * ph_sum * ip.photon_scale /
