@@ -251,6 +251,31 @@ def test_full_pipeline_bit_exact(denoise, accum, passes):
     assert np.array_equal(rgba, rrgba), first_diff(rgba, rrgba)
 
 
+@pytest.mark.parametrize("flags", [_ffi.FLAG_GRAPH, _ffi.FLAG_GRAPH | _ffi.FLAG_TILE_LAUNCH])
+def test_frame_graph_pipeline_matches_oracle(flags):
+    """Frames replayed from captured hipGraphs against the oracle (not only
+    against direct launches): gather -> a-trous -> composite -> tonemap over
+    six frames issued one at a time, the graphs captured at the first frame of
+    each output set and replayed after."""
+    scene = rt.Scene.builtin("cornell_box", 4)
+    cam = rt.Scene.builtin_camera("cornell_box", 4)
+    w, h = 48, 40
+    print(f"frame graphs against the oracle flags={flags}", flush=True)
+    with _renderer(scene, width=w, height=h, spp=2, enable_denoise=True, accumulate_frames=6, flags=flags) as r:
+        r.update_camera(cam)
+        for _ in range(6):
+            r.record_frame()
+        acc, rgba = r.read_accumulation(), r.read_output_rgba8()
+        cfg = r.config
+        captures = ctypes.c_uint32()
+        assert _ffi.lib().rt_frame_graph_captures(r._h, ctypes.byref(captures)) == 0
+    assert captures.value > 0
+    o = orc.OracleScene.from_scene(scene)
+    racc, rrgba, *_ = oracle_frames(o, cfg, rt.camera_to_uniform(cam, w, h), 6)
+    _check_equal("accumulation", acc, racc)
+    assert np.array_equal(rgba, rrgba), first_diff(rgba, rrgba)
+
+
 def test_fullframe_1080p_properties():
     """The 1M-triangle 1080p frame at full size: every pixel finite, moments
     consistent (E[l^2] >= E[l]^2 up to the clamps), G-buffer normals unit length."""
@@ -270,6 +295,38 @@ def test_fullframe_1080p_properties():
     n = np.linalg.norm(g[..., :3][hit], axis=-1)
     assert np.abs(n - 1).max() < 1e-5
     assert (out[..., 3] == 255).all()
+
+
+def test_fullframe_1080p_photon_map_bit_exact():
+    """The 1080p workload at its full shape with the photon map on (1M photons,
+    depth 8, 1M triangles): the photon records, the grid and the sorted
+    photons, then the gather's frame colour, moments and G-buffer, bit for bit
+    against the oracle on the same inputs."""
+    scene = rt.Scene.builtin("fullframe", 1)
+    cam = rt.Scene.builtin_camera("fullframe", 1)
+    n = 1_000_000
+    print("fullframe 1080p with 1M photons against the oracle", flush=True)
+    with _renderer(scene, width=1920, height=1080, spp=1, max_depth=8, enable_denoise=False,
+                   enable_photon_map=True, photon_count=n) as r:
+        r.update_camera(cam)
+        r.rebuild_photons()
+        photons, meta, srt, gp = r.read_photons(), r.read_grid_meta(), r.read_sorted_photons(), r.grid_params()
+        r.record_frame()
+        got = (r.read_frame(), r.read_moments(), r.read_gbuffer())
+        cfg = r.config
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = orc.OracleScene.from_scene(scene)
+    ref_ph = o.photon_trace(gp, (n // 256) * 256, 8, n_slots=gp.photon_slots, threads=threads)
+    _check_equal("photons", photons.view(np.float32), ref_ph.view(np.float32))
+    n_proc = min(max(n * 8 // 256, 1) * 256, gp.photon_slots)
+    ref_meta, ref_sorted = orc.photon_grid(ref_ph, n_proc, gp, len(meta))
+    assert np.array_equal(meta, ref_meta), first_diff(meta, ref_meta)
+    _check_equal("sorted photons", srt.view(np.float32), ref_sorted.view(np.float32))
+    assert len(srt) > 100_000
+    ref = o.gather(rt.camera_to_uniform(cam, 1920, 1080), image_params(cfg, 1), max_depth=8,
+                   photon_map=(gp, meta, srt), threads=threads)
+    for name, a, b in zip(("frame", "moments", "gbuffer"), got, ref[:3]):
+        _check_equal(name, a, b)
 
 
 def _view_params(cfg, w, h, spp, frame):
