@@ -329,6 +329,36 @@ def test_fullframe_1080p_photon_map_bit_exact():
         _check_equal(name, a, b)
 
 
+def test_photon_plain_720p_pipeline_bit_exact():
+    """The photon-mapped plain scene at its full shape (1280x720, 1M photons,
+    grid cell = radius = 0.05, 4 spp, depth 6, a-trous x2, 32-frame running
+    mean): the grid and sorted photons, then four frames of the whole pipeline
+    (accumulation and rgba8), bit for bit against the oracle."""
+    scene = rt.Scene.builtin("photon_plain", 1)
+    cam = rt.Scene.builtin_camera("photon_plain", 1)
+    w, h, n = 1280, 720, 1_000_000
+    print("photon_plain 720p with 1M photons against the oracle", flush=True)
+    with _renderer(scene, width=w, height=h, spp=4, max_depth=6, enable_photon_map=True, photon_count=n,
+                   grid_cell_size=0.05, enable_denoise=True, accumulate_frames=32) as r:
+        r.update_camera(cam)
+        r.rebuild_photons()
+        photons, meta, srt, gp = r.read_photons(), r.read_grid_meta(), r.read_sorted_photons(), r.grid_params()
+        r.render_frames(4)
+        acc, rgba = r.read_accumulation(), r.read_output_rgba8()
+        cfg = r.config
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = orc.OracleScene.from_scene(scene)
+    ref_ph = o.photon_trace(gp, (n // 256) * 256, 6, n_slots=gp.photon_slots, threads=threads)
+    _check_equal("photons", photons.view(np.float32), ref_ph.view(np.float32))
+    ref_meta, ref_sorted = orc.photon_grid(ref_ph, min(max(n * 6 // 256, 1) * 256, gp.photon_slots), gp, len(meta))
+    assert np.array_equal(meta, ref_meta), first_diff(meta, ref_meta)
+    _check_equal("sorted photons", srt.view(np.float32), ref_sorted.view(np.float32))
+    racc, rrgba, *_ = oracle_frames(o, cfg, rt.camera_to_uniform(cam, w, h), 4, photon_map=(gp, meta, srt),
+                                    threads=threads)
+    _check_equal("accumulation", acc, racc)
+    assert np.array_equal(rgba, rrgba), first_diff(rgba, rrgba)
+
+
 def _view_params(cfg, w, h, spp, frame):
     ip = image_params(cfg, frame)
     ip.width, ip.height, ip.spp = w, h, spp
