@@ -388,6 +388,31 @@ def test_batch_views_bit_exact(tuning):
             _check_equal(f"view {k} {name}", a, b)
 
 
+def test_batch_multiview_workload_bit_exact():
+    """The many-view workload at its full shape: the 64 seeded views of the
+    109k-triangle scene (128x128 .. 1920x1080, spp 1-16, depth 8) in one
+    persistent launch with the autotuner's pick, every view against the
+    oracle's gather for its camera, resolution, spp and seed."""
+    from rustracer_amd.workloads import multiview_views
+
+    scene = rt.Scene.builtin("multiview", 1)
+    views = multiview_views(64, seed=1)
+    print("64-view batch workload against the oracle", flush=True)
+    with _renderer(scene, width=8, height=8, max_depth=8, enable_denoise=False) as r:
+        r.set_views(views)
+        r.batch_autotune(repeats=1)
+        r.render_batch()  # with the tuning the autotuner kept
+        got = [r.read_batch_view(k) for k in range(len(views))]
+        cfg = r.config
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = orc.OracleScene.from_scene(scene)
+    for k, (cam, w, h, spp, frame) in enumerate(views):
+        ref = o.gather(rt.camera_to_uniform(cam, w, h), _view_params(cfg, w, h, spp, frame), max_depth=8,
+                       threads=threads)
+        for name, a, b in zip(("frame", "moments", "gbuffer"), got[k], ref[:3]):
+            _check_equal(f"view {k} ({w}x{h}, {spp} spp) {name}", a, b)
+
+
 def test_batch_autotune_and_totals():
     from rustracer_amd.workloads import multiview_views
 
