@@ -117,6 +117,25 @@ def test_gather_frame_bit_exact(scene_name, spp, depth, flags):
         _check_equal(name, a, b)
 
 
+def test_cornell_256_64spp_bit_exact():
+    """The Cornell box workload at its full shape: 36 triangles, 256x256, 64 spp,
+    depth 5, photon map and denoise off; frame, moments and G-buffer against
+    the oracle."""
+    scene = rt.Scene.builtin("cornell36", 1)
+    cam = rt.Scene.builtin_camera("cornell36", 1)
+    print("cornell36 256x256 64 spp against the oracle", flush=True)
+    with _renderer(scene, width=256, height=256, spp=64, max_depth=5, enable_denoise=False) as r:
+        r.update_camera(cam)
+        r.record_frame()
+        got = (r.read_frame(), r.read_moments(), r.read_gbuffer())
+        cfg = r.config
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    ref = orc.OracleScene.from_scene(scene).gather(rt.camera_to_uniform(cam, 256, 256), image_params(cfg, 1),
+                                                   max_depth=5, threads=threads)
+    for name, a, b in zip(("frame", "moments", "gbuffer"), got, ref[:3]):
+        _check_equal(name, a, b)
+
+
 @pytest.mark.parametrize("view", [1, 2, 4, 13, 17, 19, 20, 21])
 def test_debug_views_bit_exact(view):
     scene = rt.Scene.builtin("cornell_box")
