@@ -251,7 +251,7 @@ def main() -> int:
                    "width": w, "height": h, "spp": 1, "depth": args.depth, "photons": args.photons,
                    "triangles": counts["triangles"],
                    "traversal": TRAVERSALS[tinfo["mode"]], "bvh_depth": tinfo["bvh_depth"],
-                   "camera_pass": ("path queues (primary pass + compacted tail pass)" if args.path_queues else
+                   "camera_pass": ("path queues (primary pass + one compacted pass per depth)" if args.path_queues else
                                    "one workgroup per tile" if args.tile_launch else
                                    "persistent lanes taking new pixels"),
                    "photon_map": photons_on,

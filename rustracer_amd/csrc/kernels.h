@@ -58,11 +58,11 @@ struct GatherLaunch {
     uint32_t early_shadow;   // shadow rays stop at the first hit nearer than the light
     unsigned long long *counters;  // non-null: count traffic (4 counters)
     // path queues: when px_state is non-null each sample runs as a primary pass
-    // plus a tail pass over the queued paths (gather_wavefront_bytes: sizes)
+    // plus one pass per depth over the queued paths (gather_wavefront_bytes: sizes)
     rt_tex_array tex[RT_TEX_ROLES];  // texture arrays (device texels)
     void *px_state;          // 5 * n_px float4
-    void *queue[2];          // [1]: 3 * n_px float4, the paths entering depth 1; [0] unused
-    uint32_t *path_count;    // 3 counters: [0] tail work, [1] diffuse paths queued, [2] smooth paths queued
+    void *queue[2];          // 3 * n_px float4 each; queue[b & 1]: the paths entering depth b
+    uint32_t *path_count;    // 3 per depth b >= 1 at 3 (b - 1): work taken, diffuse paths queued, smooth paths queued
     // persistent form: when non-null (and px_state is null) the frame runs as
     // resident wavefronts whose lanes take pixels from 8 band counters here
     uint32_t *regen;

@@ -36,8 +36,8 @@ struct GatherArgs {
     unsigned long long *counters;
     // path-queue (wavefront) state; unused by the one-pass kernels
     float4 *px_state;       // kPixelPlanes planes of n_px float4
-    float4 *queue[2];       // [1]: kQueuePlanes planes of n_px float4 (paths entering depth 1); [0] unused
-    uint32_t *path_count;   // [0] tail work counter, [1] diffuse paths queued, [2] smooth paths queued
+    float4 *queue[2];       // kQueuePlanes planes of n_px float4 each: queue[b & 1] holds the paths entering depth b
+    uint32_t *path_count;   // per depth b >= 1 at 3 (b - 1): work taken, diffuse paths queued, smooth paths queued
     uint32_t n_px;
     uint32_t sample;        // sample index of this pass
     rt_tex_array tex[RT_TEX_ROLES];  // albedo, metallic-roughness, emissive, normal
@@ -911,20 +911,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
 
 // ─────────────────────────── path queues (wavefront) ───────────────────────────
 //
-// The same per-pixel computation split after the first bounce. A primary pass
-// runs sample s of every pixel through its first bounce; paths that go on are
-// appended to a queue (wave-aggregated atomics), and a tail pass runs each
-// queued path through its remaining bounces. The tail pass runs full
-// wavefronts of live paths, so the few long paths (glass, metal chains) no
-// longer hold a wavefront of finished pixels. Each pixel's RNG stream, its
-// accumulators and every arithmetic step are those of gather_pixel, in the
-// same order; only the place where they are kept between bounces changes
-// (registers there, HBM here). Samples run one after another, as the RNG
-// stream requires.
+// The same per-pixel computation split at every bounce. A primary pass runs
+// sample s of every pixel through its first bounce; paths that go on are
+// appended to a queue (wave-aggregated atomics). Then one pass per depth runs
+// each queued path through one bounce and compacts the survivors into the
+// other queue. Every pass runs full wavefronts of live paths, so the few long
+// paths (glass, metal chains) never hold a wavefront of finished pixels. Each
+// pixel's RNG stream, its accumulators and every arithmetic step are those of
+// gather_pixel, in the same order; only the place where they are kept between
+// bounces changes (registers there, HBM here). Samples run one after another,
+// as the RNG stream requires.
 //
-// The queue is sorted by material between the two passes, with two keys: the
-// paths whose first bounce was diffuse fill it from the front, those that left
-// a specular lobe or a dielectric fill it from the back. The tail pass takes
+// Each queue is sorted by material as it fills, with two keys: the paths
+// whose last bounce was diffuse fill it from the front, those that left a
+// specular lobe or a dielectric fill it from the back. The next pass takes
 // them in that order, so its wavefronts hold paths of one kind: the long
 // glass/metal chains run together instead of one per wavefront of short
 // diffuse paths.
@@ -935,7 +935,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RT_PERSISTEN
 // Queue entry (SoA planes): 0: ro.xyz, pixel   1: rd.xyz, rng state   2: thr.xyz, 0
 constexpr uint32_t kPixelPlanes = 5;
 constexpr uint32_t kQueuePlanes = 3;
-constexpr uint32_t kPathCounters = 3;
+constexpr uint32_t kPathCountersPerDepth = 3;
 
 __device__ __forceinline__ void store_running(const GatherArgs &A, uint32_t i, const Acc &acc, uint32_t rng) {
     A.px_state[i] = make_float4(acc.col.x, acc.col.y, acc.col.z, acc.lum_sum);
@@ -1048,13 +1048,19 @@ __global__ __launch_bounds__(64) void gather_primary_kernel(GatherArgs A) {
     }
 }
 
-// The paths that survive the primary pass, compacted: each lane runs one path
-// through all its remaining bounces (depth 1 .. max_depth - 1) and closes its
-// sample. Wavefronts take 64 paths at a time from an atomic counter
-// (path_count[0]), so the long glass/metal chains no longer keep wavefronts of
-// finished pixels resident.
+// One bounce of the queued paths at depth b (1 .. max_depth - 1). The paths
+// entering depth b are queue[b & 1] (counters path_count[3(b-1) ..]: [0] work
+// taken, [1] paths from the front, [2] paths from the back); the paths that go
+// on are compacted into queue[(b + 1) & 1] with the same two material keys
+// (diffuse from the front, specular or dielectric from the back), counters
+// path_count[3b ..]. Wavefronts take 64 paths at a time, front first, so each
+// holds paths of one kind, and the finished paths of a bounce leave no idle
+// lanes behind. A path's bounces run in the same order with the same state as
+// in the one-kernel pass (running sums in the pixel state between bounces,
+// ray, throughput and generator in the queue entry), so the bits are the same.
 template <int MODE, bool EARLY, bool COUNT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gather_tail_kernel(GatherArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gather_bounce_kernel(GatherArgs A,
+                                                                                                 uint32_t b) {
     __shared__ uint32_t stack_rows[kLdsStack * 64];
     uint32_t spill_words[kSpillAlloc];
     LaneStack stk;
@@ -1062,36 +1068,49 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ga
     __shared__ unsigned long long cnt_words[kCntWords];
     if (COUNT) count_begin(cnt_words);
     Tracer<MODE, EARLY, COUNT> tr{A.ts, stk, TraceCount{(lds_u64 *)cnt_words, 0u}};
-    const uint32_t n_front = __builtin_amdgcn_readfirstlane(A.path_count[1]);
-    const uint32_t count = n_front + __builtin_amdgcn_readfirstlane(A.path_count[2]);
-    const float4 *qin = A.queue[1];
+    uint32_t *cin = A.path_count + 3u * (b - 1u), *cout = A.path_count + 3u * b;
+    const uint32_t n_front = __builtin_amdgcn_readfirstlane(cin[1]);
+    const uint32_t count = n_front + __builtin_amdgcn_readfirstlane(cin[2]);
+    const float4 *qin = A.queue[b & 1u];
+    float4 *qout = A.queue[(b + 1u) & 1u];
     const uint32_t spp = max(1u, A.ip.spp);
+    const bool last = b + 1u >= A.max_depth;
     const Env env = find_env(A);
     for (;;) {
         uint32_t base = 0u;
-        if (threadIdx.x == 0) base = atomicAdd(&A.path_count[0], 64u);
+        if (threadIdx.x == 0) base = atomicAdd(&cin[0], 64u);
         base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
         if (base >= count) break;
         const uint32_t w = base + threadIdx.x;
+        bool push = false, smooth = false;
+        F3 ro, rd, thr;
+        Rng rng{0u};
+        uint32_t idx = 0u;
         if (w < count) {
             const uint32_t i = w < n_front ? w : A.n_px - 1u - (w - n_front);  // diffuse first, then smooth
             const float4 q0 = qin[i], q1 = qin[A.n_px + i], q2 = qin[2u * A.n_px + i];
-            F3 ro = xyz(q0);
-            const uint32_t idx = __float_as_uint(q0.w);
-            F3 rd = xyz(q1);
-            Rng rng{__float_as_uint(q1.w)};
-            F3 thr = xyz(q2);
+            ro = xyz(q0);
+            idx = __float_as_uint(q0.w);
+            rd = xyz(q1);
+            rng.state = __float_as_uint(q1.w);
+            thr = xyz(q2);
             Acc acc;
             FirstHit fh{f3(0.0f, 1.0f, 0.0f), 1e30f};  // only depth 0 touches it
             uint32_t unused;
             load_running(A, idx, acc, unused);
-            for (uint32_t b = 1u; b < A.max_depth; b++)
-                if (!bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh,
-                                                idx % A.ip.width, idx / A.ip.width))
-                    break;
-            load_samples(A, idx, acc, fh);
-            end_path(A, idx, acc, fh, rng.state, spp);
+            const bool cont = bounce<MODE, EARLY, COUNT>(A, env, tr, b, ro, rd, thr, rng, acc, fh, idx % A.ip.width,
+                                                         idx / A.ip.width, &smooth);
+            if (cont && !last) {
+                push = true;
+                store_running(A, idx, acc, 0u);
+            } else {
+                load_samples(A, idx, acc, fh);
+                end_path(A, idx, acc, fh, rng.state, spp);
+            }
         }
+        const uint32_t front = wave_append(push && !smooth, &cout[1]);
+        const uint32_t back = wave_append(push && smooth, &cout[2]);
+        if (push) queue_put(qout, A.n_px, smooth ? A.n_px - 1u - back : front, ro, rd, thr, idx, rng.state);
     }
     if (COUNT) {
         count_flush(A.counters, cnt_words);
@@ -1391,20 +1410,23 @@ static hipError_t launch_persistent(const rtd::GatherArgs &A, uint32_t max_block
 }
 
 size_t gather_wavefront_bytes(uint32_t n_px, uint32_t max_depth) {
-    (void)max_depth;
-    return (size_t)n_px * 16u * (rtd::kPixelPlanes + rtd::kQueuePlanes) + 4u * rtd::kPathCounters;
+    return (size_t)n_px * 16u * (rtd::kPixelPlanes + 2u * rtd::kQueuePlanes) +
+           4u * rtd::kPathCountersPerDepth * ((size_t)std::max(max_depth, 1u) + 1u);
 }
 
 template <int M, bool E, bool C>
 static void launch_wavefront_passes(rtd::GatherArgs A, hipStream_t s) {
     const uint32_t spp = A.ip.spp > 0u ? A.ip.spp : 1u;
-    const uint32_t tail_wgs = std::min<uint32_t>((A.n_px + 63u) / 64u, 2048u);
+    // four one-wavefront workgroups per SIMD of 256 CUs, as many as are resident
+    const uint32_t bounce_wgs = std::min<uint32_t>((A.n_px + 63u) / 64u, 4096u);
+    const uint32_t n_counters = rtd::kPathCountersPerDepth * (std::max(A.max_depth, 1u) + 1u);
     for (uint32_t k = 0; k < spp; k++) {
         A.sample = k;
-        (void)launch_fill_u32(A.path_count, 0u, rtd::kPathCounters, s);
+        (void)launch_fill_u32(A.path_count, 0u, n_counters, s);
         hipLaunchKernelGGL((rtd::gather_primary_kernel<M, E, C>), dim3(A.n_tiles), dim3(64), 0, s, A);
-        if (A.max_depth > 1u)
-            hipLaunchKernelGGL((rtd::gather_tail_kernel<M, E, C>), dim3(tail_wgs), dim3(64), 0, s, A);
+        // one launch per depth; each reads its queue's size from device memory
+        for (uint32_t b = 1u; b < A.max_depth; b++)
+            hipLaunchKernelGGL((rtd::gather_bounce_kernel<M, E, C>), dim3(bounce_wgs), dim3(64), 0, s, A, b);
     }
 }
 

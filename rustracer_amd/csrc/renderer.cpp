@@ -323,9 +323,9 @@ struct rt_renderer {
             char *w = static_cast<char *>(d_wave);
             const size_t plane = n_px() * 16u;
             g.px_state = w;
-            g.queue[0] = nullptr;
-            g.queue[1] = w + 5u * plane;
-            g.path_count = reinterpret_cast<uint32_t *>(w + 8u * plane);
+            g.queue[1] = w + 5u * plane;  // paths entering odd depths
+            g.queue[0] = w + 8u * plane;  // paths entering even depths
+            g.path_count = reinterpret_cast<uint32_t *>(w + 11u * plane);
         }
         // measured on the 1080p workload: all resident workgroups, refill as
         // soon as one lane is free (thresholds 8..32 and 3/4 or 1/2 of the
@@ -364,7 +364,7 @@ struct rt_renderer {
         hip_check(hipMemset(d_mom_hist, 0, n_px() * 8), "clear moment history");
     }
 
-    // Renderer::record_frame, lib.rs:666-851 (the present pass is not part of this backend)
+    // Renderer::record_frame, lib.rs:666-851, up to the present pass (rt_present)
     void render_frame() {
         ensure_mom_hist();
         sample_counter = sample_counter + 1u;

@@ -53,14 +53,17 @@ def test_small_and_ragged_images(w, h, flags):
         _equal(name, a, b)
 
 
+@pytest.mark.parametrize("flags", [0, _ffi.FLAG_PATH_QUEUES])
 @pytest.mark.parametrize("depth", [1, 2, 64])
-def test_depth_extremes(depth):
+def test_depth_extremes(depth, flags):
+    """Depth 1 (primary only), 2 and 64 (with path queues: one pass per depth,
+    63 of them, most finding their queue empty)."""
     scene = rt.Scene.builtin("photon_plain", 2)
     cam = rt.Scene.builtin_camera("photon_plain", 2)
     w, h = 40, 24
-    print(f"max_depth={depth}", flush=True)
-    with rt.Renderer(rt.RenderConfig(width=w, height=h, spp=2, max_depth=depth, enable_denoise=False), scene,
-                     0) as r:
+    print(f"max_depth={depth} flags={flags}", flush=True)
+    with rt.Renderer(rt.RenderConfig(width=w, height=h, spp=2, max_depth=depth, enable_denoise=False, flags=flags),
+                     scene, 0) as r:
         r.update_camera(cam)
         r.render_frames(1)
         got = (r.read_frame(), r.read_moments(), r.read_gbuffer())
