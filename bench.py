@@ -45,6 +45,9 @@ RAY_BYTES = 32         # 24 B ray + 8 B hit
 PHOTON_BYTES = 36      # per radius-gather candidate: position + power (16), power + incident (16), incident.z (4)
 PIXEL_OUT_BYTES = 48   # frame + moments + gbuffer float4 writes
 PRESENT_REPEATS = 5
+# the learned denoiser's parameters, trained on the 2000-scene training workload by
+# `python -m rustracer_amd.denoiser --scenes 2000 --steps 2000 --out <this file>`
+CONFIG4_WEIGHTS = "rustracer_amd/weights/unet_config4.f32"
 DENOISE_PX_BYTES = 64  # per pass: color in + gbuffer + moments + out, 16 B each
 COMPOSITE_TONEMAP_PX_BYTES = 52  # frame 16 + accumulation 16 read + 16 written + rgba8 4
 GRID_SLOT_BYTES = 32   # count (position + power.r, 16) + scatter key read (16) per slot
@@ -103,6 +106,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-view batch measurement")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 720p photon-map workload and the network training throughput")
+    ap.add_argument("--convergence-train", action="store_true",
+                    help="train a small network inside the run (64 scenes, 800 steps) for the convergence study "
+                         "instead of using the shipped weights trained on the 2000-scene workload")
     ap.add_argument("--no-convergence", action="store_true",
                     help="skip the ms-to-relMSE study (about two minutes: a 16384-frame FP64 reference, a dataset "
                          "and a training run)")
@@ -388,7 +394,7 @@ def main() -> int:
         if not args.no_network:
             result["network_training"] = network_training()
     if not args.no_convergence:
-        result["convergence"] = convergence()
+        result["convergence"] = convergence(args.convergence_train)
     print(json.dumps(result), flush=True)
     return 0
 
@@ -525,14 +531,18 @@ def network_roofline(r, w: int, h: int) -> dict:
                                                    "(five levels, 32-128 channels), pools, output layer, tonemap"}
 
 
-def convergence() -> dict:
+def convergence(train_in_run: bool = False) -> dict:
     """ms to relMSE <= 0.01 (learned U-Net vs a-trous) on this workload, measured
     now by `python -m rustracer_amd.convergence` in a child process: a
-    16384-frame FP64 reference, the network's dataset and training run, then
-    each method's frames until the displayed image reaches relMSE 0.01."""
+    16384-frame FP64 reference, then each method's frames until the displayed
+    image reaches relMSE 0.01, re-run back to back and timed. The network is the
+    shipped one trained on the 2000-scene training workload (CONFIG4_WEIGHTS),
+    or with train_in_run a small one trained inside the run (64 scenes, 800 steps)."""
     import subprocess
-    out = subprocess.run([sys.executable, "-m", "rustracer_amd.convergence"], cwd=ROOT, check=True,
-                         capture_output=True, text=True).stdout
+    cmd = [sys.executable, "-m", "rustracer_amd.convergence"]
+    if not train_in_run and os.path.exists(os.path.join(ROOT, CONFIG4_WEIGHTS)):
+        cmd += ["--weights", CONFIG4_WEIGHTS]
+    out = subprocess.run(cmd, cwd=ROOT, check=True, capture_output=True, text=True).stdout
     d = json.loads(out.strip().splitlines()[-1])
     methods = {m["method"]: m for m in d["methods"]}
     res = {"metric": "ms to relMSE<=0.01 (eps 1e-2) vs FP64 16384-frame reference", "source": "measured in this run",

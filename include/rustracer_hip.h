@@ -457,6 +457,7 @@ int rt_unet_create(int device, uint64_t seed, rt_unet **out);
 int rt_unet_destroy(rt_unet *u);
 int rt_unet_param_count(const rt_unet *u, size_t *n);
 int rt_unet_get_params(rt_unet *u, float *out, size_t n);
+/* New parameters also restart the optimizer (Adam's moments and step count). */
 int rt_unet_set_params(rt_unet *u, const float *in, size_t n);
 /* Training set of `capacity` images of width x height (multiples of 32). */
 int rt_unet_dataset_reset(rt_unet *u, uint32_t width, uint32_t height, uint32_t capacity);

@@ -111,7 +111,9 @@ def main(argv=None) -> int:
         unet = UNet(0, 1)
         unet.set_params(np.fromfile(args.weights, np.float32))
         train_info = {"network": "5-level U-Net, 32-128 channels, 12 input channels, L1 on log radiance",
-                      "weights": args.weights}
+                      "weights": args.weights,
+                      "trained_by": "python -m rustracer_amd.denoiser --scenes 2000 --steps 2000 --out <weights> "
+                                    "(for the shipped rustracer_amd/weights/unet_config4.f32)"}
     elif not args.no_learned:
         from .denoiser import train_denoiser
         unet, losses, (data_s, train_s) = train_denoiser(args.train_scenes, args.train_steps, log=_log)

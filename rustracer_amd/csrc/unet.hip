@@ -1474,6 +1474,10 @@ int rt_unet_set_params(rt_unet *u, const float *in, size_t n) {
         if (n != u->n_params) throw ApiError(RT_ERR_INVALID, "parameter count mismatch");
         hip_check(hipStreamSynchronize(u->stream), "sync");
         hip_check(hipMemcpy(u->d_p, in, n * 4, hipMemcpyHostToDevice), "write params");
+        // new parameters restart the optimizer: Adam's moments and step count
+        hip_check(hipMemset(u->d_m, 0, n * 4), "clear Adam moments");
+        hip_check(hipMemset(u->d_v, 0, n * 4), "clear Adam moments");
+        u->adam_t = 0;
         u->refresh_bf16();
         hip_check(hipStreamSynchronize(u->stream), "sync");
     });
