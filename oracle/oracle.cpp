@@ -22,10 +22,9 @@
 //     parameter max_depth (6 reproduces the reference);
 //   * within a grid cell photons are ordered by ascending slot index, where the
 //     reference keeps atomic arrival order (photon_scatter.wgsl:35).
-// Parity pins: see tests/test_oracle_*.py (closed forms, brute force, libm).
-// Parity unpinned (no check independent of this file): the normal-map tangent
-// frame (common.wgsl tangent_frame, gather.wgsl:330-336); the GPU matches this
-// restatement bit for bit on textured scenes.
+// Parity pins: see tests/test_oracle_*.py (closed forms, brute force, libm);
+// the normal-map tangent frame (common.wgsl tangent_frame, gather.wgsl:330-336)
+// by closed forms of its debug views on quads with known uv layouts.
 
 #include "oracle.h"
 

@@ -321,3 +321,27 @@ def test_loaded_scene_renders_through_the_renderer_api(tmp_path):
     s = G.load_gltf(b.save(tmp_path / "floor.gltf"))
     nodes, prims = s.build_bvh()
     assert len(prims) == 2 and len(nodes) >= 1
+
+
+def test_node_matrix_matches_a_textbook_trs():
+    """A node's translation / rotation (glTF quaternion order x, y, z, w) /
+    scale against T * R * S built in float64 with scipy's rotation (an
+    independent quaternion-to-matrix routine): equal to float32 rounding for
+    random rotations, and the array holds the matrix column by column (glTF's
+    `matrix` layout). A scalar-first quaternion, a transposed rotation or
+    another product order would fail."""
+    from scipy.spatial.transform import Rotation
+
+    from rustracer_amd.gltf import _node_matrix
+
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        t = rng.uniform(-5, 5, 3)
+        s = rng.uniform(0.2, 3.0, 3) * rng.choice([-1, 1], 3)
+        m = _node_matrix({"translation": t.tolist(), "rotation": q.tolist(), "scale": s.tolist()})
+        want = np.eye(4)
+        want[:3, :3] = Rotation.from_quat(q).as_matrix() @ np.diag(s)  # scalar-last (x, y, z, w)
+        want[:3, 3] = t
+        assert np.allclose(m.astype(np.float64).T, want, atol=2e-6 * max(1.0, np.abs(s).max())), (q, t, s)

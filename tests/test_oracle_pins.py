@@ -400,3 +400,69 @@ def test_alpha_cutout_pass_through(depth):
     assert np.all(frame[opaque][:, :3] == E2)
     want = E1 if depth == 2 else np.zeros(3, F)
     assert np.all(frame[clear][:, :3] == want)
+
+
+def _tangent_views(positions, normal, uvs, cam_pos, look, up):
+    """Debug views 19 and 20 (tangent * 0.5 + 0.5, bitangent * 0.5 + 0.5) at
+    the centre pixels of a quad filling a 32x32 image. At 9 spp (above the
+    firefly clamp's 8, which would scale a debug value by a luminance sum that
+    debug values do not enter) the frame holds the last sample's value / 9."""
+    s = rt.Scene()
+    m = s.add_material(rt.make_material(0))
+    s.add_mesh(positions, [0, 1, 2, 0, 2, 3], m, normals=[normal] * 4, uvs=uvs)
+    cam = rt.camera(cam_pos, look, up=up, fov_degrees=30)
+    out = []
+    for view in (19, 20):
+        cfg = rt.RenderConfig(width=32, height=32, spp=9, enable_denoise=False, max_depth=1, debug_view=view)
+        frame, _, _, _ = orc.OracleScene.from_scene(s).gather(rt.camera_to_uniform(cam, 32, 32),
+                                                              image_params(cfg, 1), max_depth=1)
+        c = frame[12:20, 12:20, :3].reshape(-1, 3).astype(np.float64) * 9.0
+        assert np.ptp(c, axis=0).max() < 1e-5  # a flat quad: one frame everywhere
+        out.append(c[0] * 2.0 - 1.0)
+    return out
+
+
+QUAD_Z = [[-1, -1, -3], [1, -1, -3], [1, 1, -3], [-1, 1, -3]]  # normal +z, towards a camera at the origin
+
+
+@pytest.mark.parametrize("uvs,t_exp,b_exp", [
+    # u along +x, v along +y: T = +x, B = n x T = +y
+    ([[0, 0], [1, 0], [1, 1], [0, 1]], (1, 0, 0), (0, 1, 0)),
+    # u mirrored (along -x): T = -x; n x T = -y points against dP/dv, so B flips to +y
+    ([[1, 0], [0, 0], [0, 1], [1, 1]], (-1, 0, 0), (0, 1, 0)),
+    # u along +y, v along -x: T = +y, B = n x T = -x (with dP/dv)
+    ([[0, 1], [0, 0], [1, 0], [1, 1]], (0, 1, 0), (-1, 0, 0)),
+    # sheared map u = x, v = x + y: dP/du = (1, -1, 0) / |.|, B = n x T
+    ([[-1, -2], [1, 0], [1, 2], [-1, 0]], (0.5 ** 0.5, -(0.5 ** 0.5), 0), (0.5 ** 0.5, 0.5 ** 0.5, 0)),
+    # degenerate uvs (det = 0): T falls back to n x (1, 0, 0) = +y, B = -(n x T) = +x (dot with b = 0 is not > 0)
+    ([[0, 0], [0, 0], [0, 0], [0, 0]], (0, 1, 0), (1, 0, 0)),
+])
+def test_tangent_frame_closed_forms(uvs, t_exp, b_exp):
+    """The normal-map tangent frame (common.wgsl:211-230, used at gather.wgsl:
+    335-340): the tangent is the surface direction of increasing u projected
+    off the normal, the bitangent n x T turned to point along increasing v;
+    with no uv parameterisation the first fallback axis is used. Closed forms
+    for a quad facing the camera under five uv layouts."""
+    t, b = _tangent_views(QUAD_Z, (0, 0, 1), uvs, (0.0, 0.0, 0.0), (0.0, 0.0, -1.0), (0.0, 1.0, 0.0))
+    assert np.allclose(t, t_exp, atol=1e-5), t
+    assert np.allclose(b, b_exp, atol=1e-5), b
+
+
+def test_tangent_frame_second_fallback_axis():
+    """Degenerate uvs on a quad whose normal is +x: n x (1, 0, 0) vanishes, so
+    the tangent is n x (0, 1, 0) = +z and the bitangent -(n x T) = +y."""
+    quad_x = [[-3, -1, 1], [-3, -1, -1], [-3, 1, -1], [-3, 1, 1]]  # normal +x
+    t, b = _tangent_views(quad_x, (1, 0, 0), [[0, 0]] * 4, (0.0, 0.0, 0.0), (-1.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    assert np.allclose(t, (0, 0, 1), atol=1e-5), t
+    assert np.allclose(b, (0, 1, 0), atol=1e-5), b
+
+
+def test_tangent_frame_projects_off_a_tilted_shading_normal():
+    """Vertex normals tilted to n = (0.6, 0, 0.8) on the quad facing +z: the
+    tangent is dP/du = +x with its component along n removed,
+    (1, 0, 0) - 0.6 n = (0.64, 0, -0.48), normalised (0.8, 0, -0.6); the
+    bitangent n x T = +y points along dP/dv."""
+    t, b = _tangent_views(QUAD_Z, (0.6, 0.0, 0.8), [[0, 0], [1, 0], [1, 1], [0, 1]], (0.0, 0.0, 0.0),
+                          (0.0, 0.0, -1.0), (0.0, 1.0, 0.0))
+    assert np.allclose(t, (0.8, 0.0, -0.6), atol=1e-5), t
+    assert np.allclose(b, (0.0, 1.0, 0.0), atol=1e-5), b
