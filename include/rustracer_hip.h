@@ -480,7 +480,10 @@ int rt_unet_eval(rt_unet *u, uint32_t n, uint32_t h, uint32_t w, const uint16_t 
  * 32, weights staged once) picked for each layer shape so far: the first use of
  * a shape times every compiled variant that fits and keeps the fastest (all of
  * them give identical results). One text line per shape, "cin cout src N H W
- * mask bias -> kc nco once"; *needed = bytes including the terminator. */
+ * mask bias -> kc nco once rpw", then one line per inference level whose pool
+ * or upsample was timed folded against separate, "fold which N H W -> 1|0"
+ * (which: 1 the level-1 pool, -1 - l decoder level l); *needed = bytes
+ * including the terminator. */
 int rt_unet_conv_choices(const rt_unet *u, char *buf, size_t cap, size_t *needed);
 /* How a convolution variant is picked: 1 (default) the autotuned fastest, 0 the
  * first compiled for the shape, k >= 2 the (k-2)-th that fits (the last if

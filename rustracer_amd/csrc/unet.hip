@@ -1048,6 +1048,7 @@ struct ConvVariant {
 };
 #define RT_CV(CI, KC, NCO, SRC, ONCE) {CI, KC, NCO, rtu::SRC, ONCE, 2, conv_launch<CI, KC, NCO, rtu::SRC, ONCE, 2>}
 #define RT_CV4(CI, KC, NCO, SRC, ONCE) {CI, KC, NCO, rtu::SRC, ONCE, 4, conv_launch<CI, KC, NCO, rtu::SRC, ONCE, 4>}
+#define RT_CV1(CI, KC, NCO, SRC, ONCE) {CI, KC, NCO, rtu::SRC, ONCE, 1, conv_launch<CI, KC, NCO, rtu::SRC, ONCE, 1>}
 const ConvVariant kConvVariants[] = {
     RT_CV(16, 16, 1, SRC_PLAIN, true),
     RT_CV(32, 32, 1, SRC_PLAIN, true),    RT_CV(32, 16, 1, SRC_PLAIN, false),
@@ -1071,9 +1072,14 @@ const ConvVariant kConvVariants[] = {
     RT_CV4(96, 16, 1, SRC_CONCAT, false), RT_CV4(96, 32, 1, SRC_CONCAT, false), RT_CV4(96, 16, 1, SRC_PLAIN, false),
     RT_CV4(128, 16, 2, SRC_PLAIN, false), RT_CV4(128, 32, 1, SRC_PLAIN, false), RT_CV4(192, 16, 2, SRC_CONCAT, false),
     RT_CV4(192, 16, 2, SRC_PLAIN, false), RT_CV4(256, 16, 2, SRC_CONCAT, false), RT_CV4(256, 16, 2, SRC_PLAIN, false),
+    // four-row tiles (one row per wavefront): twice the work items of eight-row
+    // tiles, for the levels too small to fill the chip
+    RT_CV1(128, 16, 2, SRC_PLAIN, false), RT_CV1(128, 32, 1, SRC_PLAIN, false), RT_CV1(128, 16, 4, SRC_PLAIN, false),
+    RT_CV1(256, 16, 2, SRC_CONCAT, false), RT_CV1(256, 16, 2, SRC_PLAIN, false),
 };
 #undef RT_CV
 #undef RT_CV4
+#undef RT_CV1
 
 }  // namespace
 
@@ -1100,6 +1106,12 @@ struct rt_unet {
     // convolution variant chosen per shape (autotuned at first use)
     std::map<ConvKey, int> conv_choice;
     int variant_mode = 1;  // 0 first listed, 1 autotuned, k >= 2 the (k-2)-th that fits
+    // inference: per (level, N, H, W), whether the level's pool (level 1) or
+    // upsample + concatenation (decoder levels) is folded into the next
+    // convolution's staging (true) or run as its own kernel before a plain
+    // convolution; both forms give the same bits. Timed at first use in
+    // autotuned mode, folded in every other mode.
+    std::map<std::array<int, 4>, bool> fold_choice;
 
     ~rt_unet() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -1241,6 +1253,70 @@ struct rt_unet {
                            dim3(256), 0, stream, Bsrc, c, Y, ys, off, H, W);
     }
 
+    // The inference convolution that reads a resampled input: which 1 is layer
+    // 2 over the pooled level-0 output, which -1 - l the first convolution of
+    // decoder level l over [skip, upsampled]. fold: stage the pool or the
+    // upsample inside the convolution; otherwise run pool_kernel /
+    // upsample_kernel into P[1] / CAT[l] first and convolve that.
+    void level_input(Acts &a, int which, bool fold) {
+        const int N = a.N;
+        if (which == 1) {
+            if (fold) {
+                conv(2, false, a.CAT[0], cat_ch(0), a.T[1], kEnc[1], 0, nullptr, 0, N, a.h(1), a.w(1),
+                     rtu::SRC_POOL);
+            } else {
+                pool(a.CAT[0], cat_ch(0), kEnc[0], a.P[1], N, a.h(0), a.w(0));
+                conv(2, false, a.P[1], kEnc[0], a.T[1], kEnc[1], 0, nullptr, 0, N, a.h(1), a.w(1));
+            }
+            return;
+        }
+        const int l = -1 - which, k = dec_layer(l);
+        const rtu::bf16 *below = l == kLevels - 2 ? a.B : a.V[l + 1];
+        const int bc = cat_up(l);
+        if (fold) {
+            conv(k, false, a.CAT[l], cat_ch(l), a.U[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l), rtu::SRC_CONCAT,
+                 below, bc, kEnc[l]);
+        } else {
+            upsample(below, bc, a.CAT[l], cat_ch(l), kEnc[l], N, a.h(l), a.w(l));
+            conv(k, false, a.CAT[l], cat_ch(l), a.U[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l));
+        }
+    }
+
+    // Folded or not for this level and shape: timed once in autotuned mode
+    // (each form's first run tunes its convolution, then kTuneLaunches runs
+    // of each are timed); the timing runs only rewrite buffers that the pass
+    // writes again right after.
+    bool fold_here(Acts &a, int which) {
+        if (variant_mode != 1) return true;
+        const std::array<int, 4> key{which, a.N, a.H, a.W};
+        auto it = fold_choice.find(key);
+        if (it != fold_choice.end()) return it->second;
+        level_input(a, which, true);
+        level_input(a, which, false);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hip_check(hipEventCreate(&e0), "tuning event");
+        hip_check(hipEventCreate(&e1), "tuning event");
+        float ms[2] = {0.0f, 0.0f};
+        try {
+            for (int f = 0; f < 2; f++) {
+                hip_check(hipEventRecord(e0, stream), "tuning event");
+                for (int rep = 0; rep < kTuneLaunches; rep++) level_input(a, which, f == 0);
+                hip_check(hipEventRecord(e1, stream), "tuning event");
+                hip_check(hipEventSynchronize(e1), "tuning sync");
+                hip_check(hipEventElapsedTime(&ms[f], e0, e1), "tuning time");
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const bool fold = ms[0] <= ms[1];
+        fold_choice[key] = fold;
+        return fold;
+    }
+
     // Forward pass over a.F / a.noisy. Training (with_target) keeps the pooled
     // and concatenated tensors for the backward pass, writes the loss partials
     // and runs the output layer's backward pass into a.gV[0] and the
@@ -1253,8 +1329,7 @@ struct rt_unet {
         for (int l = 1; l < kLevels; l++) {
             const int cs = kEnc[l - 1];
             if (!with_target && l == 1) {
-                conv(2, false, a.CAT[0], cat_ch(0), a.T[1], kEnc[1], 0, nullptr, 0, N, a.h(1), a.w(1),
-                     rtu::SRC_POOL);
+                level_input(a, 1, fold_here(a, 1));
             } else {
                 pool(a.CAT[l - 1], cat_ch(l - 1), cs, a.P[l], N, a.h(l - 1), a.w(l - 1));
                 conv(2 * l, false, a.P[l], cs, a.T[l], kEnc[l], 0, nullptr, 0, N, a.h(l), a.w(l));
@@ -1271,8 +1346,7 @@ struct rt_unet {
                 upsample(below, bc, a.CAT[l], cat_ch(l), kEnc[l], N, a.h(l), a.w(l));
                 conv(k, false, a.CAT[l], cat_ch(l), a.U[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l));
             } else {
-                conv(k, false, a.CAT[l], cat_ch(l), a.U[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l),
-                     rtu::SRC_CONCAT, below, bc, kEnc[l]);
+                level_input(a, -1 - l, fold_here(a, -1 - l));
             }
             conv(k + 1, false, a.U[l], kDec[l], a.V[l], kDec[l], 0, nullptr, 0, N, a.h(l), a.w(l));
         }
@@ -1601,6 +1675,12 @@ int rt_unet_conv_choices(const rt_unet *u, char *buf, size_t cap, size_t *needed
             char line[160];
             std::snprintf(line, sizeof(line), "%d %d %d %d %d %d %d %d -> %d %d %d %d\n", k[0], k[1], k[2], k[3],
                           k[4], k[5], k[6], k[7], v.kc, v.nco, v.wonce ? 1 : 0, v.rpw);
+            out += line;
+        }
+        for (const auto &kv : u->fold_choice) {
+            char line[96];
+            std::snprintf(line, sizeof(line), "fold %d %d %d %d -> %d\n", kv.first[0], kv.first[1], kv.first[2],
+                          kv.first[3], kv.second ? 1 : 0);
             out += line;
         }
         if (needed) *needed = out.size() + 1;
