@@ -174,6 +174,9 @@ def test_every_convolution_variant_gives_the_same_bits(n, h, w):
     net.set_variant_mode(0)
     base = net.eval(fb, x, t)
     base_inf = net.eval(fb, x)
+    # mode 0 folds every pool and upsample into the convolutions' staging; the
+    # training forward runs them as kernels of their own
+    assert np.array_equal(base_inf.view(np.uint32), base[0].view(np.uint32))
     for mode in range(1, 13):  # autotuned, then up to the eleventh compiled variant of each shape
         net.set_variant_mode(mode)
         pred, grads, loss = net.eval(fb, x, t)
